@@ -1,0 +1,127 @@
+/*
+ * rbrp.h -- C ABI of the MI355X-native RBRP column interpolative decomposition.
+ *
+ * Plain C: pointers, 64-bit sizes, status codes.  No torch or HIP types appear
+ * in the signatures (streams are passed as `void*` holding a hipStream_t).
+ *
+ * Conventions shared by every entry point:
+ *   - matrices are column-major FP64 device pointers with a leading dimension;
+ *   - the input matrix is never written;
+ *   - return value: 0 = success, -i = argument i is invalid (LAPACK style,
+ *     1-based), +e = HIP runtime error code e, RBRP_ERR_NUMERIC for a
+ *     non-finite input or a breakdown; nothing throws, exits or asserts;
+ *   - a handle owns its workspace and stream and no global state, so distinct
+ *     handles may be used concurrently from different host threads.
+ */
+#ifndef RBRP_H_
+#define RBRP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBRP_ERR_NUMERIC 100000
+#define RBRP_ERR_ALLOC 100001
+
+typedef struct rbrp_handle_s* rbrp_handle_t;
+
+/* Filter threshold (gpu/rbrp.cu:198-207 vs RBRP.ipynb c11:69-73). */
+enum { RBRP_FILTER_SAMPLED = 0, RBRP_FILTER_NOMINAL = 1 };
+/* Arithmetic of the residual update (FP64 is the reference's). */
+enum { RBRP_PREC_FP64 = 0 };
+
+typedef struct {
+  int64_t b;              /* nominal block size, 1..128 (rbrp.hpp:6 `blk`) */
+  double tol;             /* stop when ||A - A(:,J)X||_F <= tol ||A||_F; 0 = run to min(m,n) */
+  int64_t max_rank;       /* 0 = min(m,n) */
+  uint64_t seed;          /* Philox4x64-10 key word 0 (word 1 is 0) */
+  uint64_t stream_offset; /* Philox initial counter (numpy.random.Philox `counter`) */
+  int32_t precision;      /* RBRP_PREC_* */
+  int32_t filter;         /* RBRP_FILTER_* */
+} rbrp_opts_t;
+
+typedef struct {
+  /* caller-provided outputs */
+  int32_t* perm;       /* device, n entries: perm[0:k] = skeleton J, perm[k:n] = redundant */
+  double* X;           /* device, X = R11^{-1} R12 (k x (n-k)), column-major, or NULL to skip */
+  int64_t ldx;         /* leading dimension of X, >= min(m,n) (or >= max_rank) */
+  int64_t trace_cap;   /* capacity of the three host trace arrays below (0 = none) */
+  int64_t* ranks;      /* host: cumulative rank after each block          (rbrp.cu:475-482) */
+  double* error;       /* host: squared relative residual after each block (rbrp.cu:467) */
+  double* t_ranks;     /* host: cumulative seconds after each block       (rbrp.cu:477-483) */
+  /* filled by the call */
+  int64_t k;           /* rank found */
+  int64_t nblocks;     /* number of blocks run */
+  double t_total;      /* seconds from entry to return (rbrp.cu:485) */
+  double err_final;    /* non-squared final relative residual */
+  uint64_t draws;      /* uniforms consumed from the Philox stream */
+} rbrp_result_t;
+
+/* Per-phase device time (hipEvents), launches and algorithmic work. */
+enum {
+  RBRP_PHASE_NORMS = 0,    /* copy + squared column norms of A (one kernel) */
+  RBRP_PHASE_SAMPLE,       /* weighted sampling without replacement (one kernel) */
+  RBRP_PHASE_CGS,          /* re-orthogonalisation of the sampled block */
+  RBRP_PHASE_PANEL,        /* Gram, pivoted Cholesky, filter, CholQR2 of the block */
+  RBRP_PHASE_PERMUTE,      /* symmetric-difference column permutation */
+  RBRP_PHASE_PROJECT,      /* W = Q_b^T A_rem: the split-K MFMA GEMM kernel alone */
+  RBRP_PHASE_PROJECT_RED,  /* reduction of the split-K partials into R */
+  RBRP_PHASE_UPDATE,       /* A_rem -= Q_b W with fused norms: the NN MFMA kernel alone */
+  RBRP_PHASE_RESID,        /* residual norms / sum from the update's partials */
+  RBRP_PHASE_SOLVE,        /* X = R11^{-1} R12 */
+  RBRP_PHASE_COUNT
+};
+
+typedef struct {
+  double ms[RBRP_PHASE_COUNT];      /* summed device milliseconds */
+  int64_t launches[RBRP_PHASE_COUNT];
+  double flops[RBRP_PHASE_COUNT];   /* algorithmic flops */
+  double bytes[RBRP_PHASE_COUNT];   /* algorithmic HBM bytes */
+} rbrp_phase_stats_t;
+
+/* Handle lifetime.  `stream` is a hipStream_t (NULL = the default stream). */
+int rbrp_create(rbrp_handle_t* handle, void* stream);
+int rbrp_destroy(rbrp_handle_t handle);
+int rbrp_set_stream(rbrp_handle_t handle, void* stream);
+/* Enable (1) or disable (0) per-phase hipEvent timing; reset clears the stats. */
+int rbrp_set_timing(rbrp_handle_t handle, int enable);
+int rbrp_get_phase_stats(rbrp_handle_t handle, rbrp_phase_stats_t* out);
+int rbrp_reset_phase_stats(rbrp_handle_t handle);
+/* Release the cached workspace. */
+int rbrp_release_workspace(rbrp_handle_t handle);
+
+/*
+ * FP64 RBRP column ID.  Replaces `RBRP(int m, int n, double *A, int blk,
+ * std::vector<int>& ranks, std::vector<double>& error, std::vector<double>& t_ranks,
+ * double& t_rbrp, double& flops)` (gpu/rbrp.hpp:6-8, gpu/rbrp.cu:333-503) and adds
+ * the tolerance, skeleton/permutation and X outputs of matlab/RBRP.m:2 and
+ * matlab/driver.m:83-94.  dA is m x n, leading dimension lda >= m.
+ */
+int rbrp_colid_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                 const rbrp_opts_t* opts, rbrp_result_t* result);
+
+/* Defaults: b = 64, tol = 0, max_rank = 0, seed = 0, offset = 0, FP64, sampled filter. */
+void rbrp_opts_default(rbrp_opts_t* opts);
+
+/*
+ * Host-side Philox4x64-10 block (no GPU needed): out[0..3] = Philox(ctr, key).
+ * Exposes the generator the device sampler uses so that it can be checked
+ * against its known-answer vectors.
+ */
+void rbrp_philox4x64_10_host(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+
+/* Fill out[0..count) with the uniforms draw0, draw0+1, ... of the stream
+ * (seed, offset), computed on the device.  dOut is a device pointer. */
+int rbrp_uniforms_d(rbrp_handle_t handle, uint64_t seed, uint64_t offset, uint64_t draw0,
+                    int64_t count, double* dOut);
+
+/* Library version string. */
+const char* rbrp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBRP_H_ */

@@ -1,0 +1,51 @@
+/*
+ * rbrp_testing.h -- kernel-level entry points used by the parity tests.
+ *
+ * These have no counterpart in the reference: they expose single kernels of
+ * the RBRP library (the MFMA GEMMs and the panel factorisation) so that each
+ * can be checked against a host computation in isolation.  Same conventions as
+ * rbrp.h: column-major FP64 device pointers, status codes, `stream` is a
+ * hipStream_t passed as void*.  Every call synchronises its stream.
+ */
+#ifndef RBRP_TESTING_H_
+#define RBRP_TESTING_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C (M x N, ldc) = A^T B with A (K x M, lda), B (K x N, ldb) by the split-K
+ * MFMA kernel and its reduction.  K must be a multiple of 16. */
+int rbrp_test_gemm_tn(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double* C, int64_t ldc, int nsplit);
+
+/* C (M x N) = A B (mode 0), C -= A B (mode 1) or C += A B (mode 2) with A
+ * (M x K, lda) and B (K x N, ldb) by the NN MFMA kernel.  If normpart is not
+ * NULL it receives ceil(M/64) x N per-tile squared column norms of the new C
+ * (leading dimension N). */
+int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double* C, int64_t ldc, int mode,
+                      double* normpart);
+
+/* Pivoted Cholesky + robust filter of an s x s Gram matrix G (ldg = s) whose
+ * columns correspond to the column indices J[0:s] (device), at block start i.
+ * Outputs (device, capacity b each): Jk[0:bt] (pivot order), Rb (b x b, ld b),
+ * Jin/Jout[0:nswap]; bt, nswap and status are written to host ints. */
+int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int filter, int64_t i,
+                    const int32_t* J, int32_t* Jk, double* Rb, int32_t* Jin, int32_t* Jout,
+                    int32_t* bt, int32_t* nswap, int32_t* status);
+
+/* Sampler kernel alone: s = min(b, kmax - i, nnz(diags[i:n])) indices (absolute,
+ * first-occurrence order) into J (device, capacity b); draws_in is the number
+ * of uniforms already consumed; the new count and s go to host ints. */
+int rbrp_test_sample(void* stream, const double* diags, int64_t i, int64_t n, int64_t kmax,
+                     int64_t b, uint64_t seed, uint64_t offset, uint64_t draws_in, int32_t* J,
+                     int32_t* s_out, uint64_t* draws_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBRP_TESTING_H_ */

@@ -1,0 +1,70 @@
+"""CPU restatement of the sketch-based ID comparators (oracle; test infrastructure only).
+
+``rand_cpqr_os`` restates gpu/randCPQR_OS.cu:20-111 (spec matlab/randCPQR_OS.m:1-49)
+and ``rand_lupp_os`` restates gpu/randLUPP_OS.cu:14-119 (spec
+matlab/randLUPP_OS.m:1-54).  Both take the Gaussian sketch matrix ``G`` as an
+explicit argument so that a device implementation and this oracle can be fed
+the identical sketch.  Pivoted QR and LU come from SciPy's LAPACK
+(``dgeqp3``, ``dgetrf``), which is what the reference calls through MAGMA and
+cuSOLVER.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+
+
+def pivots_to_permutation(ipiv, P, ofs: int = 0):
+    """Apply LAPACK row interchanges ``ipiv`` (0-based here) to ``P`` in order.
+
+    gpu/permute.cu:8-23 (the reference's ``ipiv`` is 1-based and subtracts 1).
+    """
+    P = np.array(P, dtype=np.int64, copy=True)
+    for j, pj in enumerate(np.asarray(ipiv, dtype=np.int64)):
+        a, c = j + ofs, int(pj) + ofs
+        P[a], P[c] = P[c], P[a]
+    return P
+
+
+def compute_interpolation_matrix_os(Y, P, k: int):
+    """Oversampled interpolation matrix ``T`` (k x (n-k)) from sketch ``Y`` (r x n).
+
+    gpu/interp.cu:45-105: ``Yp = Y(:,P)``; QR of ``Yp(:,1:k)`` (geqrf :82);
+    ``Q^T`` applied to ``Yp(:,k+1:n)`` (ormqr :87); triangular solve with the
+    ``k x k`` R (trsm :95); ``T`` is the top ``k`` rows.
+    """
+    Y = np.asarray(Y, dtype=np.float64)
+    Yp = Y[:, np.asarray(P, dtype=np.int64)]
+    Q1, R1 = sla.qr(Yp[:, :k], mode="economic")
+    return sla.solve_triangular(R1, Q1.T @ Yp[:, k:], lower=False)
+
+
+def rand_cpqr_os(A, r: int, G):
+    """Randomised CPQR ID with 2x oversampling; returns ``(sk, rd, T)``.
+
+    gpu/randCPQR_OS.cu: ``Y = G A`` (:42, G is 2r x m), pivoted QR of
+    ``Y(1:r,:)`` (:64), ``sk = jpvt[:r]``, ``rd = jpvt[r:]`` (:91-92), ``T`` by
+    the oversampled interpolation (:95).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    Y = np.asarray(G, dtype=np.float64) @ A
+    _, _, jpvt = sla.qr(Y[:r, :], mode="economic", pivoting=True)
+    T = compute_interpolation_matrix_os(Y, jpvt, r)
+    return jpvt[:r].copy(), jpvt[r:].copy(), T
+
+
+def rand_lupp_os(A, k: int, G):
+    """Randomised LUPP ID with 2x oversampling; returns ``(sk, rd, T)``.
+
+    gpu/randLUPP_OS.cu: ``Y = G A`` (:43, G is 2k x m), LU with partial
+    pivoting of ``Y(1:k,:)^T`` (n x k, :57-81), permutation from the pivots
+    (:85, gpu/permute.cu:8-23), ``T`` by the oversampled interpolation (:96).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[1]
+    Y = np.asarray(G, dtype=np.float64) @ A
+    _, ipiv = sla.lu_factor(Y[:k, :].T.copy(), check_finite=False)
+    P = pivots_to_permutation(ipiv, np.arange(n))
+    T = compute_interpolation_matrix_os(Y, P, k)
+    return P[:k].copy(), P[k:].copy(), T

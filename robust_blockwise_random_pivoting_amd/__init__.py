@@ -1,0 +1,11 @@
+"""MI355X-native RBRP column interpolative decomposition.
+
+The computation lives in the HIP library ``librbrp.so`` (C ABI in
+``include/rbrp.h``); this package is its host-side mirror.  Importing the
+package does not load the library; the first call does, and raises
+ImportError if it has not been built.
+"""
+
+from .rbrp import RBRP, ColID, Handle, colmajor, rbrp_colid, uniforms  # noqa: F401
+
+__all__ = ["RBRP", "ColID", "Handle", "colmajor", "rbrp_colid", "uniforms"]

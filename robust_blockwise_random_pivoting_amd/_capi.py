@@ -1,0 +1,116 @@
+"""ctypes binding of the C ABI declared in ``include/rbrp.h`` and ``include/rbrp_testing.h``.
+
+Only plain pointers and integers cross this boundary.  Loading fails loudly
+(ImportError) when ``librbrp.so`` has not been built: there is no fallback
+path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_int, c_int32, c_int64,
+                    c_uint64, c_void_p)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librbrp.so")
+
+RBRP_ERR_NUMERIC = 100000
+RBRP_ERR_ALLOC = 100001
+FILTERS = {"sampled": 0, "nominal": 1}
+PRECISIONS = {"fp64": 0}
+
+PHASES = ["norms", "sample", "cgs", "panel", "permute", "project", "project_reduce", "update",
+          "resid", "solve"]
+
+
+class RbrpOpts(Structure):
+    _fields_ = [("b", c_int64), ("tol", c_double), ("max_rank", c_int64), ("seed", c_uint64),
+                ("stream_offset", c_uint64), ("precision", c_int32), ("filter", c_int32)]
+
+
+class RbrpResult(Structure):
+    _fields_ = [("perm", c_void_p), ("X", c_void_p), ("ldx", c_int64), ("trace_cap", c_int64),
+                ("ranks", c_void_p), ("error", c_void_p), ("t_ranks", c_void_p), ("k", c_int64),
+                ("nblocks", c_int64), ("t_total", c_double), ("err_final", c_double),
+                ("draws", c_uint64)]
+
+
+class RbrpPhaseStats(Structure):
+    _fields_ = [("ms", c_double * len(PHASES)), ("launches", c_int64 * len(PHASES)),
+                ("flops", c_double * len(PHASES)), ("bytes", c_double * len(PHASES))]
+
+
+_lib = None
+
+
+def lib():
+    """Load ``librbrp.so`` once; raise ImportError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python -m robust_blockwise_random_pivoting_amd.build_lib`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, u64, i32p = c_void_p, c_int64, c_uint64, c_void_p
+    sig = {
+        "rbrp_version": (c_char_p, []),
+        "rbrp_opts_default": (None, [POINTER(RbrpOpts)]),
+        "rbrp_philox4x64_10_host": (None, [POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
+        "rbrp_create": (c_int, [POINTER(c_void_p), vp]),
+        "rbrp_destroy": (c_int, [vp]),
+        "rbrp_set_stream": (c_int, [vp, vp]),
+        "rbrp_set_timing": (c_int, [vp, c_int]),
+        "rbrp_get_phase_stats": (c_int, [vp, POINTER(RbrpPhaseStats)]),
+        "rbrp_reset_phase_stats": (c_int, [vp]),
+        "rbrp_release_workspace": (c_int, [vp]),
+        "rbrp_uniforms_d": (c_int, [vp, u64, u64, u64, i64, vp]),
+        "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
+        "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
+        "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
+        "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
+                                    POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+        "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
+                                     POINTER(c_int32), POINTER(c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class RBRPError(RuntimeError):
+    """Non-zero status from the C ABI."""
+
+    def __init__(self, fn: str, code: int):
+        if code < 0:
+            what = f"argument {-code} is invalid"
+        elif code == RBRP_ERR_NUMERIC:
+            what = "non-finite input or numerical breakdown"
+        elif code == RBRP_ERR_ALLOC:
+            what = "device allocation failed"
+        else:
+            what = f"HIP error {code}"
+        super().__init__(f"{fn} returned {code}: {what}")
+        self.code = code
+
+
+def check(fn: str, code: int):
+    if code != 0:
+        raise RBRPError(fn, code)
+
+
+def philox_block(ctr, key):
+    """Philox4x64-10 of a 256-bit counter and 128-bit key (host, no GPU)."""
+    c = (c_uint64 * 4)(*[int(x) for x in ctr])
+    k = (c_uint64 * 2)(*[int(x) for x in key])
+    out = (c_uint64 * 4)()
+    lib().rbrp_philox4x64_10_host(c, k, out)
+    return [int(x) for x in out]
+
+
+__all__ = ["lib", "check", "RBRPError", "RbrpOpts", "RbrpResult", "RbrpPhaseStats", "PHASES",
+           "FILTERS", "PRECISIONS", "philox_block", "byref", "LIB_PATH"]

@@ -1,0 +1,721 @@
+// RBRP loop kernels other than the GEMMs: the HBM-bound norm pass, the
+// weighted sampler, the LDS-resident pivoted Cholesky + robust filter of the
+// sampled block, CholQR2 re-factorisation, the symmetric-difference
+// permutation, residual sums and the diagonal-block solve of X = R11^{-1} R12.
+#include "rbrp_kernels.h"
+
+// ---------------------------------------------------------------------------
+// Norm pass: Aw(:, j) = [A(0:m, j); 0 (pad rows m..mp)], diags[j] = ||A(:,j)||^2.
+// One workgroup per column; loads are coalesced along the column.
+// Reference: column_norm_squared (gpu/rbrp.cu:82-99, called at :354) plus the
+// working copy at gpu/rbrp.cu:337-338.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy_colnorm(const double* __restrict__ A, int64_t lda,
+                                                      int64_t m, int64_t mp,
+                                                      double* __restrict__ Aw,
+                                                      double* __restrict__ diags) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const double* a = A + j * lda;
+  double* w = Aw + j * mp;
+  double acc0 = 0.0, acc1 = 0.0;
+  int64_t r = threadIdx.x;
+  for (; r + 256 < m; r += 512) {
+    const double v0 = a[r], v1 = a[r + 256];
+    w[r] = v0;
+    w[r + 256] = v1;
+    acc0 = fma(v0, v0, acc0);
+    acc1 = fma(v1, v1, acc1);
+  }
+  for (; r < mp; r += 256) {
+    const double v = r < m ? a[r] : 0.0;
+    w[r] = v;
+    acc0 = fma(v, v, acc0);
+  }
+  const double t = rbrp_block_sum(acc0 + acc1, red);
+  if (threadIdx.x == 0) diags[j] = t;
+}
+
+hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
+                                    int64_t mp, int64_t n, double* Aw, double* diags) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_colnorm, dim3((unsigned)n), dim3(256), 0, st, A, lda, m, mp, Aw, diags);
+  return hipGetLastError();
+}
+
+// Sum of x[0:n) into the block state (which = 0: N0).
+__global__ __launch_bounds__(1024) void k_sum_to_state(const double* __restrict__ x, int64_t n,
+                                                       RbrpBlockState* bs, int which) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 1024) acc += x[j];
+  const double t = rbrp_block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    if (which == 0) {
+      bs->N0 = t;
+      bs->draws = 0ull;
+      bs->status = 0;
+      bs->s = 0;
+      bs->bt = 0;
+      bs->nswap = 0;
+      bs->resid = t;
+      bs->err = (t > 0.0) ? 1.0 : 0.0;
+    }
+  }
+}
+
+hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
+                                    int which) {
+  hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(1024), 0, st, x, n, bs, which);
+  return hipGetLastError();
+}
+
+__global__ void k_iota(int* perm, int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < n) perm[j] = (int)j;
+}
+
+hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, perm, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Weighted sampling without replacement (one workgroup).
+//
+// s = min(b, kmax - i, nnz(diags[i:n])) distinct positions of [i, n) with
+// weights diags[i:n], by the rounds algorithm of NumPy's
+// Generator.choice(replace=False, p) (the Python reference's sampler,
+// RBRP.ipynb c11:108): each round draws s - found uniforms, zeroes the weights
+// found so far, forms cdf = inclusive_scan(w) / cdf[N-1], maps each uniform to
+// the first index with cdf > u, and appends the new distinct picks in order of
+// first occurrence.  Replaces select_indices (gpu/rbrp.cu:102-146: Thrust
+// scan + upper_bound + sort + unique, with replacement).
+// ---------------------------------------------------------------------------
+__device__ inline double block_exclusive_scan(double v, double* wsum) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  double ex = __shfl_up(x, 1, 64);
+  if (lane == 0) ex = 0.0;
+  __syncthreads();
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  double off = 0.0;
+  for (int q = 0; q < wid; ++q) off += wsum[q];
+  return off + ex;
+}
+
+#define SAMPLE_MAXB 256
+
+__global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diags, int64_t i,
+                                                 int64_t n, int64_t kmax, int b, uint64_t key0,
+                                                 uint64_t key1, uint64_t c0, uint64_t c1,
+                                                 uint64_t c2, uint64_t c3, double* __restrict__ w,
+                                                 double* __restrict__ cdf, RbrpBlockState* bs,
+                                                 int* __restrict__ J) {
+  __shared__ int found[SAMPLE_MAXB];
+  __shared__ int newidx[SAMPLE_MAXB];
+  __shared__ int ired[16];
+  __shared__ double wsum[16];
+  __shared__ int s_nfound;
+  const int tid = threadIdx.x;
+  const int64_t N = n - i;
+  int cnt = 0;
+  for (int64_t j = tid; j < N; j += 1024) {
+    double v = diags[i + j];
+    v = (v > 0.0) ? v : 0.0;
+    w[j] = v;
+    cnt += (v > 0.0) ? 1 : 0;
+  }
+  const int nnz = rbrp_block_sum_int(cnt, ired);
+  int64_t s64 = (int64_t)b;
+  if (kmax - i < s64) s64 = kmax - i;
+  if ((int64_t)nnz < s64) s64 = nnz;
+  const int s = (int)(s64 > 0 ? s64 : 0);
+  unsigned long long draws = bs->draws;
+  if (s <= 0) {
+    if (tid == 0) {
+      bs->s = 0;
+      bs->nnz = nnz;
+    }
+    return;
+  }
+  if (tid == 0) s_nfound = 0;
+  __syncthreads();
+  const uint64_t base[4] = {c0, c1, c2, c3};
+  const int64_t chunk = (N + 1023) / 1024;
+  const int64_t j0 = min(N, (int64_t)tid * chunk);
+  const int64_t j1 = min(N, j0 + chunk);
+  // every round adds at least one new index (the first draw of a round can
+  // only land on a positive, not yet found weight), so s rounds suffice
+  for (int round = 0; round <= s; ++round) {
+    const int nf = s_nfound;
+    if (nf >= s) break;
+    const int r = s - nf;
+    for (int t = tid; t < nf; t += 1024) w[found[t]] = 0.0;
+    __syncthreads();
+    double loc = 0.0;
+    for (int64_t j = j0; j < j1; ++j) loc += w[j];
+    double run = block_exclusive_scan(loc, wsum);
+    for (int64_t j = j0; j < j1; ++j) {
+      run += w[j];
+      cdf[j] = run;
+    }
+    __syncthreads();
+    const double total = cdf[N - 1];
+    if (tid < r) {
+      const double u = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draws + tid));
+      int64_t lo = 0, hi = N;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (cdf[mid] / total > u)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      if (lo >= N) lo = N - 1;
+      newidx[tid] = (int)lo;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int nfl = nf;
+      for (int t = 0; t < r; ++t) {
+        const int v = newidx[t];
+        bool dup = false;
+        for (int q = 0; q < nfl; ++q)
+          if (found[q] == v) {
+            dup = true;
+            break;
+          }
+        if (!dup) found[nfl++] = v;
+      }
+      s_nfound = nfl;
+    }
+    draws += (unsigned long long)r;
+    __syncthreads();
+  }
+  if (s_nfound < s) {
+    if (tid == 0) {
+      bs->s = 0;
+      bs->status = 4;
+    }
+    return;
+  }
+  for (int t = tid; t < s; t += 1024) J[t] = (int)(i + found[t]);
+  if (tid == 0) {
+    bs->s = s;
+    bs->draws = draws;
+    bs->nnz = nnz;
+  }
+}
+
+hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
+                              int64_t kmax, int b, uint64_t key0, uint64_t key1,
+                              const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
+                              int* J) {
+  if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), 0, st, diags, i, n, kmax, b, key0, key1, ctr[0],
+                     ctr[1], ctr[2], ctr[3], w, cdf, bs, J);
+  return hipGetLastError();
+}
+
+// dst(r, cols[c]) += src(r, c), r < rows, c < *ncols_dyn.
+__global__ void k_add_cols(int64_t rows, const int* ncols_dyn, const double* __restrict__ src,
+                           int64_t lds, double* __restrict__ dst, int64_t ldd,
+                           const int* __restrict__ cols) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows || c >= *ncols_dyn) return;
+  dst[r + (int64_t)cols[c] * ldd] += src[r + (int64_t)c * lds];
+}
+
+hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_dyn, int64_t ncols_max,
+                                const double* src, int64_t lds, double* dst, int64_t ldd,
+                                const int* cols) {
+  if (rows <= 0 || ncols_max <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_cols, dim3((unsigned)((rows + 255) / 256), (unsigned)ncols_max), dim3(256),
+                     0, st, rows, ncols_dyn, src, lds, dst, ldd, cols);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Pivoted Cholesky of the sampled block's Gram matrix G = V^T V (s x s, LDS
+// resident) + robust filter.
+//
+// The pivot at step j is the first index of the largest trailing Schur
+// diagonal, i.e. the column of V with the largest residual norm after
+// projecting out the previous pivots: the pivot rule of LAPACK dgeqp3 (first
+// maximum, with the swapped column moving to the vacated position), which the
+// reference calls through magma_dgeqp3_gpu (gpu/rbrp.cu:410).  Row j of the
+// factor is the row j of dgeqp3's R up to sign.
+//
+// Filter (gpu/rbrp.cu:193-207): sqr[r] = sum_{c>=r} R[r][c]^2, csum = running
+// sum of sqr from the last row up, keep bt = s - k where k is the first index
+// with csum[k] >= csum[s-1] / s (or / b for the nominal rule of
+// RBRP.ipynb c11:69-73).
+//
+// Outputs: Jk = J[piv[0:bt]] (pivot order), Rb = R[0:bt][0:bt], its inverse,
+// and the ascending symmetric differences Jin = sort(Jk) \ [i, i+bt),
+// Jout = [i, i+bt) \ Jk (gpu/rbrp.cu:211-234).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict__ G, int64_t ldg,
+                                                        int b, int filter_mode, int64_t i,
+                                                        const int* __restrict__ J,
+                                                        RbrpBlockState* bs, double* __restrict__ Rb,
+                                                        double* __restrict__ Rinv,
+                                                        int* __restrict__ Jk, int* __restrict__ Jin,
+                                                        int* __restrict__ Jout) {
+  extern __shared__ double S[];  // s x (s + 1)
+  __shared__ int piv[SAMPLE_MAXB];
+  __shared__ int sj[SAMPLE_MAXB];
+  __shared__ double sqr[SAMPLE_MAXB];
+  __shared__ int s_pvt, s_stop, s_bt;
+  __shared__ double s_rjj;
+  const int tid = threadIdx.x;
+  const int s = bs->s;
+  if (s <= 0) {
+    if (tid == 0) {
+      bs->bt = 0;
+      bs->nswap = 0;
+    }
+    return;
+  }
+  const int ld = s + 1;
+  for (int idx = tid; idx < s * s; idx += 256) {
+    const int r = idx % s, c = idx / s;
+    S[r * ld + c] = G[r + (int64_t)c * ldg];
+  }
+  for (int t = tid; t < s; t += 256) piv[t] = t;
+  __syncthreads();
+  int rank = s;
+  for (int j = 0; j < s; ++j) {
+    if (tid == 0) {
+      int p = j;
+      double best = S[j * ld + j];
+      for (int c = j + 1; c < s; ++c) {
+        const double d = S[c * ld + c];
+        if (d > best) {
+          best = d;
+          p = c;
+        }
+      }
+      s_pvt = p;
+      s_stop = (best > 0.0) ? 0 : 1;
+    }
+    __syncthreads();
+    if (s_stop) {
+      rank = j;
+      break;
+    }
+    const int p = s_pvt;
+    if (p != j) {
+      for (int q = tid; q < s; q += 256) {
+        const double t0 = S[q * ld + j];
+        S[q * ld + j] = S[q * ld + p];
+        S[q * ld + p] = t0;
+      }
+      __syncthreads();
+      for (int c = j + tid; c < s; c += 256) {
+        const double t0 = S[j * ld + c];
+        S[j * ld + c] = S[p * ld + c];
+        S[p * ld + c] = t0;
+      }
+      if (tid == 0) {
+        const int t0 = piv[j];
+        piv[j] = piv[p];
+        piv[p] = t0;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const double rjj = sqrt(S[j * ld + j]);
+      S[j * ld + j] = rjj;
+      s_rjj = rjj;
+    }
+    __syncthreads();
+    const double rjj = s_rjj;
+    for (int c = j + 1 + tid; c < s; c += 256) S[j * ld + c] /= rjj;
+    __syncthreads();
+    const int nt = s - j - 1;
+    for (int idx = tid; idx < nt * nt; idx += 256) {
+      const int c1 = j + 1 + idx / nt, c2 = j + 1 + idx % nt;
+      S[c1 * ld + c2] -= S[j * ld + c1] * S[j * ld + c2];
+    }
+    __syncthreads();
+  }
+  // rows at and beyond the numerical rank are zero
+  for (int idx = tid; idx < s * s; idx += 256) {
+    const int r = idx / s, c = idx % s;
+    if (r >= rank && c >= r) S[r * ld + c] = 0.0;
+  }
+  __syncthreads();
+  for (int r = tid; r < s; r += 256) {
+    double acc = 0.0;
+    for (int c = r; c < s; ++c) acc += S[r * ld + c] * S[r * ld + c];
+    sqr[r] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // csum[t] = sqr[s-1] + ... + sqr[s-1-t] accumulated in that order
+    double total = 0.0;
+    for (int t = 0; t < s; ++t) total += sqr[s - 1 - t];
+    const double thr = total / (double)(filter_mode == 0 ? s : b);
+    double csum = 0.0;
+    int k = 0;
+    for (; k < s; ++k) {
+      csum += sqr[s - 1 - k];
+      if (csum >= thr) break;
+    }
+    int bt = s - k;
+    if (rank == 0 || !(total > 0.0)) {
+      bt = 0;
+      bs->status = 1;
+    }
+    if (bt > rank) bt = rank;
+    s_bt = bt;
+  }
+  __syncthreads();
+  const int bt = s_bt;
+  for (int t = tid; t < bt; t += 256) {
+    Jk[t] = J[piv[t]];
+    sj[t] = J[piv[t]];
+  }
+  for (int idx = tid; idx < bt * bt; idx += 256) {
+    const int r = idx % bt, c = idx / bt;
+    Rb[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
+  }
+  __syncthreads();
+  // inverse of the upper-triangular Rb, one column per thread
+  for (int c = tid; c < bt; c += 256) {
+    double* col = Rinv + (int64_t)c * b;
+    for (int r = c + 1; r < bt; ++r) col[r] = 0.0;
+    col[c] = 1.0 / S[c * ld + c];
+    for (int r = c - 1; r >= 0; --r) {
+      double acc = 0.0;
+      for (int l = r + 1; l <= c; ++l) acc += S[r * ld + l] * col[l];
+      col[r] = -acc / S[r * ld + r];
+    }
+  }
+  if (tid == 0) {
+    for (int a = 1; a < bt; ++a) {
+      const int v = sj[a];
+      int q = a - 1;
+      while (q >= 0 && sj[q] > v) {
+        sj[q + 1] = sj[q];
+        --q;
+      }
+      sj[q + 1] = v;
+    }
+    int nin = 0, nout = 0, ptr = 0;
+    for (int a = 0; a < bt; ++a)
+      if ((int64_t)sj[a] >= i + bt) Jin[nin++] = sj[a];
+    for (int64_t q = i; q < i + bt; ++q) {
+      while (ptr < bt && (int64_t)sj[ptr] < q) ++ptr;
+      const bool member = (ptr < bt && (int64_t)sj[ptr] == q);
+      if (!member) Jout[nout++] = (int)q;
+    }
+    if (nin != nout) bs->status = 3;
+    bs->bt = bt;
+    bs->nswap = nin;
+  }
+}
+
+hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
+                                      int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout) {
+  if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
+  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
+  if (lds > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_filter,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_pivchol_filter, dim3(1), dim3(256), lds, st, G, ldg, b, filter_mode, i, J, bs,
+                     Rb, Rinv, Jk, Jin, Jout);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Second Cholesky-QR pass: G2 = Q1^T Q1 = R2^T R2; R2inv = R2^{-1};
+// R(i:i+bt, i:i+bt) = R2 * Rb (upper triangular, strictly-lower part zeroed).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, int64_t ldg, int b,
+                                               RbrpBlockState* bs, const double* __restrict__ Rb,
+                                               double* __restrict__ R2inv, double* __restrict__ Rbig,
+                                               int64_t ldr, int64_t i) {
+  extern __shared__ double S[];
+  __shared__ int s_bad;
+  const int tid = threadIdx.x;
+  const int bt = bs->bt;
+  if (bt <= 0) return;
+  const int ld = bt + 1;
+  for (int idx = tid; idx < bt * bt; idx += 256) {
+    const int r = idx % bt, c = idx / bt;
+    S[r * ld + c] = G2[r + (int64_t)c * ldg];
+  }
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  for (int j = 0; j < bt; ++j) {
+    if (tid == 0) {
+      const double d = S[j * ld + j];
+      if (!(d > 0.0)) s_bad = 1;
+      S[j * ld + j] = sqrt(d > 0.0 ? d : 1.0);
+    }
+    __syncthreads();
+    const double rjj = S[j * ld + j];
+    for (int c = j + 1 + tid; c < bt; c += 256) S[j * ld + c] /= rjj;
+    __syncthreads();
+    const int nt = bt - j - 1;
+    for (int idx = tid; idx < nt * nt; idx += 256) {
+      const int c1 = j + 1 + idx / nt, c2 = j + 1 + idx % nt;
+      S[c1 * ld + c2] -= S[j * ld + c1] * S[j * ld + c2];
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && s_bad) bs->status = 2;
+  for (int c = tid; c < bt; c += 256) {
+    double* col = R2inv + (int64_t)c * b;
+    for (int r = c + 1; r < bt; ++r) col[r] = 0.0;
+    col[c] = 1.0 / S[c * ld + c];
+    for (int r = c - 1; r >= 0; --r) {
+      double acc = 0.0;
+      for (int l = r + 1; l <= c; ++l) acc += S[r * ld + l] * col[l];
+      col[r] = -acc / S[r * ld + r];
+    }
+  }
+  for (int idx = tid; idx < bt * bt; idx += 256) {
+    const int r = idx % bt, c = idx / bt;
+    double v = 0.0;
+    if (r <= c)
+      for (int l = r; l <= c; ++l) v += S[r * ld + l] * Rb[l + (int64_t)c * b];
+    Rbig[(i + r) + (i + c) * ldr] = v;
+  }
+}
+
+hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
+                             RbrpBlockState* bs, const double* Rb, double* R2inv, double* Rbig,
+                             int64_t ldr, int64_t i) {
+  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
+  if (lds > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_chol2,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, st, G2, ldg, b, bs, Rb, R2inv, Rbig, ldr, i);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric-difference permutation (gpu/rbrp.cu:237-294):
+//   perm: tmp = perm[Jk]; perm[Jin] = perm[Jout]; perm[i:i+bt] = tmp
+//   diags: diags[Jin] = diags[Jout]
+//   R(0:i, :) columns: same move as perm (matlab/RBRP.m:91-93)
+//   Aw columns: Aw(:, Jin) = Aw(:, Jout) (the block columns then receive Q_b)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_perm_small(int64_t i, RbrpBlockState* bs,
+                                                    const int* __restrict__ Jk,
+                                                    const int* __restrict__ Jin,
+                                                    const int* __restrict__ Jout, int* perm,
+                                                    double* diags) {
+  __shared__ int tk[SAMPLE_MAXB], to[SAMPLE_MAXB];
+  __shared__ double dk[SAMPLE_MAXB], dout[SAMPLE_MAXB];
+  const int bt = bs->bt, ns = bs->nswap;
+  const int tid = threadIdx.x;
+  if (tid < bt) {
+    tk[tid] = perm[Jk[tid]];
+    dk[tid] = diags[Jk[tid]];
+  }
+  if (tid < ns) {
+    to[tid] = perm[Jout[tid]];
+    dout[tid] = diags[Jout[tid]];
+  }
+  __syncthreads();
+  if (tid < ns) {
+    perm[Jin[tid]] = to[tid];
+    diags[Jin[tid]] = dout[tid];
+  }
+  __syncthreads();
+  if (tid < bt) {
+    perm[i + tid] = tk[tid];
+    diags[i + tid] = dk[tid];
+  }
+}
+
+hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
+                                  const int* Jin, const int* Jout, int* perm, double* diags) {
+  hipLaunchKernelGGL(k_perm_small, dim3(1), dim3(256), 0, st, i, bs, Jk, Jin, Jout, perm, diags);
+  return hipGetLastError();
+}
+
+__global__ void k_perm_gather_r(int64_t rows, RbrpBlockState* bs, const double* __restrict__ R,
+                                int64_t ldr, const int* __restrict__ Jk,
+                                const int* __restrict__ Jout, double* __restrict__ tmpk,
+                                double* __restrict__ tmpo, int64_t ldt) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows) return;
+  if (c < bs->bt) tmpk[r + c * ldt] = R[r + (int64_t)Jk[c] * ldr];
+  if (c < bs->nswap) tmpo[r + c * ldt] = R[r + (int64_t)Jout[c] * ldr];
+}
+
+hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
+                                     const double* R, int64_t ldr, const int* Jk, const int* Jout,
+                                     double* tmpk, double* tmpo, int64_t ldt) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_perm_gather_r, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
+                     st, rows, bs, R, ldr, Jk, Jout, tmpk, tmpo, ldt);
+  return hipGetLastError();
+}
+
+__global__ void k_perm_scatter_r(int64_t rows, int64_t i, RbrpBlockState* bs, double* __restrict__ R,
+                                 int64_t ldr, const int* __restrict__ Jin,
+                                 const double* __restrict__ tmpk, const double* __restrict__ tmpo,
+                                 int64_t ldt) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows) return;
+  if (c < bs->nswap) R[r + (int64_t)Jin[c] * ldr] = tmpo[r + c * ldt];
+  if (c < bs->bt) R[r + (i + c) * ldr] = tmpk[r + c * ldt];
+}
+
+hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64_t i,
+                                      RbrpBlockState* bs, double* R, int64_t ldr, const int* Jin,
+                                      const double* tmpk, const double* tmpo, int64_t ldt) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_perm_scatter_r, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256),
+                     0, st, rows, i, bs, R, ldr, Jin, tmpk, tmpo, ldt);
+  return hipGetLastError();
+}
+
+__global__ void k_move_cols(int64_t rows, RbrpBlockState* bs, double* __restrict__ Aw, int64_t ld,
+                            const int* __restrict__ Jin, const int* __restrict__ Jout) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows || c >= bs->nswap) return;
+  Aw[r + (int64_t)Jin[c] * ld] = Aw[r + (int64_t)Jout[c] * ld];
+}
+
+hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
+                                 int64_t ld, const int* Jin, const int* Jout) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_move_cols, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0, st,
+                     rows, bs, Aw, ld, Jin, Jout);
+  return hipGetLastError();
+}
+
+// diags[i + bt + c] = sum_{t < ntiles} part[t * ldp + c], c < n - i - bt.
+__global__ void k_colnorm_reduce(const double* __restrict__ part, int64_t ldp, int64_t ntiles,
+                                 int64_t i, int64_t n, RbrpBlockState* bs,
+                                 double* __restrict__ diags) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t c0 = i + bs->bt;
+  if (c0 + c >= n) return;
+  double acc = 0.0;
+  for (int64_t t = 0; t < ntiles; ++t) acc += part[t * ldp + c];
+  diags[c0 + c] = acc;
+}
+
+hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags) {
+  const int64_t cmax = n - i;
+  if (cmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 255) / 256)), dim3(256), 0, st, part,
+                     ldp, ntiles, i, n, bs, diags);
+  return hipGetLastError();
+}
+
+// resid = sum(diags[i + bt : n]); err = resid / N0 (gpu/rbrp.cu:467).
+__global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__ diags, int64_t i,
+                                                      int64_t n, RbrpBlockState* bs) {
+  __shared__ double red[16];
+  const int64_t c0 = i + bs->bt;
+  double acc = 0.0;
+  for (int64_t j = c0 + threadIdx.x; j < n; j += 1024) acc += diags[j];
+  const double t = rbrp_block_sum(acc, red);
+  if (threadIdx.x == 0) {
+    bs->resid = t;
+    bs->err = (bs->N0 > 0.0) ? t / bs->N0 : 0.0;
+  }
+}
+
+hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
+                                   RbrpBlockState* bs) {
+  hipLaunchKernelGGL(k_resid_total, dim3(1), dim3(1024), 0, st, diags, i, n, bs);
+  return hipGetLastError();
+}
+
+// dst(0:rows, 0:cols) = src(0:rows, 0:cols)
+__global__ void k_copy_block(int64_t rows, const double* __restrict__ src, int64_t lds,
+                             double* __restrict__ dst, int64_t ldd) {
+  const int64_t r = (int64_t)blockIdx.y * 256 + threadIdx.x;
+  const int64_t c = blockIdx.x;
+  if (r >= rows) return;
+  dst[r + c * ldd] = src[r + c * lds];
+}
+
+hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                  int64_t lds, double* dst, int64_t ldd) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_block, dim3((unsigned)cols, (unsigned)((rows + 255) / 256)), dim3(256), 0,
+                     st, rows, src, lds, dst, ldd);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal-block back substitution of X = R11^{-1} R12: X(0:nb, c) =
+// Rd^{-1} X(0:nb, c) for every column c; Rd is nb x nb upper (nb <= 64).
+// One wave per 64 columns, Rd and the right-hand sides resident in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_trsm_diag(int nb, int64_t ncols, const double* __restrict__ Rd,
+                                                  int64_t ldr, double* __restrict__ X, int64_t ldx) {
+  __shared__ double Rs[64 * 65];
+  __shared__ double xs[64 * 65];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += 64) {
+    const int r = idx % nb, c = idx / nb;
+    Rs[r * 65 + c] = Rd[r + (int64_t)c * ldr];
+  }
+  const int64_t c = (int64_t)blockIdx.x * 64 + tid;
+  __syncthreads();
+  if (c >= ncols) return;
+  double* xc = X + c * ldx;
+  for (int r = 0; r < nb; ++r) xs[r * 65 + tid] = xc[r];
+  for (int r = nb - 1; r >= 0; --r) {
+    double v = xs[r * 65 + tid];
+    for (int l = r + 1; l < nb; ++l) v -= Rs[r * 65 + l] * xs[l * 65 + tid];
+    xs[r * 65 + tid] = v / Rs[r * 65 + r];
+  }
+  for (int r = 0; r < nb; ++r) xc[r] = xs[r * 65 + tid];
+}
+
+hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,
+                                 int64_t ldr, double* X, int64_t ldx) {
+  if (nb <= 0 || ncols <= 0) return hipSuccess;
+  if (nb > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_trsm_diag, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, (int)nb, ncols,
+                     Rdiag, ldr, X, ldx);
+  return hipGetLastError();
+}
+
+// out[t] = uniform number draw0 + t of the stream (key0, key1, ctr).
+__global__ void k_uniforms(uint64_t key0, uint64_t key1, uint64_t c0, uint64_t c1, uint64_t c2,
+                           uint64_t c3, uint64_t draw0, int64_t count, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const uint64_t base[4] = {c0, c1, c2, c3};
+  out[t] = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draw0 + (uint64_t)t));
+}
+
+hipError_t rbrp_launch_uniforms(hipStream_t st, uint64_t key0, uint64_t key1, const uint64_t ctr[4],
+                                uint64_t draw0, int64_t count, double* out) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_uniforms, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, key0, key1,
+                     ctr[0], ctr[1], ctr[2], ctr[3], draw0, count, out);
+  return hipGetLastError();
+}

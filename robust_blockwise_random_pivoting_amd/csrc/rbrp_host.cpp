@@ -1,0 +1,532 @@
+// Host orchestration of the RBRP column ID and the C ABI (include/rbrp.h).
+//
+// One handle = one stream + one cached device workspace.  The block loop
+// follows RBRP() of gpu/rbrp.cu:333-503 in right-looking form: per block the
+// host launches the sampler, the re-orthogonalisation, the panel (Gram +
+// pivoted Cholesky + filter + CholQR2), the permutation and the trailing
+// projection/update, then reads one small state record (kept count, residual)
+// back to decide whether to continue.  All sizes that depend on the block's
+// data (sampled count s, kept count bt) are read by the kernels from device
+// memory, so no other host round trip happens inside a block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/rbrp.h"
+#include "rbrp_kernels.h"
+
+#define RBRP_MAX_B 128
+
+#define HIPCHK(x)                               \
+  do {                                          \
+    hipError_t e_ = (x);                        \
+    if (e_ != hipSuccess) return (int)e_;       \
+  } while (0)
+
+namespace {
+
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+struct Workspace {
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  // carved pointers
+  double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2inv, *Cbuf, *Qbuf, *P, *normpart, *tmpk,
+      *tmpo;
+  int *J, *Jk, *Jin, *Jout;
+  RbrpBlockState* st;
+  int64_t Pcap;
+};
+
+struct Carver {
+  size_t off = 0;
+  template <typename T>
+  size_t take(int64_t count) {
+    size_t o = off;
+    off += round_up((int64_t)(count * sizeof(T)), 256);
+    return o;
+  }
+};
+
+}  // namespace
+
+struct rbrp_handle_s {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  Workspace ws;
+  RbrpBlockState* host_state = nullptr;  // pinned
+  bool timing = false;
+  rbrp_phase_stats_t stats;
+  std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b) {
+  const int64_t ldr = std::max<int64_t>(kmax, 1);
+  const int64_t Pcap = 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
+  Carver c;
+  size_t oAw = c.take<double>(mp * n);
+  size_t oR = c.take<double>(ldr * n);
+  size_t odiags = c.take<double>(n);
+  size_t ow = c.take<double>(n);
+  size_t ocdf = c.take<double>(n);
+  size_t oG = c.take<double>(b * b);
+  size_t oG2 = c.take<double>(b * b);
+  size_t oRb = c.take<double>(b * b);
+  size_t oRinv = c.take<double>(b * b);
+  size_t oR2inv = c.take<double>(b * b);
+  size_t oCbuf = c.take<double>(ldr * b);
+  size_t oQbuf = c.take<double>(mp * b);
+  size_t oP = c.take<double>(Pcap);
+  size_t onp = c.take<double>((mp / 64) * n);
+  size_t otk = c.take<double>(ldr * b);
+  size_t oto = c.take<double>(ldr * b);
+  size_t oJ = c.take<int>(b);
+  size_t oJk = c.take<int>(b);
+  size_t oJin = c.take<int>(b);
+  size_t oJout = c.take<int>(b);
+  size_t ost = c.take<RbrpBlockState>(1);
+  const size_t need = c.off;
+  Workspace& ws = h->ws;
+  if (ws.arena_bytes < need) {
+    if (ws.arena) {
+      (void)hipFree(ws.arena);
+      ws.arena = nullptr;
+      ws.arena_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&ws.arena, need);
+    if (e != hipSuccess) {
+      ws.arena = nullptr;
+      return RBRP_ERR_ALLOC;
+    }
+    ws.arena_bytes = need;
+  }
+  char* base = (char*)ws.arena;
+  ws.Aw = (double*)(base + oAw);
+  ws.R = (double*)(base + oR);
+  ws.diags = (double*)(base + odiags);
+  ws.w = (double*)(base + ow);
+  ws.cdf = (double*)(base + ocdf);
+  ws.G = (double*)(base + oG);
+  ws.G2 = (double*)(base + oG2);
+  ws.Rb = (double*)(base + oRb);
+  ws.Rinv = (double*)(base + oRinv);
+  ws.R2inv = (double*)(base + oR2inv);
+  ws.Cbuf = (double*)(base + oCbuf);
+  ws.Qbuf = (double*)(base + oQbuf);
+  ws.P = (double*)(base + oP);
+  ws.normpart = (double*)(base + onp);
+  ws.tmpk = (double*)(base + otk);
+  ws.tmpo = (double*)(base + oto);
+  ws.J = (int*)(base + oJ);
+  ws.Jk = (int*)(base + oJk);
+  ws.Jin = (int*)(base + oJin);
+  ws.Jout = (int*)(base + oJout);
+  ws.st = (RbrpBlockState*)(base + ost);
+  ws.Pcap = Pcap;
+  return 0;
+}
+
+// Split-K shape for a TN product of Mmax x Nmax over K rows: aim for about
+// 1024 workgroups, keep the partials within the workspace.
+void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pcap, int* nsplit,
+                  int64_t* kchunk) {
+  const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + 63) / 64);
+  int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
+  ns = std::min<int64_t>(ns, std::max<int64_t>(1, K / 64));
+  int64_t kc = round_up((K + ns - 1) / ns, 16);
+  ns = (K + kc - 1) / kc;
+  *nsplit = (int)std::max<int64_t>(ns, 1);
+  *kchunk = kc;
+}
+
+// Per-phase event timing: events are recorded around launches during a block
+// and read after the block's synchronisation.
+struct PhaseTimer {
+  rbrp_handle_t h;
+  struct Rec {
+    int phase;
+    int e0, e1;
+    int launches;
+    double flops, bytes;
+  };
+  std::vector<Rec> recs;
+  int next = 0;
+  explicit PhaseTimer(rbrp_handle_t hh) : h(hh) {}
+  hipEvent_t ev(int idx) {
+    while ((int)h->events.size() <= idx) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      h->events.push_back(e);
+    }
+    return h->events[idx];
+  }
+  int begin(int phase) {
+    if (!h->timing) return -1;
+    Rec r{phase, next, next + 1, 0, 0.0, 0.0};
+    next += 2;
+    hipEvent_t e = ev(r.e0);
+    if (e) (void)hipEventRecord(e, h->stream);
+    recs.push_back(r);
+    return (int)recs.size() - 1;
+  }
+  void end(int id, int launches, double flops = 0.0, double bytes = 0.0) {
+    if (id < 0) return;
+    Rec& r = recs[id];
+    r.launches = launches;
+    r.flops = flops;
+    r.bytes = bytes;
+    hipEvent_t e = ev(r.e1);
+    if (e) (void)hipEventRecord(e, h->stream);
+  }
+  // Set the work figures of a record once the block's sizes are known.
+  void set_work(int id, double flops, double bytes) {
+    if (id < 0) return;
+    recs[id].flops = flops;
+    recs[id].bytes = bytes;
+  }
+  void flush() {  // after a stream synchronisation
+    for (const Rec& r : recs) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, h->events[r.e0], h->events[r.e1]) == hipSuccess)
+        h->stats.ms[r.phase] += ms;
+      h->stats.launches[r.phase] += r.launches;
+      h->stats.flops[r.phase] += r.flops;
+      h->stats.bytes[r.phase] += r.bytes;
+    }
+    recs.clear();
+    next = 0;
+  }
+};
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rbrp_version(void) { return "rbrp-mi355x 0.1 (gfx950, FP64 MFMA)"; }
+
+void rbrp_opts_default(rbrp_opts_t* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->b = 64;
+  o->tol = 0.0;
+  o->precision = RBRP_PREC_FP64;
+  o->filter = RBRP_FILTER_SAMPLED;
+}
+
+void rbrp_philox4x64_10_host(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]) {
+  Philox4x64Out o = rbrp_philox4x64_10(ctr, key[0], key[1]);
+  for (int q = 0; q < 4; ++q) out[q] = o.v[q];
+}
+
+int rbrp_create(rbrp_handle_t* handle, void* stream) {
+  if (!handle) return -1;
+  *handle = nullptr;
+  rbrp_handle_t h = new (std::nothrow) rbrp_handle_s();
+  if (!h) return RBRP_ERR_ALLOC;
+  h->stream = (hipStream_t)stream;
+  std::memset(&h->stats, 0, sizeof(h->stats));
+  hipError_t e = hipGetDevice(&h->device);
+  if (e != hipSuccess) {
+    delete h;
+    return (int)e;
+  }
+  e = hipHostMalloc((void**)&h->host_state, sizeof(RbrpBlockState), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    delete h;
+    return (int)e;
+  }
+  *handle = h;
+  return 0;
+}
+
+int rbrp_destroy(rbrp_handle_t h) {
+  if (!h) return -1;
+  if (h->ws.arena) (void)hipFree(h->ws.arena);
+  if (h->host_state) (void)hipHostFree(h->host_state);
+  for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
+  delete h;
+  return 0;
+}
+
+int rbrp_set_stream(rbrp_handle_t h, void* stream) {
+  if (!h) return -1;
+  h->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int rbrp_set_timing(rbrp_handle_t h, int enable) {
+  if (!h) return -1;
+  h->timing = enable != 0;
+  return 0;
+}
+
+int rbrp_get_phase_stats(rbrp_handle_t h, rbrp_phase_stats_t* out) {
+  if (!h) return -1;
+  if (!out) return -2;
+  *out = h->stats;
+  return 0;
+}
+
+int rbrp_reset_phase_stats(rbrp_handle_t h) {
+  if (!h) return -1;
+  std::memset(&h->stats, 0, sizeof(h->stats));
+  return 0;
+}
+
+int rbrp_release_workspace(rbrp_handle_t h) {
+  if (!h) return -1;
+  if (h->ws.arena) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->ws.arena);
+  }
+  h->ws.arena = nullptr;
+  h->ws.arena_bytes = 0;
+  return 0;
+}
+
+int rbrp_uniforms_d(rbrp_handle_t h, uint64_t seed, uint64_t offset, uint64_t draw0, int64_t count,
+                    double* dOut) {
+  if (!h) return -1;
+  if (count < 0) return -5;
+  if (count > 0 && !dOut) return -6;
+  const uint64_t ctr[4] = {offset, 0, 0, 0};
+  HIPCHK(rbrp_launch_uniforms(h->stream, seed, 0, ctr, draw0, count, dOut));
+  return 0;
+}
+
+int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                 const rbrp_opts_t* opts, rbrp_result_t* res) {
+  const double t_start = now_s();
+  if (!h) return -1;
+  if (m < 0) return -3;
+  if (n < 0) return -4;
+  if (lda < std::max<int64_t>(m, 1)) return -5;
+  if (!opts) return -6;
+  if (!res) return -7;
+  if (m > 0 && n > 0 && !dA) return -2;
+  if (opts->b < 1 || opts->b > RBRP_MAX_B) return -6;
+  if (!(opts->tol >= 0.0) || !std::isfinite(opts->tol)) return -6;
+  if (opts->max_rank < 0) return -6;
+  if (opts->precision != RBRP_PREC_FP64) return -6;
+  if (opts->filter != RBRP_FILTER_SAMPLED && opts->filter != RBRP_FILTER_NOMINAL) return -6;
+  if (n > 0 && !res->perm) return -7;
+  if (m > 0x7fffffffLL || n > 0x7fffffffLL) return -3;
+
+  const int64_t p = std::min(m, n);
+  const int64_t kmax = opts->max_rank > 0 ? std::min<int64_t>(opts->max_rank, p) : p;
+  if (res->X && res->ldx < std::max<int64_t>(kmax, 1)) return -7;
+  if (res->trace_cap > 0 && (!res->ranks || !res->error || !res->t_ranks)) return -7;
+  res->k = 0;
+  res->nblocks = 0;
+  res->err_final = 0.0;
+  res->draws = 0;
+  hipStream_t st = h->stream;
+
+  if (n == 0) {
+    res->t_total = now_s() - t_start;
+    return 0;
+  }
+  HIPCHK(rbrp_launch_iota(st, res->perm, n));
+  if (m == 0) {
+    HIPCHK(hipStreamSynchronize(st));
+    res->t_total = now_s() - t_start;
+    return 0;
+  }
+
+  const int b = (int)opts->b;
+  const int64_t mp = round_up(m, 64);
+  const int64_t ldr = std::max<int64_t>(kmax, 1);
+  {
+    int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1));
+    if (e) return e;
+  }
+  Workspace& ws = h->ws;
+  int* perm = res->perm;
+  const double tol2 = opts->tol * opts->tol;
+  const uint64_t key0 = opts->seed, key1 = 0;
+  const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
+  const int64_t ntiles = mp / 64;
+  PhaseTimer pt(h);
+
+  HIPCHK(hipMemsetAsync(ws.R, 0, sizeof(double) * ldr * n, st));
+  int ph = pt.begin(RBRP_PHASE_NORMS);
+  HIPCHK(rbrp_launch_copy_colnorm(st, dA, lda, m, mp, n, ws.Aw, ws.diags));
+  pt.end(ph, 1, 2.0 * m * n, 8.0 * m * n + 8.0 * mp * n + 8.0 * n);
+  HIPCHK(rbrp_launch_sum_to_state(st, ws.diags, n, ws.st, 0));
+  HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  pt.flush();
+  const double N0 = h->host_state->N0;
+  if (!std::isfinite(N0)) return RBRP_ERR_NUMERIC;
+
+  int64_t i = 0, nb = 0;
+  double err = (N0 > 0.0) ? 1.0 : 0.0;
+  while (i < kmax && N0 > 0.0) {
+    // ---- sample s columns of [i, n) ------------------------------------
+    ph = pt.begin(RBRP_PHASE_SAMPLE);
+    HIPCHK(rbrp_launch_sample(st, ws.diags, i, n, kmax, b, key0, key1, ctr, ws.w, ws.cdf, ws.st, ws.J));
+    pt.end(ph, 1, 0.0, 16.0 * (n - i));
+    const int* d_s = &ws.st->s;
+    const int* d_bt = &ws.st->bt;
+
+    // ---- second Gram-Schmidt pass of the sampled columns ----------------
+    int ph_cgs = pt.begin(RBRP_PHASE_CGS);
+    if (i > 0) {
+      int ns;
+      int64_t kc;
+      choose_split(i, b, mp, i, ws.Pcap, &ns, &kc);
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, mp, ns, kc,
+                                 mat(ws.Aw, mp), mat_cols(ws.Aw, mp, ws.J), ws.P, i, i * (int64_t)b));
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, ws.P, i,
+                                       i * (int64_t)b, ns, mat(ws.Cbuf, ldr), GEMM_SET));
+      HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
+      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_s, 1), b, dim_static(i), mat(ws.Aw, mp),
+                                 mat(ws.Cbuf, ldr), mat_cols(ws.Aw, mp, ws.J), GEMM_SUB, nullptr, 0));
+    }
+    pt.end(ph_cgs, i > 0 ? 4 : 0);
+
+    // ---- panel: Gram, pivoted Cholesky + filter, CholQR2 ----------------
+    ph = pt.begin(RBRP_PHASE_PANEL);
+    {
+      int ns;
+      int64_t kc;
+      choose_split(b, b, mp, b, ws.Pcap, &ns, &kc);
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, mp, ns, kc,
+                                 mat_cols(ws.Aw, mp, ws.J), mat_cols(ws.Aw, mp, ws.J), ws.P, b,
+                                 (int64_t)b * b));
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, ws.P, b,
+                                       (int64_t)b * b, ns, mat(ws.G, b), GEMM_SET));
+      HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
+                                        ws.Jk, ws.Jin, ws.Jout));
+      // Q1 = V(:, Jk) Rb^{-1}
+      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
+                                 mat_cols(ws.Aw, mp, ws.Jk), mat(ws.Rinv, b), mat(ws.Qbuf, mp),
+                                 GEMM_SET, nullptr, 0));
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, mp, ns, kc,
+                                 mat(ws.Qbuf, mp), mat(ws.Qbuf, mp), ws.P, b, (int64_t)b * b));
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
+                                       (int64_t)b * b, ns, mat(ws.G2, b), GEMM_SET));
+      HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.Rb, ws.R2inv, ws.R, ldr, i));
+    }
+    pt.end(ph, 7);
+
+    // ---- symmetric-difference permutation -------------------------------
+    ph = pt.begin(RBRP_PHASE_PERMUTE);
+    HIPCHK(rbrp_launch_perm_small(st, i, ws.st, ws.Jk, ws.Jin, ws.Jout, perm, ws.diags));
+    if (i > 0) {
+      HIPCHK(rbrp_launch_perm_gather_r(st, i, b, ws.st, ws.R, ldr, ws.Jk, ws.Jout, ws.tmpk, ws.tmpo, ldr));
+      HIPCHK(rbrp_launch_perm_scatter_r(st, i, b, i, ws.st, ws.R, ldr, ws.Jin, ws.tmpk, ws.tmpo, ldr));
+    }
+    HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
+    // Q_b = Q1 R2^{-1} into the block's columns of the working matrix
+    HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1), mat(ws.Qbuf, mp),
+                               mat(ws.R2inv, b), mat_off(ws.Aw, mp, i), GEMM_SET, nullptr, 0));
+    pt.end(ph, i > 0 ? 5 : 3);
+
+    // ---- trailing projection and update ----------------------------------
+    const int64_t nrem_max = n - i - 1;
+    int ph_proj = -1, ph_upd = -1;
+    if (nrem_max > 0) {
+      int ns;
+      int64_t kc;
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc);
+      ph_proj = pt.begin(RBRP_PHASE_PROJECT);
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max, mp,
+                                 ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt), ws.P, b,
+                                 (int64_t)b * nrem_max));
+      pt.end(ph_proj, 1);
+      ph = pt.begin(RBRP_PHASE_PROJECT_RED);
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
+                                       ws.P, b, (int64_t)b * nrem_max, ns,
+                                       mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET));
+      pt.end(ph, 1);
+      ph_upd = pt.begin(RBRP_PHASE_UPDATE);
+      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
+                                 mat_off(ws.Aw, mp, i), mat_off(ws.R + i, ldr, i, d_bt),
+                                 mat_off(ws.Aw, mp, i, d_bt), GEMM_SUB, ws.normpart, nrem_max));
+      pt.end(ph_upd, 1);
+      ph = pt.begin(RBRP_PHASE_RESID);
+      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, ntiles, i, n, ws.st, ws.diags));
+    } else {
+      ph = pt.begin(RBRP_PHASE_RESID);
+    }
+    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st));
+    pt.end(ph, nrem_max > 0 ? 2 : 1);
+
+    HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const RbrpBlockState hs = *h->host_state;
+    if (hs.status != 0) {
+      pt.flush();
+      return RBRP_ERR_NUMERIC;
+    }
+    if (hs.s <= 0 || hs.bt <= 0) {
+      pt.recs.clear();
+      pt.next = 0;
+      break;
+    }
+    const int64_t bt = hs.bt, s = hs.s;
+    const int64_t nrem = n - i - bt;
+    if (h->timing) {
+      // algorithmic work, now that the block's sizes are known
+      pt.set_work(ph_cgs, i > 0 ? 4.0 * m * i * s : 0.0, i > 0 ? 2.0 * 8.0 * m * i + 16.0 * m * s : 0.0);
+      pt.set_work(ph_proj, 2.0 * m * bt * nrem, 8.0 * (m * nrem + m * bt + bt * nrem));
+      pt.set_work(ph_upd, 2.0 * m * bt * nrem, 8.0 * (2.0 * m * nrem + m * bt + bt * nrem));
+    }
+    pt.flush();
+    res->draws = hs.draws;
+    err = hs.err;
+    i += bt;
+    if (nb < res->trace_cap) {
+      res->ranks[nb] = i;
+      res->error[nb] = err;
+      res->t_ranks[nb] = now_s() - t_start;
+    }
+    ++nb;
+    if (err <= tol2) break;
+  }
+  const int64_t k = i;
+  res->k = k;
+  res->nblocks = nb;
+  res->err_final = std::sqrt(std::max(err, 0.0));
+
+  // ---- X = R11^{-1} R12 (matlab/driver.m:86) --------------------------------
+  if (res->X && k > 0 && n > k) {
+    ph = pt.begin(RBRP_PHASE_SOLVE);
+    const int64_t nc = n - k;
+    HIPCHK(rbrp_launch_copy_block(st, k, nc, ws.R + k * ldr, ldr, res->X, res->ldx));
+    const int64_t nbk = (k + 63) / 64;
+    int launches = 1;
+    for (int64_t bi = nbk - 1; bi >= 0; --bi) {
+      const int64_t r0 = bi * 64, r1 = std::min<int64_t>(k, r0 + 64);
+      if (r1 < k) {
+        HIPCHK(rbrp_launch_gemm_nn(st, r1 - r0, dim_static(nc), nc, dim_static(k - r1),
+                                   mat_off(ws.R + r0, ldr, r1), mat(res->X + r1, res->ldx),
+                                   mat(res->X + r0, res->ldx), GEMM_SUB, nullptr, 0));
+        ++launches;
+      }
+      HIPCHK(rbrp_launch_trsm_diag(st, r1 - r0, nc, ws.R + r0 + r0 * ldr, ldr, res->X + r0, res->ldx));
+      ++launches;
+    }
+    pt.end(ph, launches, (double)k * k * nc, 8.0 * (0.5 * k * k + 2.0 * k * nc));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  pt.flush();
+  res->t_total = now_s() - t_start;
+  return 0;
+}
+
+}  // extern "C"

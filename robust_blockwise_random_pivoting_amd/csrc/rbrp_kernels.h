@@ -1,0 +1,112 @@
+// Kernel launchers of the RBRP HIP library.  Each kernel translation unit
+// exports plain host functions that launch its kernels on a given stream, so
+// the host orchestration never names a __global__ symbol directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rbrp_common.h"
+
+// A size that is either static (`base`) or `base + sign * (*dyn)` read on the
+// device, so one launch shape serves every block of the loop.
+struct DimSpec {
+  int64_t base;
+  const int* dyn;
+  int sign;
+};
+
+static inline DimSpec dim_static(int64_t v) { return DimSpec{v, nullptr, 0}; }
+static inline DimSpec dim_dyn(int64_t base, const int* dyn, int sign) {
+  return DimSpec{base, dyn, sign};
+}
+
+// Column-major operand: column c lives at p + col(c) * ld, where col(c) is
+// cols[c] when an index list is given, else coff + (*coff_dyn) + c.
+struct MatRef {
+  double* p;
+  int64_t ld;
+  const int* cols;
+  int64_t coff;
+  const int* coff_dyn;
+};
+
+static inline MatRef mat(double* p, int64_t ld) { return MatRef{p, ld, nullptr, 0, nullptr}; }
+static inline MatRef mat_off(double* p, int64_t ld, int64_t coff, const int* coff_dyn = nullptr) {
+  return MatRef{p, ld, nullptr, coff, coff_dyn};
+}
+static inline MatRef mat_cols(double* p, int64_t ld, const int* cols) {
+  return MatRef{p, ld, cols, 0, nullptr};
+}
+
+#if defined(__HIPCC__)
+__device__ inline int64_t dim_eval(const DimSpec& d) {
+  return d.dyn ? d.base + (int64_t)d.sign * (int64_t)(*d.dyn) : d.base;
+}
+__device__ inline double* mat_col(const MatRef& r, int64_t c) {
+  const int64_t cc = r.cols ? (int64_t)r.cols[c] : (r.coff + (r.coff_dyn ? (int64_t)(*r.coff_dyn) : 0) + c);
+  return r.p + cc * r.ld;
+}
+#endif
+
+enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
+
+// ---- k_gemm.hip -----------------------------------------------------------
+// Split-K TN product: P[z](r, c) = sum_{kk in chunk z} A(kk, r) * B(kk, c),
+// r < M, c < N.  K must be a multiple of 16 and rows kk >= true height must be
+// zero (the padded buffers guarantee it).  Grid: ceil(Nmax/64) x ceil(Mmax/64) x nsplit.
+hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                               int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B,
+                               double* P, int64_t ldp, int64_t pstride);
+
+// dst(r, c) (=|+=) sum_z P[z](r, c) in split order.
+hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                                     const double* P, int64_t ldp, int64_t pstride, int nsplit,
+                                     MatRef dst, int mode);
+
+// NN product with update: C(r, c) = A(r,:) B(:,c)  (GEMM_SET) or C -= A B (GEMM_SUB),
+// rows r < Mr, columns c < N, inner dimension K.  If normpart != nullptr, writes
+// normpart[tile_row * ldnp + c] = sum over the 64-row tile of C_new(r, c)^2.
+hipError_t rbrp_launch_gemm_nn(hipStream_t st, int64_t Mr, DimSpec N, int64_t Nmax, DimSpec K,
+                               MatRef A, MatRef B, MatRef C, int mode, double* normpart,
+                               int64_t ldnp);
+
+// ---- k_rbrp.hip -----------------------------------------------------------
+hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
+                                    int64_t mp, int64_t n, double* Aw, double* diags);
+hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
+                                    int which);
+hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n);
+hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
+                              int64_t kmax, int b, uint64_t key0, uint64_t key1,
+                              const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
+                              int* J);
+hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_dyn, int64_t ncols_max,
+                                const double* src, int64_t lds, double* dst, int64_t ldd,
+                                const int* cols);
+hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
+                                      int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout);
+hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
+                             RbrpBlockState* bs, const double* Rb, double* R2inv, double* Rbig,
+                             int64_t ldr, int64_t i);
+hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
+                                  const int* Jin, const int* Jout, int* perm, double* diags);
+hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
+                                     const double* R, int64_t ldr, const int* Jk, const int* Jout,
+                                     double* tmpk, double* tmpo, int64_t ldt);
+hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64_t i,
+                                      RbrpBlockState* bs, double* R, int64_t ldr, const int* Jin,
+                                      const double* tmpk, const double* tmpo, int64_t ldt);
+hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
+                                 int64_t ld, const int* Jin, const int* Jout);
+hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
+hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
+                                   RbrpBlockState* bs);
+hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                  int64_t lds, double* dst, int64_t ldd);
+hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,
+                                 int64_t ldr, double* X, int64_t ldx);
+hipError_t rbrp_launch_uniforms(hipStream_t st, uint64_t key0, uint64_t key1, const uint64_t ctr[4],
+                                uint64_t draw0, int64_t count, double* out);

@@ -1,0 +1,106 @@
+// Kernel-level entry points for the parity tests (include/rbrp_testing.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/rbrp_testing.h"
+#include "rbrp_kernels.h"
+
+#define HIPCHK(x)                         \
+  do {                                    \
+    hipError_t e_ = (x);                  \
+    if (e_ != hipSuccess) return (int)e_; \
+  } while (0)
+
+extern "C" {
+
+int rbrp_test_gemm_tn(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double* C, int64_t ldc, int nsplit) {
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K % 16) return -4;
+  if (nsplit < 1) return -11;
+  if (M == 0 || N == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t kc = (K + nsplit - 1) / nsplit;
+  kc = std::max<int64_t>(16, (kc + 15) / 16 * 16);
+  const int ns = (int)std::max<int64_t>(1, (K + kc - 1) / kc);
+  double* P = nullptr;
+  HIPCHK(hipMalloc(&P, sizeof(double) * M * N * ns));
+  hipError_t e = rbrp_launch_gemm_tn(st, dim_static(M), dim_static(N), M, N, K, ns, kc,
+                                     mat((double*)A, lda), mat((double*)B, ldb), P, M, M * N);
+  if (e == hipSuccess)
+    e = rbrp_launch_reduce_splits(st, dim_static(M), dim_static(N), M, N, P, M, M * N, ns,
+                                  mat(C, ldc), GEMM_SET);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(P);
+  return (int)e;
+}
+
+int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                      const double* B, int64_t ldb, double* C, int64_t ldc, int mode,
+                      double* normpart) {
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0) return -4;
+  if (mode < 0 || mode > 2) return -11;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(rbrp_launch_gemm_nn(st, M, dim_static(N), N, dim_static(K), mat((double*)A, lda),
+                             mat((double*)B, ldb), mat(C, ldc), mode, normpart, N));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int filter, int64_t i,
+                    const int32_t* J, int32_t* Jk, double* Rb, int32_t* Jin, int32_t* Jout,
+                    int32_t* bt, int32_t* nswap, int32_t* status) {
+  if (s < 1 || s > b) return -3;
+  if (b < 1 || b > 128) return -4;
+  hipStream_t st = (hipStream_t)stream;
+  RbrpBlockState* d = nullptr;
+  double* Rinv = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(RbrpBlockState)));
+  hipError_t e = hipMalloc(&Rinv, sizeof(double) * b * b);
+  RbrpBlockState hs{};
+  hs.s = (int)s;
+  if (e == hipSuccess) e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = rbrp_launch_pivchol_filter(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk, Jin, Jout);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  (void)hipFree(Rinv);
+  if (e != hipSuccess) return (int)e;
+  *bt = hs.bt;
+  *nswap = hs.nswap;
+  *status = hs.status;
+  return 0;
+}
+
+int rbrp_test_sample(void* stream, const double* diags, int64_t i, int64_t n, int64_t kmax,
+                     int64_t b, uint64_t seed, uint64_t offset, uint64_t draws_in, int32_t* J,
+                     int32_t* s_out, uint64_t* draws_out) {
+  if (n < 1 || i < 0 || i >= n) return -3;
+  if (b < 1 || b > 256) return -6;
+  hipStream_t st = (hipStream_t)stream;
+  RbrpBlockState* d = nullptr;
+  double* scratch = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(RbrpBlockState)));
+  hipError_t e = hipMalloc(&scratch, sizeof(double) * 2 * n);
+  RbrpBlockState hs{};
+  hs.draws = draws_in;
+  if (e == hipSuccess) e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
+  const uint64_t ctr[4] = {offset, 0, 0, 0};
+  if (e == hipSuccess)
+    e = rbrp_launch_sample(st, diags, i, n, kmax, (int)b, seed, 0, ctr, scratch, scratch + n, d, J);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  (void)hipFree(scratch);
+  if (e != hipSuccess) return (int)e;
+  *s_out = hs.s;
+  *draws_out = hs.draws;
+  return hs.status ? 100000 : 0;
+}
+
+}  // extern "C"

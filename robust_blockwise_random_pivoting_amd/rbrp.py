@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's RBRP call surface over the HIP C ABI.
+
+``RBRP(m, n, A, blk)`` mirrors ``void RBRP(int m, int n, double *A, int blk,
+ranks, error, t_ranks, t_rbrp, flops)`` (gpu/rbrp.hpp:6-8): no tolerance, runs
+to ``min(m, n)``, returns the cumulative ranks, squared relative residuals and
+cumulative times per block.  ``rbrp_colid`` is the added form with a
+tolerance and the skeleton / permutation / interpolation-matrix outputs
+(matlab/RBRP.m:2, matlab/driver.m:83-94).
+
+PyTorch is used only to hold device memory and streams; every computation
+runs in ``librbrp.so``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import _capi
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("rbrp: no GPU device is visible to HIP")
+    return torch
+
+
+def colmajor(A):
+    """A column-major (stride (1, ld)) FP64 device view of ``A`` (copies only if needed)."""
+    torch = _torch()
+    if not (A.is_cuda and A.dtype == torch.float64):
+        A = A.to(device="cuda", dtype=torch.float64)
+    m, n = A.shape
+    if A.stride(0) == 1 and (n <= 1 or A.stride(1) >= max(m, 1)):
+        return A
+    return A.t().contiguous().t()
+
+
+def leading_dim(A) -> int:
+    m, n = A.shape
+    return int(A.stride(1)) if n > 1 else max(m, 1)
+
+
+class Handle:
+    """Owns a library handle bound to a HIP stream (default: torch's current stream)."""
+
+    def __init__(self, stream=None):
+        torch = _torch()
+        self._lib = _capi.lib()
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        h = ctypes.c_void_p()
+        _capi.check("rbrp_create", self._lib.rbrp_create(ctypes.byref(h),
+                                                        ctypes.c_void_p(self.stream.cuda_stream)))
+        self._h = h
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def set_timing(self, enable: bool):
+        _capi.check("rbrp_set_timing", self._lib.rbrp_set_timing(self._h, int(bool(enable))))
+
+    def reset_stats(self):
+        _capi.check("rbrp_reset_phase_stats", self._lib.rbrp_reset_phase_stats(self._h))
+
+    def stats(self) -> dict:
+        s = _capi.RbrpPhaseStats()
+        _capi.check("rbrp_get_phase_stats", self._lib.rbrp_get_phase_stats(self._h, ctypes.byref(s)))
+        return {ph: dict(ms=s.ms[q], launches=int(s.launches[q]), flops=s.flops[q], bytes=s.bytes[q])
+                for q, ph in enumerate(_capi.PHASES)}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.rbrp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+@dataclasses.dataclass
+class ColID:
+    k: int
+    perm: "object"          # torch int32 (n,), device
+    X: "object"             # torch float64 (k, n-k) column-major view, device (None if skipped)
+    ranks: np.ndarray       # cumulative rank per block
+    error: np.ndarray       # squared relative residual per block
+    t_ranks: np.ndarray     # cumulative seconds per block
+    t_total: float
+    err_final: float        # non-squared final relative residual
+    draws: int              # uniforms consumed
+
+    @property
+    def skeleton(self):
+        return self.perm[: self.k]
+
+    @property
+    def redundant(self):
+        return self.perm[self.k:]
+
+
+def rbrp_colid(A, b: int = 64, tol: float = 0.0, seed: int = 0, offset: int = 0, max_rank: int = 0,
+               filter: str = "sampled", precision: str = "fp64", want_x: bool = True,
+               handle: Optional[Handle] = None) -> ColID:
+    """RBRP column ID of the device matrix ``A`` (m x n FP64); see the module docstring."""
+    torch = _torch()
+    A = colmajor(A)
+    m, n = A.shape
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        p = min(m, n)
+        kmax = min(max_rank, p) if max_rank else p
+        ldx = max(kmax, 1)
+        perm = torch.empty(n, dtype=torch.int32, device=A.device)
+        xbuf = torch.empty((max(n, 1), ldx), dtype=torch.float64, device=A.device) if want_x else None
+        cap = max(kmax, 1) + 1
+        ranks = np.zeros(cap, dtype=np.int64)
+        error = np.zeros(cap, dtype=np.float64)
+        t_ranks = np.zeros(cap, dtype=np.float64)
+        opts = _capi.RbrpOpts(b=int(b), tol=float(tol), max_rank=int(max_rank), seed=int(seed),
+                              stream_offset=int(offset), precision=_capi.PRECISIONS[precision],
+                              filter=_capi.FILTERS[filter])
+        res = _capi.RbrpResult()
+        res.perm = perm.data_ptr()
+        res.X = xbuf.data_ptr() if want_x else None
+        res.ldx = ldx
+        res.trace_cap = cap
+        res.ranks = ranks.ctypes.data
+        res.error = error.ctypes.data
+        res.t_ranks = t_ranks.ctypes.data
+        code = h._lib.rbrp_colid_d(h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A),
+                                   ctypes.byref(opts), ctypes.byref(res))
+        _capi.check("rbrp_colid_d", code)
+        k = int(res.k)
+        X = xbuf[: n - k, :k].t() if want_x else None
+        nb = int(res.nblocks)
+        return ColID(k=k, perm=perm, X=X, ranks=ranks[:nb].copy(), error=error[:nb].copy(),
+                     t_ranks=t_ranks[:nb].copy(), t_total=float(res.t_total),
+                     err_final=float(res.err_final), draws=int(res.draws))
+    finally:
+        if own:
+            h.close()
+
+
+def RBRP(m: int, n: int, A, blk: int, seed: int = 0, offset: int = 0):
+    """Mirror of ``RBRP(m, n, A, blk, ranks, error, t_ranks, t_rbrp, flops)`` (gpu/rbrp.hpp:6-8).
+
+    Returns ``(ranks, error, t_ranks, t_rbrp, flops)``; like the reference it
+    runs to ``min(m, n)`` with no tolerance and leaves ``A`` untouched.
+    ``flops`` is the algorithmic FP64 count of the trailing updates,
+    ``sum_t 4 m b_t (n - i_t - b_t)`` (the reference declares but never sets it).
+    """
+    torch = _torch()
+    A = torch.as_strided(A, (m, n), (1, m)) if A.dim() == 1 else A
+    r = rbrp_colid(A[:m, :n], b=blk, tol=0.0, seed=seed, offset=offset, want_x=False)
+    prev = np.concatenate([[0], r.ranks[:-1]]) if len(r.ranks) else np.zeros(0)
+    bts = r.ranks - prev
+    flops = float(np.sum(4.0 * m * bts * (n - r.ranks)))
+    return (list(map(int, r.ranks)), list(map(float, r.error)), list(map(float, r.t_ranks)),
+            r.t_total, flops)
+
+
+def uniforms(seed: int, offset: int, draw0: int, count: int, handle: Optional[Handle] = None):
+    """Uniforms ``draw0 .. draw0+count-1`` of the Philox stream, computed on the device."""
+    torch = _torch()
+    out = torch.empty(count, dtype=torch.float64, device="cuda")
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_uniforms_d", h._lib.rbrp_uniforms_d(h.ptr, seed, offset, draw0, count,
+                                                              ctypes.c_void_p(out.data_ptr())))
+        h.stream.synchronize()
+    finally:
+        if own:
+            h.close()
+    return out

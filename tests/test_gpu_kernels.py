@@ -1,0 +1,148 @@
+"""Kernel-level parity of the HIP library against host computations (GPU)."""
+
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import robust_filter, sample_without_replacement, symmetric_difference, uniform_stream
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dtype=None):
+    import torch
+    return torch.as_tensor(np.asfortranarray(x) if np.ndim(x) == 2 else x, dtype=dtype).cuda()
+
+
+def _cm(a):
+    """Column-major device copy of a host 2-D array."""
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)).cuda().t()
+
+
+def _host(t):
+    return t.cpu().numpy()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def test_uniforms_match_numpy_philox(librbrp):
+    from robust_blockwise_random_pivoting_amd import uniforms
+    for seed, off, d0, cnt in [(0, 0, 0, 1000), (42, 0, 3, 513), (2 ** 63 + 5, 2 ** 64 - 2, 1, 64)]:
+        ref = uniform_stream(seed, off).random(d0 + cnt)[d0:]
+        got = _host(uniforms(seed, off, d0, cnt))
+        assert np.array_equal(got, ref), (seed, off, d0)
+
+
+@pytest.mark.parametrize("M,N,K,nsplit", [(1, 1, 16, 1), (17, 5, 64, 3), (64, 64, 256, 4),
+                                          (100, 130, 1024, 7), (29, 64, 4096, 16)])
+def test_gemm_tn_matches_numpy(librbrp, M, N, K, nsplit):
+    import torch
+    rng = np.random.default_rng(M * 1000 + N)
+    A = rng.standard_normal((K, M))
+    B = rng.standard_normal((K, N))
+    dA, dB = _cm(A), _cm(B)
+    dC = _cm(np.zeros((M, N)))
+    code = librbrp.rbrp_test_gemm_tn(_stream(), M, N, K, _ptr(dA), K, _ptr(dB), K, _ptr(dC), M, nsplit)
+    assert code == 0
+    ref = A.T @ B
+    assert np.abs(_host(dC) - ref).max() <= 1e-12 * np.abs(ref).max() * np.sqrt(K)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("M,N,K,mode", [(64, 64, 16, 0), (130, 70, 33, 1), (5, 3, 1, 2),
+                                        (256, 200, 64, 1), (64, 1, 100, 0)])
+def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    dA, dB, dC = _cm(A), _cm(B), _cm(C0)
+    ntile = (M + 63) // 64
+    import torch
+    dP = torch.zeros(ntile * N, dtype=torch.float64, device="cuda")
+    code = librbrp.rbrp_test_gemm_nn(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode,
+                                     _ptr(dP))
+    assert code == 0
+    AB = A @ B
+    ref = AB if mode == 0 else (C0 - AB if mode == 1 else C0 + AB)
+    got = _host(dC)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()) * np.sqrt(K + 1)
+    parts = _host(dP).reshape(ntile, N)
+    for t in range(ntile):
+        blk = ref[t * 64:(t + 1) * 64]
+        assert np.allclose(parts[t], np.sum(blk * blk, axis=0), rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("s,b,deps", [(1, 8, 0), (8, 8, 0), (40, 40, 0), (64, 64, 5), (64, 64, 30),
+                                      (100, 128, 10), (128, 128, 0)])
+def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
+    import torch
+    rng = np.random.default_rng(s * 7 + deps)
+    m = 600
+    V = rng.standard_normal((m, s)) * np.exp(-rng.random(s) * 8)
+    for q in range(deps):   # nearly dependent columns for the filter to drop
+        a, c = rng.integers(0, s, size=2)
+        V[:, a] = V[:, c] * 2.0 + 1e-9 * rng.standard_normal(m)
+    G = V.T @ V
+    i = 1000
+    J = (i + rng.permutation(4 * s)[:s]).astype(np.int32)
+    dG = _cm(G)
+    dJ = torch.as_tensor(J).cuda()
+    dJk = torch.zeros(b, dtype=torch.int32, device="cuda")
+    dJin = torch.zeros(b, dtype=torch.int32, device="cuda")
+    dJout = torch.zeros(b, dtype=torch.int32, device="cuda")
+    dRb = torch.zeros(b * b, dtype=torch.float64, device="cuda")
+    bt, ns, status = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    code = librbrp.rbrp_test_panel(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk), _ptr(dRb),
+                                   _ptr(dJin), _ptr(dJout), ctypes.byref(bt), ctypes.byref(ns),
+                                   ctypes.byref(status))
+    assert code == 0 and status.value == 0
+    _, R, piv = sla.qr(V, mode="economic", pivoting=True)
+    bt_ref = robust_filter(R, s, b, "sampled")
+    assert bt.value == bt_ref
+    Jk_ref = J[piv[:bt_ref]]
+    assert np.array_equal(_host(dJk)[:bt_ref], Jk_ref)
+    Rb = _host(dRb).reshape(b, b).T[:bt_ref, :bt_ref]
+    Rref = np.abs(R[:bt_ref, :bt_ref])
+    assert np.abs(np.abs(Rb) - Rref).max() <= 1e-9 * Rref.max()
+    jin, jout = symmetric_difference(i, bt_ref, Jk_ref)
+    assert ns.value == len(jin)
+    assert np.array_equal(_host(dJin)[:len(jin)], jin)
+    assert np.array_equal(_host(dJout)[:len(jout)], jout)
+
+
+@pytest.mark.parametrize("N,i,b,skew,seed", [(10, 0, 4, 1, 0), (4000, 0, 64, 1, 1),
+                                             (4000, 1800, 64, 6, 2), (300, 250, 64, 1, 3),
+                                             (5000, 10, 128, 12, 4), (1, 0, 8, 1, 5)])
+def test_sampler_matches_oracle(librbrp, N, i, b, skew, seed):
+    import torch
+    rng = np.random.default_rng(seed)
+    diags = rng.random(N) ** skew
+    diags[rng.random(N) < 0.1] = 0.0
+    if N > 1:
+        diags[rng.integers(i, N)] = diags.sum() * 50   # dominant weight: redraw rounds
+    kmax = N
+    w = diags[i:]
+    s_ref = min(b, kmax - i, int(np.count_nonzero(w > 0)))
+    ga = uniform_stream(seed, 9)
+    ga.random(7)   # 7 draws already consumed
+    Js, d = sample_without_replacement(w, s_ref, ga)
+    dd = torch.as_tensor(diags).cuda()
+    dJ = torch.zeros(b, dtype=torch.int32, device="cuda")
+    s_out, draws = ctypes.c_int32(), ctypes.c_uint64()
+    code = librbrp.rbrp_test_sample(_stream(), _ptr(dd), i, N, kmax, b, seed, 9, 7, _ptr(dJ),
+                                    ctypes.byref(s_out), ctypes.byref(draws))
+    assert code == 0
+    assert s_out.value == s_ref
+    assert np.array_equal(_host(dJ)[:s_ref], i + Js)
+    assert draws.value == 7 + d

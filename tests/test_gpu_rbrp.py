@@ -1,0 +1,155 @@
+"""End-to-end parity of the HIP RBRP column ID against the CPU oracle (GPU).
+
+Bar (fed the same Philox uniform stream):
+* skeleton indices and block ranks identical to the oracle's;
+* X = R11^{-1} R12 within 1e-10 relative (Frobenius) where the a-priori bound
+  kappa(R11) * eps allows it, else within 100 * kappa(R11) * eps;
+* ID error ||A(:,rd) - A(:,J) X||_F / ||A||_F within 1% of the oracle's.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import id_error, rbrp_colid as oracle_colid
+from robust_blockwise_random_pivoting_amd.matrices import (exp_spectrum, gmm_rows, kernel_block,
+                                                            spectrum_matrix)
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def _dev(A):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(A.T)).cuda().t()
+
+
+def run_both(A, b, tol, seed=7, offset=0, filt="sampled", max_rank=0):
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    g = rbrp_colid(_dev(A), b=b, tol=tol, seed=seed, offset=offset, filter=filt, max_rank=max_rank)
+    r = oracle_colid(A, b, tol, seed=seed, offset=offset, filter_mode=filt, max_rank=max_rank)
+    return g, r
+
+
+def assert_parity(A, g, r, name=""):
+    perm = g.perm.cpu().numpy()
+    assert g.k == r.k, name
+    assert np.array_equal(perm[: g.k], r.perm[: r.k]), name
+    assert np.array_equal(np.sort(perm), np.arange(A.shape[1])), name
+    assert np.array_equal(g.ranks, r.ranks), name
+    assert g.draws == r.draws, name
+    if g.k and g.k < A.shape[1]:
+        X = g.X.cpu().numpy()
+        kappa = np.linalg.cond(r.R[:, : r.k])
+        rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
+        bound = max(1e-10, 100 * kappa * EPS)
+        assert rel <= bound, (name, rel, kappa)
+        eg = id_error(A, perm, g.k, X)
+        er = id_error(A, r.perm, r.k, r.X)
+        assert abs(eg - er) <= 0.01 * er + 1e-15, (name, eg, er)
+        # the tracked residual equals the recomputed ID error
+        assert np.sqrt(g.error[-1]) == pytest.approx(eg, rel=1e-4, abs=1e-14)
+    np.testing.assert_allclose(g.error, r.error, rtol=1e-6, atol=1e-15)
+
+
+CASES = [
+    ("exp 300x200 b16", lambda: spectrum_matrix(300, 200, exp_spectrum(200, 10.0), 5), 16, 1e-6),
+    ("exp 200x300 m<n b24", lambda: spectrum_matrix(200, 300, exp_spectrum(200, 8.0), 6), 24, 1e-7),
+    ("gmm 400x120 b16 full rank", lambda: gmm_rows(400, 120, 7, components=8), 16, 1e-3),
+    ("kernel 200x150 b16", lambda: kernel_block(200, 150, np.random.default_rng(8)), 16, 1e-7),
+    ("exp 1000x700 b64", lambda: spectrum_matrix(1000, 700, exp_spectrum(700, 30.0), 9), 64, 1e-8),
+    ("exp 700x500 b128", lambda: spectrum_matrix(700, 500, exp_spectrum(500, 40.0), 10), 128, 1e-9),
+]
+
+
+@pytest.mark.parametrize("name,gen,b,tol", CASES)
+def test_rbrp_parity_with_oracle(name, gen, b, tol):
+    A = gen()
+    g, r = run_both(A, b, tol)
+    assert_parity(A, g, r, name)
+
+
+def test_rbrp_config1_parity():
+    """1000 x 1000, s_i = exp(-i/20), b = 40, tol = 1e-6 (the CPU-runnable configuration)."""
+    A = spectrum_matrix(1000, 1000, exp_spectrum(1000, 20.0), 1)
+    g, r = run_both(A, 40, 1e-6, seed=42)
+    assert_parity(A, g, r, "config1")
+    assert g.err_final <= 1e-6
+
+
+def test_rbrp_nominal_filter_and_offset():
+    A = spectrum_matrix(400, 300, exp_spectrum(300, 12.0), 3)
+    g, r = run_both(A, 32, 1e-7, seed=5, offset=1234, filt="nominal")
+    assert_parity(A, g, r, "nominal")
+
+
+def test_rbrp_edge_cases():
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    rng = np.random.default_rng(9)
+    g = rbrp_colid(_dev(np.zeros((30, 20))), b=8, tol=1e-6)
+    assert g.k == 0 and np.array_equal(g.perm.cpu().numpy(), np.arange(20))
+    A = np.outer(rng.standard_normal(50), rng.standard_normal(40))
+    g, r = run_both(A, 8, 1e-12)
+    assert g.k == 1 == r.k
+    B = rng.standard_normal((60, 5))
+    A = np.hstack([B, B, B])
+    g, r = run_both(A, 15, 1e-10, seed=1)
+    assert g.k == 5
+    assert id_error(A, g.perm.cpu().numpy(), g.k, g.X.cpu().numpy()) <= 1e-10
+    for A, b in [(rng.standard_normal((10, 1)), 4), (rng.standard_normal((1, 10)), 4),
+                 (rng.standard_normal((3, 70)), 64)]:
+        g, r = run_both(A, b, 0.0)
+        assert g.k == r.k == min(A.shape)
+    A = spectrum_matrix(80, 60, exp_spectrum(60, 5.0), 2)
+    g, r = run_both(A, 1, 1e-6, seed=4)
+    assert_parity(A, g, r, "b=1")
+    g, r = run_both(A, 7, 0.0, max_rank=20)
+    assert g.k == r.k == 20
+    assert_parity(A, g, r, "max_rank")
+
+
+def test_rbrp_rejects_bad_arguments():
+    import torch
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    from robust_blockwise_random_pivoting_amd._capi import RBRPError
+    A = _dev(np.ones((10, 10)))
+    with pytest.raises(RBRPError):
+        rbrp_colid(A, b=0)
+    with pytest.raises(RBRPError):
+        rbrp_colid(A, b=129)
+    with pytest.raises(RBRPError):
+        rbrp_colid(A, b=4, tol=-1.0)
+    bad = torch.full((10, 10), float("nan"), dtype=torch.float64, device="cuda")
+    with pytest.raises(RBRPError):
+        rbrp_colid(bad, b=4)
+
+
+def test_reference_call_surface_mirror():
+    import torch
+    from robust_blockwise_random_pivoting_amd import RBRP
+    A = spectrum_matrix(300, 200, exp_spectrum(200, 10.0), 5)
+    dA = torch.as_tensor(np.ascontiguousarray(A.T)).cuda()   # column-major buffer (lda = m)
+    before = dA.clone()
+    ranks, error, t_ranks, t_rbrp, flops = RBRP(300, 200, dA.reshape(-1), 16)
+    assert ranks[-1] == 200                     # no tolerance: runs to min(m, n)
+    assert all(a < b for a, b in zip(ranks, ranks[1:]))
+    assert all(e2 <= e1 + 1e-15 for e1, e2 in zip(error, error[1:]))
+    assert t_rbrp >= t_ranks[-1] >= 0 and flops > 0
+    assert torch.equal(before, dA)              # the input is never written
+
+
+def test_rbrp_large_properties():
+    """50000 x 4000 FP64, s_i = exp(-i/100), b = 64, tol = 1e-8: size-independent checks."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    from robust_blockwise_random_pivoting_amd.matrices import spectrum_matrix_torch
+    m, n = 50000, 4000
+    A = spectrum_matrix_torch(m, n, exp_spectrum(n, 100.0), seed=2, device="cuda")
+    g = rbrp_colid(A, b=64, tol=1e-8, seed=1)
+    perm = g.perm.long()
+    assert torch.equal(torch.sort(perm).values, torch.arange(n, device="cuda"))
+    E = A[:, perm[g.k:]] - A[:, perm[: g.k]] @ g.X
+    err = (torch.linalg.norm(E) / torch.linalg.norm(A)).item()
+    assert err <= 1e-8 * 1.001
+    assert np.sqrt(g.error[-1]) == pytest.approx(err, rel=1e-3)
+    assert 1800 <= g.k <= 2300

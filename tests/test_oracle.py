@@ -1,0 +1,294 @@
+"""Pins for the CPU oracle (no GPU).
+
+Each test checks one oracle function against something that does not share
+its code: NumPy / SciPy library routines, the Random123 known-answer vectors,
+brute force on tiny inputs, closed forms, or an independent left-looking
+restatement of matlab/RBRP.m fed the same uniform stream.
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import (id_error, interpolation_matrix, rand_cpqr_os, rand_lupp_os, rbrp_colid,
+                    robust_filter, sample_without_replacement, symmetric_difference,
+                    uniform_stream, pivots_to_permutation, compute_interpolation_matrix_os)
+from robust_blockwise_random_pivoting_amd.matrices import (exp_spectrum, gmm_rows, kernel_block,
+                                                            optimal_error, spectrum_matrix)
+
+# Random123 philox4x64_10 known-answer vectors (Salmon et al., SC'11; kat_vectors)
+M64 = (1 << 64) - 1
+KAT = [
+    ((0, 0, 0, 0), (0, 0),
+     (0x16554d9eca36314c, 0xdb20fe9d672d0fdc, 0xd7e772cee186176b, 0x7e68b68aec7ba23b)),
+    ((M64, M64, M64, M64), (M64, M64),
+     (0x87b092c3013fe90b, 0x438c3c67be8d0224, 0x9cc7d7c69cd777b6, 0xa09caebf594f0ba0)),
+    ((0x243f6a8885a308d3, 0x13198a2e03707344, 0xa4093822299f31d0, 0x082efa98ec4e6c89),
+     (0x452821e638d01377, 0xbe5466cf34e90c6c),
+     (0xa528f45403e61d95, 0x38c72dbd566e9788, 0xa5a1610e72fd18b5, 0x57bd43b5e52b7fe6)),
+]
+
+
+def numpy_philox_block(ctr, key):
+    """Philox4x64-10(ctr, key) read out of numpy.random.Philox (counter pre-incremented)."""
+    c = sum(int(w) << (64 * q) for q, w in enumerate(ctr))
+    k = sum(int(w) << (64 * q) for q, w in enumerate(key))
+    bg = np.random.Philox(key=k)
+    st = bg.state
+    c = (c - 1) % (1 << 256)
+    st["state"]["counter"] = np.array([(c >> (64 * q)) & M64 for q in range(4)], dtype=np.uint64)
+    st["buffer_pos"] = 4
+    bg.state = st
+    return tuple(int(x) for x in bg.random_raw(4))
+
+
+@pytest.mark.parametrize("ctr,key,want", KAT)
+def test_numpy_philox_matches_random123_kat(ctr, key, want):
+    assert numpy_philox_block(ctr, key) == want
+
+
+def test_uniform_stream_word_order():
+    # draw t = word t%4 of Philox(offset + 1 + t//4), top 53 bits * 2^-53
+    seed, off = 12345, 77
+    u = uniform_stream(seed, off).random(11)
+    for t in range(11):
+        blk = numpy_philox_block(((off + 1 + t // 4), 0, 0, 0), (seed, 0))
+        assert u[t] == (blk[t % 4] >> 11) * 2.0 ** -53
+
+
+def test_sampler_matches_numpy_choice():
+    rng = np.random.default_rng(0)
+    for case in range(300):
+        N = int(rng.integers(1, 200))
+        w = rng.random(N) ** rng.integers(1, 8)
+        w[rng.random(N) < 0.2] = 0.0
+        if case % 7 == 0:
+            w[rng.integers(0, N)] = 1e6   # a dominant weight forces redraw rounds
+        nnz = int(np.count_nonzero(w))
+        if nnz == 0:
+            continue
+        s = int(rng.integers(1, nnz + 1))
+        p = w / w.sum()
+        seed = int(rng.integers(0, 2 ** 63))
+        ga, gb = uniform_stream(seed, case), uniform_stream(seed, case)
+        mine, draws = sample_without_replacement(p, s, ga)
+        ref = gb.choice(N, size=s, replace=False, p=p)
+        assert np.array_equal(mine, ref), case
+        sa, sb = ga.bit_generator.state, gb.bit_generator.state
+        assert np.array_equal(sa["state"]["counter"], sb["state"]["counter"]), case
+        assert sa["buffer_pos"] == sb["buffer_pos"], case
+        assert draws >= s
+
+
+def test_sampler_unnormalised_weights_same_indices():
+    rng = np.random.default_rng(1)
+    w = rng.random(500) * 1e-20
+    a, _ = sample_without_replacement(w, 40, uniform_stream(3))
+    b, _ = sample_without_replacement(w / w.sum(), 40, uniform_stream(3))
+    assert np.array_equal(a, b)
+
+
+def _filter_brute(R, s, div):
+    sqr = [sum(R[r, c] ** 2 for c in range(r, s)) for r in range(s)]
+    total = sum(sqr)
+    for k in range(s):
+        if sum(sqr[s - 1 - k:]) >= total / div:
+            return s - k
+    return 0
+
+
+def _python_reference_clip(R, tol):
+    # RBRP.ipynb c11:69-73 restated: rncs = reverse cumulative row-norm sums
+    rncs = np.flip(np.flip(np.linalg.norm(R, axis=1) ** 2).cumsum())
+    rel = rncs / rncs[0]
+    return R.shape[0] if rel[-1] >= tol else int(np.argmax(rel < tol))
+
+
+def test_filter_bruteforce_and_python_rule():
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        s = int(rng.integers(1, 24))
+        V = rng.standard_normal((40, s)) * (10.0 ** -rng.integers(0, 12, size=s))
+        _, R, _ = sla.qr(V, mode="economic", pivoting=True)
+        assert robust_filter(R, s, 32, "sampled") == _filter_brute(R, s, s)
+        assert robust_filter(R, s, 32, "nominal") == _filter_brute(R, s, 32)
+        assert robust_filter(R, s, s, "nominal") == _python_reference_clip(R, 1.0 / s)
+
+
+def test_filter_boundary_is_inclusive():
+    # rbrp.cu:204 compares with >=: equal tail mass keeps the row
+    R = np.diag([np.sqrt(2.0), np.sqrt(2.0)])
+    assert robust_filter(R, 2, 2) == 2
+    R = np.diag([np.sqrt(3.0), 1.0])     # tail 1 < 4/2 -> drop the last row
+    assert robust_filter(R, 2, 2) == 1
+
+
+def test_symmetric_difference_bruteforce():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        n = int(rng.integers(2, 40))
+        i = int(rng.integers(0, n - 1))
+        bt = int(rng.integers(1, n - i + 1))
+        jk = rng.choice(np.arange(i, n), size=bt, replace=False)
+        jin, jout = symmetric_difference(i, bt, jk)
+        assert list(jin) == sorted(set(jk) - set(range(i, i + bt)))
+        assert list(jout) == sorted(set(range(i, i + bt)) - set(jk))
+        # the rbrp.cu:271-280 statements keep perm a permutation and move Jk to the block
+        perm = rng.permutation(n)
+        old = perm.copy()
+        tmp = perm[jk].copy()
+        perm[jin] = perm[jout]
+        perm[i:i + bt] = tmp
+        assert sorted(perm) == list(range(n))
+        assert list(perm[i:i + bt]) == list(old[jk])
+        assert list(perm[jin]) == list(old[jout])
+
+
+def left_looking_rbrp(A, b, tol, seed, offset=0, filter_mode="sampled"):
+    """Independent restatement: left-looking Q/R on the original columns (matlab/RBRP.m:30-121),
+    one stored-R Gram-Schmidt pass plus a second pass, residual norms recomputed directly."""
+    A = np.array(A, dtype=np.float64, order="F")
+    m, n = A.shape
+    p = min(m, n)
+    perm = np.arange(n)
+    Aw = A.copy()
+    diags = np.sum(Aw * Aw, axis=0)
+    N0 = diags.sum()
+    Q = np.zeros((m, p))
+    R = np.zeros((p, n))
+    rng = uniform_stream(seed, offset)
+    i = 0
+    while i < p:
+        w = np.maximum(diags[i:], 0.0)
+        nnz = np.count_nonzero(w)
+        if nnz == 0:
+            break
+        s = min(b, p - i, nnz)
+        J = i + sample_without_replacement(w, s, rng)[0]
+        X = Aw[:, J] - Q[:, :i] @ R[:i, J]
+        C = Q[:, :i].T @ X
+        X -= Q[:, :i] @ C
+        R[:i, J] += C
+        Q2, R2, p2 = sla.qr(X, mode="economic", pivoting=True)
+        bt = robust_filter(R2, s, b, filter_mode)
+        Jk = J[p2[:bt]]
+        jin, jout = symmetric_difference(i, bt, Jk)
+        tmp = perm[Jk].copy(); perm[jin] = perm[jout]; perm[i:i + bt] = tmp
+        tmp = R[:i, Jk].copy(); R[:i, jin] = R[:i, jout]; R[:i, i:i + bt] = tmp
+        tmp = Aw[:, Jk].copy(); Aw[:, jin] = Aw[:, jout]; Aw[:, i:i + bt] = tmp
+        Q[:, i:i + bt] = Q2[:, :bt]
+        R[i:i + bt, i:i + bt] = R2[:bt, :bt]
+        R[i:i + bt, i + bt:] = Q2[:, :bt].T @ Aw[:, i + bt:]
+        E = Aw[:, i + bt:] - Q[:, :i + bt] @ R[:i + bt, i + bt:]
+        diags[i + bt:] = np.sum(E * E, axis=0)
+        i += bt
+        if diags[i:].sum() / N0 <= tol ** 2:
+            break
+    X = sla.solve_triangular(R[:i, :i], R[:i, i:])
+    return i, perm, X
+
+
+CASES = [
+    ("exp 300x200", lambda: spectrum_matrix(300, 200, exp_spectrum(200, 10.0), 5), 16, 1e-6),
+    ("exp 200x300 (m<n)", lambda: spectrum_matrix(200, 300, exp_spectrum(200, 8.0), 6), 24, 1e-7),
+    ("gmm 400x120", lambda: gmm_rows(400, 120, 7, components=8), 16, 1e-3),
+    ("kernel 200x150", lambda: kernel_block(200, 150, np.random.default_rng(8)), 16, 1e-7),
+]
+
+
+@pytest.mark.parametrize("name,gen,b,tol", CASES)
+def test_oracle_matches_left_looking_restatement(name, gen, b, tol):
+    A = gen()
+    r = rbrp_colid(A, b, tol, seed=11)
+    k2, perm2, X2 = left_looking_rbrp(A, b, tol, seed=11)
+    assert r.k == k2
+    assert np.array_equal(r.perm[: r.k], perm2[:k2])
+    assert np.array_equal(np.sort(r.perm), np.arange(A.shape[1]))
+    if X2.size:
+        scale = max(1.0, np.abs(X2).max())
+        assert np.abs(r.X - X2).max() <= 1e-7 * scale
+
+
+@pytest.mark.parametrize("name,gen,b,tol", CASES)
+def test_oracle_qr_identity_and_tolerance(name, gen, b, tol):
+    A = gen()
+    r = rbrp_colid(A, b, tol, seed=3)
+    e = id_error(A, r.perm, r.k, r.X)
+    # ||A(:,rd) - A(:,J) X||^2 equals the residual sum the loop tracked
+    assert e ** 2 == pytest.approx(r.resid.sum() / r.N0, rel=1e-6, abs=1e-24)
+    assert r.error[-1] == pytest.approx(r.resid.sum() / r.N0, rel=1e-12, abs=1e-30)
+    if r.k < min(A.shape):
+        assert e <= tol * (1 + 1e-6)
+    assert np.all(np.diff(r.ranks) > 0)
+    assert np.all(np.diff(r.error) <= 1e-12)
+
+
+def test_oracle_config1_and_optimum():
+    s = exp_spectrum(1000, 20.0)
+    A = spectrum_matrix(1000, 1000, s, 1)
+    r = rbrp_colid(A, 40, 1e-6, seed=42)
+    e = id_error(A, r.perm, r.k, r.X)
+    assert e <= 1e-6
+    assert e >= optimal_error(s, r.k) * (1 - 1e-9)   # no rank-k ID beats the SVD
+    assert 277 <= r.k <= 400
+
+
+def test_oracle_edge_cases():
+    z = np.zeros((30, 20))
+    r = rbrp_colid(z, 8, 1e-6)
+    assert r.k == 0 and np.array_equal(r.perm, np.arange(20))
+    rng = np.random.default_rng(9)
+    u, v = rng.standard_normal(50), rng.standard_normal(40)
+    r = rbrp_colid(np.outer(u, v), 8, 1e-12)
+    assert r.k == 1
+    # duplicated columns: the filter drops the copy sampled into the same block
+    B = rng.standard_normal((60, 5))
+    A = np.hstack([B, B, B])
+    r = rbrp_colid(A, 15, 1e-10, seed=1)
+    assert r.k == 5
+    assert id_error(A, r.perm, r.k, r.X) <= 1e-10
+    # one column / one row / b = 1
+    assert rbrp_colid(rng.standard_normal((10, 1)), 4, 0.0).k == 1
+    assert rbrp_colid(rng.standard_normal((1, 10)), 4, 0.0).k == 1
+    A = spectrum_matrix(80, 60, exp_spectrum(60, 5.0), 2)
+    r = rbrp_colid(A, 1, 1e-6, seed=4)
+    assert np.all(np.diff(np.concatenate([[0], r.ranks])) == 1)
+    assert id_error(A, r.perm, r.k, r.X) <= 1e-6
+    # max_rank caps the run
+    assert rbrp_colid(A, 7, 0.0, max_rank=20).k == 20
+
+
+def test_interpolation_matrix_is_triangular_solve():
+    rng = np.random.default_rng(10)
+    R = np.triu(rng.standard_normal((12, 30))) + 5 * np.eye(12, 30)
+    X = interpolation_matrix(R, 12)
+    assert np.allclose(R[:, :12] @ X, R[:, 12:])
+
+
+def test_pivots_to_permutation_matches_scipy_lu():
+    rng = np.random.default_rng(11)
+    Y = rng.standard_normal((50, 12))
+    lu, ipiv = sla.lu_factor(Y)
+    P = pivots_to_permutation(ipiv, np.arange(50))
+    L = np.tril(lu, -1)[:, :12] + np.eye(50, 12)
+    U = np.triu(lu)[:12]
+    assert np.allclose(Y[P], L @ U)
+
+
+def test_sketch_comparators():
+    rng = np.random.default_rng(12)
+    A = spectrum_matrix(300, 120, exp_spectrum(120, 10.0), 13)
+    r = 20
+    G = rng.standard_normal((2 * r, 300)) / np.sqrt(300)
+    Y = G @ A
+    sk, rd, T = rand_cpqr_os(A, r, G)
+    _, _, piv = sla.qr(Y[:r], pivoting=True)
+    assert np.array_equal(sk, piv[:r])
+    Tls = np.linalg.lstsq(Y[:, sk], Y[:, rd], rcond=None)[0]
+    assert np.allclose(T, Tls, atol=1e-8 * max(1, np.abs(Tls).max()))
+    sk2, rd2, T2 = rand_lupp_os(A, r, G)
+    assert np.array_equal(np.sort(np.concatenate([sk2, rd2])), np.arange(120))
+    Tls2 = np.linalg.lstsq(Y[:, sk2], Y[:, rd2], rcond=None)[0]
+    assert np.allclose(T2, Tls2, atol=1e-8 * max(1, np.abs(Tls2).max()))
+    T3 = compute_interpolation_matrix_os(Y, np.concatenate([sk, rd]), r)
+    assert np.allclose(T3, T)
