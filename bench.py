@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: RBRP column ID time-to-tolerance on one MI355X.
+
+Workload (one "step" = one complete ID to tolerance, X included): A = U diag(s) V^T
+with Haar U (50000 x 4000), Haar V (4000 x 4000) and s_i = exp(-i/100), FP64,
+block size b = 64, tol = 1e-8 (relative Frobenius, non-squared).  A is generated
+on the device and is resident in HBM before the timed region starts.
+
+Prints one JSON line: value = IDs per second (higher is better), the roofline
+of the dominant kernel (per-launch hipEvent durations measured by the library
+over the timed region, on the stream the kernels run on) and the CPU baseline
+(the NumPy/SciPy oracle on the same matrix and Philox stream, timed on this
+host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RBRP ID time-to-tol, mixed vs FP64 on 1 GPU at equal rel. error; batched IDs/sec"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=50000)
+    ap.add_argument("--n", type=int, default=4000)
+    ap.add_argument("--rate", type=float, default=100.0, help="s_i = exp(-i/rate)")
+    ap.add_argument("--b", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--matrix-seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-steps", action="store_true",
+                    help="print a line per step (for profiler runs)")
+    return ap.parse_args()
+
+
+def cpu_baseline(A_host, args, k_gpu, perm_gpu):
+    """The oracle (NumPy/SciPy LAPACK, multi-threaded BLAS) on the same matrix and stream."""
+    from oracle import rbrp_colid as oracle_colid
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    try:
+        from threadpoolctl import threadpool_info
+        info = threadpool_info()
+        threads = max([t.get("num_threads", 1) for t in info if t.get("user_api") == "blas"] or [threads])
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    r = oracle_colid(A_host, args.b, args.tol, seed=args.seed)
+    dt = time.perf_counter() - t0
+    same = bool(r.k == k_gpu and np.array_equal(r.perm[: r.k], perm_gpu[:k_gpu]))
+    return {
+        "value": 1.0 / dt,
+        "unit": "IDs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full ID of the same {A_host.shape[0]}x{A_host.shape[1]} matrix to tol "
+                  f"{args.tol:g} (b={args.b}), {dt:.2f} s; host cores visible {cores}",
+        "seconds": dt,
+        "k": int(r.k),
+        "skeleton_equal_to_gpu": same,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the sharded design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from robust_blockwise_random_pivoting_amd import Handle, rbrp_colid
+    from robust_blockwise_random_pivoting_amd.matrices import exp_spectrum, spectrum_matrix_torch
+
+    torch.cuda.set_device(0)
+    m, n = args.m, args.n
+    A = spectrum_matrix_torch(m, n, exp_spectrum(n, args.rate), seed=args.matrix_seed, device="cuda")
+    torch.cuda.synchronize()
+    print(f"[bench] matrix {m}x{n} ready", file=sys.stderr, flush=True)
+
+    h = Handle()
+    h.set_timing(True)
+    res = None
+    for w in range(args.warmup):
+        res = rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=h)
+        if args.profile_steps:
+            print(f"[bench] warmup {w} k={res.k}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    h.reset_stats()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        res = rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=h)
+        if args.profile_steps:
+            print(f"[bench] step {s} k={res.k}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = h.stats()
+
+    # independent check of the returned ID on the device
+    perm = res.perm.long()
+    E = A[:, perm[res.k:]] - A[:, perm[: res.k]] @ res.X
+    err = (torch.linalg.norm(E) / torch.linalg.norm(A)).item()
+    del E
+
+    kern = {}
+    for name, bound in (("project", "mfma"), ("update", "hbm")):
+        st = stats[name]
+        if st["launches"] == 0 or st["ms"] <= 0:
+            continue
+        avg_ms = st["ms"] / st["launches"]
+        if bound == "mfma":
+            ach = st["flops"] / (st["ms"] * 1e-3) / 1e12
+            kern[name] = dict(bound="mfma", achieved=ach, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+                              frac=ach / PEAK_FP64_TFLOPS, avg_ms=avg_ms, launches=st["launches"],
+                              flops_per_launch=st["flops"] / st["launches"])
+        else:
+            ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+            kern[name] = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
+                              frac=ach / PEAK_HBM_GBS, avg_ms=avg_ms, launches=st["launches"],
+                              bytes_per_launch=st["bytes"] / st["launches"])
+    dom = max(kern, key=lambda k: stats[k]["ms"]) if kern else None
+    roof = None
+    if dom:
+        d = kern[dom]
+        roof = {"kernel": {"project": "k_gemm_tn", "update": "k_gemm_nn"}[dom], "bound": d["bound"],
+                "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
+                "traffic": None, "avg_ms": d["avg_ms"], "launches": d["launches"]}
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        A_host = A.cpu().numpy()
+        cpu = cpu_baseline(A_host, args, res.k, res.perm.cpu().numpy())
+
+    ms = dt / args.steps * 1e3
+    out = {
+        "metric": METRIC,
+        "value": args.steps / dt,
+        "unit": "IDs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Haar U, V; s_i = exp(-i/100); generated on device)",
+        "config": {"workload": f"FP64 RBRP column ID to tol, {m}x{n} exp-decay, b={args.b}, "
+                               f"tol={args.tol:g}", "m": m, "n": n, "b": args.b, "tol": args.tol,
+                   "parallelism": "single GPU"},
+        "k": res.k,
+        "blocks": len(res.ranks),
+        "rel_error": err,
+        "roofline": roof,
+        "kernels": kern,
+        "phase_ms_per_step": {k: v["ms"] / args.steps for k, v in stats.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

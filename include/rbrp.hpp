@@ -1,0 +1,138 @@
+// rbrp.hpp -- C++ call surface of the reference (gpu/rbrp.hpp, gpu/rid.hpp) over the C ABI.
+//
+// Header-only: every function is an inline wrapper around include/rbrp.h, so a
+// caller of the reference library recompiles against this header and links
+// librbrp.so.  Errors are reported as return codes of the C ABI; the wrappers
+// keep the reference's void signatures and return the status separately
+// where the reference had none (RBRP_last_status()).
+#ifndef RBRP_HPP_
+#define RBRP_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "rbrp.h"
+
+namespace rbrp_detail {
+inline int& last_status() {
+  static thread_local int s = 0;
+  return s;
+}
+}  // namespace rbrp_detail
+
+// Status of the last wrapper call on this thread (0 = success; see rbrp.h).
+inline int RBRP_last_status() { return rbrp_detail::last_status(); }
+
+// gpu/rbrp.hpp:6-8.  A is an m x n column-major FP64 device pointer (lda = m),
+// left unchanged.  Runs to min(m, n) with no tolerance, like the reference.
+// ranks / error / t_ranks are cumulative per block; flops is the algorithmic
+// FP64 count of the trailing updates (the reference declares it but never sets it).
+inline void RBRP(int m, int n, double* A, int blk, std::vector<int>& ranks,
+                 std::vector<double>& error, std::vector<double>& t_ranks, double& t_rbrp,
+                 double& flops) {
+  ranks.clear();
+  error.clear();
+  t_ranks.clear();
+  t_rbrp = 0.0;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (st) {
+    rbrp_detail::last_status() = st;
+    return;
+  }
+  int32_t* perm = nullptr;
+  if (hipMalloc(&perm, sizeof(int32_t) * (size_t)(n > 0 ? n : 1)) != hipSuccess) {
+    rbrp_destroy(h);
+    rbrp_detail::last_status() = RBRP_ERR_ALLOC;
+    return;
+  }
+  const int p = m < n ? m : n;
+  std::vector<int64_t> r64((size_t)p + 1);
+  error.resize((size_t)p + 1);
+  t_ranks.resize((size_t)p + 1);
+  rbrp_opts_t o;
+  rbrp_opts_default(&o);
+  o.b = blk;
+  rbrp_result_t res{};
+  res.perm = perm;
+  res.X = nullptr;
+  res.trace_cap = p + 1;
+  res.ranks = r64.data();
+  res.error = error.data();
+  res.t_ranks = t_ranks.data();
+  st = rbrp_colid_d(h, A, m, n, m, &o, &res);
+  (void)hipFree(perm);
+  rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  const size_t nb = st ? 0 : (size_t)res.nblocks;
+  error.resize(nb);
+  t_ranks.resize(nb);
+  ranks.resize(nb);
+  int64_t prev = 0;
+  for (size_t t = 0; t < nb; ++t) {
+    ranks[t] = (int)r64[t];
+    flops += 4.0 * (double)m * (double)(r64[t] - prev) * (double)(n - r64[t]);
+    prev = r64[t];
+  }
+  t_rbrp = st ? 0.0 : res.t_total;
+}
+
+// Added overload with the tolerance and the skeleton / redundant sets and X.
+// sk (k entries) and rd (n - k entries) are host vectors; X is a device buffer
+// of k x (n - k) doubles (leading dimension k) allocated with hipMalloc that
+// the caller releases with hipFree.  Returns the C-ABI status.
+inline int RBRP(int m, int n, const double* A, int blk, double tol, std::vector<int>& sk,
+                std::vector<int>& rd, double*& X, std::vector<int>& ranks,
+                std::vector<double>& error, uint64_t seed = 0) {
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (st) return st;
+  const int p = m < n ? m : n;
+  int32_t* perm = nullptr;
+  double* xbuf = nullptr;
+  st = (int)hipMalloc(&perm, sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  if (!st) st = (int)hipMalloc(&xbuf, sizeof(double) * (size_t)(p > 0 ? p : 1) * (size_t)(n > 0 ? n : 1));
+  std::vector<int64_t> r64((size_t)p + 1);
+  std::vector<double> err((size_t)p + 1), tr((size_t)p + 1);
+  rbrp_result_t res{};
+  if (!st) {
+    rbrp_opts_t o;
+    rbrp_opts_default(&o);
+    o.b = blk;
+    o.tol = tol;
+    o.seed = seed;
+    res.perm = perm;
+    res.X = xbuf;
+    res.ldx = p > 0 ? p : 1;
+    res.trace_cap = p + 1;
+    res.ranks = r64.data();
+    res.error = err.data();
+    res.t_ranks = tr.data();
+    st = rbrp_colid_d(h, A, m, n, m, &o, &res);
+  }
+  if (!st) {
+    const int k = (int)res.k;
+    std::vector<int32_t> hp((size_t)n);
+    st = (int)hipMemcpy(hp.data(), perm, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost);
+    sk.assign(hp.begin(), hp.begin() + k);
+    rd.assign(hp.begin() + k, hp.end());
+    X = nullptr;
+    if (!st && k > 0 && n > k) {
+      st = (int)hipMalloc(&X, sizeof(double) * (size_t)k * (size_t)(n - k));
+      if (!st)
+        st = (int)hipMemcpy2D(X, sizeof(double) * (size_t)k, xbuf, sizeof(double) * (size_t)res.ldx,
+                              sizeof(double) * (size_t)k, (size_t)(n - k), hipMemcpyDeviceToDevice);
+    }
+    ranks.assign(r64.begin(), r64.begin() + res.nblocks);
+    error.assign(err.begin(), err.begin() + res.nblocks);
+  }
+  if (perm) (void)hipFree(perm);
+  if (xbuf) (void)hipFree(xbuf);
+  rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  return st;
+}
+
+#endif  // RBRP_HPP_

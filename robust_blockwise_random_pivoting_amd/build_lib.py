@@ -18,6 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "librbrp.so")
+CXX_CHECK = os.path.join(PKG, "bin", "rbrp_hpp_check")
 
 SOURCES = ["k_gemm.hip", "k_rbrp.hip", "rbrp_host.cpp", "rbrp_testing.cpp"]
 
@@ -64,6 +65,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ drop-in check of include/rbrp.hpp, linked against the library
+    os.makedirs(os.path.dirname(CXX_CHECK), exist_ok=True)
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cxx", "rbrp_hpp_check.cpp"), "-L", PKG, "-lrbrp",
+           "-Wl,-rpath,$ORIGIN/..", "-o", CXX_CHECK]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rbrp.hpp check failed to build:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -1,0 +1,54 @@
+// Drop-in check of include/rbrp.hpp: calls the reference's RBRP signature and
+// the added tolerance overload on a device matrix and prints one JSON line.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "rbrp.hpp"
+
+int main() {
+  const int m = 400, n = 300;
+  // A = sum_k exp(-k/8) u_k v_k^T with random u, v (rapidly decaying rank)
+  std::mt19937_64 g(7);
+  std::normal_distribution<double> nd;
+  std::vector<double> A((size_t)m * n, 0.0);
+  for (int k = 0; k < 80; ++k) {
+    std::vector<double> u(m), v(n);
+    for (auto& x : u) x = nd(g);
+    for (auto& x : v) x = nd(g);
+    const double s = std::exp(-k / 8.0);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) A[i + (size_t)j * m] += s * u[i] * v[j];
+  }
+  double* dA = nullptr;
+  if (hipMalloc(&dA, sizeof(double) * A.size()) != hipSuccess) return 2;
+  if (hipMemcpy(dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice) != hipSuccess) return 2;
+  std::vector<int> ranks;
+  std::vector<double> error, t_ranks;
+  double t_rbrp = 0, flops = 0;
+  RBRP(m, n, dA, 32, ranks, error, t_ranks, t_rbrp, flops);
+  const int st1 = RBRP_last_status();
+  std::vector<int> sk, rd, r2;
+  std::vector<double> e2;
+  double* X = nullptr;
+  const int st2 = RBRP(m, n, dA, 32, 1e-6, sk, rd, X, r2, e2, 3);
+  // host check of the ID error ||A(:,rd) - A(:,sk) X|| / ||A||
+  const int k = (int)sk.size();
+  std::vector<double> hX((size_t)k * (n - k));
+  if (X) (void)hipMemcpy(hX.data(), X, sizeof(double) * hX.size(), hipMemcpyDeviceToHost);
+  double num = 0, den = 0;
+  for (int j = 0; j < n - k; ++j)
+    for (int i = 0; i < m; ++i) {
+      double v = A[i + (size_t)rd[j] * m];
+      for (int q = 0; q < k; ++q) v -= A[i + (size_t)sk[q] * m] * hX[q + (size_t)j * k];
+      num += v * v;
+    }
+  for (double x : A) den += x * x;
+  std::printf("{\"status1\": %d, \"final_rank\": %d, \"nblocks\": %zu, \"status2\": %d, \"k\": %d, "
+              "\"id_error\": %.6e, \"flops\": %.6e}\n",
+              st1, ranks.empty() ? 0 : ranks.back(), ranks.size(), st2, k, std::sqrt(num / den), flops);
+  if (X) (void)hipFree(X);
+  (void)hipFree(dA);
+  return (st1 == 0 && st2 == 0) ? 0 : 1;
+}

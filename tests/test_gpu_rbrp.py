@@ -153,3 +153,17 @@ def test_rbrp_large_properties():
     assert err <= 1e-8 * 1.001
     assert np.sqrt(g.error[-1]) == pytest.approx(err, rel=1e-3)
     assert 1800 <= g.k <= 2300
+
+
+def test_cxx_call_surface_drop_in():
+    """include/rbrp.hpp compiled into a C++ program (the reference's RBRP signature)."""
+    import json
+    import os
+    import subprocess
+    from robust_blockwise_random_pivoting_amd import build_lib
+    assert os.path.exists(build_lib.CXX_CHECK), "build the library first"
+    r = subprocess.run([build_lib.CXX_CHECK], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["status1"] == 0 and out["final_rank"] == 300
+    assert out["status2"] == 0 and out["id_error"] <= 1e-6 and out["k"] > 0
