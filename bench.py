@@ -141,7 +141,7 @@ def main():
     roof = None
     if dom:
         d = kern[dom]
-        roof = {"kernel": {"project": "k_gemm_tn", "update": "k_gemm_nn"}[dom], "bound": d["bound"],
+        roof = {"kernel": {"project": "k_gemm_tn", "update": "k_gemm_trail"}[dom], "bound": d["bound"],
                 "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
                 "traffic": None, "avg_ms": d["avg_ms"], "launches": d["launches"]}
 

@@ -29,6 +29,13 @@ int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const doubl
                       const double* B, int64_t ldb, double* C, int64_t ldc, int mode,
                       double* normpart);
 
+/* Trailing-update kernel (K <= 128): C (M x N) -= A Bt^T-layout product, i.e.
+ * C(r, c) -= sum_k A(r, k) Bt[k * ldbt + c] with A (M x K, lda) column-major
+ * and Bt row-major.  normpart (optional) receives ceil(M/128) x N per-128-row
+ * tile squared column norms of the new C (leading dimension N). */
+int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                       const double* Bt, int64_t ldbt, double* C, int64_t ldc, double* normpart);
+
 /* Pivoted Cholesky + robust filter of an s x s Gram matrix G (ldg = s) whose
  * columns correspond to the column indices J[0:s] (device), at block start i.
  * Outputs (device, capacity b each): Jk[0:bt] (pivot order), Rb (b x b, ld b),

@@ -69,6 +69,7 @@ def lib():
         "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
         "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
+        "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
         "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
                                     POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
         "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,

@@ -9,6 +9,8 @@
 //   A[i][k] in lane i + 16k,  B[k][j] in lane j + 16k,
 //   D[i][j] in lane j + 16*(i%4), accumulator element i / 4
 //   (measured on gfx950 with a one-hot probe of the A operand).
+#include <algorithm>
+
 #include "rbrp_kernels.h"
 
 #define KT 16
@@ -104,46 +106,67 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 }
 
 // ---------------------------------------------------------------------------
-// Split reduction: dst(r, c) (=|+=) sum_{z=0}^{nsplit-1} P[z](r, c), fixed order.
+// Split reduction: dst(r, c) (=|+=|-=) sum_{z=0}^{nsplit-1} P[z](r, c).
+// One workgroup per (64-row slab, column): four waves each sum every fourth
+// split (two accumulators), then the four partials are added in a fixed
+// order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_reduce_splits(DimSpec dM, DimSpec dN,
                                                        const double* __restrict__ P, int64_t ldp,
                                                        int64_t pstride, int nsplit, MatRef dst,
-                                                       int mode) {
+                                                       int mode, double* __restrict__ dst2,
+                                                       int64_t ld2) {
+  __shared__ double part[4][64];
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t r = (int64_t)blockIdx.y * 256 + threadIdx.x;
   const int64_t c = blockIdx.x;
-  if (r >= M || c >= N) return;
-  double s = 0.0;
-  for (int z = 0; z < nsplit; ++z) s += P[(int64_t)z * pstride + r + c * ldp];
-  double* d = mat_col(dst, c) + r;
-  if (mode == GEMM_ADD)
-    *d += s;
-  else if (mode == GEMM_SUB)
-    *d -= s;
-  else
-    *d = s;
+  const int lr = threadIdx.x & 63, zg = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.y * 64 + lr;
+  if (c >= N || (int64_t)blockIdx.y * 64 >= M) return;  // uniform per workgroup
+  double s0 = 0.0, s1 = 0.0;
+  if (r < M) {
+    const double* p = P + r + c * ldp;
+    int z = zg;
+    for (; z + 4 < nsplit; z += 8) {
+      s0 += p[(int64_t)z * pstride];
+      s1 += p[(int64_t)(z + 4) * pstride];
+    }
+    for (; z < nsplit; z += 4) s0 += p[(int64_t)z * pstride];
+  }
+  part[zg][lr] = s0 + s1;
+  __syncthreads();
+  if (zg == 0 && r < M) {
+    const double s = (part[0][lr] + part[1][lr]) + (part[2][lr] + part[3][lr]);
+    double* d = mat_col(dst, c) + r;
+    if (mode == GEMM_ADD)
+      *d += s;
+    else if (mode == GEMM_SUB)
+      *d -= s;
+    else
+      *d = s;
+    if (dst2) dst2[r * ld2 + c] = s;  // row-major copy
+  }
 }
 
 hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
-                                     MatRef dst, int mode) {
+                                     MatRef dst, int mode, double* dst2, int64_t ld2) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
-  dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 255) / 256));
+  dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
   hipLaunchKernelGGL(k_reduce_splits, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit, dst,
-                     mode);
+                     mode, dst2, ld2);
   return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // NN with update: C(r, c) = (mode) A(r, 0:K) B(0:K, c); optional per-tile
 // squared column norms of the result.  The accumulator tile is staged through
-// LDS so that C is read and written in 512-byte column runs.
+// LDS so that C is read and written in 512-byte column runs; each wave issues
+// the loads of all 16 of its C columns before using any of them.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gemm_nn(int64_t Mr, DimSpec dN, DimSpec dK, MatRef A,
+__global__ __launch_bounds__(256) void k_gemm_nn(DimSpec dM, DimSpec dN, DimSpec dK, MatRef A,
                                                  MatRef B, MatRef C, int mode,
                                                  double* __restrict__ normpart, int64_t ldnp) {
-  const int64_t N = dim_eval(dN), K = dim_eval(dK);
+  const int64_t Mr = dim_eval(dM), N = dim_eval(dN), K = dim_eval(dK);
   const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
   if (r0 >= Mr || c0 >= N) return;
 
@@ -200,48 +223,230 @@ __global__ __launch_bounds__(256) void k_gemm_nn(int64_t Mr, DimSpec dN, DimSpec
       acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
   }
-  // stage the accumulators as Cs[c][r]
+  // wave `wid` updates columns wid, wid+4, ..., wid+60; lane = row within the tile.
+  // Issue the C loads first (independent of the accumulators) ...
+  const int64_t r = r0 + lane;
+  const bool rok = r < Mr;
+  const int ncols = (int)min((int64_t)64, N - c0);
+  double cv[16];
+  if (mode != GEMM_SET) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = wid + 4 * q;
+      cv[q] = (c < ncols && rok) ? mat_col(C, c0 + c)[r] : 0.0;
+    }
+  }
+  // ... then stage the accumulators as Cs[c][r]
 #pragma unroll
   for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
     for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = wm * 32 + mi * 16 + mf_row(lane, v);
+        const int rr = wm * 32 + mi * 16 + mf_row(lane, v);
         const int c = wn * 32 + ni * 16 + mf_col(lane);
-        Cs[c * 65 + r] = acc[mi][ni][v];
+        Cs[c * 65 + rr] = acc[mi][ni][v];
       }
   __syncthreads();
-  // wave `wid` updates columns wid, wid+4, ...; lane = row within the tile
-  const int64_t r = r0 + lane;
-  const bool rok = r < Mr;
+  double nsq[16];
+#pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = wid + 4 * q;
-    if (c0 + c >= N) break;  // uniform across the wave
-    double* cp = mat_col(C, c0 + c);
     const double v = Cs[c * 65 + lane];
-    double nv = 0.0;
-    if (rok) {
-      if (mode == GEMM_SUB)
-        nv = cp[r] - v;
-      else if (mode == GEMM_ADD)
-        nv = cp[r] + v;
-      else
-        nv = v;
-      cp[r] = nv;
-    }
-    if (normpart) {
-      const double t = rbrp_wave_sum(nv * nv);
-      if (lane == 0) normpart[(int64_t)blockIdx.y * ldnp + c0 + c] = t;
+    double nv;
+    if (mode == GEMM_SUB)
+      nv = cv[q] - v;
+    else if (mode == GEMM_ADD)
+      nv = cv[q] + v;
+    else
+      nv = v;
+    if (c < ncols && rok) mat_col(C, c0 + c)[r] = nv;
+    nsq[q] = (c < ncols && rok) ? nv * nv : 0.0;
+  }
+  if (normpart) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = wid + 4 * q;
+      const double t = rbrp_wave_sum(nsq[q]);
+      if (lane == 0 && c < ncols) normpart[(int64_t)blockIdx.y * ldnp + c0 + c] = t;
     }
   }
 }
 
-hipError_t rbrp_launch_gemm_nn(hipStream_t st, int64_t Mr, DimSpec N, int64_t Nmax, DimSpec K,
-                               MatRef A, MatRef B, MatRef C, int mode, double* normpart,
+hipError_t rbrp_launch_gemm_nn(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
+                               DimSpec K, MatRef A, MatRef B, MatRef C, int mode, double* normpart,
                                int64_t ldnp) {
-  if (Mr <= 0 || Nmax <= 0) return hipSuccess;
-  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mr + 63) / 64));
-  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(256), 0, st, Mr, N, K, A, B, C, mode, normpart, ldnp);
+  if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
+  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mmax + 63) / 64));
+  hipLaunchKernelGGL(k_gemm_nn, grid, dim3(256), 0, st, M, N, K, A, B, C, mode, normpart, ldnp);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tall rank-K update (the trailing-matrix update and the CholQR2 products):
+// C(r, c) (op)= sum_k A(r, k) B(k, c) for r < M, c < N, K <= 128, optional
+// squared column norms of the new C per 128-row tile.
+//
+// The MFMA computes the transposed product D = B^T A^T so that the result
+// layout puts consecutive rows of C in consecutive lanes: lane l holds
+// C(row0 + l%16, col0 + 4v + l/16), i.e. every C load/store instruction moves
+// four 128-byte column runs, with no LDS staging of C.
+//
+// Workgroup = 4 waves stacked along rows, 128 rows x 32 columns per tile,
+// sweeping `T` consecutive row tiles: B (K x 32) sits in LDS for the whole
+// sweep; for each tile the A fragments (16 consecutive rows of one A column
+// per 16 lanes) are loaded first and the next tile's C is prefetched right
+// after them, so the A loads can be waited on while the C prefetch is still
+// in flight (the vector-memory counter retires loads in issue order).
+// ---------------------------------------------------------------------------
+// A and C are plain column-major matrices whose first column is
+// coff + (*coff_dyn) (the trailing block of the working matrix); Bt is the
+// row-major copy of B written by the projection's split reduction.
+template <int JB, int KSTEPS, bool NORMS>
+__global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSpec dK, int T,
+                                                    const double* __restrict__ A0, int64_t lda,
+                                                    int64_t acoff, const int* acoff_dyn,
+                                                    const double* __restrict__ Bt, int64_t ldbt,
+                                                    double* __restrict__ C0, int64_t ldc,
+                                                    int64_t ccoff, const int* ccoff_dyn,
+                                                    double* __restrict__ normpart, int64_t ldnp) {
+  __shared__ double Ws[4 * KSTEPS * 32];  // [k][32]
+  __shared__ double red[4][32];
+  const int64_t N = dim_eval(dN), K = dim_eval(dK);
+  const int64_t c0 = (int64_t)blockIdx.x * 32;
+  const int64_t rows_per_tile = 64 * JB;
+  const int64_t rbase = (int64_t)blockIdx.y * T * rows_per_tile;
+  if (c0 >= N || rbase >= M) return;  // uniform per workgroup
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = A0 + (acoff + (acoff_dyn ? *acoff_dyn : 0)) * lda;
+  double* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + c0) * ldc;
+  for (int idx = threadIdx.x; idx < 4 * KSTEPS * 32; idx += 256) {
+    const int k = idx >> 5, c = idx & 31;
+    Ws[idx] = (k < K && c0 + c < N) ? Bt[(int64_t)k * ldbt + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  // column offsets (within the 32-column tile) held by this lane: 16*ib + 4v + lk
+  const int ncol = (int)min((int64_t)32, N - c0);
+
+  double cvA[JB][2][4], cvB[JB][2][4];
+  auto loadC = [&](int64_t r0, double (&cv)[JB][2][4]) {
+    const int64_t rw = r0 + 16 * JB * wid;
+#pragma unroll
+    for (int jb = 0; jb < JB; ++jb)
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int cc = 16 * ib + 4 * v + lk;
+          const int64_t r = rw + 16 * jb + li;
+          cv[jb][ib][v] = (cc < ncol && r < M) ? C[r + cc * ldc] : 0.0;
+        }
+  };
+  loadC(rbase, cvA);
+  for (int t = 0; t < T; ++t) {
+    const int64_t r0 = rbase + (int64_t)t * rows_per_tile;
+    if (r0 >= M) break;  // uniform per workgroup
+    const int64_t rw = r0 + 16 * JB * wid;
+    double fa[KSTEPS][JB];
+#pragma unroll
+    for (int q = 0; q < KSTEPS; ++q) {
+      const int k = 4 * q + lk;
+#pragma unroll
+      for (int jb = 0; jb < JB; ++jb) {
+        const int64_t r = rw + 16 * jb + li;
+        fa[q][jb] = (k < K && r < M) ? A[r + (int64_t)k * lda] : 0.0;
+      }
+    }
+    if (t + 1 < T) loadC(r0 + rows_per_tile, cvB);
+    v4d acc[JB][2];
+#pragma unroll
+    for (int a = 0; a < JB; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < KSTEPS; ++q) {
+      const int k = 4 * q + lk;
+      const double b0 = Ws[k * 32 + li], b1 = Ws[k * 32 + 16 + li];
+#pragma unroll
+      for (int jb = 0; jb < JB; ++jb) {
+        acc[jb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, fa[q][jb], acc[jb][0], 0, 0, 0);
+        acc[jb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, fa[q][jb], acc[jb][1], 0, 0, 0);
+      }
+    }
+    // D[i][j] (i = column within the 16-column block, j = row) sits in lane
+    // j + 16*(i%4), element i/4: with i = 4v + lk that is lane li + 16*lk.
+    double nsq[2][4];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) nsq[ib][v] = 0.0;
+#pragma unroll
+    for (int jb = 0; jb < JB; ++jb)
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int cc = 16 * ib + 4 * v + lk;
+          const int64_t r = rw + 16 * jb + li;
+          const double nv = cvA[jb][ib][v] - acc[jb][ib][v];
+          if (cc < ncol && r < M) {
+            C[r + cc * ldc] = nv;
+            nsq[ib][v] += nv * nv;
+          }
+        }
+    if (NORMS) {
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          double x = nsq[ib][v];
+          x += __shfl_xor(x, 1, 64);
+          x += __shfl_xor(x, 2, 64);
+          x += __shfl_xor(x, 4, 64);
+          x += __shfl_xor(x, 8, 64);
+          if (li == 0) red[wid][16 * ib + 4 * v + lk] = x;
+        }
+      __syncthreads();
+      if (wid == 0 && lane < ncol)
+        normpart[(r0 >> 7) * ldnp + c0 + lane] =
+            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int jb = 0; jb < JB; ++jb)
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cvA[jb][ib][v] = cvB[jb][ib][v];
+  }
+}
+
+hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
+                                  int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
+                                  const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
+                                  int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
+                                  double* normpart, int64_t ldnp) {
+  if (M <= 0 || Nmax <= 0) return hipSuccess;
+  if (Kmax > 128) return hipErrorInvalidValue;
+  const bool small = Kmax <= 64;
+  const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 127) / 128;
+  int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 2048));
+  dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
+  if (small) {
+    if (normpart)
+      hipLaunchKernelGGL((k_gemm_trail<2, 16, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+                         acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+    else
+      hipLaunchKernelGGL((k_gemm_trail<2, 16, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+                         acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+  } else {
+    if (normpart)
+      hipLaunchKernelGGL((k_gemm_trail<2, 32, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+                         acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+    else
+      hipLaunchKernelGGL((k_gemm_trail<2, 32, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+                         acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+  }
   return hipGetLastError();
 }

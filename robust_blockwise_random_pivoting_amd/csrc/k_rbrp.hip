@@ -113,19 +113,28 @@ __device__ inline double block_exclusive_scan(double v, double* wsum) {
 
 #define SAMPLE_MAXB 256
 
+// Weights and CDF live in LDS when the candidate range fits (N <= SAMPLE_LDS_N),
+// otherwise in the global scratch arrays w / cdf.
+#define SAMPLE_LDS_N 8192
+
 __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diags, int64_t i,
                                                  int64_t n, int64_t kmax, int b, uint64_t key0,
                                                  uint64_t key1, uint64_t c0, uint64_t c1,
-                                                 uint64_t c2, uint64_t c3, double* __restrict__ w,
-                                                 double* __restrict__ cdf, RbrpBlockState* bs,
+                                                 uint64_t c2, uint64_t c3, double* __restrict__ gw,
+                                                 double* __restrict__ gcdf, RbrpBlockState* bs,
                                                  int* __restrict__ J) {
+  extern __shared__ double lds[];
   __shared__ int found[SAMPLE_MAXB];
   __shared__ int newidx[SAMPLE_MAXB];
+  __shared__ int isnew[SAMPLE_MAXB];
   __shared__ int ired[16];
   __shared__ double wsum[16];
   __shared__ int s_nfound;
   const int tid = threadIdx.x;
   const int64_t N = n - i;
+  const bool in_lds = N <= SAMPLE_LDS_N;
+  double* w = in_lds ? lds : gw;
+  double* cdf = in_lds ? lds + SAMPLE_LDS_N : gcdf;
   int cnt = 0;
   for (int64_t j = tid; j < N; j += 1024) {
     double v = diags[i + j];
@@ -183,19 +192,24 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
       newidx[tid] = (int)lo;
     }
     __syncthreads();
+    // keep the first occurrence of each index not found before
+    if (tid < r) {
+      const int v = newidx[tid];
+      int dup = 0;
+      for (int q = 0; q < nf; ++q) dup |= (found[q] == v) ? 1 : 0;
+      for (int q = 0; q < tid; ++q) dup |= (newidx[q] == v) ? 1 : 0;
+      isnew[tid] = !dup;
+    }
+    __syncthreads();
+    if (tid < r && isnew[tid]) {
+      int pos = nf;
+      for (int q = 0; q < tid; ++q) pos += isnew[q];
+      found[pos] = newidx[tid];
+    }
     if (tid == 0) {
-      int nfl = nf;
-      for (int t = 0; t < r; ++t) {
-        const int v = newidx[t];
-        bool dup = false;
-        for (int q = 0; q < nfl; ++q)
-          if (found[q] == v) {
-            dup = true;
-            break;
-          }
-        if (!dup) found[nfl++] = v;
-      }
-      s_nfound = nfl;
+      int add = 0;
+      for (int q = 0; q < r; ++q) add += isnew[q];
+      s_nfound = nf + add;
     }
     draws += (unsigned long long)r;
     __syncthreads();
@@ -220,7 +234,14 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
                               int* J) {
   if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), 0, st, diags, i, n, kmax, b, key0, key1, ctr[0],
+  const bool in_lds = (n - i) <= SAMPLE_LDS_N;
+  const size_t lds = in_lds ? 2 * SAMPLE_LDS_N * sizeof(double) : 0;
+  if (lds > 48 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_sample,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), lds, st, diags, i, n, kmax, b, key0, key1, ctr[0],
                      ctr[1], ctr[2], ctr[3], w, cdf, bs, J);
   return hipGetLastError();
 }
@@ -264,6 +285,57 @@ hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_d
 // and the ascending symmetric differences Jin = sort(Jk) \ [i, i+bt),
 // Jout = [i, i+bt) \ Jk (gpu/rbrp.cu:211-234).
 // ---------------------------------------------------------------------------
+// Upper-triangular inverse of the bt x bt factor held in the upper triangle of
+// S (leading dimension ld), computed by rows from the bottom up: row r of the
+// inverse needs only rows r+1.. of it.  Inverse entry (l, c), l < c, is kept
+// in S[c][l] (the strictly-lower triangle, free once the factorisation is
+// done) and the diagonal in dinv; the result is written to Rinv (ld ldi) with
+// zeros below the diagonal.
+__device__ inline void upper_inverse_lds(double* S, int ld, int bt, double* dinv,
+                                         double* __restrict__ Rinv, int64_t ldi) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int c = tid; c < bt; c += nt) dinv[c] = 1.0 / S[c * ld + c];
+  __syncthreads();
+  for (int r = bt - 2; r >= 0; --r) {
+    for (int c = r + 1 + tid; c < bt; c += nt) {
+      double acc = S[r * ld + c] * dinv[c];
+      for (int l = r + 1; l < c; ++l) acc += S[r * ld + l] * S[c * ld + l];
+      S[c * ld + r] = -acc * dinv[r];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < bt * bt; idx += nt) {
+    const int r = idx % bt, c = idx / bt;
+    Rinv[r + (int64_t)c * ldi] = (r < c) ? S[c * ld + r] : ((r == c) ? dinv[c] : 0.0);
+  }
+}
+
+// Wave-level argmax of S[c][c] over c in [j, s): largest value, lowest index on ties.
+__device__ inline void wave_argmax_diag(const double* S, int ld, int j, int s, double* best_out,
+                                        int* idx_out) {
+  const int lane = threadIdx.x & 63;
+  double bv = -1.0;
+  int bi = 0x7fffffff;
+  for (int c = j + lane; c < s; c += 64) {
+    const double d = S[c * ld + c];
+    if (d > bv || (d == bv && c < bi)) {
+      bv = d;
+      bi = c;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ov = __shfl_xor(bv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > bv || (ov == bv && oi < bi)) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  *best_out = bv;
+  *idx_out = bi;
+}
+
 __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict__ G, int64_t ldg,
                                                         int b, int filter_mode, int64_t i,
                                                         const int* __restrict__ J,
@@ -274,7 +346,10 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   extern __shared__ double S[];  // s x (s + 1)
   __shared__ int piv[SAMPLE_MAXB];
   __shared__ int sj[SAMPLE_MAXB];
+  __shared__ int sorted[SAMPLE_MAXB];
+  __shared__ int flag[SAMPLE_MAXB];
   __shared__ double sqr[SAMPLE_MAXB];
+  __shared__ double dinv[SAMPLE_MAXB];
   __shared__ int s_pvt, s_stop, s_bt;
   __shared__ double s_rjj;
   const int tid = threadIdx.x;
@@ -295,18 +370,15 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   __syncthreads();
   int rank = s;
   for (int j = 0; j < s; ++j) {
-    if (tid == 0) {
-      int p = j;
-      double best = S[j * ld + j];
-      for (int c = j + 1; c < s; ++c) {
-        const double d = S[c * ld + c];
-        if (d > best) {
-          best = d;
-          p = c;
-        }
+    if (tid < 64) {
+      double best;
+      int p;
+      wave_argmax_diag(S, ld, j, s, &best, &p);
+      if (tid == 0) {
+        s_pvt = p;
+        s_stop = (best > 0.0) ? 0 : 1;
+        s_rjj = (best > 0.0) ? sqrt(best) : 0.0;
       }
-      s_pvt = p;
-      s_stop = (best > 0.0) ? 0 : 1;
     }
     __syncthreads();
     if (s_stop) {
@@ -333,14 +405,8 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
       }
       __syncthreads();
     }
-    if (tid == 0) {
-      const double rjj = sqrt(S[j * ld + j]);
-      S[j * ld + j] = rjj;
-      s_rjj = rjj;
-    }
-    __syncthreads();
     const double rjj = s_rjj;
-    for (int c = j + 1 + tid; c < s; c += 256) S[j * ld + c] /= rjj;
+    for (int c = j + tid; c < s; c += 256) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
     __syncthreads();
     const int nt = s - j - 1;
     for (int idx = tid; idx < nt * nt; idx += 256) {
@@ -391,35 +457,35 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
     Rb[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
   }
   __syncthreads();
-  // inverse of the upper-triangular Rb, one column per thread
-  for (int c = tid; c < bt; c += 256) {
-    double* col = Rinv + (int64_t)c * b;
-    for (int r = c + 1; r < bt; ++r) col[r] = 0.0;
-    col[c] = 1.0 / S[c * ld + c];
-    for (int r = c - 1; r >= 0; --r) {
-      double acc = 0.0;
-      for (int l = r + 1; l <= c; ++l) acc += S[r * ld + l] * col[l];
-      col[r] = -acc / S[r * ld + r];
+  upper_inverse_lds(S, ld, bt, dinv, Rinv, b);
+  // ascending sort of Jk by rank counting (entries are distinct)
+  for (int t = tid; t < bt; t += 256) {
+    const int v = sj[t];
+    int rk = 0;
+    for (int q = 0; q < bt; ++q) rk += (sj[q] < v) ? 1 : 0;
+    sorted[rk] = v;
+  }
+  // Jout membership flags for the block positions i .. i+bt-1
+  for (int t = tid; t < bt; t += 256) {
+    const int q = (int)(i + t);
+    int member = 0;
+    for (int a = 0; a < bt; ++a) member |= (sj[a] == q) ? 1 : 0;
+    flag[t] = !member;
+  }
+  __syncthreads();
+  int nin = 0;
+  for (int a = 0; a < bt; ++a) nin += ((int64_t)sorted[a] >= i + bt) ? 1 : 0;
+  for (int t = tid; t < nin; t += 256) Jin[t] = sorted[bt - nin + t];
+  for (int t = tid; t < bt; t += 256) {
+    if (flag[t]) {
+      int pos = 0;
+      for (int q = 0; q < t; ++q) pos += flag[q];
+      Jout[pos] = (int)(i + t);
     }
   }
   if (tid == 0) {
-    for (int a = 1; a < bt; ++a) {
-      const int v = sj[a];
-      int q = a - 1;
-      while (q >= 0 && sj[q] > v) {
-        sj[q + 1] = sj[q];
-        --q;
-      }
-      sj[q + 1] = v;
-    }
-    int nin = 0, nout = 0, ptr = 0;
-    for (int a = 0; a < bt; ++a)
-      if ((int64_t)sj[a] >= i + bt) Jin[nin++] = sj[a];
-    for (int64_t q = i; q < i + bt; ++q) {
-      while (ptr < bt && (int64_t)sj[ptr] < q) ++ptr;
-      const bool member = (ptr < bt && (int64_t)sj[ptr] == q);
-      if (!member) Jout[nout++] = (int)q;
-    }
+    int nout = 0;
+    for (int t = 0; t < bt; ++t) nout += flag[t];
     if (nin != nout) bs->status = 3;
     bs->bt = bt;
     bs->nswap = nin;
@@ -442,14 +508,14 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
 }
 
 // ---------------------------------------------------------------------------
-// Second Cholesky-QR pass: G2 = Q1^T Q1 = R2^T R2; R2inv = R2^{-1};
-// R(i:i+bt, i:i+bt) = R2 * Rb (upper triangular, strictly-lower part zeroed).
+// Second Cholesky-QR pass: G2 = Q1^T Q1 = R2^T R2.  Writes R2 (b x b, zeros
+// below the diagonal) and R2inv = R2^{-1}; the caller forms R2 * Rb.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, int64_t ldg, int b,
-                                               RbrpBlockState* bs, const double* __restrict__ Rb,
-                                               double* __restrict__ R2inv, double* __restrict__ Rbig,
-                                               int64_t ldr, int64_t i) {
+                                               RbrpBlockState* bs, double* __restrict__ R2,
+                                               double* __restrict__ R2inv) {
   extern __shared__ double S[];
+  __shared__ double dinv[SAMPLE_MAXB];
   __shared__ int s_bad;
   const int tid = threadIdx.x;
   const int bt = bs->bt;
@@ -462,14 +528,11 @@ __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, in
   if (tid == 0) s_bad = 0;
   __syncthreads();
   for (int j = 0; j < bt; ++j) {
-    if (tid == 0) {
-      const double d = S[j * ld + j];
-      if (!(d > 0.0)) s_bad = 1;
-      S[j * ld + j] = sqrt(d > 0.0 ? d : 1.0);
-    }
+    const double d = S[j * ld + j];
+    const double rjj = sqrt(d > 0.0 ? d : 1.0);
     __syncthreads();
-    const double rjj = S[j * ld + j];
-    for (int c = j + 1 + tid; c < bt; c += 256) S[j * ld + c] /= rjj;
+    if (tid == 0 && !(d > 0.0)) s_bad = 1;
+    for (int c = j + tid; c < bt; c += 256) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
     __syncthreads();
     const int nt = bt - j - 1;
     for (int idx = tid; idx < nt * nt; idx += 256) {
@@ -479,35 +542,23 @@ __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, in
     __syncthreads();
   }
   if (tid == 0 && s_bad) bs->status = 2;
-  for (int c = tid; c < bt; c += 256) {
-    double* col = R2inv + (int64_t)c * b;
-    for (int r = c + 1; r < bt; ++r) col[r] = 0.0;
-    col[c] = 1.0 / S[c * ld + c];
-    for (int r = c - 1; r >= 0; --r) {
-      double acc = 0.0;
-      for (int l = r + 1; l <= c; ++l) acc += S[r * ld + l] * col[l];
-      col[r] = -acc / S[r * ld + r];
-    }
-  }
   for (int idx = tid; idx < bt * bt; idx += 256) {
     const int r = idx % bt, c = idx / bt;
-    double v = 0.0;
-    if (r <= c)
-      for (int l = r; l <= c; ++l) v += S[r * ld + l] * Rb[l + (int64_t)c * b];
-    Rbig[(i + r) + (i + c) * ldr] = v;
+    R2[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
   }
+  __syncthreads();
+  upper_inverse_lds(S, ld, bt, dinv, R2inv, b);
 }
 
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
-                             RbrpBlockState* bs, const double* Rb, double* R2inv, double* Rbig,
-                             int64_t ldr, int64_t i) {
+                             RbrpBlockState* bs, double* R2, double* R2inv) {
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k_chol2,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, st, G2, ldg, b, bs, Rb, R2inv, Rbig, ldr, i);
+  hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, st, G2, ldg, b, bs, R2, R2inv);
   return hipGetLastError();
 }
 
@@ -610,22 +661,44 @@ hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockS
 }
 
 // diags[i + bt + c] = sum_{t < ntiles} part[t * ldp + c], c < n - i - bt.
-__global__ void k_colnorm_reduce(const double* __restrict__ part, int64_t ldp, int64_t ntiles,
-                                 int64_t i, int64_t n, RbrpBlockState* bs,
-                                 double* __restrict__ diags) {
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// One workgroup per 64 columns; wave w sums tiles w, w+16, ... (lane = column,
+// so every load is a 512-byte run), then the 16 wave partials are added in
+// wave order.
+__global__ __launch_bounds__(1024) void k_colnorm_reduce(const double* __restrict__ part,
+                                                         int64_t ldp, int64_t ntiles, int64_t i,
+                                                         int64_t n, RbrpBlockState* bs,
+                                                         double* __restrict__ diags) {
+  __shared__ double red[16][64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int64_t c0 = i + bs->bt;
-  if (c0 + c >= n) return;
-  double acc = 0.0;
-  for (int64_t t = 0; t < ntiles; ++t) acc += part[t * ldp + c];
-  diags[c0 + c] = acc;
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  if (c0 + (int64_t)blockIdx.x * 64 >= n) return;  // uniform per workgroup
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (c0 + c < n) {
+    const double* p = part + c;
+    int64_t t = wid;
+    for (; t + 48 < ntiles; t += 64) {
+      a0 += p[t * ldp];
+      a1 += p[(t + 16) * ldp];
+      a2 += p[(t + 32) * ldp];
+      a3 += p[(t + 48) * ldp];
+    }
+    for (; t < ntiles; t += 16) a0 += p[t * ldp];
+  }
+  red[wid][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (wid == 0 && c0 + c < n) {
+    double acc = 0.0;
+    for (int w = 0; w < 16; ++w) acc += red[w][lane];
+    diags[c0 + c] = acc;
+  }
 }
 
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
                                       int64_t i, int64_t n, RbrpBlockState* bs, double* diags) {
   const int64_t cmax = n - i;
   if (cmax <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 255) / 256)), dim3(256), 0, st, part,
+  hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 63) / 64)), dim3(1024), 0, st, part,
                      ldp, ntiles, i, n, bs, diags);
   return hipGetLastError();
 }

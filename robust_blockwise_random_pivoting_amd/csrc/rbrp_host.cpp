@@ -37,8 +37,8 @@ struct Workspace {
   void* arena = nullptr;
   size_t arena_bytes = 0;
   // carved pointers
-  double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2inv, *Cbuf, *Qbuf, *P, *normpart, *tmpk,
-      *tmpo;
+  double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
+      *tmpk, *tmpo, *Wt;
   int *J, *Jk, *Jin, *Jout;
   RbrpBlockState* st;
   int64_t Pcap;
@@ -81,11 +81,13 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   size_t oG2 = c.take<double>(b * b);
   size_t oRb = c.take<double>(b * b);
   size_t oRinv = c.take<double>(b * b);
+  size_t oR2 = c.take<double>(b * b);
   size_t oR2inv = c.take<double>(b * b);
   size_t oCbuf = c.take<double>(ldr * b);
   size_t oQbuf = c.take<double>(mp * b);
   size_t oP = c.take<double>(Pcap);
-  size_t onp = c.take<double>((mp / 64) * n);
+  size_t onp = c.take<double>(((mp + 127) / 128) * n);
+  size_t oWt = c.take<double>(b * n);
   size_t otk = c.take<double>(ldr * b);
   size_t oto = c.take<double>(ldr * b);
   size_t oJ = c.take<int>(b);
@@ -118,11 +120,13 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   ws.G2 = (double*)(base + oG2);
   ws.Rb = (double*)(base + oRb);
   ws.Rinv = (double*)(base + oRinv);
+  ws.R2 = (double*)(base + oR2);
   ws.R2inv = (double*)(base + oR2inv);
   ws.Cbuf = (double*)(base + oCbuf);
   ws.Qbuf = (double*)(base + oQbuf);
   ws.P = (double*)(base + oP);
   ws.normpart = (double*)(base + onp);
+  ws.Wt = (double*)(base + oWt);
   ws.tmpk = (double*)(base + otk);
   ws.tmpo = (double*)(base + oto);
   ws.J = (int*)(base + oJ);
@@ -140,6 +144,7 @@ void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pc
                   int64_t* kchunk) {
   const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + 63) / 64);
   int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  ns = std::min<int64_t>(ns, 128);
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, K / 64));
   int64_t kc = round_up((K + ns - 1) / ns, 16);
@@ -358,7 +363,7 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
   const double tol2 = opts->tol * opts->tol;
   const uint64_t key0 = opts->seed, key1 = 0;
   const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
-  const int64_t ntiles = mp / 64;
+  const int64_t ntiles = (mp + 127) / 128;  // row tiles of the update's norm partials
   PhaseTimer pt(h);
 
   HIPCHK(hipMemsetAsync(ws.R, 0, sizeof(double) * ldr * n, st));
@@ -393,8 +398,9 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
       HIPCHK(rbrp_launch_reduce_splits(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, ws.P, i,
                                        i * (int64_t)b, ns, mat(ws.Cbuf, ldr), GEMM_SET));
       HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_s, 1), b, dim_static(i), mat(ws.Aw, mp),
-                                 mat(ws.Cbuf, ldr), mat_cols(ws.Aw, mp, ws.J), GEMM_SUB, nullptr, 0));
+      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i),
+                                 mat(ws.Aw, mp), mat(ws.Cbuf, ldr), mat_cols(ws.Aw, mp, ws.J),
+                                 GEMM_SUB, nullptr, 0));
     }
     pt.end(ph_cgs, i > 0 ? 4 : 0);
 
@@ -412,16 +418,20 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
                                         ws.Jk, ws.Jin, ws.Jout));
       // Q1 = V(:, Jk) Rb^{-1}
-      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
+      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
                                  mat_cols(ws.Aw, mp, ws.Jk), mat(ws.Rinv, b), mat(ws.Qbuf, mp),
                                  GEMM_SET, nullptr, 0));
       HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, mp, ns, kc,
                                  mat(ws.Qbuf, mp), mat(ws.Qbuf, mp), ws.P, b, (int64_t)b * b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G2, b), GEMM_SET));
-      HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.Rb, ws.R2inv, ws.R, ldr, i));
+      HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.R2, ws.R2inv));
+      // R(i:i+bt, i:i+bt) = R2 Rb (product of two upper-triangular factors)
+      HIPCHK(rbrp_launch_gemm_nn(st, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1), b,
+                                 dim_dyn(0, d_bt, 1), mat(ws.R2, b), mat(ws.Rb, b),
+                                 mat_off(ws.R + i, ldr, i), GEMM_SET, nullptr, 0));
     }
-    pt.end(ph, 7);
+    pt.end(ph, 8);
 
     // ---- symmetric-difference permutation -------------------------------
     ph = pt.begin(RBRP_PHASE_PERMUTE);
@@ -432,8 +442,9 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     }
     HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
     // Q_b = Q1 R2^{-1} into the block's columns of the working matrix
-    HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1), mat(ws.Qbuf, mp),
-                               mat(ws.R2inv, b), mat_off(ws.Aw, mp, i), GEMM_SET, nullptr, 0));
+    HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
+                               mat(ws.Qbuf, mp), mat(ws.R2inv, b), mat_off(ws.Aw, mp, i), GEMM_SET,
+                               nullptr, 0));
     pt.end(ph, i > 0 ? 5 : 3);
 
     // ---- trailing projection and update ----------------------------------
@@ -451,12 +462,12 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                        ws.P, b, (int64_t)b * nrem_max, ns,
-                                       mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET));
+                                       mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
       ph_upd = pt.begin(RBRP_PHASE_UPDATE);
-      HIPCHK(rbrp_launch_gemm_nn(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
-                                 mat_off(ws.Aw, mp, i), mat_off(ws.R + i, ldr, i, d_bt),
-                                 mat_off(ws.Aw, mp, i, d_bt), GEMM_SUB, ws.normpart, nrem_max));
+      HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
+                                    b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
+                                    ws.normpart, nrem_max));
       pt.end(ph_upd, 1);
       ph = pt.begin(RBRP_PHASE_RESID);
       HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, ntiles, i, n, ws.st, ws.diags));
@@ -513,7 +524,7 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     for (int64_t bi = nbk - 1; bi >= 0; --bi) {
       const int64_t r0 = bi * 64, r1 = std::min<int64_t>(k, r0 + 64);
       if (r1 < k) {
-        HIPCHK(rbrp_launch_gemm_nn(st, r1 - r0, dim_static(nc), nc, dim_static(k - r1),
+        HIPCHK(rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc, dim_static(k - r1),
                                    mat_off(ws.R + r0, ldr, r1), mat(res->X + r1, res->ldx),
                                    mat(res->X + r0, res->ldx), GEMM_SUB, nullptr, 0));
         ++launches;

@@ -59,17 +59,30 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
                                int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B,
                                double* P, int64_t ldp, int64_t pstride);
 
-// dst(r, c) (=|+=) sum_z P[z](r, c) in split order.
+// dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order; when dst2 != nullptr
+// the sum is also stored row-major at dst2[r * ld2 + c].
 hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
-                                     MatRef dst, int mode);
+                                     MatRef dst, int mode, double* dst2 = nullptr, int64_t ld2 = 0);
 
-// NN product with update: C(r, c) = A(r,:) B(:,c)  (GEMM_SET) or C -= A B (GEMM_SUB),
-// rows r < Mr, columns c < N, inner dimension K.  If normpart != nullptr, writes
+// NN product with update: C(r, c) = A(r,:) B(:,c) (GEMM_SET), C -= A B (GEMM_SUB) or
+// C += A B (GEMM_ADD), rows r < M, columns c < N, inner dimension K (any size),
+// grid sized by Mmax x Nmax.  If normpart != nullptr, writes
 // normpart[tile_row * ldnp + c] = sum over the 64-row tile of C_new(r, c)^2.
-hipError_t rbrp_launch_gemm_nn(hipStream_t st, int64_t Mr, DimSpec N, int64_t Nmax, DimSpec K,
-                               MatRef A, MatRef B, MatRef C, int mode, double* normpart,
+hipError_t rbrp_launch_gemm_nn(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
+                               DimSpec K, MatRef A, MatRef B, MatRef C, int mode, double* normpart,
                                int64_t ldnp);
+
+// Trailing update C(r, c) -= sum_k A(r, k) Bt[k * ldbt + c] for r < M, c < N,
+// K <= Kmax <= 128.  A and C are column-major with leading dimensions lda /
+// ldc whose first column is acoff + *acoff_dyn / ccoff + *ccoff_dyn (the
+// dynamic parts optional); Bt is row-major.  Optional squared column norms of
+// the new C per 128-row tile at normpart[tile * ldnp + c].
+hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
+                                  int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
+                                  const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
+                                  int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
+                                  double* normpart, int64_t ldnp);
 
 // ---- k_rbrp.hip -----------------------------------------------------------
 hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
@@ -88,8 +101,7 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout);
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
-                             RbrpBlockState* bs, const double* Rb, double* R2inv, double* Rbig,
-                             int64_t ldr, int64_t i);
+                             RbrpBlockState* bs, double* R2, double* R2inv);
 hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
                                   const int* Jin, const int* Jout, int* perm, double* diags);
 hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,

@@ -45,8 +45,20 @@ int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const doubl
   if (K < 0) return -4;
   if (mode < 0 || mode > 2) return -11;
   hipStream_t st = (hipStream_t)stream;
-  HIPCHK(rbrp_launch_gemm_nn(st, M, dim_static(N), N, dim_static(K), mat((double*)A, lda),
+  HIPCHK(rbrp_launch_gemm_nn(st, dim_static(M), M, dim_static(N), N, dim_static(K), mat((double*)A, lda),
                              mat((double*)B, ldb), mat(C, ldc), mode, normpart, N));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                       const double* Bt, int64_t ldbt, double* C, int64_t ldc, double* normpart) {
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K > 128) return -4;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(rbrp_launch_gemm_trail(st, M, dim_static(N), N, dim_static(K), K, A, lda, 0, nullptr, Bt,
+                                ldbt, C, ldc, 0, nullptr, normpart, N));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
