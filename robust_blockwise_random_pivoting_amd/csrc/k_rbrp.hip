@@ -291,22 +291,66 @@ hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_d
 // in S[c][l] (the strictly-lower triangle, free once the factorisation is
 // done) and the diagonal in dinv; the result is written to Rinv (ld ldi) with
 // zeros below the diagonal.
-__device__ inline void upper_inverse_lds(double* S, int ld, int bt, double* dinv,
+__device__ inline void upper_inverse_lds(double* S, int ld, int bt, double* dinv, double* red,
                                          double* __restrict__ Rinv, int64_t ldi) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int c = tid; c < bt; c += nt) dinv[c] = 1.0 / S[c * ld + c];
+  // red: workspace of (blockDim.x / 64) x 128 doubles.  Row r of the inverse is
+  // computed by all waves: wave w sums the terms l = r+1+w, r+1+w+nw, ...,
+  // then wave 0 adds the nw partials in wave order.
+  const int tid = threadIdx.x, NT = blockDim.x, nw = NT >> 6, w = tid >> 6, lane = tid & 63;
+  for (int c = tid; c < bt; c += NT) dinv[c] = 1.0 / S[c * ld + c];
   __syncthreads();
   for (int r = bt - 2; r >= 0; --r) {
-    for (int c = r + 1 + tid; c < bt; c += nt) {
-      double acc = S[r * ld + c] * dinv[c];
-      for (int l = r + 1; l < c; ++l) acc += S[r * ld + l] * S[c * ld + l];
-      S[c * ld + r] = -acc * dinv[r];
+    for (int cc = 0; cc * 64 < bt; ++cc) {
+      const int c = lane + 64 * cc;
+      double acc = 0.0;
+      if (c > r && c < bt) {
+#pragma unroll 4
+        for (int l = r + 1 + w; l <= c; l += nw)
+          acc += S[r * ld + l] * ((l == c) ? dinv[c] : S[c * ld + l]);
+      }
+      red[w * 128 + c] = acc;
+    }
+    __syncthreads();
+    if (w == 0) {
+      for (int cc = 0; cc * 64 < bt; ++cc) {
+        const int c = lane + 64 * cc;
+        if (c > r && c < bt) {
+          double a = 0.0;
+          for (int q = 0; q < nw; ++q) a += red[q * 128 + c];
+          S[c * ld + r] = -a * dinv[r];
+        }
+      }
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < bt * bt; idx += nt) {
+  for (int idx = tid; idx < bt * bt; idx += NT) {
     const int r = idx % bt, c = idx / bt;
     Rinv[r + (int64_t)c * ldi] = (r < c) ? S[c * ld + r] : ((r == c) ? dinv[c] : 0.0);
+  }
+}
+
+// Rank-1 update of the trailing part of the symmetric LDS matrix S after step
+// j: S[r][c] -= S[j][r] * S[j][c] for j < r, c < s.  Lane = column (two
+// passes of 64 columns when s > 64), the waves take interleaved rows, four
+// rows per batch of loads.
+__device__ inline void trailing_rank1_lds(double* S, int ld, int j, int s) {
+  const int tid = threadIdx.x, nw = blockDim.x >> 6, w = tid >> 6, lane = tid & 63;
+  for (int cc = 0; cc * 64 < s; ++cc) {
+    const int c = lane + 64 * cc;
+    if (c <= j || c >= s) continue;
+    const double rc = S[j * ld + c];
+    int r = j + 1 + w;
+    for (; r + 3 * nw < s; r += 4 * nw) {
+      const double a0 = S[j * ld + r], a1 = S[j * ld + r + nw];
+      const double a2 = S[j * ld + r + 2 * nw], a3 = S[j * ld + r + 3 * nw];
+      const double b0 = S[r * ld + c], b1 = S[(r + nw) * ld + c];
+      const double b2 = S[(r + 2 * nw) * ld + c], b3 = S[(r + 3 * nw) * ld + c];
+      S[r * ld + c] = b0 - a0 * rc;
+      S[(r + nw) * ld + c] = b1 - a1 * rc;
+      S[(r + 2 * nw) * ld + c] = b2 - a2 * rc;
+      S[(r + 3 * nw) * ld + c] = b3 - a3 * rc;
+    }
+    for (; r < s; r += nw) S[r * ld + c] -= S[j * ld + r] * rc;
   }
 }
 
@@ -350,9 +394,10 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   __shared__ int flag[SAMPLE_MAXB];
   __shared__ double sqr[SAMPLE_MAXB];
   __shared__ double dinv[SAMPLE_MAXB];
+  __shared__ double red[4 * 128];
   __shared__ int s_pvt, s_stop, s_bt;
   __shared__ double s_rjj;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, NT = blockDim.x;
   const int s = bs->s;
   if (s <= 0) {
     if (tid == 0) {
@@ -362,11 +407,11 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
     return;
   }
   const int ld = s + 1;
-  for (int idx = tid; idx < s * s; idx += 256) {
+  for (int idx = tid; idx < s * s; idx += NT) {
     const int r = idx % s, c = idx / s;
     S[r * ld + c] = G[r + (int64_t)c * ldg];
   }
-  for (int t = tid; t < s; t += 256) piv[t] = t;
+  for (int t = tid; t < s; t += NT) piv[t] = t;
   __syncthreads();
   int rank = s;
   for (int j = 0; j < s; ++j) {
@@ -387,13 +432,13 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
     }
     const int p = s_pvt;
     if (p != j) {
-      for (int q = tid; q < s; q += 256) {
+      for (int q = tid; q < s; q += NT) {
         const double t0 = S[q * ld + j];
         S[q * ld + j] = S[q * ld + p];
         S[q * ld + p] = t0;
       }
       __syncthreads();
-      for (int c = j + tid; c < s; c += 256) {
+      for (int c = j + tid; c < s; c += NT) {
         const double t0 = S[j * ld + c];
         S[j * ld + c] = S[p * ld + c];
         S[p * ld + c] = t0;
@@ -406,22 +451,18 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
       __syncthreads();
     }
     const double rjj = s_rjj;
-    for (int c = j + tid; c < s; c += 256) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
+    for (int c = j + tid; c < s; c += NT) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
     __syncthreads();
-    const int nt = s - j - 1;
-    for (int idx = tid; idx < nt * nt; idx += 256) {
-      const int c1 = j + 1 + idx / nt, c2 = j + 1 + idx % nt;
-      S[c1 * ld + c2] -= S[j * ld + c1] * S[j * ld + c2];
-    }
+    trailing_rank1_lds(S, ld, j, s);
     __syncthreads();
   }
   // rows at and beyond the numerical rank are zero
-  for (int idx = tid; idx < s * s; idx += 256) {
+  for (int idx = tid; idx < s * s; idx += NT) {
     const int r = idx / s, c = idx % s;
     if (r >= rank && c >= r) S[r * ld + c] = 0.0;
   }
   __syncthreads();
-  for (int r = tid; r < s; r += 256) {
+  for (int r = tid; r < s; r += NT) {
     double acc = 0.0;
     for (int c = r; c < s; ++c) acc += S[r * ld + c] * S[r * ld + c];
     sqr[r] = acc;
@@ -448,25 +489,25 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   }
   __syncthreads();
   const int bt = s_bt;
-  for (int t = tid; t < bt; t += 256) {
+  for (int t = tid; t < bt; t += NT) {
     Jk[t] = J[piv[t]];
     sj[t] = J[piv[t]];
   }
-  for (int idx = tid; idx < bt * bt; idx += 256) {
+  for (int idx = tid; idx < bt * bt; idx += NT) {
     const int r = idx % bt, c = idx / bt;
     Rb[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
   }
   __syncthreads();
-  upper_inverse_lds(S, ld, bt, dinv, Rinv, b);
+  upper_inverse_lds(S, ld, bt, dinv, red, Rinv, b);
   // ascending sort of Jk by rank counting (entries are distinct)
-  for (int t = tid; t < bt; t += 256) {
+  for (int t = tid; t < bt; t += NT) {
     const int v = sj[t];
     int rk = 0;
     for (int q = 0; q < bt; ++q) rk += (sj[q] < v) ? 1 : 0;
     sorted[rk] = v;
   }
   // Jout membership flags for the block positions i .. i+bt-1
-  for (int t = tid; t < bt; t += 256) {
+  for (int t = tid; t < bt; t += NT) {
     const int q = (int)(i + t);
     int member = 0;
     for (int a = 0; a < bt; ++a) member |= (sj[a] == q) ? 1 : 0;
@@ -475,8 +516,8 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   __syncthreads();
   int nin = 0;
   for (int a = 0; a < bt; ++a) nin += ((int64_t)sorted[a] >= i + bt) ? 1 : 0;
-  for (int t = tid; t < nin; t += 256) Jin[t] = sorted[bt - nin + t];
-  for (int t = tid; t < bt; t += 256) {
+  for (int t = tid; t < nin; t += NT) Jin[t] = sorted[bt - nin + t];
+  for (int t = tid; t < bt; t += NT) {
     if (flag[t]) {
       int pos = 0;
       for (int q = 0; q < t; ++q) pos += flag[q];
@@ -516,12 +557,13 @@ __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, in
                                                double* __restrict__ R2inv) {
   extern __shared__ double S[];
   __shared__ double dinv[SAMPLE_MAXB];
+  __shared__ double red[4 * 128];
   __shared__ int s_bad;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, NT = blockDim.x;
   const int bt = bs->bt;
   if (bt <= 0) return;
   const int ld = bt + 1;
-  for (int idx = tid; idx < bt * bt; idx += 256) {
+  for (int idx = tid; idx < bt * bt; idx += NT) {
     const int r = idx % bt, c = idx / bt;
     S[r * ld + c] = G2[r + (int64_t)c * ldg];
   }
@@ -532,22 +574,18 @@ __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, in
     const double rjj = sqrt(d > 0.0 ? d : 1.0);
     __syncthreads();
     if (tid == 0 && !(d > 0.0)) s_bad = 1;
-    for (int c = j + tid; c < bt; c += 256) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
+    for (int c = j + tid; c < bt; c += NT) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
     __syncthreads();
-    const int nt = bt - j - 1;
-    for (int idx = tid; idx < nt * nt; idx += 256) {
-      const int c1 = j + 1 + idx / nt, c2 = j + 1 + idx % nt;
-      S[c1 * ld + c2] -= S[j * ld + c1] * S[j * ld + c2];
-    }
+    trailing_rank1_lds(S, ld, j, bt);
     __syncthreads();
   }
   if (tid == 0 && s_bad) bs->status = 2;
-  for (int idx = tid; idx < bt * bt; idx += 256) {
+  for (int idx = tid; idx < bt * bt; idx += NT) {
     const int r = idx % bt, c = idx / bt;
     R2[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
   }
   __syncthreads();
-  upper_inverse_lds(S, ld, bt, dinv, R2inv, b);
+  upper_inverse_lds(S, ld, bt, dinv, red, R2inv, b);
 }
 
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
@@ -742,29 +780,35 @@ hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, co
 
 // ---------------------------------------------------------------------------
 // Diagonal-block back substitution of X = R11^{-1} R12: X(0:nb, c) =
-// Rd^{-1} X(0:nb, c) for every column c; Rd is nb x nb upper (nb <= 64).
-// One wave per 64 columns, Rd and the right-hand sides resident in LDS.
+// Rd^{-1} X(0:nb, c) for every column c; Rd is nb x nb upper (nb <= 64),
+// padded to a 64 x 64 unit triangle.  One thread per column, its 64 entries in
+// registers (fully unrolled substitution); Rd in LDS, read as broadcasts.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_trsm_diag(int nb, int64_t ncols, const double* __restrict__ Rd,
                                                   int64_t ldr, double* __restrict__ X, int64_t ldx) {
   __shared__ double Rs[64 * 65];
-  __shared__ double xs[64 * 65];
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += 64) {
-    const int r = idx % nb, c = idx / nb;
-    Rs[r * 65 + c] = Rd[r + (int64_t)c * ldr];
+  for (int idx = tid; idx < 64 * 64; idx += 64) {
+    const int r = idx & 63, c = idx >> 6;
+    Rs[r * 65 + c] = (r < nb && c < nb) ? Rd[r + (int64_t)c * ldr] : (r == c ? 1.0 : 0.0);
   }
-  const int64_t c = (int64_t)blockIdx.x * 64 + tid;
   __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 64 + tid;
   if (c >= ncols) return;
   double* xc = X + c * ldx;
-  for (int r = 0; r < nb; ++r) xs[r * 65 + tid] = xc[r];
-  for (int r = nb - 1; r >= 0; --r) {
-    double v = xs[r * 65 + tid];
-    for (int l = r + 1; l < nb; ++l) v -= Rs[r * 65 + l] * xs[l * 65 + tid];
-    xs[r * 65 + tid] = v / Rs[r * 65 + r];
+  double x[64];
+#pragma unroll
+  for (int r = 0; r < 64; ++r) x[r] = (r < nb) ? xc[r] : 0.0;
+#pragma unroll
+  for (int r = 63; r >= 0; --r) {
+    double v = x[r];
+#pragma unroll
+    for (int l = r + 1; l < 64; ++l) v -= Rs[r * 65 + l] * x[l];
+    x[r] = v / Rs[r * 65 + r];
   }
-  for (int r = 0; r < nb; ++r) xc[r] = xs[r * 65 + tid];
+#pragma unroll
+  for (int r = 0; r < 64; ++r)
+    if (r < nb) xc[r] = x[r];
 }
 
 hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,

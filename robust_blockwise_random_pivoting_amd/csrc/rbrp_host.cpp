@@ -144,7 +144,7 @@ void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pc
                   int64_t* kchunk) {
   const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + 63) / 64);
   int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
-  ns = std::min<int64_t>(ns, 128);
+  ns = std::min<int64_t>(ns, tiles <= 4 ? 256 : 128);
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, K / 64));
   int64_t kc = round_up((K + ns - 1) / ns, 16);
@@ -519,18 +519,30 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     ph = pt.begin(RBRP_PHASE_SOLVE);
     const int64_t nc = n - k;
     HIPCHK(rbrp_launch_copy_block(st, k, nc, ws.R + k * ldr, ldr, res->X, res->ldx));
-    const int64_t nbk = (k + 63) / 64;
+    // backward blocked solve: 256-row super-blocks solved with 64-row diagonal
+    // blocks, then one large update of every row above the super-block
     int launches = 1;
-    for (int64_t bi = nbk - 1; bi >= 0; --bi) {
-      const int64_t r0 = bi * 64, r1 = std::min<int64_t>(k, r0 + 64);
-      if (r1 < k) {
-        HIPCHK(rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc, dim_static(k - r1),
-                                   mat_off(ws.R + r0, ldr, r1), mat(res->X + r1, res->ldx),
-                                   mat(res->X + r0, res->ldx), GEMM_SUB, nullptr, 0));
+    for (int64_t s1 = k; s1 > 0; s1 -= 256) {
+      const int64_t s0 = std::max<int64_t>(0, s1 - 256);
+      for (int64_t r1 = s1; r1 > s0; r1 -= 64) {
+        const int64_t r0 = std::max<int64_t>(s0, r1 - 64);
+        if (r1 < s1) {
+          HIPCHK(rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
+                                     dim_static(s1 - r1), mat_off(ws.R + r0, ldr, r1),
+                                     mat(res->X + r1, res->ldx), mat(res->X + r0, res->ldx),
+                                     GEMM_SUB, nullptr, 0));
+          ++launches;
+        }
+        HIPCHK(rbrp_launch_trsm_diag(st, r1 - r0, nc, ws.R + r0 + r0 * ldr, ldr, res->X + r0,
+                                     res->ldx));
         ++launches;
       }
-      HIPCHK(rbrp_launch_trsm_diag(st, r1 - r0, nc, ws.R + r0 + r0 * ldr, ldr, res->X + r0, res->ldx));
-      ++launches;
+      if (s0 > 0) {
+        HIPCHK(rbrp_launch_gemm_nn(st, dim_static(s0), s0, dim_static(nc), nc, dim_static(s1 - s0),
+                                   mat_off(ws.R, ldr, s0), mat(res->X + s0, res->ldx),
+                                   mat(res->X, res->ldx), GEMM_SUB, nullptr, 0));
+        ++launches;
+      }
     }
     pt.end(ph, launches, (double)k * k * nc, 8.0 * (0.5 * k * k + 2.0 * k * nc));
   }
