@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--matrix-seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-comparators", action="store_true",
+                    help="skip the RandCPQR_OS / RandLUPP_OS comparison runs")
     ap.add_argument("--profile-steps", action="store_true",
                     help="print a line per step (for profiler runs)")
     return ap.parse_args()
@@ -145,6 +147,26 @@ def main():
                 "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
                 "traffic": None, "avg_ms": d["avg_ms"], "launches": d["launches"]}
 
+    # comparison baselines at the rank RBRP found: Gaussian
+    # sketch of 2k rows, oversampling 2, interpolation as gpu/interp.cu
+    comps = {}
+    if not args.no_comparators:
+        from robust_blockwise_random_pivoting_amd import RandCPQR_OS, RandLUPP_OS
+        for name, fn in (("randcpqr_os", RandCPQR_OS), ("randlupp_os", RandLUPP_OS)):
+            fn(A, res.k, seed=args.seed, handle=h)            # warm-up (workspace growth)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            sk, rd, T = fn(A, res.k, seed=args.seed, handle=h)
+            torch.cuda.synchronize()
+            dtc = time.perf_counter() - t1
+            sk = torch.as_tensor(sk, device="cuda").long()
+            rd = torch.as_tensor(rd, device="cuda").long()
+            E = A[:, rd] - A[:, sk] @ T
+            comps[name] = {"ms": dtc * 1e3, "rank": res.k,
+                           "rel_error": (torch.linalg.norm(E) / torch.linalg.norm(A)).item()}
+            del E
+            print(f"[bench] {name}: {dtc * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -174,6 +196,7 @@ def main():
         "roofline": roof,
         "kernels": kern,
         "phase_ms_per_step": {k: v["ms"] / args.steps for k, v in stats.items()},
+        "comparators": comps,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

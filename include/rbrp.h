@@ -102,6 +102,48 @@ int rbrp_release_workspace(rbrp_handle_t handle);
 int rbrp_colid_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
                  const rbrp_opts_t* opts, rbrp_result_t* result);
 
+/*
+ * Gaussian sketch: G(row, col) = scale * z_t, t = row + col * rows, z from the
+ * Box-Muller pairs of the Philox stream with key (seed, 1) and counter
+ * `offset`.  Replaces Random::Gaussian (gpu/random.cu:44-48).  dG is rows x cols
+ * (device, leading dimension ldg >= rows).
+ */
+int rbrp_gaussian_sketch_d(rbrp_handle_t handle, uint64_t seed, uint64_t offset, int64_t rows,
+                           int64_t cols, double scale, double* dG, int64_t ldg);
+
+/*
+ * Randomised CPQR ID with 2x oversampling.  Replaces `void RandCPQR_OS(const
+ * double *A, int m, int n, int rank, int *h_sk, int *h_rd, double *&T, double&
+ * flops)` (gpu/rid.hpp:19-20, gpu/randCPQR_OS.cu:20-111): Y = G A with G
+ * (2r x m) ~ N(0, 1/m); pivoted QR of Y(1:r, :) gives sk = jpvt[0:r] and
+ * rd = jpvt[r:n] (host arrays, as in the reference); T (r x (n-r), device,
+ * leading dimension ldt, caller-allocated) by the oversampled interpolation.
+ * dG (2r x m, ldg) is an optional caller sketch; NULL draws it from (seed, offset).
+ */
+int rbrp_randcpqr_os_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                       int64_t rank, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                       int32_t* h_sk, int32_t* h_rd, double* dT, int64_t ldt);
+
+/*
+ * Randomised LUPP ID with 2x oversampling.  Replaces `void RandLUPP_OS(const
+ * double *A, int m, int n, int rank, int *&sk, int *&rd, double *&T, double&
+ * flops)` (gpu/rid.hpp:12-13, gpu/randLUPP_OS.cu:14-119): Y = G A (G is 2k x m);
+ * LU with partial pivoting of Y(1:k, :)^T; the row interchanges applied to the
+ * identity give sk = P[0:k], rd = P[k:n] (device arrays here, caller-allocated);
+ * T as for rbrp_randcpqr_os_d.
+ */
+int rbrp_randlupp_os_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                       int64_t rank, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                       int32_t* d_sk, int32_t* d_rd, double* dT, int64_t ldt);
+
+/*
+ * Oversampled interpolation matrix.  Replaces Compute_interpolation_matrix
+ * (gpu/interp.cu:45-105): Yp = Y(:, P); QR of Yp(:, 0:k); T = R^{-1} Q^T Yp(:, k:n)
+ * (k x (n-k), leading dimension ldt).  Y is l x n (ldy); P (device) has n entries.
+ */
+int rbrp_interp_os_d(rbrp_handle_t handle, const double* dY, int64_t l, int64_t n, int64_t ldy,
+                     int64_t k, const int32_t* dP, double* dT, int64_t ldt);
+
 /* Defaults: b = 64, tol = 0, max_rank = 0, seed = 0, offset = 0, FP64, sampled filter. */
 void rbrp_opts_default(rbrp_opts_t* opts);
 

@@ -19,6 +19,15 @@ inline int& last_status() {
   static thread_local int s = 0;
   return s;
 }
+// Householder QR flop count of an m x n matrix (LAPACK Working Note 41
+// operation counts, as tabulated in gpu/flops.hpp:31-40).
+inline double flops_dgeqrf(double m, double n) {
+  const double muls = (m > n) ? n * (n * (0.5 - n / 3.0 + m) + m + 23.0 / 6.0)
+                              : m * (m * (-0.5 - m / 3.0 + n) + 2.0 * n + 23.0 / 6.0);
+  const double adds = (m > n) ? n * (n * (0.5 - n / 3.0 + m) + 5.0 / 6.0)
+                              : m * (m * (-0.5 - m / 3.0 + n) + n + 5.0 / 6.0);
+  return muls + adds;
+}
 }  // namespace rbrp_detail
 
 // Status of the last wrapper call on this thread (0 = success; see rbrp.h).
@@ -133,6 +142,49 @@ inline int RBRP(int m, int n, const double* A, int blk, double tol, std::vector<
   rbrp_destroy(h);
   rbrp_detail::last_status() = st;
   return st;
+}
+
+// gpu/rid.hpp:19-20.  h_sk (rank) and h_rd (n - rank) are host arrays the
+// caller provides, T (rank x (n - rank), leading dimension rank) is allocated
+// here with hipMalloc and released by the caller with hipFree.  flops is the
+// reference's count 2 m n (2r) + FLOPS_DGEQRF(2r, n) + (2r)^2 (n - 2r)
+// (randCPQR_OS.cu:199).  The sketch is drawn from the Philox stream (seed, 0);
+// A is m x n with leading dimension m.
+inline void RandCPQR_OS(const double* A, int m, int n, int rank, int* h_sk, int* h_rd, double*& T,
+                        double& flops, uint64_t seed = 0) {
+  T = nullptr;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (!st && n > rank) st = (int)hipMalloc(&T, sizeof(double) * (size_t)rank * (size_t)(n - rank));
+  if (!st)
+    st = rbrp_randcpqr_os_d(h, A, m, n, m, rank, seed, 0, nullptr, 0, h_sk, h_rd, T, rank);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  const double k = 2.0 * rank, mm = m, nn = n;
+  flops = 2.0 * mm * nn * k + rbrp_detail::flops_dgeqrf(k, nn) + k * k * (nn - k);
+}
+
+// gpu/rid.hpp:12-13.  sk, rd (device int arrays) and T (device, rank x
+// (n - rank), leading dimension rank) are allocated here with hipMalloc; the
+// caller frees them with hipFree.  flops as randLUPP_OS.cu:100.
+inline void RandLUPP_OS(const double* A, int m, int n, int rank, int*& sk, int*& rd, double*& T,
+                        double& flops, uint64_t seed = 0) {
+  sk = rd = nullptr;
+  T = nullptr;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (!st) st = (int)hipMalloc(&sk, sizeof(int) * (size_t)rank);
+  if (!st && n > rank) st = (int)hipMalloc(&rd, sizeof(int) * (size_t)(n - rank));
+  if (!st && n > rank) st = (int)hipMalloc(&T, sizeof(double) * (size_t)rank * (size_t)(n - rank));
+  if (!st)
+    st = rbrp_randlupp_os_d(h, A, m, n, m, rank, seed, 0, nullptr, 0, (int32_t*)sk, (int32_t*)rd, T,
+                            rank);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  const double k = rank, mm = m, nn = n;
+  flops = 2.0 * mm * nn * k + 2.0 * mm * k * k / 3.0 + k * k * (mm - k);
 }
 
 #endif  // RBRP_HPP_

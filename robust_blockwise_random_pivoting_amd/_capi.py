@@ -67,6 +67,12 @@ def lib():
         "rbrp_release_workspace": (c_int, [vp]),
         "rbrp_uniforms_d": (c_int, [vp, u64, u64, u64, i64, vp]),
         "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
+        "rbrp_gaussian_sketch_d": (c_int, [vp, u64, u64, i64, i64, c_double, vp, i64]),
+        "rbrp_randcpqr_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
+                                       i64]),
+        "rbrp_randlupp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
+                                       i64]),
+        "rbrp_interp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, i64]),
         "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
         "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),

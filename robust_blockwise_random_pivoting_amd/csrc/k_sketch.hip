@@ -1,0 +1,312 @@
+// Kernels of the sketch-based ID comparators (RandCPQR_OS, RandLUPP_OS and the
+// oversampled interpolation matrix): Gaussian sketch generation, Householder
+// QR steps with optional column pivoting, LU steps with partial pivoting, and
+// column gathers / transposes.  The GEMM Y = G A runs on k_gemm_nn.
+#include "rbrp_kernels.h"
+
+#define SK_NT 256
+
+// ---------------------------------------------------------------------------
+// Gaussian sketch: out(row, col) = scale * z_t, t = row + col * rows, where
+// (z_{2q}, z_{2q+1}) is the Box-Muller pair of uniforms (u_{2q}, u_{2q+1}) of
+// the Philox stream (key (seed, 1), counter `offset`):
+//   z_{2q} = sqrt(-2 log(1 - u_{2q})) cos(2 pi u_{2q+1}),
+//   z_{2q+1} = sqrt(-2 log(1 - u_{2q})) sin(2 pi u_{2q+1}).
+// Replaces Random::Gaussian (gpu/random.cu:44-48, Thrust normal_distribution
+// on a global minstd stream) with an explicit, reproducible stream.
+// ---------------------------------------------------------------------------
+__global__ void k_gaussian(uint64_t key0, uint64_t c0, int64_t rows, int64_t cols, double scale,
+                           double* __restrict__ out, int64_t ldo) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= rows * cols) return;
+  const uint64_t base[4] = {c0, 0, 0, 0};
+  const uint64_t q = (uint64_t)t >> 1;
+  const double u1 = rbrp_u64_to_unit(rbrp_philox_word(base, key0, 1, 2 * q));
+  const double u2 = rbrp_u64_to_unit(rbrp_philox_word(base, key0, 1, 2 * q + 1));
+  const double rad = sqrt(-2.0 * log(1.0 - u1));
+  double sn, cs;
+  sincos(2.0 * 3.14159265358979323846 * u2, &sn, &cs);
+  const double z = (t & 1) ? rad * sn : rad * cs;
+  const int64_t row = t % rows, col = t / rows;
+  out[row + col * ldo] = scale * z;
+}
+
+hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,
+                                int64_t cols, double scale, double* out, int64_t ldo) {
+  const int64_t total = rows * cols;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gaussian, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, seed, offset,
+                     rows, cols, scale, out, ldo);
+  return hipGetLastError();
+}
+
+// dst(:, c) = src(:, perm[c]) for c < cols, rows rows.
+__global__ void k_gather_cols(int64_t rows, const double* __restrict__ src, int64_t lds,
+                              const int* __restrict__ perm, double* __restrict__ dst, int64_t ldd) {
+  const int64_t r = (int64_t)blockIdx.y * 256 + threadIdx.x;
+  const int64_t c = blockIdx.x;
+  if (r >= rows) return;
+  dst[r + c * ldd] = src[r + (int64_t)perm[c] * lds];
+}
+
+hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                   int64_t lds, const int* perm, double* dst, int64_t ldd) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_cols, dim3((unsigned)cols, (unsigned)((rows + 255) / 256)), dim3(256), 0,
+                     st, rows, src, lds, perm, dst, ldd);
+  return hipGetLastError();
+}
+
+// dst (cols x rows, ldd) = src(0:rows, 0:cols)^T, 32x32 tiles through LDS.
+__global__ void k_transpose(int64_t rows, int64_t cols, const double* __restrict__ src, int64_t lds,
+                            double* __restrict__ dst, int64_t ldd) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t r = r0 + tx, c = c0 + q;
+    tile[q][tx] = (r < rows && c < cols) ? src[r + c * lds] : 0.0;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int64_t c = c0 + tx, r = r0 + q;  // dst(c, r)
+    if (r < rows && c < cols) dst[c + r * ldd] = tile[tx][q];
+  }
+}
+
+hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                 int64_t lds, double* dst, int64_t ldd) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_transpose, dim3((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32)),
+                     dim3(256), 0, st, rows, cols, src, lds, dst, ldd);
+  return hipGetLastError();
+}
+
+// vn[c] = sum_{i < rows} W(i, c)^2, one workgroup per column.
+__global__ __launch_bounds__(SK_NT) void k_colnorm2(int64_t rows, const double* __restrict__ W,
+                                                    int64_t ld, double* __restrict__ vn) {
+  __shared__ double red[SK_NT / 64];
+  const int64_t c = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < rows; i += SK_NT) acc = fma(W[i + c * ld], W[i + c * ld], acc);
+  const double t = rbrp_block_sum(acc, red);
+  if (threadIdx.x == 0) vn[c] = t;
+}
+
+hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, const double* W,
+                                int64_t ld, double* vn) {
+  if (cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colnorm2, dim3((unsigned)cols), dim3(SK_NT), 0, st, rows, W, ld, vn);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR step j on W (rows x cols): optional column pivoting on the
+// trailing squared column norms vn (first maximum, LAPACK dgeqp3 rule: the
+// pivot column swaps with column j), then the reflector of W(j:rows, j) with
+// LAPACK dlarfg conventions: beta = -sign(alpha) ||x||, tau = (beta - alpha) /
+// beta, v = x(2:) / (alpha - beta), stored below the diagonal, beta on it.
+// One workgroup.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SK_NT) void k_hh_reflect(int64_t rows, int64_t cols, int64_t j,
+                                                      double* __restrict__ W, int64_t ld,
+                                                      double* __restrict__ vn, int* __restrict__ jpvt,
+                                                      double* __restrict__ tau, int pivoting) {
+  __shared__ double rv[SK_NT];
+  __shared__ int ri[SK_NT];
+  __shared__ double red[SK_NT / 64];
+  __shared__ int s_p;
+  const int tid = threadIdx.x;
+  if (pivoting) {
+    double bv = -1.0;
+    int bi = 0x7fffffff;
+    for (int64_t c = j + tid; c < cols; c += SK_NT) {
+      const double d = vn[c];
+      if (d > bv) {  // increasing c per thread: keeps the first maximum
+        bv = d;
+        bi = (int)c;
+      }
+    }
+    rv[tid] = bv;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int o = SK_NT / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+        const double ov = rv[tid + o];
+        const int oi = ri[tid + o];
+        if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
+          rv[tid] = ov;
+          ri[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) s_p = (ri[0] == 0x7fffffff) ? (int)j : ri[0];
+    __syncthreads();
+    const int64_t p = s_p;
+    if (p != j) {
+      for (int64_t i = tid; i < rows; i += SK_NT) {
+        const double t0 = W[i + j * ld];
+        W[i + j * ld] = W[i + p * ld];
+        W[i + p * ld] = t0;
+      }
+      if (tid == 0) {
+        const int t1 = jpvt[j];
+        jpvt[j] = jpvt[p];
+        jpvt[p] = t1;
+        const double t2 = vn[j];
+        vn[j] = vn[p];
+        vn[p] = t2;
+      }
+      __syncthreads();
+    }
+  }
+  double* x = W + j * ld;
+  double acc = 0.0;
+  for (int64_t i = j + 1 + tid; i < rows; i += SK_NT) acc = fma(x[i], x[i], acc);
+  const double xn2 = rbrp_block_sum(acc, red);
+  const double alpha = x[j];
+  __syncthreads();
+  if (xn2 == 0.0) {
+    if (tid == 0) tau[j] = 0.0;
+    return;
+  }
+  const double nrm = sqrt(alpha * alpha + xn2);
+  const double beta = (alpha >= 0.0) ? -nrm : nrm;
+  const double scal = 1.0 / (alpha - beta);
+  for (int64_t i = j + 1 + tid; i < rows; i += SK_NT) x[i] *= scal;
+  if (tid == 0) {
+    tau[j] = (beta - alpha) / beta;
+    x[j] = beta;
+  }
+}
+
+hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
+                                  int64_t ld, double* vn, int* jpvt, double* tau, int pivoting) {
+  hipLaunchKernelGGL(k_hh_reflect, dim3(1), dim3(SK_NT), 0, st, rows, cols, j, W, ld, vn, jpvt, tau,
+                     pivoting);
+  return hipGetLastError();
+}
+
+// Apply H_j = I - tau v v^T (v(j) = 1) to columns c in (j, cols) of W, one
+// workgroup per column; optionally recompute vn[c] = ||W(j+1:rows, c)||^2
+// from the updated values (exact norms instead of downdating).
+__global__ __launch_bounds__(SK_NT) void k_hh_apply(int64_t rows, int64_t cols, int64_t j,
+                                                    double* __restrict__ W, int64_t ld,
+                                                    const double* __restrict__ tau,
+                                                    double* __restrict__ vn, int norms) {
+  __shared__ double red[SK_NT / 64];
+  const int64_t c = j + 1 + blockIdx.x;
+  if (c >= cols) return;
+  const double tj = tau[j];
+  const double* v = W + j * ld;
+  double* w = W + c * ld;
+  double acc = 0.0;
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) acc = fma(v[i], w[i], acc);
+  const double dot = rbrp_block_sum(acc, red) + w[j];
+  const double f = tj * dot;
+  double nacc = 0.0;
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) {
+    const double nv = w[i] - f * v[i];
+    w[i] = nv;
+    nacc = fma(nv, nv, nacc);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) w[j] -= f;
+  if (norms) {
+    const double t = rbrp_block_sum(nacc, red);
+    if (threadIdx.x == 0) vn[c] = t;
+  }
+}
+
+hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
+                                int64_t ld, const double* tau, double* vn, int norms) {
+  if (cols - j - 1 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hh_apply, dim3((unsigned)(cols - j - 1)), dim3(SK_NT), 0, st, rows, cols, j, W,
+                     ld, tau, vn, norms);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// LU with partial pivoting, step j of Y (nrows x ncols): pivot = first index of
+// max |Y(i, j)| over i >= j (LAPACK idamax), swap rows j and pivot across all
+// columns, ipiv[j] = pivot (0-based), scale Y(j+1:, j) by 1 / Y(j, j) (skipped
+// for an exact zero pivot, as dgetf2 does).  One workgroup.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SK_NT) void k_lu_pivot(int64_t nrows, int64_t ncols, int64_t j,
+                                                    double* __restrict__ Y, int64_t ld,
+                                                    int* __restrict__ ipiv) {
+  __shared__ double rv[SK_NT];
+  __shared__ int ri[SK_NT];
+  __shared__ int s_p;
+  const int tid = threadIdx.x;
+  double bv = -1.0;
+  int bi = 0x7fffffff;
+  for (int64_t i = j + tid; i < nrows; i += SK_NT) {
+    const double a = fabs(Y[i + j * ld]);
+    if (a > bv) {
+      bv = a;
+      bi = (int)i;
+    }
+  }
+  rv[tid] = bv;
+  ri[tid] = bi;
+  __syncthreads();
+  for (int o = SK_NT / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      const double ov = rv[tid + o];
+      const int oi = ri[tid + o];
+      if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
+        rv[tid] = ov;
+        ri[tid] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    s_p = (ri[0] == 0x7fffffff) ? (int)j : ri[0];
+    ipiv[j] = s_p;
+  }
+  __syncthreads();
+  const int64_t p = s_p;
+  if (p != j) {
+    for (int64_t c = tid; c < ncols; c += SK_NT) {
+      const double t0 = Y[j + c * ld];
+      Y[j + c * ld] = Y[p + c * ld];
+      Y[p + c * ld] = t0;
+    }
+    __syncthreads();
+  }
+  const double piv = Y[j + j * ld];
+  if (piv != 0.0) {
+    const double inv = 1.0 / piv;
+    for (int64_t i = j + 1 + tid; i < nrows; i += SK_NT) Y[i + j * ld] *= inv;
+  }
+}
+
+hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
+                                int64_t ld, int* ipiv) {
+  hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(SK_NT), 0, st, nrows, ncols, j, Y, ld, ipiv);
+  return hipGetLastError();
+}
+
+// Rank-1 update Y(j+1:, c) -= Y(j+1:, j) * Y(j, c) for c in (j, ncols); one
+// workgroup per column.
+__global__ __launch_bounds__(SK_NT) void k_lu_update(int64_t nrows, int64_t ncols, int64_t j,
+                                                     double* __restrict__ Y, int64_t ld) {
+  const int64_t c = j + 1 + blockIdx.x;
+  if (c >= ncols) return;
+  const double u = Y[j + c * ld];
+  if (u == 0.0) return;
+  const double* l = Y + j * ld;
+  double* y = Y + c * ld;
+  for (int64_t i = j + 1 + threadIdx.x; i < nrows; i += SK_NT) y[i] = fma(-l[i], u, y[i]);
+}
+
+hipError_t rbrp_launch_lu_update(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
+                                 int64_t ld) {
+  if (ncols - j - 1 <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lu_update, dim3((unsigned)(ncols - j - 1)), dim3(SK_NT), 0, st, nrows, ncols, j,
+                     Y, ld);
+  return hipGetLastError();
+}

@@ -1,0 +1,39 @@
+// Internal definition of the library handle (one stream + cached workspaces).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/rbrp.h"
+#include "rbrp_common.h"
+
+struct RbrpWorkspace {
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  // carved pointers of the RBRP loop
+  double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
+      *tmpk, *tmpo, *Wt;
+  int *J, *Jk, *Jin, *Jout;
+  RbrpBlockState* st;
+  int64_t Pcap;
+};
+
+struct rbrp_handle_s {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  RbrpWorkspace ws;
+  void* sk_arena = nullptr;  // workspace of the sketch comparators
+  size_t sk_bytes = 0;
+  RbrpBlockState* host_state = nullptr;  // pinned
+  bool timing = false;
+  rbrp_phase_stats_t stats;
+  std::vector<hipEvent_t> events;
+};
+
+// Grow-only device buffer of the handle for the sketch comparators.
+hipError_t rbrp_sketch_arena(rbrp_handle_t h, size_t bytes, void** out);
+
+// X (k x nc, ldx) <- R^{-1} X, R upper triangular (k x k, ldr); blocked, on st.
+hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R, int64_t ldr,
+                                double* X, int64_t ldx, int* launches);

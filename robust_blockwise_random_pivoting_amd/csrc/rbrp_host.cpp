@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/rbrp.h"
+#include "rbrp_handle.h"
 #include "rbrp_kernels.h"
 
 #define RBRP_MAX_B 128
@@ -33,17 +34,6 @@ namespace {
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
-struct Workspace {
-  void* arena = nullptr;
-  size_t arena_bytes = 0;
-  // carved pointers
-  double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
-      *tmpk, *tmpo, *Wt;
-  int *J, *Jk, *Jin, *Jout;
-  RbrpBlockState* st;
-  int64_t Pcap;
-};
-
 struct Carver {
   size_t off = 0;
   template <typename T>
@@ -55,16 +45,6 @@ struct Carver {
 };
 
 }  // namespace
-
-struct rbrp_handle_s {
-  hipStream_t stream = nullptr;
-  int device = 0;
-  Workspace ws;
-  RbrpBlockState* host_state = nullptr;  // pinned
-  bool timing = false;
-  rbrp_phase_stats_t stats;
-  std::vector<hipEvent_t> events;
-};
 
 namespace {
 
@@ -96,7 +76,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   size_t oJout = c.take<int>(b);
   size_t ost = c.take<RbrpBlockState>(1);
   const size_t need = c.off;
-  Workspace& ws = h->ws;
+  RbrpWorkspace& ws = h->ws;
   if (ws.arena_bytes < need) {
     if (ws.arena) {
       (void)hipFree(ws.arena);
@@ -218,6 +198,59 @@ double now_s() {
 
 }  // namespace
 
+// X (k x nc, ldx) <- R^{-1} X for the k x k upper-triangular R (ldr): backward
+// blocked solve with 256-row super-blocks, each solved with 64-row diagonal
+// blocks (k_trsm_diag) and small updates, followed by one large update of
+// every row above the super-block (k_gemm_nn).
+hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R, int64_t ldr,
+                                double* X, int64_t ldx, int* launches) {
+  hipError_t e;
+  for (int64_t s1 = k; s1 > 0; s1 -= 256) {
+    const int64_t s0 = std::max<int64_t>(0, s1 - 256);
+    for (int64_t r1 = s1; r1 > s0; r1 -= 64) {
+      const int64_t r0 = std::max<int64_t>(s0, r1 - 64);
+      if (r1 < s1) {
+        e = rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
+                                dim_static(s1 - r1), mat_off(R + r0, ldr, r1), mat(X + r1, ldx),
+                                mat(X + r0, ldx), GEMM_SUB, nullptr, 0);
+        if (e != hipSuccess) return e;
+        if (launches) ++*launches;
+      }
+      e = rbrp_launch_trsm_diag(st, r1 - r0, nc, R + r0 + r0 * ldr, ldr, X + r0, ldx);
+      if (e != hipSuccess) return e;
+      if (launches) ++*launches;
+    }
+    if (s0 > 0) {
+      e = rbrp_launch_gemm_nn(st, dim_static(s0), s0, dim_static(nc), nc, dim_static(s1 - s0),
+                              mat_off(R, ldr, s0), mat(X + s0, ldx), mat(X, ldx), GEMM_SUB, nullptr,
+                              0);
+      if (e != hipSuccess) return e;
+      if (launches) ++*launches;
+    }
+  }
+  return hipSuccess;
+}
+
+hipError_t rbrp_sketch_arena(rbrp_handle_t h, size_t bytes, void** out) {
+  if (h->sk_bytes < bytes) {
+    if (h->sk_arena) {
+      hipError_t e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) return e;
+      (void)hipFree(h->sk_arena);
+      h->sk_arena = nullptr;
+      h->sk_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->sk_arena, bytes);
+    if (e != hipSuccess) {
+      h->sk_arena = nullptr;
+      return e;
+    }
+    h->sk_bytes = bytes;
+  }
+  *out = h->sk_arena;
+  return hipSuccess;
+}
+
 extern "C" {
 
 const char* rbrp_version(void) { return "rbrp-mi355x 0.1 (gfx950, FP64 MFMA)"; }
@@ -260,6 +293,7 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
 int rbrp_destroy(rbrp_handle_t h) {
   if (!h) return -1;
   if (h->ws.arena) (void)hipFree(h->ws.arena);
+  if (h->sk_arena) (void)hipFree(h->sk_arena);
   if (h->host_state) (void)hipHostFree(h->host_state);
   for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
   delete h;
@@ -358,7 +392,7 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1));
     if (e) return e;
   }
-  Workspace& ws = h->ws;
+  RbrpWorkspace& ws = h->ws;
   int* perm = res->perm;
   const double tol2 = opts->tol * opts->tol;
   const uint64_t key0 = opts->seed, key1 = 0;
@@ -519,31 +553,8 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     ph = pt.begin(RBRP_PHASE_SOLVE);
     const int64_t nc = n - k;
     HIPCHK(rbrp_launch_copy_block(st, k, nc, ws.R + k * ldr, ldr, res->X, res->ldx));
-    // backward blocked solve: 256-row super-blocks solved with 64-row diagonal
-    // blocks, then one large update of every row above the super-block
     int launches = 1;
-    for (int64_t s1 = k; s1 > 0; s1 -= 256) {
-      const int64_t s0 = std::max<int64_t>(0, s1 - 256);
-      for (int64_t r1 = s1; r1 > s0; r1 -= 64) {
-        const int64_t r0 = std::max<int64_t>(s0, r1 - 64);
-        if (r1 < s1) {
-          HIPCHK(rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
-                                     dim_static(s1 - r1), mat_off(ws.R + r0, ldr, r1),
-                                     mat(res->X + r1, res->ldx), mat(res->X + r0, res->ldx),
-                                     GEMM_SUB, nullptr, 0));
-          ++launches;
-        }
-        HIPCHK(rbrp_launch_trsm_diag(st, r1 - r0, nc, ws.R + r0 + r0 * ldr, ldr, res->X + r0,
-                                     res->ldx));
-        ++launches;
-      }
-      if (s0 > 0) {
-        HIPCHK(rbrp_launch_gemm_nn(st, dim_static(s0), s0, dim_static(nc), nc, dim_static(s1 - s0),
-                                   mat_off(ws.R, ldr, s0), mat(res->X + s0, res->ldx),
-                                   mat(res->X, res->ldx), GEMM_SUB, nullptr, 0));
-        ++launches;
-      }
-    }
+    HIPCHK(rbrp_trsm_upper_left(st, k, nc, ws.R, ldr, res->X, res->ldx, &launches));
     pt.end(ph, launches, (double)k * k * nc, 8.0 * (0.5 * k * k + 2.0 * k * nc));
   }
   HIPCHK(hipStreamSynchronize(st));

@@ -122,3 +122,21 @@ hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, cons
                                  int64_t ldr, double* X, int64_t ldx);
 hipError_t rbrp_launch_uniforms(hipStream_t st, uint64_t key0, uint64_t key1, const uint64_t ctr[4],
                                 uint64_t draw0, int64_t count, double* out);
+
+// ---- k_sketch.hip ---------------------------------------------------------
+hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,
+                                int64_t cols, double scale, double* out, int64_t ldo);
+hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                   int64_t lds, const int* perm, double* dst, int64_t ldd);
+hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
+                                 int64_t lds, double* dst, int64_t ldd);
+hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, const double* W,
+                                int64_t ld, double* vn);
+hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
+                                  int64_t ld, double* vn, int* jpvt, double* tau, int pivoting);
+hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
+                                int64_t ld, const double* tau, double* vn, int norms);
+hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
+                                int64_t ld, int* ipiv);
+hipError_t rbrp_launch_lu_update(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
+                                 int64_t ld);

@@ -1,0 +1,213 @@
+// Sketch-based ID comparators behind the C ABI: RandCPQR_OS (gpu/randCPQR_OS.cu:20-111),
+// RandLUPP_OS (gpu/randLUPP_OS.cu:14-119) and the oversampled interpolation
+// matrix (gpu/interp.cu:45-105).  The sketch GEMM runs on the MFMA kernel;
+// the pivoted QR / LU and the QR of the interpolation step are Householder /
+// right-looking LU steps, one pair of launches per column.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/rbrp.h"
+#include "rbrp_handle.h"
+#include "rbrp_kernels.h"
+
+#define HIPCHK(x)                         \
+  do {                                    \
+    hipError_t e_ = (x);                  \
+    if (e_ != hipSuccess) return (int)e_; \
+  } while (0)
+
+namespace {
+
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Y (l x n, ldy = l) = G (l x m, ldg) * A (m x n, lda).
+hipError_t sketch_gemm(hipStream_t st, int64_t l, int64_t m, int64_t n, const double* G, int64_t ldg,
+                       const double* A, int64_t lda, double* Y) {
+  return rbrp_launch_gemm_nn(st, dim_static(l), l, dim_static(n), n, dim_static(m),
+                             mat((double*)G, ldg), mat((double*)A, lda), mat(Y, l), GEMM_SET,
+                             nullptr, 0);
+}
+
+// In place on Yp (l x n, ldy): k unpivoted Householder steps applied to all n
+// columns, then T (k x (n-k), ldt) = R(0:k,0:k)^{-1} Yp(0:k, k:n).  This is
+// geqrf + ormqr(Q^T) + trsm of interp.cu:82-97 with the same result.
+hipError_t interp_from_permuted(hipStream_t st, int64_t l, int64_t n, int64_t k, double* Yp,
+                                int64_t ldy, double* tau, double* T, int64_t ldt) {
+  hipError_t e;
+  for (int64_t j = 0; j < k; ++j) {
+    e = rbrp_launch_hh_reflect(st, l, n, j, Yp, ldy, nullptr, nullptr, tau, 0);
+    if (e != hipSuccess) return e;
+    e = rbrp_launch_hh_apply(st, l, n, j, Yp, ldy, tau, nullptr, 0);
+    if (e != hipSuccess) return e;
+  }
+  if (n > k) {
+    e = rbrp_launch_copy_block(st, k, n - k, Yp + k * ldy, ldy, T, ldt);
+    if (e != hipSuccess) return e;
+    e = rbrp_trsm_upper_left(st, k, n - k, Yp, ldy, T, ldt, nullptr);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+int check_common(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda, int64_t r,
+                 const double* dG, int64_t ldg, double* dT, int64_t ldt) {
+  if (!h) return -1;
+  if (m < 1) return -3;
+  if (n < 1) return -4;
+  if (lda < m) return -5;
+  if (!dA) return -2;
+  if (r < 1 || r > std::min(m, n)) return -6;
+  if (dG && ldg < 2 * r) return -10;
+  if (n > r && (!dT || ldt < r)) return -13;
+  (void)dG;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbrp_gaussian_sketch_d(rbrp_handle_t h, uint64_t seed, uint64_t offset, int64_t rows,
+                           int64_t cols, double scale, double* dG, int64_t ldg) {
+  if (!h) return -1;
+  if (rows < 0) return -4;
+  if (cols < 0) return -5;
+  if (rows * cols > 0 && !dG) return -7;
+  if (ldg < rows) return -8;
+  HIPCHK(rbrp_launch_gaussian(h->stream, seed, offset, rows, cols, scale, dG, ldg));
+  return 0;
+}
+
+int rbrp_interp_os_d(rbrp_handle_t h, const double* dY, int64_t l, int64_t n, int64_t ldy, int64_t k,
+                     const int32_t* dP, double* dT, int64_t ldt) {
+  if (!h) return -1;
+  if (!dY) return -2;
+  if (l < 1) return -3;
+  if (n < 1) return -4;
+  if (ldy < l) return -5;
+  if (k < 1 || k > std::min(l, n)) return -6;
+  if (!dP) return -7;
+  if (n > k && (!dT || ldt < k)) return -8;
+  hipStream_t st = h->stream;
+  void* arena = nullptr;
+  const size_t bYp = align256(sizeof(double) * l * n), btau = align256(sizeof(double) * k);
+  HIPCHK(rbrp_sketch_arena(h, bYp + btau, &arena));
+  double* Yp = (double*)arena;
+  double* tau = (double*)((char*)arena + bYp);
+  HIPCHK(rbrp_launch_gather_cols(st, l, n, dY, ldy, dP, Yp, l));
+  HIPCHK(interp_from_permuted(st, l, n, k, Yp, l, tau, dT, ldt));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                       int64_t r, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                       int32_t* h_sk, int32_t* h_rd, double* dT, int64_t ldt) {
+  int c = check_common(h, dA, m, n, lda, r, dG, ldg, dT, ldt);
+  if (c) return c;
+  if (!h_sk) return -11;
+  if (n > r && !h_rd) return -12;
+  hipStream_t st = h->stream;
+  const int64_t l = 2 * r;  // oversampling (randCPQR_OS.cu:24)
+  const size_t bG = dG ? 0 : align256(sizeof(double) * l * m);
+  const size_t bY = align256(sizeof(double) * l * n), bW = align256(sizeof(double) * r * n);
+  const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * l);
+  const size_t bjp = align256(sizeof(int) * n);
+  void* arena = nullptr;
+  HIPCHK(rbrp_sketch_arena(h, bG + 2 * bY + bW + bvn + btau + bjp, &arena));
+  char* p = (char*)arena;
+  double* G = dG ? (double*)dG : (double*)p;
+  const int64_t ldG = dG ? ldg : l;
+  p += bG;
+  double* Y = (double*)p;
+  p += bY;
+  double* Yp = (double*)p;
+  p += bY;
+  double* W = (double*)p;
+  p += bW;
+  double* vn = (double*)p;
+  p += bvn;
+  double* tau = (double*)p;
+  p += btau;
+  int* jpvt = (int*)p;
+  if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
+  HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randCPQR_OS.cu:42
+  // pivoted QR of Y(0:r, :) (randCPQR_OS.cu:55-72): only the pivots are used
+  HIPCHK(rbrp_launch_copy_block(st, r, n, Y, l, W, r));
+  HIPCHK(rbrp_launch_colnorm2(st, r, n, W, r, vn));
+  HIPCHK(rbrp_launch_iota(st, jpvt, n));
+  const int64_t steps = std::min(r, n);
+  for (int64_t j = 0; j < steps; ++j) {
+    HIPCHK(rbrp_launch_hh_reflect(st, r, n, j, W, r, vn, jpvt, tau, 1));
+    HIPCHK(rbrp_launch_hh_apply(st, r, n, j, W, r, tau, vn, 1));
+  }
+  // T by the oversampled interpolation (randCPQR_OS.cu:95, interp.cu:45-105)
+  HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, jpvt, Yp, l));
+  HIPCHK(interp_from_permuted(st, l, n, r, Yp, l, tau, dT, ldt));
+  std::vector<int32_t> hp((size_t)n);
+  HIPCHK(hipMemcpyAsync(hp.data(), jpvt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::copy(hp.begin(), hp.begin() + r, h_sk);
+  if (n > r) std::copy(hp.begin() + r, hp.end(), h_rd);
+  return 0;
+}
+
+int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                       int64_t k, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                       int32_t* d_sk, int32_t* d_rd, double* dT, int64_t ldt) {
+  int c = check_common(h, dA, m, n, lda, k, dG, ldg, dT, ldt);
+  if (c) return c;
+  if (!d_sk) return -11;
+  if (n > k && !d_rd) return -12;
+  hipStream_t st = h->stream;
+  const int64_t l = 2 * k;  // oversampling (randLUPP_OS.cu:18)
+  const size_t bG = dG ? 0 : align256(sizeof(double) * l * m);
+  const size_t bY = align256(sizeof(double) * l * n), bYt = align256(sizeof(double) * n * k);
+  const size_t btau = align256(sizeof(double) * l), bip = align256(sizeof(int) * k);
+  const size_t bP = align256(sizeof(int) * n);
+  void* arena = nullptr;
+  HIPCHK(rbrp_sketch_arena(h, bG + 2 * bY + bYt + btau + bip + bP, &arena));
+  char* p = (char*)arena;
+  double* G = dG ? (double*)dG : (double*)p;
+  const int64_t ldG = dG ? ldg : l;
+  p += bG;
+  double* Y = (double*)p;
+  p += bY;
+  double* Yp = (double*)p;
+  p += bY;
+  double* Yt = (double*)p;
+  p += bYt;
+  double* tau = (double*)p;
+  p += btau;
+  int* ipiv = (int*)p;
+  p += bip;
+  int* P = (int*)p;
+  if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
+  HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randLUPP_OS.cu:43
+  // LU with partial pivoting of Y(0:k, :)^T (n x k) (randLUPP_OS.cu:52-82)
+  HIPCHK(rbrp_launch_transpose(st, k, n, Y, l, Yt, n));
+  const int64_t steps = std::min(n, k);
+  for (int64_t j = 0; j < steps; ++j) {
+    HIPCHK(rbrp_launch_lu_pivot(st, n, k, j, Yt, n, ipiv));
+    HIPCHK(rbrp_launch_lu_update(st, n, k, j, Yt, n));
+  }
+  // pivots -> permutation (permute.cu:8-23), applied in order on the host
+  std::vector<int> hip_((size_t)steps), hP((size_t)n);
+  HIPCHK(hipMemcpyAsync(hip_.data(), ipiv, sizeof(int) * steps, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int64_t q = 0; q < n; ++q) hP[q] = (int)q;
+  for (int64_t j = 0; j < steps; ++j) std::swap(hP[j], hP[hip_[j]]);
+  HIPCHK(hipMemcpyAsync(P, hP.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_sk, P, sizeof(int) * k, hipMemcpyDeviceToDevice, st));
+  if (n > k) HIPCHK(hipMemcpyAsync(d_rd, P + k, sizeof(int) * (n - k), hipMemcpyDeviceToDevice, st));
+  // T by the oversampled interpolation (randLUPP_OS.cu:96, interp.cu:45-105)
+  HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, P, Yp, l));
+  HIPCHK(interp_from_permuted(st, l, n, k, Yp, l, tau, dT, ldt));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // extern "C"

@@ -188,3 +188,100 @@ def uniforms(seed: int, offset: int, draw0: int, count: int, handle: Optional[Ha
         if own:
             h.close()
     return out
+
+
+def gaussian_sketch(rows: int, cols: int, scale: float, seed: int = 0, offset: int = 0,
+                    handle: Optional[Handle] = None):
+    """Device Gaussian sketch (rows x cols, column-major) from the Philox stream (seed, offset)."""
+    torch = _torch()
+    buf = torch.empty((cols, rows), dtype=torch.float64, device="cuda")
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_gaussian_sketch_d", h._lib.rbrp_gaussian_sketch_d(
+            h.ptr, seed, offset, rows, cols, float(scale), ctypes.c_void_p(buf.data_ptr()), rows))
+        h.stream.synchronize()
+    finally:
+        if own:
+            h.close()
+    return buf.t()
+
+
+def _sketch_args(A, G):
+    A = colmajor(A)
+    if G is not None:
+        G = colmajor(G)
+        return A, ctypes.c_void_p(G.data_ptr()), leading_dim(G)
+    return A, None, 0
+
+
+def RandCPQR_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
+                handle: Optional[Handle] = None):
+    """Mirror of ``RandCPQR_OS(A, m, n, rank, h_sk, h_rd, T, flops)`` (gpu/rid.hpp:19-20).
+
+    Returns ``(sk, rd, T)``: host int arrays (as the reference's ``h_sk``/``h_rd``)
+    and the device ``T`` (rank x (n - rank)).  ``G`` optionally supplies the
+    2*rank x m sketch; otherwise it is drawn from the Philox stream (seed, offset).
+    """
+    torch = _torch()
+    A, gp, ldg = _sketch_args(A, G)
+    m, n = A.shape
+    sk = np.zeros(rank, dtype=np.int32)
+    rd = np.zeros(max(n - rank, 1), dtype=np.int32)
+    Tb = torch.empty((max(n - rank, 1), rank), dtype=torch.float64, device=A.device)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_randcpqr_os_d", h._lib.rbrp_randcpqr_os_d(
+            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), rank, seed, offset, gp, ldg,
+            ctypes.c_void_p(sk.ctypes.data), ctypes.c_void_p(rd.ctypes.data),
+            ctypes.c_void_p(Tb.data_ptr()), rank))
+    finally:
+        if own:
+            h.close()
+    return sk, rd[: n - rank], Tb[: n - rank].t()
+
+
+def RandLUPP_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
+                handle: Optional[Handle] = None):
+    """Mirror of ``RandLUPP_OS(A, m, n, rank, sk, rd, T, flops)`` (gpu/rid.hpp:12-13).
+
+    Returns ``(sk, rd, T)`` as device tensors (the reference's ``sk``/``rd`` are
+    device pointers too).
+    """
+    torch = _torch()
+    A, gp, ldg = _sketch_args(A, G)
+    m, n = A.shape
+    sk = torch.empty(rank, dtype=torch.int32, device=A.device)
+    rd = torch.empty(max(n - rank, 1), dtype=torch.int32, device=A.device)
+    Tb = torch.empty((max(n - rank, 1), rank), dtype=torch.float64, device=A.device)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_randlupp_os_d", h._lib.rbrp_randlupp_os_d(
+            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), rank, seed, offset, gp, ldg,
+            ctypes.c_void_p(sk.data_ptr()), ctypes.c_void_p(rd.data_ptr()),
+            ctypes.c_void_p(Tb.data_ptr()), rank))
+    finally:
+        if own:
+            h.close()
+    return sk, rd[: n - rank], Tb[: n - rank].t()
+
+
+def interp_os(Y, P, k: int, handle: Optional[Handle] = None):
+    """Mirror of ``Compute_interpolation_matrix`` (gpu/interp.cu:45-105): T (k x (n-k))."""
+    torch = _torch()
+    Y = colmajor(Y)
+    l, n = Y.shape
+    P = torch.as_tensor(P, dtype=torch.int32, device=Y.device).contiguous()
+    Tb = torch.empty((max(n - k, 1), k), dtype=torch.float64, device=Y.device)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_interp_os_d", h._lib.rbrp_interp_os_d(
+            h.ptr, ctypes.c_void_p(Y.data_ptr()), l, n, leading_dim(Y), k,
+            ctypes.c_void_p(P.data_ptr()), ctypes.c_void_p(Tb.data_ptr()), k))
+    finally:
+        if own:
+            h.close()
+    return Tb[: n - k].t()

@@ -45,10 +45,44 @@ int main() {
       num += v * v;
     }
   for (double x : A) den += x * x;
+  // sketch comparators at the rank the ID found
+  std::vector<int> csk((size_t)k), crd((size_t)(n - k));
+  double* cT = nullptr;
+  double cfl = 0;
+  RandCPQR_OS(dA, m, n, k, csk.data(), crd.data(), cT, cfl, 5);
+  const int st3 = RBRP_last_status();
+  int *lsk = nullptr, *lrd = nullptr;
+  double* lT = nullptr;
+  double lfl = 0;
+  RandLUPP_OS(dA, m, n, k, lsk, lrd, lT, lfl, 6);
+  const int st4 = RBRP_last_status();
+  auto id_err = [&](const std::vector<int>& s_, const std::vector<int>& r_, const double* dT) {
+    std::vector<double> hT((size_t)k * (n - k));
+    (void)hipMemcpy(hT.data(), dT, sizeof(double) * hT.size(), hipMemcpyDeviceToHost);
+    double num2 = 0;
+    for (int j = 0; j < n - k; ++j)
+      for (int i = 0; i < m; ++i) {
+        double v = A[i + (size_t)r_[j] * m];
+        for (int q = 0; q < k; ++q) v -= A[i + (size_t)s_[q] * m] * hT[q + (size_t)j * k];
+        num2 += v * v;
+      }
+    return std::sqrt(num2 / den);
+  };
+  std::vector<int> lskh((size_t)k), lrdh((size_t)(n - k));
+  (void)hipMemcpy(lskh.data(), lsk, sizeof(int) * k, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(lrdh.data(), lrd, sizeof(int) * (n - k), hipMemcpyDeviceToHost);
+  const double ecp = (st3 == 0) ? id_err(csk, crd, cT) : -1.0;
+  const double elu = (st4 == 0) ? id_err(lskh, lrdh, lT) : -1.0;
+  if (cT) (void)hipFree(cT);
+  if (lT) (void)hipFree(lT);
+  if (lsk) (void)hipFree(lsk);
+  if (lrd) (void)hipFree(lrd);
+  std::printf("{\"status3\": %d, \"status4\": %d, \"cpqr_os_error\": %.6e, \"lupp_os_error\": %.6e}\n",
+              st3, st4, ecp, elu);
   std::printf("{\"status1\": %d, \"final_rank\": %d, \"nblocks\": %zu, \"status2\": %d, \"k\": %d, "
               "\"id_error\": %.6e, \"flops\": %.6e}\n",
               st1, ranks.empty() ? 0 : ranks.back(), ranks.size(), st2, k, std::sqrt(num / den), flops);
   if (X) (void)hipFree(X);
   (void)hipFree(dA);
-  return (st1 == 0 && st2 == 0) ? 0 : 1;
+  return (st1 == 0 && st2 == 0 && st3 == 0 && st4 == 0) ? 0 : 1;
 }

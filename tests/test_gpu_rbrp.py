@@ -164,6 +164,11 @@ def test_cxx_call_surface_drop_in():
     assert os.path.exists(build_lib.CXX_CHECK), "build the library first"
     r = subprocess.run([build_lib.CXX_CHECK], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    out = json.loads(r.stdout.strip().splitlines()[-1])
+    lines = r.stdout.strip().splitlines()
+    out = json.loads(lines[-1])
     assert out["status1"] == 0 and out["final_rank"] == 300
     assert out["status2"] == 0 and out["id_error"] <= 1e-6 and out["k"] > 0
+    cmp = json.loads(lines[-2])
+    assert cmp["status3"] == 0 and cmp["status4"] == 0
+    # the sketch IDs at the same rank are accurate to within a modest factor
+    assert 0 <= cmp["cpqr_os_error"] <= 1e-4 and 0 <= cmp["lupp_os_error"] <= 1e-4
