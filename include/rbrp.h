@@ -29,8 +29,16 @@ typedef struct rbrp_handle_s* rbrp_handle_t;
 
 /* Filter threshold (gpu/rbrp.cu:198-207 vs RBRP.ipynb c11:69-73). */
 enum { RBRP_FILTER_SAMPLED = 0, RBRP_FILTER_NOMINAL = 1 };
-/* Arithmetic of the residual update (FP64 is the reference's). */
-enum { RBRP_PREC_FP64 = 0 };
+/*
+ * Arithmetic of the residual.  FP64 is the reference's.  FP32 is the mixed
+ * form: the trailing residual is stored in FP32 and projected / updated with
+ * FP32 MFMA products and FP32 accumulation (split-K partials and column norms
+ * summed in FP64), while the sampling weights' sum and stopping test, the
+ * re-orthogonalisation, the pivoted factorisation of the sampled block, R and
+ * X stay FP64.  Its resolution is bounded by FP32 storage: relative errors
+ * below about 1e-7 ||A||_F are not resolved.
+ */
+enum { RBRP_PREC_FP64 = 0, RBRP_PREC_FP32 = 1 };
 
 typedef struct {
   int64_t b;              /* nominal block size, 1..128 (rbrp.hpp:6 `blk`) */
@@ -100,6 +108,15 @@ int rbrp_release_workspace(rbrp_handle_t handle);
  * matlab/driver.m:83-94.  dA is m x n, leading dimension lda >= m.
  */
 int rbrp_colid_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                 const rbrp_opts_t* opts, rbrp_result_t* result);
+
+/*
+ * The same column ID for an FP32-stored input (dA is m x n FP32, leading
+ * dimension lda).  With opts->precision = RBRP_PREC_FP32 the residual stays
+ * FP32; with RBRP_PREC_FP64 the loop runs on an FP64 copy.  No reference
+ * counterpart: the reference takes FP64 input only (gpu/rbrp.hpp:6-8).
+ */
+int rbrp_colid_s(rbrp_handle_t handle, const float* dA, int64_t m, int64_t n, int64_t lda,
                  const rbrp_opts_t* opts, rbrp_result_t* result);
 
 /*

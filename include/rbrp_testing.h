@@ -36,6 +36,19 @@ int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const doubl
 int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                        const double* Bt, int64_t ldbt, double* C, int64_t ldc, double* normpart);
 
+/* FP32 split-K TN kernel: C (M x N, FP64, ldc) = A^T B with A (K x M) and B
+ * (K x N) FP32 column-major with the common leading dimension ld (a multiple
+ * of 4), FP32 products and accumulation per split, FP64 split sum.  K must be
+ * a multiple of 32. */
+int rbrp_test_gemm_tn32(void* stream, int64_t M, int64_t N, int64_t K, const float* A,
+                        const float* B, int64_t ld, double* C, int64_t ldc, int nsplit);
+
+/* FP32 trailing-update kernel (K <= 128, M, lda, ldc even): C(r, c) -= sum_k
+ * A(r, k) Bt[k * ldbt + c], FP32 data; normpart (optional, FP64) receives
+ * ceil(M/128) x N per-128-row tile squared column norms (leading dimension N). */
+int rbrp_test_gemm_upd32(void* stream, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
+                         const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* normpart);
+
 /* Pivoted Cholesky + robust filter of an s x s Gram matrix G (ldg = s) whose
  * columns correspond to the column indices J[0:s] (device), at block start i.
  * Outputs (device, capacity b each): Jk[0:bt] (pivot order), Rb (b x b, ld b),

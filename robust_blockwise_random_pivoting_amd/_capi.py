@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librbrp.so"
 RBRP_ERR_NUMERIC = 100000
 RBRP_ERR_ALLOC = 100001
 FILTERS = {"sampled": 0, "nominal": 1}
-PRECISIONS = {"fp64": 0}
+PRECISIONS = {"fp64": 0, "fp32": 1}
 
 PHASES = ["norms", "sample", "cgs", "panel", "permute", "project", "project_reduce", "update",
           "resid", "solve"]
@@ -67,6 +67,7 @@ def lib():
         "rbrp_release_workspace": (c_int, [vp]),
         "rbrp_uniforms_d": (c_int, [vp, u64, u64, u64, i64, vp]),
         "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
+        "rbrp_colid_s": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
         "rbrp_gaussian_sketch_d": (c_int, [vp, u64, u64, i64, i64, c_double, vp, i64]),
         "rbrp_randcpqr_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
                                        i64]),
@@ -76,6 +77,8 @@ def lib():
         "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
         "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
+        "rbrp_test_gemm_tn32": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
+        "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
         "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
                                     POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
         "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,

@@ -111,10 +111,11 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // split (two accumulators), then the four partials are added in a fixed
 // order, so the result does not depend on scheduling.
 // ---------------------------------------------------------------------------
+template <typename T2>
 __global__ __launch_bounds__(256) void k_reduce_splits(DimSpec dM, DimSpec dN,
                                                        const double* __restrict__ P, int64_t ldp,
                                                        int64_t pstride, int nsplit, MatRef dst,
-                                                       int mode, double* __restrict__ dst2,
+                                                       int mode, T2* __restrict__ dst2,
                                                        int64_t ld2) {
   __shared__ double part[4][64];
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void k_reduce_splits(DimSpec dM, DimSpec dN,
       *d -= s;
     else
       *d = s;
-    if (dst2) dst2[r * ld2 + c] = s;  // row-major copy
+    if (dst2) dst2[r * ld2 + c] = (T2)s;  // row-major copy
   }
 }
 
@@ -152,8 +153,19 @@ hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64
                                      MatRef dst, int mode, double* dst2, int64_t ld2) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_reduce_splits, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit, dst,
-                     mode, dst2, ld2);
+  hipLaunchKernelGGL(k_reduce_splits<double>, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit,
+                     dst, mode, dst2, ld2);
+  return hipGetLastError();
+}
+
+hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax,
+                                             int64_t Nmax, const double* P, int64_t ldp,
+                                             int64_t pstride, int nsplit, MatRef dst, int mode,
+                                             float* dst2, int64_t ld2) {
+  if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
+  dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
+  hipLaunchKernelGGL(k_reduce_splits<float>, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit,
+                     dst, mode, dst2, ld2);
   return hipGetLastError();
 }
 

@@ -1,0 +1,413 @@
+// Kernels of the mixed-precision RBRP loop (gfx950): the residual of the
+// trailing columns is stored in FP32 and projected / updated with
+// v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation), while the
+// re-orthogonalisation, the pivoted Cholesky / CholQR2 of the sampled block
+// and the triangular solve run in FP64 on an FP64 copy of the block.
+//
+// Operand layout of v_mfma_f32_16x16x4f32 (A 16x4, B 4x16, D 16x16):
+//   A[i][k] in lane i + 16k,  B[k][j] in lane j + 16k,
+//   D[i][j] in lane j + 16*(i/4), accumulator element i % 4.
+#include <algorithm>
+
+#include "rbrp_kernels.h"
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Norm pass into the FP32 working matrix: Aw(:, j) = fp32([A(0:m, j); 0]),
+// diags[j] = sum of the squares of the stored FP32 values, accumulated in FP64
+// (so the weights describe exactly the residual that is stored).
+// ---------------------------------------------------------------------------
+template <typename TIn>
+__global__ __launch_bounds__(256) void k_copy_colnorm32(const TIn* __restrict__ A, int64_t lda,
+                                                        int64_t m, int64_t mp,
+                                                        float* __restrict__ Aw,
+                                                        double* __restrict__ diags) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const TIn* a = A + j * lda;
+  float* w = Aw + j * mp;
+  double acc0 = 0.0, acc1 = 0.0;
+  int64_t r = threadIdx.x;
+  for (; r + 256 < m; r += 512) {
+    const float v0 = (float)a[r], v1 = (float)a[r + 256];
+    w[r] = v0;
+    w[r + 256] = v1;
+    acc0 = fma((double)v0, (double)v0, acc0);
+    acc1 = fma((double)v1, (double)v1, acc1);
+  }
+  for (; r < mp; r += 256) {
+    const float v = r < m ? (float)a[r] : 0.0f;
+    w[r] = v;
+    acc0 = fma((double)v, (double)v, acc0);
+  }
+  const double t = rbrp_block_sum(acc0 + acc1, red);
+  if (threadIdx.x == 0) diags[j] = t;
+}
+
+hipError_t rbrp_launch_copy_colnorm32(hipStream_t st, const void* A, int a_is_f32, int64_t lda,
+                                      int64_t m, int64_t mp, int64_t n, float* Aw, double* diags) {
+  if (n <= 0) return hipSuccess;
+  if (a_is_f32)
+    hipLaunchKernelGGL(k_copy_colnorm32<float>, dim3((unsigned)n), dim3(256), 0, st,
+                       (const float*)A, lda, m, mp, Aw, diags);
+  else
+    hipLaunchKernelGGL(k_copy_colnorm32<double>, dim3((unsigned)n), dim3(256), 0, st,
+                       (const double*)A, lda, m, mp, Aw, diags);
+  return hipGetLastError();
+}
+
+// FP64 working copy of an FP32 input (the FP64 loop on FP32-stored data).
+__global__ __launch_bounds__(256) void k_copy_colnorm_from32(const float* __restrict__ A,
+                                                             int64_t lda, int64_t m, int64_t mp,
+                                                             double* __restrict__ Aw,
+                                                             double* __restrict__ diags) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const float* a = A + j * lda;
+  double* w = Aw + j * mp;
+  double acc = 0.0;
+  for (int64_t r = threadIdx.x; r < mp; r += 256) {
+    const double v = r < m ? (double)a[r] : 0.0;
+    w[r] = v;
+    acc = fma(v, v, acc);
+  }
+  const double t = rbrp_block_sum(acc, red);
+  if (threadIdx.x == 0) diags[j] = t;
+}
+
+hipError_t rbrp_launch_copy_colnorm_from32(hipStream_t st, const float* A, int64_t lda, int64_t m,
+                                           int64_t mp, int64_t n, double* Aw, double* diags) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_colnorm_from32, dim3((unsigned)n), dim3(256), 0, st, A, lda, m, mp, Aw,
+                     diags);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sampled block <-> FP64 copy.  V(:, p) = fp64(Aw(:, J[p])) for p < s, and the
+// re-orthogonalised columns written back, Aw(:, J[p]) = fp32(V(:, p)).
+// ---------------------------------------------------------------------------
+__global__ void k_gather_v64(int64_t rows, const RbrpBlockState* bs, const float* __restrict__ Aw,
+                             int64_t ld, const int* __restrict__ J, double* __restrict__ V,
+                             int64_t ldv) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int p = blockIdx.y;
+  if (r >= rows || p >= bs->s) return;
+  V[r + (int64_t)p * ldv] = (double)Aw[r + (int64_t)J[p] * ld];
+}
+
+__global__ void k_scatter_v32(int64_t rows, const RbrpBlockState* bs, const double* __restrict__ V,
+                              int64_t ldv, const int* __restrict__ J, float* __restrict__ Aw,
+                              int64_t ld) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int p = blockIdx.y;
+  if (r >= rows || p >= bs->s) return;
+  Aw[r + (int64_t)J[p] * ld] = (float)V[r + (int64_t)p * ldv];
+}
+
+hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                  const float* Aw, int64_t ld, const int* J, double* V,
+                                  int64_t ldv) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_v64, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
+                     st, rows, bs, Aw, ld, J, V, ldv);
+  return hipGetLastError();
+}
+
+hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                   const double* V, int64_t ldv, const int* J, float* Aw,
+                                   int64_t ld) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_v32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
+                     st, rows, bs, V, ldv, J, Aw, ld);
+  return hipGetLastError();
+}
+
+// Symmetric-difference column moves in the FP32 working matrix.
+__global__ void k_move_cols32(int64_t rows, const RbrpBlockState* bs, float* __restrict__ Aw,
+                              int64_t ld, const int* __restrict__ Jin,
+                              const int* __restrict__ Jout) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows || c >= bs->nswap) return;
+  Aw[r + (int64_t)Jin[c] * ld] = Aw[r + (int64_t)Jout[c] * ld];
+}
+
+hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                   float* Aw, int64_t ld, const int* Jin, const int* Jout) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_move_cols32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
+                     st, rows, bs, Aw, ld, Jin, Jout);
+  return hipGetLastError();
+}
+
+// FP32 copy of the new orthonormal block: Aw(:, i + c) = fp32(Q(:, i + c)), c < bt.
+__global__ void k_store_q32(int64_t rows, int64_t i, const RbrpBlockState* bs,
+                            const double* __restrict__ Q, int64_t ldq, float* __restrict__ Aw,
+                            int64_t ld) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int c = blockIdx.y;
+  if (r >= rows || c >= bs->bt) return;
+  Aw[r + (i + c) * ld] = (float)Q[r + (i + c) * ldq];
+}
+
+hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
+                                 const RbrpBlockState* bs, const double* Q, int64_t ldq, float* Aw,
+                                 int64_t ld) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_store_q32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0, st,
+                     rows, i, bs, Q, ldq, Aw, ld);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Split-K TN in FP32: P[z](r, c) = sum_{kk in chunk z} A(kk, r) B(kk, c) with
+// FP32 MFMA products and accumulation, stored as FP64 partials (the split sum
+// then runs in FP64).  A and B are column-major FP32 with a common leading
+// dimension; their first columns are acoff + *a_dyn and bcoff + *b_dyn.
+// 64x64 tiles, 4 waves in 2x2, 32-deep K tiles through LDS: each thread
+// loads two float4 (8 consecutive rows of one column), so a column segment
+// is one 128-byte run.  kchunk is a multiple of 32 and rows >= K are zero.
+// ---------------------------------------------------------------------------
+#define KT32 32
+
+__global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64_t K,
+                                                   int64_t kchunk, const float* __restrict__ A0,
+                                                   int64_t acoff, const int* a_dyn,
+                                                   const float* __restrict__ B0, int64_t bcoff,
+                                                   const int* b_dyn, int64_t ld,
+                                                   double* __restrict__ P, int64_t ldp,
+                                                   int64_t pstride) {
+  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t m0 = (int64_t)blockIdx.y * 64, n0 = (int64_t)blockIdx.x * 64;
+  if (m0 >= M || n0 >= N) return;
+  const int64_t kb0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t kb1 = min(K, kb0 + kchunk);
+
+  __shared__ float As[64 * (KT32 + 4)];  // [col][k]
+  __shared__ float Bs[64 * (KT32 + 4)];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid >> 1, wn = wid & 1;
+  const int lc = tid >> 2, kq = (tid & 3) * 8;
+  const float* A = A0 + (acoff + (a_dyn ? *a_dyn : 0)) * ld;
+  const float* B = B0 + (bcoff + (b_dyn ? *b_dyn : 0)) * ld;
+  const float* ap = (m0 + lc < M) ? A + (m0 + lc) * ld : nullptr;
+  const float* bp = (n0 + lc < N) ? B + (n0 + lc) * ld : nullptr;
+
+  float4 ra[2], rb[2];
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  v4f acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  auto load = [&](int64_t kb) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      ra[h] = ap ? *(const float4*)(ap + kb + kq + 4 * h) : z4;
+      rb[h] = bp ? *(const float4*)(bp + kb + kq + 4 * h) : z4;
+    }
+  };
+  if (kb0 < kb1) load(kb0);
+  for (int64_t kb = kb0; kb < kb1; kb += KT32) {
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      *(float4*)&As[lc * (KT32 + 4) + kq + 4 * h] = ra[h];
+      *(float4*)&Bs[lc * (KT32 + 4) + kq + 4 * h] = rb[h];
+    }
+    __syncthreads();
+    if (kb + KT32 < kb1) load(kb + KT32);
+#pragma unroll
+    for (int kk = 0; kk < KT32; kk += 4) {
+      const int ko = kk + (lane >> 4);
+      const float a0 = As[(wm * 32 + (lane & 15)) * (KT32 + 4) + ko];
+      const float a1 = As[(wm * 32 + 16 + (lane & 15)) * (KT32 + 4) + ko];
+      const float b0 = Bs[(wn * 32 + (lane & 15)) * (KT32 + 4) + ko];
+      const float b1 = Bs[(wn * 32 + 16 + (lane & 15)) * (KT32 + 4) + ko];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  double* Pz = P + (int64_t)blockIdx.z * pstride;
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + v;
+        const int64_t c = n0 + wn * 32 + ni * 16 + (lane & 15);
+        if (r < M && c < N) Pz[r + c * ldp] = (double)acc[mi][ni][v];
+      }
+}
+
+hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                                 int64_t K, int nsplit, int64_t kchunk, const float* A,
+                                 int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,
+                                 const int* b_dyn, int64_t ld, double* P, int64_t ldp,
+                                 int64_t pstride) {
+  if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
+  if (kchunk % KT32 != 0 || K % KT32 != 0 || ld % 4 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
+  hipLaunchKernelGGL(k_gemm_tn32, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff, a_dyn, B, bcoff,
+                     b_dyn, ld, P, ldp, pstride);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP32 trailing update with fused norms: C(r, c) -= sum_k A(r, k) Bt[k * ldbt + c]
+// for r < M, c < N, K <= Kmax <= 128; optional FP64 per-128-row-tile squared
+// column norms of the new C (FP32 squares summed in FP32 within a wave, then
+// in FP64 across the four waves).
+//
+// The MFMA computes D = B^T A^T (D[i][j] = C(row j, column i) of a 16x16
+// block) so that C needs no LDS staging.  The 16 "j" rows of the block are
+// chosen as r0 + 2l + jb (l = 0..15, jb = 0..1) over the two row blocks a
+// wave owns, so that lane l holds rows r0 + 2l and r0 + 2l + 1 of the same
+// four columns: every C access is a float2, and one instruction moves four
+// 128-byte column runs.  The A fragments follow the same row order (one
+// float2 per k per lane).
+//
+// Workgroup = 4 waves stacked along rows (32 rows each), 128 rows x 32
+// columns per tile, sweeping T consecutive row tiles; B (K x 32) in LDS; the
+// next tile's C is prefetched after the current tile's A fragments.
+// ---------------------------------------------------------------------------
+template <int KSTEPS, bool NORMS>
+__global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, DimSpec dK, int T,
+                                                      const float* __restrict__ A0, int64_t lda,
+                                                      int64_t acoff, const int* acoff_dyn,
+                                                      const float* __restrict__ Bt, int64_t ldbt,
+                                                      float* __restrict__ C0, int64_t ldc,
+                                                      int64_t ccoff, const int* ccoff_dyn,
+                                                      double* __restrict__ normpart, int64_t ldnp) {
+  __shared__ float Ws[4 * KSTEPS * 32];  // [k][32]
+  __shared__ double red[4][32];
+  const int64_t N = dim_eval(dN), K = dim_eval(dK);
+  const int64_t c0 = (int64_t)blockIdx.x * 32;
+  const int64_t rbase = (int64_t)blockIdx.y * T * 128;
+  if (c0 >= N || rbase >= M) return;  // uniform per workgroup
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const float* A = A0 + (acoff + (acoff_dyn ? *acoff_dyn : 0)) * lda;
+  float* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + c0) * ldc;
+  for (int idx = threadIdx.x; idx < 4 * KSTEPS * 32; idx += 256) {
+    const int k = idx >> 5, c = idx & 31;
+    Ws[idx] = (k < K && c0 + c < N) ? Bt[(int64_t)k * ldbt + c0 + c] : 0.0f;
+  }
+  __syncthreads();
+  const int ncol = (int)min((int64_t)32, N - c0);
+  // M is a multiple of 2 (the working matrices have even padded heights), so
+  // a float2 at an even row r < M is in bounds.
+  float2 cvA[2][4], cvB[2][4];
+  const float2 z2 = make_float2(0.f, 0.f);
+  auto loadC = [&](int64_t r0, float2 (&cv)[2][4]) {
+    const int64_t r = r0 + 32 * wid + 2 * li;
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int cc = 16 * ib + 4 * lk + v;
+        cv[ib][v] = (cc < ncol && r < M) ? *(const float2*)(C + r + cc * ldc) : z2;
+      }
+  };
+  loadC(rbase, cvA);
+  for (int t = 0; t < T; ++t) {
+    const int64_t r0 = rbase + (int64_t)t * 128;
+    if (r0 >= M) break;  // uniform per workgroup
+    const int64_t r = r0 + 32 * wid + 2 * li;
+    float2 fa[KSTEPS];
+#pragma unroll
+    for (int q = 0; q < KSTEPS; ++q) {
+      const int k = 4 * q + lk;
+      fa[q] = (k < K && r < M) ? *(const float2*)(A + r + (int64_t)k * lda) : z2;
+    }
+    if (t + 1 < T) loadC(r0 + 128, cvB);
+    v4f acc[2][2];  // [jb][ib]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < KSTEPS; ++q) {
+      const int k = 4 * q + lk;
+      const float b0 = Ws[k * 32 + li], b1 = Ws[k * 32 + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].x, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].x, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].y, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].y, acc[1][1], 0, 0, 0);
+    }
+    // lane (li, lk) holds D[4lk + v][li]: column 16ib + 4lk + v, rows r (jb 0) and r + 1 (jb 1)
+    float nsq[2][4];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int cc = 16 * ib + 4 * lk + v;
+        float2 nv;
+        nv.x = cvA[ib][v].x - acc[0][ib][v];
+        nv.y = cvA[ib][v].y - acc[1][ib][v];
+        nsq[ib][v] = 0.f;
+        if (cc < ncol && r < M) {
+          *(float2*)(C + r + cc * ldc) = nv;
+          nsq[ib][v] = fmaf(nv.x, nv.x, nv.y * nv.y);
+        }
+      }
+    if (NORMS) {
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float x = nsq[ib][v];
+          x += __shfl_xor(x, 1, 64);
+          x += __shfl_xor(x, 2, 64);
+          x += __shfl_xor(x, 4, 64);
+          x += __shfl_xor(x, 8, 64);
+          if (li == 0) red[wid][16 * ib + 4 * lk + v] = (double)x;
+        }
+      __syncthreads();
+      if (wid == 0 && lane < ncol)
+        normpart[(r0 >> 7) * ldnp + c0 + lane] =
+            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cvA[ib][v] = cvB[ib][v];
+  }
+}
+
+hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
+                                    int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
+                                    const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
+                                    int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
+                                    double* normpart, int64_t ldnp) {
+  if (M <= 0 || Nmax <= 0) return hipSuccess;
+  if (Kmax > 128 || (M & 1) || (lda & 1) || (ldc & 1)) return hipErrorInvalidValue;
+  const bool small = Kmax <= 64;
+  const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 127) / 128;
+  int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 2048));
+  dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
+#define TRAIL32_LAUNCH(KS, NM)                                                                   \
+  hipLaunchKernelGGL((k_gemm_trail32<KS, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,  \
+                     acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp)
+  if (small) {
+    if (normpart)
+      TRAIL32_LAUNCH(16, true);
+    else
+      TRAIL32_LAUNCH(16, false);
+  } else {
+    if (normpart)
+      TRAIL32_LAUNCH(32, true);
+    else
+      TRAIL32_LAUNCH(32, false);
+  }
+#undef TRAIL32_LAUNCH
+  return hipGetLastError();
+}

@@ -386,7 +386,8 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
                                                         RbrpBlockState* bs, double* __restrict__ Rb,
                                                         double* __restrict__ Rinv,
                                                         int* __restrict__ Jk, int* __restrict__ Jin,
-                                                        int* __restrict__ Jout) {
+                                                        int* __restrict__ Jout,
+                                                        int* __restrict__ Jkpos) {
   extern __shared__ double S[];  // s x (s + 1)
   __shared__ int piv[SAMPLE_MAXB];
   __shared__ int sj[SAMPLE_MAXB];
@@ -492,6 +493,7 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   for (int t = tid; t < bt; t += NT) {
     Jk[t] = J[piv[t]];
     sj[t] = J[piv[t]];
+    if (Jkpos) Jkpos[t] = piv[t];
   }
   for (int idx = tid; idx < bt * bt; idx += NT) {
     const int r = idx % bt, c = idx / bt;
@@ -535,7 +537,8 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
 
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
-                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout) {
+                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                      int* Jkpos) {
   if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
@@ -544,7 +547,7 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_pivchol_filter, dim3(1), dim3(256), lds, st, G, ldg, b, filter_mode, i, J, bs,
-                     Rb, Rinv, Jk, Jin, Jout);
+                     Rb, Rinv, Jk, Jin, Jout, Jkpos);
   return hipGetLastError();
 }
 

@@ -14,7 +14,12 @@ struct RbrpWorkspace {
   // carved pointers of the RBRP loop
   double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
       *tmpk, *tmpo, *Wt;
-  int *J, *Jk, *Jin, *Jout;
+  // mixed precision: FP32 working matrix and row-major W, FP64 skeleton basis
+  // and FP64 copy of the sampled block (null in the FP64 loop, where Aw, Wt
+  // are used instead)
+  float *Aw32, *Wt32;
+  double *Q64, *Vd;
+  int *J, *Jk, *Jin, *Jout, *Jkpos;
   RbrpBlockState* st;
   int64_t Pcap;
 };

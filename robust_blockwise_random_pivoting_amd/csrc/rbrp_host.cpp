@@ -48,11 +48,13 @@ struct Carver {
 
 namespace {
 
-int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b) {
+int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed) {
   const int64_t ldr = std::max<int64_t>(kmax, 1);
   const int64_t Pcap = 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
   Carver c;
-  size_t oAw = c.take<double>(mp * n);
+  size_t oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
+  size_t oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
+  size_t oVd = mixed ? c.take<double>(mp * b) : 0;
   size_t oR = c.take<double>(ldr * n);
   size_t odiags = c.take<double>(n);
   size_t ow = c.take<double>(n);
@@ -67,13 +69,14 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   size_t oQbuf = c.take<double>(mp * b);
   size_t oP = c.take<double>(Pcap);
   size_t onp = c.take<double>(((mp + 127) / 128) * n);
-  size_t oWt = c.take<double>(b * n);
+  size_t oWt = c.take<double>(b * n);  // FP32 in the mixed loop (over-sized)
   size_t otk = c.take<double>(ldr * b);
   size_t oto = c.take<double>(ldr * b);
   size_t oJ = c.take<int>(b);
   size_t oJk = c.take<int>(b);
   size_t oJin = c.take<int>(b);
   size_t oJout = c.take<int>(b);
+  size_t oJkpos = c.take<int>(b);
   size_t ost = c.take<RbrpBlockState>(1);
   const size_t need = c.off;
   RbrpWorkspace& ws = h->ws;
@@ -91,7 +94,10 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
     ws.arena_bytes = need;
   }
   char* base = (char*)ws.arena;
-  ws.Aw = (double*)(base + oAw);
+  ws.Aw = mixed ? nullptr : (double*)(base + oAw);
+  ws.Aw32 = mixed ? (float*)(base + oAw) : nullptr;
+  ws.Q64 = mixed ? (double*)(base + oQ64) : nullptr;
+  ws.Vd = mixed ? (double*)(base + oVd) : nullptr;
   ws.R = (double*)(base + oR);
   ws.diags = (double*)(base + odiags);
   ws.w = (double*)(base + ow);
@@ -106,13 +112,15 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   ws.Qbuf = (double*)(base + oQbuf);
   ws.P = (double*)(base + oP);
   ws.normpart = (double*)(base + onp);
-  ws.Wt = (double*)(base + oWt);
+  ws.Wt = mixed ? nullptr : (double*)(base + oWt);
+  ws.Wt32 = mixed ? (float*)(base + oWt) : nullptr;
   ws.tmpk = (double*)(base + otk);
   ws.tmpo = (double*)(base + oto);
   ws.J = (int*)(base + oJ);
   ws.Jk = (int*)(base + oJk);
   ws.Jin = (int*)(base + oJin);
   ws.Jout = (int*)(base + oJout);
+  ws.Jkpos = (int*)(base + oJkpos);
   ws.st = (RbrpBlockState*)(base + ost);
   ws.Pcap = Pcap;
   return 0;
@@ -121,13 +129,13 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
 // Split-K shape for a TN product of Mmax x Nmax over K rows: aim for about
 // 1024 workgroups, keep the partials within the workspace.
 void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pcap, int* nsplit,
-                  int64_t* kchunk) {
+                  int64_t* kchunk, int64_t align = 16) {
   const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + 63) / 64);
   int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
   ns = std::min<int64_t>(ns, tiles <= 4 ? 256 : 128);
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, K / 64));
-  int64_t kc = round_up((K + ns - 1) / ns, 16);
+  int64_t kc = round_up((K + ns - 1) / ns, align);
   ns = (K + kc - 1) / kc;
   *nsplit = (int)std::max<int64_t>(ns, 1);
   *kchunk = kc;
@@ -253,7 +261,7 @@ hipError_t rbrp_sketch_arena(rbrp_handle_t h, size_t bytes, void** out) {
 
 extern "C" {
 
-const char* rbrp_version(void) { return "rbrp-mi355x 0.1 (gfx950, FP64 MFMA)"; }
+const char* rbrp_version(void) { return "rbrp-mi355x 0.2 (gfx950, FP64 and FP32 MFMA)"; }
 
 void rbrp_opts_default(rbrp_opts_t* o) {
   if (!o) return;
@@ -346,8 +354,17 @@ int rbrp_uniforms_d(rbrp_handle_t h, uint64_t seed, uint64_t offset, uint64_t dr
   return 0;
 }
 
-int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
-                 const rbrp_opts_t* opts, rbrp_result_t* res) {
+}  // extern "C"
+
+namespace {
+
+// The block loop for both precisions.  In the FP64 loop the working matrix Aw
+// (FP64) holds the skeleton basis Q in columns [0, i) and the residual in
+// [i, n).  In the mixed loop Aw32 (FP32) holds the residual and an FP32 copy
+// of Q; the FP64 Q lives in Q64 and the sampled block is re-orthogonalised and
+// factorised in the FP64 buffer Vd.
+int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n, int64_t lda,
+               const rbrp_opts_t* opts, rbrp_result_t* res) {
   const double t_start = now_s();
   if (!h) return -1;
   if (m < 0) return -3;
@@ -359,10 +376,11 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
   if (opts->b < 1 || opts->b > RBRP_MAX_B) return -6;
   if (!(opts->tol >= 0.0) || !std::isfinite(opts->tol)) return -6;
   if (opts->max_rank < 0) return -6;
-  if (opts->precision != RBRP_PREC_FP64) return -6;
+  if (opts->precision != RBRP_PREC_FP64 && opts->precision != RBRP_PREC_FP32) return -6;
   if (opts->filter != RBRP_FILTER_SAMPLED && opts->filter != RBRP_FILTER_NOMINAL) return -6;
   if (n > 0 && !res->perm) return -7;
   if (m > 0x7fffffffLL || n > 0x7fffffffLL) return -3;
+  const bool mixed = opts->precision == RBRP_PREC_FP32;
 
   const int64_t p = std::min(m, n);
   const int64_t kmax = opts->max_rank > 0 ? std::min<int64_t>(opts->max_rank, p) : p;
@@ -389,7 +407,7 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
   const int64_t mp = round_up(m, 64);
   const int64_t ldr = std::max<int64_t>(kmax, 1);
   {
-    int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1));
+    int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
     if (e) return e;
   }
   RbrpWorkspace& ws = h->ws;
@@ -398,18 +416,30 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
   const uint64_t key0 = opts->seed, key1 = 0;
   const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
   const int64_t ntiles = (mp + 127) / 128;  // row tiles of the update's norm partials
+  const double ea = a_f32 ? 4.0 : 8.0;      // bytes per element of the input
+  const double ew = mixed ? 4.0 : 8.0;      // bytes per element of the working matrix
   PhaseTimer pt(h);
 
   HIPCHK(hipMemsetAsync(ws.R, 0, sizeof(double) * ldr * n, st));
   int ph = pt.begin(RBRP_PHASE_NORMS);
-  HIPCHK(rbrp_launch_copy_colnorm(st, dA, lda, m, mp, n, ws.Aw, ws.diags));
-  pt.end(ph, 1, 2.0 * m * n, 8.0 * m * n + 8.0 * mp * n + 8.0 * n);
+  if (mixed)
+    HIPCHK(rbrp_launch_copy_colnorm32(st, dA, a_f32 ? 1 : 0, lda, m, mp, n, ws.Aw32, ws.diags));
+  else if (a_f32)
+    HIPCHK(rbrp_launch_copy_colnorm_from32(st, (const float*)dA, lda, m, mp, n, ws.Aw, ws.diags));
+  else
+    HIPCHK(rbrp_launch_copy_colnorm(st, (const double*)dA, lda, m, mp, n, ws.Aw, ws.diags));
+  pt.end(ph, 1, 2.0 * m * n, ea * m * n + ew * mp * n + 8.0 * n);
   HIPCHK(rbrp_launch_sum_to_state(st, ws.diags, n, ws.st, 0));
   HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   pt.flush();
   const double N0 = h->host_state->N0;
   if (!std::isfinite(N0)) return RBRP_ERR_NUMERIC;
+
+  // FP64 operands of the sampled block and of the skeleton basis
+  const MatRef Vblk = mixed ? mat(ws.Vd, mp) : mat_cols(ws.Aw, mp, ws.J);
+  const MatRef Vkept = mixed ? mat_cols(ws.Vd, mp, ws.Jkpos) : mat_cols(ws.Aw, mp, ws.Jk);
+  const MatRef Qall = mixed ? mat(ws.Q64, mp) : mat(ws.Aw, mp);
 
   int64_t i = 0, nb = 0;
   double err = (N0 > 0.0) ? 1.0 : 0.0;
@@ -423,20 +453,31 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
 
     // ---- second Gram-Schmidt pass of the sampled columns ----------------
     int ph_cgs = pt.begin(RBRP_PHASE_CGS);
+    int cgs_launches = 0;
+    if (mixed) {
+      HIPCHK(rbrp_launch_gather_v64(st, mp, b, ws.st, ws.Aw32, mp, ws.J, ws.Vd, mp));
+      ++cgs_launches;
+    }
     if (i > 0) {
       int ns;
       int64_t kc;
       choose_split(i, b, mp, i, ws.Pcap, &ns, &kc);
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, mp, ns, kc,
-                                 mat(ws.Aw, mp), mat_cols(ws.Aw, mp, ws.J), ws.P, i, i * (int64_t)b));
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, mp, ns, kc, Qall, Vblk,
+                                 ws.P, i, i * (int64_t)b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, ws.P, i,
                                        i * (int64_t)b, ns, mat(ws.Cbuf, ldr), GEMM_SET));
       HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i),
-                                 mat(ws.Aw, mp), mat(ws.Cbuf, ldr), mat_cols(ws.Aw, mp, ws.J),
-                                 GEMM_SUB, nullptr, 0));
+      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i), Qall,
+                                 mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, nullptr, 0));
+      cgs_launches += 4;
+      if (mixed) {
+        // the corrected columns replace the stored residual (the filtered-out
+        // ones stay in the trailing matrix)
+        HIPCHK(rbrp_launch_scatter_v32(st, mp, b, ws.st, ws.Vd, mp, ws.J, ws.Aw32, mp));
+        ++cgs_launches;
+      }
     }
-    pt.end(ph_cgs, i > 0 ? 4 : 0);
+    pt.end(ph_cgs, cgs_launches);
 
     // ---- panel: Gram, pivoted Cholesky + filter, CholQR2 ----------------
     ph = pt.begin(RBRP_PHASE_PANEL);
@@ -444,17 +485,15 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
       int ns;
       int64_t kc;
       choose_split(b, b, mp, b, ws.Pcap, &ns, &kc);
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, mp, ns, kc,
-                                 mat_cols(ws.Aw, mp, ws.J), mat_cols(ws.Aw, mp, ws.J), ws.P, b,
-                                 (int64_t)b * b));
+      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, mp, ns, kc, Vblk,
+                                 Vblk, ws.P, b, (int64_t)b * b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G, b), GEMM_SET));
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
-                                        ws.Jk, ws.Jin, ws.Jout));
+                                        ws.Jk, ws.Jin, ws.Jout, ws.Jkpos));
       // Q1 = V(:, Jk) Rb^{-1}
       HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
-                                 mat_cols(ws.Aw, mp, ws.Jk), mat(ws.Rinv, b), mat(ws.Qbuf, mp),
-                                 GEMM_SET, nullptr, 0));
+                                 Vkept, mat(ws.Rinv, b), mat(ws.Qbuf, mp), GEMM_SET, nullptr, 0));
       HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, mp, ns, kc,
                                  mat(ws.Qbuf, mp), mat(ws.Qbuf, mp), ws.P, b, (int64_t)b * b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
@@ -469,17 +508,27 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
 
     // ---- symmetric-difference permutation -------------------------------
     ph = pt.begin(RBRP_PHASE_PERMUTE);
+    int perm_launches = 3;
     HIPCHK(rbrp_launch_perm_small(st, i, ws.st, ws.Jk, ws.Jin, ws.Jout, perm, ws.diags));
     if (i > 0) {
       HIPCHK(rbrp_launch_perm_gather_r(st, i, b, ws.st, ws.R, ldr, ws.Jk, ws.Jout, ws.tmpk, ws.tmpo, ldr));
       HIPCHK(rbrp_launch_perm_scatter_r(st, i, b, i, ws.st, ws.R, ldr, ws.Jin, ws.tmpk, ws.tmpo, ldr));
+      perm_launches += 2;
     }
-    HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
-    // Q_b = Q1 R2^{-1} into the block's columns of the working matrix
+    if (mixed)
+      HIPCHK(rbrp_launch_move_cols32(st, mp, b, ws.st, ws.Aw32, mp, ws.Jin, ws.Jout));
+    else
+      HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
+    // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
     HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
-                               mat(ws.Qbuf, mp), mat(ws.R2inv, b), mat_off(ws.Aw, mp, i), GEMM_SET,
+                               mat(ws.Qbuf, mp), mat(ws.R2inv, b),
+                               mixed ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
                                nullptr, 0));
-    pt.end(ph, i > 0 ? 5 : 3);
+    if (mixed) {
+      HIPCHK(rbrp_launch_store_q32(st, mp, b, i, ws.st, ws.Q64, mp, ws.Aw32, mp));
+      ++perm_launches;
+    }
+    pt.end(ph, perm_launches);
 
     // ---- trailing projection and update ----------------------------------
     const int64_t nrem_max = n - i - 1;
@@ -487,21 +536,37 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
-      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc);
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16);
       ph_proj = pt.begin(RBRP_PHASE_PROJECT);
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max, mp,
-                                 ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt), ws.P, b,
-                                 (int64_t)b * nrem_max));
+      if (mixed)
+        HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
+                                     mp, ns, kc, ws.Aw32, i, nullptr, ws.Aw32, i, d_bt, mp, ws.P, b,
+                                     (int64_t)b * nrem_max));
+      else
+        HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max, mp,
+                                   ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt), ws.P,
+                                   b, (int64_t)b * nrem_max));
       pt.end(ph_proj, 1);
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
-      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
-                                       ws.P, b, (int64_t)b * nrem_max, ns,
-                                       mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
+      if (mixed)
+        HIPCHK(rbrp_launch_reduce_splits_f32copy(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b,
+                                                 nrem_max, ws.P, b, (int64_t)b * nrem_max, ns,
+                                                 mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt32,
+                                                 nrem_max));
+      else
+        HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b,
+                                         nrem_max, ws.P, b, (int64_t)b * nrem_max, ns,
+                                         mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
       ph_upd = pt.begin(RBRP_PHASE_UPDATE);
-      HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
-                                    b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
-                                    ws.normpart, nrem_max));
+      if (mixed)
+        HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
+                                        b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
+                                        d_bt, ws.normpart, nrem_max));
+      else
+        HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
+                                      b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
+                                      ws.normpart, nrem_max));
       pt.end(ph_upd, 1);
       ph = pt.begin(RBRP_PHASE_RESID);
       HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, ntiles, i, n, ws.st, ws.diags));
@@ -528,8 +593,8 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
     if (h->timing) {
       // algorithmic work, now that the block's sizes are known
       pt.set_work(ph_cgs, i > 0 ? 4.0 * m * i * s : 0.0, i > 0 ? 2.0 * 8.0 * m * i + 16.0 * m * s : 0.0);
-      pt.set_work(ph_proj, 2.0 * m * bt * nrem, 8.0 * (m * nrem + m * bt + bt * nrem));
-      pt.set_work(ph_upd, 2.0 * m * bt * nrem, 8.0 * (2.0 * m * nrem + m * bt + bt * nrem));
+      pt.set_work(ph_proj, 2.0 * m * bt * nrem, ew * (m * nrem + m * bt) + 8.0 * bt * nrem);
+      pt.set_work(ph_upd, 2.0 * m * bt * nrem, ew * (2.0 * m * nrem + m * bt + bt * nrem));
     }
     pt.flush();
     res->draws = hs.draws;
@@ -561,6 +626,20 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
   pt.flush();
   res->t_total = now_s() - t_start;
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                 const rbrp_opts_t* opts, rbrp_result_t* res) {
+  return colid_impl(h, dA, false, m, n, lda, opts, res);
+}
+
+int rbrp_colid_s(rbrp_handle_t h, const float* dA, int64_t m, int64_t n, int64_t lda,
+                 const rbrp_opts_t* opts, rbrp_result_t* res) {
+  return colid_impl(h, dA, true, m, n, lda, opts, res);
 }
 
 }  // extern "C"

@@ -65,6 +65,12 @@ hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
                                      MatRef dst, int mode, double* dst2 = nullptr, int64_t ld2 = 0);
 
+// As rbrp_launch_reduce_splits, with the row-major copy stored in FP32.
+hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax,
+                                             int64_t Nmax, const double* P, int64_t ldp,
+                                             int64_t pstride, int nsplit, MatRef dst, int mode,
+                                             float* dst2, int64_t ld2);
+
 // NN product with update: C(r, c) = A(r,:) B(:,c) (GEMM_SET), C -= A B (GEMM_SUB) or
 // C += A B (GEMM_ADD), rows r < M, columns c < N, inner dimension K (any size),
 // grid sized by Mmax x Nmax.  If normpart != nullptr, writes
@@ -99,7 +105,8 @@ hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_d
                                 const int* cols);
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
-                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout);
+                                      double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                      int* Jkpos = nullptr);
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
                              RbrpBlockState* bs, double* R2, double* R2inv);
 hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
@@ -122,6 +129,42 @@ hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, cons
                                  int64_t ldr, double* X, int64_t ldx);
 hipError_t rbrp_launch_uniforms(hipStream_t st, uint64_t key0, uint64_t key1, const uint64_t ctr[4],
                                 uint64_t draw0, int64_t count, double* out);
+
+// ---- k_mixed.hip (FP32-stored residual) ------------------------------------
+// Aw (mp x n, FP32) = fp32 copy of A (FP64, or FP32 when a_is_f32), zero pad
+// rows; diags[j] = squared norm of the stored column, summed in FP64.
+hipError_t rbrp_launch_copy_colnorm32(hipStream_t st, const void* A, int a_is_f32, int64_t lda,
+                                      int64_t m, int64_t mp, int64_t n, float* Aw, double* diags);
+// FP64 working copy (and norms) of an FP32 input.
+hipError_t rbrp_launch_copy_colnorm_from32(hipStream_t st, const float* A, int64_t lda, int64_t m,
+                                           int64_t mp, int64_t n, double* Aw, double* diags);
+// V(:, p) = fp64(Aw(:, J[p])), p < bs->s  /  Aw(:, J[p]) = fp32(V(:, p)).
+hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                  const float* Aw, int64_t ld, const int* J, double* V, int64_t ldv);
+hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                   const double* V, int64_t ldv, const int* J, float* Aw, int64_t ld);
+hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
+                                   float* Aw, int64_t ld, const int* Jin, const int* Jout);
+// Aw(:, i + c) = fp32(Q(:, i + c)) for c < bs->bt.
+hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
+                                 const RbrpBlockState* bs, const double* Q, int64_t ldq, float* Aw,
+                                 int64_t ld);
+// Split-K TN product of FP32 operands with FP64 partials (see k_gemm_tn);
+// A's and B's first columns are acoff + *a_dyn and bcoff + *b_dyn (dyn
+// optional), common leading dimension ld (multiple of 4); K and kchunk are
+// multiples of 32.
+hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                                 int64_t K, int nsplit, int64_t kchunk, const float* A,
+                                 int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,
+                                 const int* b_dyn, int64_t ld, double* P, int64_t ldp,
+                                 int64_t pstride);
+// FP32 trailing update with FP64 per-128-row-tile norm partials (see
+// rbrp_launch_gemm_trail); M, lda and ldc must be even.
+hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
+                                    int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
+                                    const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
+                                    int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
+                                    double* normpart, int64_t ldnp);
 
 // ---- k_sketch.hip ---------------------------------------------------------
 hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,

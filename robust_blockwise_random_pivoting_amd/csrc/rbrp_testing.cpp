@@ -63,6 +63,44 @@ int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const doub
   return 0;
 }
 
+int rbrp_test_gemm_tn32(void* stream, int64_t M, int64_t N, int64_t K, const float* A,
+                        const float* B, int64_t ld, double* C, int64_t ldc, int nsplit) {
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K % 32) return -4;
+  if (ld < std::max<int64_t>(K, 1) || ld % 4) return -7;
+  if (nsplit < 1) return -10;
+  if (M == 0 || N == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t kc = (K + nsplit - 1) / nsplit;
+  kc = std::max<int64_t>(32, (kc + 31) / 32 * 32);
+  const int ns = (int)std::max<int64_t>(1, (K + kc - 1) / kc);
+  double* P = nullptr;
+  HIPCHK(hipMalloc(&P, sizeof(double) * M * N * ns));
+  hipError_t e = rbrp_launch_gemm_tn32(st, dim_static(M), dim_static(N), M, N, K, ns, kc, A, 0,
+                                       nullptr, B, 0, nullptr, ld, P, M, M * N);
+  if (e == hipSuccess)
+    e = rbrp_launch_reduce_splits(st, dim_static(M), dim_static(N), M, N, P, M, M * N, ns,
+                                  mat(C, ldc), GEMM_SET);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(P);
+  return (int)e;
+}
+
+int rbrp_test_gemm_upd32(void* stream, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
+                         const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* normpart) {
+  if (M < 0 || (M & 1)) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K > 128) return -4;
+  if ((lda & 1) || lda < std::max<int64_t>(M, 1)) return -6;
+  if ((ldc & 1) || ldc < std::max<int64_t>(M, 1)) return -10;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(rbrp_launch_gemm_trail32(st, M, dim_static(N), N, dim_static(K), K, A, lda, 0, nullptr, Bt,
+                                  ldbt, C, ldc, 0, nullptr, normpart, N));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int filter, int64_t i,
                     const int32_t* J, int32_t* Jk, double* Rb, int32_t* Jin, int32_t* Jout,
                     int32_t* bt, int32_t* nswap, int32_t* status) {

@@ -29,10 +29,14 @@ def _torch():
     return torch
 
 
-def colmajor(A):
-    """A column-major (stride (1, ld)) FP64 device view of ``A`` (copies only if needed)."""
+def colmajor(A, keep_fp32: bool = False):
+    """A column-major (stride (1, ld)) FP64 device view of ``A`` (copies only if needed).
+
+    With ``keep_fp32`` an FP32 device matrix stays FP32."""
     torch = _torch()
-    if not (A.is_cuda and A.dtype == torch.float64):
+    if keep_fp32 and A.is_cuda and A.dtype == torch.float32:
+        pass
+    elif not (A.is_cuda and A.dtype == torch.float64):
         A = A.to(device="cuda", dtype=torch.float64)
     m, n = A.shape
     if A.stride(0) == 1 and (n <= 1 or A.stride(1) >= max(m, 1)):
@@ -115,9 +119,15 @@ class ColID:
 def rbrp_colid(A, b: int = 64, tol: float = 0.0, seed: int = 0, offset: int = 0, max_rank: int = 0,
                filter: str = "sampled", precision: str = "fp64", want_x: bool = True,
                handle: Optional[Handle] = None) -> ColID:
-    """RBRP column ID of the device matrix ``A`` (m x n FP64); see the module docstring."""
+    """RBRP column ID of the device matrix ``A``; see the module docstring.
+
+    ``A`` is m x n FP64, or FP32 (kept in FP32 and passed to ``rbrp_colid_s``).
+    ``precision`` is "fp64" (the reference's arithmetic) or "fp32" (FP32-stored
+    residual with FP32 MFMA projection / update; FP64 sampled-block
+    factorisation, re-orthogonalisation, stopping sum and solve)."""
     torch = _torch()
-    A = colmajor(A)
+    A = colmajor(A, keep_fp32=True)
+    fn = "rbrp_colid_s" if A.dtype == torch.float32 else "rbrp_colid_d"
     m, n = A.shape
     own = handle is None
     h = Handle() if own else handle
@@ -142,9 +152,9 @@ def rbrp_colid(A, b: int = 64, tol: float = 0.0, seed: int = 0, offset: int = 0,
         res.ranks = ranks.ctypes.data
         res.error = error.ctypes.data
         res.t_ranks = t_ranks.ctypes.data
-        code = h._lib.rbrp_colid_d(h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A),
+        code = getattr(h._lib, fn)(h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A),
                                    ctypes.byref(opts), ctypes.byref(res))
-        _capi.check("rbrp_colid_d", code)
+        _capi.check(fn, code)
         k = int(res.k)
         X = xbuf[: n - k, :k].t() if want_x else None
         nb = int(res.nblocks)
