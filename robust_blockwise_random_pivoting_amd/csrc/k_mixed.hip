@@ -164,13 +164,26 @@ hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
 // ---------------------------------------------------------------------------
 // Split-K TN in FP32: P[z](r, c) = sum_{kk in chunk z} A(kk, r) B(kk, c) with
 // FP32 MFMA products and accumulation, stored as FP64 partials (the split sum
-// then runs in FP64).  A and B are column-major FP32 with a common leading
-// dimension; their first columns are acoff + *a_dyn and bcoff + *b_dyn.
-// 64x64 tiles, 4 waves in 2x2, 32-deep K tiles through LDS: each thread
-// loads two float4 (8 consecutive rows of one column), so a column segment
-// is one 128-byte run.  kchunk is a multiple of 32 and rows >= K are zero.
+// then runs in FP64).  A (the orthonormal block) and B (the trailing matrix)
+// are column-major FP32 with a common leading dimension; their first columns
+// are acoff + *a_dyn and bcoff + *b_dyn.
+//
+// Workgroup tile: 64 rows (blockIdx.y) x 128 columns; wave w owns columns
+// 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row blocks of
+// A.  Each 32-row step of both operands is staged through double-buffered
+// LDS ([column][row], one 128-byte run per column and step): the next step's
+// global loads are issued into staging registers before the current step's
+// MFMAs and written to LDS after them, so the only wait on global memory
+// comes after the MFMAs (one barrier per step).  Lane (j, g) reads rows
+// 8g .. 8g+7 of its column as two ds_read_b128, and MFMA q of the step
+// consumes element q: the eight MFMAs cover the 32 rows in the order 8g + q.
+// The MFMA computes the transposed block (B's columns as its rows), so lane
+// l holds P(r0 + l%16, c0 + 4(l/16) + v) and each partial store instruction
+// writes four 128-byte runs of P.  kchunk and K are multiples of 32; rows >=
+// the true height are zero.
 // ---------------------------------------------------------------------------
-#define KT32 32
+#define TN32_KS 32
+#define TN32_LD (TN32_KS + 4)  // LDS pitch (floats) of one staged column
 
 __global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64_t K,
                                                    int64_t kchunk, const float* __restrict__ A0,
@@ -180,70 +193,103 @@ __global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64
                                                    double* __restrict__ P, int64_t ldp,
                                                    int64_t pstride) {
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t m0 = (int64_t)blockIdx.y * 64, n0 = (int64_t)blockIdx.x * 64;
-  if (m0 >= M || n0 >= N) return;
+  const int64_t n0 = (int64_t)blockIdx.x * 128, m0 = (int64_t)blockIdx.y * 64;
+  if (m0 >= M || n0 >= N) return;  // uniform per workgroup
   const int64_t kb0 = (int64_t)blockIdx.z * kchunk;
   const int64_t kb1 = min(K, kb0 + kchunk);
+  if (kb0 >= kb1) return;  // empty chunk (uniform); its partials are never read
 
-  __shared__ float As[64 * (KT32 + 4)];  // [col][k]
-  __shared__ float Bs[64 * (KT32 + 4)];
+  __shared__ float As[2][64 * TN32_LD];   // [buf][col][row]
+  __shared__ float Bs[2][128 * TN32_LD];
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int wm = wid >> 1, wn = wid & 1;
-  const int lc = tid >> 2, kq = (tid & 3) * 8;
+  const int lj = lane & 15, lg = lane >> 4;
   const float* A = A0 + (acoff + (a_dyn ? *a_dyn : 0)) * ld;
   const float* B = B0 + (bcoff + (b_dyn ? *b_dyn : 0)) * ld;
-  const float* ap = (m0 + lc < M) ? A + (m0 + lc) * ld : nullptr;
-  const float* bp = (n0 + lc < N) ? B + (n0 + lc) * ld : nullptr;
+  // Stagers.  Out-of-range columns are clamped to the last valid one: they
+  // only feed output rows / columns that are never stored.
+  const int ac = tid >> 2, aq = (tid & 3) * 8;    // A: column ac, rows aq .. aq+7
+  const int bc = tid >> 1, bq = (tid & 1) * 16;   // B: column bc, rows bq .. bq+15
+  const float* ap = A + min(m0 + ac, M - 1) * ld + aq;
+  const float* bp = B + min(n0 + bc, N - 1) * ld + bq;
+  const int aoff = ac * TN32_LD + aq, boff = bc * TN32_LD + bq;
 
-  float4 ra[2], rb[2];
-  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  v4f acc[2][2];
+  v4f acc[4][2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-  auto load = [&](int64_t kb) {
+  // staging registers (named scalars: small arrays here get placed in LDS)
+  float4 sa0 = *(const float4*)(ap + kb0), sa1 = *(const float4*)(ap + kb0 + 4);
+  float4 sb0 = *(const float4*)(bp + kb0), sb1 = *(const float4*)(bp + kb0 + 4);
+  float4 sb2 = *(const float4*)(bp + kb0 + 8), sb3 = *(const float4*)(bp + kb0 + 12);
+  *(float4*)&As[0][aoff] = sa0;
+  *(float4*)&As[0][aoff + 4] = sa1;
+  *(float4*)&Bs[0][boff] = sb0;
+  *(float4*)&Bs[0][boff + 4] = sb1;
+  *(float4*)&Bs[0][boff + 8] = sb2;
+  *(float4*)&Bs[0][boff + 12] = sb3;
+  __syncthreads();
+
+  int buf = 0;
+  for (int64_t kb = kb0; kb < kb1; kb += TN32_KS) {
+    const bool more = kb + TN32_KS < kb1;  // uniform
+    if (more) {
+      const int64_t kn = kb + TN32_KS;
+      sa0 = *(const float4*)(ap + kn);
+      sa1 = *(const float4*)(ap + kn + 4);
+      sb0 = *(const float4*)(bp + kn);
+      sb1 = *(const float4*)(bp + kn + 4);
+      sb2 = *(const float4*)(bp + kn + 8);
+      sb3 = *(const float4*)(bp + kn + 12);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the prefetch goes out before the MFMAs
+    const float* as = &As[buf][0];
+    const float* bs = &Bs[buf][0];
+    float bv[2][8];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      ra[h] = ap ? *(const float4*)(ap + kb + kq + 4 * h) : z4;
-      rb[h] = bp ? *(const float4*)(bp + kb + kq + 4 * h) : z4;
+      const float4 x0 = *(const float4*)&bs[(32 * wid + 16 * h + lj) * TN32_LD + 8 * lg];
+      const float4 x1 = *(const float4*)&bs[(32 * wid + 16 * h + lj) * TN32_LD + 8 * lg + 4];
+      bv[h][0] = x0.x; bv[h][1] = x0.y; bv[h][2] = x0.z; bv[h][3] = x0.w;
+      bv[h][4] = x1.x; bv[h][5] = x1.y; bv[h][6] = x1.z; bv[h][7] = x1.w;
     }
-  };
-  if (kb0 < kb1) load(kb0);
-  for (int64_t kb = kb0; kb < kb1; kb += KT32) {
-    __syncthreads();
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      *(float4*)&As[lc * (KT32 + 4) + kq + 4 * h] = ra[h];
-      *(float4*)&Bs[lc * (KT32 + 4) + kq + 4 * h] = rb[h];
+    for (int mb = 0; mb < 4; ++mb) {
+      const float4 a0 = *(const float4*)&as[(16 * mb + lj) * TN32_LD + 8 * lg];
+      const float4 a1 = *(const float4*)&as[(16 * mb + lj) * TN32_LD + 8 * lg + 4];
+      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[0][q], av[q], acc[mb][0], 0, 0, 0);
+        acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[1][q], av[q], acc[mb][1], 0, 0, 0);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the staging stores wait for the loads after the MFMAs
+    if (more) {
+      float* an = &As[buf ^ 1][aoff];
+      float* bn = &Bs[buf ^ 1][boff];
+      *(float4*)an = sa0;
+      *(float4*)(an + 4) = sa1;
+      *(float4*)bn = sb0;
+      *(float4*)(bn + 4) = sb1;
+      *(float4*)(bn + 8) = sb2;
+      *(float4*)(bn + 12) = sb3;
     }
     __syncthreads();
-    if (kb + KT32 < kb1) load(kb + KT32);
-#pragma unroll
-    for (int kk = 0; kk < KT32; kk += 4) {
-      const int ko = kk + (lane >> 4);
-      const float a0 = As[(wm * 32 + (lane & 15)) * (KT32 + 4) + ko];
-      const float a1 = As[(wm * 32 + 16 + (lane & 15)) * (KT32 + 4) + ko];
-      const float b0 = Bs[(wn * 32 + (lane & 15)) * (KT32 + 4) + ko];
-      const float b1 = Bs[(wn * 32 + 16 + (lane & 15)) * (KT32 + 4) + ko];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-    }
+    buf ^= 1;
   }
   double* Pz = P + (int64_t)blockIdx.z * pstride;
 #pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
+  for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t r = m0 + wm * 32 + mi * 16 + 4 * (lane >> 4) + v;
-        const int64_t c = n0 + wn * 32 + ni * 16 + (lane & 15);
-        if (r < M && c < N) Pz[r + c * ldp] = (double)acc[mi][ni][v];
+        const int64_t r = m0 + 16 * mb + lj;
+        const int64_t c = n0 + 32 * wid + 16 * h + 4 * lg + v;
+        if (r < M && c < N) Pz[r + c * ldp] = (double)acc[mb][h][v];
       }
 }
 
@@ -253,8 +299,8 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  const int* b_dyn, int64_t ld, double* P, int64_t ldp,
                                  int64_t pstride) {
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
-  if (kchunk % KT32 != 0 || K % KT32 != 0 || ld % 4 != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
+  if (kchunk % TN32_KS != 0 || K % TN32_KS != 0 || ld % 4 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((Nmax + 127) / 128), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
   hipLaunchKernelGGL(k_gemm_tn32, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff, a_dyn, B, bcoff,
                      b_dyn, ld, P, ldp, pstride);
   return hipGetLastError();
@@ -291,7 +337,7 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
   const int64_t N = dim_eval(dN), K = dim_eval(dK);
   const int64_t c0 = (int64_t)blockIdx.x * 32;
   const int64_t rbase = (int64_t)blockIdx.y * T * 128;
-  if (c0 >= N || rbase >= M) return;  // uniform per workgroup
+  if (c0 >= N || rbase >= M || K <= 0) return;  // uniform per workgroup
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const float* A = A0 + (acoff + (acoff_dyn ? *acoff_dyn : 0)) * lda;
@@ -302,62 +348,77 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
   }
   __syncthreads();
   const int ncol = (int)min((int64_t)32, N - c0);
-  // M is a multiple of 2 (the working matrices have even padded heights), so
-  // a float2 at an even row r < M is in bounds.
-  float2 cvA[2][4], cvB[2][4];
-  const float2 z2 = make_float2(0.f, 0.f);
-  auto loadC = [&](int64_t r0, float2 (&cv)[2][4]) {
-    const int64_t r = r0 + 32 * wid + 2 * li;
+  // Loads are unconditional, with the column clamped to the last valid one
+  // and the row to M - 2 (M is even, rows are read in pairs): the clamped
+  // values only reach results that are not stored, and A's clamped columns
+  // (k >= K) meet zero rows of Ws.
+  const float* ccol[2][4];
 #pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
+  for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int cc = 16 * ib + 4 * lk + v;
-        cv[ib][v] = (cc < ncol && r < M) ? *(const float2*)(C + r + cc * ldc) : z2;
-      }
-  };
-  loadC(rbase, cvA);
+    for (int v = 0; v < 4; ++v) ccol[ib][v] = C + (int64_t)min(16 * ib + 4 * lk + v, ncol - 1) * ldc;
+  const float* acol[KSTEPS];
+#pragma unroll
+  for (int q = 0; q < KSTEPS; ++q) acol[q] = A + min((int64_t)(4 * q + lk), K - 1) * lda;
+
+#define TRAIL32_LOADC(R0, X00, X01, X02, X03, X10, X11, X12, X13)     \
+  {                                                                    \
+    const int64_t rr = min((R0) + 32 * wid + 2 * li, M - 2);          \
+    X00 = *(const float2*)(ccol[0][0] + rr);                           \
+    X01 = *(const float2*)(ccol[0][1] + rr);                           \
+    X02 = *(const float2*)(ccol[0][2] + rr);                           \
+    X03 = *(const float2*)(ccol[0][3] + rr);                           \
+    X10 = *(const float2*)(ccol[1][0] + rr);                           \
+    X11 = *(const float2*)(ccol[1][1] + rr);                           \
+    X12 = *(const float2*)(ccol[1][2] + rr);                           \
+    X13 = *(const float2*)(ccol[1][3] + rr);                           \
+  }
+  float2 a00, a01, a02, a03, a10, a11, a12, a13;  // C of the current tile
+  float2 n00, n01, n02, n03, n10, n11, n12, n13;  // C of the next tile
+  TRAIL32_LOADC(rbase, a00, a01, a02, a03, a10, a11, a12, a13);
   for (int t = 0; t < T; ++t) {
     const int64_t r0 = rbase + (int64_t)t * 128;
     if (r0 >= M) break;  // uniform per workgroup
     const int64_t r = r0 + 32 * wid + 2 * li;
+    const int64_t rc = min(r, M - 2);
     float2 fa[KSTEPS];
 #pragma unroll
-    for (int q = 0; q < KSTEPS; ++q) {
-      const int k = 4 * q + lk;
-      fa[q] = (k < K && r < M) ? *(const float2*)(A + r + (int64_t)k * lda) : z2;
-    }
-    if (t + 1 < T) loadC(r0 + 128, cvB);
-    v4f acc[2][2];  // [jb][ib]
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < KSTEPS; ++q) fa[q] = *(const float2*)(acol[q] + rc);
+    if (t + 1 < T) TRAIL32_LOADC(r0 + 128, n00, n01, n02, n03, n10, n11, n12, n13);
+    v4f acc00 = {0.f, 0.f, 0.f, 0.f}, acc01 = acc00, acc10 = acc00, acc11 = acc00;  // [jb][ib]
 #pragma unroll
     for (int q = 0; q < KSTEPS; ++q) {
       const int k = 4 * q + lk;
       const float b0 = Ws[k * 32 + li], b1 = Ws[k * 32 + 16 + li];
-      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].x, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].x, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].y, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].y, acc[1][1], 0, 0, 0);
+      acc00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].x, acc00, 0, 0, 0);
+      acc01 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].x, acc01, 0, 0, 0);
+      acc10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, fa[q].y, acc10, 0, 0, 0);
+      acc11 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, fa[q].y, acc11, 0, 0, 0);
     }
     // lane (li, lk) holds D[4lk + v][li]: column 16ib + 4lk + v, rows r (jb 0) and r + 1 (jb 1)
+    const bool rok = r < M;
     float nsq[2][4];
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int cc = 16 * ib + 4 * lk + v;
-        float2 nv;
-        nv.x = cvA[ib][v].x - acc[0][ib][v];
-        nv.y = cvA[ib][v].y - acc[1][ib][v];
-        nsq[ib][v] = 0.f;
-        if (cc < ncol && r < M) {
-          *(float2*)(C + r + cc * ldc) = nv;
-          nsq[ib][v] = fmaf(nv.x, nv.x, nv.y * nv.y);
-        }
-      }
+#define TRAIL32_STORE(IB, V, X)                                                 \
+  {                                                                             \
+    const int cc = 16 * (IB) + 4 * lk + (V);                                    \
+    float2 nv;                                                                  \
+    nv.x = X.x - acc0##IB[V];                                                   \
+    nv.y = X.y - acc1##IB[V];                                                   \
+    nsq[IB][V] = 0.f;                                                           \
+    if (cc < ncol && rok) {                                                     \
+      *(float2*)(C + r + (int64_t)cc * ldc) = nv;                               \
+      nsq[IB][V] = fmaf(nv.x, nv.x, nv.y * nv.y);                               \
+    }                                                                           \
+  }
+    TRAIL32_STORE(0, 0, a00);
+    TRAIL32_STORE(0, 1, a01);
+    TRAIL32_STORE(0, 2, a02);
+    TRAIL32_STORE(0, 3, a03);
+    TRAIL32_STORE(1, 0, a10);
+    TRAIL32_STORE(1, 1, a11);
+    TRAIL32_STORE(1, 2, a12);
+    TRAIL32_STORE(1, 3, a13);
+#undef TRAIL32_STORE
     if (NORMS) {
 #pragma unroll
       for (int ib = 0; ib < 2; ++ib)
@@ -376,11 +437,10 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
             (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
       __syncthreads();
     }
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) cvA[ib][v] = cvB[ib][v];
+    a00 = n00; a01 = n01; a02 = n02; a03 = n03;
+    a10 = n10; a11 = n11; a12 = n12; a13 = n13;
   }
+#undef TRAIL32_LOADC
 }
 
 hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,

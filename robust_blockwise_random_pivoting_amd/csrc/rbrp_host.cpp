@@ -129,8 +129,8 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
 // Split-K shape for a TN product of Mmax x Nmax over K rows: aim for about
 // 1024 workgroups, keep the partials within the workspace.
 void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pcap, int* nsplit,
-                  int64_t* kchunk, int64_t align = 16) {
-  const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + 63) / 64);
+                  int64_t* kchunk, int64_t align = 16, int64_t tile_n = 64) {
+  const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + tile_n - 1) / tile_n);
   int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
   ns = std::min<int64_t>(ns, tiles <= 4 ? 256 : 128);
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
@@ -498,13 +498,11 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                  mat(ws.Qbuf, mp), mat(ws.Qbuf, mp), ws.P, b, (int64_t)b * b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G2, b), GEMM_SET));
-      HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.R2, ws.R2inv));
-      // R(i:i+bt, i:i+bt) = R2 Rb (product of two upper-triangular factors)
-      HIPCHK(rbrp_launch_gemm_nn(st, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1), b,
-                                 dim_dyn(0, d_bt, 1), mat(ws.R2, b), mat(ws.Rb, b),
-                                 mat_off(ws.R + i, ldr, i), GEMM_SET, nullptr, 0));
+      // R2, R2^{-1} and R(i:i+bt, i:i+bt) = R2 Rb
+      HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.Rb, ws.R2, ws.R2inv, ws.R + i + i * ldr,
+                               ldr));
     }
-    pt.end(ph, 8);
+    pt.end(ph, b <= 64 ? 7 : 8);
 
     // ---- symmetric-difference permutation -------------------------------
     ph = pt.begin(RBRP_PHASE_PERMUTE);
@@ -536,7 +534,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
-      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16);
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, mixed ? 128 : 64);
       ph_proj = pt.begin(RBRP_PHASE_PROJECT);
       if (mixed)
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,

@@ -107,8 +107,10 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
                                       int* Jkpos = nullptr);
+// G2 = R2^T R2: writes R2, R2inv = R2^{-1} (ld b) and Rout = R2 * Rb (ldr).
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
-                             RbrpBlockState* bs, double* R2, double* R2inv);
+                             RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
+                             double* Rout, int64_t ldr);
 hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
                                   const int* Jin, const int* Jout, int* perm, double* diags);
 hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
