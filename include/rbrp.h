@@ -161,6 +161,53 @@ int rbrp_randlupp_os_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_
 int rbrp_interp_os_d(rbrp_handle_t handle, const double* dY, int64_t l, int64_t n, int64_t ldy,
                      int64_t k, const int32_t* dP, double* dT, int64_t ldt);
 
+/*
+ * Batched column IDs (many independent FP64 problems, e.g. the off-diagonal
+ * blocks of a hierarchical kernel matrix).  No reference counterpart: the
+ * reference runs one matrix per call (gpu/rbrp.hpp:6-8, gpu/driver.cpp:176).
+ * Every problem follows rbrp_colid_d with the shared options (b, tol,
+ * max_rank, filter; opts->seed and opts->stream_offset are replaced by the
+ * problem's own), so a problem gives the same result in any batch.
+ */
+typedef struct {
+  const double* A;        /* device, m x n column-major, leading dimension lda */
+  int64_t m, n, lda;
+  uint64_t seed;          /* Philox key word 0 of this problem's stream */
+  uint64_t stream_offset; /* its initial counter */
+  int32_t* perm;          /* device, n entries: perm[0:k] skeleton, perm[k:n] redundant */
+  double* X;              /* device, k x (n-k) with leading dimension ldx, or NULL */
+  int64_t ldx;            /* >= min(m, n) (>= max_rank when set) when X is given */
+} rbrp_problem_t;
+
+typedef struct {
+  int64_t k;          /* rank found */
+  int64_t nblocks;    /* blocks run */
+  double err_final;   /* non-squared final relative residual */
+  uint64_t draws;     /* uniforms consumed */
+  int32_t status;     /* 0 ok; nonzero: numerical failure of this problem (outputs undefined) */
+  int32_t pad;
+} rbrp_problem_result_t;
+
+/* RBRP_BATCH_PERSISTENT: one persistent launch in which each workgroup takes
+ * whole problems from a device work queue (largest first) and runs their
+ * complete loop; the launch is recorded in a hipGraph and replayed by later
+ * calls with the same launch shape.  Needs n <= 1024 and b <= 32 for every
+ * problem.  RBRP_BATCH_LOOP: one rbrp_colid_d call per problem.
+ * RBRP_BATCH_AUTO: persistent when the limits allow, else the loop. */
+enum { RBRP_BATCH_AUTO = 0, RBRP_BATCH_PERSISTENT = 1, RBRP_BATCH_LOOP = 2 };
+
+/* results: host array of nprob entries.  Returns 0, -i for a bad argument i,
+ * or a HIP error; per-problem numerical failures are reported in results. */
+int rbrp_colid_batched_d(rbrp_handle_t handle, int64_t nprob, const rbrp_problem_t* problems,
+                         const rbrp_opts_t* opts, rbrp_problem_result_t* results, int mode);
+
+/* Autotuner of the persistent launch: runs the batch once per candidate
+ * number of workgroups per compute unit, keeps the fastest for later calls of
+ * the same shape on this handle and returns it in *wg_per_cu (results of the
+ * last run are left in the problems' outputs). */
+int rbrp_batched_tune(rbrp_handle_t handle, int64_t nprob, const rbrp_problem_t* problems,
+                      const rbrp_opts_t* opts, int* wg_per_cu);
+
 /* Defaults: b = 64, tol = 0, max_rank = 0, seed = 0, offset = 0, FP64, sampled filter. */
 void rbrp_opts_default(rbrp_opts_t* opts);
 

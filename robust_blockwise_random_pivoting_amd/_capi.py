@@ -35,6 +35,20 @@ class RbrpResult(Structure):
                 ("draws", c_uint64)]
 
 
+class RbrpProblem(Structure):
+    _fields_ = [("A", c_void_p), ("m", c_int64), ("n", c_int64), ("lda", c_int64),
+                ("seed", c_uint64), ("stream_offset", c_uint64), ("perm", c_void_p),
+                ("X", c_void_p), ("ldx", c_int64)]
+
+
+class RbrpProblemResult(Structure):
+    _fields_ = [("k", c_int64), ("nblocks", c_int64), ("err_final", c_double), ("draws", c_uint64),
+                ("status", c_int32), ("pad", c_int32)]
+
+
+BATCH_MODES = {"auto": 0, "persistent": 1, "loop": 2}
+
+
 class RbrpPhaseStats(Structure):
     _fields_ = [("ms", c_double * len(PHASES)), ("launches", c_int64 * len(PHASES)),
                 ("flops", c_double * len(PHASES)), ("bytes", c_double * len(PHASES))]
@@ -68,6 +82,10 @@ def lib():
         "rbrp_uniforms_d": (c_int, [vp, u64, u64, u64, i64, vp]),
         "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
         "rbrp_colid_s": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
+        "rbrp_colid_batched_d": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
+                                         POINTER(RbrpProblemResult), c_int]),
+        "rbrp_batched_tune": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
+                                      POINTER(c_int)]),
         "rbrp_gaussian_sketch_d": (c_int, [vp, u64, u64, i64, i64, c_double, vp, i64]),
         "rbrp_randcpqr_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
                                        i64]),

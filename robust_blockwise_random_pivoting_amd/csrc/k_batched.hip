@@ -1,0 +1,733 @@
+// Batched RBRP column IDs in one persistent launch (gfx950).
+//
+// Each workgroup (4 waves) repeatedly takes the next problem from a device
+// work-queue counter and runs the whole RBRP loop of that problem in its own
+// workspace slot: copy + column norms, then per block sampling, second
+// Gram-Schmidt pass, Gram + pivoted Cholesky + robust filter, CholQR2,
+// symmetric-difference permutation, projection and update with exact
+// residual norms, and at the end X = R11^{-1} R12.  The steps are the ones of
+// the single-problem loop (rbrp_host.cpp), so a problem gives the same
+// skeleton whichever path runs it; all of a problem's arithmetic stays in one
+// workgroup (no launches, no host round trips), and thousands of uneven
+// problems share the chip through the queue.
+//
+// Tall-by-small products use v_mfma_f64_16x16x4f64 on 16 x 16 tiles inside
+// the workgroup (operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16*(i%4), element i/4).  Small dense work (the
+// s x s panel, at most 32 x 32) runs in one wave out of LDS.
+#include <algorithm>
+
+#include "rbrp_batched.h"
+#include "rbrp_kernels.h"
+
+#define BQ_NT 256
+
+// ---------------------------------------------------------------------------
+// LDS carve-up (dynamic, sized from n_max and b at launch)
+// ---------------------------------------------------------------------------
+struct BqLds {
+  double *diags, *w, *cdf;                       // n_max each
+  double *G, *Rb, *Rinv, *G2, *R2, *R2inv, *Rn;  // b x (b + 1) each, [r * ldS + c]
+  double *part;                                  // 4 x 256 (K-split partial tiles)
+  double *dk, *dout;                             // b each
+  double *red;                                   // 8
+  int *J, *Jk, *Jkpos, *Jin, *Jout, *found, *newidx, *isnew, *tk, *to;  // b each
+  int *ired;                                     // 8
+  int *sc;                                       // scalars
+  int ldS;
+};
+
+enum { SC_S = 0, SC_BT, SC_NSWAP, SC_STATUS, SC_NFOUND, SC_PROB, SC_RANK, SC_COUNT };
+
+__host__ __device__ inline size_t bq_lds_bytes(int n_max, int b) {
+  const size_t ldS = (size_t)b + 1;
+  size_t d = 3 * (size_t)n_max + 7 * (size_t)b * ldS + 4 * 256 + 2 * (size_t)b + 8;
+  size_t i = 10 * (size_t)b + 8 + SC_COUNT;
+  return d * sizeof(double) + i * sizeof(int);
+}
+
+__device__ inline BqLds bq_carve(double* base, int n_max, int b) {
+  BqLds L;
+  L.ldS = b + 1;
+  const int sz = b * L.ldS;
+  double* p = base;
+  L.diags = p; p += n_max;
+  L.w = p; p += n_max;
+  L.cdf = p; p += n_max;
+  L.G = p; p += sz;
+  L.Rb = p; p += sz;
+  L.Rinv = p; p += sz;
+  L.G2 = p; p += sz;
+  L.R2 = p; p += sz;
+  L.R2inv = p; p += sz;
+  L.Rn = p; p += sz;
+  L.part = p; p += 4 * 256;
+  L.dk = p; p += b;
+  L.dout = p; p += b;
+  L.red = p; p += 8;
+  int* q = (int*)p;
+  L.J = q; q += b;
+  L.Jk = q; q += b;
+  L.Jkpos = q; q += b;
+  L.Jin = q; q += b;
+  L.Jout = q; q += b;
+  L.found = q; q += b;
+  L.newidx = q; q += b;
+  L.isnew = q; q += b;
+  L.tk = q; q += b;
+  L.to = q; q += b;
+  L.ired = q; q += 8;
+  L.sc = q;
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup MFMA helpers
+// ---------------------------------------------------------------------------
+// Out(r, c) = sum_{k < K} A(k, r) B(k, c) for r < M, c < N.  colA(r) / colB(c)
+// return the FP64 column pointers; K is a multiple of 16 and both operands
+// are zero in rows [true height, K).  With fewer than four 16 x 16 output
+// tiles the K range is split over the waves and the partial tiles are added
+// in wave order (deterministic).  store(r, c, v) receives each entry.
+template <class FA, class FB, class FS>
+__device__ void bq_tn(int M, int N, int K, FA colA, FB colB, double* part, FS store) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int TM = (M + 15) >> 4, TNn = (N + 15) >> 4, T = TM * TNn;
+  if (T == 0) return;  // uniform
+  const int nsplit = T >= 4 ? 1 : 4 / T;
+  const int kc = (((K + nsplit - 1) / nsplit) + 15) & ~15;
+  for (int item = w; item < T * nsplit; item += 4) {
+    const int tile = item / nsplit, pid = item % nsplit;
+    const int r0 = (tile % TM) * 16, c0 = (tile / TM) * 16;
+    const double* pa = colA(min(r0 + li, M - 1));
+    const double* pb = colB(min(c0 + li, N - 1));
+    const int kb = pid * kc, ke = min(K, kb + kc);
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+      const double2 a0 = *(const double2*)(pa + k0 + 4 * lg);
+      const double2 a1 = *(const double2*)(pa + k0 + 4 * lg + 2);
+      const double2 b0 = *(const double2*)(pb + k0 + 4 * lg);
+      const double2 b1 = *(const double2*)(pb + k0 + 4 * lg + 2);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, b0.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, b0.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.x, b1.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, b1.y, acc, 0, 0, 0);
+    }
+    if (nsplit == 1) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = r0 + 4 * v + lg, c = c0 + li;
+        if (r < M && c < N) store(r, c, acc[v]);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) part[w * 256 + lane * 4 + v] = acc[v];
+    }
+  }
+  if (nsplit > 1) {
+    __syncthreads();
+    if (w < T * nsplit && w % nsplit == 0) {
+      const int tile = w / nsplit;
+      const int r0 = (tile % TM) * 16, c0 = (tile / TM) * 16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double s = 0.0;
+        for (int q = 0; q < nsplit; ++q) s += part[(w + q) * 256 + lane * 4 + v];
+        const int r = r0 + 4 * v + lg, c = c0 + li;
+        if (r < M && c < N) store(r, c, s);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// C(r, c) = [C(r, c) -] sum_{k < K} A(r, k) B(k, c) for r < rows, c < N.  The
+// MFMA computes the transposed block, so lane l covers row r0 + l%16 and
+// columns c0 + 4v + l/16: every C access moves four 128-byte runs.  Wave w
+// owns the column tiles w, w + 4, ...; with NORMS, normout(c, sum over rows of
+// the new C(r, c)^2) for each column.  colA(k) / colC(c) give column pointers
+// (rows addressed directly); belem(k, c) gives B.  Rows >= rows are neither
+// read nor written; rows is rounded up to 16 internally only for the loop.
+template <bool SUB, bool NORMS, class FA, class FB, class FC, class FN>
+__device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN normout) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int TNn = (N + 15) >> 4;
+  for (int tn = w; tn < TNn; tn += 4) {
+    const int c0 = tn * 16;
+    double nsq[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int r0 = 0; r0 < rows; r0 += 16) {
+      const int r = r0 + li;
+      const bool rok = r < rows;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + lg;
+        const double bv = (k < K && c0 + li < N) ? belem(k, c0 + li) : 0.0;
+        const double av = (k < K && rok) ? colA(k)[r] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int c = c0 + 4 * v + lg;
+        if (c < N && rok) {
+          double* pc = colC(c) + r;
+          const double nv = SUB ? *pc - acc[v] : acc[v];
+          *pc = nv;
+          nsq[v] = fma(nv, nv, nsq[v]);
+        }
+      }
+    }
+    if (NORMS) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = nsq[v];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        const int c = c0 + 4 * v + lg;
+        if (li == 0 && c < N) normout(c, x);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Weighted sampling without replacement: NumPy's Generator.choice rounds on
+// the Philox stream (key (k0, 0), counter base); the same algorithm as
+// k_sample, for the 256 threads of one workgroup and weights in LDS.
+// Returns s (uniform); sets sc[SC_STATUS] = 4 if a round fails to progress.
+// ---------------------------------------------------------------------------
+__device__ int bq_sample(const BqLds& L, int i, int n, int kmax, int b, uint64_t k0,
+                         const uint64_t base[4], unsigned long long& draws) {
+  const int tid = threadIdx.x;
+  const int N = n - i;
+  int cnt = 0;
+  for (int j = tid; j < N; j += BQ_NT) {
+    double v = L.diags[i + j];
+    v = (v > 0.0) ? v : 0.0;
+    L.w[j] = v;
+    cnt += (v > 0.0) ? 1 : 0;
+  }
+  const int nnz = rbrp_block_sum_int(cnt, L.ired);
+  int s = b;
+  if (kmax - i < s) s = kmax - i;
+  if (nnz < s) s = nnz;
+  if (s <= 0) return 0;
+  if (tid == 0) L.sc[SC_NFOUND] = 0;
+  __syncthreads();
+  const int chunk = (N + BQ_NT - 1) / BQ_NT;
+  const int j0 = min(N, tid * chunk), j1 = min(N, j0 + chunk);
+  for (int round = 0; round <= s; ++round) {
+    const int nf = L.sc[SC_NFOUND];
+    if (nf >= s) break;  // uniform
+    const int r = s - nf;
+    for (int t = tid; t < nf; t += BQ_NT) L.w[L.found[t]] = 0.0;
+    __syncthreads();
+    double loc = 0.0;
+    for (int j = j0; j < j1; ++j) loc += L.w[j];
+    // exclusive scan of the chunk sums over the workgroup (wave scan + wave offsets)
+    const int lane = tid & 63, wid = tid >> 6;
+    double x = loc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) L.red[wid] = x;
+    __syncthreads();
+    double run = ex;
+    for (int q = 0; q < wid; ++q) run += L.red[q];
+    for (int j = j0; j < j1; ++j) {
+      run += L.w[j];
+      L.cdf[j] = run;
+    }
+    __syncthreads();
+    const double total = L.cdf[N - 1];
+    if (tid < r) {
+      const double u = rbrp_u64_to_unit(rbrp_philox_word(base, k0, 0, draws + tid));
+      int lo = 0, hi = N;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (L.cdf[mid] / total > u)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      if (lo >= N) lo = N - 1;
+      L.newidx[tid] = lo;
+    }
+    __syncthreads();
+    if (tid < r) {
+      const int v = L.newidx[tid];
+      int dup = 0;
+      for (int q = 0; q < nf; ++q) dup |= (L.found[q] == v) ? 1 : 0;
+      for (int q = 0; q < tid; ++q) dup |= (L.newidx[q] == v) ? 1 : 0;
+      L.isnew[tid] = !dup;
+    }
+    __syncthreads();
+    if (tid < r && L.isnew[tid]) {
+      int pos = nf;
+      for (int q = 0; q < tid; ++q) pos += L.isnew[q];
+      L.found[pos] = L.newidx[tid];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int add = 0;
+      for (int q = 0; q < r; ++q) add += L.isnew[q];
+      L.sc[SC_NFOUND] = nf + add;
+    }
+    draws += (unsigned long long)r;
+    __syncthreads();
+  }
+  if (L.sc[SC_NFOUND] < s) {
+    if (tid == 0) L.sc[SC_STATUS] = 4;
+    __syncthreads();
+    return 0;
+  }
+  for (int t = tid; t < s; t += BQ_NT) L.J[t] = i + L.found[t];
+  __syncthreads();
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// One-wave panel routines on LDS matrices [r * ldS + c] of at most 32 x 32.
+// ---------------------------------------------------------------------------
+// X = U^{-1} (n x n upper triangular, U, X with pitch ld; X's lower part 0).
+__device__ inline void bq_upper_inverse(const double* U, double* X, int n, int ld) {
+  const int t = threadIdx.x;  // lane t: column t
+  if (t < 64) {
+    for (int r = 0; r < n; ++r)
+      if (t < n) X[r * ld + t] = 0.0;
+    if (t < n) X[t * ld + t] = 1.0 / U[t * ld + t];
+    for (int r = n - 2; r >= 0; --r) {
+      if (t < n && t > r) {
+        double acc = 0.0;
+        for (int l = r + 1; l <= t; ++l) acc = fma(U[r * ld + l], X[l * ld + t], acc);
+        X[r * ld + t] = -acc / U[r * ld + r];
+      }
+    }
+  }
+}
+
+// Pivoted Cholesky of the s x s Gram matrix G (dgeqp3 pivot rule: largest
+// Schur diagonal, lowest position on ties; the column at the pivot's position
+// takes the vacated one), robust filter, kept factor Rb and its inverse, and
+// the symmetric-difference lists.  Wave 0 works; every thread must call.
+__device__ void bq_pivchol(const BqLds& L, int s, int b, int filter_mode, int i) {
+  const int c = threadIdx.x;
+  const int ld = L.ldS;
+  double* R = L.R2;  // scratch factor rows R[t][col] (col = sampled index)
+  __shared__ int s_pos2col[64];
+  __shared__ double s_sqr[64];
+  if (c < 64) {
+    const bool act = c < s;
+    double d = act ? L.G[c * ld + c] : 0.0;
+    int pos = c;
+    int rank = s;
+    for (int j = 0; j < s; ++j) {
+      double bv = -1.0;
+      int bp = 0x7fffffff, bc = -1;
+      if (act && pos >= j && (d > bv || (d == bv && pos < bp))) {
+        bv = d;
+        bp = pos;
+        bc = c;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int op = __shfl_xor(bp, o, 64);
+        const int oc = __shfl_xor(bc, o, 64);
+        if (ov > bv || (ov == bv && op < bp)) {
+          bv = ov;
+          bp = op;
+          bc = oc;
+        }
+      }
+      if (!(bv > 0.0)) {
+        rank = j;
+        break;
+      }
+      const int P = bc;
+      const double rjj = sqrt(bv);
+      if (pos == j) pos = bp;
+      if (c == P) pos = j;
+      if (act) {
+        double val = 0.0;
+        if (c == P) {
+          val = rjj;
+        } else if (pos > j) {
+          double acc = L.G[P * ld + c];
+          for (int t = 0; t < j; ++t) acc = fma(-R[t * ld + P], R[t * ld + c], acc);
+          val = acc / rjj;
+          d = fma(-val, val, d);
+        }
+        R[j * ld + c] = val;
+      }
+    }
+    for (int j = rank; j < s; ++j)
+      if (act) R[j * ld + c] = 0.0;
+    if (act) s_pos2col[pos] = c;
+    if (c == 0) L.sc[SC_RANK] = rank;
+  }
+  __syncthreads();
+  if (c < s) {
+    double e = 0.0;
+    for (int q = 0; q < s; ++q) e = fma(R[c * ld + q], R[c * ld + q], e);
+    s_sqr[c] = e;
+  }
+  __syncthreads();
+  if (c == 0) {
+    const int rank = L.sc[SC_RANK];
+    double total = 0.0;
+    for (int t = 0; t < s; ++t) total += s_sqr[s - 1 - t];
+    const double thr = total / (double)(filter_mode == 0 ? s : b);
+    double csum = 0.0;
+    int k = 0;
+    for (; k < s; ++k) {
+      csum += s_sqr[s - 1 - k];
+      if (csum >= thr) break;
+    }
+    int bt = s - k;
+    if (rank == 0 || !(total > 0.0)) {
+      bt = 0;
+      L.sc[SC_STATUS] = 1;
+    }
+    if (bt > rank) bt = rank;
+    L.sc[SC_BT] = bt;
+  }
+  __syncthreads();
+  const int bt = L.sc[SC_BT];
+  if (c < bt) {
+    const int col = s_pos2col[c];
+    L.Jkpos[c] = col;
+    L.Jk[c] = L.J[col];
+    for (int r = 0; r < bt; ++r) L.Rb[r * ld + c] = (r <= c) ? R[r * ld + col] : 0.0;
+  }
+  __syncthreads();
+  bq_upper_inverse(L.Rb, L.Rinv, bt, ld);
+  // symmetric difference of the kept set and the block positions [i, i + bt)
+  if (c < bt) {
+    const int v = L.Jk[c];
+    int rk = 0;
+    for (int q = 0; q < bt; ++q) rk += (L.Jk[q] < v) ? 1 : 0;
+    L.found[rk] = v;  // ascending kept indices (reuses the sampler's list)
+    const int q = i + c;
+    int member = 0;
+    for (int a = 0; a < bt; ++a) member |= (L.Jk[a] == q) ? 1 : 0;
+    L.isnew[c] = !member;  // position i + c is vacated by a column that leaves
+  }
+  __syncthreads();
+  if (c == 0) {
+    int nin = 0, nout = 0;
+    for (int a = 0; a < bt; ++a) nin += (L.found[a] >= i + bt) ? 1 : 0;
+    for (int a = 0; a < bt; ++a) {
+      if (L.isnew[a]) L.Jout[nout++] = i + a;
+    }
+    for (int a = 0; a < nin; ++a) L.Jin[a] = L.found[bt - nin + a];
+    if (nin != nout) L.sc[SC_STATUS] = 3;
+    L.sc[SC_NSWAP] = nin;
+  }
+  __syncthreads();
+}
+
+// G2 = R2^T R2 (G2 in L.G2), R2inv, and Rn = R2 * Rb (bt x bt), one wave.
+__device__ void bq_chol2(const BqLds& L, int bt) {
+  const int c = threadIdx.x;
+  const int ld = L.ldS;
+  if (c < 64) {
+    const bool act = c < bt;
+    double d = act ? L.G2[c * ld + c] : 0.0;
+    bool bad = false;
+    for (int j = 0; j < bt; ++j) {
+      const double dj = __shfl(d, j, 64);
+      const double rjj = sqrt(dj > 0.0 ? dj : 1.0);
+      bad |= !(dj > 0.0);
+      if (act) {
+        double val = 0.0;
+        if (c == j) {
+          val = rjj;
+        } else if (c > j) {
+          double acc = L.G2[j * ld + c];
+          for (int t = 0; t < j; ++t) acc = fma(-L.R2[t * ld + j], L.R2[t * ld + c], acc);
+          val = acc / rjj;
+          d = fma(-val, val, d);
+        }
+        L.R2[j * ld + c] = val;
+      }
+    }
+    if (c == 0 && bad) L.sc[SC_STATUS] = 2;
+    if (act) {
+      for (int r = 0; r < bt; ++r) {
+        double acc = 0.0;
+        for (int l = r; l <= c; ++l) acc = fma(L.R2[r * ld + l], L.Rb[l * ld + c], acc);
+        L.Rn[r * ld + c] = (r <= c) ? acc : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  bq_upper_inverse(L.R2, L.R2inv, bt, ld);
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// The persistent kernel
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BQ_NT) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
+                                                         const int* __restrict__ order, int nprob,
+                                                         int* queue, RbrpBatchResult* results,
+                                                         RbrpBatchShape shape) {
+  extern __shared__ double lds_raw[];
+  const BqLds L = bq_carve(lds_raw, shape.n_max, shape.b);
+  const int tid = threadIdx.x;
+  const int b = shape.b, ld = L.ldS;
+  char* slot = (char*)shape.ws + (size_t)blockIdx.x * shape.slot_bytes;
+  double* Aw = (double*)slot;
+  double* R = (double*)(slot + shape.off_R);
+  double* Qbuf = (double*)(slot + shape.off_Q);
+  double* Cbuf = (double*)(slot + shape.off_C);
+  double* Tbuf = (double*)(slot + shape.off_T);
+  const int64_t ldr = shape.ldr, ldc = shape.ldr;
+
+  for (;;) {
+    if (tid == 0) L.sc[SC_PROB] = atomicAdd(queue, 1);
+    __syncthreads();
+    const int qi = L.sc[SC_PROB];
+    __syncthreads();
+    if (qi >= nprob) break;  // uniform: every workgroup leaves once the queue is drained
+    const int p = order[qi];
+    const RbrpBatchProblem P = probs[p];
+    const int m = (int)P.m, n = (int)P.n;
+    const int mp = (m + 15) & ~15;
+    const int pmn = min(m, n);
+    const int kmax = shape.max_rank > 0 ? min(shape.max_rank, pmn) : pmn;
+    if (tid == 0) {
+      L.sc[SC_STATUS] = 0;
+    }
+    for (int j = tid; j < n; j += BQ_NT) P.perm[j] = j;
+    // ---- copy + squared column norms (wave per column) -------------------
+    {
+      const int lane = tid & 63, w = tid >> 6;
+      for (int j = w; j < n; j += 4) {
+        const double* a = P.A + (int64_t)j * P.lda;
+        double* aw = Aw + (int64_t)j * mp;
+        double acc = 0.0;
+        for (int r = lane; r < mp; r += 64) {
+          const double v = r < m ? a[r] : 0.0;
+          aw[r] = v;
+          acc = fma(v, v, acc);
+        }
+        acc = rbrp_wave_sum(acc);
+        if (lane == 0) L.diags[j] = acc;
+      }
+    }
+    __syncthreads();
+    double part0 = 0.0;
+    for (int j = tid; j < n; j += BQ_NT) part0 += L.diags[j];
+    const double N0 = rbrp_block_sum(part0, L.red);
+    const uint64_t base[4] = {P.stream_offset, 0, 0, 0};
+    unsigned long long draws = 0;
+    double err = (N0 > 0.0) ? 1.0 : 0.0;
+    int i = 0, nb = 0;
+    bool bad = !(N0 == N0) || N0 > 1.7e308;
+    while (!bad && i < kmax && N0 > 0.0) {
+      // ---- sample ---------------------------------------------------------
+      const int s = bq_sample(L, i, n, kmax, b, P.seed, base, draws);
+      if (L.sc[SC_STATUS]) {
+        bad = true;
+        break;
+      }
+      if (s <= 0) break;
+      // ---- second Gram-Schmidt pass: C = Q^T V, R(0:i, J) += C, V -= Q C ----
+      if (i > 0) {
+        bq_tn(i, s, mp, [&](int r) { return (const double*)(Aw + (int64_t)r * mp); },
+              [&](int c) { return (const double*)(Aw + (int64_t)L.J[c] * mp); }, L.part,
+              [&](int r, int c, double v) {
+                Cbuf[r + c * ldc] = v;
+                R[r + (int64_t)L.J[c] * ldr] += v;
+              });
+        bq_nn<true, false>(
+            mp, s, i, [&](int k) { return (const double*)(Aw + (int64_t)k * mp); },
+            [&](int k, int c) { return Cbuf[k + c * ldc]; },
+            [&](int c) { return Aw + (int64_t)L.J[c] * mp; }, [&](int, double) {});
+      }
+      // ---- Gram, pivoted Cholesky, filter -------------------------------------
+      bq_tn(s, s, mp, [&](int r) { return (const double*)(Aw + (int64_t)L.J[r] * mp); },
+            [&](int c) { return (const double*)(Aw + (int64_t)L.J[c] * mp); }, L.part,
+            [&](int r, int c, double v) { L.G[r * ld + c] = v; });
+      bq_pivchol(L, s, b, shape.filter, i);
+      const int bt = L.sc[SC_BT];
+      if (L.sc[SC_STATUS] || bt <= 0) {
+        bad = L.sc[SC_STATUS] != 0;
+        break;
+      }
+      // ---- CholQR2: Q1 = V(:, Jk) Rb^{-1}, G2 = Q1^T Q1, R2 -------------------
+      bq_nn<false, false>(
+          mp, bt, bt, [&](int k) { return (const double*)(Aw + (int64_t)L.Jk[k] * mp); },
+          [&](int k, int c) { return L.Rinv[k * ld + c]; },
+          [&](int c) { return Qbuf + (int64_t)c * mp; }, [&](int, double) {});
+      bq_tn(bt, bt, mp, [&](int r) { return (const double*)(Qbuf + (int64_t)r * mp); },
+            [&](int c) { return (const double*)(Qbuf + (int64_t)c * mp); }, L.part,
+            [&](int r, int c, double v) { L.G2[r * ld + c] = v; });
+      bq_chol2(L, bt);
+      if (L.sc[SC_STATUS]) {
+        bad = true;
+        break;
+      }
+      const int nsw = L.sc[SC_NSWAP];
+      // ---- symmetric-difference permutation ----------------------------------
+      if (tid < bt) {
+        L.tk[tid] = P.perm[L.Jk[tid]];
+        L.dk[tid] = L.diags[L.Jk[tid]];
+      }
+      if (tid < nsw) {
+        L.to[tid] = P.perm[L.Jout[tid]];
+        L.dout[tid] = L.diags[L.Jout[tid]];
+      }
+      // R rows 0..i of the kept / leaving columns into Tbuf
+      for (int idx = tid; idx < i * (bt + nsw); idx += BQ_NT) {
+        const int r = idx % i, q = idx / i;
+        const int col = q < bt ? L.Jk[q] : L.Jout[q - bt];
+        Tbuf[r + (int64_t)q * ldr] = R[r + (int64_t)col * ldr];
+      }
+      __syncthreads();
+      if (tid < nsw) {
+        P.perm[L.Jin[tid]] = L.to[tid];
+        L.diags[L.Jin[tid]] = L.dout[tid];
+      }
+      for (int idx = tid; idx < i * nsw; idx += BQ_NT) {
+        const int r = idx % i, q = idx / i;
+        R[r + (int64_t)L.Jin[q] * ldr] = Tbuf[r + (int64_t)(bt + q) * ldr];
+      }
+      // working-matrix columns leaving the block slots move to the vacated ones
+      for (int idx = tid; idx < mp * nsw; idx += BQ_NT) {
+        const int r = idx % mp, q = idx / mp;
+        Aw[r + (int64_t)L.Jin[q] * mp] = Aw[r + (int64_t)L.Jout[q] * mp];
+      }
+      __syncthreads();
+      if (tid < bt) {
+        P.perm[i + tid] = L.tk[tid];
+        L.diags[i + tid] = L.dk[tid];
+      }
+      for (int idx = tid; idx < i * bt; idx += BQ_NT) {
+        const int r = idx % i, q = idx / i;
+        R[r + (int64_t)(i + q) * ldr] = Tbuf[r + (int64_t)q * ldr];
+      }
+      // new diagonal block of R, zeros left of it in its rows
+      for (int idx = tid; idx < bt * bt; idx += BQ_NT) {
+        const int r = idx % bt, q = idx / bt;
+        R[(i + r) + (int64_t)(i + q) * ldr] = L.Rn[r * ld + q];
+      }
+      for (int idx = tid; idx < bt * i; idx += BQ_NT) {
+        const int r = idx % bt, q = idx / bt;
+        R[(i + r) + (int64_t)q * ldr] = 0.0;
+      }
+      __syncthreads();
+      // ---- Q_b = Q1 R2^{-1} into the block's columns ---------------------------
+      bq_nn<false, false>(
+          mp, bt, bt, [&](int k) { return (const double*)(Qbuf + (int64_t)k * mp); },
+          [&](int k, int c) { return L.R2inv[k * ld + c]; },
+          [&](int c) { return Aw + (int64_t)(i + c) * mp; }, [&](int, double) {});
+      // ---- projection W = Q_b^T A_rem into R's block rows, then update -------
+      const int nrem = n - i - bt;
+      if (nrem > 0) {
+        bq_tn(bt, nrem, mp, [&](int r) { return (const double*)(Aw + (int64_t)(i + r) * mp); },
+              [&](int c) { return (const double*)(Aw + (int64_t)(i + bt + c) * mp); }, L.part,
+              [&](int r, int c, double v) { R[(i + r) + (int64_t)(i + bt + c) * ldr] = v; });
+        bq_nn<true, true>(
+            mp, nrem, bt, [&](int k) { return (const double*)(Aw + (int64_t)(i + k) * mp); },
+            [&](int k, int c) { return R[(i + k) + (int64_t)(i + bt + c) * ldr]; },
+            [&](int c) { return Aw + (int64_t)(i + bt + c) * mp; },
+            [&](int c, double v) { L.diags[i + bt + c] = v; });
+      }
+      double pr = 0.0;
+      for (int j = i + bt + tid; j < n; j += BQ_NT) pr += L.diags[j];
+      const double resid = rbrp_block_sum(pr, L.red);
+      err = resid / N0;
+      i += bt;
+      ++nb;
+      if (err <= shape.tol2) break;
+    }
+    const int k = i;
+    // ---- X = R11^{-1} R12 ------------------------------------------------------
+    if (!bad && P.X && k > 0 && n > k) {
+      const int nc = n - k;
+      for (int idx = tid; idx < k * nc; idx += BQ_NT) {
+        const int r = idx % k, c = idx / k;
+        P.X[r + (int64_t)c * P.ldx] = R[r + (int64_t)(k + c) * ldr];
+      }
+      __syncthreads();
+      for (int rb = ((k - 1) / 16) * 16; rb >= 0; rb -= 16) {
+        const int re = min(rb + 16, k);
+        if (re < k) {  // X[rb:re, :] -= R[rb:re, re:k] X[re:k, :]
+          bq_nn<true, false>(
+              re - rb, nc, k - re,
+              [&](int kk) { return (const double*)(R + rb + (int64_t)(re + kk) * ldr); },
+              [&](int kk, int c) { return P.X[(re + kk) + (int64_t)c * P.ldx]; },
+              [&](int c) { return P.X + rb + (int64_t)c * P.ldx; }, [&](int, double) {});
+        }
+        // the diagonal triangle, one column per thread
+        for (int c = tid; c < nc; c += BQ_NT) {
+          double* xc = P.X + (int64_t)c * P.ldx;
+          double x[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) x[q] = (rb + q < re) ? xc[rb + q] : 0.0;
+#pragma unroll
+          for (int q = 15; q >= 0; --q) {
+            if (rb + q < re) {
+              double v = x[q];
+#pragma unroll
+              for (int l = q + 1; l < 16; ++l)
+                if (rb + l < re) v = fma(-R[(rb + q) + (int64_t)(rb + l) * ldr], x[l], v);
+              x[q] = v / R[(rb + q) + (int64_t)(rb + q) * ldr];
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < 16; ++q)
+            if (rb + q < re) xc[rb + q] = x[q];
+        }
+        __syncthreads();
+      }
+    }
+    if (tid == 0) {
+      RbrpBatchResult res;
+      res.k = k;
+      res.nblocks = nb;
+      res.err_final = sqrt(err > 0.0 ? err : 0.0);
+      res.draws = draws;
+      res.status = bad ? (L.sc[SC_STATUS] ? L.sc[SC_STATUS] : 5) : 0;
+      res.pad = 0;
+      results[p] = res;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_queue_reset(int* queue) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0;
+}
+
+hipError_t rbrp_launch_colid_batched(hipStream_t st, int grid, const RbrpBatchProblem* probs,
+                                     const int* order, int nprob, int* queue,
+                                     RbrpBatchResult* results, const RbrpBatchShape& shape) {
+  if (grid <= 0 || nprob <= 0) return hipSuccess;
+  if (shape.b < 1 || shape.b > RBRP_BATCH_MAXB || shape.n_max > RBRP_BATCH_MAXN)
+    return hipErrorInvalidValue;
+  const size_t lds = rbrp_batched_lds_bytes(shape.n_max, shape.b);
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, queue);
+  hipLaunchKernelGGL(k_colid_batched, dim3((unsigned)grid), dim3(BQ_NT), lds, st, probs, order, nprob,
+                     queue, results, shape);
+  return hipGetLastError();
+}
+
+size_t rbrp_batched_lds_bytes(int n_max, int b) { return bq_lds_bytes(n_max, b); }
+
+hipError_t rbrp_batched_prepare(int n_max, int b) {
+  return hipFuncSetAttribute((const void*)k_colid_batched,
+                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bq_lds_bytes(n_max, b));
+}

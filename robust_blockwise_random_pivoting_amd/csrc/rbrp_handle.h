@@ -34,7 +34,23 @@ struct rbrp_handle_s {
   bool timing = false;
   rbrp_phase_stats_t stats;
   std::vector<hipEvent_t> events;
+  // batched calls: slot workspace, problem metadata, pinned staging, a
+  // private stream for graph capture, the recorded launch and the tuner's pick
+  void* bt_ws = nullptr;
+  size_t bt_ws_bytes = 0;
+  void* bt_meta = nullptr;
+  size_t bt_meta_bytes = 0;
+  void* bt_host = nullptr;
+  size_t bt_host_bytes = 0;
+  hipStream_t bt_stream = nullptr;
+  hipGraph_t bt_graph = nullptr;
+  hipGraphExec_t bt_exec = nullptr;
+  std::vector<uint64_t> bt_key;
+  std::vector<std::pair<std::vector<uint64_t>, int>> bt_tuned;  // shape key -> workgroups per CU
 };
+
+// Releases the batched-call resources of a handle (rbrp_batched.cpp).
+void rbrp_batched_release(rbrp_handle_t h);
 
 // Grow-only device buffer of the handle for the sketch comparators.
 hipError_t rbrp_sketch_arena(rbrp_handle_t h, size_t bytes, void** out);

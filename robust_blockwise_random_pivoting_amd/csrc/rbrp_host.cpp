@@ -300,6 +300,7 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
 
 int rbrp_destroy(rbrp_handle_t h) {
   if (!h) return -1;
+  rbrp_batched_release(h);
   if (h->ws.arena) (void)hipFree(h->ws.arena);
   if (h->sk_arena) (void)hipFree(h->sk_arena);
   if (h->host_state) (void)hipHostFree(h->host_state);
@@ -341,6 +342,7 @@ int rbrp_release_workspace(rbrp_handle_t h) {
   }
   h->ws.arena = nullptr;
   h->ws.arena_bytes = 0;
+  rbrp_batched_release(h);
   return 0;
 }
 
@@ -355,6 +357,11 @@ int rbrp_uniforms_d(rbrp_handle_t h, uint64_t seed, uint64_t offset, uint64_t dr
 }
 
 }  // extern "C"
+
+// The block loop for both precisions (also the per-problem path of the
+// batched entry point).
+int rbrp_colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n, int64_t lda,
+                    const rbrp_opts_t* opts, rbrp_result_t* res);
 
 namespace {
 
@@ -627,6 +634,11 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
 }
 
 }  // namespace
+
+int rbrp_colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n, int64_t lda,
+                    const rbrp_opts_t* opts, rbrp_result_t* res) {
+  return colid_impl(h, dA, a_f32, m, n, lda, opts, res);
+}
 
 extern "C" {
 

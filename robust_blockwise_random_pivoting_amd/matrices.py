@@ -114,3 +114,28 @@ def kernel_blocks(count: int, seed: int, m_range=(64, 1024), n_range=(64, 512)):
     ms = rng.integers(m_range[0], m_range[1] + 1, size=count)
     ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
     return [kernel_block(int(a), int(c), rng) for a, c in zip(ms, ns)]
+
+
+def kernel_block_torch(m: int, n: int, gen, sigma: float = 0.5, shift: float = 4.0,
+                       device="cuda"):
+    """``kernel_block`` generated on the device from the torch generator ``gen``;
+    returns a column-major (m, n) view."""
+    import torch
+    x = torch.rand((m, 3), generator=gen, dtype=torch.float64, device=device)
+    y = torch.rand((n, 3), generator=gen, dtype=torch.float64, device=device)
+    y[:, 0] += shift
+    d2 = ((y[:, None, :] - x[None, :, :]) ** 2).sum(-1)      # (n, m) row-major == (m, n) col-major
+    return torch.exp(-d2 / (2.0 * sigma * sigma)).t()
+
+
+def kernel_blocks_torch(count: int, seed: int, m_range=(64, 1024), n_range=(64, 512),
+                        device="cuda"):
+    """``count`` device kernel blocks with ``m ~ U{m_range}``, ``n ~ U{n_range}`` (seeded)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(m_range[0], m_range[1] + 1, size=count)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=count)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    return [kernel_block_torch(int(a), int(c), gen, device=device) for a, c in zip(ms, ns)]
+

@@ -184,6 +184,90 @@ def RBRP(m: int, n: int, A, blk: int, seed: int = 0, offset: int = 0):
             r.t_total, flops)
 
 
+def _batch_setup(As, b, tol, seeds, offsets, max_rank, filter, want_x):
+    """Descriptors and output buffers of a batch (keeps the tensors alive in the return)."""
+    torch = _torch()
+    nprob = len(As)
+    seeds = list(range(nprob)) if seeds is None else list(seeds)
+    offsets = [0] * nprob if offsets is None else list(offsets)
+    probs = (_capi.RbrpProblem * max(nprob, 1))()
+    keep, perms, xbufs = [], [], []
+    for q, A in enumerate(As):
+        A = colmajor(A)
+        m, n = A.shape
+        pmn = min(m, n)
+        kmax = min(max_rank, pmn) if max_rank else pmn
+        ldx = max(kmax, 1)
+        perm = torch.empty(max(n, 1), dtype=torch.int32, device=A.device)
+        xb = torch.empty((max(n, 1), ldx), dtype=torch.float64, device=A.device) if want_x else None
+        keep.append(A)
+        perms.append(perm)
+        xbufs.append(xb)
+        probs[q].A = A.data_ptr()
+        probs[q].m, probs[q].n, probs[q].lda = m, n, leading_dim(A)
+        probs[q].seed, probs[q].stream_offset = int(seeds[q]), int(offsets[q])
+        probs[q].perm = perm.data_ptr()
+        probs[q].X = xb.data_ptr() if want_x else None
+        probs[q].ldx = ldx
+    opts = _capi.RbrpOpts(b=int(b), tol=float(tol), max_rank=int(max_rank), seed=0, stream_offset=0,
+                          precision=0, filter=_capi.FILTERS[filter])
+    return probs, opts, keep, perms, xbufs
+
+
+def rbrp_colid_batched(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None,
+                       max_rank: int = 0, filter: str = "sampled", mode: str = "auto",
+                       want_x: bool = True, handle: Optional[Handle] = None):
+    """Batched RBRP column IDs of the device FP64 matrices ``As`` (a list).
+
+    Problem q uses the Philox stream (seeds[q], offsets[q]) (default (q, 0)); the
+    other options are shared.  ``mode``: "persistent" (one persistent launch
+    with a device work queue, replayed from a hipGraph), "loop" (one
+    ``rbrp_colid_d`` per problem) or "auto".  Returns a list of ``ColID`` (per-
+    block traces empty) and the array of per-problem statuses.
+    """
+    nprob = len(As)
+    probs, opts, keep, perms, xbufs = _batch_setup(As, b, tol, seeds, offsets, max_rank, filter,
+                                                   want_x)
+    res = (_capi.RbrpProblemResult * max(nprob, 1))()
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_colid_batched_d", h._lib.rbrp_colid_batched_d(
+            h.ptr, nprob, probs, ctypes.byref(opts), res, _capi.BATCH_MODES[mode]))
+        h.stream.synchronize()
+    finally:
+        if own:
+            h.close()
+    out = []
+    for q in range(nprob):
+        k, n = int(res[q].k), int(keep[q].shape[1])
+        X = xbufs[q][: n - k, :k].t() if want_x else None
+        out.append(ColID(k=k, perm=perms[q][:n], X=X, ranks=np.zeros(0, np.int64),
+                         error=np.zeros(0), t_ranks=np.zeros(0), t_total=0.0,
+                         err_final=float(res[q].err_final), draws=int(res[q].draws)))
+    status = np.array([int(res[q].status) for q in range(nprob)], dtype=np.int32)
+    return out, status
+
+
+def batched_tune(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None, max_rank: int = 0,
+                 filter: str = "sampled", handle: Optional[Handle] = None) -> int:
+    """Run the persistent batched launch once per candidate workgroups-per-CU and
+    keep the fastest on ``handle`` (returned)."""
+    probs, opts, keep, perms, xbufs = _batch_setup(As, b, tol, seeds, offsets, max_rank, filter,
+                                                   True)
+    pick = ctypes.c_int(0)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_batched_tune", h._lib.rbrp_batched_tune(
+            h.ptr, len(As), probs, ctypes.byref(opts), ctypes.byref(pick)))
+    finally:
+        if own:
+            h.close()
+    del keep, perms, xbufs
+    return int(pick.value)
+
+
 def uniforms(seed: int, offset: int, draw0: int, count: int, handle: Optional[Handle] = None):
     """Uniforms ``draw0 .. draw0+count-1`` of the Philox stream, computed on the device."""
     torch = _torch()

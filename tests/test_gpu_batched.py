@@ -1,0 +1,210 @@
+"""Batched column IDs (one persistent launch with a device work queue) against
+the oracle and against the per-problem path (GPU).
+
+Bar, per problem (problem q uses the Philox stream (q, 0) unless stated):
+* skeleton indices, rank and uniform draws identical to the oracle's;
+* X within 1e-10 relative where kappa(R11) eps allows it, else 100 kappa eps;
+* ID error within 1% of the oracle's;
+* persistent and per-problem launches give identical skeletons and ranks.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import id_error, rbrp_colid as oracle_colid
+from robust_blockwise_random_pivoting_amd.matrices import (exp_spectrum, gmm_rows, kernel_block,
+                                                            spectrum_matrix)
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def _dev(A):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(A.T)).cuda().t()
+
+
+def _kernel_batch(nprob, seed, m_range=(64, 300), n_range=(64, 200)):
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(m_range[0], m_range[1] + 1, size=nprob)
+    ns = rng.integers(n_range[0], n_range[1] + 1, size=nprob)
+    return [kernel_block(int(m), int(n), rng) for m, n in zip(ms, ns)]
+
+
+def _check_against_oracle(A, g, b, tol, seed, offset=0, filt="sampled", max_rank=0, name=""):
+    r = oracle_colid(A, b, tol, seed=seed, offset=offset, filter_mode=filt, max_rank=max_rank)
+    n = A.shape[1]
+    perm = g.perm.cpu().numpy()
+    assert g.k == r.k, name
+    assert np.array_equal(perm[: g.k], r.perm[: r.k]), name
+    assert np.array_equal(np.sort(perm), np.arange(n)), name
+    assert g.draws == r.draws, name
+    if g.X is not None and 0 < g.k < n:
+        X = g.X.cpu().numpy()
+        kappa = np.linalg.cond(r.R[:, : r.k])
+        rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
+        assert rel <= max(1e-10, 100 * kappa * EPS), (name, rel, kappa)
+        eg = id_error(A, perm, g.k, X)
+        er = id_error(A, r.perm, r.k, r.X)
+        assert abs(eg - er) <= 0.01 * er + 1e-15, (name, eg, er)
+    return r
+
+
+def test_batched_persistent_matches_oracle():
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    As = _kernel_batch(24, 11)
+    out, status = rbrp_colid_batched([_dev(A) for A in As], b=16, tol=1e-7, mode="persistent")
+    assert (status == 0).all()
+    for q, A in enumerate(As):
+        _check_against_oracle(A, out[q], 16, 1e-7, seed=q, name=f"kernel {q} {A.shape}")
+
+
+def test_batched_mixed_families_and_options():
+    """Exp-spectrum, full-rank Gaussian-mixture, wide and kernel problems in one
+    batch; own seeds and offsets; nominal filter; b = 24 and 32."""
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    rng = np.random.default_rng(5)
+    As = [spectrum_matrix(300, 200, exp_spectrum(200, 10.0), 5),
+          gmm_rows(400, 120, 7, components=8),
+          spectrum_matrix(120, 340, exp_spectrum(120, 8.0), 6),
+          kernel_block(500, 260, rng),
+          spectrum_matrix(1000, 700, exp_spectrum(700, 30.0), 9)]
+    seeds = [3, 99, 2 ** 40 + 7, 12, 5]
+    offsets = [0, 5, 2 ** 63, 1, 17]
+    for b, tol, filt in [(24, 1e-6, "sampled"), (32, 1e-7, "nominal"), (16, 1e-3, "sampled")]:
+        out, status = rbrp_colid_batched([_dev(A) for A in As], b=b, tol=tol, seeds=seeds,
+                                         offsets=offsets, filter=filt, mode="persistent")
+        assert (status == 0).all()
+        for q, A in enumerate(As):
+            _check_against_oracle(A, out[q], b, tol, seeds[q], offsets[q], filt,
+                                  name=f"{q} b{b} {filt}")
+
+
+def test_batched_persistent_equals_loop():
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    from robust_blockwise_random_pivoting_amd.matrices import kernel_blocks_torch
+    As = kernel_blocks_torch(160, 21, m_range=(64, 1024), n_range=(64, 512))
+    p_out, p_st = rbrp_colid_batched(As, b=16, tol=1e-7, mode="persistent")
+    l_out, l_st = rbrp_colid_batched(As, b=16, tol=1e-7, mode="loop")
+    assert (p_st == 0).all() and (l_st == 0).all()
+    for q in range(len(As)):
+        assert p_out[q].k == l_out[q].k, q
+        assert np.array_equal(p_out[q].perm.cpu().numpy(), l_out[q].perm.cpu().numpy()), q
+        assert p_out[q].draws == l_out[q].draws, q
+        if p_out[q].X is not None and p_out[q].k < As[q].shape[1]:
+            a, c = p_out[q].X.cpu().numpy(), l_out[q].X.cpu().numpy()
+            assert np.linalg.norm(a - c) <= 1e-6 * max(np.linalg.norm(c), 1e-300), q
+        assert p_out[q].err_final <= 1e-7 * 1.0001 or p_out[q].k == min(As[q].shape), q
+
+
+def test_batched_edge_cases():
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    rng = np.random.default_rng(2)
+    base = spectrum_matrix(80, 60, exp_spectrum(60, 6.0), 3)
+    dup = np.concatenate([base[:, :20], base[:, :20], base[:, 20:]], axis=1)  # repeated columns
+    As = [np.zeros((40, 30)),                        # zero matrix: k = 0
+          spectrum_matrix(50, 5, exp_spectrum(5, 2.0), 1),   # n < b
+          spectrum_matrix(20, 100, exp_spectrum(20, 3.0), 2),  # m < n
+          rng.standard_normal((1, 7)),              # one row
+          dup,
+          kernel_block(130, 70, rng)]
+    out, status = rbrp_colid_batched([_dev(A) for A in As], b=16, tol=1e-8, mode="persistent")
+    assert (status == 0).all()
+    for q, A in enumerate(As):
+        if q == 4:
+            continue
+        _check_against_oracle(A, out[q], 16, 1e-8, seed=q, name=f"edge {q}")
+    assert out[0].k == 0
+    # Exact duplicates tie exactly on the device, while the oracle's BLAS leaves
+    # them a last-bit apart, so either member of a pair can be picked; after
+    # that the permutation histories (hence later samples) differ.  Compare the
+    # skeletons modulo the duplicate pairs through the block holding the first
+    # tie (block 2, positions 8..14), then the rank, the error and that no pair
+    # is selected twice.
+    r = oracle_colid(dup, 16, 1e-8, seed=4)
+    cls = np.arange(dup.shape[1])
+    cls[20:40] = np.arange(20)
+    perm = out[4].perm.cpu().numpy()
+    upto = int(r.ranks[1])
+    assert np.array_equal(cls[perm[:upto]], cls[r.perm[:upto]])
+    assert out[4].k == r.k
+    assert len(set(cls[perm[: out[4].k]])) == out[4].k
+    eg = id_error(dup, perm, out[4].k, out[4].X.cpu().numpy())
+    er = id_error(dup, r.perm, r.k, r.X)
+    assert eg <= 1e-8 and er <= 1e-8
+    # a non-finite problem fails alone
+    bad = As[5].copy()
+    bad[3, 4] = np.nan
+    out2, status2 = rbrp_colid_batched([_dev(A) for A in [As[5], bad, As[1]]], b=16, tol=1e-8,
+                                       seeds=[5, 6, 1], mode="persistent")
+    assert status2[0] == 0 and status2[1] != 0 and status2[2] == 0
+    _check_against_oracle(As[5], out2[0], 16, 1e-8, seed=5)
+    _check_against_oracle(As[1], out2[2], 16, 1e-8, seed=1)
+
+
+def test_batched_max_rank_and_no_x():
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    As = _kernel_batch(6, 8)
+    out, status = rbrp_colid_batched([_dev(A) for A in As], b=16, tol=0.0, max_rank=20,
+                                     want_x=False, mode="persistent")
+    assert (status == 0).all()
+    for q, A in enumerate(As):
+        r = _check_against_oracle(A, out[q], 16, 0.0, seed=q, max_rank=20)
+        assert out[q].k == r.k <= 20
+
+
+def test_batched_graph_replay_and_tuner():
+    """A second call with the same launch shape replays the recorded graph with
+    the new descriptors; the tuner's choice gives the same results."""
+    from robust_blockwise_random_pivoting_amd import Handle, batched_tune, rbrp_colid_batched
+    As = _kernel_batch(12, 31)
+    dA = [_dev(A) for A in As]
+    with Handle() as h:
+        o1, _ = rbrp_colid_batched(dA, b=16, tol=1e-7, mode="persistent", handle=h)
+        o2, _ = rbrp_colid_batched(dA, b=16, tol=1e-7, mode="persistent", handle=h)
+        for a, c in zip(o1, o2):
+            assert a.k == c.k and np.array_equal(a.perm.cpu().numpy(), c.perm.cpu().numpy())
+            if a.X is not None and a.k < a.perm.shape[0]:
+                assert np.array_equal(a.X.cpu().numpy(), c.X.cpu().numpy())
+        seeds = [100 + q for q in range(len(As))]
+        o3, st3 = rbrp_colid_batched(dA, b=16, tol=1e-7, seeds=seeds, mode="persistent", handle=h)
+        assert (st3 == 0).all()
+        for q, A in enumerate(As):
+            _check_against_oracle(A, o3[q], 16, 1e-7, seed=seeds[q])
+        pick = batched_tune(dA, b=16, tol=1e-7, handle=h)
+        assert pick in (1, 2, 3, 4, 6, 8)
+        o4, _ = rbrp_colid_batched(dA, b=16, tol=1e-7, mode="persistent", handle=h)
+        for a, c in zip(o1, o4):
+            assert a.k == c.k and np.array_equal(a.perm.cpu().numpy(), c.perm.cpu().numpy())
+
+
+def test_batched_rejects_bad_arguments(librbrp):
+    import ctypes
+    import torch
+    from robust_blockwise_random_pivoting_amd import Handle, _capi, rbrp_colid_batched
+    A = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
+    perm = torch.zeros(8, dtype=torch.int32, device="cuda")
+    probs = (_capi.RbrpProblem * 1)()
+    probs[0].A, probs[0].m, probs[0].n, probs[0].lda = A.data_ptr(), 8, 8, 8
+    probs[0].perm, probs[0].X, probs[0].ldx = perm.data_ptr(), None, 8
+    res = (_capi.RbrpProblemResult * 1)()
+    opts = _capi.RbrpOpts()
+    librbrp.rbrp_opts_default(ctypes.byref(opts))
+    with Handle() as h:
+        f = librbrp.rbrp_colid_batched_d
+        assert f(None, 1, probs, ctypes.byref(opts), res, 0) == -1
+        assert f(h.ptr, -1, probs, ctypes.byref(opts), res, 0) == -2
+        assert f(h.ptr, 1, probs, ctypes.byref(opts), res, 7) == -6
+        opts.b = 0
+        assert f(h.ptr, 1, probs, ctypes.byref(opts), res, 0) == -4
+        opts.b = 64
+        assert f(h.ptr, 1, probs, ctypes.byref(opts), res, 1) == -4   # b > 32 in persistent mode
+        opts.b = 16
+        probs[0].lda = 4
+        assert f(h.ptr, 1, probs, ctypes.byref(opts), res, 0) == -3
+    # n above the persistent limit: AUTO falls back to the per-problem path
+    B = spectrum_matrix(40, 1100, exp_spectrum(40, 4.0), 3)
+    out, status = rbrp_colid_batched([_dev(B)], b=16, tol=1e-6, mode="auto")
+    assert status[0] == 0
+    _check_against_oracle(B, out[0], 16, 1e-6, seed=0)
