@@ -10,7 +10,12 @@ Prints one JSON line: value = IDs per second (higher is better), the roofline
 of the dominant kernel (per-launch hipEvent durations measured by the library
 over the timed region, on the stream the kernels run on) and the CPU baseline
 (the NumPy/SciPy oracle on the same matrix and Philox stream, timed on this
-host).
+host).  Further legs in the same line:
+  mixed    -- FP32-stored residual vs FP64 at the same tolerance (1e-6) on the
+              50000 x 4000 exp-decay and the 100000 x 2000 Gaussian-mixture matrices;
+  batched  -- 4096 Gaussian-kernel blocks (m ~ U[64,1024], n ~ U[64,512]), b = 16,
+              tol = 1e-7: one persistent launch vs one launch sequence per problem;
+  comparators -- RandCPQR_OS / RandLUPP_OS at the rank RBRP found.
 """
 
 from __future__ import annotations
@@ -48,7 +53,129 @@ def parse():
                     help="skip the RandCPQR_OS / RandLUPP_OS comparison runs")
     ap.add_argument("--profile-steps", action="store_true",
                     help="print a line per step (for profiler runs)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-cpu-sample", type=int, default=48,
+                    help="problems of the batch timed with the CPU oracle")
     return ap.parse_args()
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def id_rel_error(A, res):
+    """||A(:, rd) - A(:, J) X||_F / ||A||_F of a returned ID, on the device."""
+    import torch
+    n = A.shape[1]
+    if res.k >= n:
+        return 0.0
+    perm = res.perm.long()
+    E = A[:, perm[res.k:]] - A[:, perm[: res.k]] @ res.X
+    e = (torch.linalg.norm(E) / torch.linalg.norm(A)).item()
+    del E
+    return e
+
+
+def time_colid(A, b, tol, seed, precision, h, reps):
+    """One warm-up and `reps` timed IDs; returns (last result, seconds per ID)."""
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    rbrp_colid(A, b=b, tol=tol, seed=seed, precision=precision, handle=h)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = rbrp_colid(A, b=b, tol=tol, seed=seed, precision=precision, handle=h)
+    _sync()
+    return res, (time.perf_counter() - t0) / reps
+
+
+def mixed_leg(A2, args):
+    """Mixed precision (FP32-stored residual, FP32 MFMA) vs FP64 at the same tolerance:
+    the 50000 x 4000 exp-decay matrix at tol 1e-6 (the tightest tolerance FP32
+    storage resolves on it) and the 100000 x 2000 Gaussian-mixture matrix at 1e-6."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import Handle
+    from robust_blockwise_random_pivoting_amd.matrices import gmm_rows_torch
+    out = {}
+    with Handle() as h:
+        cases = [("exp_50000x4000_b64_tol1e-6", lambda: A2, 64, 1e-6),
+                 ("gmm_100000x2000_b64_tol1e-6",
+                  lambda: gmm_rows_torch(100000, 2000, 7, components=20), 64, 1e-6)]
+        for label, gen, b, tol in cases:
+            A = gen()
+            _sync()
+            r64, t64 = time_colid(A, b, tol, args.seed, "fp64", h, 2)
+            r32, t32 = time_colid(A, b, tol, args.seed, "fp32", h, 2)
+            e64, e32 = id_rel_error(A, r64), id_rel_error(A, r32)
+            out[label] = {"fp64_ms": t64 * 1e3, "fp32_ms": t32 * 1e3, "speedup": t64 / t32,
+                          "k_fp64": r64.k, "k_fp32": r32.k, "rel_error_fp64": e64,
+                          "rel_error_fp32": e32, "tracked_error_fp32": r32.err_final}
+            print(f"[bench] mixed {label}: fp64 {t64 * 1e3:.1f} ms k={r64.k} err={e64:.3e} | "
+                  f"fp32 {t32 * 1e3:.1f} ms k={r32.k} err={e32:.3e}", file=sys.stderr, flush=True)
+            if A is not A2:
+                del A
+                torch.cuda.empty_cache()
+    return out
+
+
+def batched_leg(args):
+    """4096 Gaussian-kernel blocks (m ~ U[64,1024], n ~ U[64,512]), b = 16, tol = 1e-7:
+    one persistent launch (autotuned, replayed from its hipGraph) vs one launch
+    sequence per problem, and the CPU oracle on a sample of the same problems."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import Batch, Handle, batched_tune
+    from robust_blockwise_random_pivoting_amd.matrices import kernel_blocks_torch
+    nprob = args.batch_count
+    As = kernel_blocks_torch(nprob, seed=4, m_range=(64, 1024), n_range=(64, 512))
+    _sync()
+    print(f"[bench] batched: {nprob} kernel blocks ready", file=sys.stderr, flush=True)
+    b, tol = 16, 1e-7
+    seeds = list(range(nprob))
+    with Handle() as h:
+        wg = batched_tune(As, b=b, tol=tol, seeds=seeds, handle=h)
+        pb = Batch(As, b=b, tol=tol, seeds=seeds)
+        st = pb.run("persistent", h)                       # warm-up: records the graph
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            st = pb.run("persistent", h)
+        tp = (time.perf_counter() - t0) / reps
+        print(f"[bench] batched persistent: {tp * 1e3:.1f} ms ({wg} WG/CU)", file=sys.stderr,
+              flush=True)
+        lb = Batch(As, b=b, tol=tol, seeds=seeds)
+        t0 = time.perf_counter()
+        stl = lb.run("loop", h)
+        tl = time.perf_counter() - t0
+        print(f"[bench] batched loop: {tl * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    rp, rl = pb.results(), lb.results()
+    same = all(a.k == c.k and torch.equal(a.perm, c.perm) for a, c in zip(rp, rl))
+    ks = np.array([r.k for r in rp])
+    errs = np.array([r.err_final for r in rp])
+    out = {"problems": nprob, "b": b, "tol": tol, "persistent_ms": tp * 1e3,
+           "persistent_ids_per_s": nprob / tp, "loop_ms": tl * 1e3, "loop_ids_per_s": nprob / tl,
+           "speedup_vs_loop": tl / tp, "wg_per_cu": wg, "status_nonzero": int((st != 0).sum()),
+           "loop_status_nonzero": int((stl != 0).sum()), "same_as_loop": bool(same),
+           "k_mean": float(ks.mean()), "k_max": int(ks.max()),
+           "max_tracked_error": float(errs.max())}
+    if not args.no_cpu and args.batch_cpu_sample > 0:
+        from oracle import rbrp_colid as oracle_colid
+        q = min(args.batch_cpu_sample, nprob)
+        hostA = [As[i].cpu().numpy() for i in range(q)]
+        t0 = time.perf_counter()
+        agree = 0
+        for i in range(q):
+            r = oracle_colid(hostA[i], b, tol, seed=seeds[i])
+            agree += int(r.k == rp[i].k and np.array_equal(r.perm[: r.k],
+                                                          rp[i].perm[: r.k].cpu().numpy()))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": q / dt, "unit": "IDs/s", "kind": "port",
+                               "sample": f"first {q} problems of the batch, one after another",
+                               "skeletons_equal_to_gpu": f"{agree}/{q}"}
+    del As
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(A_host, args, k_gpu, perm_gpu):
@@ -118,10 +245,7 @@ def main():
     stats = h.stats()
 
     # independent check of the returned ID on the device
-    perm = res.perm.long()
-    E = A[:, perm[res.k:]] - A[:, perm[: res.k]] @ res.X
-    err = (torch.linalg.norm(E) / torch.linalg.norm(A)).item()
-    del E
+    err = id_rel_error(A, res)
 
     kern = {}
     for name, bound in (("project", "mfma"), ("update", "hbm")):
@@ -167,6 +291,9 @@ def main():
             del E
             print(f"[bench] {name}: {dtc * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
+    mixed = None if args.no_mixed else mixed_leg(A, args)
+    batched = None if args.no_batched else batched_leg(args)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
@@ -197,6 +324,8 @@ def main():
         "kernels": kern,
         "phase_ms_per_step": {k: v["ms"] / args.steps for k, v in stats.items()},
         "comparators": comps,
+        "mixed": mixed,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -726,6 +726,14 @@ hipError_t rbrp_launch_colid_batched(hipStream_t st, int grid, const RbrpBatchPr
 
 size_t rbrp_batched_lds_bytes(int n_max, int b) { return bq_lds_bytes(n_max, b); }
 
+int rbrp_batched_max_blocks_per_cu(int n_max, int b) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_colid_batched, BQ_NT,
+                                                   bq_lds_bytes(n_max, b)) != hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
 hipError_t rbrp_batched_prepare(int n_max, int b) {
   return hipFuncSetAttribute((const void*)k_colid_batched,
                              hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -132,16 +132,21 @@ int compute_units(rbrp_handle_t h, int* cus, size_t* lds_max) {
 // recorded graph when use_graph), downloads the results.  ms (optional)
 // receives the device time of the launch itself.
 int run_persistent(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, BatchPlan& pl,
-                   int wg_per_cu, bool use_graph, rbrp_problem_result_t* results, float* ms) {
+                   int wg_per_cu, bool use_graph, rbrp_problem_result_t* results, float* ms,
+                   int* per_cu_used = nullptr) {
   int cus;
   size_t lds_cu;
   {
     int e = compute_units(h, &cus, &lds_cu);
     if (e) return e;
   }
-  const size_t lds = rbrp_batched_lds_bytes(pl.shape.n_max, pl.shape.b);
-  const int fit_lds = (int)std::max<size_t>(1, lds_cu / std::max<size_t>(lds, 1));
-  const int per_cu = std::max(1, std::min({wg_per_cu, fit_lds, 8}));
+  (void)lds_cu;
+  HIPCHK(rbrp_batched_prepare(pl.shape.n_max, pl.shape.b));
+  // persistent workgroups beyond what is co-resident would only start once
+  // the queue is drained: cap at the occupancy limit
+  const int fit = rbrp_batched_max_blocks_per_cu(pl.shape.n_max, pl.shape.b);
+  const int per_cu = std::max(1, std::min(wg_per_cu, fit));
+  if (per_cu_used) *per_cu_used = per_cu;
   const int grid = (int)std::min<int64_t>((int64_t)per_cu * cus, nprob);
   // workspace and metadata
   {
@@ -174,7 +179,6 @@ int run_persistent(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, Bat
   std::stable_sort(h_order, h_order + nprob, [&](int a, int c) {
     return pr[a].m * pr[a].n > pr[c].m * pr[c].n;  // largest first
   });
-  HIPCHK(rbrp_batched_prepare(pl.shape.n_max, pl.shape.b));
   if (!h->bt_stream) HIPCHK(hipStreamCreateWithFlags(&h->bt_stream, hipStreamNonBlocking));
   hipStream_t st = h->bt_stream;
   HIPCHK(hipStreamSynchronize(h->stream));  // order after earlier work of the handle
@@ -311,15 +315,18 @@ int rbrp_batched_tune(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* prob
   std::vector<rbrp_problem_result_t> res((size_t)nprob);
   const int cand[] = {1, 2, 3, 4, 6, 8};
   float best = 0.f;
-  int pick = 0;
+  int pick = 0, last_used = 0;
   for (int c : cand) {
     float ms = 0.f;
     BatchPlan p2 = pl;
-    int e = run_persistent(h, nprob, problems, p2, c, false, res.data(), &ms);
+    int used = 0;
+    int e = run_persistent(h, nprob, problems, p2, c, false, res.data(), &ms, &used);
     if (e) return e;
+    if (used == last_used) continue;  // capped to the same launch as the previous candidate
+    last_used = used;
     if (pick == 0 || ms < best) {
       best = ms;
-      pick = c;
+      pick = used;
     }
   }
   const std::vector<uint64_t> key = tune_key(pl);

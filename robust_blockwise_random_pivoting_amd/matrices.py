@@ -139,3 +139,25 @@ def kernel_blocks_torch(count: int, seed: int, m_range=(64, 1024), n_range=(64, 
     gen.manual_seed(int(seed))
     return [kernel_block_torch(int(a), int(c), gen, device=device) for a, c in zip(ms, ns)]
 
+
+def gmm_rows_torch(m: int, d: int, seed: int, components: int = 20, var: float = 0.01,
+                   outlier_frac: float = 1e-3, outlier_scale: float = 50.0, device="cuda"):
+    """``gmm_rows`` drawn on the device (same recipe; the mixture parameters come
+    from NumPy, the rows from a seeded torch generator); column-major (m, d) view."""
+    import torch
+    rng = np.random.default_rng(seed)
+    mu = torch.as_tensor(rng.standard_normal((components + 1, d)), device=device)
+    wts = rng.dirichlet(np.ones(components))
+    n_out = int(round(m * outlier_frac))
+    n_in = m - n_out
+    lab = torch.as_tensor(rng.choice(components, size=n_in, p=wts), device=device)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    Xt = torch.empty((d, m), dtype=torch.float64, device=device)   # column-major (m, d)
+    Xt[:, :n_in] = (mu[lab] + float(np.sqrt(var)) *
+                    torch.randn((n_in, d), generator=gen, dtype=torch.float64, device=device)).t()
+    Xt[:, n_in:] = (outlier_scale * (mu[components] + float(np.sqrt(var)) * torch.randn(
+        (n_out, d), generator=gen, dtype=torch.float64, device=device))).t()
+    order = torch.as_tensor(rng.permutation(m), device=device)
+    return Xt[:, order].t()
+

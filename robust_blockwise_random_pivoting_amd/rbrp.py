@@ -214,39 +214,59 @@ def _batch_setup(As, b, tol, seeds, offsets, max_rank, filter, want_x):
     return probs, opts, keep, perms, xbufs
 
 
+class Batch:
+    """A batch of device FP64 problems with its descriptors and output buffers,
+    built once and run any number of times (``run``).  Problem q uses the
+    Philox stream (seeds[q], offsets[q]) (default (q, 0)); b, tol, max_rank and
+    filter are shared."""
+
+    def __init__(self, As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None,
+                 max_rank: int = 0, filter: str = "sampled", want_x: bool = True):
+        self.nprob = len(As)
+        self.want_x = want_x
+        (self.probs, self.opts, self._keep, self.perms,
+         self.xbufs) = _batch_setup(As, b, tol, seeds, offsets, max_rank, filter, want_x)
+        self.res = (_capi.RbrpProblemResult * max(self.nprob, 1))()
+
+    def run(self, mode: str = "persistent", handle: Optional[Handle] = None) -> np.ndarray:
+        """One batched call; returns the per-problem status array."""
+        own = handle is None
+        h = Handle() if own else handle
+        try:
+            _capi.check("rbrp_colid_batched_d", h._lib.rbrp_colid_batched_d(
+                h.ptr, self.nprob, self.probs, ctypes.byref(self.opts), self.res,
+                _capi.BATCH_MODES[mode]))
+            h.stream.synchronize()
+        finally:
+            if own:
+                h.close()
+        return np.array([int(self.res[q].status) for q in range(self.nprob)], dtype=np.int32)
+
+    def results(self):
+        """Per-problem ``ColID`` of the last run (per-block traces empty)."""
+        out = []
+        for q in range(self.nprob):
+            k, n = int(self.res[q].k), int(self._keep[q].shape[1])
+            X = self.xbufs[q][: n - k, :k].t() if self.want_x else None
+            out.append(ColID(k=k, perm=self.perms[q][:n], X=X, ranks=np.zeros(0, np.int64),
+                             error=np.zeros(0), t_ranks=np.zeros(0), t_total=0.0,
+                             err_final=float(self.res[q].err_final),
+                             draws=int(self.res[q].draws)))
+        return out
+
+
 def rbrp_colid_batched(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None,
                        max_rank: int = 0, filter: str = "sampled", mode: str = "auto",
                        want_x: bool = True, handle: Optional[Handle] = None):
     """Batched RBRP column IDs of the device FP64 matrices ``As`` (a list).
 
-    Problem q uses the Philox stream (seeds[q], offsets[q]) (default (q, 0)); the
-    other options are shared.  ``mode``: "persistent" (one persistent launch
-    with a device work queue, replayed from a hipGraph), "loop" (one
-    ``rbrp_colid_d`` per problem) or "auto".  Returns a list of ``ColID`` (per-
-    block traces empty) and the array of per-problem statuses.
+    ``mode``: "persistent" (one persistent launch with a device work queue,
+    replayed from a hipGraph), "loop" (one ``rbrp_colid_d`` per problem) or
+    "auto".  Returns the list of per-problem ``ColID`` and the status array.
     """
-    nprob = len(As)
-    probs, opts, keep, perms, xbufs = _batch_setup(As, b, tol, seeds, offsets, max_rank, filter,
-                                                   want_x)
-    res = (_capi.RbrpProblemResult * max(nprob, 1))()
-    own = handle is None
-    h = Handle() if own else handle
-    try:
-        _capi.check("rbrp_colid_batched_d", h._lib.rbrp_colid_batched_d(
-            h.ptr, nprob, probs, ctypes.byref(opts), res, _capi.BATCH_MODES[mode]))
-        h.stream.synchronize()
-    finally:
-        if own:
-            h.close()
-    out = []
-    for q in range(nprob):
-        k, n = int(res[q].k), int(keep[q].shape[1])
-        X = xbufs[q][: n - k, :k].t() if want_x else None
-        out.append(ColID(k=k, perm=perms[q][:n], X=X, ranks=np.zeros(0, np.int64),
-                         error=np.zeros(0), t_ranks=np.zeros(0), t_total=0.0,
-                         err_final=float(res[q].err_final), draws=int(res[q].draws)))
-    status = np.array([int(res[q].status) for q in range(nprob)], dtype=np.int32)
-    return out, status
+    batch = Batch(As, b, tol, seeds, offsets, max_rank, filter, want_x)
+    status = batch.run(mode, handle)
+    return batch.results(), status
 
 
 def batched_tune(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None, max_rank: int = 0,
