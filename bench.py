@@ -53,12 +53,46 @@ def parse():
                     help="skip the RandCPQR_OS / RandLUPP_OS comparison runs")
     ap.add_argument("--profile-steps", action="store_true",
                     help="print a line per step (for profiler runs)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run one ID and a 1 GiB device copy, print nothing else (for "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)")
+    ap.add_argument("--save-matrix", default="",
+                    help="write the generated matrix (column-major .npy) and exit")
+    ap.add_argument("--load-matrix", default="",
+                    help="read the matrix from this .npy instead of generating it")
+    ap.add_argument("--legs", default="main,comparators,mixed,batched,cpu",
+                    help="comma list of legs to run (main = the timed FP64 IDs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-batch-loop", action="store_true",
+                    help="skip the per-problem launch loop of the batched leg")
     ap.add_argument("--batch-cpu-sample", type=int, default=48,
                     help="problems of the batch timed with the CPU oracle")
-    return ap.parse_args()
+    args = ap.parse_args()
+    legs = set(x.strip() for x in args.legs.split(",") if x.strip())
+    args.no_comparators = args.no_comparators or "comparators" not in legs
+    args.no_mixed = args.no_mixed or "mixed" not in legs
+    args.no_batched = args.no_batched or "batched" not in legs
+    args.no_cpu = args.no_cpu or "cpu" not in legs
+    args.no_main = "main" not in legs
+    return args
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_gemm_trail_fp64_50000x4000.json")
+
+
+def measured_traffic(args):
+    """HBM bytes per launch of the update kernel from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE x its copy-kernel factor + WRITE_SIZE), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("m"), t.get("n"), t.get("b"), t.get("tol")) != (args.m, args.n, args.b, args.tol):
+        return None
+    return t.get("bytes_per_launch")
 
 
 def _sync():
@@ -144,19 +178,23 @@ def batched_leg(args):
         tp = (time.perf_counter() - t0) / reps
         print(f"[bench] batched persistent: {tp * 1e3:.1f} ms ({wg} WG/CU)", file=sys.stderr,
               flush=True)
-        lb = Batch(As, b=b, tol=tol, seeds=seeds)
-        t0 = time.perf_counter()
-        stl = lb.run("loop", h)
-        tl = time.perf_counter() - t0
-        print(f"[bench] batched loop: {tl * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    rp, rl = pb.results(), lb.results()
-    same = all(a.k == c.k and torch.equal(a.perm, c.perm) for a, c in zip(rp, rl))
+        if args.no_batch_loop:
+            lb, stl, tl = None, np.zeros(nprob, np.int32), float("nan")
+        else:
+            lb = Batch(As, b=b, tol=tol, seeds=seeds)
+            t0 = time.perf_counter()
+            stl = lb.run("loop", h)
+            tl = time.perf_counter() - t0
+            print(f"[bench] batched loop: {tl * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    rp = pb.results()
+    same = None if lb is None else all(
+        a.k == c.k and torch.equal(a.perm, c.perm) for a, c in zip(rp, lb.results()))
     ks = np.array([r.k for r in rp])
     errs = np.array([r.err_final for r in rp])
     out = {"problems": nprob, "b": b, "tol": tol, "persistent_ms": tp * 1e3,
            "persistent_ids_per_s": nprob / tp, "loop_ms": tl * 1e3, "loop_ids_per_s": nprob / tl,
            "speedup_vs_loop": tl / tp, "wg_per_cu": wg, "status_nonzero": int((st != 0).sum()),
-           "loop_status_nonzero": int((stl != 0).sum()), "same_as_loop": bool(same),
+           "loop_status_nonzero": int((stl != 0).sum()), "same_as_loop": same,
            "k_mean": float(ks.mean()), "k_max": int(ks.max()),
            "max_tracked_error": float(errs.max())}
     if not args.no_cpu and args.batch_cpu_sample > 0:
@@ -222,9 +260,36 @@ def main():
 
     torch.cuda.set_device(0)
     m, n = args.m, args.n
-    A = spectrum_matrix_torch(m, n, exp_spectrum(n, args.rate), seed=args.matrix_seed, device="cuda")
+    if args.load_matrix:
+        At = np.load(args.load_matrix, mmap_mode="r")            # (n, m) row-major == A column-major
+        A = torch.from_numpy(np.ascontiguousarray(At)).to("cuda").t()
+        del At
+    else:
+        A = spectrum_matrix_torch(m, n, exp_spectrum(n, args.rate), seed=args.matrix_seed,
+                                  device="cuda")
     torch.cuda.synchronize()
     print(f"[bench] matrix {m}x{n} ready", file=sys.stderr, flush=True)
+    if args.save_matrix:
+        np.save(args.save_matrix, A.t().contiguous().cpu().numpy())
+        return
+    if args.traffic_probe:
+        # calibration: a plain device copy of exactly 2^30 bytes read and written
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        with Handle() as hp:
+            r = rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=hp)
+        torch.cuda.synchronize()
+        print(f"[bench] traffic probe: copy 1 GiB, one ID k={r.k}", file=sys.stderr, flush=True)
+        return
+
+    if args.no_main:
+        out = {"mixed": None if args.no_mixed else mixed_leg(A, args),
+               "batched": None if args.no_batched else batched_leg(args)}
+        print(json.dumps(out))
+        return
 
     h = Handle()
     h.set_timing(True)
@@ -269,7 +334,9 @@ def main():
         d = kern[dom]
         roof = {"kernel": {"project": "k_gemm_tn", "update": "k_gemm_trail"}[dom], "bound": d["bound"],
                 "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
-                "traffic": None, "avg_ms": d["avg_ms"], "launches": d["launches"]}
+                "traffic": measured_traffic(args) if dom == "update" else None,
+                "algorithmic_bytes_per_launch": d.get("bytes_per_launch"),
+                "avg_ms": d["avg_ms"], "launches": d["launches"]}
 
     # comparison baselines at the rank RBRP found: Gaussian
     # sketch of 2k rows, oversampling 2, interpolation as gpu/interp.cu
