@@ -31,8 +31,9 @@ int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const doubl
 
 /* Trailing-update kernel (K <= 128): C (M x N) -= A Bt^T-layout product, i.e.
  * C(r, c) -= sum_k A(r, k) Bt[k * ldbt + c] with A (M x K, lda) column-major
- * and Bt row-major.  normpart (optional) receives ceil(M/128) x N per-128-row
- * tile squared column norms of the new C (leading dimension N). */
+ * and Bt row-major.  normpart (optional, ceil(M/64) x N, leading dimension N,
+ * zero-initialised by the caller) receives partial squared column norms of the
+ * new C over row groups: its rows sum to the column norms. */
 int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                        const double* Bt, int64_t ldbt, double* C, int64_t ldc, double* normpart);
 
@@ -44,8 +45,9 @@ int rbrp_test_gemm_tn32(void* stream, int64_t M, int64_t N, int64_t K, const flo
                         const float* B, int64_t ld, double* C, int64_t ldc, int nsplit);
 
 /* FP32 trailing-update kernel (K <= 128, M, lda, ldc even): C(r, c) -= sum_k
- * A(r, k) Bt[k * ldbt + c], FP32 data; normpart (optional, FP64) receives
- * ceil(M/128) x N per-128-row tile squared column norms (leading dimension N). */
+ * A(r, k) Bt[k * ldbt + c], FP32 data; normpart (optional, FP64, ceil(M/128)
+ * x N, leading dimension N, zero-initialised by the caller) receives partial
+ * squared column norms over row groups: its rows sum to the column norms. */
 int rbrp_test_gemm_upd32(void* stream, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
                          const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* normpart);
 

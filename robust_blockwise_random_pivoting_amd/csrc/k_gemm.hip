@@ -15,6 +15,18 @@
 
 #define KT 16
 
+// Trailing-update launch shape: rows per wave / 16 (JB), longest row sweep per
+// workgroup (TMAX) and the grid size the sweep length aims at (WGS workgroups).
+#ifndef RBRP_TRAIL_JB
+#define RBRP_TRAIL_JB 1
+#endif
+#ifndef RBRP_TRAIL_TMAX
+#define RBRP_TRAIL_TMAX 8
+#endif
+#ifndef RBRP_TRAIL_WGS
+#define RBRP_TRAIL_WGS 2048
+#endif
+
 __device__ inline int mf_row(int lane, int v) { return 4 * v + (lane >> 4); }
 __device__ inline int mf_col(int lane) { return lane & 15; }
 
@@ -297,15 +309,17 @@ hipError_t rbrp_launch_gemm_nn(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec 
 // ---------------------------------------------------------------------------
 // Tall rank-K update (the trailing-matrix update and the CholQR2 products):
 // C(r, c) (op)= sum_k A(r, k) B(k, c) for r < M, c < N, K <= 128, optional
-// squared column norms of the new C per 128-row tile.
+// squared column norms of the new C, one partial per workgroup row group
+// (accumulated per lane over the workgroup's sweep, reduced once at its end).
 //
 // The MFMA computes the transposed product D = B^T A^T so that the result
 // layout puts consecutive rows of C in consecutive lanes: lane l holds
 // C(row0 + l%16, col0 + 4v + l/16), i.e. every C load/store instruction moves
 // four 128-byte column runs, with no LDS staging of C.
 //
-// Workgroup = 4 waves stacked along rows, 128 rows x 32 columns per tile,
-// sweeping `T` consecutive row tiles: B (K x 32) sits in LDS for the whole
+// Workgroup = 4 waves stacked along rows, 64 JB rows x 32 columns per tile
+// (JB = 1 in the loop: 16 rows per wave keeps the register footprint low
+// enough for four waves per SIMD), sweeping `T` consecutive row tiles: B (K x 32) sits in LDS for the whole
 // sweep; for each tile the A fragments (16 consecutive rows of one A column
 // per 16 lanes) are loaded first and the next tile's C is prefetched right
 // after them, so the A loads can be waited on while the C prefetch is still
@@ -355,6 +369,13 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
           cv[jb][ib][v] = (cc < ncol && r < M) ? C[r + cc * ldc] : 0.0;
         }
   };
+  // squared norms of the new C, accumulated per lane over the whole sweep and
+  // reduced once at its end (one partial per workgroup row group)
+  double nsq[2][4];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) nsq[ib][v] = 0.0;
   loadC(rbase, cvA);
   for (int t = 0; t < T; ++t) {
     const int64_t r0 = rbase + (int64_t)t * rows_per_tile;
@@ -388,11 +409,6 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
     }
     // D[i][j] (i = column within the 16-column block, j = row) sits in lane
     // j + 16*(i%4), element i/4: with i = 4v + lk that is lane li + 16*lk.
-    double nsq[2][4];
-#pragma unroll
-    for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) nsq[ib][v] = 0.0;
 #pragma unroll
     for (int jb = 0; jb < JB; ++jb)
 #pragma unroll
@@ -404,27 +420,9 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
           const double nv = cvA[jb][ib][v] - acc[jb][ib][v];
           if (cc < ncol && r < M) {
             C[r + cc * ldc] = nv;
-            nsq[ib][v] += nv * nv;
+            nsq[ib][v] = fma(nv, nv, nsq[ib][v]);
           }
         }
-    if (NORMS) {
-#pragma unroll
-      for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          double x = nsq[ib][v];
-          x += __shfl_xor(x, 1, 64);
-          x += __shfl_xor(x, 2, 64);
-          x += __shfl_xor(x, 4, 64);
-          x += __shfl_xor(x, 8, 64);
-          if (li == 0) red[wid][16 * ib + 4 * v + lk] = x;
-        }
-      __syncthreads();
-      if (wid == 0 && lane < ncol)
-        normpart[(r0 >> 7) * ldnp + c0 + lane] =
-            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-      __syncthreads();
-    }
 #pragma unroll
     for (int jb = 0; jb < JB; ++jb)
 #pragma unroll
@@ -432,32 +430,53 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
 #pragma unroll
         for (int v = 0; v < 4; ++v) cvA[jb][ib][v] = cvB[jb][ib][v];
   }
+  if (NORMS) {
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = nsq[ib][v];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        if (li == 0) red[wid][16 * ib + 4 * v + lk] = x;
+      }
+    __syncthreads();
+    if (wid == 0 && lane < ncol)
+      normpart[(int64_t)blockIdx.y * ldnp + c0 + lane] =
+          (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  }
 }
 
 hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                  double* normpart, int64_t ldnp) {
+                                  double* normpart, int64_t ldnp, int64_t* groups) {
+  if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128) return hipErrorInvalidValue;
   const bool small = Kmax <= 64;
-  const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 127) / 128;
-  int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 2048));
+  const int64_t jb = small ? RBRP_TRAIL_JB : 1;  // rows per wave / 16 of the variant launched
+  const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 64 * jb - 1) / (64 * jb);
+  int64_t T = std::max<int64_t>(
+      1, std::min<int64_t>(RBRP_TRAIL_TMAX, (ncol * nrow) / RBRP_TRAIL_WGS));
   dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
+  if (groups) *groups = (int64_t)grid.y;
   if (small) {
     if (normpart)
-      hipLaunchKernelGGL((k_gemm_trail<2, 16, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      hipLaunchKernelGGL((k_gemm_trail<RBRP_TRAIL_JB, 16, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
     else
-      hipLaunchKernelGGL((k_gemm_trail<2, 16, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      hipLaunchKernelGGL((k_gemm_trail<RBRP_TRAIL_JB, 16, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
   } else {
     if (normpart)
-      hipLaunchKernelGGL((k_gemm_trail<2, 32, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      hipLaunchKernelGGL((k_gemm_trail<1, 32, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
     else
-      hipLaunchKernelGGL((k_gemm_trail<2, 32, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      hipLaunchKernelGGL((k_gemm_trail<1, 32, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
   }
   return hipGetLastError();

@@ -308,9 +308,9 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
 
 // ---------------------------------------------------------------------------
 // FP32 trailing update with fused norms: C(r, c) -= sum_k A(r, k) Bt[k * ldbt + c]
-// for r < M, c < N, K <= Kmax <= 128; optional FP64 per-128-row-tile squared
-// column norms of the new C (FP32 squares summed in FP32 within a wave, then
-// in FP64 across the four waves).
+// for r < M, c < N, K <= Kmax <= 128; optional squared column norms of the new
+// C, one FP64 partial per workgroup row group (FP32 squares accumulated per
+// lane over the sweep, then summed in FP64 across lanes and waves).
 //
 // The MFMA computes D = B^T A^T (D[i][j] = C(row j, column i) of a 16x16
 // block) so that C needs no LDS staging.  The 16 "j" rows of the block are
@@ -373,6 +373,8 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
     X12 = *(const float2*)(ccol[1][2] + rr);                           \
     X13 = *(const float2*)(ccol[1][3] + rr);                           \
   }
+  // squared norms of the new C over the whole sweep, reduced once at its end
+  float nsq[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
   float2 a00, a01, a02, a03, a10, a11, a12, a13;  // C of the current tile
   float2 n00, n01, n02, n03, n10, n11, n12, n13;  // C of the next tile
   TRAIL32_LOADC(rbase, a00, a01, a02, a03, a10, a11, a12, a13);
@@ -397,17 +399,15 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
     }
     // lane (li, lk) holds D[4lk + v][li]: column 16ib + 4lk + v, rows r (jb 0) and r + 1 (jb 1)
     const bool rok = r < M;
-    float nsq[2][4];
 #define TRAIL32_STORE(IB, V, X)                                                 \
   {                                                                             \
     const int cc = 16 * (IB) + 4 * lk + (V);                                    \
     float2 nv;                                                                  \
     nv.x = X.x - acc0##IB[V];                                                   \
     nv.y = X.y - acc1##IB[V];                                                   \
-    nsq[IB][V] = 0.f;                                                           \
     if (cc < ncol && rok) {                                                     \
       *(float2*)(C + r + (int64_t)cc * ldc) = nv;                               \
-      nsq[IB][V] = fmaf(nv.x, nv.x, nv.y * nv.y);                               \
+      nsq[IB][V] = fmaf(nv.x, nv.x, fmaf(nv.y, nv.y, nsq[IB][V]));              \
     }                                                                           \
   }
     TRAIL32_STORE(0, 0, a00);
@@ -419,41 +419,42 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
     TRAIL32_STORE(1, 2, a12);
     TRAIL32_STORE(1, 3, a13);
 #undef TRAIL32_STORE
-    if (NORMS) {
-#pragma unroll
-      for (int ib = 0; ib < 2; ++ib)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          float x = nsq[ib][v];
-          x += __shfl_xor(x, 1, 64);
-          x += __shfl_xor(x, 2, 64);
-          x += __shfl_xor(x, 4, 64);
-          x += __shfl_xor(x, 8, 64);
-          if (li == 0) red[wid][16 * ib + 4 * lk + v] = (double)x;
-        }
-      __syncthreads();
-      if (wid == 0 && lane < ncol)
-        normpart[(r0 >> 7) * ldnp + c0 + lane] =
-            (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-      __syncthreads();
-    }
     a00 = n00; a01 = n01; a02 = n02; a03 = n03;
     a10 = n10; a11 = n11; a12 = n12; a13 = n13;
   }
 #undef TRAIL32_LOADC
+  if (NORMS) {
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = (double)nsq[ib][v];  // FP64 across lanes and waves
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        if (li == 0) red[wid][16 * ib + 4 * lk + v] = x;
+      }
+    __syncthreads();
+    if (wid == 0 && lane < ncol)
+      normpart[(int64_t)blockIdx.y * ldnp + c0 + lane] =
+          (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+  }
 }
 
 hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                     int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
                                     const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
                                     int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                    double* normpart, int64_t ldnp) {
+                                    double* normpart, int64_t ldnp, int64_t* groups) {
+  if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128 || (M & 1) || (lda & 1) || (ldc & 1)) return hipErrorInvalidValue;
   const bool small = Kmax <= 64;
   const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 127) / 128;
   int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 2048));
   dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
+  if (groups) *groups = (int64_t)grid.y;
 #define TRAIL32_LAUNCH(KS, NM)                                                                   \
   hipLaunchKernelGGL((k_gemm_trail32<KS, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,  \
                      acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp)

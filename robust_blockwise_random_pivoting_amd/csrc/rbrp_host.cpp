@@ -68,7 +68,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   size_t oCbuf = c.take<double>(ldr * b);
   size_t oQbuf = c.take<double>(mp * b);
   size_t oP = c.take<double>(Pcap);
-  size_t onp = c.take<double>(((mp + 127) / 128) * n);
+  size_t onp = c.take<double>(((mp + 63) / 64) * n);  // 64-row (FP64) or 128-row (FP32) tiles
   size_t oWt = c.take<double>(b * n);  // FP32 in the mixed loop (over-sized)
   size_t otk = c.take<double>(ldr * b);
   size_t oto = c.take<double>(ldr * b);
@@ -422,7 +422,6 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const double tol2 = opts->tol * opts->tol;
   const uint64_t key0 = opts->seed, key1 = 0;
   const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
-  const int64_t ntiles = (mp + 127) / 128;  // row tiles of the update's norm partials
   const double ea = a_f32 ? 4.0 : 8.0;      // bytes per element of the input
   const double ew = mixed ? 4.0 : 8.0;      // bytes per element of the working matrix
   PhaseTimer pt(h);
@@ -564,17 +563,18 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                          mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
       ph_upd = pt.begin(RBRP_PHASE_UPDATE);
+      int64_t groups = 0;  // norm partials written by the update (one per row group)
       if (mixed)
         HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                         b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
-                                        d_bt, ws.normpart, nrem_max));
+                                        d_bt, ws.normpart, nrem_max, &groups));
       else
         HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                       b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
-                                      ws.normpart, nrem_max));
+                                      ws.normpart, nrem_max, &groups));
       pt.end(ph_upd, 1);
       ph = pt.begin(RBRP_PHASE_RESID);
-      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, ntiles, i, n, ws.st, ws.diags));
+      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags));
     } else {
       ph = pt.begin(RBRP_PHASE_RESID);
     }

@@ -83,12 +83,13 @@ hipError_t rbrp_launch_gemm_nn(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec 
 // K <= Kmax <= 128.  A and C are column-major with leading dimensions lda /
 // ldc whose first column is acoff + *acoff_dyn / ccoff + *ccoff_dyn (the
 // dynamic parts optional); Bt is row-major.  Optional squared column norms of
-// the new C per 128-row tile at normpart[tile * ldnp + c].
+// the new C as partial sums over row groups: normpart[g * ldnp + c], g <
+// *groups (at most ceil(M / 64) groups).
 hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                  double* normpart, int64_t ldnp);
+                                  double* normpart, int64_t ldnp, int64_t* groups = nullptr);
 
 // ---- k_rbrp.hip -----------------------------------------------------------
 hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
@@ -160,13 +161,13 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,
                                  const int* b_dyn, int64_t ld, double* P, int64_t ldp,
                                  int64_t pstride);
-// FP32 trailing update with FP64 per-128-row-tile norm partials (see
-// rbrp_launch_gemm_trail); M, lda and ldc must be even.
+// FP32 trailing update with FP64 norm partials over row groups (at most
+// ceil(M / 128)); see rbrp_launch_gemm_trail.  M, lda and ldc must be even.
 hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                     int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
                                     const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
                                     int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                    double* normpart, int64_t ldnp);
+                                    double* normpart, int64_t ldnp, int64_t* groups = nullptr);
 
 // ---- k_sketch.hip ---------------------------------------------------------
 hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,
