@@ -1,0 +1,521 @@
+// One-wave panel kernels for blocks of at most 64 columns: the pivoted
+// Cholesky + robust filter of the sampled block's Gram matrix, and the second
+// Cholesky-QR pass (with R2^{-1} and R2 * Rb).
+//
+// Lane c owns column c.  The factor is kept transposed in LDS, one row per
+// column (RT[c][t] = R(t, c), pitch PW_LDE), so that step j of the
+// left-looking factorisation
+//   R(j, c) = (G(P, c) - sum_{t<j} R(t, P) R(t, c)) / R(j, j)
+// reads the pivot's row as 16-byte broadcast loads and its own row as 16-byte
+// per-lane loads (the pitch of 132 dwords puts the 16 lanes of a pass on
+// distinct banks), sixteen terms per batch.  The loops stay rolled: a single wave runs these kernels, so their
+// speed depends on the code fitting the instruction cache.  The pivot is the
+// largest Schur diagonal among the columns not yet chosen, the lowest position
+// on ties (a wave max and a ballot; the position tie-break runs only when
+// values tie); the Schur diagonals are downdated as d -= R(j, c)^2 by a fused
+// multiply-add.  The triangular inverses (back substitution) and R2 * Rb use
+// the same row-dot pattern.
+//
+// Reference: the pivoted QR of the sampled block and its filter
+// (gpu/rbrp.cu:166-234, rbrp.hpp CPQR + filter), CholQR2 of the kept columns
+// (gpu/rbrp.cu:236-266).
+#include <stdint.h>
+
+#include "rbrp_kernels.h"
+
+#define PW_N 64
+#define PW_LDO 65  // odd row pitch (doubles): lane-strided row access is conflict-free
+#define PW_LDE 66  // even row pitch: rows are 16-byte aligned for double2 loads
+#define PW_TILE_O (PW_N * PW_LDO)
+#define PW_TILE_E (PW_N * PW_LDE)
+
+namespace {
+
+template <int CTRL>
+__device__ __forceinline__ int pw_dpp(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double pw_dpp(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = pw_dpp<CTRL>((int)b), hi = pw_dpp<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Lanes l and l ^ 16 (W = 16) or l ^ 32 (W = 32): each lane receives its own
+// value in one slot and its partner's in the other.
+template <int W>
+__device__ __forceinline__ double pw_swap_max(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = (int)b, hi = (int)(b >> 32);
+  auto sl = W == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                    : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  auto sh = W == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                    : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const double a = __longlong_as_double(((long long)sh[0] << 32) | (unsigned int)sl[0]);
+  const double c = __longlong_as_double(((long long)sh[1] << 32) | (unsigned int)sl[1]);
+  return fmax(v, fmax(a, c));
+}
+template <int W>
+__device__ __forceinline__ int pw_swap_min(int v) {
+  auto s = W == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false)
+                   : __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return min(v, min((int)s[0], (int)s[1]));
+}
+// Wave-wide maximum / minimum; every lane ends with the result.
+__device__ __forceinline__ double pw_wave_max(double v) {
+  v = fmax(v, pw_dpp<0x128>(v));  // row_ror:8
+  v = fmax(v, pw_dpp<0x124>(v));  // row_ror:4
+  v = fmax(v, pw_dpp<0x122>(v));  // row_ror:2
+  v = fmax(v, pw_dpp<0x121>(v));  // row_ror:1
+  v = pw_swap_max<16>(v);
+  return pw_swap_max<32>(v);
+}
+__device__ __forceinline__ int pw_wave_min(int v) {
+  v = min(v, pw_dpp<0x128>(v));
+  v = min(v, pw_dpp<0x124>(v));
+  v = min(v, pw_dpp<0x122>(v));
+  v = min(v, pw_dpp<0x121>(v));
+  v = pw_swap_min<16>(v);
+  return pw_swap_min<32>(v);
+}
+__device__ __forceinline__ double pw_readlane(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// sum_{t < t1} x[t] * y[t] over two 16-byte aligned LDS rows (either may be
+// a broadcast row), in batches of sixteen terms with all sixteen 16-byte loads
+// of a batch issued together; the first batch is loaded unconditionally.
+// Terms in [t1, max(16, round16(t1))) must vanish (a zero in x or y) and
+// round16(t1) <= PW_N.  Four partial sums, term t into sum t mod 4.
+__device__ __forceinline__ double pw_ldot0(const double* x, const double* y, int t1) {
+  const double2* X = reinterpret_cast<const double2*>(x);
+  const double2* Y = reinterpret_cast<const double2*>(y);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int t = 0;
+  do {
+    double2 a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = X[t + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = Y[t + u];
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      s0 = fma(a[u].x, b[u].x, s0);
+      s1 = fma(a[u].y, b[u].y, s1);
+      s2 = fma(a[u + 1].x, b[u + 1].x, s2);
+      s3 = fma(a[u + 1].y, b[u + 1].y, s3);
+    }
+    t += 8;
+  } while (2 * t < t1);
+  return (s0 + s1) + (s2 + s3);
+}
+
+// Lane c zeroes its row of an even-pitch tile (PW_LDE doubles).
+__device__ __forceinline__ void pw_zero_row(double* T) {
+  double2* r = reinterpret_cast<double2*>(T + threadIdx.x * PW_LDE);
+#pragma unroll 11
+  for (int q = 0; q < PW_LDE / 2; ++q) r[q] = make_double2(0.0, 0.0);
+}
+// Lane r loads row r of the column-major X (n x n, coalesced across lanes),
+// every load in flight at once: v[q] = X(r, q), indices clamped to n - 1.
+__device__ __forceinline__ void pw_fetch_row(const double* __restrict__ X, int64_t ldx, int n,
+                                             double (&v)[PW_N]) {
+  const int rc = min((int)threadIdx.x, n - 1);
+#pragma unroll
+  for (int q = 0; q < PW_N; ++q) v[q] = X[rc + (int64_t)min(q, n - 1) * ldx];
+}
+// Y(r, q) = T[r * PW_LDO + q] for r, q < n (zero below the diagonal when
+// upper is set); lane r writes row r.
+__device__ __forceinline__ void pw_store_rows(const double* T, int n, bool upper,
+                                              double* __restrict__ Y, int64_t ldy) {
+  const int r = threadIdx.x;
+  if (r >= n) return;
+  double* y = Y + r;
+  for (int q0 = 0; q0 < n; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = T[r * PW_LDO + q0 + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < n) y[(int64_t)(q0 + u) * ldy] = (upper && r > q0 + u) ? 0.0 : v[u];
+  }
+}
+
+// 1 / sqrt(x) for a positive normal x: the hardware estimate refined by two
+// Newton steps (no division).
+__device__ __forceinline__ double pw_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return y;
+}
+
+// acc[q] += sum_{l in [l0, l1)} U[(r0 + q) * PW_LDE + l] * x[l] for q < 8: eight
+// broadcast rows of U against one row x (each 16-byte load of x feeds eight
+// rows; sixteen independent accumulation chains, even and odd l).  l0 is a
+// multiple of 8; terms in [l1, round8(l1)) must vanish and round8(l1) <= PW_N.
+__device__ __forceinline__ void pw_ldot8(const double* U, int r0, const double* x, int l0, int l1,
+                                         double (&acc)[8]) {
+  double e[8], o[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    e[q] = acc[q];
+    o[q] = 0.0;
+  }
+  const double* u = U + r0 * PW_LDE;
+#pragma unroll 4
+  for (int l = l0; l < l1; l += 2) {
+    const double2 xv = *reinterpret_cast<const double2*>(x + l);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double2 uv = *reinterpret_cast<const double2*>(u + q * PW_LDE + l);
+      e[q] = fma(uv.x, xv.x, e[q]);
+      o[q] = fma(uv.y, xv.y, o[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = e[q] + o[q];
+}
+
+// X = U^{-1} by back substitution, lane t computing column t, eight rows per
+// block from the bottom up.  U is bt x bt upper triangular with row r at
+// U + r * PW_LDE (zero left of the diagonal and in columns bt .. PW_N - 1, rows
+// bt .. round8(bt) zero); dinv[r] = 1 / U(r, r).  XT (lane t's row holds column
+// t of X) must be zero on entry; X(r, t) also goes to Xo[r * PW_LDO + t] for a
+// coalesced store.
+__device__ __forceinline__ void pw_upper_inverse(const double* U, const double* dinv, double* XT,
+                                                 double* Xo, int bt) {
+  const int t = threadIdx.x;
+  double* xr = XT + t * PW_LDE;
+  for (int r0 = (bt - 1) & ~7; r0 >= 0; r0 -= 8) {
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    pw_ldot8(U, r0, xr, r0 + 8, bt, acc);  // rows of X already computed
+    double x[8];
+#pragma unroll
+    for (int q = 7; q >= 0; --q) {
+      const int r = r0 + q;
+      double a = acc[q];
+#pragma unroll
+      for (int p = q + 1; p < 8; ++p) a = fma(U[r * PW_LDE + r0 + p], x[p], a);
+      const double dr = dinv[r];
+      x[q] = (r < bt && r <= t) ? ((r == t) ? dr : -a * dr) : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      xr[r0 + q] = x[q];
+      Xo[(r0 + q) * PW_LDO + t] = x[q];
+    }
+  }
+}
+
+}  // namespace
+
+// LDS (doubles): Gs (odd pitch) | RT (even) | Ws (even) | Bt (even) | three
+// vectors of PW_N.
+#define PW_LDS_DOUBLES (PW_TILE_O + 3 * PW_TILE_E + 3 * PW_N)
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_pivchol_wave(const double* __restrict__ G, int64_t ldg,
+                                                     int b, int filter_mode, int64_t i,
+                                                     const int* __restrict__ J,
+                                                     RbrpBlockState* bs, double* __restrict__ Rb,
+                                                     double* __restrict__ Rinv,
+                                                     int* __restrict__ Jk, int* __restrict__ Jin,
+                                                     int* __restrict__ Jout,
+                                                     int* __restrict__ Jkpos) {
+  extern __shared__ __align__(16) double lds[];
+  double* Gs = lds;                // Gram rows, later store staging (odd pitch)
+  double* RT = Gs + PW_TILE_O;     // RT[c][t] = R(t, c); later the inverse, transposed
+  double* Ws = RT + PW_TILE_E;     // kept factor, row-major
+  double* sqr = Ws + 2 * PW_TILE_E;
+  double* dinv = sqr + PW_N;
+  int* pos2col = reinterpret_cast<int*>(dinv + PW_N);
+  const int c = threadIdx.x;
+  const int s = bs->s;
+  if (s <= 0) {
+    if (c == 0) {
+      bs->bt = 0;
+      bs->nswap = 0;
+    }
+    return;
+  }
+  const int jv = J[min(c, s - 1)];  // sampled column index of lane c
+  {
+    double v[PW_N];
+    pw_fetch_row(G, ldg, s, v);
+    pw_zero_row(RT);
+#pragma unroll
+    for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // Gs[r][q] = G(r, q), r, q < s
+  }
+  const bool act = c < s;
+  double d = act ? Gs[c * PW_LDO + c] : 0.0;
+  int pos = c;
+  int rank = s;
+  double* myrow = RT + c * PW_LDE;
+  for (int j = 0; j < s; ++j) {
+    const double key = (act && pos >= j) ? d : -1.0;
+    const double m = pw_wave_max(key);
+    if (!(m > 0.0)) {
+      rank = j;
+      break;
+    }
+    const uint64_t ties = __builtin_amdgcn_ballot_w64(key == m);
+    int P, bp;
+    if (__builtin_popcountll(ties) == 1) {
+      P = (int)__builtin_ctzll(ties);
+      bp = __builtin_amdgcn_readlane(pos, P);
+    } else {
+      bp = pw_wave_min(key == m ? pos : 0x7fffffff);
+      P = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(key == m && pos == bp));
+    }
+    const double rjj = sqrt(m);
+    const double inv = pw_rsqrt(m);
+    const double g = Gs[P * PW_LDO + c];
+    const double acc = pw_ldot0(RT + P * PW_LDE, myrow, j);
+    // the column at position j moves to the pivot's position, the pivot to j
+    if (pos == j) pos = bp;
+    if (c == P) pos = j;
+    double val = 0.0;
+    if (act) {
+      if (c == P) {
+        val = rjj;
+      } else if (pos > j) {
+        val = (g - acc) * inv;
+        d = fma(-val, val, d);
+      }
+    }
+    myrow[j] = val;
+  }
+  if (act) pos2col[pos] = c;
+  // energy of factor row r (rows at and beyond the rank are zero)
+  {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (c < rank) {
+#pragma unroll 4
+      for (int q = 0; q < PW_N; q += 4) {
+        const double a0 = RT[q * PW_LDE + c], a1 = RT[(q + 1) * PW_LDE + c];
+        const double a2 = RT[(q + 2) * PW_LDE + c], a3 = RT[(q + 3) * PW_LDE + c];
+        s0 = fma(a0, a0, s0);
+        s1 = fma(a1, a1, s1);
+        s2 = fma(a2, a2, s2);
+        s3 = fma(a3, a3, s3);
+      }
+    }
+    sqr[c] = (s0 + s1) + (s2 + s3);
+  }
+  // robust filter: csum over the rows from the last one up (the same serial
+  // order in every lane); keep bt = s - k for the first k with csum >= thr
+  double total = 0.0;
+  for (int q0 = s - 1; q0 >= 0; q0 -= 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = sqr[max(q0 - u, 0)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 - u >= 0) total += v[u];
+  }
+  const double thr = total / (double)(filter_mode == 0 ? s : b);
+  int k = s;
+  {
+    double csum = 0.0;
+    for (int q0 = 0; q0 < s && k == s; q0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = sqr[max(s - 1 - q0 - u, 0)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k == s && q0 + u < s) {
+          csum += v[u];
+          if (csum >= thr) k = q0 + u;
+        }
+      }
+    }
+  }
+  const bool degenerate = rank == 0 || !(total > 0.0);
+  int bt = degenerate ? 0 : s - k;
+  if (bt > rank) bt = rank;
+  // kept factor in pivot order: column p = column pos2col[p] of R, rows <= p,
+  // row-major into Ws (broadcast rows) and Gs (store staging)
+  {
+    const int col = pos2col[min(c, max(bt - 1, 0))];
+    const double* src = RT + col * PW_LDE;
+    for (int t0 = 0; t0 < PW_N; t0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = src[t0 + u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double x = (c < bt && t0 + u <= c) ? v[u] : 0.0;
+        Ws[(t0 + u) * PW_LDE + c] = x;
+        Gs[(t0 + u) * PW_LDO + c] = x;
+      }
+    }
+  }
+  if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
+  pw_store_rows(Gs, bt, true, Rb, b);
+  pw_zero_row(RT);
+  pw_upper_inverse(Ws, dinv, RT, Gs, bt);
+  pw_store_rows(Gs, bt, true, Rinv, b);
+  // index sets: Jk (pivot order), Jin = sort(Jk) beyond the block, Jout =
+  // block positions not in Jk (ascending)
+  int v = 0x7fffffff;
+  {
+    const int col = pos2col[min(c, max(bt - 1, 0))];
+    const int jc = __builtin_amdgcn_ds_bpermute(col << 2, jv);  // J[col] from lane col
+    if (c < bt) {
+      v = jc;
+      Jk[c] = v;
+      if (Jkpos) Jkpos[c] = col;
+    }
+  }
+  const int q = (int)(i + c);
+  int rk = 0, member = 0;
+  for (int a = 0; a < bt; ++a) {
+    const int va = __builtin_amdgcn_readlane(v, a);
+    rk += (va < v) ? 1 : 0;
+    member |= (va == q) ? 1 : 0;
+  }
+  const bool flag = c < bt && !member;
+  const int nin = (int)__builtin_popcountll(
+      __builtin_amdgcn_ballot_w64(c < bt && (int64_t)v >= i + bt));
+  if (c < bt && rk >= bt - nin) Jin[rk - (bt - nin)] = v;
+  const uint64_t fl = __builtin_amdgcn_ballot_w64(flag);
+  if (flag) Jout[__builtin_popcountll(fl & ((1ull << c) - 1ull))] = q;
+  if (c == 0) {
+    const int nout = (int)__builtin_popcountll(fl);
+    if (degenerate) bs->status = 1;
+    if (nin != nout) bs->status = 3;
+    bs->bt = bt;
+    bs->nswap = nin;
+  }
+}
+
+// Second Cholesky-QR pass fused with the new diagonal block of R: G2 = R2^T R2,
+// R2inv = R2^{-1}, Rout = R2 * Rb (bt x bt, upper triangular, zeros below).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_chol2_wave(const double* __restrict__ G2, int64_t ldg,
+                                                   int b, RbrpBlockState* bs,
+                                                   const double* __restrict__ Rb,
+                                                   double* __restrict__ R2,
+                                                   double* __restrict__ R2inv,
+                                                   double* __restrict__ Rout, int64_t ldr) {
+  extern __shared__ __align__(16) double lds[];
+  double* Gs = lds;               // G2 rows, later store staging (odd pitch)
+  double* RT = Gs + PW_TILE_O;    // RT[c][t] = R2(t, c); later the inverse, transposed
+  double* Ws = RT + PW_TILE_E;    // R2, row-major
+  double* Bt = Ws + PW_TILE_E;    // Bt[c][l] = Rb(l, c)
+  double* dinv = Bt + PW_TILE_E;
+  const int c = threadIdx.x;
+  const int bt = bs->bt;
+  if (bt <= 0) return;
+  {
+    double v[PW_N];
+    pw_fetch_row(G2, ldg, bt, v);
+    pw_zero_row(RT);
+#pragma unroll
+    for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // G2(r, q), r, q < bt
+  }
+  // Rb transposed: Bt[q][c] = Rb(c, q) (zero below the diagonal and beyond bt)
+  {
+    double v[PW_N];
+    pw_fetch_row(Rb, b, bt, v);
+#pragma unroll
+    for (int q = 0; q < PW_N; ++q)
+      Bt[q * PW_LDE + c] = (c < bt && q < bt && c <= q) ? v[q] : 0.0;
+  }
+  double d = c < bt ? Gs[c * PW_LDO + c] : 0.0;
+  bool bad = false;
+  double* myrow = RT + c * PW_LDE;
+  // left-looking by blocks of eight columns: the terms of earlier blocks for
+  // all eight at once, then the in-block terms from the registers of lane j
+  for (int j0 = 0; j0 < bt; j0 += 8) {
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    pw_ldot8(RT, j0, myrow, 0, j0, acc);  // RT row j = column j of R2
+    double g[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) g[q] = Gs[(j0 + q) * PW_LDO + c];
+    double rv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = j0 + q;
+      rv[q] = 0.0;
+      if (j < bt) {
+        const double dj = pw_readlane(d, j);
+        double a = acc[q];
+#pragma unroll
+        for (int p = 0; p < q; ++p) a = fma(pw_readlane(rv[p], j), rv[p], a);
+        const double dd = dj > 0.0 ? dj : 1.0;
+        const double rjj = sqrt(dd);
+        const double inv = pw_rsqrt(dd);
+        bad |= !(dj > 0.0);
+        if (c < bt) {
+          if (c == j) {
+            rv[q] = rjj;
+          } else if (c > j) {
+            rv[q] = (g[q] - a) * inv;
+            d = fma(-rv[q], rv[q], d);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
+  }
+  if (c == 0 && bad) bs->status = 2;
+  // R2 row-major into Ws (broadcast rows) and Gs (store staging)
+  for (int t0 = 0; t0 < PW_N; t0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = myrow[t0 + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double x = (c < bt && t0 + u < bt) ? v[u] : 0.0;
+      Ws[(t0 + u) * PW_LDE + c] = x;
+      Gs[(t0 + u) * PW_LDO + c] = x;
+    }
+  }
+  if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
+  pw_store_rows(Gs, bt, true, R2, b);
+  // Rout(r, c) = sum_{l >= r} R2(r, l) Rb(l, c), eight rows at a time
+  for (int r0 = 0; r0 < bt; r0 += 8) {
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    pw_ldot8(Ws, r0, Bt + c * PW_LDE, r0, bt, acc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Gs[(r0 + q) * PW_LDO + c] = acc[q];
+  }
+  pw_store_rows(Gs, bt, true, Rout, ldr);
+  pw_zero_row(RT);
+  pw_upper_inverse(Ws, dinv, RT, Gs, bt);
+  pw_store_rows(Gs, bt, true, R2inv, b);
+}
+
+static size_t pw_lds_bytes() { return (size_t)PW_LDS_DOUBLES * sizeof(double); }
+
+hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg, int b,
+                                    int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                    double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                    int* Jkpos) {
+  if (b > PW_N) return hipErrorInvalidValue;
+  const size_t lds = pw_lds_bytes();
+  hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_wave,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pivchol_wave, dim3(1), dim3(64), lds, st, G, ldg, b, filter_mode, i, J, bs,
+                     Rb, Rinv, Jk, Jin, Jout, Jkpos);
+  return hipGetLastError();
+}
+
+hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg, int b,
+                                  RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
+                                  double* Rout, int64_t ldr) {
+  if (b > PW_N) return hipErrorInvalidValue;
+  const size_t lds = pw_lds_bytes();
+  hipError_t e = hipFuncSetAttribute((const void*)k_chol2_wave,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_chol2_wave, dim3(1), dim3(64), lds, st, G2, ldg, b, bs, Rb, R2, R2inv, Rout,
+                     ldr);
+  return hipGetLastError();
+}

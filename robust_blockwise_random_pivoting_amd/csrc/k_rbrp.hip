@@ -535,412 +535,14 @@ __global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict
   }
 }
 
-// ---------------------------------------------------------------------------
-// One-wave panel kernels for blocks of at most 64 columns.  Lane c owns
-// sampled column c; the factor rows live in LDS as R[t][c] (row pitch
-// PW_LD), so step j of the left-looking factorisation reads its own column
-// (conflict-free) and the pivot column (a broadcast).  No workgroup barrier
-// is involved beyond one-wave LDS ordering.  The Schur diagonals are downdated
-// by the same fused multiply-add sequence as the right-looking kernels above
-// (k_pivchol_filter / k_chol2); the factor rows are formed as
-// G(P, c) - sum_t R(t, P) R(t, c) with four interleaved partial sums.
-// ---------------------------------------------------------------------------
-#define PW_N 64
-#define PW_LD 65
-#define PW_TILE ((PW_N + 8) * PW_LD)  // one LDS tile: 72 rows (8 rows of zero padding)
-
-// Cross-lane moves that stay in the VALU (no LDS round trip): DPP rotations
-// within a 16-lane row and the gfx950 row-pair / half-wave swaps.
-template <int CTRL>
-__device__ inline int pw_dpp(int x) {
-  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL>
-__device__ inline double pw_dpp(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = pw_dpp<CTRL>((int)b), hi = pw_dpp<CTRL>((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ inline void pw_amax_take(double& bv, int& bp, int& bc, double ov, int op, int oc) {
-  if (ov > bv || (ov == bv && op < bp)) {
-    bv = ov;
-    bp = op;
-    bc = oc;
-  }
-}
-template <int CTRL>
-__device__ inline void pw_amax_dpp(double& bv, int& bp, int& bc) {
-  pw_amax_take(bv, bp, bc, pw_dpp<CTRL>(bv), pw_dpp<CTRL>(bp), pw_dpp<CTRL>(bc));
-}
-// Lanes l and l ^ 16 (W = 16) or l ^ 32 (W = 32) exchange their candidates.
-template <int W>
-__device__ inline void pw_amax_swap(double& bv, int& bp, int& bc) {
-  const long long b = __double_as_longlong(bv);
-  int lo = (int)b, hi = (int)(b >> 32);
-  auto sl = W == 16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
-                    : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-  auto sh = W == 16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
-                    : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-  auto sp = W == 16 ? __builtin_amdgcn_permlane16_swap(bp, bp, false, false)
-                    : __builtin_amdgcn_permlane32_swap(bp, bp, false, false);
-  auto sc = W == 16 ? __builtin_amdgcn_permlane16_swap(bc, bc, false, false)
-                    : __builtin_amdgcn_permlane32_swap(bc, bc, false, false);
-  // each lane receives its own candidate in one slot and its partner's in the other
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const double ov = __longlong_as_double(((long long)sh[q] << 32) | (unsigned int)sl[q]);
-    pw_amax_take(bv, bp, bc, ov, sp[q], sc[q]);
-  }
-}
-// Wave-wide argmax of (value, position) -- largest value, lowest position on
-// ties -- carrying a column index; every lane ends with the same triple.
-__device__ inline void pw_wave_argmax(double& bv, int& bp, int& bc) {
-  pw_amax_dpp<0x128>(bv, bp, bc);  // row_ror:8
-  pw_amax_dpp<0x124>(bv, bp, bc);  // row_ror:4
-  pw_amax_dpp<0x122>(bv, bp, bc);  // row_ror:2
-  pw_amax_dpp<0x121>(bv, bp, bc);  // row_ror:1
-  pw_amax_swap<16>(bv, bp, bc);
-  pw_amax_swap<32>(bv, bp, bc);
-}
-__device__ inline double pw_readlane(double x, int lane) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// sum_{t in [t0, t1)} X[t * PW_LD + p] * Y[t * PW_LD + q] where the caller
-// guarantees that the terms for t in [t0 & ~7, t0) and [t1, round8(t1)) are
-// exact zeros (zero-initialised LDS tiles): the sum runs over whole batches
-// of eight with all sixteen loads of a batch in flight and four partial sums.
-__device__ inline double pw_dot(const double* X, int p, const double* Y, int q, int t0, int t1) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int t = t0 & ~7; t < t1; t += 8) {
-    double x[8], y[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      x[u] = X[(t + u) * PW_LD + p];
-      y[u] = Y[(t + u) * PW_LD + q];
-    }
-    s0 = fma(x[0], y[0], s0);
-    s1 = fma(x[1], y[1], s1);
-    s2 = fma(x[2], y[2], s2);
-    s3 = fma(x[3], y[3], s3);
-    s0 = fma(x[4], y[4], s0);
-    s1 = fma(x[5], y[5], s1);
-    s2 = fma(x[6], y[6], s2);
-    s3 = fma(x[7], y[7], s3);
-  }
-  return (s0 + s1) + (s2 + s3);
-}
-
-// sum_{l in [l0, l1)} xr[l] * Y[l * PW_LD + q] (xr is a row of a tile), with
-// the same zero-padding contract as pw_dot.
-__device__ inline double pw_dot_row(const double* xr, const double* Y, int q, int l0, int l1) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int l = l0 & ~7; l < l1; l += 8) {
-    double x[8], y[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      x[u] = xr[l + u];
-      y[u] = Y[(l + u) * PW_LD + q];
-    }
-    s0 = fma(x[0], y[0], s0);
-    s1 = fma(x[1], y[1], s1);
-    s2 = fma(x[2], y[2], s2);
-    s3 = fma(x[3], y[3], s3);
-    s0 = fma(x[4], y[4], s0);
-    s1 = fma(x[5], y[5], s1);
-    s2 = fma(x[6], y[6], s2);
-    s3 = fma(x[7], y[7], s3);
-  }
-  return (s0 + s1) + (s2 + s3);
-}
-
-// sum_{l < n} xr[l]^2 for a tile row zero beyond n (up to round8(n)).
-__device__ inline double pw_rowsq(const double* xr, int n) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int l = 0; l < n; l += 8) {
-    double x[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) x[u] = xr[l + u];
-    s0 = fma(x[0], x[0], s0);
-    s1 = fma(x[1], x[1], s1);
-    s2 = fma(x[2], x[2], s2);
-    s3 = fma(x[3], x[3], s3);
-    s0 = fma(x[4], x[4], s0);
-    s1 = fma(x[5], x[5], s1);
-    s2 = fma(x[6], x[6], s2);
-    s3 = fma(x[7], x[7], s3);
-  }
-  return (s0 + s1) + (s2 + s3);
-}
-
-// Zero rows 0 .. PW_N + 7 of an LDS tile (lane = column), so that padded
-// batches read exact zeros.
-__device__ inline void pw_zero(double* X) {
-  const int c = threadIdx.x;
-#pragma unroll 8
-  for (int r = 0; r < PW_N + 8; ++r) X[r * PW_LD + c] = 0.0;
-}
-
-// Xs[r * PW_LD + q] = X(r, q) for r, q < n of the column-major X (ldx): lane r
-// reads row r, eight columns per batch (indices clamped, stores predicated).
-__device__ inline void pw_load(const double* __restrict__ X, int64_t ldx, int n, double* Xs) {
-  const int r = threadIdx.x;
-  const int rc = min(r, n - 1);
-  for (int q0 = 0; q0 < n; q0 += 8) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = X[rc + (int64_t)min(q0 + u, n - 1) * ldx];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (r < n && q0 + u < n) Xs[r * PW_LD + q0 + u] = v[u];
-  }
-}
-// Y(r, c) = Xs[r * PW_LD + c] for r, c < n, zero below the diagonal when
-// upper is set; lane r writes row r (one 512-byte run per column).
-__device__ inline void pw_store(const double* Xs, int n, bool upper, double* __restrict__ Y,
-                                int64_t ldy) {
-  const int r = threadIdx.x;
-  for (int c0 = 0; c0 < n; c0 += 8) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = Xs[r * PW_LD + c0 + u];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (r < n && c0 + u < n) Y[r + (int64_t)(c0 + u) * ldy] = (upper && r > c0 + u) ? 0.0 : v[u];
-  }
-}
-
-// X = U^{-1} for the bt x bt upper-triangular U (U[r][l] at Us[r * PW_LD + l],
-// zero below the diagonal and in columns bt .. round8(bt)); X[r][t] at
-// Xs[r * PW_LD + t], which must be zero on entry.  Lane t computes column t by back
-// substitution; the row-r sum over l in [r+1, t] runs over whole batches of
-// eight because the extra terms meet zeros of U (l <= r or beyond bt) or of
-// X (rows not yet computed, rows below the diagonal).
-__device__ inline void pw_upper_inverse(const double* Us, double* Xs, int bt) {
-  const int t = threadIdx.x;
-  if (t < bt) Xs[t * PW_LD + t] = 1.0 / Us[t * PW_LD + t];
-  for (int r = bt - 2; r >= 0; --r) {
-    // uniform bounds: the widest lane (t = bt - 1) sets the range
-    const double acc = pw_dot_row(Us + r * PW_LD, Xs, t, r + 1, bt);
-    if (t < bt && t > r) Xs[r * PW_LD + t] = -acc / Us[r * PW_LD + r];
-  }
-  __syncthreads();
-}
-
-__global__ __launch_bounds__(64) void k_pivchol_wave(const double* __restrict__ G, int64_t ldg,
-                                                     int b, int filter_mode, int64_t i,
-                                                     const int* __restrict__ J,
-                                                     RbrpBlockState* bs, double* __restrict__ Rb,
-                                                     double* __restrict__ Rinv,
-                                                     int* __restrict__ Jk, int* __restrict__ Jin,
-                                                     int* __restrict__ Jout,
-                                                     int* __restrict__ Jkpos) {
-  extern __shared__ double lds[];
-  double* Gs = lds;                // Gram, later the inverse
-  double* R = lds + PW_TILE;       // R[t][c]: step t, sampled column c
-  double* Rp = lds + 2 * PW_TILE;  // kept factor in pivot order
-  __shared__ int pos2col[PW_N];
-  __shared__ int sj[PW_N];
-  __shared__ int sorted[PW_N];
-  __shared__ int flag[PW_N];
-  __shared__ double sqr[PW_N];
-  __shared__ int s_bt;
-  const int c = threadIdx.x;
-  const int s = bs->s;
-  if (s <= 0) {
-    if (c == 0) {
-      bs->bt = 0;
-      bs->nswap = 0;
-    }
-    return;
-  }
-  pw_zero(R);
-  pw_zero(Rp);
-  pw_load(G, ldg, s, Gs);  // Gs[r][q] = G(r, q)
-  __syncthreads();
-  const bool act = c < s;
-  double d = act ? Gs[c * PW_LD + c] : 0.0;
-  int pos = c;
-  int rank = s;
-  for (int j = 0; j < s; ++j) {
-    // pivot: largest Schur diagonal among positions >= j, lowest position on ties
-    double bv = -1.0;
-    int bp = 0x7fffffff, bc = -1;
-    if (act && pos >= j) pw_amax_take(bv, bp, bc, d, pos, c);
-    pw_wave_argmax(bv, bp, bc);
-    if (!(bv > 0.0)) {
-      rank = j;
-      break;
-    }
-    const int P = bc;
-    const double rjj = sqrt(bv);
-    // the column at position j moves to the pivot's position, the pivot to j
-    if (pos == j) pos = bp;
-    if (c == P) pos = j;
-    if (act) {
-      double val = 0.0;
-      if (c == P) {
-        val = rjj;
-      } else if (pos > j) {
-        val = (Gs[P * PW_LD + c] - pw_dot(R, P, R, c, 0, j)) / rjj;
-        d = fma(-val, val, d);
-      }
-      R[j * PW_LD + c] = val;
-    }
-    __syncthreads();
-  }
-  for (int j = rank; j < s; ++j)
-    if (act) R[j * PW_LD + c] = 0.0;
-  if (act) pos2col[pos] = c;
-  __syncthreads();
-  if (act) sqr[c] = pw_rowsq(R + c * PW_LD, s);  // lane r: energy of factor row r
-  __syncthreads();
-  if (c == 0) {
-    double total = 0.0;
-    for (int t = 0; t < s; ++t) total += sqr[s - 1 - t];
-    const double thr = total / (double)(filter_mode == 0 ? s : b);
-    double csum = 0.0;
-    int k = 0;
-    for (; k < s; ++k) {
-      csum += sqr[s - 1 - k];
-      if (csum >= thr) break;
-    }
-    int bt = s - k;
-    if (rank == 0 || !(total > 0.0)) {
-      bt = 0;
-      bs->status = 1;
-    }
-    if (bt > rank) bt = rank;
-    s_bt = bt;
-  }
-  __syncthreads();
-  const int bt = s_bt;
-  if (c < bt) {
-    const int col = pos2col[c];
-    Jk[c] = J[col];
-    sj[c] = J[col];
-    if (Jkpos) Jkpos[c] = col;
-  }
-  // column c (pivot order) of the kept factor: rows <= c (the rest stay zero)
-  for (int r0 = 0; r0 < bt; r0 += 8) {
-    double v[8];
-    const int col = pos2col[min(c, bt - 1)];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = R[(r0 + u) * PW_LD + col];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (c < bt && r0 + u <= c) Rp[(r0 + u) * PW_LD + c] = v[u];
-  }
-  pw_zero(Gs);  // the inverse is built over the Gram tile
-  __syncthreads();
-  pw_store(Rp, bt, true, Rb, b);
-  pw_upper_inverse(Rp, Gs, bt);
-  pw_store(Gs, bt, true, Rinv, b);
-  // ascending sort of Jk by rank counting (entries are distinct)
-  if (c < bt) {
-    const int v = sj[c];
-    int rk = 0;
-    for (int q = 0; q < bt; ++q) rk += (sj[q] < v) ? 1 : 0;
-    sorted[rk] = v;
-  }
-  // Jout membership flags for the block positions i .. i+bt-1
-  if (c < bt) {
-    const int q = (int)(i + c);
-    int member = 0;
-    for (int a = 0; a < bt; ++a) member |= (sj[a] == q) ? 1 : 0;
-    flag[c] = !member;
-  }
-  __syncthreads();
-  int nin = 0;
-  for (int a = 0; a < bt; ++a) nin += ((int64_t)sorted[a] >= i + bt) ? 1 : 0;
-  if (c < nin) Jin[c] = sorted[bt - nin + c];
-  if (c < bt && flag[c]) {
-    int p = 0;
-    for (int q = 0; q < c; ++q) p += flag[q];
-    Jout[p] = (int)(i + c);
-  }
-  if (c == 0) {
-    int nout = 0;
-    for (int t = 0; t < bt; ++t) nout += flag[t];
-    if (nin != nout) bs->status = 3;
-    bs->bt = bt;
-    bs->nswap = nin;
-  }
-}
-
-// Second Cholesky-QR pass fused with the new diagonal block of R: G2 = R2^T R2,
-// R2inv = R2^{-1}, Rout = R2 * Rb (bt x bt, upper triangular, zeros below).
-__global__ __launch_bounds__(64) void k_chol2_wave(const double* __restrict__ G2, int64_t ldg,
-                                                   int b, RbrpBlockState* bs,
-                                                   const double* __restrict__ Rb,
-                                                   double* __restrict__ R2,
-                                                   double* __restrict__ R2inv,
-                                                   double* __restrict__ Rout, int64_t ldr) {
-  extern __shared__ double lds[];
-  double* Gs = lds;                 // G2, later R2^{-1}
-  double* R = lds + PW_TILE;        // R2
-  double* Rbs = lds + 2 * PW_TILE;  // Rb, later R2 Rb
-  const int c = threadIdx.x;
-  const int bt = bs->bt;
-  if (bt <= 0) return;
-  pw_zero(R);
-  pw_zero(Rbs);
-  pw_load(G2, ldg, bt, Gs);
-  pw_load(Rb, b, bt, Rbs);
-  __syncthreads();
-  const bool act = c < bt;
-  double d = act ? Gs[c * PW_LD + c] : 0.0;
-  bool bad = false;
-  for (int j = 0; j < bt; ++j) {
-    const double dj = pw_readlane(d, j);
-    const double rjj = sqrt(dj > 0.0 ? dj : 1.0);
-    bad |= !(dj > 0.0);
-    if (act) {
-      double val = 0.0;
-      if (c == j) {
-        val = rjj;
-      } else if (c > j) {
-        val = (Gs[j * PW_LD + c] - pw_dot(R, j, R, c, 0, j)) / rjj;
-        d = fma(-val, val, d);
-      }
-      R[j * PW_LD + c] = val;
-    }
-    __syncthreads();
-  }
-  if (c == 0 && bad) bs->status = 2;
-  pw_zero(Gs);  // the inverse is built over the Gram tile
-  pw_store(R, bt, true, R2, b);
-  // column c of R2 * Rb, in place over column c of Rb: row r needs Rb(l, c)
-  // for l >= r only, and the rows are produced in ascending order
-  // (the padded terms meet R2's zeros left of the diagonal and Rb's zeros
-  // below it)
-  for (int r = 0; r < bt; ++r) {
-    const double acc = pw_dot_row(R + r * PW_LD, Rbs, c, r, bt);
-    if (act && r <= c) Rbs[r * PW_LD + c] = acc;
-  }
-  __syncthreads();
-  pw_store(Rbs, bt, true, Rout, ldr);
-  pw_upper_inverse(R, Gs, bt);
-  pw_store(Gs, bt, true, R2inv, b);
-}
-
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
                                       int* Jkpos) {
   if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
-  if (b <= PW_N) {
-    const size_t lds = 3 * PW_TILE * sizeof(double);
-    hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_wave,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pivchol_wave, dim3(1), dim3(64), lds, st, G, ldg, b, filter_mode, i, J, bs,
-                       Rb, Rinv, Jk, Jin, Jout, Jkpos);
-    return hipGetLastError();
-  }
+  if (b <= 64)  // one-wave kernel
+    return rbrp_launch_pivchol_wave(st, G, ldg, b, filter_mode, i, J, bs, Rb, Rinv, Jk, Jin, Jout,
+                                    Jkpos);
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_filter,
@@ -995,15 +597,7 @@ __global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, in
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
                              RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
                              double* Rout, int64_t ldr) {
-  if (b <= PW_N) {
-    const size_t lds = 3 * PW_TILE * sizeof(double);
-    hipError_t e = hipFuncSetAttribute((const void*)k_chol2_wave,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chol2_wave, dim3(1), dim3(64), lds, st, G2, ldg, b, bs, Rb, R2, R2inv, Rout,
-                       ldr);
-    return hipGetLastError();
-  }
+  if (b <= 64) return rbrp_launch_chol2_wave(st, G2, ldg, b, bs, Rb, R2, R2inv, Rout, ldr);
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
     hipError_t e = hipFuncSetAttribute((const void*)k_chol2,

@@ -112,6 +112,14 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
                              RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
                              double* Rout, int64_t ldr);
+// One-wave forms of the two launches above (b <= 64, k_panel.hip).
+hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg, int b,
+                                    int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                    double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                    int* Jkpos);
+hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg, int b,
+                                  RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
+                                  double* Rout, int64_t ldr);
 hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
                                   const int* Jin, const int* Jout, int* perm, double* diags);
 hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
