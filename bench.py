@@ -15,6 +15,8 @@ host).  Further legs in the same line:
               50000 x 4000 exp-decay and the 100000 x 2000 Gaussian-mixture matrices;
   batched  -- 4096 Gaussian-kernel blocks (m ~ U[64,1024], n ~ U[64,512]), b = 16,
               tol = 1e-7: one persistent launch vs one launch sequence per problem;
+  scale    -- 1,000,000 x 1024 FP32-stored matrix (s_i = 1/i, 500 Cauchy rows),
+              b = 128, tol = 1e-5: update GB/s and projection MFMA utilisation;
   comparators -- RandCPQR_OS / RandLUPP_OS at the rank RBRP found.
 """
 
@@ -34,6 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = "RBRP ID time-to-tol, mixed vs FP64 on 1 GPU at equal rel. error; batched IDs/sec"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
 
 
 def parse():
@@ -60,10 +63,12 @@ def parse():
                     help="write the generated matrix (column-major .npy) and exit")
     ap.add_argument("--load-matrix", default="",
                     help="read the matrix from this .npy instead of generating it")
-    ap.add_argument("--legs", default="main,comparators,mixed,batched,cpu",
+    ap.add_argument("--legs", default="main,comparators,mixed,batched,scale,cpu",
                     help="comma list of legs to run (main = the timed FP64 IDs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--scale-m", type=int, default=1000000,
+                    help="rows of the 1024-column FP32 matrix of the scale leg")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-batch-loop", action="store_true",
                     help="skip the per-problem launch loop of the batched leg")
@@ -75,6 +80,7 @@ def parse():
     args.no_mixed = args.no_mixed or "mixed" not in legs
     args.no_batched = args.no_batched or "batched" not in legs
     args.no_cpu = args.no_cpu or "cpu" not in legs
+    args.no_scale = "scale" not in legs
     args.no_main = "main" not in legs
     return args
 
@@ -151,6 +157,57 @@ def mixed_leg(A2, args):
             if A is not A2:
                 del A
                 torch.cuda.empty_cache()
+    return out
+
+
+def scale_leg(args):
+    """Mixed form at scale: 1,000,000 x 1024 FP32-stored matrix, s_i = 1/i (Haar U, V),
+    500 rows replaced by standard-Cauchy rows; b = 128, tol = 1e-5.  The whole
+    matrix (4.1 GB) is resident in HBM.  Reports the wall time of the ID, the
+    per-phase device time, the update's HBM GB/s and the projection's FP32 MFMA
+    utilisation, and the FP64 form on the widened matrix for comparison."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import Handle, rbrp_colid
+    from robust_blockwise_random_pivoting_amd.matrices import poly_outlier_torch
+    m, n, b, tol = args.scale_m, 1024, 128, 1e-5
+    A = poly_outlier_torch(m, n, seed=5)
+    _sync()
+    print(f"[bench] scale: {m}x{n} FP32 matrix ready", file=sys.stderr, flush=True)
+    out = {"m": m, "n": n, "b": b, "tol": tol, "stored": "fp32"}
+    with Handle() as h:
+        r32, t32 = time_colid(A, b, tol, args.seed, "fp32", h, 2)
+        h.set_timing(True)
+        h.reset_stats()
+        rbrp_colid(A, b=b, tol=tol, seed=args.seed, precision="fp32", handle=h)
+        _sync()
+        st = h.stats()
+        h.set_timing(False)
+        upd, prj = st["update"], st["project"]
+        out.update({
+            "fp32_ms": t32 * 1e3, "k": r32.k, "blocks": len(r32.ranks),
+            "tracked_error": r32.err_final,
+            "phase_ms": {k: v["ms"] for k, v in st.items()},
+            "update_GBs": upd["bytes"] / (upd["ms"] * 1e-3) / 1e9 if upd["ms"] > 0 else None,
+            "update_hbm_frac": (upd["bytes"] / (upd["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS
+                                if upd["ms"] > 0 else None),
+            "project_TFLOPs": prj["flops"] / (prj["ms"] * 1e-3) / 1e12 if prj["ms"] > 0 else None,
+            "project_mfma_frac_fp32": (prj["flops"] / (prj["ms"] * 1e-3) / 1e12 / PEAK_FP32_TFLOPS
+                                       if prj["ms"] > 0 else None),
+            "project_GBs": prj["bytes"] / (prj["ms"] * 1e-3) / 1e9 if prj["ms"] > 0 else None,
+        })
+        print(f"[bench] scale fp32: {t32 * 1e3:.1f} ms k={r32.k} update "
+              f"{out['update_GBs']:.0f} GB/s project {out['project_GBs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+        A64 = A.double()
+        r64, t64 = time_colid(A64, b, tol, args.seed, "fp64", h, 1)
+        out.update({"fp64_ms": t64 * 1e3, "k_fp64": r64.k, "speedup": t64 / t32,
+                    "tracked_error_fp64": r64.err_final,
+                    "rel_error_fp32": id_rel_error(A64, r32),
+                    "rel_error_fp64": id_rel_error(A64, r64)})
+        print(f"[bench] scale fp64: {t64 * 1e3:.1f} ms k={r64.k}", file=sys.stderr, flush=True)
+        del A64
+    del A
+    torch.cuda.empty_cache()
     return out
 
 
@@ -287,7 +344,8 @@ def main():
 
     if args.no_main:
         out = {"mixed": None if args.no_mixed else mixed_leg(A, args),
-               "batched": None if args.no_batched else batched_leg(args)}
+               "batched": None if args.no_batched else batched_leg(args),
+               "scale": None if args.no_scale else scale_leg(args)}
         print(json.dumps(out))
         return
 
@@ -360,6 +418,7 @@ def main():
 
     mixed = None if args.no_mixed else mixed_leg(A, args)
     batched = None if args.no_batched else batched_leg(args)
+    scale = None if args.no_scale else scale_leg(args)
 
     cpu = None
     if not args.no_cpu:
@@ -393,6 +452,7 @@ def main():
         "comparators": comps,
         "mixed": mixed,
         "batched": batched,
+        "scale": scale,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -67,6 +67,33 @@ def spectrum_matrix_torch(m: int, n: int, s: np.ndarray, seed: int, device="cuda
     return At.t()
 
 
+def poly_outlier_torch(m: int, n: int, seed: int, power: float = 1.0, n_outlier=None,
+                       device="cuda"):
+    """FP32-stored ``A = U diag(s) V^T`` with ``s_i = i^-power`` (Haar U, V, built in
+    FP64 on the device), then ``n_outlier`` rows (default 0.05% of m) replaced by
+    i.i.d. standard-Cauchy rows.  Returns a column-major (m, n) float32 view."""
+    import torch
+    if n_outlier is None:
+        n_outlier = int(round(5e-4 * m))
+    r = min(m, n)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    V = haar_torch(n, r, gen, device)
+    U = haar_torch(m, r, gen, device)
+    st = torch.as_tensor(poly_spectrum(r, power), device=device)
+    At = torch.empty((n, m), dtype=torch.float32, device=device)   # column-major (m, n)
+    step = 1 << 17
+    for r0 in range(0, m, step):                                   # bounded FP64 temporaries
+        r1 = min(m, r0 + step)
+        At[:, r0:r1] = (V @ (U[r0:r1] * st).t()).float()
+    del U
+    rows = torch.randperm(m, generator=gen, device=device)[:n_outlier]
+    out = torch.empty((n_outlier, n), dtype=torch.float32, device=device)
+    out.cauchy_(generator=gen)
+    At[:, rows] = out.t()
+    return At.t()
+
+
 def optimal_error(s: np.ndarray, k: int) -> float:
     """Best rank-k relative Frobenius error ``sqrt(sum_{i>k} s_i^2) / ||s||``."""
     s = np.asarray(s, dtype=np.float64)
