@@ -37,6 +37,18 @@ int rbrp_test_gemm_nn(void* stream, int64_t M, int64_t N, int64_t K, const doubl
 int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                        const double* Bt, int64_t ldbt, double* C, int64_t ldc, double* normpart);
 
+/* FP64 projection kernel (64 x 128 tiles): C (M x N, ldc) = A^T B with A
+ * (K x M) and B (K x N) column-major with the common leading dimension ld (even),
+ * K a multiple of 16, split over about nsplit chunks of K. */
+int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A,
+                        const double* B, int64_t ld, double* C, int64_t ldc, int nsplit);
+
+/* FP64 tall NN kernel (64 x 64 tiles, long reduction): C (M x N, ldc) (op)=
+ * A B with A (M x K, lda even) and B (K x N, ldb) column-major; M a multiple
+ * of 4; mode 0 sets, 1 subtracts, 2 adds. */
+int rbrp_test_gemm_nn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double* C, int64_t ldc, int mode);
+
 /* FP32 split-K TN kernel: C (M x N, FP64, ldc) = A^T B with A (K x M) and B
  * (K x N) FP32 column-major with the common leading dimension ld (a multiple
  * of 4), FP32 products and accumulation per split, FP64 split sum.  K must be

@@ -41,6 +41,7 @@ def compile_flags():
         "-fPIC",
         "-Wall",
         "-Wno-unused-function",
+        "-Werror=uninitialized",
         # no packed-FP32 instructions anywhere in the library
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-I", os.path.join(ROOT, "include"),

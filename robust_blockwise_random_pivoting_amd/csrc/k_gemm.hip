@@ -118,6 +118,312 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 }
 
 // ---------------------------------------------------------------------------
+// Split-K TN for the projection W = Q_b^T A_rem and the re-orthogonalisation
+// coefficients: P[z](r, c) = sum_{kk in chunk z} A(kk, r) B(kk, c) for
+// column-major operands (MatRef: plain, offset or index-listed columns).
+//
+// Workgroup tile: 64 rows (blockIdx.y) x 128 columns; wave w owns columns
+// 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row blocks of
+// A.  Each 16-row step of both operands is staged through double-buffered
+// LDS ([column][row], one 128-byte run per column and step; the eight 16-byte
+// units of a column are XOR-swizzled with bits 1..3 of the column index so
+// that the 16 lanes of a pass read distinct banks): the next step's global
+// loads are issued into staging registers before the current step's MFMAs and
+// written to LDS after them (one barrier per step).  Lane (j, g) reads rows
+// 4g .. 4g+3 of its column as two ds_read_b128 and MFMA q of the step consumes
+// element q, so the four MFMAs of a block cover the 16 rows in the order
+// 4g + q.  The MFMA computes the transposed block (B's columns as its rows):
+// lane l holds P(r0 + l%16, c0 + 4v + l/16), and each partial store
+// instruction writes four 128-byte runs of P.  K and kchunk are multiples of
+// 16 and both leading dimensions are even; rows >= the true height are zero.
+// ---------------------------------------------------------------------------
+#define TN64_KS 16
+// LDS offset (doubles) of 16-byte unit u (rows 2u, 2u+1) of staged column col
+#define TN64_AT(col, u) ((col) * TN64_KS + 2 * ((u) ^ (((col) >> 1) & 7)))
+
+__global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, int64_t K,
+                                                      int64_t kchunk, MatRef A, MatRef B,
+                                                      double* __restrict__ P, int64_t ldp,
+                                                      int64_t pstride) {
+  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t n0 = (int64_t)blockIdx.x * 128, m0 = (int64_t)blockIdx.y * 64;
+  if (m0 >= M || n0 >= N) return;  // uniform per workgroup
+  const int64_t kb0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t kb1 = min(K, kb0 + kchunk);
+  if (kb0 >= kb1) return;  // empty chunk (uniform); its partials are never read
+
+  __shared__ double As[2][64 * TN64_KS];   // [buf][col][row], swizzled
+  __shared__ double Bs[2][128 * TN64_KS];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lj = lane & 15, lg = lane >> 4;
+  // Stagers.  Out-of-range columns are clamped to the last valid one: they
+  // only feed output rows / columns that are never stored.
+  const int ac = tid >> 2, aq = (tid & 3) * 4;   // A: column ac, rows aq .. aq+3
+  const int bc = tid >> 1, bq = (tid & 1) * 8;   // B: column bc, rows bq .. bq+7
+  const double* ap = mat_col(A, min(m0 + ac, M - 1)) + aq;
+  const double* bp = mat_col(B, min(n0 + bc, N - 1)) + bq;
+  const int au = aq >> 1, bu = bq >> 1;  // first 16-byte unit of the stager
+
+  v4d acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  // Two sets of staging registers (named scalars: small arrays here get
+  // placed in LDS): the loads of step k+2 are issued before the MFMAs of step
+  // k, and step k+1's registers are written to LDS after them.
+#define TN64_LOAD(S, kk)                                  \
+  do {                                                    \
+    S##a0 = *(const double2*)(ap + (kk));                 \
+    S##a1 = *(const double2*)(ap + (kk) + 2);             \
+    S##b0 = *(const double2*)(bp + (kk));                 \
+    S##b1 = *(const double2*)(bp + (kk) + 2);             \
+    S##b2 = *(const double2*)(bp + (kk) + 4);             \
+    S##b3 = *(const double2*)(bp + (kk) + 6);             \
+  } while (0)
+#define TN64_STORE(S, bb)                                 \
+  do {                                                    \
+    *(double2*)&As[bb][TN64_AT(ac, au)] = S##a0;          \
+    *(double2*)&As[bb][TN64_AT(ac, au + 1)] = S##a1;      \
+    *(double2*)&Bs[bb][TN64_AT(bc, bu)] = S##b0;          \
+    *(double2*)&Bs[bb][TN64_AT(bc, bu + 1)] = S##b1;      \
+    *(double2*)&Bs[bb][TN64_AT(bc, bu + 2)] = S##b2;      \
+    *(double2*)&Bs[bb][TN64_AT(bc, bu + 3)] = S##b3;      \
+  } while (0)
+#define TN64_STEP(bb)                                                                           \
+  do {                                                                                          \
+    const double* as = &As[bb][0];                                                              \
+    const double* bs = &Bs[bb][0];                                                              \
+    const double2 x00 = *(const double2*)&bs[TN64_AT(32 * wid + lj, 2 * lg)];                   \
+    const double2 x01 = *(const double2*)&bs[TN64_AT(32 * wid + lj, 2 * lg + 1)];               \
+    const double2 x10 = *(const double2*)&bs[TN64_AT(32 * wid + 16 + lj, 2 * lg)];              \
+    const double2 x11 = *(const double2*)&bs[TN64_AT(32 * wid + 16 + lj, 2 * lg + 1)];          \
+    _Pragma("unroll") for (int mb = 0; mb < 4; ++mb) {                                          \
+      const double2 a0 = *(const double2*)&as[TN64_AT(16 * mb + lj, 2 * lg)];                   \
+      const double2 a1 = *(const double2*)&as[TN64_AT(16 * mb + lj, 2 * lg + 1)];               \
+      acc[mb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x00.x, a0.x, acc[mb][0], 0, 0, 0);     \
+      acc[mb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x10.x, a0.x, acc[mb][1], 0, 0, 0);     \
+      acc[mb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x00.y, a0.y, acc[mb][0], 0, 0, 0);     \
+      acc[mb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x10.y, a0.y, acc[mb][1], 0, 0, 0);     \
+      acc[mb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x01.x, a1.x, acc[mb][0], 0, 0, 0);     \
+      acc[mb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x11.x, a1.x, acc[mb][1], 0, 0, 0);     \
+      acc[mb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x01.y, a1.y, acc[mb][0], 0, 0, 0);     \
+      acc[mb][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x11.y, a1.y, acc[mb][1], 0, 0, 0);     \
+    }                                                                                           \
+  } while (0)
+
+  // The loads are unconditional (a step beyond the chunk re-reads its last
+  // step), so that the vector-memory counter waits of the stores count only
+  // the loads issued before them.
+  const int64_t klast = kb1 - TN64_KS;
+  double2 Xa0, Xa1, Xb0, Xb1, Xb2, Xb3;
+  double2 Ya0, Ya1, Yb0, Yb1, Yb2, Yb3;
+  TN64_LOAD(X, kb0);
+  TN64_STORE(X, 0);
+  TN64_LOAD(X, min(kb0 + TN64_KS, klast));
+  __syncthreads();
+  // step k reads buffer k & 1; X holds step k+1 at even k, Y at odd k
+  for (int64_t kb = kb0; kb < kb1; kb += 2 * TN64_KS) {
+    {
+      const bool more1 = kb + TN64_KS < kb1;  // uniform
+      TN64_LOAD(Y, min(kb + 2 * TN64_KS, klast));
+      __builtin_amdgcn_sched_barrier(0);
+      TN64_STEP(0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more1) TN64_STORE(X, 1);
+      __syncthreads();
+      if (!more1) break;
+    }
+    {
+      const bool more1 = kb + 2 * TN64_KS < kb1;
+      TN64_LOAD(X, min(kb + 3 * TN64_KS, klast));
+      __builtin_amdgcn_sched_barrier(0);
+      TN64_STEP(1);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more1) TN64_STORE(Y, 0);
+      __syncthreads();
+    }
+  }
+#undef TN64_LOAD
+#undef TN64_STORE
+#undef TN64_STEP
+  double* Pz = P + (int64_t)blockIdx.z * pstride;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = m0 + 16 * mb + lj;
+        const int64_t c = n0 + 32 * wid + 16 * h + 4 * v + lg;
+        if (r < M && c < N) Pz[r + c * ldp] = acc[mb][h][v];
+      }
+}
+
+hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                                 int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
+                                 int64_t ldp, int64_t pstride) {
+  if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
+  if (kchunk % TN64_KS != 0 || K % TN64_KS != 0 || A.ld % 2 != 0 || B.ld % 2 != 0)
+    return hipErrorInvalidValue;
+  dim3 grid((unsigned)((Nmax + 127) / 128), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
+  hipLaunchKernelGGL(k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tall NN with a long reduction (the re-orthogonalisation V -= Q C):
+// C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for r < M, c < N, any K.
+//
+// Workgroup tile: 64 rows (blockIdx.y) x 64 columns (blockIdx.x); wave w owns
+// rows 16w .. 16w+15 against the four 16-column blocks.  Each 16-deep step of
+// both operands is staged through double-buffered LDS with two steps of
+// global loads in flight: A as [row][k] (16 k per row), B as [column][k], the
+// eight 16-byte units of a row / column XOR-swizzled with bits 1..3 of its
+// index.  Lane (j, g) reads k = 4g .. 4g+3 of its row / column as two
+// ds_read_b128 and MFMA q of the step consumes element q.  The MFMA computes
+// the transposed block (B's columns as its rows): lane l holds
+// C(r0 + 16w + l%16, c0 + 16cb + 4v + l/16), so every C load / store
+// instruction moves four 128-byte column runs.  Reduction indices >= K read
+// as zero (clamped loads, then a bit mask).  A is a plain or offset MatRef (no
+// column list).
+// ---------------------------------------------------------------------------
+#define NN64_KS 16
+#define NN64_AT(idx, u) ((idx) * NN64_KS + 2 * ((u) ^ (((idx) >> 1) & 7)))
+// x with its bits and-ed with the mask m (0 or all ones): a loaded value kept
+// or replaced by +0 without a branch, so that every load stays unconditional
+__device__ __forceinline__ double nn64_mask(double x, long long m) {
+  return __longlong_as_double(__double_as_longlong(x) & m);
+}
+
+__global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, int64_t K, MatRef A,
+                                                      MatRef B, MatRef C, int mode) {
+  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  if (r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
+
+  __shared__ double As[2][64 * NN64_KS];  // [buf][row][k], swizzled
+  __shared__ double Bs[2][64 * NN64_KS];  // [buf][col][k], swizzled
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lj = lane & 15, lg = lane >> 4;
+  // A stager: column k = ak of the step, rows ar .. ar+3 (coalesced along rows)
+  const int ak = tid >> 4, ar = (tid & 15) * 4;
+  const int64_t arow = min(r0 + ar, M - 4 > 0 ? M - 4 : 0);  // M >= 4 is not assumed
+  // B stager: column bc, k = bk .. bk+3 (contiguous in B's column)
+  const int bc = tid >> 2, bk = (tid & 3) * 4;
+  const double* bcol = mat_col(B, min(c0 + bc, N - 1));
+  const double* a0p = mat_col(A, 0) + arow;  // A has no column list: column k at a0p + k * ld
+
+// (the macro's locals carry an nl_ prefix: a local named like a variable of
+// the argument expression would be initialised from itself)
+#define NN64_LOAD(S, kk)                                                                  \
+  do {                                                                                    \
+    const int64_t nl_ka = (kk) + ak;                                                      \
+    const double* nl_ac = a0p + min(nl_ka, K - 1) * A.ld;                                 \
+    const double2 nl_t0 = *(const double2*)nl_ac;                                         \
+    const double2 nl_t1 = *(const double2*)(nl_ac + 2);                                   \
+    const long long nl_am = nl_ka < K ? -1ll : 0ll;                                       \
+    S##a0 = make_double2(nn64_mask(nl_t0.x, nl_am), nn64_mask(nl_t0.y, nl_am));           \
+    S##a1 = make_double2(nn64_mask(nl_t1.x, nl_am), nn64_mask(nl_t1.y, nl_am));           \
+    const int64_t nl_kb = (kk) + bk;                                                      \
+    const double nl_u0 = bcol[min(nl_kb, K - 1)], nl_u1 = bcol[min(nl_kb + 1, K - 1)];    \
+    const double nl_u2 = bcol[min(nl_kb + 2, K - 1)], nl_u3 = bcol[min(nl_kb + 3, K - 1)];\
+    S##b0 = make_double2(nn64_mask(nl_u0, nl_kb < K ? -1ll : 0ll),                        \
+                         nn64_mask(nl_u1, nl_kb + 1 < K ? -1ll : 0ll));                   \
+    S##b1 = make_double2(nn64_mask(nl_u2, nl_kb + 2 < K ? -1ll : 0ll),                    \
+                         nn64_mask(nl_u3, nl_kb + 3 < K ? -1ll : 0ll));                   \
+  } while (0)
+#define NN64_STORE(S, bb)                                                           \
+  do {                                                                              \
+    As[bb][NN64_AT(ar, ak >> 1) + (ak & 1)] = S##a0.x;                              \
+    As[bb][NN64_AT(ar + 1, ak >> 1) + (ak & 1)] = S##a0.y;                          \
+    As[bb][NN64_AT(ar + 2, ak >> 1) + (ak & 1)] = S##a1.x;                          \
+    As[bb][NN64_AT(ar + 3, ak >> 1) + (ak & 1)] = S##a1.y;                          \
+    *(double2*)&Bs[bb][NN64_AT(bc, bk >> 1)] = S##b0;                               \
+    *(double2*)&Bs[bb][NN64_AT(bc, (bk >> 1) + 1)] = S##b1;                         \
+  } while (0)
+#define NN64_STEP(bb)                                                                      \
+  do {                                                                                     \
+    const double* as = &As[bb][0];                                                         \
+    const double* bs = &Bs[bb][0];                                                         \
+    const double2 y0 = *(const double2*)&as[NN64_AT(16 * wid + lj, 2 * lg)];               \
+    const double2 y1 = *(const double2*)&as[NN64_AT(16 * wid + lj, 2 * lg + 1)];           \
+    _Pragma("unroll") for (int cb = 0; cb < 4; ++cb) {                                     \
+      const double2 x0 = *(const double2*)&bs[NN64_AT(16 * cb + lj, 2 * lg)];              \
+      const double2 x1 = *(const double2*)&bs[NN64_AT(16 * cb + lj, 2 * lg + 1)];          \
+      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.x, y0.x, acc[cb], 0, 0, 0);       \
+      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.y, y0.y, acc[cb], 0, 0, 0);       \
+      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, y1.x, acc[cb], 0, 0, 0);       \
+      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, y1.y, acc[cb], 0, 0, 0);       \
+    }                                                                                      \
+  } while (0)
+
+  v4d acc[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  // two steps of loads in flight; the loads are unconditional (beyond K they
+  // read clamped addresses and yield zeros), so the vector-memory waits of the
+  // stores count only the loads issued before them
+  double2 Xa0, Xa1, Xb0, Xb1;
+  double2 Ya0, Ya1, Yb0, Yb1;
+  NN64_LOAD(X, 0);
+  NN64_STORE(X, 0);
+  NN64_LOAD(X, NN64_KS);
+  __syncthreads();
+  for (int64_t kb = 0; kb < K; kb += 2 * NN64_KS) {
+    {
+      const bool more1 = kb + NN64_KS < K;  // uniform
+      NN64_LOAD(Y, kb + 2 * NN64_KS);
+      __builtin_amdgcn_sched_barrier(0);
+      NN64_STEP(0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more1) NN64_STORE(X, 1);
+      __syncthreads();
+      if (!more1) break;
+    }
+    {
+      const bool more1 = kb + 2 * NN64_KS < K;
+      NN64_LOAD(X, kb + 3 * NN64_KS);
+      __builtin_amdgcn_sched_barrier(0);
+      NN64_STEP(1);
+      __builtin_amdgcn_sched_barrier(0);
+      if (more1) NN64_STORE(Y, 0);
+      __syncthreads();
+    }
+  }
+#undef NN64_LOAD
+#undef NN64_STORE
+#undef NN64_STEP
+  const int64_t r = r0 + 16 * wid + lj;
+  if (r < M) {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t c = c0 + 16 * cb + 4 * v + lg;
+        if (c < N) {
+          double* d = mat_col(C, c) + r;
+          const double x = acc[cb][v];
+          *d = (mode == GEMM_SUB) ? *d - x : ((mode == GEMM_ADD) ? *d + x : x);
+        }
+      }
+  }
+}
+
+hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
+                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode) {
+  if (Mmax <= 0 || Nmax <= 0 || K <= 0) return hipSuccess;
+  if (A.cols != nullptr || A.ld % 2 != 0 || Mmax % 4 != 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mmax + 63) / 64));
+  hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Split reduction: dst(r, c) (=|+=|-=) sum_{z=0}^{nsplit-1} P[z](r, c).
 // One workgroup per (64-row slab, column): four waves each sum every fourth
 // split (two accumulators), then the four partials are added in a fixed
@@ -149,13 +455,15 @@ __global__ __launch_bounds__(256) void k_reduce_splits(DimSpec dM, DimSpec dN,
   __syncthreads();
   if (zg == 0 && r < M) {
     const double s = (part[0][lr] + part[1][lr]) + (part[2][lr] + part[3][lr]);
-    double* d = mat_col(dst, c) + r;
-    if (mode == GEMM_ADD)
-      *d += s;
-    else if (mode == GEMM_SUB)
-      *d -= s;
-    else
-      *d = s;
+    if (dst.p) {
+      double* d = mat_col(dst, c) + r;
+      if (mode == GEMM_ADD)
+        *d += s;
+      else if (mode == GEMM_SUB)
+        *d -= s;
+      else
+        *d = s;
+    }
     if (dst2) dst2[r * ld2 + c] = (T2)s;  // row-major copy
   }
 }

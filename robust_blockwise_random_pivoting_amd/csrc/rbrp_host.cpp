@@ -467,14 +467,17 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (i > 0) {
       int ns;
       int64_t kc;
-      choose_split(i, b, mp, i, ws.Pcap, &ns, &kc);
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, mp, ns, kc, Qall, Vblk,
-                                 ws.P, i, i * (int64_t)b));
-      HIPCHK(rbrp_launch_reduce_splits(st, dim_static(i), dim_dyn(0, d_s, 1), i, b, ws.P, i,
-                                       i * (int64_t)b, ns, mat(ws.Cbuf, ldr), GEMM_SET));
+      // C = Q^T V computed as its transpose V^T Q (the sampled block on the
+      // 64-row side of the tiles), reduced into C column-major
+      choose_split(b, i, mp, b, ws.Pcap, &ns, &kc, 16, 128);
+      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, mp, ns, kc, Vblk, Qall,
+                                   ws.P, b, i * (int64_t)b));
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, ws.P, b,
+                                       i * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
+                                       GEMM_SET, ws.Cbuf, ldr));
       HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i), Qall,
-                                 mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, nullptr, 0));
+      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, i, Qall,
+                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB));
       cgs_launches += 4;
       if (mixed) {
         // the corrected columns replace the stored residual (the filtered-out
@@ -540,16 +543,16 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
-      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, mixed ? 128 : 64);
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128);
       ph_proj = pt.begin(RBRP_PHASE_PROJECT);
       if (mixed)
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, ws.Aw32, i, nullptr, ws.Aw32, i, d_bt, mp, ws.P, b,
                                      (int64_t)b * nrem_max));
       else
-        HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max, mp,
-                                   ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt), ws.P,
-                                   b, (int64_t)b * nrem_max));
+        HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
+                                     mp, ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt),
+                                     ws.P, b, (int64_t)b * nrem_max));
       pt.end(ph_proj, 1);
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
       if (mixed)

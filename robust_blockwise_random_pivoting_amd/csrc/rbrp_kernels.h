@@ -59,8 +59,22 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
                                int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B,
                                double* P, int64_t ldp, int64_t pstride);
 
-// dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order; when dst2 != nullptr
-// the sum is also stored row-major at dst2[r * ld2 + c].
+// The same product on 64 x 128 output tiles with double-buffered LDS (the
+// projection and the re-orthogonalisation coefficients).  K and kchunk
+// multiples of 16, both leading dimensions even.
+hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
+                                 int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
+                                 int64_t ldp, int64_t pstride);
+
+// C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) on 64 x 64 output tiles with
+// double-buffered K steps (the re-orthogonalisation V -= Q C); any K, M a
+// multiple of 4 (static), A's leading dimension even.
+hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
+                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode);
+
+// dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order (skipped when dst.p is
+// null); when dst2 != nullptr the sum is also stored row-major at
+// dst2[r * ld2 + c] (the transpose, for a column-major dst2).
 hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
                                      MatRef dst, int mode, double* dst2 = nullptr, int64_t ld2 = 0);

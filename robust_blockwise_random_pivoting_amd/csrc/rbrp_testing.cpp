@@ -63,6 +63,45 @@ int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const doub
   return 0;
 }
 
+int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A,
+                        const double* B, int64_t ld, double* C, int64_t ldc, int nsplit) {
+  if (M < 0) return -2;
+  if (N < 0) return -3;
+  if (K < 0 || K % 16) return -4;
+  if (ld < std::max<int64_t>(K, 1) || ld % 2) return -7;
+  if (nsplit < 1) return -10;
+  if (M == 0 || N == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  int64_t kc = (K + nsplit - 1) / nsplit;
+  kc = std::max<int64_t>(16, (kc + 15) / 16 * 16);
+  const int ns = (int)std::max<int64_t>(1, (K + kc - 1) / kc);
+  double* P = nullptr;
+  HIPCHK(hipMalloc(&P, sizeof(double) * M * N * ns));
+  hipError_t e = rbrp_launch_gemm_tn64(st, dim_static(M), dim_static(N), M, N, K, ns, kc,
+                                       mat((double*)A, ld), mat((double*)B, ld), P, M, M * N);
+  if (e == hipSuccess)
+    e = rbrp_launch_reduce_splits(st, dim_static(M), dim_static(N), M, N, P, M, M * N, ns,
+                                  mat(C, ldc), GEMM_SET);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(P);
+  return (int)e;
+}
+
+int rbrp_test_gemm_nn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double* C, int64_t ldc, int mode) {
+  if (M < 0 || M % 4) return -2;
+  if (N < 0) return -3;
+  if (K < 0) return -4;
+  if (lda < std::max<int64_t>(M, 1) || lda % 2) return -6;
+  if (mode < GEMM_SET || mode > GEMM_ADD) return -11;
+  if (M == 0 || N == 0 || K == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(M), M, dim_static(N), N, K, mat((double*)A, lda),
+                               mat((double*)B, ldb), mat(C, ldc), mode));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
 int rbrp_test_gemm_tn32(void* stream, int64_t M, int64_t N, int64_t K, const float* A,
                         const float* B, int64_t ld, double* C, int64_t ldc, int nsplit) {
   if (M < 0) return -2;

@@ -59,6 +59,57 @@ def test_gemm_tn_matches_numpy(librbrp, M, N, K, nsplit):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("M,N,K,nsplit", [(1, 1, 16, 1), (17, 5, 64, 3), (64, 64, 256, 4),
+                                          (100, 130, 1024, 7), (29, 64, 4096, 16),
+                                          (64, 300, 50048, 40), (64, 129, 512, 1)])
+def test_gemm_tn64_matches_numpy(librbrp, M, N, K, nsplit):
+    """The 64 x 128-tile projection kernel against NumPy at FP64 rounding."""
+    import torch
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = rng.standard_normal((K, M))
+    B = rng.standard_normal((K, N))
+    dA, dB = _cm(A), _cm(B)
+    dC = _cm(np.zeros((M, N)))
+    code = librbrp.rbrp_test_gemm_tn64(_stream(), M, N, K, _ptr(dA), _ptr(dB), K, _ptr(dC), M, nsplit)
+    assert code == 0
+    ref = A.T @ B
+    assert np.abs(_host(dC) - ref).max() <= 1e-12 * np.abs(ref).max() * np.sqrt(K)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("M,N,K,mode", [(64, 64, 16, 0), (132, 70, 33, 1), (4, 3, 1, 2),
+                                        (50048, 64, 1000, 1), (256, 17, 2047, 2), (128, 64, 48, 1)])
+def test_gemm_nn64_matches_numpy(librbrp, M, N, K, mode):
+    """The long-reduction NN kernel against NumPy at FP64 rounding (set / subtract / add)."""
+    import torch
+    rng = np.random.default_rng(M + 7 * N + 11 * K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    dA, dB, dC = _cm(A), _cm(B), _cm(C0)
+    code = librbrp.rbrp_test_gemm_nn64(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode)
+    assert code == 0
+    prod = A @ B
+    ref = prod if mode == 0 else (C0 - prod if mode == 1 else C0 + prod)
+    scale = np.abs(A).max() * np.abs(B).max() * K + np.abs(C0).max()
+    assert np.abs(_host(dC) - ref).max() <= 1e-14 * scale
+    torch.cuda.synchronize()
+
+
+def test_gemm_nn64_rejects_bad_shapes(librbrp):
+    import torch
+    z = torch.zeros(64, dtype=torch.float64, device="cuda")
+    assert librbrp.rbrp_test_gemm_nn64(_stream(), 6, 1, 1, _ptr(z), 6, _ptr(z), 1, _ptr(z), 6, 0) == -2
+    assert librbrp.rbrp_test_gemm_nn64(_stream(), 4, 1, 1, _ptr(z), 5, _ptr(z), 1, _ptr(z), 4, 0) == -6
+
+
+def test_gemm_tn64_rejects_bad_shapes(librbrp):
+    import torch
+    z = torch.zeros(64, dtype=torch.float64, device="cuda")
+    assert librbrp.rbrp_test_gemm_tn64(_stream(), 1, 1, 8, _ptr(z), _ptr(z), 8, _ptr(z), 1, 1) == -4
+    assert librbrp.rbrp_test_gemm_tn64(_stream(), 1, 1, 16, _ptr(z), _ptr(z), 17, _ptr(z), 1, 1) == -7
+
+
 @pytest.mark.parametrize("M,N,K,mode", [(64, 64, 16, 0), (130, 70, 33, 1), (5, 3, 1, 2),
                                         (256, 200, 64, 1), (64, 1, 100, 0)])
 def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
