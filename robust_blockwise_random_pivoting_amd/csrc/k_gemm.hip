@@ -757,6 +757,168 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 trailing update for K <= 64 and M a multiple of 64 (the RBRP loop's
+// shape): C(r, c) -= sum_{k<K} A(r, k) Bt[k * ldbt + c], optional squared column
+// norms of the new C (one partial per workgroup row group).
+//
+// Workgroup = 4 waves side by side along columns (32 each, 128 per workgroup),
+// sweeping T row tiles of 64 rows.  The 64 x 64 block of A of a row tile is
+// staged once in LDS ([row][k], pitch UPD_LDA) and read by the four waves,
+// the next tile's block being loaded into registers meanwhile, so A is read
+// from L2 once per 128 columns of C; each wave keeps its 64 x 32 block of B
+// in registers for the whole sweep.  A wave takes its row tile as four 16-row
+// sub-blocks and prefetches the next sub-block's C while the current one is
+// multiplied.  MFMA q of a sub-block uses k = 16 (lane / 16) + q, so a lane's
+// A fragment is eight ds_read_b128 of one LDS row.  The MFMA computes the
+// transposed block: lane l holds C(r + l%16, c + 4v + l/16), i.e. each C
+// load / store instruction moves four 128-byte column runs.  Every global
+// load is unconditional (clamped rows and columns) so that the vector-memory
+// waits count only loads issued earlier.
+// ---------------------------------------------------------------------------
+#define UPD_LDA 66
+
+template <bool NORMS>
+__global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, DimSpec dK, int T,
+                                                       const double* __restrict__ A0, int64_t lda,
+                                                       int64_t acoff, const int* acoff_dyn,
+                                                       const double* __restrict__ Bt, int64_t ldbt,
+                                                       double* __restrict__ C0, int64_t ldc,
+                                                       int64_t ccoff, const int* ccoff_dyn,
+                                                       double* __restrict__ normpart, int64_t ldnp) {
+  __shared__ double As[2][64 * UPD_LDA];
+  const int64_t N = dim_eval(dN), K = dim_eval(dK);
+  const int64_t c0 = (int64_t)blockIdx.x * 128;
+  const int64_t rbase = (int64_t)blockIdx.y * T * 64;
+  if (c0 >= N || rbase >= M || K <= 0) return;  // uniform per workgroup
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* A = A0 + (acoff + (acoff_dyn ? *acoff_dyn : 0)) * lda;
+  // this wave's columns cw .. cw + ncw - 1; a wave past N keeps the workgroup's
+  // first column as a (never stored) load target
+  const int64_t cw = c0 + 32 * wid;
+  const int ncw = (int)max((int64_t)0, min((int64_t)32, N - cw));
+  double* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + (ncw > 0 ? cw : c0)) * ldc;
+  const int cmax = max(ncw, 1) - 1;
+  // B block of this wave: b0[q] = B(16 lk + q, cw + li), b1[q] = B(16 lk + q, cw + 16 + li)
+  double b0[16], b1[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int64_t k = 16 * lk + q;
+    b0[q] = (k < K && li < ncw) ? Bt[k * ldbt + cw + li] : 0.0;
+    b1[q] = (k < K && 16 + li < ncw) ? Bt[k * ldbt + cw + 16 + li] : 0.0;
+  }
+  // A stager: column ka (clamped to K - 1: B is zero there), rows ar .. ar + 15 of a tile
+  const int ka = tid >> 2, ar = 16 * (tid & 3);
+  const double* acol = A + min((int64_t)ka, K - 1) * lda + ar;
+  // the next tile's A goes through registers in two halves of eight rows
+  // (loaded during sub-blocks 0 and 2, written to LDS during 1 and 3)
+  double2 s0, s1, s2, s3;
+#define UPD_LOADA(row0, half)                            \
+  do {                                                   \
+    const double* ap_ = acol + (row0) + 8 * (half);      \
+    s0 = *(const double2*)(ap_);                         \
+    s1 = *(const double2*)(ap_ + 2);                     \
+    s2 = *(const double2*)(ap_ + 4);                     \
+    s3 = *(const double2*)(ap_ + 6);                     \
+  } while (0)
+#define UPD_PUT(dst, j, v)                               \
+  do {                                                   \
+    (dst)[(ar + 2 * (j)) * UPD_LDA + ka] = (v).x;        \
+    (dst)[(ar + 2 * (j) + 1) * UPD_LDA + ka] = (v).y;    \
+  } while (0)
+#define UPD_STOREA(bb, half)                             \
+  do {                                                   \
+    double* d_ = &As[bb][0] + 8 * (half) * UPD_LDA;      \
+    UPD_PUT(d_, 0, s0);                                  \
+    UPD_PUT(d_, 1, s1);                                  \
+    UPD_PUT(d_, 2, s2);                                  \
+    UPD_PUT(d_, 3, s3);                                  \
+  } while (0)
+  // C of one 16-row sub-block: cv[h][v] = C(r + li, 16 h + 4 v + lk)
+#define UPD_LOADC(cv, r)                                                        \
+  do {                                                                          \
+    _Pragma("unroll") for (int h = 0; h < 2; ++h)                               \
+      _Pragma("unroll") for (int v = 0; v < 4; ++v)                             \
+        cv[h][v] = C[(r) + li + (int64_t)min(16 * h + 4 * v + lk, cmax) * ldc]; \
+  } while (0)
+
+  double nsq[2][4];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) nsq[h][v] = 0.0;
+  UPD_LOADA(rbase, 0);
+  UPD_STOREA(0, 0);
+  UPD_LOADA(rbase, 1);
+  UPD_STOREA(0, 1);
+  double cc[2][4], cn[2][4];
+  UPD_LOADC(cc, rbase);
+  __syncthreads();
+  int buf = 0;
+  for (int t = 0; t < T; ++t) {
+    const int64_t row0 = rbase + (int64_t)t * 64;
+    if (row0 >= M) break;  // uniform
+    const bool more = t + 1 < T && row0 + 64 < M;
+    const int64_t rnext = min(row0 + 64, M - 64);
+    const double* as = &As[buf][0];
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+      const int64_t r = row0 + 16 * sb;
+      if (sb == 0 || sb == 2) UPD_LOADA(rnext, sb >> 1);
+      if ((sb == 1 || sb == 3) && more) UPD_STOREA(buf ^ 1, sb >> 1);
+      UPD_LOADC(cn, sb < 3 ? r + 16 : rnext);
+      const double* arow = as + (16 * sb + li) * UPD_LDA + 16 * lk;
+      v4d acc0 = (v4d){0.0, 0.0, 0.0, 0.0}, acc1 = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 16; q += 2) {
+        const double2 a = *(const double2*)(arow + q);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0[q], a.x, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1[q], a.x, acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(b0[q + 1], a.y, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b1[q + 1], a.y, acc1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ca = 4 * v + lk, cb = 16 + 4 * v + lk;
+        const double na = cc[0][v] - acc0[v], nb = cc[1][v] - acc1[v];
+        if (ca < ncw) {
+          C[r + li + (int64_t)ca * ldc] = na;
+          if (NORMS) nsq[0][v] = fma(na, na, nsq[0][v]);
+        }
+        if (cb < ncw) {
+          C[r + li + (int64_t)cb * ldc] = nb;
+          if (NORMS) nsq[1][v] = fma(nb, nb, nsq[1][v]);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cc[h][v] = cn[h][v];
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+#undef UPD_LOADA
+#undef UPD_PUT
+#undef UPD_STOREA
+#undef UPD_LOADC
+  if (NORMS) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        double x = nsq[h][v];
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        x += __shfl_xor(x, 8, 64);
+        const int cl = 16 * h + 4 * v + lk;
+        if (li == 0 && cl < ncw) normpart[(int64_t)blockIdx.y * ldnp + cw + cl] = x;
+      }
+  }
+}
+
 hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
@@ -765,6 +927,21 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128) return hipErrorInvalidValue;
+  if (Kmax <= 64 && M % 64 == 0 && lda % 2 == 0) {
+    // 128-column workgroups with the A block shared through LDS
+    const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;
+    const int64_t T = std::max<int64_t>(
+        1, std::min<int64_t>(RBRP_TRAIL_TMAX, (ncol * nrow) / (RBRP_TRAIL_WGS / 4)));
+    dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
+    if (groups) *groups = (int64_t)grid.y;
+    if (normpart)
+      hipLaunchKernelGGL(k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
+                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+    else
+      hipLaunchKernelGGL(k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
+                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+    return hipGetLastError();
+  }
   const bool small = Kmax <= 64;
   const int64_t jb = small ? RBRP_TRAIL_JB : 1;  // rows per wave / 16 of the variant launched
   const int64_t ncol = (Nmax + 31) / 32, nrow = (M + 64 * jb - 1) / (64 * jb);

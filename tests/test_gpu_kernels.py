@@ -134,8 +134,12 @@ def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
 
 @pytest.mark.parametrize("M,N,K", [(128, 32, 16), (200, 70, 33), (5, 3, 1), (1000, 130, 64),
                                    (300, 17, 128), (64, 1, 100), (5000, 300, 64), (129, 33, 97),
-                                   (70000, 40, 60)])
+                                   (70000, 40, 60), (50048, 300, 64), (640, 257, 33), (64, 1, 7),
+                                   (1024, 129, 64), (4096, 96, 1)])
 def test_gemm_trail_update_matches_numpy(librbrp, M, N, K):
+    """The trailing-update launcher against NumPy: shapes with M a multiple of 64
+    and K <= 64 take the 128-column kernel with the A block shared through LDS,
+    the others the 32-column kernel."""
     import torch
     rng = np.random.default_rng(M * 3 + N + K)
     A = rng.standard_normal((M, K))
