@@ -57,7 +57,7 @@ int rbrp_test_gemm_tn32(void* stream, int64_t M, int64_t N, int64_t K, const flo
                         const float* B, int64_t ld, double* C, int64_t ldc, int nsplit);
 
 /* FP32 trailing-update kernel (K <= 128, M, lda, ldc even): C(r, c) -= sum_k
- * A(r, k) Bt[k * ldbt + c], FP32 data; normpart (optional, FP64, ceil(M/128)
+ * A(r, k) Bt[k * ldbt + c], FP32 data; normpart (optional, FP64, ceil(M/64)
  * x N, leading dimension N, zero-initialised by the caller) receives partial
  * squared column norms over row groups: its rows sum to the column norms. */
 int rbrp_test_gemm_upd32(void* stream, int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,

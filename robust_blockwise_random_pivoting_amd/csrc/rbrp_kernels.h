@@ -184,7 +184,7 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  const int* b_dyn, int64_t ld, double* P, int64_t ldp,
                                  int64_t pstride);
 // FP32 trailing update with FP64 norm partials over row groups (at most
-// ceil(M / 128)); see rbrp_launch_gemm_trail.  M, lda and ldc must be even.
+// ceil(M / 64)); see rbrp_launch_gemm_trail.  M, lda and ldc must be even.
 hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                     int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
                                     const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,

@@ -66,8 +66,12 @@ def test_gemm_tn32_within_fp32_bound(librbrp, M, N, K, nsplit):
 
 @pytest.mark.parametrize("M,N,K", [(128, 32, 16), (200, 70, 33), (6, 3, 1), (1000, 130, 64),
                                    (300, 17, 128), (64, 1, 100), (5000, 300, 64), (130, 33, 97),
-                                   (70000, 40, 60)])
+                                   (70000, 40, 60), (50048, 300, 64), (640, 257, 33), (64, 1, 7),
+                                   (1024, 129, 64), (4096, 96, 1)])
 def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
+    """The FP32 update launcher against an FP64 product: shapes with M a multiple
+    of 64 and K <= 64 take the 128-column kernel with A shared through LDS, the
+    others the 32-column kernel."""
     import torch
     rng = np.random.default_rng(M * 3 + N + K)
     A = rng.standard_normal((M, K)).astype(np.float32)
@@ -75,7 +79,7 @@ def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
     C0 = rng.standard_normal((M, N)).astype(np.float32)
     dA, dC = _cm32(A), _cm32(C0)
     dBt = torch.as_tensor(np.ascontiguousarray(B)).cuda()   # row-major (k, c) at k*N + c
-    ntile = (M + 127) // 128
+    ntile = (M + 63) // 64      # at most one partial row per 64 rows
     dP = torch.zeros(ntile * N, dtype=torch.float64, device="cuda")
     code = librbrp.rbrp_test_gemm_upd32(_stream(), M, N, K, _ptr(dA), M, _ptr(dBt), N, _ptr(dC), M,
                                         _ptr(dP))
