@@ -491,7 +491,55 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   pw_store_rows(Gs, bt, true, R2inv, b);
 }
 
+// Inverses of the diagonal blocks of a k x k upper-triangular R, in the
+// partition of the blocked back substitution: super-blocks of 256 rows from the
+// bottom (s1 = k - 256 j, s0 = max(0, s1 - 256)), each cut into 64-row blocks
+// from its bottom (r1 = s1 - 64 i, r0 = max(s0, r1 - 64)).  Workgroup 4 j + i
+// writes the inverse of R(r0:r1, r0:r1) to out + (4 j + i) * 4096 (leading
+// dimension 64, zeros below the diagonal); blocks that do not exist are
+// skipped.  One wave per block, as in the panel kernels.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_upper_inv_blocks(
+    int64_t k, const double* __restrict__ R, int64_t ldr, double* __restrict__ out) {
+  extern __shared__ __align__(16) double lds[];
+  double* Gs = lds;                // inverse rows (store staging, odd pitch)
+  double* RT = Gs + PW_TILE_O;     // inverse, transposed (zero on entry)
+  double* Ws = RT + PW_TILE_E;     // the block, row-major
+  double* dinv = Ws + 2 * PW_TILE_E;
+  const int64_t j = blockIdx.x >> 2, ib = blockIdx.x & 3;
+  const int64_t s1 = k - 256 * j, s0 = max((int64_t)0, s1 - 256);
+  const int64_t r1 = s1 - 64 * ib;
+  if (r1 <= s0) return;  // uniform
+  const int64_t r0 = max(s0, r1 - 64);
+  const int nb = (int)(r1 - r0);
+  const int c = threadIdx.x;
+  const double* rc = R + r0 + (r0 + min(c, nb - 1)) * ldr;  // column c of the block
+  for (int t0 = 0; t0 < PW_N; t0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = rc[min(t0 + u, nb - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      Ws[(t0 + u) * PW_LDE + c] = (c < nb && t0 + u <= c) ? v[u] : 0.0;
+  }
+  if (c < nb) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
+  pw_zero_row(RT);
+  pw_upper_inverse(Ws, dinv, RT, Gs, nb);
+  pw_store_rows(Gs, nb, true, out + (int64_t)blockIdx.x * 4096, 64);
+}
+
 static size_t pw_lds_bytes() { return (size_t)PW_LDS_DOUBLES * sizeof(double); }
+
+hipError_t rbrp_launch_upper_inv_blocks(hipStream_t st, int64_t k, const double* R, int64_t ldr,
+                                        double* out) {
+  if (k <= 0) return hipSuccess;
+  const size_t lds = pw_lds_bytes();
+  hipError_t e = hipFuncSetAttribute((const void*)k_upper_inv_blocks,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  const unsigned nblk = (unsigned)(4 * ((k + 255) / 256));
+  hipLaunchKernelGGL(k_upper_inv_blocks, dim3(nblk), dim3(64), lds, st, k, R, ldr, out);
+  return hipGetLastError();
+}
 
 hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg, int b,
                                     int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,

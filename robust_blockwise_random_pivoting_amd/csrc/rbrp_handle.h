@@ -14,6 +14,7 @@ struct RbrpWorkspace {
   // carved pointers of the RBRP loop
   double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
       *tmpk, *tmpo, *Wt;
+  double* Dinv;  // diagonal-block inverses of R for the X solve
   // mixed precision: FP32 working matrix and row-major W, FP64 skeleton basis
   // and FP64 copy of the sampled block (null in the FP64 loop, where Aw, Wt
   // are used instead)
@@ -56,5 +57,7 @@ void rbrp_batched_release(rbrp_handle_t h);
 hipError_t rbrp_sketch_arena(rbrp_handle_t h, size_t bytes, void** out);
 
 // X (k x nc, ldx) <- R^{-1} X, R upper triangular (k x k, ldr); blocked, on st.
+// dinv (optional): scratch of 4 * ceil(k / 256) * 4096 doubles for the
+// diagonal-block inverses.
 hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R, int64_t ldr,
-                                double* X, int64_t ldx, int* launches);
+                                double* X, int64_t ldx, int* launches, double* dinv = nullptr);

@@ -77,6 +77,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   size_t oJin = c.take<int>(b);
   size_t oJout = c.take<int>(b);
   size_t oJkpos = c.take<int>(b);
+  size_t oDinv = c.take<double>(4 * ((kmax + 255) / 256) * 4096);  // diagonal-block inverses
   size_t ost = c.take<RbrpBlockState>(1);
   const size_t need = c.off;
   RbrpWorkspace& ws = h->ws;
@@ -121,6 +122,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   ws.Jin = (int*)(base + oJin);
   ws.Jout = (int*)(base + oJout);
   ws.Jkpos = (int*)(base + oJkpos);
+  ws.Dinv = (double*)(base + oDinv);
   ws.st = (RbrpBlockState*)(base + ost);
   ws.Pcap = Pcap;
   return 0;
@@ -208,14 +210,25 @@ double now_s() {
 
 // X (k x nc, ldx) <- R^{-1} X for the k x k upper-triangular R (ldr): backward
 // blocked solve with 256-row super-blocks, each solved with 64-row diagonal
-// blocks (k_trsm_diag) and small updates, followed by one large update of
-// every row above the super-block (k_gemm_nn).
+// blocks and small updates, followed by one large update of every row above
+// the super-block (k_gemm_nn).  With a scratch `dinv` (4 * ceil(k / 256) x
+// 4096 doubles) the diagonal blocks are inverted in one launch up front and
+// applied as in-place products (a workgroup reads all rows of its columns
+// before writing them); without it each block is solved by substitution
+// (k_trsm_diag, one thread per column).
 hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R, int64_t ldr,
-                                double* X, int64_t ldx, int* launches) {
+                                double* X, int64_t ldx, int* launches, double* dinv) {
   hipError_t e;
-  for (int64_t s1 = k; s1 > 0; s1 -= 256) {
+  if (dinv && k > 0 && nc > 0) {
+    e = rbrp_launch_upper_inv_blocks(st, k, R, ldr, dinv);
+    if (e != hipSuccess) return e;
+    if (launches) ++*launches;
+  }
+  int64_t jsup = 0;
+  for (int64_t s1 = k; s1 > 0; s1 -= 256, ++jsup) {
     const int64_t s0 = std::max<int64_t>(0, s1 - 256);
-    for (int64_t r1 = s1; r1 > s0; r1 -= 64) {
+    int64_t iblk = 0;
+    for (int64_t r1 = s1; r1 > s0; r1 -= 64, ++iblk) {
       const int64_t r0 = std::max<int64_t>(s0, r1 - 64);
       if (r1 < s1) {
         e = rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
@@ -224,7 +237,12 @@ hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R
         if (e != hipSuccess) return e;
         if (launches) ++*launches;
       }
-      e = rbrp_launch_trsm_diag(st, r1 - r0, nc, R + r0 + r0 * ldr, ldr, X + r0, ldx);
+      if (dinv)
+        e = rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
+                                dim_static(r1 - r0), mat(dinv + (4 * jsup + iblk) * 4096, 64),
+                                mat(X + r0, ldx), mat(X + r0, ldx), GEMM_SET, nullptr, 0);
+      else
+        e = rbrp_launch_trsm_diag(st, r1 - r0, nc, R + r0 + r0 * ldr, ldr, X + r0, ldx);
       if (e != hipSuccess) return e;
       if (launches) ++*launches;
     }
@@ -627,7 +645,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     const int64_t nc = n - k;
     HIPCHK(rbrp_launch_copy_block(st, k, nc, ws.R + k * ldr, ldr, res->X, res->ldx));
     int launches = 1;
-    HIPCHK(rbrp_trsm_upper_left(st, k, nc, ws.R, ldr, res->X, res->ldx, &launches));
+    HIPCHK(rbrp_trsm_upper_left(st, k, nc, ws.R, ldr, res->X, res->ldx, &launches, ws.Dinv));
     pt.end(ph, launches, (double)k * k * nc, 8.0 * (0.5 * k * k + 2.0 * k * nc));
   }
   HIPCHK(hipStreamSynchronize(st));

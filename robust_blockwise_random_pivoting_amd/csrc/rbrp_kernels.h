@@ -134,6 +134,12 @@ hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg
 hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg, int b,
                                   RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
                                   double* Rout, int64_t ldr);
+// Inverses of the diagonal blocks of the k x k upper-triangular R in the
+// partition of the blocked back substitution (256-row super-blocks from the
+// bottom, 64-row blocks within): 4 * ceil(k / 256) blocks of 64 x 64 doubles
+// (ld 64) at out.
+hipError_t rbrp_launch_upper_inv_blocks(hipStream_t st, int64_t k, const double* R, int64_t ldr,
+                                        double* out);
 hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
                                   const int* Jin, const int* Jout, int* perm, double* diags);
 hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
