@@ -443,21 +443,19 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
 }
 
 // ---------------------------------------------------------------------------
-// FP32 trailing update for K <= 64 and M a multiple of 64 (the mixed loop's
-// shape), the FP32 form of k_gemm_upd64: 128 columns per workgroup (4 waves
-// side by side, 32 columns each), row tiles of 64 rows whose 64 x 64 block of
-// A is staged once in double-buffered LDS and shared by the four waves, each
-// wave's 64 x 32 block of B in registers for the whole sweep.  A wave takes a
-// row tile as two 32-row sub-blocks (lane l owns rows 2(l%16) and 2(l%16)+1 of
-// a sub-block, as float2) and prefetches the next sub-block's C.  LDS holds A
-// as row pairs ([pair][k][2], pitch UPD32_LDP floats) so that a lane reads its
-// 16 k values of both rows as eight ds_read_b128; MFMA q uses
-// k = 16 (lane / 16) + q.  Norm partials: FP32 squares per lane over the sweep,
-// summed in FP64 across the 16 lanes of a column.
+// FP32 trailing update for K <= KB (64 or 128) and M a multiple of 64 (the
+// mixed loop's shape), the FP32 form of k_gemm_upd64: 128 columns per
+// workgroup (4 waves side by side, 32 columns each), row tiles of 64 rows
+// whose 64 x KB block of A is staged once in double-buffered LDS and shared by
+// the four waves, each wave's KB x 32 block of B in registers for the whole
+// sweep.  A wave takes a row tile as two 32-row sub-blocks (lane l owns rows
+// 2(l%16) and 2(l%16)+1 of a sub-block, as float2) and prefetches the next
+// sub-block's C.  LDS holds A as row pairs ([pair][k][2], 2 KB + 4 floats per
+// pair) so that a lane reads its KB / 4 k values of both rows with 16-byte
+// loads; MFMA q uses k = (KB / 4) (lane / 16) + q.  Norm partials: FP32 squares
+// per lane over the sweep, summed in FP64 across the 16 lanes of a column.
 // ---------------------------------------------------------------------------
-#define UPD32_LDP 132  // floats per row pair: 64 k x 2 rows + 4 (16 lanes on distinct banks)
-
-template <bool NORMS>
+template <int KB, bool NORMS>
 __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, DimSpec dK, int T,
                                                        const float* __restrict__ A0, int64_t lda,
                                                        int64_t acoff, const int* acoff_dyn,
@@ -465,7 +463,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
                                                        float* __restrict__ C0, int64_t ldc,
                                                        int64_t ccoff, const int* ccoff_dyn,
                                                        double* __restrict__ normpart, int64_t ldnp) {
-  __shared__ float As[2][32 * UPD32_LDP];
+  constexpr int KQ = KB / 4;          // k values per lane (and MFMAs per column block)
+  constexpr int LDP = 2 * KB + 4;     // floats per row pair: 16 lanes on distinct banks
+  constexpr int TPC = 256 / KB;       // stager threads per column of A
+  constexpr int RPT = 64 / TPC;       // rows per stager thread
+  __shared__ float As[2][32 * LDP];
   const int64_t N = dim_eval(dN), K = dim_eval(dK);
   const int64_t c0 = (int64_t)blockIdx.x * 128;
   const int64_t rbase = (int64_t)blockIdx.y * T * 64;
@@ -477,38 +479,29 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
   const int ncw = (int)max((int64_t)0, min((int64_t)32, N - cw));
   float* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + (ncw > 0 ? cw : c0)) * ldc;
   const int cmax = max(ncw, 1) - 1;
-  float b0[16], b1[16];  // B(16 lk + q, cw + li) and B(16 lk + q, cw + 16 + li)
+  float b0[KQ], b1[KQ];  // B(KQ lk + q, cw + li) and B(KQ lk + q, cw + 16 + li)
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int64_t k = 16 * lk + q;
+  for (int q = 0; q < KQ; ++q) {
+    const int64_t k = KQ * lk + q;
     b0[q] = (k < K && li < ncw) ? Bt[k * ldbt + cw + li] : 0.0f;
     b1[q] = (k < K && 16 + li < ncw) ? Bt[k * ldbt + cw + 16 + li] : 0.0f;
   }
-  // A stager: column ka (clamped to K - 1: B is zero there), rows ar .. ar + 15 of a tile
-  const int ka = tid >> 2, ar = 16 * (tid & 3);
+  // A stager: column ka (clamped to K - 1: B is zero there), rows ar .. ar + RPT - 1 of a tile
+  const int ka = tid / TPC, ar = RPT * (tid % TPC);
   const float* acol = A + min((int64_t)ka, K - 1) * lda + ar;
-  float4 s0, s1, s2, s3;
-#define UPD32_LOADA(row0)                                                     \
-  do {                                                                        \
-    const float* ap_ = acol + (row0);                                         \
-    s0 = *(const float4*)(ap_);                                               \
-    s1 = *(const float4*)(ap_ + 4);                                           \
-    s2 = *(const float4*)(ap_ + 8);                                           \
-    s3 = *(const float4*)(ap_ + 12);                                          \
+  float4 sv[RPT / 4];
+#define UPD32_LOADA(row0)                                                             \
+  do {                                                                                \
+    const float* ap_ = acol + (row0);                                                 \
+    _Pragma("unroll") for (int j = 0; j < RPT / 4; ++j) sv[j] = *(const float4*)(ap_ + 4 * j); \
   } while (0)
-#define UPD32_PUT(dst, j, x, y)                                               \
-  *(float2*)&(dst)[(ar / 2 + (j)) * UPD32_LDP + 2 * ka] = make_float2((x), (y))
-#define UPD32_STOREA(bb)                                                      \
-  do {                                                                        \
-    float* d_ = &As[bb][0];                                                   \
-    UPD32_PUT(d_, 0, s0.x, s0.y);                                             \
-    UPD32_PUT(d_, 1, s0.z, s0.w);                                             \
-    UPD32_PUT(d_, 2, s1.x, s1.y);                                             \
-    UPD32_PUT(d_, 3, s1.z, s1.w);                                             \
-    UPD32_PUT(d_, 4, s2.x, s2.y);                                             \
-    UPD32_PUT(d_, 5, s2.z, s2.w);                                             \
-    UPD32_PUT(d_, 6, s3.x, s3.y);                                             \
-    UPD32_PUT(d_, 7, s3.z, s3.w);                                             \
+#define UPD32_STOREA(bb)                                                              \
+  do {                                                                                \
+    float* d_ = &As[bb][0] + (ar / 2) * LDP + 2 * ka;                                 \
+    _Pragma("unroll") for (int j = 0; j < RPT / 4; ++j) {                             \
+      *(float2*)(d_ + (2 * j) * LDP) = make_float2(sv[j].x, sv[j].y);                 \
+      *(float2*)(d_ + (2 * j + 1) * LDP) = make_float2(sv[j].z, sv[j].w);             \
+    }                                                                                 \
   } while (0)
   // C of one 32-row sub-block: cv[h][v] = C(r + 2 li .. +1, 16 h + 4 lk + v)
 #define UPD32_LOADC(cv, r)                                                            \
@@ -541,10 +534,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
     for (int sb = 0; sb < 2; ++sb) {
       const int64_t r = row0 + 32 * sb;
       UPD32_LOADC(cn, sb == 0 ? r + 32 : rnext);
-      const float* apair = as + (16 * sb + li) * UPD32_LDP + 32 * lk;  // k = 16 lk .. , two rows
+      const float* apair = as + (16 * sb + li) * LDP + 2 * KQ * lk;  // k = KQ lk .., two rows
       v4f a00 = (v4f){0.f, 0.f, 0.f, 0.f}, a01 = a00, a10 = a00, a11 = a00;  // [row of pair][h]
 #pragma unroll
-      for (int q = 0; q < 16; q += 2) {
+      for (int q = 0; q < KQ; q += 2) {
         const float4 x = *(const float4*)(apair + 2 * q);  // (k, row 0), (k, row 1), (k+1, row 0), (k+1, row 1)
         a00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[q], x.x, a00, 0, 0, 0);
         a01 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[q], x.x, a01, 0, 0, 0);
@@ -580,7 +573,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
     buf ^= 1;
   }
 #undef UPD32_LOADA
-#undef UPD32_PUT
 #undef UPD32_STOREA
 #undef UPD32_LOADC
   if (NORMS) {
@@ -607,18 +599,27 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128 || (M & 1) || (lda & 1) || (ldc & 1)) return hipErrorInvalidValue;
-  if (Kmax <= 64 && M % 64 == 0 && lda % 4 == 0) {
+  if (M % 64 == 0 && lda % 4 == 0) {
     // 128-column workgroups with the A block shared through LDS
     const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;
     const int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 512));
     dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
     if (groups) *groups = (int64_t)grid.y;
-    if (normpart)
-      hipLaunchKernelGGL(k_gemm_upd32<true>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
-                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
-    else
-      hipLaunchKernelGGL(k_gemm_upd32<false>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
-                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+#define UPD32_LAUNCH(KB, NM)                                                                    \
+  hipLaunchKernelGGL((k_gemm_upd32<KB, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,   \
+                     acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp)
+    if (Kmax <= 64) {
+      if (normpart)
+        UPD32_LAUNCH(64, true);
+      else
+        UPD32_LAUNCH(64, false);
+    } else {
+      if (normpart)
+        UPD32_LAUNCH(128, true);
+      else
+        UPD32_LAUNCH(128, false);
+    }
+#undef UPD32_LAUNCH
     return hipGetLastError();
   }
   const bool small = Kmax <= 64;

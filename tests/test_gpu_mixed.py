@@ -67,11 +67,12 @@ def test_gemm_tn32_within_fp32_bound(librbrp, M, N, K, nsplit):
 @pytest.mark.parametrize("M,N,K", [(128, 32, 16), (200, 70, 33), (6, 3, 1), (1000, 130, 64),
                                    (300, 17, 128), (64, 1, 100), (5000, 300, 64), (130, 33, 97),
                                    (70000, 40, 60), (50048, 300, 64), (640, 257, 33), (64, 1, 7),
-                                   (1024, 129, 64), (4096, 96, 1)])
+                                   (1024, 129, 64), (4096, 96, 1), (640, 200, 128), (1024, 129, 97),
+                                   (50048, 64, 128), (128, 33, 65)])
 def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
     """The FP32 update launcher against an FP64 product: shapes with M a multiple
-    of 64 and K <= 64 take the 128-column kernel with A shared through LDS, the
-    others the 32-column kernel."""
+    of 64 take the 128-column kernel with A shared through LDS (K <= 64 and
+    K <= 128 forms), the others the 32-column kernel."""
     import torch
     rng = np.random.default_rng(M * 3 + N + K)
     A = rng.standard_normal((M, K)).astype(np.float32)
