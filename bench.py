@@ -85,7 +85,7 @@ def parse():
     return args
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_gemm_trail_fp64_50000x4000.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_gemm_upd64_fp64_50000x4000.json")
 
 
 def measured_traffic(args):
@@ -390,7 +390,7 @@ def main():
     roof = None
     if dom:
         d = kern[dom]
-        roof = {"kernel": {"project": "k_gemm_tn", "update": "k_gemm_trail"}[dom], "bound": d["bound"],
+        roof = {"kernel": {"project": "k_gemm_tn64", "update": "k_gemm_upd64"}[dom], "bound": d["bound"],
                 "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
                 "traffic": measured_traffic(args) if dom == "update" else None,
                 "algorithmic_bytes_per_launch": d.get("bytes_per_launch"),

@@ -122,7 +122,7 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // coefficients: P[z](r, c) = sum_{kk in chunk z} A(kk, r) B(kk, c) for
 // column-major operands (MatRef: plain, offset or index-listed columns).
 //
-// Workgroup tile: 64 rows (blockIdx.y) x 128 columns; wave w owns columns
+// Workgroup tile: 64 rows x 128 columns (an XCD-aware task); wave w owns columns
 // 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row blocks of
 // A.  Each 16-row step of both operands is staged through double-buffered
 // LDS ([column][row], one 128-byte run per column and step; the eight 16-byte
@@ -144,11 +144,16 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, int64_t K,
                                                       int64_t kchunk, MatRef A, MatRef B,
                                                       double* __restrict__ P, int64_t ldp,
-                                                      int64_t pstride) {
+                                                      int64_t pstride, int gx, int gy, int gz) {
+  // task = N tile + gx * (M tile + gy * split), XCD-aware (the tiles of one
+  // split share its rows of A through one L2)
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy * gz);
+  if (task >= (int64_t)gx * gy * gz) return;
+  const int64_t bx = task % gx, by = (task / gx) % gy, bz = task / ((int64_t)gx * gy);
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t n0 = (int64_t)blockIdx.x * 128, m0 = (int64_t)blockIdx.y * 64;
+  const int64_t n0 = bx * 128, m0 = by * 64;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup
-  const int64_t kb0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t kb0 = bz * kchunk;
   const int64_t kb1 = min(K, kb0 + kchunk);
   if (kb0 >= kb1) return;  // empty chunk (uniform); its partials are never read
 
@@ -249,7 +254,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
 #undef TN64_LOAD
 #undef TN64_STORE
 #undef TN64_STEP
-  double* Pz = P + (int64_t)blockIdx.z * pstride;
+  double* Pz = P + bz * pstride;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -268,8 +273,10 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
   if (kchunk % TN64_KS != 0 || K % TN64_KS != 0 || A.ld % 2 != 0 || B.ld % 2 != 0)
     return hipErrorInvalidValue;
-  dim3 grid((unsigned)((Nmax + 127) / 128), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
-  hipLaunchKernelGGL(k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride);
+  const int64_t gx = (Nmax + 127) / 128, gy = (Mmax + 63) / 64;
+  dim3 grid((unsigned)rbrp_xcd_grid(gx * gy * nsplit));
+  hipLaunchKernelGGL(k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride,
+                     (int)gx, (int)gy, nsplit);
   return hipGetLastError();
 }
 
@@ -780,6 +787,7 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
 
 template <bool NORMS>
 __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, DimSpec dK, int T,
+                                                       int ncol, int nrg,
                                                        const double* __restrict__ A0, int64_t lda,
                                                        int64_t acoff, const int* acoff_dyn,
                                                        const double* __restrict__ Bt, int64_t ldbt,
@@ -788,8 +796,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
                                                        double* __restrict__ normpart, int64_t ldnp) {
   __shared__ double As[2][64 * UPD_LDA];
   const int64_t N = dim_eval(dN), K = dim_eval(dK);
-  const int64_t c0 = (int64_t)blockIdx.x * 128;
-  const int64_t rbase = (int64_t)blockIdx.y * T * 64;
+  // task = column tile + ncol * row group, XCD-aware (a row group's column
+  // tiles share its A rows through one L2)
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)ncol * nrg);
+  if (task >= (int64_t)ncol * nrg) return;
+  const int64_t bx = task % ncol, by = task / ncol;
+  const int64_t c0 = bx * 128;
+  const int64_t rbase = by * T * 64;
   if (c0 >= N || rbase >= M || K <= 0) return;  // uniform per workgroup
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -914,7 +927,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
         x += __shfl_xor(x, 4, 64);
         x += __shfl_xor(x, 8, 64);
         const int cl = 16 * h + 4 * v + lk;
-        if (li == 0 && cl < ncw) normpart[(int64_t)blockIdx.y * ldnp + cw + cl] = x;
+        if (li == 0 && cl < ncw) normpart[by * ldnp + cw + cl] = x;
       }
   }
 }
@@ -932,14 +945,17 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
     const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;
     const int64_t T = std::max<int64_t>(
         1, std::min<int64_t>(RBRP_TRAIL_TMAX, (ncol * nrow) / (RBRP_TRAIL_WGS / 4)));
-    dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
-    if (groups) *groups = (int64_t)grid.y;
+    const int64_t nrg = (nrow + T - 1) / T;
+    dim3 grid((unsigned)rbrp_xcd_grid(ncol * nrg));
+    if (groups) *groups = nrg;
     if (normpart)
-      hipLaunchKernelGGL(k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
-                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+      hipLaunchKernelGGL(k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
+                         (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
+                         normpart, ldnp);
     else
-      hipLaunchKernelGGL(k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, A, lda, acoff,
-                         acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
+      hipLaunchKernelGGL(k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
+                         (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
+                         normpart, ldnp);
     return hipGetLastError();
   }
   const bool small = Kmax <= 64;

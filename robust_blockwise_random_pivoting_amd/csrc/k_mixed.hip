@@ -457,6 +457,7 @@ __global__ __launch_bounds__(256) void k_gemm_trail32(int64_t M, DimSpec dN, Dim
 // ---------------------------------------------------------------------------
 template <int KB, bool NORMS>
 __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, DimSpec dK, int T,
+                                                       int ncol, int nrg,
                                                        const float* __restrict__ A0, int64_t lda,
                                                        int64_t acoff, const int* acoff_dyn,
                                                        const float* __restrict__ Bt, int64_t ldbt,
@@ -469,8 +470,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
   constexpr int RPT = 64 / TPC;       // rows per stager thread
   __shared__ float As[2][32 * LDP];
   const int64_t N = dim_eval(dN), K = dim_eval(dK);
-  const int64_t c0 = (int64_t)blockIdx.x * 128;
-  const int64_t rbase = (int64_t)blockIdx.y * T * 64;
+  // task = column tile + ncol * row group, XCD-aware (a row group's column
+  // tiles share its A rows through one L2)
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)ncol * nrg);
+  if (task >= (int64_t)ncol * nrg) return;
+  const int64_t bx = task % ncol, by = task / ncol;
+  const int64_t c0 = bx * 128;
+  const int64_t rbase = by * T * 64;
   if (c0 >= N || rbase >= M || K <= 0) return;  // uniform per workgroup
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -586,7 +592,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
         x += __shfl_xor(x, 4, 64);
         x += __shfl_xor(x, 8, 64);
         const int cl = 16 * h + 4 * lk + v;
-        if (li == 0 && cl < ncw) normpart[(int64_t)blockIdx.y * ldnp + cw + cl] = x;
+        if (li == 0 && cl < ncw) normpart[by * ldnp + cw + cl] = x;
       }
   }
 }
@@ -603,11 +609,13 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
     // 128-column workgroups with the A block shared through LDS
     const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;
     const int64_t T = std::max<int64_t>(1, std::min<int64_t>(8, (ncol * nrow) / 512));
-    dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
-    if (groups) *groups = (int64_t)grid.y;
-#define UPD32_LAUNCH(KB, NM)                                                                    \
-  hipLaunchKernelGGL((k_gemm_upd32<KB, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,   \
-                     acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp)
+    const int64_t nrg = (nrow + T - 1) / T;
+    dim3 grid((unsigned)rbrp_xcd_grid(ncol * nrg));
+    if (groups) *groups = nrg;
+#define UPD32_LAUNCH(KB, NM)                                                                     \
+  hipLaunchKernelGGL((k_gemm_upd32<KB, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol, \
+                     (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,     \
+                     normpart, ldnp)
     if (Kmax <= 64) {
       if (normpart)
         UPD32_LAUNCH(64, true);

@@ -118,6 +118,19 @@ __device__ inline int rbrp_block_sum_int(int v, int* red) {
   return t;
 }
 
+// XCD-aware workgroup order.  The hardware hands workgroups to the 8 XCDs
+// (each with its own L2) round-robin by linear index.  A kernel launched on a
+// 1-D grid of rbrp_xcd_grid(W) workgroups maps workgroup L to the logical task
+// rbrp_xcd_task(L, W) = (L % 8) * ceil(W / 8) + L / 8, so each XCD runs one
+// contiguous range of tasks (and tasks that share an operand, numbered
+// consecutively, share that XCD's L2); tasks >= W are empty.
+static inline int64_t rbrp_xcd_grid(int64_t W) { return (W + 7) / 8 * 8; }
+#if defined(__HIPCC__)
+__device__ inline int64_t rbrp_xcd_task(int64_t L, int64_t W) {
+  return (L % 8) * ((W + 7) / 8) + L / 8;
+}
+#endif
+
 // Per-block device state shared by the kernels of one RBRP block step.
 struct RbrpBlockState {
   int s;            // number of sampled columns
