@@ -43,6 +43,10 @@ int rbrp_test_gemm_upd(void* stream, int64_t M, int64_t N, int64_t K, const doub
 int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A,
                         const double* B, int64_t ld, double* C, int64_t ldc, int nsplit);
 
+/* Workgroups of the split-K projection kernel resident at once on the current
+ * device: which = 0 for the FP64 kernel, 1 for the FP32 one (0 if unknown). */
+int rbrp_test_resident(int which);
+
 /* FP64 tall NN kernel (64 x 64 tiles, long reduction): C (M x N, ldc) (op)=
  * A B with A (M x K, lda even) and B (K x N, ldb) column-major; M a multiple
  * of 4; mode 0 sets, 1 subtracts, 2 adds. */

@@ -97,6 +97,7 @@ def lib():
         "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
         "rbrp_test_gemm_tn64": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn64": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
+        "rbrp_test_resident": (c_int, [c_int]),
         "rbrp_test_gemm_tn32": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
         "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,

@@ -219,9 +219,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
     }                                                                                           \
   } while (0)
 
-  // The loads are unconditional (a step beyond the chunk re-reads its last
-  // step), so that the vector-memory counter waits of the stores count only
-  // the loads issued before them.
+  // The loads and the LDS stores are unconditional (a step beyond the chunk
+  // re-reads its last step; a store past the last step writes the buffer that
+  // is not read), so that on every path the vector-memory counter waits of the
+  // stores count only the loads issued before them.
   const int64_t klast = kb1 - TN64_KS;
   double2 Xa0, Xa1, Xb0, Xb1, Xb2, Xb3;
   double2 Ya0, Ya1, Yb0, Yb1, Yb2, Yb3;
@@ -229,28 +230,26 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
   TN64_STORE(X, 0);
   TN64_LOAD(X, min(kb0 + TN64_KS, klast));
   __syncthreads();
-  // step k reads buffer k & 1; X holds step k+1 at even k, Y at odd k
-  for (int64_t kb = kb0; kb < kb1; kb += 2 * TN64_KS) {
-    {
-      const bool more1 = kb + TN64_KS < kb1;  // uniform
-      TN64_LOAD(Y, min(kb + 2 * TN64_KS, klast));
-      __builtin_amdgcn_sched_barrier(0);
-      TN64_STEP(0);
-      __builtin_amdgcn_sched_barrier(0);
-      if (more1) TN64_STORE(X, 1);
-      __syncthreads();
-      if (!more1) break;
-    }
-    {
-      const bool more1 = kb + 2 * TN64_KS < kb1;
-      TN64_LOAD(X, min(kb + 3 * TN64_KS, klast));
-      __builtin_amdgcn_sched_barrier(0);
-      TN64_STEP(1);
-      __builtin_amdgcn_sched_barrier(0);
-      if (more1) TN64_STORE(Y, 0);
-      __syncthreads();
-    }
+  // Whole pairs of steps, then an odd last step: no branch inside the loop
+  // body, so no path reaches the loop head with a load still unstored.  Step k
+  // reads buffer k & 1; X holds step k+1 at even k, Y at odd k.
+  const int64_t nsteps = (kb1 - kb0) / TN64_KS;
+  int64_t kb = kb0;
+  for (int64_t s2 = 2; s2 <= nsteps; s2 += 2, kb += 2 * TN64_KS) {
+    TN64_LOAD(Y, min(kb + 2 * TN64_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    TN64_STEP(0);
+    __builtin_amdgcn_sched_barrier(0);
+    TN64_STORE(X, 1);
+    __syncthreads();
+    TN64_LOAD(X, min(kb + 3 * TN64_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    TN64_STEP(1);
+    __builtin_amdgcn_sched_barrier(0);
+    TN64_STORE(Y, 0);
+    __syncthreads();
   }
+  if (nsteps & 1) TN64_STEP(0);
 #undef TN64_LOAD
 #undef TN64_STORE
 #undef TN64_STEP
@@ -265,6 +264,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
         const int64_t c = n0 + 32 * wid + 16 * h + 4 * v + lg;
         if (r < M && c < N) Pz[r + c * ldp] = acc[mb][h][v];
       }
+}
+
+int rbrp_gemm_tn64_resident() {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_tn64, 256, 0) != hipSuccess)
+    return 0;
+  return cus * per;
 }
 
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
@@ -326,31 +334,31 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
 
 // (the macro's locals carry an nl_ prefix: a local named like a variable of
 // the argument expression would be initialised from itself)
-#define NN64_LOAD(S, kk)                                                                  \
-  do {                                                                                    \
-    const int64_t nl_ka = (kk) + ak;                                                      \
-    const double* nl_ac = a0p + min(nl_ka, K - 1) * A.ld;                                 \
-    const double2 nl_t0 = *(const double2*)nl_ac;                                         \
-    const double2 nl_t1 = *(const double2*)(nl_ac + 2);                                   \
-    const long long nl_am = nl_ka < K ? -1ll : 0ll;                                       \
-    S##a0 = make_double2(nn64_mask(nl_t0.x, nl_am), nn64_mask(nl_t0.y, nl_am));           \
-    S##a1 = make_double2(nn64_mask(nl_t1.x, nl_am), nn64_mask(nl_t1.y, nl_am));           \
-    const int64_t nl_kb = (kk) + bk;                                                      \
-    const double nl_u0 = bcol[min(nl_kb, K - 1)], nl_u1 = bcol[min(nl_kb + 1, K - 1)];    \
-    const double nl_u2 = bcol[min(nl_kb + 2, K - 1)], nl_u3 = bcol[min(nl_kb + 3, K - 1)];\
-    S##b0 = make_double2(nn64_mask(nl_u0, nl_kb < K ? -1ll : 0ll),                        \
-                         nn64_mask(nl_u1, nl_kb + 1 < K ? -1ll : 0ll));                   \
-    S##b1 = make_double2(nn64_mask(nl_u2, nl_kb + 2 < K ? -1ll : 0ll),                    \
-                         nn64_mask(nl_u3, nl_kb + 3 < K ? -1ll : 0ll));                   \
+#define NN64_LOAD(S, kk)                                                                 \
+  do {                                                                                   \
+    const double* nl_ac = a0p + min((int64_t)(kk) + ak, K - 1) * A.ld;                   \
+    S##a0 = *(const double2*)nl_ac;                                                      \
+    S##a1 = *(const double2*)(nl_ac + 2);                                                \
+    const int64_t nl_kb = (kk) + bk;                                                     \
+    S##b0 = make_double2(bcol[min(nl_kb, K - 1)], bcol[min(nl_kb + 1, K - 1)]);          \
+    S##b1 = make_double2(bcol[min(nl_kb + 2, K - 1)], bcol[min(nl_kb + 3, K - 1)]);      \
   } while (0)
-#define NN64_STORE(S, bb)                                                           \
-  do {                                                                              \
-    As[bb][NN64_AT(ar, ak >> 1) + (ak & 1)] = S##a0.x;                              \
-    As[bb][NN64_AT(ar + 1, ak >> 1) + (ak & 1)] = S##a0.y;                          \
-    As[bb][NN64_AT(ar + 2, ak >> 1) + (ak & 1)] = S##a1.x;                          \
-    As[bb][NN64_AT(ar + 3, ak >> 1) + (ak & 1)] = S##a1.y;                          \
-    *(double2*)&Bs[bb][NN64_AT(bc, bk >> 1)] = S##b0;                               \
-    *(double2*)&Bs[bb][NN64_AT(bc, (bk >> 1) + 1)] = S##b1;                         \
+// B's rows >= K are zeroed here, at the store (a mask applied where the loads
+// are issued would make the compiler wait for them there).  A's clamped
+// columns are not masked: they only meet those zero rows of B.
+#define NN64_STORE(S, bb, kk)                                                              \
+  do {                                                                                     \
+    const int64_t nl_kb = (kk) + bk;                                                       \
+    As[bb][NN64_AT(ar, ak >> 1) + (ak & 1)] = S##a0.x;                                     \
+    As[bb][NN64_AT(ar + 1, ak >> 1) + (ak & 1)] = S##a0.y;                                 \
+    As[bb][NN64_AT(ar + 2, ak >> 1) + (ak & 1)] = S##a1.x;                                 \
+    As[bb][NN64_AT(ar + 3, ak >> 1) + (ak & 1)] = S##a1.y;                                 \
+    *(double2*)&Bs[bb][NN64_AT(bc, bk >> 1)] =                                             \
+        make_double2(nn64_mask(S##b0.x, nl_kb < K ? -1ll : 0ll),                           \
+                     nn64_mask(S##b0.y, nl_kb + 1 < K ? -1ll : 0ll));                      \
+    *(double2*)&Bs[bb][NN64_AT(bc, (bk >> 1) + 1)] =                                       \
+        make_double2(nn64_mask(S##b1.x, nl_kb + 2 < K ? -1ll : 0ll),                       \
+                     nn64_mask(S##b1.y, nl_kb + 3 < K ? -1ll : 0ll));                      \
   } while (0)
 #define NN64_STEP(bb)                                                                      \
   do {                                                                                     \
@@ -373,35 +381,35 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
   for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
 
   // two steps of loads in flight; the loads are unconditional (beyond K they
-  // read clamped addresses and yield zeros), so the vector-memory waits of the
-  // stores count only the loads issued before them
+  // read clamped addresses), so the vector-memory waits of the stores count
+  // only the loads issued before them
   double2 Xa0, Xa1, Xb0, Xb1;
   double2 Ya0, Ya1, Yb0, Yb1;
   NN64_LOAD(X, 0);
-  NN64_STORE(X, 0);
+  NN64_STORE(X, 0, 0);
   NN64_LOAD(X, NN64_KS);
   __syncthreads();
-  for (int64_t kb = 0; kb < K; kb += 2 * NN64_KS) {
-    {
-      const bool more1 = kb + NN64_KS < K;  // uniform
-      NN64_LOAD(Y, kb + 2 * NN64_KS);
-      __builtin_amdgcn_sched_barrier(0);
-      NN64_STEP(0);
-      __builtin_amdgcn_sched_barrier(0);
-      if (more1) NN64_STORE(X, 1);
-      __syncthreads();
-      if (!more1) break;
-    }
-    {
-      const bool more1 = kb + 2 * NN64_KS < K;
-      NN64_LOAD(X, kb + 3 * NN64_KS);
-      __builtin_amdgcn_sched_barrier(0);
-      NN64_STEP(1);
-      __builtin_amdgcn_sched_barrier(0);
-      if (more1) NN64_STORE(Y, 0);
-      __syncthreads();
-    }
+  // Whole pairs of steps, then an odd last step: no branch inside the loop
+  // body, and the stores are unconditional (one past the last step writes the
+  // buffer that is not read), so every path waits for its loads at the store
+  // and nowhere else.
+  const int64_t nsteps = (K + NN64_KS - 1) / NN64_KS;
+  int64_t kb = 0;
+  for (int64_t s2 = 2; s2 <= nsteps; s2 += 2, kb += 2 * NN64_KS) {
+    NN64_LOAD(Y, kb + 2 * NN64_KS);
+    __builtin_amdgcn_sched_barrier(0);
+    NN64_STEP(0);
+    __builtin_amdgcn_sched_barrier(0);
+    NN64_STORE(X, 1, kb + NN64_KS);
+    __syncthreads();
+    NN64_LOAD(X, kb + 3 * NN64_KS);
+    __builtin_amdgcn_sched_barrier(0);
+    NN64_STEP(1);
+    __builtin_amdgcn_sched_barrier(0);
+    NN64_STORE(Y, 0, kb + 2 * NN64_KS);
+    __syncthreads();
   }
+  if (nsteps & 1) NN64_STEP(0);
 #undef NN64_LOAD
 #undef NN64_STORE
 #undef NN64_STEP

@@ -293,6 +293,15 @@ __global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64
       }
 }
 
+int rbrp_gemm_tn32_resident() {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_tn32, 256, 0) != hipSuccess)
+    return 0;
+  return cus * per;
+}
+
 hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, const float* A,
                                  int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,

@@ -130,10 +130,14 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
 
 // Split-K shape for a TN product of Mmax x Nmax over K rows: aim for about
 // 1024 workgroups, keep the partials within the workspace.
+// Split count and chunk length of a split-K TN launch: as many splits as fill
+// `resident` workgroups (the kernel's whole-device residency: one round, no
+// tail of a few late workgroups) or about 1024 workgroups when that is unknown.
 void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pcap, int* nsplit,
-                  int64_t* kchunk, int64_t align = 16, int64_t tile_n = 64) {
+                  int64_t* kchunk, int64_t align = 16, int64_t tile_n = 64, int resident = 0) {
   const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + tile_n - 1) / tile_n);
-  int64_t ns = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  const int64_t target = resident > 0 ? resident : 1024;
+  int64_t ns = std::max<int64_t>(1, target / std::max<int64_t>(tiles, 1));
   ns = std::min<int64_t>(ns, tiles <= 4 ? 256 : 128);
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, Pcap / std::max<int64_t>(ldp * Nmax, 1)));
   ns = std::min<int64_t>(ns, std::max<int64_t>(1, K / 64));
@@ -312,6 +316,8 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
     delete h;
     return (int)e;
   }
+  h->tn64_resident = rbrp_gemm_tn64_resident();
+  h->tn32_resident = rbrp_gemm_tn32_resident();
   *handle = h;
   return 0;
 }
@@ -487,7 +493,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       int64_t kc;
       // C = Q^T V computed as its transpose V^T Q (the sampled block on the
       // 64-row side of the tiles), reduced into C column-major
-      choose_split(b, i, mp, b, ws.Pcap, &ns, &kc, 16, 128);
+      choose_split(b, i, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
       HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, mp, ns, kc, Vblk, Qall,
                                    ws.P, b, i * (int64_t)b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, ws.P, b,
@@ -561,7 +567,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
-      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128);
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128,
+                   mixed ? h->tn32_resident : h->tn64_resident);
       ph_proj = pt.begin(RBRP_PHASE_PROJECT);
       if (mixed)
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,

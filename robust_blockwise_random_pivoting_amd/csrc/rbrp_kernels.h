@@ -65,6 +65,10 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
                                  int64_t ldp, int64_t pstride);
+// Workgroups of k_gemm_tn64 / k_gemm_tn32 resident at once on the current
+// device (compute units x occupancy); 0 if the runtime cannot tell.
+int rbrp_gemm_tn64_resident();
+int rbrp_gemm_tn32_resident();
 
 // C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) on 64 x 64 output tiles with
 // double-buffered K steps (the re-orthogonalisation V -= Q C); any K, M a

@@ -87,6 +87,10 @@ int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
   return (int)e;
 }
 
+int rbrp_test_resident(int which) {
+  return which == 0 ? rbrp_gemm_tn64_resident() : rbrp_gemm_tn32_resident();
+}
+
 int rbrp_test_gemm_nn64(void* stream, int64_t M, int64_t N, int64_t K, const double* A, int64_t lda,
                         const double* B, int64_t ldb, double* C, int64_t ldc, int mode) {
   if (M < 0 || M % 4) return -2;
