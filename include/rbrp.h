@@ -93,7 +93,9 @@ typedef struct {
 int rbrp_create(rbrp_handle_t* handle, void* stream);
 int rbrp_destroy(rbrp_handle_t handle);
 int rbrp_set_stream(rbrp_handle_t handle, void* stream);
-/* Enable (1) or disable (0) per-phase hipEvent timing; reset clears the stats. */
+/* Per-phase hipEvent timing: 0 off, 1 every phase, 2 only the projection and
+ * update phases (RBRP_PHASE_PROJECT, RBRP_PHASE_UPDATE: two events per block
+ * around each, the least added to the stream).  reset clears the stats. */
 int rbrp_set_timing(rbrp_handle_t handle, int enable);
 int rbrp_get_phase_stats(rbrp_handle_t handle, rbrp_phase_stats_t* out);
 int rbrp_reset_phase_stats(rbrp_handle_t handle);

@@ -126,8 +126,9 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row blocks of
 // A.  Each 16-row step of both operands is staged through double-buffered
 // LDS ([column][row], one 128-byte run per column and step; the eight 16-byte
-// units of a column are XOR-swizzled with bits 1..3 of the column index so
-// that the 16 lanes of a pass read distinct banks): the next step's global
+// units of a column are XOR-swizzled with the column index mod 8, so that the
+// eight lanes of each 128-byte pass of a ds_read_b128 / ds_write_b128 touch
+// eight distinct 16-byte bank groups): the next step's global
 // loads are issued into staging registers before the current step's MFMAs and
 // written to LDS after them (one barrier per step).  Lane (j, g) reads rows
 // 4g .. 4g+3 of its column as two ds_read_b128 and MFMA q of the step consumes
@@ -139,7 +140,7 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // ---------------------------------------------------------------------------
 #define TN64_KS 16
 // LDS offset (doubles) of 16-byte unit u (rows 2u, 2u+1) of staged column col
-#define TN64_AT(col, u) ((col) * TN64_KS + 2 * ((u) ^ (((col) >> 1) & 7)))
+#define TN64_AT(col, u) ((col) * TN64_KS + 2 * ((u) ^ ((col) & 7)))
 
 __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, int64_t K,
                                                       int64_t kchunk, MatRef A, MatRef B,
@@ -277,13 +278,13 @@ int rbrp_gemm_tn64_resident() {
 
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
-                                 int64_t ldp, int64_t pstride) {
+                                 int64_t ldp, int64_t pstride, hipEvent_t e0, hipEvent_t e1) {
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
   if (kchunk % TN64_KS != 0 || K % TN64_KS != 0 || A.ld % 2 != 0 || B.ld % 2 != 0)
     return hipErrorInvalidValue;
   const int64_t gx = (Nmax + 127) / 128, gy = (Mmax + 63) / 64;
   dim3 grid((unsigned)rbrp_xcd_grid(gx * gy * nsplit));
-  hipLaunchKernelGGL(k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride,
+  RBRP_LAUNCH_EV(e0, e1, k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride,
                      (int)gx, (int)gy, nsplit);
   return hipGetLastError();
 }
@@ -292,21 +293,22 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
 // Tall NN with a long reduction (the re-orthogonalisation V -= Q C):
 // C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for r < M, c < N, any K.
 //
-// Workgroup tile: 64 rows (blockIdx.y) x 64 columns (blockIdx.x); wave w owns
-// rows 16w .. 16w+15 against the four 16-column blocks.  Each 16-deep step of
+// Workgroup tile: 64 rows x 64 columns (an XCD-aware task); wave w owns rows
+// 16w .. 16w+15 against the four 16-column blocks.  Each 16-deep step of
 // both operands is staged through double-buffered LDS with two steps of
 // global loads in flight: A as [row][k] (16 k per row), B as [column][k], the
-// eight 16-byte units of a row / column XOR-swizzled with bits 1..3 of its
-// index.  Lane (j, g) reads k = 4g .. 4g+3 of its row / column as two
-// ds_read_b128 and MFMA q of the step consumes element q.  The MFMA computes
+// eight 16-byte units of a row / column XOR-swizzled with its index mod 8 (the
+// eight lanes of each 128-byte LDS pass touch distinct bank groups).  Lane
+// (j, g) reads k = 4g .. 4g+3 of its row / column as two ds_read_b128 and
+// MFMA q of the step consumes element q.  The MFMA computes
 // the transposed block (B's columns as its rows): lane l holds
 // C(r0 + 16w + l%16, c0 + 16cb + 4v + l/16), so every C load / store
 // instruction moves four 128-byte column runs.  Reduction indices >= K read
-// as zero (clamped loads, then a bit mask).  A is a plain or offset MatRef (no
-// column list).
+// as zero (clamped loads, then a bit mask on B).  A is a plain or offset
+// MatRef (no column list).
 // ---------------------------------------------------------------------------
 #define NN64_KS 16
-#define NN64_AT(idx, u) ((idx) * NN64_KS + 2 * ((u) ^ (((idx) >> 1) & 7)))
+#define NN64_AT(idx, u) ((idx) * NN64_KS + 2 * ((u) ^ ((idx) & 7)))
 // x with its bits and-ed with the mask m (0 or all ones): a loaded value kept
 // or replaced by +0 without a branch, so that every load stays unconditional
 __device__ __forceinline__ double nn64_mask(double x, long long m) {
@@ -314,9 +316,14 @@ __device__ __forceinline__ double nn64_mask(double x, long long m) {
 }
 
 __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, int64_t K, MatRef A,
-                                                      MatRef B, MatRef C, int mode) {
+                                                      MatRef B, MatRef C, int mode, int gx,
+                                                      int gy) {
+  // task = column tile + gx * row tile, XCD-aware (the column tiles of one
+  // row tile share its rows of A through one L2)
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy);
+  if (task >= (int64_t)gx * gy) return;
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const int64_t r0 = (task / gx) * 64, c0 = (task % gx) * 64;
   if (r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
 
   __shared__ double As[2][64 * NN64_KS];  // [buf][row][k], swizzled
@@ -324,9 +331,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lj = lane & 15, lg = lane >> 4;
-  // A stager: column k = ak of the step, rows ar .. ar+3 (coalesced along rows)
-  const int ak = tid >> 4, ar = (tid & 15) * 4;
-  const int64_t arow = min(r0 + ar, M - 4 > 0 ? M - 4 : 0);  // M >= 4 is not assumed
+  // A stager: row ar of the tile, k = ak .. ak+3 of the step (each of the four
+  // loads of a wave reads 64 consecutive rows of one column; the two LDS
+  // stores are 16-byte units of that row).  Rows >= M re-read row M-1: they
+  // only feed output rows that are never stored.
+  const int ar = tid & 63, ak = (tid >> 6) * 4;
+  const int64_t arow = min(r0 + ar, M - 1);
   // B stager: column bc, k = bk .. bk+3 (contiguous in B's column)
   const int bc = tid >> 2, bk = (tid & 3) * 4;
   const double* bcol = mat_col(B, min(c0 + bc, N - 1));
@@ -336,9 +346,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
 // the argument expression would be initialised from itself)
 #define NN64_LOAD(S, kk)                                                                 \
   do {                                                                                   \
-    const double* nl_ac = a0p + min((int64_t)(kk) + ak, K - 1) * A.ld;                   \
-    S##a0 = *(const double2*)nl_ac;                                                      \
-    S##a1 = *(const double2*)(nl_ac + 2);                                                \
+    const int64_t nl_ka = (kk) + ak;                                                     \
+    S##a0 = make_double2(a0p[min(nl_ka, K - 1) * A.ld], a0p[min(nl_ka + 1, K - 1) * A.ld]);\
+    S##a1 = make_double2(a0p[min(nl_ka + 2, K - 1) * A.ld],                              \
+                         a0p[min(nl_ka + 3, K - 1) * A.ld]);                             \
     const int64_t nl_kb = (kk) + bk;                                                     \
     S##b0 = make_double2(bcol[min(nl_kb, K - 1)], bcol[min(nl_kb + 1, K - 1)]);          \
     S##b1 = make_double2(bcol[min(nl_kb + 2, K - 1)], bcol[min(nl_kb + 3, K - 1)]);      \
@@ -349,10 +360,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
 #define NN64_STORE(S, bb, kk)                                                              \
   do {                                                                                     \
     const int64_t nl_kb = (kk) + bk;                                                       \
-    As[bb][NN64_AT(ar, ak >> 1) + (ak & 1)] = S##a0.x;                                     \
-    As[bb][NN64_AT(ar + 1, ak >> 1) + (ak & 1)] = S##a0.y;                                 \
-    As[bb][NN64_AT(ar + 2, ak >> 1) + (ak & 1)] = S##a1.x;                                 \
-    As[bb][NN64_AT(ar + 3, ak >> 1) + (ak & 1)] = S##a1.y;                                 \
+    *(double2*)&As[bb][NN64_AT(ar, ak >> 1)] = S##a0;                                      \
+    *(double2*)&As[bb][NN64_AT(ar, (ak >> 1) + 1)] = S##a1;                                \
     *(double2*)&Bs[bb][NN64_AT(bc, bk >> 1)] =                                             \
         make_double2(nn64_mask(S##b0.x, nl_kb < K ? -1ll : 0ll),                           \
                      nn64_mask(S##b0.y, nl_kb + 1 < K ? -1ll : 0ll));                      \
@@ -433,8 +442,9 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
                                  int64_t K, MatRef A, MatRef B, MatRef C, int mode) {
   if (Mmax <= 0 || Nmax <= 0 || K <= 0) return hipSuccess;
   if (A.cols != nullptr || A.ld % 2 != 0 || Mmax % 4 != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((Nmax + 63) / 64), (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode);
+  const int64_t gx = (Nmax + 63) / 64, gy = (Mmax + 63) / 64;
+  dim3 grid((unsigned)rbrp_xcd_grid(gx * gy));
+  hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode, (int)gx, (int)gy);
   return hipGetLastError();
 }
 
@@ -944,7 +954,8 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                  double* normpart, int64_t ldnp, int64_t* groups) {
+                                  double* normpart, int64_t ldnp, int64_t* groups, hipEvent_t e0,
+                                  hipEvent_t e1) {
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128) return hipErrorInvalidValue;
@@ -957,11 +968,11 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
     dim3 grid((unsigned)rbrp_xcd_grid(ncol * nrg));
     if (groups) *groups = nrg;
     if (normpart)
-      hipLaunchKernelGGL(k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
+      RBRP_LAUNCH_EV(e0, e1, k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
                          (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
                          normpart, ldnp);
     else
-      hipLaunchKernelGGL(k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
+      RBRP_LAUNCH_EV(e0, e1, k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
                          (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
                          normpart, ldnp);
     return hipGetLastError();
@@ -975,17 +986,17 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
   if (groups) *groups = (int64_t)grid.y;
   if (small) {
     if (normpart)
-      hipLaunchKernelGGL((k_gemm_trail<RBRP_TRAIL_JB, 16, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_trail<RBRP_TRAIL_JB, 16, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
     else
-      hipLaunchKernelGGL((k_gemm_trail<RBRP_TRAIL_JB, 16, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_trail<RBRP_TRAIL_JB, 16, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
   } else {
     if (normpart)
-      hipLaunchKernelGGL((k_gemm_trail<1, 32, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_trail<1, 32, true>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
     else
-      hipLaunchKernelGGL((k_gemm_trail<1, 32, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_trail<1, 32, false>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,
                          acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp);
   }
   return hipGetLastError();

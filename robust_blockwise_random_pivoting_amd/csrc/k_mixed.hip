@@ -168,39 +168,48 @@ hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
 // are column-major FP32 with a common leading dimension; their first columns
 // are acoff + *a_dyn and bcoff + *b_dyn.
 //
-// Workgroup tile: 64 rows (blockIdx.y) x 128 columns; wave w owns columns
-// 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row blocks of
-// A.  Each 32-row step of both operands is staged through double-buffered
-// LDS ([column][row], one 128-byte run per column and step): the next step's
-// global loads are issued into staging registers before the current step's
-// MFMAs and written to LDS after them, so the only wait on global memory
-// comes after the MFMAs (one barrier per step).  Lane (j, g) reads rows
-// 8g .. 8g+7 of its column as two ds_read_b128, and MFMA q of the step
-// consumes element q: the eight MFMAs cover the 32 rows in the order 8g + q.
-// The MFMA computes the transposed block (B's columns as its rows), so lane
-// l holds P(r0 + l%16, c0 + 4(l/16) + v) and each partial store instruction
-// writes four 128-byte runs of P.  kchunk and K are multiples of 32; rows >=
-// the true height are zero.
+// Workgroup tile: 64 rows x 128 columns (an XCD-aware task); wave w owns
+// columns 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row
+// blocks of A.  Each 32-row step of both operands is staged through
+// double-buffered LDS ([column][row], one 128-byte run per column and step,
+// its eight 16-byte units XOR-swizzled with the column index mod 8) with two
+// steps of global loads in flight: the loads of step k+2 are issued before
+// the MFMAs of step k and step k+1's registers are written to LDS after them
+// (one barrier per step).  Lane (j, g) reads rows 8g .. 8g+7 of its column as
+// two ds_read_b128, and MFMA q of the step consumes element q: the eight
+// MFMAs cover the 32 rows in the order 8g + q.  The MFMA computes the
+// transposed block (B's columns as its rows), so lane l holds
+// P(r0 + l%16, c0 + 4(l/16) + v) and each partial store instruction writes
+// four 128-byte runs of P.  kchunk and K are multiples of 32; rows >= the
+// true height are zero.  Every load and LDS store is unconditional (a step
+// past the chunk re-reads its last step and writes the buffer not read), so
+// the vector-memory waits count only the loads issued before each store.
 // ---------------------------------------------------------------------------
 #define TN32_KS 32
-#define TN32_LD (TN32_KS + 4)  // LDS pitch (floats) of one staged column
+// LDS offset (floats) of 16-byte unit u (rows 4u .. 4u+3) of staged column col
+#define TN32_AT(col, u) ((col) * TN32_KS + 4 * ((u) ^ ((col) & 7)))
 
-__global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64_t K,
-                                                   int64_t kchunk, const float* __restrict__ A0,
-                                                   int64_t acoff, const int* a_dyn,
-                                                   const float* __restrict__ B0, int64_t bcoff,
-                                                   const int* b_dyn, int64_t ld,
-                                                   double* __restrict__ P, int64_t ldp,
-                                                   int64_t pstride) {
+__global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64_t K,
+                                                      int64_t kchunk, const float* __restrict__ A0,
+                                                      int64_t acoff, const int* a_dyn,
+                                                      const float* __restrict__ B0, int64_t bcoff,
+                                                      const int* b_dyn, int64_t ld,
+                                                      double* __restrict__ P, int64_t ldp,
+                                                      int64_t pstride, int gx, int gy, int gz) {
+  // task = N tile + gx * (M tile + gy * split), XCD-aware (the tiles of one
+  // split share its rows of A through one L2)
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy * gz);
+  if (task >= (int64_t)gx * gy * gz) return;
+  const int64_t bx = task % gx, by = (task / gx) % gy, bz = task / ((int64_t)gx * gy);
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t n0 = (int64_t)blockIdx.x * 128, m0 = (int64_t)blockIdx.y * 64;
+  const int64_t n0 = bx * 128, m0 = by * 64;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup
-  const int64_t kb0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t kb0 = bz * kchunk;
   const int64_t kb1 = min(K, kb0 + kchunk);
   if (kb0 >= kb1) return;  // empty chunk (uniform); its partials are never read
 
-  __shared__ float As[2][64 * TN32_LD];   // [buf][col][row]
-  __shared__ float Bs[2][128 * TN32_LD];
+  __shared__ float As[2][64 * TN32_KS];   // [buf][col][row], swizzled
+  __shared__ float Bs[2][128 * TN32_KS];
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lj = lane & 15, lg = lane >> 4;
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64
   const int bc = tid >> 1, bq = (tid & 1) * 16;   // B: column bc, rows bq .. bq+15
   const float* ap = A + min(m0 + ac, M - 1) * ld + aq;
   const float* bp = B + min(n0 + bc, N - 1) * ld + bq;
-  const int aoff = ac * TN32_LD + aq, boff = bc * TN32_LD + bq;
+  const int au = aq >> 2, bu = bq >> 2;  // first 16-byte unit of the stager
 
   v4f acc[4][2];
 #pragma unroll
@@ -220,67 +229,78 @@ __global__ __launch_bounds__(256) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-  // staging registers (named scalars: small arrays here get placed in LDS)
-  float4 sa0 = *(const float4*)(ap + kb0), sa1 = *(const float4*)(ap + kb0 + 4);
-  float4 sb0 = *(const float4*)(bp + kb0), sb1 = *(const float4*)(bp + kb0 + 4);
-  float4 sb2 = *(const float4*)(bp + kb0 + 8), sb3 = *(const float4*)(bp + kb0 + 12);
-  *(float4*)&As[0][aoff] = sa0;
-  *(float4*)&As[0][aoff + 4] = sa1;
-  *(float4*)&Bs[0][boff] = sb0;
-  *(float4*)&Bs[0][boff + 4] = sb1;
-  *(float4*)&Bs[0][boff + 8] = sb2;
-  *(float4*)&Bs[0][boff + 12] = sb3;
-  __syncthreads();
+  // two sets of staging registers (named scalars: small arrays here get
+  // placed in LDS)
+#define TN32_LOAD(S, kk)                                  \
+  do {                                                    \
+    S##a0 = *(const float4*)(ap + (kk));                  \
+    S##a1 = *(const float4*)(ap + (kk) + 4);              \
+    S##b0 = *(const float4*)(bp + (kk));                  \
+    S##b1 = *(const float4*)(bp + (kk) + 4);              \
+    S##b2 = *(const float4*)(bp + (kk) + 8);              \
+    S##b3 = *(const float4*)(bp + (kk) + 12);             \
+  } while (0)
+#define TN32_STORE(S, bb)                                 \
+  do {                                                    \
+    *(float4*)&As[bb][TN32_AT(ac, au)] = S##a0;           \
+    *(float4*)&As[bb][TN32_AT(ac, au + 1)] = S##a1;       \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu)] = S##b0;           \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 1)] = S##b1;       \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 2)] = S##b2;       \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 3)] = S##b3;       \
+  } while (0)
+#define TN32_STEP(bb)                                                                         \
+  do {                                                                                        \
+    const float* as = &As[bb][0];                                                             \
+    const float* bs = &Bs[bb][0];                                                             \
+    float bv[2][8];                                                                           \
+    _Pragma("unroll") for (int h = 0; h < 2; ++h) {                                           \
+      const float4 x0 = *(const float4*)&bs[TN32_AT(32 * wid + 16 * h + lj, 2 * lg)];         \
+      const float4 x1 = *(const float4*)&bs[TN32_AT(32 * wid + 16 * h + lj, 2 * lg + 1)];     \
+      bv[h][0] = x0.x; bv[h][1] = x0.y; bv[h][2] = x0.z; bv[h][3] = x0.w;                     \
+      bv[h][4] = x1.x; bv[h][5] = x1.y; bv[h][6] = x1.z; bv[h][7] = x1.w;                     \
+    }                                                                                         \
+    _Pragma("unroll") for (int mb = 0; mb < 4; ++mb) {                                        \
+      const float4 a0 = *(const float4*)&as[TN32_AT(16 * mb + lj, 2 * lg)];                   \
+      const float4 a1 = *(const float4*)&as[TN32_AT(16 * mb + lj, 2 * lg + 1)];               \
+      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};                   \
+      _Pragma("unroll") for (int q = 0; q < 8; ++q) {                                         \
+        acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[0][q], av[q], acc[mb][0], 0, 0, 0); \
+        acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[1][q], av[q], acc[mb][1], 0, 0, 0); \
+      }                                                                                       \
+    }                                                                                         \
+  } while (0)
 
-  int buf = 0;
-  for (int64_t kb = kb0; kb < kb1; kb += TN32_KS) {
-    const bool more = kb + TN32_KS < kb1;  // uniform
-    if (more) {
-      const int64_t kn = kb + TN32_KS;
-      sa0 = *(const float4*)(ap + kn);
-      sa1 = *(const float4*)(ap + kn + 4);
-      sb0 = *(const float4*)(bp + kn);
-      sb1 = *(const float4*)(bp + kn + 4);
-      sb2 = *(const float4*)(bp + kn + 8);
-      sb3 = *(const float4*)(bp + kn + 12);
-    }
-    __builtin_amdgcn_sched_barrier(0);  // the prefetch goes out before the MFMAs
-    const float* as = &As[buf][0];
-    const float* bs = &Bs[buf][0];
-    float bv[2][8];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const float4 x0 = *(const float4*)&bs[(32 * wid + 16 * h + lj) * TN32_LD + 8 * lg];
-      const float4 x1 = *(const float4*)&bs[(32 * wid + 16 * h + lj) * TN32_LD + 8 * lg + 4];
-      bv[h][0] = x0.x; bv[h][1] = x0.y; bv[h][2] = x0.z; bv[h][3] = x0.w;
-      bv[h][4] = x1.x; bv[h][5] = x1.y; bv[h][6] = x1.z; bv[h][7] = x1.w;
-    }
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      const float4 a0 = *(const float4*)&as[(16 * mb + lj) * TN32_LD + 8 * lg];
-      const float4 a1 = *(const float4*)&as[(16 * mb + lj) * TN32_LD + 8 * lg + 4];
-      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[0][q], av[q], acc[mb][0], 0, 0, 0);
-        acc[mb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[1][q], av[q], acc[mb][1], 0, 0, 0);
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);  // the staging stores wait for the loads after the MFMAs
-    if (more) {
-      float* an = &As[buf ^ 1][aoff];
-      float* bn = &Bs[buf ^ 1][boff];
-      *(float4*)an = sa0;
-      *(float4*)(an + 4) = sa1;
-      *(float4*)bn = sb0;
-      *(float4*)(bn + 4) = sb1;
-      *(float4*)(bn + 8) = sb2;
-      *(float4*)(bn + 12) = sb3;
-    }
+  const int64_t klast = kb1 - TN32_KS;
+  float4 Xa0, Xa1, Xb0, Xb1, Xb2, Xb3;
+  float4 Ya0, Ya1, Yb0, Yb1, Yb2, Yb3;
+  TN32_LOAD(X, kb0);
+  TN32_STORE(X, 0);
+  TN32_LOAD(X, min(kb0 + TN32_KS, klast));
+  __syncthreads();
+  // whole pairs of steps, then an odd last step: no branch inside the loop
+  // body.  Step k reads buffer k & 1; X holds step k+1 at even k, Y at odd k.
+  const int64_t nsteps = (kb1 - kb0) / TN32_KS;
+  int64_t kb = kb0;
+  for (int64_t s2 = 2; s2 <= nsteps; s2 += 2, kb += 2 * TN32_KS) {
+    TN32_LOAD(Y, min(kb + 2 * TN32_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    TN32_STEP(0);
+    __builtin_amdgcn_sched_barrier(0);
+    TN32_STORE(X, 1);
     __syncthreads();
-    buf ^= 1;
+    TN32_LOAD(X, min(kb + 3 * TN32_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    TN32_STEP(1);
+    __builtin_amdgcn_sched_barrier(0);
+    TN32_STORE(Y, 0);
+    __syncthreads();
   }
-  double* Pz = P + (int64_t)blockIdx.z * pstride;
+  if (nsteps & 1) TN32_STEP(0);
+#undef TN32_LOAD
+#undef TN32_STORE
+#undef TN32_STEP
+  double* Pz = P + bz * pstride;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -306,12 +326,13 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t K, int nsplit, int64_t kchunk, const float* A,
                                  int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,
                                  const int* b_dyn, int64_t ld, double* P, int64_t ldp,
-                                 int64_t pstride) {
+                                 int64_t pstride, hipEvent_t e0, hipEvent_t e1) {
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
   if (kchunk % TN32_KS != 0 || K % TN32_KS != 0 || ld % 4 != 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((Nmax + 127) / 128), (unsigned)((Mmax + 63) / 64), (unsigned)nsplit);
-  hipLaunchKernelGGL(k_gemm_tn32, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff, a_dyn, B, bcoff,
-                     b_dyn, ld, P, ldp, pstride);
+  const int64_t gx = (Nmax + 127) / 128, gy = (Mmax + 63) / 64;
+  dim3 grid((unsigned)rbrp_xcd_grid(gx * gy * nsplit));
+  RBRP_LAUNCH_EV(e0, e1, k_gemm_tn32, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff, a_dyn, B,
+                 bcoff, b_dyn, ld, P, ldp, pstride, (int)gx, (int)gy, nsplit);
   return hipGetLastError();
 }
 
@@ -610,7 +631,8 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
                                     int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
                                     const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
                                     int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                    double* normpart, int64_t ldnp, int64_t* groups) {
+                                    double* normpart, int64_t ldnp, int64_t* groups,
+                                    hipEvent_t e0, hipEvent_t e1) {
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128 || (M & 1) || (lda & 1) || (ldc & 1)) return hipErrorInvalidValue;
@@ -622,7 +644,7 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
     dim3 grid((unsigned)rbrp_xcd_grid(ncol * nrg));
     if (groups) *groups = nrg;
 #define UPD32_LAUNCH(KB, NM)                                                                     \
-  hipLaunchKernelGGL((k_gemm_upd32<KB, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol, \
+  RBRP_LAUNCH_EV(e0, e1, (k_gemm_upd32<KB, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol, \
                      (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,     \
                      normpart, ldnp)
     if (Kmax <= 64) {
@@ -645,7 +667,7 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
   dim3 grid((unsigned)ncol, (unsigned)((nrow + T - 1) / T));
   if (groups) *groups = (int64_t)grid.y;
 #define TRAIL32_LAUNCH(KS, NM)                                                                   \
-  hipLaunchKernelGGL((k_gemm_trail32<KS, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,  \
+  RBRP_LAUNCH_EV(e0, e1, (k_gemm_trail32<KS, NM>), grid, dim3(256), 0, st, M, N, K, (int)T, A, lda,  \
                      acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn, normpart, ldnp)
   if (small) {
     if (normpart)

@@ -755,22 +755,40 @@ hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_
 }
 
 // resid = sum(diags[i + bt : n]); err = resid / N0 (gpu/rbrp.cu:467).
+// Residual total after the block.  With `pub` (pinned, host-coherent memory)
+// the finished state is also published to the host: the fields are written
+// first, then `seq` with a system-scope release, so a host that reads seq ==
+// the expected number (with acquire) sees the whole state.
 __global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__ diags, int64_t i,
-                                                      int64_t n, RbrpBlockState* bs) {
+                                                      int64_t n, RbrpBlockState* bs,
+                                                      RbrpBlockState* pub, int seq) {
   __shared__ double red[16];
   const int64_t c0 = i + bs->bt;
   double acc = 0.0;
   for (int64_t j = c0 + threadIdx.x; j < n; j += 1024) acc += diags[j];
   const double t = rbrp_block_sum(acc, red);
   if (threadIdx.x == 0) {
+    const double err = (bs->N0 > 0.0) ? t / bs->N0 : 0.0;
     bs->resid = t;
-    bs->err = (bs->N0 > 0.0) ? t / bs->N0 : 0.0;
+    bs->err = err;
+    if (pub) {
+      pub->s = bs->s;
+      pub->bt = bs->bt;
+      pub->nswap = bs->nswap;
+      pub->status = bs->status;
+      pub->draws = bs->draws;
+      pub->resid = t;
+      pub->err = err;
+      pub->N0 = bs->N0;
+      pub->nnz = bs->nnz;
+      __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
-                                   RbrpBlockState* bs) {
-  hipLaunchKernelGGL(k_resid_total, dim3(1), dim3(1024), 0, st, diags, i, n, bs);
+                                   RbrpBlockState* bs, RbrpBlockState* pub, int seq) {
+  hipLaunchKernelGGL(k_resid_total, dim3(1), dim3(1024), 0, st, diags, i, n, bs, pub, seq);
   return hipGetLastError();
 }
 

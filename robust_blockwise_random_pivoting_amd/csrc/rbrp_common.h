@@ -142,5 +142,5 @@ struct RbrpBlockState {
   double err;       // resid / N0
   double N0;        // ||A||_F^2
   int nnz;          // nonzero weights seen by the sampler
-  int pad;
+  int seq;          // publication number of the host copy (k_resid_total)
 };

@@ -33,8 +33,9 @@ struct rbrp_handle_s {
   RbrpWorkspace ws;
   void* sk_arena = nullptr;  // workspace of the sketch comparators
   size_t sk_bytes = 0;
-  RbrpBlockState* host_state = nullptr;  // pinned
-  bool timing = false;
+  RbrpBlockState* host_state = nullptr;  // pinned, host-coherent
+  int pub_seq = 0;                        // last publication number handed out
+  int timing = 0;  // 0 off, 1 every phase, 2 projection and update only
   rbrp_phase_stats_t stats;
   std::vector<hipEvent_t> events;
   // batched calls: slot workspace, problem metadata, pinned staging, a

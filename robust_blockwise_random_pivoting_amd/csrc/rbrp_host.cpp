@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -156,6 +157,7 @@ struct PhaseTimer {
     int e0, e1;
     int launches;
     double flops, bytes;
+    bool by_kernel;  // events recorded by the kernel's dispatch
   };
   std::vector<Rec> recs;
   int next = 0;
@@ -168,14 +170,28 @@ struct PhaseTimer {
     }
     return h->events[idx];
   }
-  int begin(int phase) {
+  int open(int phase, bool by_kernel) {
     if (!h->timing) return -1;
-    Rec r{phase, next, next + 1, 0, 0.0, 0.0};
+    if (h->timing == 2 && phase != RBRP_PHASE_PROJECT && phase != RBRP_PHASE_UPDATE) return -1;
+    Rec r{phase, next, next + 1, 0, 0.0, 0.0, by_kernel};
     next += 2;
-    hipEvent_t e = ev(r.e0);
-    if (e) (void)hipEventRecord(e, h->stream);
+    if (!ev(r.e0) || !ev(r.e1)) return -1;
     recs.push_back(r);
     return (int)recs.size() - 1;
+  }
+  // A phase of several launches: marker events recorded around them.
+  int begin(int phase) {
+    const int id = open(phase, false);
+    if (id >= 0) (void)hipEventRecord(h->events[recs[id].e0], h->stream);
+    return id;
+  }
+  // A phase that is one launch: its events are handed to the launch and timed
+  // by the dispatch itself (*e0 / *e1 stay null when the phase is not timed).
+  int begin_kernel(int phase, hipEvent_t* e0, hipEvent_t* e1) {
+    const int id = open(phase, true);
+    *e0 = id >= 0 ? h->events[recs[id].e0] : nullptr;
+    *e1 = id >= 0 ? h->events[recs[id].e1] : nullptr;
+    return id;
   }
   void end(int id, int launches, double flops = 0.0, double bytes = 0.0) {
     if (id < 0) return;
@@ -183,8 +199,7 @@ struct PhaseTimer {
     r.launches = launches;
     r.flops = flops;
     r.bytes = bytes;
-    hipEvent_t e = ev(r.e1);
-    if (e) (void)hipEventRecord(e, h->stream);
+    if (!r.by_kernel) (void)hipEventRecord(h->events[r.e1], h->stream);
   }
   // Set the work figures of a record once the block's sizes are known.
   void set_work(int id, double flops, double bytes) {
@@ -192,10 +207,11 @@ struct PhaseTimer {
     recs[id].flops = flops;
     recs[id].bytes = bytes;
   }
-  void flush() {  // after a stream synchronisation
+  void flush() {  // waits for each record's end event
     for (const Rec& r : recs) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, h->events[r.e0], h->events[r.e1]) == hipSuccess)
+      if (hipEventSynchronize(h->events[r.e1]) == hipSuccess &&
+          hipEventElapsedTime(&ms, h->events[r.e0], h->events[r.e1]) == hipSuccess)
         h->stats.ms[r.phase] += ms;
       h->stats.launches[r.phase] += r.launches;
       h->stats.flops[r.phase] += r.flops;
@@ -205,6 +221,26 @@ struct PhaseTimer {
     next = 0;
   }
 };
+
+// Wait until the block's last kernel has published state number `seq` into
+// the pinned host copy (k_resid_total): spin on the sequence number, polling
+// the stream now and then so that an error or a stream that finished without
+// the publication ends the wait (the state is then copied).
+hipError_t wait_published(rbrp_handle_t h, hipStream_t st, const RbrpBlockState* dev_state, int seq) {
+  for (uint64_t it = 1;; ++it) {
+    if (__atomic_load_n(&h->host_state->seq, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
+    if ((it & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return q;
+      if (__atomic_load_n(&h->host_state->seq, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
+      return hipMemcpy(h->host_state, dev_state, sizeof(RbrpBlockState), hipMemcpyDeviceToHost);
+    }
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+}
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -311,7 +347,8 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
     delete h;
     return (int)e;
   }
-  e = hipHostMalloc((void**)&h->host_state, sizeof(RbrpBlockState), hipHostMallocDefault);
+  e = hipHostMalloc((void**)&h->host_state, sizeof(RbrpBlockState),
+                    hipHostMallocMapped | hipHostMallocCoherent);
   if (e != hipSuccess) {
     delete h;
     return (int)e;
@@ -341,7 +378,8 @@ int rbrp_set_stream(rbrp_handle_t h, void* stream) {
 
 int rbrp_set_timing(rbrp_handle_t h, int enable) {
   if (!h) return -1;
-  h->timing = enable != 0;
+  if (enable < 0 || enable > 2) return -2;
+  h->timing = enable;
   return 0;
 }
 
@@ -462,6 +500,9 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   HIPCHK(rbrp_launch_sum_to_state(st, ws.diags, n, ws.st, 0));
   HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  // the device copy's seq is never written: mark the host copy as holding the
+  // last publication handed out, so only the next one can match
+  __atomic_store_n(&h->host_state->seq, h->pub_seq, __ATOMIC_RELEASE);
   pt.flush();
   const double N0 = h->host_state->N0;
   if (!std::isfinite(N0)) return RBRP_ERR_NUMERIC;
@@ -515,11 +556,19 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     // ---- panel: Gram, pivoted Cholesky + filter, CholQR2 ----------------
     ph = pt.begin(RBRP_PHASE_PANEL);
     {
+      // Gram products: 64 x 64 tiles up to b = 64, the 64 x 128-tile kernel above
+      const bool wide = b > 64;
       int ns;
       int64_t kc;
-      choose_split(b, b, mp, b, ws.Pcap, &ns, &kc);
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, mp, ns, kc, Vblk,
-                                 Vblk, ws.P, b, (int64_t)b * b));
+      if (wide)
+        choose_split(b, b, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
+      else
+        choose_split(b, b, mp, b, ws.Pcap, &ns, &kc);
+      auto gram = [&](DimSpec d, MatRef X) {
+        return wide ? rbrp_launch_gemm_tn64(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b)
+                    : rbrp_launch_gemm_tn(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b);
+      };
+      HIPCHK(gram(dim_dyn(0, d_s, 1), Vblk));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G, b), GEMM_SET));
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
@@ -527,8 +576,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       // Q1 = V(:, Jk) Rb^{-1}
       HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
                                  Vkept, mat(ws.Rinv, b), mat(ws.Qbuf, mp), GEMM_SET, nullptr, 0));
-      HIPCHK(rbrp_launch_gemm_tn(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, mp, ns, kc,
-                                 mat(ws.Qbuf, mp), mat(ws.Qbuf, mp), ws.P, b, (int64_t)b * b));
+      HIPCHK(gram(dim_dyn(0, d_bt, 1), mat(ws.Qbuf, mp)));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G2, b), GEMM_SET));
       // R2, R2^{-1} and R(i:i+bt, i:i+bt) = R2 Rb
@@ -569,15 +617,16 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       int64_t kc;
       choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128,
                    mixed ? h->tn32_resident : h->tn64_resident);
-      ph_proj = pt.begin(RBRP_PHASE_PROJECT);
+      hipEvent_t ev0, ev1;
+      ph_proj = pt.begin_kernel(RBRP_PHASE_PROJECT, &ev0, &ev1);
       if (mixed)
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, ws.Aw32, i, nullptr, ws.Aw32, i, d_bt, mp, ws.P, b,
-                                     (int64_t)b * nrem_max));
+                                     (int64_t)b * nrem_max, ev0, ev1));
       else
         HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt),
-                                     ws.P, b, (int64_t)b * nrem_max));
+                                     ws.P, b, (int64_t)b * nrem_max, ev0, ev1));
       pt.end(ph_proj, 1);
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
       if (mixed)
@@ -590,27 +639,28 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                          nrem_max, ws.P, b, (int64_t)b * nrem_max, ns,
                                          mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
-      ph_upd = pt.begin(RBRP_PHASE_UPDATE);
+      ph_upd = pt.begin_kernel(RBRP_PHASE_UPDATE, &ev0, &ev1);
       int64_t groups = 0;  // norm partials written by the update (one per row group)
       if (mixed)
         HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                         b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
-                                        d_bt, ws.normpart, nrem_max, &groups));
+                                        d_bt, ws.normpart, nrem_max, &groups, ev0, ev1));
       else
         HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                       b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
-                                      ws.normpart, nrem_max, &groups));
+                                      ws.normpart, nrem_max, &groups, ev0, ev1));
       pt.end(ph_upd, 1);
       ph = pt.begin(RBRP_PHASE_RESID);
       HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags));
     } else {
       ph = pt.begin(RBRP_PHASE_RESID);
     }
-    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st));
+    h->pub_seq = h->pub_seq == INT_MAX ? 1 : h->pub_seq + 1;
+    const int seq = h->pub_seq;
+    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
     pt.end(ph, nrem_max > 0 ? 2 : 1);
 
-    HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(wait_published(h, st, ws.st, seq));
     const RbrpBlockState hs = *h->host_state;
     if (hs.status != 0) {
       pt.flush();

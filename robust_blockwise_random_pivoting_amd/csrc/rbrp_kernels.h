@@ -3,10 +3,22 @@
 // the host orchestration never names a __global__ symbol directly.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "rbrp_common.h"
+
+// Kernel launch that, given start / stop events (either may be null), has the
+// dispatch itself time them (hipExtLaunchKernelGGL): no marker packets enter
+// the stream around the kernel.  The kernel name may be parenthesised.
+#define RBRP_LAUNCH_EV(e0, e1, kern, grid, block, shm, st, ...)                    \
+  do {                                                                            \
+    if ((e0) != nullptr || (e1) != nullptr)                                       \
+      hipExtLaunchKernelGGL(kern, grid, block, shm, st, e0, e1, 0, __VA_ARGS__);  \
+    else                                                                          \
+      hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                \
+  } while (0)
 
 // A size that is either static (`base`) or `base + sign * (*dyn)` read on the
 // device, so one launch shape serves every block of the loop.
@@ -64,7 +76,8 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // multiples of 16, both leading dimensions even.
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
-                                 int64_t ldp, int64_t pstride);
+                                 int64_t ldp, int64_t pstride, hipEvent_t e0 = nullptr,
+                                 hipEvent_t e1 = nullptr);
 // Workgroups of k_gemm_tn64 / k_gemm_tn32 resident at once on the current
 // device (compute units x occupancy); 0 if the runtime cannot tell.
 int rbrp_gemm_tn64_resident();
@@ -107,7 +120,8 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                  double* normpart, int64_t ldnp, int64_t* groups = nullptr);
+                                  double* normpart, int64_t ldnp, int64_t* groups = nullptr,
+                                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // ---- k_rbrp.hip -----------------------------------------------------------
 hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
@@ -156,8 +170,10 @@ hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockS
                                  int64_t ld, const int* Jin, const int* Jout);
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
                                       int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
+// bs->resid / err after the block; with pub (pinned host-coherent memory) the
+// state is also published there, seq written last with a system-scope release.
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
-                                   RbrpBlockState* bs);
+                                   RbrpBlockState* bs, RbrpBlockState* pub = nullptr, int seq = 0);
 hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                   int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,
@@ -192,14 +208,15 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t K, int nsplit, int64_t kchunk, const float* A,
                                  int64_t acoff, const int* a_dyn, const float* B, int64_t bcoff,
                                  const int* b_dyn, int64_t ld, double* P, int64_t ldp,
-                                 int64_t pstride);
+                                 int64_t pstride, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // FP32 trailing update with FP64 norm partials over row groups (at most
 // ceil(M / 64)); see rbrp_launch_gemm_trail.  M, lda and ldc must be even.
 hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                     int64_t Kmax, const float* A, int64_t lda, int64_t acoff,
                                     const int* acoff_dyn, const float* Bt, int64_t ldbt, float* C,
                                     int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
-                                    double* normpart, int64_t ldnp, int64_t* groups = nullptr);
+                                    double* normpart, int64_t ldnp, int64_t* groups = nullptr,
+                                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
 // ---- k_sketch.hip ---------------------------------------------------------
 hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,

@@ -65,8 +65,11 @@ class Handle:
     def ptr(self):
         return self._h
 
-    def set_timing(self, enable: bool):
-        _capi.check("rbrp_set_timing", self._lib.rbrp_set_timing(self._h, int(bool(enable))))
+    def set_timing(self, enable):
+        """Per-phase event timing: False/0 off, True/1 every phase, 2 only the
+        projection and update phases (the least added to the stream)."""
+        level = int(enable)
+        _capi.check("rbrp_set_timing", self._lib.rbrp_set_timing(self._h, level))
 
     def reset_stats(self):
         _capi.check("rbrp_reset_phase_stats", self._lib.rbrp_reset_phase_stats(self._h))
