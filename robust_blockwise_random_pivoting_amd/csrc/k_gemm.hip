@@ -450,36 +450,45 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
 
 // ---------------------------------------------------------------------------
 // Split reduction: dst(r, c) (=|+=|-=) sum_{z=0}^{nsplit-1} P[z](r, c).
-// One workgroup per (64-row slab, column): four waves each sum every fourth
-// split (two accumulators), then the four partials are added in a fixed
-// order, so the result does not depend on scheduling.
+// One workgroup per (64-row slab, column): sixteen waves each sum every
+// sixteenth split with eight loads in flight per lane (eight accumulators),
+// then the sixteen partials are added in a fixed tree order, so the result
+// does not depend on scheduling.
 // ---------------------------------------------------------------------------
+#define RED_GROUPS 16
 template <typename T2>
-__global__ __launch_bounds__(256) void k_reduce_splits(DimSpec dM, DimSpec dN,
-                                                       const double* __restrict__ P, int64_t ldp,
-                                                       int64_t pstride, int nsplit, MatRef dst,
-                                                       int mode, T2* __restrict__ dst2,
-                                                       int64_t ld2) {
-  __shared__ double part[4][64];
+__global__ __launch_bounds__(64 * RED_GROUPS) void k_reduce_splits(
+    DimSpec dM, DimSpec dN, const double* __restrict__ P, int64_t ldp, int64_t pstride, int nsplit,
+    MatRef dst, int mode, T2* __restrict__ dst2, int64_t ld2) {
+  __shared__ double part[RED_GROUPS][64];
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
   const int64_t c = blockIdx.x;
   const int lr = threadIdx.x & 63, zg = threadIdx.x >> 6;
   const int64_t r = (int64_t)blockIdx.y * 64 + lr;
   if (c >= N || (int64_t)blockIdx.y * 64 >= M) return;  // uniform per workgroup
-  double s0 = 0.0, s1 = 0.0;
+  double acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = 0.0;
   if (r < M) {
     const double* p = P + r + c * ldp;
     int z = zg;
-    for (; z + 4 < nsplit; z += 8) {
-      s0 += p[(int64_t)z * pstride];
-      s1 += p[(int64_t)(z + 4) * pstride];
+    for (; z + 7 * RED_GROUPS < nsplit; z += 8 * RED_GROUPS) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += p[(int64_t)(z + u * RED_GROUPS) * pstride];
     }
-    for (; z < nsplit; z += 4) s0 += p[(int64_t)z * pstride];
+    for (; z < nsplit; z += RED_GROUPS) acc[0] += p[(int64_t)z * pstride];
   }
-  part[zg][lr] = s0 + s1;
+  part[zg][lr] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   __syncthreads();
   if (zg == 0 && r < M) {
-    const double s = (part[0][lr] + part[1][lr]) + (part[2][lr] + part[3][lr]);
+    double t[RED_GROUPS / 2];
+#pragma unroll
+    for (int g = 0; g < RED_GROUPS / 2; ++g) t[g] = part[2 * g][lr] + part[2 * g + 1][lr];
+#pragma unroll
+    for (int w = RED_GROUPS / 4; w >= 1; w >>= 1)
+#pragma unroll
+      for (int g = 0; g < w; ++g) t[g] = t[2 * g] + t[2 * g + 1];
+    const double s = t[0];
     if (dst.p) {
       double* d = mat_col(dst, c) + r;
       if (mode == GEMM_ADD)
@@ -498,8 +507,8 @@ hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64
                                      MatRef dst, int mode, double* dst2, int64_t ld2) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_reduce_splits<double>, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit,
-                     dst, mode, dst2, ld2);
+  hipLaunchKernelGGL(k_reduce_splits<double>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
+                     pstride, nsplit, dst, mode, dst2, ld2);
   return hipGetLastError();
 }
 
@@ -509,8 +518,8 @@ hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec 
                                              float* dst2, int64_t ld2) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_reduce_splits<float>, grid, dim3(256), 0, st, M, N, P, ldp, pstride, nsplit,
-                     dst, mode, dst2, ld2);
+  hipLaunchKernelGGL(k_reduce_splits<float>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
+                     pstride, nsplit, dst, mode, dst2, ld2);
   return hipGetLastError();
 }
 

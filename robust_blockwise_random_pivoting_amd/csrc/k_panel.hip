@@ -533,8 +533,8 @@ hipError_t rbrp_launch_upper_inv_blocks(hipStream_t st, int64_t k, const double*
                                         double* out) {
   if (k <= 0) return hipSuccess;
   const size_t lds = pw_lds_bytes();
-  hipError_t e = hipFuncSetAttribute((const void*)k_upper_inv_blocks,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+  hipError_t e = rbrp_allow_lds((const void*)k_upper_inv_blocks, (int)lds, lds_set);
   if (e != hipSuccess) return e;
   const unsigned nblk = (unsigned)(4 * ((k + 255) / 256));
   hipLaunchKernelGGL(k_upper_inv_blocks, dim3(nblk), dim3(64), lds, st, k, R, ldr, out);
@@ -547,8 +547,8 @@ hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg
                                     int* Jkpos) {
   if (b > PW_N) return hipErrorInvalidValue;
   const size_t lds = pw_lds_bytes();
-  hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_wave,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+  hipError_t e = rbrp_allow_lds((const void*)k_pivchol_wave, (int)lds, lds_set);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_pivchol_wave, dim3(1), dim3(64), lds, st, G, ldg, b, filter_mode, i, J, bs,
                      Rb, Rinv, Jk, Jin, Jout, Jkpos);
@@ -560,8 +560,8 @@ hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg,
                                   double* Rout, int64_t ldr) {
   if (b > PW_N) return hipErrorInvalidValue;
   const size_t lds = pw_lds_bytes();
-  hipError_t e = hipFuncSetAttribute((const void*)k_chol2_wave,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+  hipError_t e = rbrp_allow_lds((const void*)k_chol2_wave, (int)lds, lds_set);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_chol2_wave, dim3(1), dim3(64), lds, st, G2, ldg, b, bs, Rb, R2, R2inv, Rout,
                      ldr);

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(1024) void k_sum_to_state(const double* __restrict_
       bs->nswap = 0;
       bs->resid = t;
       bs->err = (t > 0.0) ? 1.0 : 0.0;
+      bs->arrive = 0u;
     }
   }
 }
@@ -237,8 +238,8 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
   const bool in_lds = (n - i) <= SAMPLE_LDS_N;
   const size_t lds = in_lds ? 2 * SAMPLE_LDS_N * sizeof(double) : 0;
   if (lds > 48 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_sample,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+    hipError_t e = rbrp_allow_lds((const void*)k_sample, (int)lds, lds_set);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), lds, st, diags, i, n, kmax, b, key0, key1, ctr[0],
@@ -545,8 +546,8 @@ hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t l
                                     Jkpos);
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_pivchol_filter,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+    hipError_t e = rbrp_allow_lds((const void*)k_pivchol_filter, (int)lds, lds_set);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_pivchol_filter, dim3(1), dim3(256), lds, st, G, ldg, b, filter_mode, i, J, bs,
@@ -600,8 +601,8 @@ hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int 
   if (b <= 64) return rbrp_launch_chol2_wave(st, G2, ldg, b, bs, Rb, R2, R2inv, Rout, ldr);
   const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
   if (lds > 48 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_chol2,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+    hipError_t e = rbrp_allow_lds((const void*)k_chol2, (int)lds, lds_set);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, st, G2, ldg, b, bs, R2, R2inv);
@@ -712,58 +713,13 @@ hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockS
 }
 
 // diags[i + bt + c] = sum_{t < ntiles} part[t * ldp + c], c < n - i - bt.
-// One workgroup per 64 columns; wave w sums tiles w, w+16, ... (lane = column,
-// so every load is a 512-byte run), then the 16 wave partials are added in
-// wave order.
-__global__ __launch_bounds__(1024) void k_colnorm_reduce(const double* __restrict__ part,
-                                                         int64_t ldp, int64_t ntiles, int64_t i,
-                                                         int64_t n, RbrpBlockState* bs,
-                                                         double* __restrict__ diags) {
-  __shared__ double red[16][64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int64_t c0 = i + bs->bt;
-  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
-  if (c0 + (int64_t)blockIdx.x * 64 >= n) return;  // uniform per workgroup
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  if (c0 + c < n) {
-    const double* p = part + c;
-    int64_t t = wid;
-    for (; t + 48 < ntiles; t += 64) {
-      a0 += p[t * ldp];
-      a1 += p[(t + 16) * ldp];
-      a2 += p[(t + 32) * ldp];
-      a3 += p[(t + 48) * ldp];
-    }
-    for (; t < ntiles; t += 16) a0 += p[t * ldp];
-  }
-  red[wid][lane] = (a0 + a1) + (a2 + a3);
-  __syncthreads();
-  if (wid == 0 && c0 + c < n) {
-    double acc = 0.0;
-    for (int w = 0; w < 16; ++w) acc += red[w][lane];
-    diags[c0 + c] = acc;
-  }
-}
-
-hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
-                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags) {
-  const int64_t cmax = n - i;
-  if (cmax <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 63) / 64)), dim3(1024), 0, st, part,
-                     ldp, ntiles, i, n, bs, diags);
-  return hipGetLastError();
-}
-
-// resid = sum(diags[i + bt : n]); err = resid / N0 (gpu/rbrp.cu:467).
-// Residual total after the block.  With `pub` (pinned, host-coherent memory)
-// the finished state is also published to the host: the fields are written
-// first, then `seq` with a system-scope release, so a host that reads seq ==
-// the expected number (with acquire) sees the whole state.
-__global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__ diags, int64_t i,
-                                                      int64_t n, RbrpBlockState* bs,
-                                                      RbrpBlockState* pub, int seq) {
-  __shared__ double red[16];
-  const int64_t c0 = i + bs->bt;
+// Residual total over diags[c0, n) by one 1024-thread workgroup (a fixed
+// summation order), stored in bs and, with `pub`, published to the host: the
+// fields are written first, then `seq` with a system-scope release, so a host
+// that reads seq == the expected number (with acquire) sees the whole state.
+__device__ void resid_total_publish(const double* __restrict__ diags, int64_t c0, int64_t n,
+                                    RbrpBlockState* bs, RbrpBlockState* pub, int seq,
+                                    double* red) {
   double acc = 0.0;
   for (int64_t j = c0 + threadIdx.x; j < n; j += 1024) acc += diags[j];
   const double t = rbrp_block_sum(acc, red);
@@ -784,6 +740,79 @@ __global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__
       __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
+}
+
+// Squared norms of 64 columns from the per-row-group partials: wave w sums
+// tiles w, w+16, ... (lane = column, so every load is a 512-byte run), then the
+// 16 wave partials are added in wave order.
+__device__ void colnorm_tile(const double* __restrict__ part, int64_t ldp, int64_t ntiles,
+                             int64_t c0, int64_t c, int64_t n, double* __restrict__ diags,
+                             double (*red)[64]) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (c0 + c < n) {
+    const double* p = part + c;
+    int64_t t = wid;
+    for (; t + 48 < ntiles; t += 64) {
+      a0 += p[t * ldp];
+      a1 += p[(t + 16) * ldp];
+      a2 += p[(t + 32) * ldp];
+      a3 += p[(t + 48) * ldp];
+    }
+    for (; t < ntiles; t += 16) a0 += p[t * ldp];
+  }
+  red[wid][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (wid == 0 && c0 + c < n) {
+    double acc = 0.0;
+    for (int w = 0; w < 16; ++w) acc += red[w][lane];
+    diags[c0 + c] = acc;
+  }
+  __syncthreads();  // red is reused by the caller
+}
+
+// One workgroup per 64 columns (colnorm_tile).  With `total`, the last
+// workgroup to finish (an arrival count in bs, reset by that workgroup) also
+// forms the residual total and publishes the state (resid_total_publish).
+__global__ __launch_bounds__(1024) void k_colnorm_reduce(const double* __restrict__ part,
+                                                         int64_t ldp, int64_t ntiles, int64_t i,
+                                                         int64_t n, RbrpBlockState* bs,
+                                                         double* __restrict__ diags, int total,
+                                                         RbrpBlockState* pub, int seq) {
+  __shared__ double red[16][64];
+  __shared__ int last;
+  const int64_t c0 = i + bs->bt;
+  const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  if (c0 + (int64_t)blockIdx.x * 64 < n) colnorm_tile(part, ldp, ntiles, c0, c, n, diags, red);
+  if (!total) return;
+  // arrival: this workgroup's diags are written before the count is raised
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&bs->arrive, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) bs->arrive = 0u;
+  resid_total_publish(diags, c0, n, bs, pub, seq, &red[0][0]);
+}
+
+hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags,
+                                      int total, RbrpBlockState* pub, int seq) {
+  const int64_t cmax = n - i;
+  if (cmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 63) / 64)), dim3(1024), 0, st, part,
+                     ldp, ntiles, i, n, bs, diags, total, pub, seq);
+  return hipGetLastError();
+}
+
+// resid = sum(diags[i + bt : n]); err = resid / N0 (gpu/rbrp.cu:467).
+// Residual total after the block (see resid_total_publish).
+__global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__ diags, int64_t i,
+                                                      int64_t n, RbrpBlockState* bs,
+                                                      RbrpBlockState* pub, int seq) {
+  __shared__ double red[16];
+  resid_total_publish(diags, i + bs->bt, n, bs, pub, seq, red);
 }
 
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,

@@ -148,8 +148,9 @@ void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pc
   *kchunk = kc;
 }
 
-// Per-phase event timing: events are recorded around launches during a block
-// and read after the block's synchronisation.
+// Per-phase event timing: events are recorded around launches (or handed to a
+// launch that times its own dispatch) and read when the ID ends, so the blocks
+// run without event waits in between.
 struct PhaseTimer {
   rbrp_handle_t h;
   struct Rec {
@@ -515,6 +516,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   int64_t i = 0, nb = 0;
   double err = (N0 > 0.0) ? 1.0 : 0.0;
   while (i < kmax && N0 > 0.0) {
+    const size_t rec_mark = pt.recs.size();
     // ---- sample s columns of [i, n) ------------------------------------
     ph = pt.begin(RBRP_PHASE_SAMPLE);
     HIPCHK(rbrp_launch_sample(st, ws.diags, i, n, kmax, b, key0, key1, ctr, ws.w, ws.cdf, ws.st, ws.J));
@@ -612,6 +614,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     // ---- trailing projection and update ----------------------------------
     const int64_t nrem_max = n - i - 1;
     int ph_proj = -1, ph_upd = -1;
+    int64_t groups = 0;  // norm partials written by the update (one per row group)
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
@@ -640,7 +643,6 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                          mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
       ph_upd = pt.begin_kernel(RBRP_PHASE_UPDATE, &ev0, &ev1);
-      int64_t groups = 0;  // norm partials written by the update (one per row group)
       if (mixed)
         HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                         b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
@@ -650,15 +652,17 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                       b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
                                       ws.normpart, nrem_max, &groups, ev0, ev1));
       pt.end(ph_upd, 1);
-      ph = pt.begin(RBRP_PHASE_RESID);
-      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags));
-    } else {
-      ph = pt.begin(RBRP_PHASE_RESID);
     }
+    // column norms, residual total and the state's publication to the host
     h->pub_seq = h->pub_seq == INT_MAX ? 1 : h->pub_seq + 1;
     const int seq = h->pub_seq;
-    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
-    pt.end(ph, nrem_max > 0 ? 2 : 1);
+    ph = pt.begin(RBRP_PHASE_RESID);
+    if (nrem_max > 0)
+      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags, 1,
+                                        h->host_state, seq));
+    else
+      HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
+    pt.end(ph, 1);
 
     HIPCHK(wait_published(h, st, ws.st, seq));
     const RbrpBlockState hs = *h->host_state;
@@ -667,8 +671,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       return RBRP_ERR_NUMERIC;
     }
     if (hs.s <= 0 || hs.bt <= 0) {
-      pt.recs.clear();
-      pt.next = 0;
+      pt.recs.resize(rec_mark);  // the empty block's records
       break;
     }
     const int64_t bt = hs.bt, s = hs.s;
@@ -679,7 +682,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       pt.set_work(ph_proj, 2.0 * m * bt * nrem, ew * (m * nrem + m * bt) + 8.0 * bt * nrem);
       pt.set_work(ph_upd, 2.0 * m * bt * nrem, ew * (2.0 * m * nrem + m * bt + bt * nrem));
     }
-    pt.flush();
+    // the records stay until the ID ends (no event waits between blocks)
     res->draws = hs.draws;
     err = hs.err;
     i += bt;

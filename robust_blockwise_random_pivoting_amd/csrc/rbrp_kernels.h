@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "rbrp_common.h"
 
 // Kernel launch that, given start / stop events (either may be null), has the
@@ -19,6 +21,21 @@
     else                                                                          \
       hipLaunchKernelGGL(kern, grid, block, shm, st, __VA_ARGS__);                \
   } while (0)
+
+// hipFuncSetAttribute(fn, MaxDynamicSharedMemorySize, bytes), called only when
+// the largest size set so far for this function on the current device is
+// smaller.  `cache` is the call site's zero-initialised static array (one slot
+// per device); concurrent callers at worst set the same value twice.
+#define RBRP_MAX_DEVICES 64
+static inline hipError_t rbrp_allow_lds(const void* fn, int bytes, std::atomic<int>* cache) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= RBRP_MAX_DEVICES)
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (cache[dev].load(std::memory_order_relaxed) >= bytes) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e == hipSuccess) cache[dev].store(bytes, std::memory_order_relaxed);
+  return e;
+}
 
 // A size that is either static (`base`) or `base + sign * (*dyn)` read on the
 // device, so one launch shape serves every block of the loop.
@@ -168,8 +185,12 @@ hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64
                                       const double* tmpk, const double* tmpo, int64_t ldt);
 hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
                                  int64_t ld, const int* Jin, const int* Jout);
+// diags[c] for the remaining columns from the update's norm partials; with
+// total != 0 the last workgroup also forms the residual total and publishes the
+// state as rbrp_launch_resid_total does.
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
-                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags,
+                                      int total = 0, RbrpBlockState* pub = nullptr, int seq = 0);
 // bs->resid / err after the block; with pub (pinned host-coherent memory) the
 // state is also published there, seq written last with a system-scope release.
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
