@@ -293,21 +293,28 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
 // Tall NN with a long reduction (the re-orthogonalisation V -= Q C):
 // C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for r < M, c < N, any K.
 //
-// Workgroup tile: 64 rows x 64 columns (an XCD-aware task); wave w owns rows
-// 16w .. 16w+15 against the four 16-column blocks.  Each 16-deep step of
-// both operands is staged through double-buffered LDS with two steps of
-// global loads in flight: A as [row][k] (16 k per row), B as [column][k], the
-// eight 16-byte units of a row / column XOR-swizzled with its index mod 8 (the
-// eight lanes of each 128-byte LDS pass touch distinct bank groups).  Lane
-// (j, g) reads k = 4g .. 4g+3 of its row / column as two ds_read_b128 and
-// MFMA q of the step consumes element q.  The MFMA computes
-// the transposed block (B's columns as its rows): lane l holds
-// C(r0 + 16w + l%16, c0 + 16cb + 4v + l/16), so every C load / store
-// instruction moves four 128-byte column runs.  Reduction indices >= K read
-// as zero (clamped loads, then a bit mask on B).  A is a plain or offset
-// MatRef (no column list).
+// The rows are cut into 16-row blocks and the blocks into gr contiguous
+// ranges of R <= 4 blocks (R differs by at most one between ranges); a
+// workgroup takes one range against one 64-column tile (an XCD-aware task, so
+// the column tiles of a range share its rows of A through one L2).  Wave w
+// owns the 16-column block w of the tile against the R row blocks, so a
+// range of R blocks puts R units of MFMA work on each SIMD: with gr sized to
+// the resident workgroups, every SIMD carries about the same number of row
+// blocks whatever M is (64-row tiles would leave some compute units one tile
+// more than the rest).  Each 16-deep step of both operands is staged through
+// double-buffered LDS with two steps of global loads in flight: A as
+// [row][k] (16 k per row), B as [column][k], the eight 16-byte units of a row
+// / column XOR-swizzled with its index mod 8 (the eight lanes of each 128-byte
+// LDS pass touch distinct bank groups).  Lane (j, g) reads k = 4g .. 4g+3 of
+// its row / column as two ds_read_b128 and MFMA q of the step consumes element
+// q.  The MFMA computes the transposed block (B's columns as its rows): lane l
+// holds C(r0 + 16 rb + l%16, c0 + 16w + 4v + l/16), so every C load / store
+// instruction moves four 128-byte column runs.  Reduction indices >= K read as
+// zero (clamped loads, then a bit mask on B).  A is a plain or offset MatRef
+// (no column list).
 // ---------------------------------------------------------------------------
 #define NN64_KS 16
+#define NN64_RB 4  // row blocks per workgroup at most
 #define NN64_AT(idx, u) ((idx) * NN64_KS + 2 * ((u) ^ ((idx) & 7)))
 // x with its bits and-ed with the mask m (0 or all ones): a loaded value kept
 // or replaced by +0 without a branch, so that every load stays unconditional
@@ -315,41 +322,32 @@ __device__ __forceinline__ double nn64_mask(double x, long long m) {
   return __longlong_as_double(__double_as_longlong(x) & m);
 }
 
-__global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, int64_t K, MatRef A,
-                                                      MatRef B, MatRef C, int mode, int gx,
-                                                      int gy) {
-  // task = column tile + gx * row tile, XCD-aware (the column tiles of one
-  // row tile share its rows of A through one L2)
-  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy);
-  if (task >= (int64_t)gx * gy) return;
-  const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t r0 = (task / gx) * 64, c0 = (task % gx) * 64;
-  if (r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
-
-  __shared__ double As[2][64 * NN64_KS];  // [buf][row][k], swizzled
-  __shared__ double Bs[2][64 * NN64_KS];  // [buf][col][k], swizzled
-
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+// The K loop of one workgroup for R row blocks (R a template argument so that
+// a range of R blocks issues exactly R blocks of MFMAs).
+template <int R>
+__device__ __forceinline__ void nn64_loop(int64_t K, const double* __restrict__ abase,
+                                          uint32_t aoff, int64_t lda,
+                                          const double* __restrict__ bcol, double* As0,
+                                          double* Bs0, v4d* acc) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
   const int lj = lane & 15, lg = lane >> 4;
-  // A stager: row ar of the tile, k = ak .. ak+3 of the step (each of the four
-  // loads of a wave reads 64 consecutive rows of one column; the two LDS
-  // stores are 16-byte units of that row).  Rows >= M re-read row M-1: they
-  // only feed output rows that are never stored.
-  const int ar = tid & 63, ak = (tid >> 6) * 4;
-  const int64_t arow = min(r0 + ar, M - 1);
-  // B stager: column bc, k = bk .. bk+3 (contiguous in B's column)
-  const int bc = tid >> 2, bk = (tid & 3) * 4;
-  const double* bcol = mat_col(B, min(c0 + bc, N - 1));
-  const double* a0p = mat_col(A, 0) + arow;  // A has no column list: column k at a0p + k * ld
+  // A stager: row ar, k = ak .. ak+3 (ak is uniform per wave, so each A load is
+  // a scalar column address plus the lane's 32-bit row offset aoff)
+  const int ar = tid & 63, ak = wid * 4;
+  const int bc = tid >> 2, bk = (tid & 3) * 4;   // B stager: column bc, k = bk .. bk+3
+  double* As[2] = {As0, As0 + 64 * NN64_KS};
+  double* Bs[2] = {Bs0, Bs0 + 64 * NN64_KS};
 
 // (the macro's locals carry an nl_ prefix: a local named like a variable of
 // the argument expression would be initialised from itself)
 #define NN64_LOAD(S, kk)                                                                 \
   do {                                                                                   \
     const int64_t nl_ka = (kk) + ak;                                                     \
-    S##a0 = make_double2(a0p[min(nl_ka, K - 1) * A.ld], a0p[min(nl_ka + 1, K - 1) * A.ld]);\
-    S##a1 = make_double2(a0p[min(nl_ka + 2, K - 1) * A.ld],                              \
-                         a0p[min(nl_ka + 3, K - 1) * A.ld]);                             \
+    S##a0 = make_double2((abase + min(nl_ka, K - 1) * lda)[aoff],                        \
+                         (abase + min(nl_ka + 1, K - 1) * lda)[aoff]);                   \
+    S##a1 = make_double2((abase + min(nl_ka + 2, K - 1) * lda)[aoff],                    \
+                         (abase + min(nl_ka + 3, K - 1) * lda)[aoff]);                   \
     const int64_t nl_kb = (kk) + bk;                                                     \
     S##b0 = make_double2(bcol[min(nl_kb, K - 1)], bcol[min(nl_kb + 1, K - 1)]);          \
     S##b1 = make_double2(bcol[min(nl_kb + 2, K - 1)], bcol[min(nl_kb + 3, K - 1)]);      \
@@ -371,23 +369,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
   } while (0)
 #define NN64_STEP(bb)                                                                      \
   do {                                                                                     \
-    const double* as = &As[bb][0];                                                         \
-    const double* bs = &Bs[bb][0];                                                         \
-    const double2 y0 = *(const double2*)&as[NN64_AT(16 * wid + lj, 2 * lg)];               \
-    const double2 y1 = *(const double2*)&as[NN64_AT(16 * wid + lj, 2 * lg + 1)];           \
-    _Pragma("unroll") for (int cb = 0; cb < 4; ++cb) {                                     \
-      const double2 x0 = *(const double2*)&bs[NN64_AT(16 * cb + lj, 2 * lg)];              \
-      const double2 x1 = *(const double2*)&bs[NN64_AT(16 * cb + lj, 2 * lg + 1)];          \
-      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.x, y0.x, acc[cb], 0, 0, 0);       \
-      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.y, y0.y, acc[cb], 0, 0, 0);       \
-      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, y1.x, acc[cb], 0, 0, 0);       \
-      acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, y1.y, acc[cb], 0, 0, 0);       \
+    const double* as = As[bb];                                                             \
+    const double* bs = Bs[bb];                                                             \
+    const double2 x0 = *(const double2*)&bs[NN64_AT(16 * wid + lj, 2 * lg)];               \
+    const double2 x1 = *(const double2*)&bs[NN64_AT(16 * wid + lj, 2 * lg + 1)];           \
+    _Pragma("unroll") for (int rb = 0; rb < R; ++rb) {                                     \
+      const double2 y0 = *(const double2*)&as[NN64_AT(16 * rb + lj, 2 * lg)];             \
+      const double2 y1 = *(const double2*)&as[NN64_AT(16 * rb + lj, 2 * lg + 1)];         \
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.x, y0.x, acc[rb], 0, 0, 0);        \
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0.y, y0.y, acc[rb], 0, 0, 0);        \
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.x, y1.x, acc[rb], 0, 0, 0);        \
+      acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1.y, y1.y, acc[rb], 0, 0, 0);        \
     }                                                                                      \
   } while (0)
-
-  v4d acc[4];
-#pragma unroll
-  for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
 
   // two steps of loads in flight; the loads are unconditional (beyond K they
   // read clamped addresses), so the vector-memory waits of the stores count
@@ -422,29 +416,81 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
 #undef NN64_LOAD
 #undef NN64_STORE
 #undef NN64_STEP
-  const int64_t r = r0 + 16 * wid + lj;
-  if (r < M) {
+}
+
+__global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, int64_t K, MatRef A,
+                                                      MatRef B, MatRef C, int mode, int gx,
+                                                      int gr, int64_t rbt) {
+  // task = column tile + gx * row range, XCD-aware
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gr);
+  if (task >= (int64_t)gx * gr) return;
+  const int64_t g = task / gx;
+  const int64_t rb0 = g * rbt / gr, rb1 = (g + 1) * rbt / gr;
+  const int R = (int)(rb1 - rb0);
+  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t r0 = rb0 * 16, c0 = (task % gx) * 64;
+  if (R <= 0 || r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
+
+  __shared__ double As[2 * 64 * NN64_KS];  // [buf][row][k], swizzled
+  __shared__ double Bs[2 * 64 * NN64_KS];  // [buf][col][k], swizzled
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lj = lane & 15, lg = lane >> 4;
+  // A stager row: rows past the range re-read its last row (the loads stay
+  // unconditional and hit the cache), rows >= M re-read row M-1; neither feeds
+  // a stored output
+  const int64_t arow = min(r0 + min(tid & 63, 16 * R - 1), M - 1);
+  // A has no column list: element (row, k) at abase + k * ld + row
+  const double* abase = mat_col(A, 0) + r0;
+  const uint32_t aoff = (uint32_t)(arow - r0);
+  const double* bcol = mat_col(B, min(c0 + (tid >> 2), N - 1));
+
+  v4d acc[NN64_RB];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+  for (int rb = 0; rb < NN64_RB; ++rb) acc[rb] = (v4d){0.0, 0.0, 0.0, 0.0};
+  switch (R) {
+    case 1: nn64_loop<1>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
+    case 2: nn64_loop<2>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
+    case 3: nn64_loop<3>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
+    default: nn64_loop<4>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
+  }
+  const int64_t c = c0 + 16 * wid + lg;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t c = c0 + 16 * cb + 4 * v + lg;
-        if (c < N) {
-          double* d = mat_col(C, c) + r;
-          const double x = acc[cb][v];
+  for (int rb = 0; rb < NN64_RB; ++rb) {
+    const int64_t r = r0 + 16 * rb + lj;
+    if (rb < R && r < M) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (c + 4 * v < N) {
+          double* d = mat_col(C, c + 4 * v) + r;
+          const double x = acc[rb][v];
           *d = (mode == GEMM_SUB) ? *d - x : ((mode == GEMM_ADD) ? *d + x : x);
         }
-      }
+    }
   }
 }
 
+int rbrp_gemm_nn64_resident() {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_nn64, 256, 0) != hipSuccess)
+    return 0;
+  return cus * per;
+}
+
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
-                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode) {
+                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode, int resident) {
   if (Mmax <= 0 || Nmax <= 0 || K <= 0) return hipSuccess;
-  if (A.cols != nullptr || A.ld % 2 != 0 || Mmax % 4 != 0) return hipErrorInvalidValue;
-  const int64_t gx = (Nmax + 63) / 64, gy = (Mmax + 63) / 64;
-  dim3 grid((unsigned)rbrp_xcd_grid(gx * gy));
-  hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode, (int)gx, (int)gy);
+  if (A.cols != nullptr) return hipErrorInvalidValue;
+  const int64_t gx = (Nmax + 63) / 64, rbt = (Mmax + 15) / 16;
+  // as many row ranges as resident workgroups per column tile, each of at
+  // most NN64_RB row blocks
+  const int64_t per = std::max<int64_t>(1, (resident > 0 ? resident : 1024) / gx);
+  const int64_t gr = std::max<int64_t>((rbt + NN64_RB - 1) / NN64_RB, std::min<int64_t>(rbt, per));
+  dim3 grid((unsigned)rbrp_xcd_grid(gx * gr));
+  hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode, (int)gx, (int)gr,
+                     rbt);
   return hipGetLastError();
 }
 

@@ -356,6 +356,7 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
   }
   h->tn64_resident = rbrp_gemm_tn64_resident();
   h->tn32_resident = rbrp_gemm_tn32_resident();
+  h->nn64_resident = rbrp_gemm_nn64_resident();
   *handle = h;
   return 0;
 }
@@ -544,7 +545,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                        GEMM_SET, ws.Cbuf, ldr));
       HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, i, Qall,
-                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB));
+                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
       if (mixed) {
         // the corrected columns replace the stored residual (the filtered-out

@@ -100,11 +100,15 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
 int rbrp_gemm_tn64_resident();
 int rbrp_gemm_tn32_resident();
 
-// C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) on 64 x 64 output tiles with
-// double-buffered K steps (the re-orthogonalisation V -= Q C); any K, M a
-// multiple of 4 (static), A's leading dimension even.
+// C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for a tall C with 64-column tiles
+// and row ranges of at most 64 rows balanced over `resident` workgroups (the
+// kernel's whole-device residency, rbrp_gemm_nn64_resident; 0: about 1024),
+// double-buffered K steps (the re-orthogonalisation V -= Q C); any K; A is a
+// plain or offset MatRef.
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
-                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode);
+                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode,
+                                 int resident = 0);
+int rbrp_gemm_nn64_resident();
 
 // dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order (skipped when dst.p is
 // null); when dst2 != nullptr the sum is also stored row-major at
