@@ -418,7 +418,7 @@ __device__ __forceinline__ void nn64_loop(int64_t K, const double* __restrict__ 
 #undef NN64_STEP
 }
 
-__global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, int64_t K, MatRef A,
+__global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, DimSpec dK, MatRef A,
                                                       MatRef B, MatRef C, int mode, int gx,
                                                       int gr, int64_t rbt) {
   // task = column tile + gx * row range, XCD-aware
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, in
   const int64_t g = task / gx;
   const int64_t rb0 = g * rbt / gr, rb1 = (g + 1) * rbt / gr;
   const int R = (int)(rb1 - rb0);
-  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t M = dim_eval(dM), N = dim_eval(dN), K = dim_eval(dK);
   const int64_t r0 = rb0 * 16, c0 = (task % gx) * 64;
   if (R <= 0 || r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
 
@@ -480,8 +480,8 @@ int rbrp_gemm_nn64_resident() {
 }
 
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
-                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode, int resident) {
-  if (Mmax <= 0 || Nmax <= 0 || K <= 0) return hipSuccess;
+                                 DimSpec K, MatRef A, MatRef B, MatRef C, int mode, int resident) {
+  if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   if (A.cols != nullptr) return hipErrorInvalidValue;
   const int64_t gx = (Nmax + 63) / 64, rbt = (Mmax + 15) / 16;
   // as many row ranges as resident workgroups per column tile, each of at

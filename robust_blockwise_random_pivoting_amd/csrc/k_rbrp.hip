@@ -60,7 +60,6 @@ __global__ __launch_bounds__(1024) void k_sum_to_state(const double* __restrict_
       bs->nswap = 0;
       bs->resid = t;
       bs->err = (t > 0.0) ? 1.0 : 0.0;
-      bs->arrive = 0u;
     }
   }
 }
@@ -771,38 +770,23 @@ __device__ void colnorm_tile(const double* __restrict__ part, int64_t ldp, int64
   __syncthreads();  // red is reused by the caller
 }
 
-// One workgroup per 64 columns (colnorm_tile).  With `total`, the last
-// workgroup to finish (an arrival count in bs, reset by that workgroup) also
-// forms the residual total and publishes the state (resid_total_publish).
+// One workgroup per 64 columns (colnorm_tile).
 __global__ __launch_bounds__(1024) void k_colnorm_reduce(const double* __restrict__ part,
                                                          int64_t ldp, int64_t ntiles, int64_t i,
-                                                         int64_t n, RbrpBlockState* bs,
-                                                         double* __restrict__ diags, int total,
-                                                         RbrpBlockState* pub, int seq) {
+                                                         int64_t n, const RbrpBlockState* bs,
+                                                         double* __restrict__ diags) {
   __shared__ double red[16][64];
-  __shared__ int last;
   const int64_t c0 = i + bs->bt;
   const int64_t c = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
   if (c0 + (int64_t)blockIdx.x * 64 < n) colnorm_tile(part, ldp, ntiles, c0, c, n, diags, red);
-  if (!total) return;
-  // arrival: this workgroup's diags are written before the count is raised
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(&bs->arrive, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x == 0) bs->arrive = 0u;
-  resid_total_publish(diags, c0, n, bs, pub, seq, &red[0][0]);
 }
 
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
-                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags,
-                                      int total, RbrpBlockState* pub, int seq) {
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags) {
   const int64_t cmax = n - i;
   if (cmax <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_colnorm_reduce, dim3((unsigned)((cmax + 63) / 64)), dim3(1024), 0, st, part,
-                     ldp, ntiles, i, n, bs, diags, total, pub, seq);
+                     ldp, ntiles, i, n, bs, diags);
   return hipGetLastError();
 }
 

@@ -143,6 +143,4 @@ struct RbrpBlockState {
   double N0;        // ||A||_F^2
   int nnz;          // nonzero weights seen by the sampler
   int seq;          // publication number of the host copy (k_resid_total)
-  unsigned arrive;  // workgroups of the column-norm reduction done (the last resets it)
-  unsigned pad;
 };

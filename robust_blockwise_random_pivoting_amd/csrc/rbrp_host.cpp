@@ -544,8 +544,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                        i * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
                                        GEMM_SET, ws.Cbuf, ldr));
       HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, i, Qall,
-                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
+      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i),
+                                   Qall, mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
       if (mixed) {
         // the corrected columns replace the stored residual (the filtered-out
@@ -602,10 +602,10 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     else
       HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
     // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
-    HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
-                               mat(ws.Qbuf, mp), mat(ws.R2inv, b),
-                               mixed ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
-                               nullptr, 0));
+    HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
+                                 mat(ws.Qbuf, mp), mat(ws.R2inv, b),
+                                 mixed ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
+                                 h->nn64_resident));
     if (mixed) {
       HIPCHK(rbrp_launch_store_q32(st, mp, b, i, ws.st, ws.Q64, mp, ws.Aw32, mp));
       ++perm_launches;
@@ -659,11 +659,9 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     const int seq = h->pub_seq;
     ph = pt.begin(RBRP_PHASE_RESID);
     if (nrem_max > 0)
-      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags, 1,
-                                        h->host_state, seq));
-    else
-      HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
-    pt.end(ph, 1);
+      HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags));
+    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
+    pt.end(ph, nrem_max > 0 ? 2 : 1);
 
     HIPCHK(wait_published(h, st, ws.st, seq));
     const RbrpBlockState hs = *h->host_state;

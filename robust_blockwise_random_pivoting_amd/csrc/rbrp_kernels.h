@@ -103,10 +103,11 @@ int rbrp_gemm_tn32_resident();
 // C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for a tall C with 64-column tiles
 // and row ranges of at most 64 rows balanced over `resident` workgroups (the
 // kernel's whole-device residency, rbrp_gemm_nn64_resident; 0: about 1024),
-// double-buffered K steps (the re-orthogonalisation V -= Q C); any K; A is a
-// plain or offset MatRef.
+// double-buffered K steps (the re-orthogonalisation V -= Q C, the new block
+// Q_b = Q1 R2^{-1}); any K (a device size allowed); A is a plain or offset
+// MatRef.
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
-                                 int64_t K, MatRef A, MatRef B, MatRef C, int mode,
+                                 DimSpec K, MatRef A, MatRef B, MatRef C, int mode,
                                  int resident = 0);
 int rbrp_gemm_nn64_resident();
 
@@ -189,12 +190,9 @@ hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64
                                       const double* tmpk, const double* tmpo, int64_t ldt);
 hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
                                  int64_t ld, const int* Jin, const int* Jout);
-// diags[c] for the remaining columns from the update's norm partials; with
-// total != 0 the last workgroup also forms the residual total and publishes the
-// state as rbrp_launch_resid_total does.
+// diags[c] for the remaining columns from the update's norm partials.
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
-                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags,
-                                      int total = 0, RbrpBlockState* pub = nullptr, int seq = 0);
+                                      int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
 // bs->resid / err after the block; with pub (pinned host-coherent memory) the
 // state is also published there, seq written last with a system-scope release.
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,

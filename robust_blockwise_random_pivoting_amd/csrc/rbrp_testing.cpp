@@ -100,8 +100,8 @@ int rbrp_test_gemm_nn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
   if (mode < GEMM_SET || mode > GEMM_ADD) return -11;
   if (M == 0 || N == 0 || K == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(M), M, dim_static(N), N, K, mat((double*)A, lda),
-                               mat((double*)B, ldb), mat(C, ldc), mode));
+  HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(M), M, dim_static(N), N, dim_static(K),
+                               mat((double*)A, lda), mat((double*)B, ldb), mat(C, ldc), mode));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
