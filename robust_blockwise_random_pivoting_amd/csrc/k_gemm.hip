@@ -310,8 +310,8 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
 // q.  The MFMA computes the transposed block (B's columns as its rows): lane l
 // holds C(r0 + 16 rb + l%16, c0 + 16w + 4v + l/16), so every C load / store
 // instruction moves four 128-byte column runs.  Reduction indices >= K read as
-// zero (clamped loads, then a bit mask on B).  A is a plain or offset MatRef
-// (no column list).
+// zero (clamped loads, then a bit mask on B).  A may be a plain, offset or
+// index-listed MatRef (the column address of an A load is uniform per wave).
 // ---------------------------------------------------------------------------
 #define NN64_KS 16
 #define NN64_RB 4  // row blocks per workgroup at most
@@ -325,15 +325,15 @@ __device__ __forceinline__ double nn64_mask(double x, long long m) {
 // The K loop of one workgroup for R row blocks (R a template argument so that
 // a range of R blocks issues exactly R blocks of MFMAs).
 template <int R>
-__device__ __forceinline__ void nn64_loop(int64_t K, const double* __restrict__ abase,
-                                          uint32_t aoff, int64_t lda,
+__device__ __forceinline__ void nn64_loop(int64_t K, const MatRef& A, uint32_t aoff,
                                           const double* __restrict__ bcol, double* As0,
                                           double* Bs0, v4d* acc) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform per wave
   const int lj = lane & 15, lg = lane >> 4;
   // A stager: row ar, k = ak .. ak+3 (ak is uniform per wave, so each A load is
-  // a scalar column address plus the lane's 32-bit row offset aoff)
+  // a scalar column address plus the lane's 32-bit row offset aoff from the
+  // workgroup's first row)
   const int ar = tid & 63, ak = wid * 4;
   const int bc = tid >> 2, bk = (tid & 3) * 4;   // B stager: column bc, k = bk .. bk+3
   double* As[2] = {As0, As0 + 64 * NN64_KS};
@@ -344,10 +344,10 @@ __device__ __forceinline__ void nn64_loop(int64_t K, const double* __restrict__ 
 #define NN64_LOAD(S, kk)                                                                 \
   do {                                                                                   \
     const int64_t nl_ka = (kk) + ak;                                                     \
-    S##a0 = make_double2((abase + min(nl_ka, K - 1) * lda)[aoff],                        \
-                         (abase + min(nl_ka + 1, K - 1) * lda)[aoff]);                   \
-    S##a1 = make_double2((abase + min(nl_ka + 2, K - 1) * lda)[aoff],                    \
-                         (abase + min(nl_ka + 3, K - 1) * lda)[aoff]);                   \
+    S##a0 = make_double2(mat_col(A, min(nl_ka, K - 1))[aoff],                            \
+                         mat_col(A, min(nl_ka + 1, K - 1))[aoff]);                       \
+    S##a1 = make_double2(mat_col(A, min(nl_ka + 2, K - 1))[aoff],                        \
+                         mat_col(A, min(nl_ka + 3, K - 1))[aoff]);                       \
     const int64_t nl_kb = (kk) + bk;                                                     \
     S##b0 = make_double2(bcol[min(nl_kb, K - 1)], bcol[min(nl_kb + 1, K - 1)]);          \
     S##b1 = make_double2(bcol[min(nl_kb + 2, K - 1)], bcol[min(nl_kb + 3, K - 1)]);      \
@@ -440,8 +440,8 @@ __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, Di
   // unconditional and hit the cache), rows >= M re-read row M-1; neither feeds
   // a stored output
   const int64_t arow = min(r0 + min(tid & 63, 16 * R - 1), M - 1);
-  // A has no column list: element (row, k) at abase + k * ld + row
-  const double* abase = mat_col(A, 0) + r0;
+  MatRef Ar = A;  // A's rows from the workgroup's first row on
+  Ar.p += r0;
   const uint32_t aoff = (uint32_t)(arow - r0);
   const double* bcol = mat_col(B, min(c0 + (tid >> 2), N - 1));
 
@@ -449,10 +449,10 @@ __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, Di
 #pragma unroll
   for (int rb = 0; rb < NN64_RB; ++rb) acc[rb] = (v4d){0.0, 0.0, 0.0, 0.0};
   switch (R) {
-    case 1: nn64_loop<1>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
-    case 2: nn64_loop<2>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
-    case 3: nn64_loop<3>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
-    default: nn64_loop<4>(K, abase, aoff, A.ld, bcol, As, Bs, acc); break;
+    case 1: nn64_loop<1>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    case 2: nn64_loop<2>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    case 3: nn64_loop<3>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    default: nn64_loop<4>(K, Ar, aoff, bcol, As, Bs, acc); break;
   }
   const int64_t c = c0 + 16 * wid + lg;
 #pragma unroll
@@ -482,7 +482,6 @@ int rbrp_gemm_nn64_resident() {
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
                                  DimSpec K, MatRef A, MatRef B, MatRef C, int mode, int resident) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
-  if (A.cols != nullptr) return hipErrorInvalidValue;
   const int64_t gx = (Nmax + 63) / 64, rbt = (Mmax + 15) / 16;
   // as many row ranges as resident workgroups per column tile, each of at
   // most NN64_RB row blocks

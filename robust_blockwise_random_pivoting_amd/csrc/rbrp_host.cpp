@@ -577,8 +577,9 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
                                         ws.Jk, ws.Jin, ws.Jout, ws.Jkpos));
       // Q1 = V(:, Jk) Rb^{-1}
-      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
-                                 Vkept, mat(ws.Rinv, b), mat(ws.Qbuf, mp), GEMM_SET, nullptr, 0));
+      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b,
+                                   dim_dyn(0, d_bt, 1), Vkept, mat(ws.Rinv, b), mat(ws.Qbuf, mp),
+                                   GEMM_SET, h->nn64_resident));
       HIPCHK(gram(dim_dyn(0, d_bt, 1), mat(ws.Qbuf, mp)));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G2, b), GEMM_SET));

@@ -104,8 +104,8 @@ int rbrp_gemm_tn32_resident();
 // and row ranges of at most 64 rows balanced over `resident` workgroups (the
 // kernel's whole-device residency, rbrp_gemm_nn64_resident; 0: about 1024),
 // double-buffered K steps (the re-orthogonalisation V -= Q C, the new block
-// Q_b = Q1 R2^{-1}); any K (a device size allowed); A is a plain or offset
-// MatRef.
+// Q_b = Q1 R2^{-1} and Q1 = V(:, Jk) Rb^{-1}); any K (a device size allowed);
+// any MatRef for A.
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
                                  DimSpec K, MatRef A, MatRef B, MatRef C, int mode,
                                  int resident = 0);
