@@ -117,10 +117,18 @@ __device__ inline double block_exclusive_scan(double v, double* wsum) {
 // otherwise in the global scratch arrays w / cdf.
 #define SAMPLE_LDS_N 8192
 
-__global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diags, int64_t i,
-                                                 int64_t n, int64_t kmax, int b, uint64_t key0,
-                                                 uint64_t key1, uint64_t c0, uint64_t c1,
-                                                 uint64_t c2, uint64_t c3, double* __restrict__ gw,
+// The block's first column is i0 + *i_dyn (i_dyn optional).  With check_prev
+// the state still holds the previous block's results, and the call is a
+// no-op (s = 0, so the following kernels of the block do nothing) when that
+// block ended the loop: a numerical failure, an empty block, the squared
+// error at or below tol2, or the rank cap reached.  in_lds says whether the
+// launch reserved the LDS arrays.
+__global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diags, int64_t i0,
+                                                 const int* i_dyn, int check_prev, double tol2,
+                                                 int in_lds_ok, int64_t n, int64_t kmax, int b,
+                                                 uint64_t key0, uint64_t key1, uint64_t c0,
+                                                 uint64_t c1, uint64_t c2, uint64_t c3,
+                                                 double* __restrict__ gw,
                                                  double* __restrict__ gcdf, RbrpBlockState* bs,
                                                  int* __restrict__ J) {
   extern __shared__ double lds[];
@@ -130,9 +138,20 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
   __shared__ int ired[16];
   __shared__ double wsum[16];
   __shared__ int s_nfound;
+  __shared__ int s_done;
   const int tid = threadIdx.x;
+  const int64_t i = i0 + (i_dyn ? (int64_t)*i_dyn : 0);
+  if (check_prev) {
+    if (tid == 0)
+      s_done = bs->status != 0 || bs->s <= 0 || bs->bt <= 0 || bs->err <= tol2 || i >= kmax;
+    __syncthreads();
+    if (s_done) {
+      if (tid == 0) bs->s = 0;
+      return;
+    }
+  }
   const int64_t N = n - i;
-  const bool in_lds = N <= SAMPLE_LDS_N;
+  const bool in_lds = in_lds_ok && N <= SAMPLE_LDS_N;
   double* w = in_lds ? lds : gw;
   double* cdf = in_lds ? lds + SAMPLE_LDS_N : gcdf;
   int cnt = 0;
@@ -232,36 +251,39 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
 hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
                               int64_t kmax, int b, uint64_t key0, uint64_t key1,
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
-                              int* J) {
+                              int* J, const int* i_dyn, double tol2) {
   if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
-  const bool in_lds = (n - i) <= SAMPLE_LDS_N;
+  // with i_dyn the first column is at least i + 1 (the previous block kept a
+  // column, or the call does nothing)
+  const bool in_lds = (n - i - (i_dyn ? 1 : 0)) <= SAMPLE_LDS_N;
   const size_t lds = in_lds ? 2 * SAMPLE_LDS_N * sizeof(double) : 0;
   if (lds > 48 * 1024) {
     static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
     hipError_t e = rbrp_allow_lds((const void*)k_sample, (int)lds, lds_set);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), lds, st, diags, i, n, kmax, b, key0, key1, ctr[0],
-                     ctr[1], ctr[2], ctr[3], w, cdf, bs, J);
+  hipLaunchKernelGGL(k_sample, dim3(1), dim3(1024), lds, st, diags, i, i_dyn, i_dyn ? 1 : 0, tol2,
+                     in_lds ? 1 : 0, n, kmax, b, key0, key1, ctr[0], ctr[1], ctr[2], ctr[3], w, cdf,
+                     bs, J);
   return hipGetLastError();
 }
 
 // dst(r, cols[c]) += src(r, c), r < rows, c < *ncols_dyn.
-__global__ void k_add_cols(int64_t rows, const int* ncols_dyn, const double* __restrict__ src,
+__global__ void k_add_cols(DimSpec drows, const int* ncols_dyn, const double* __restrict__ src,
                            int64_t lds, double* __restrict__ dst, int64_t ldd,
                            const int* __restrict__ cols) {
   const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int c = blockIdx.y;
-  if (r >= rows || c >= *ncols_dyn) return;
+  if (r >= dim_eval(drows) || c >= *ncols_dyn) return;
   dst[r + (int64_t)cols[c] * ldd] += src[r + (int64_t)c * lds];
 }
 
-hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_dyn, int64_t ncols_max,
-                                const double* src, int64_t lds, double* dst, int64_t ldd,
-                                const int* cols) {
-  if (rows <= 0 || ncols_max <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_add_cols, dim3((unsigned)((rows + 255) / 256), (unsigned)ncols_max), dim3(256),
-                     0, st, rows, ncols_dyn, src, lds, dst, ldd, cols);
+hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
+                                const int* ncols_dyn, int64_t ncols_max, const double* src,
+                                int64_t lds, double* dst, int64_t ldd, const int* cols) {
+  if (rows_max <= 0 || ncols_max <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_cols, dim3((unsigned)((rows_max + 255) / 256), (unsigned)ncols_max),
+                     dim3(256), 0, st, rows, ncols_dyn, src, lds, dst, ldd, cols);
   return hipGetLastError();
 }
 

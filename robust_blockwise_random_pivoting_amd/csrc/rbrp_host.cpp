@@ -514,38 +514,41 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const MatRef Vkept = mixed ? mat_cols(ws.Vd, mp, ws.Jkpos) : mat_cols(ws.Aw, mp, ws.Jk);
   const MatRef Qall = mixed ? mat(ws.Q64, mp) : mat(ws.Aw, mp);
 
-  int64_t i = 0, nb = 0;
-  double err = (N0 > 0.0) ? 1.0 : 0.0;
-  while (i < kmax && N0 > 0.0) {
-    const size_t rec_mark = pt.recs.size();
-    // ---- sample s columns of [i, n) ------------------------------------
-    ph = pt.begin(RBRP_PHASE_SAMPLE);
-    HIPCHK(rbrp_launch_sample(st, ws.diags, i, n, kmax, b, key0, key1, ctr, ws.w, ws.cdf, ws.st, ws.J));
-    pt.end(ph, 1, 0.0, 16.0 * (n - i));
-    const int* d_s = &ws.st->s;
-    const int* d_bt = &ws.st->bt;
-
-    // ---- second Gram-Schmidt pass of the sampled columns ----------------
-    int ph_cgs = pt.begin(RBRP_PHASE_CGS);
+  const int* d_s = &ws.st->s;
+  const int* d_bt = &ws.st->bt;
+  // Front of a block: sampling and the second Gram-Schmidt pass.  With `dyn`
+  // the block's first column is ibase + *d_bt (the state still holds the
+  // previous block's kept count): the front is queued before the previous
+  // block's state reaches the host, sized for the largest first column
+  // ibase + b, and does nothing (the sampler sets s = 0) if that block ends
+  // the loop.  *ph_cgs receives the CGS phase record.
+  auto queue_front = [&](int64_t ibase, bool dyn, int* ph_cgs) -> int {
+    int ph = pt.begin(RBRP_PHASE_SAMPLE);
+    HIPCHK(rbrp_launch_sample(st, ws.diags, ibase, n, kmax, b, key0, key1, ctr, ws.w, ws.cdf, ws.st,
+                              ws.J, dyn ? d_bt : nullptr, tol2));
+    pt.end(ph, 1, 0.0, 16.0 * (n - ibase));
+    *ph_cgs = pt.begin(RBRP_PHASE_CGS);
     int cgs_launches = 0;
     if (mixed) {
       HIPCHK(rbrp_launch_gather_v64(st, mp, b, ws.st, ws.Aw32, mp, ws.J, ws.Vd, mp));
       ++cgs_launches;
     }
-    if (i > 0) {
+    const int64_t imax = dyn ? std::min<int64_t>(ibase + b, kmax) : ibase;
+    if (imax > 0) {
+      const DimSpec di = dyn ? dim_dyn(ibase, d_bt, 1) : dim_static(ibase);
       int ns;
       int64_t kc;
       // C = Q^T V computed as its transpose V^T Q (the sampled block on the
       // 64-row side of the tiles), reduced into C column-major
-      choose_split(b, i, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
-      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, mp, ns, kc, Vblk, Qall,
-                                   ws.P, b, i * (int64_t)b));
-      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_static(i), b, i, ws.P, b,
-                                       i * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
+      choose_split(b, imax, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
+      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, Vblk, Qall, ws.P,
+                                   b, imax * (int64_t)b));
+      HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), di, b, imax, ws.P, b,
+                                       imax * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
                                        GEMM_SET, ws.Cbuf, ldr));
-      HIPCHK(rbrp_launch_add_cols(st, i, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, dim_static(i),
-                                   Qall, mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
+      HIPCHK(rbrp_launch_add_cols(st, di, imax, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
+      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, Qall,
+                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
       if (mixed) {
         // the corrected columns replace the stored residual (the filtered-out
@@ -554,8 +557,19 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
         ++cgs_launches;
       }
     }
-    pt.end(ph_cgs, cgs_launches);
+    pt.end(*ph_cgs, cgs_launches);
+    return 0;
+  };
 
+  int64_t i = 0, nb = 0;
+  double err = (N0 > 0.0) ? 1.0 : 0.0;
+  int ph_cgs = -1;
+  if (i < kmax && N0 > 0.0) {
+    const int e = queue_front(0, false, &ph_cgs);
+    if (e != 0) return e;
+  }
+  while (i < kmax && N0 > 0.0) {
+    // (this block's sampling and CGS are already on the stream)
     // ---- panel: Gram, pivoted Cholesky + filter, CholQR2 ----------------
     ph = pt.begin(RBRP_PHASE_PANEL);
     {
@@ -664,6 +678,14 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
     pt.end(ph, nrem_max > 0 ? 2 : 1);
 
+    // the next block's front goes on the stream before this block's state is
+    // read back, so the device does not idle while the host decides
+    const size_t next_mark = pt.recs.size();
+    int ph_cgs_next = -1;
+    {
+      const int e = queue_front(i, true, &ph_cgs_next);
+      if (e != 0) return e;
+    }
     HIPCHK(wait_published(h, st, ws.st, seq));
     const RbrpBlockState hs = *h->host_state;
     if (hs.status != 0) {
@@ -671,7 +693,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       return RBRP_ERR_NUMERIC;
     }
     if (hs.s <= 0 || hs.bt <= 0) {
-      pt.recs.resize(rec_mark);  // the empty block's records
+      pt.recs.resize(next_mark);  // the front that does nothing
       break;
     }
     const int64_t bt = hs.bt, s = hs.s;
@@ -692,7 +714,11 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       res->t_ranks[nb] = now_s() - t_start;
     }
     ++nb;
-    if (err <= tol2) break;
+    if (err <= tol2 || i >= kmax) {
+      pt.recs.resize(next_mark);  // the front that does nothing
+      break;
+    }
+    ph_cgs = ph_cgs_next;
   }
   const int64_t k = i;
   res->k = k;

@@ -151,13 +151,16 @@ hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda
 hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
                                     int which);
 hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n);
+// Sampler of a block whose first column is i (+ *i_dyn with i_dyn: then the
+// state holds the previous block's results and the call does nothing, with
+// s = 0, when that block ended the loop; tol2 is the squared tolerance).
 hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
                               int64_t kmax, int b, uint64_t key0, uint64_t key1,
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
-                              int* J);
-hipError_t rbrp_launch_add_cols(hipStream_t st, int64_t rows, const int* ncols_dyn, int64_t ncols_max,
-                                const double* src, int64_t lds, double* dst, int64_t ldd,
-                                const int* cols);
+                              int* J, const int* i_dyn = nullptr, double tol2 = 0.0);
+hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
+                                const int* ncols_dyn, int64_t ncols_max, const double* src,
+                                int64_t lds, double* dst, int64_t ldd, const int* cols);
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
