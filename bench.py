@@ -310,8 +310,8 @@ def cpu_baseline(A_host, args, k_gpu, perm_gpu):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the sharded design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: the column ID runs on one GPU per call (asked for --gpus {args.gpus}); "
+              "there is no multi-GPU form to measure", file=sys.stderr)
         sys.exit(2)
     import torch
 
