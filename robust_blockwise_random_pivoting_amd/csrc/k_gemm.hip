@@ -170,6 +170,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
   const double* ap = mat_col(A, min(m0 + ac, M - 1)) + aq;
   const double* bp = mat_col(B, min(n0 + bc, N - 1)) + bq;
   const int au = aq >> 1, bu = bq >> 1;  // first 16-byte unit of the stager
+  // row blocks holding rows < M (the kept count is often below the tile height)
+  const int mba = (int)min((int64_t)4, (M - m0 + 15) / 16);
 
   v4d acc[4][2];
 #pragma unroll
@@ -207,6 +209,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
     const double2 x10 = *(const double2*)&bs[TN64_AT(32 * wid + 16 + lj, 2 * lg)];              \
     const double2 x11 = *(const double2*)&bs[TN64_AT(32 * wid + 16 + lj, 2 * lg + 1)];          \
     _Pragma("unroll") for (int mb = 0; mb < 4; ++mb) {                                          \
+      if (mb >= mba) break; /* uniform */                                                       \
       const double2 a0 = *(const double2*)&as[TN64_AT(16 * mb + lj, 2 * lg)];                   \
       const double2 a1 = *(const double2*)&as[TN64_AT(16 * mb + lj, 2 * lg + 1)];               \
       acc[mb][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x00.x, a0.x, acc[mb][0], 0, 0, 0);     \

@@ -168,9 +168,10 @@ hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
 // are column-major FP32 with a common leading dimension; their first columns
 // are acoff + *a_dyn and bcoff + *b_dyn.
 //
-// Workgroup tile: 64 rows x 128 columns (an XCD-aware task); wave w owns
-// columns 32w .. 32w+31 (two 16-column MFMA blocks) against the four 16-row
-// blocks of A.  Each 32-row step of both operands is staged through
+// Workgroup tile: MT = 64 or 128 rows (128 when A is wider than 64 columns, so
+// that B is streamed once per split) x 128 columns (an XCD-aware task); wave w
+// owns columns 32w .. 32w+31 (two 16-column MFMA blocks) against the MT / 16
+// row blocks of A.  Each 32-row step of both operands is staged through
 // double-buffered LDS ([column][row], one 128-byte run per column and step,
 // its eight 16-byte units XOR-swizzled with the column index mod 8) with two
 // steps of global loads in flight: the loads of step k+2 are issued before
@@ -189,6 +190,7 @@ hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
 // LDS offset (floats) of 16-byte unit u (rows 4u .. 4u+3) of staged column col
 #define TN32_AT(col, u) ((col) * TN32_KS + 4 * ((u) ^ ((col) & 7)))
 
+template <int MT>
 __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, int64_t K,
                                                       int64_t kchunk, const float* __restrict__ A0,
                                                       int64_t acoff, const int* a_dyn,
@@ -196,19 +198,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
                                                       const int* b_dyn, int64_t ld,
                                                       double* __restrict__ P, int64_t ldp,
                                                       int64_t pstride, int gx, int gy, int gz) {
+  constexpr int MB = MT / 16;  // 16-row MFMA blocks of A per tile
   // task = N tile + gx * (M tile + gy * split), XCD-aware (the tiles of one
   // split share its rows of A through one L2)
   const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy * gz);
   if (task >= (int64_t)gx * gy * gz) return;
   const int64_t bx = task % gx, by = (task / gx) % gy, bz = task / ((int64_t)gx * gy);
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
-  const int64_t n0 = bx * 128, m0 = by * 64;
+  const int64_t n0 = bx * 128, m0 = by * MT;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup
   const int64_t kb0 = bz * kchunk;
   const int64_t kb1 = min(K, kb0 + kchunk);
   if (kb0 >= kb1) return;  // empty chunk (uniform); its partials are never read
 
-  __shared__ float As[2][64 * TN32_KS];   // [buf][col][row], swizzled
+  __shared__ float As[2][MT * TN32_KS];   // [buf][col][row], swizzled
   __shared__ float Bs[2][128 * TN32_KS];
 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -217,37 +220,48 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
   const float* B = B0 + (bcoff + (b_dyn ? *b_dyn : 0)) * ld;
   // Stagers.  Out-of-range columns are clamped to the last valid one: they
   // only feed output rows / columns that are never stored.
-  const int ac = tid >> 2, aq = (tid & 3) * 8;    // A: column ac, rows aq .. aq+7
+  constexpr int AQ = MT / 8;  // A rows per thread and step: 8 (MT = 64) or 16
+  const int ac = tid / (256 / MT), aq = (tid % (256 / MT)) * AQ;  // A: column ac, rows aq ..
   const int bc = tid >> 1, bq = (tid & 1) * 16;   // B: column bc, rows bq .. bq+15
   const float* ap = A + min(m0 + ac, M - 1) * ld + aq;
   const float* bp = B + min(n0 + bc, N - 1) * ld + bq;
   const int au = aq >> 2, bu = bq >> 2;  // first 16-byte unit of the stager
+  // row blocks holding rows < M (A's width is often below the tile height)
+  const int mba = (int)min((int64_t)MB, (M - m0 + 15) / 16);
 
-  v4f acc[4][2];
+  v4f acc[MB][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
   // two sets of staging registers (named scalars: small arrays here get
-  // placed in LDS)
-#define TN32_LOAD(S, kk)                                  \
-  do {                                                    \
-    S##a0 = *(const float4*)(ap + (kk));                  \
-    S##a1 = *(const float4*)(ap + (kk) + 4);              \
-    S##b0 = *(const float4*)(bp + (kk));                  \
-    S##b1 = *(const float4*)(bp + (kk) + 4);              \
-    S##b2 = *(const float4*)(bp + (kk) + 8);              \
-    S##b3 = *(const float4*)(bp + (kk) + 12);             \
+  // placed in LDS); the A units past AQ / 4 exist only for MT = 128
+#define TN32_LOAD(S, kk)                                                  \
+  do {                                                                    \
+    S##a0 = *(const float4*)(ap + (kk));                                  \
+    S##a1 = *(const float4*)(ap + (kk) + 4);                              \
+    if (MT == 128) {                                                      \
+      S##a2 = *(const float4*)(ap + (kk) + 8);                            \
+      S##a3 = *(const float4*)(ap + (kk) + 12);                           \
+    }                                                                     \
+    S##b0 = *(const float4*)(bp + (kk));                                  \
+    S##b1 = *(const float4*)(bp + (kk) + 4);                              \
+    S##b2 = *(const float4*)(bp + (kk) + 8);                              \
+    S##b3 = *(const float4*)(bp + (kk) + 12);                             \
   } while (0)
-#define TN32_STORE(S, bb)                                 \
-  do {                                                    \
-    *(float4*)&As[bb][TN32_AT(ac, au)] = S##a0;           \
-    *(float4*)&As[bb][TN32_AT(ac, au + 1)] = S##a1;       \
-    *(float4*)&Bs[bb][TN32_AT(bc, bu)] = S##b0;           \
-    *(float4*)&Bs[bb][TN32_AT(bc, bu + 1)] = S##b1;       \
-    *(float4*)&Bs[bb][TN32_AT(bc, bu + 2)] = S##b2;       \
-    *(float4*)&Bs[bb][TN32_AT(bc, bu + 3)] = S##b3;       \
+#define TN32_STORE(S, bb)                                                 \
+  do {                                                                    \
+    *(float4*)&As[bb][TN32_AT(ac, au)] = S##a0;                           \
+    *(float4*)&As[bb][TN32_AT(ac, au + 1)] = S##a1;                       \
+    if (MT == 128) {                                                      \
+      *(float4*)&As[bb][TN32_AT(ac, au + 2)] = S##a2;                     \
+      *(float4*)&As[bb][TN32_AT(ac, au + 3)] = S##a3;                     \
+    }                                                                     \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu)] = S##b0;                           \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 1)] = S##b1;                       \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 2)] = S##b2;                       \
+    *(float4*)&Bs[bb][TN32_AT(bc, bu + 3)] = S##b3;                       \
   } while (0)
 #define TN32_STEP(bb)                                                                         \
   do {                                                                                        \
@@ -260,7 +274,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
       bv[h][0] = x0.x; bv[h][1] = x0.y; bv[h][2] = x0.z; bv[h][3] = x0.w;                     \
       bv[h][4] = x1.x; bv[h][5] = x1.y; bv[h][6] = x1.z; bv[h][7] = x1.w;                     \
     }                                                                                         \
-    _Pragma("unroll") for (int mb = 0; mb < 4; ++mb) {                                        \
+    _Pragma("unroll") for (int mb = 0; mb < MB; ++mb) {                                       \
+      if (mb >= mba) break; /* uniform */                                                     \
       const float4 a0 = *(const float4*)&as[TN32_AT(16 * mb + lj, 2 * lg)];                   \
       const float4 a1 = *(const float4*)&as[TN32_AT(16 * mb + lj, 2 * lg + 1)];               \
       const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};                   \
@@ -272,8 +287,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
   } while (0)
 
   const int64_t klast = kb1 - TN32_KS;
-  float4 Xa0, Xa1, Xb0, Xb1, Xb2, Xb3;
-  float4 Ya0, Ya1, Yb0, Yb1, Yb2, Yb3;
+  float4 Xa0, Xa1, Xa2, Xa3, Xb0, Xb1, Xb2, Xb3;
+  float4 Ya0, Ya1, Ya2, Ya3, Yb0, Yb1, Yb2, Yb3;
   TN32_LOAD(X, kb0);
   TN32_STORE(X, 0);
   TN32_LOAD(X, min(kb0 + TN32_KS, klast));
@@ -302,7 +317,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
 #undef TN32_STEP
   double* Pz = P + bz * pstride;
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -313,11 +328,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn32(DimSpec dM, DimSpec dN, in
       }
 }
 
-int rbrp_gemm_tn32_resident() {
+int rbrp_gemm_tn32_resident(int wide) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_tn32, 256, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per, wide ? k_gemm_tn32<128> : k_gemm_tn32<64>, 256, 0) != hipSuccess)
     return 0;
   return cus * per;
 }
@@ -329,10 +345,17 @@ hipError_t rbrp_launch_gemm_tn32(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t pstride, hipEvent_t e0, hipEvent_t e1) {
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
   if (kchunk % TN32_KS != 0 || K % TN32_KS != 0 || ld % 4 != 0) return hipErrorInvalidValue;
-  const int64_t gx = (Nmax + 127) / 128, gy = (Mmax + 63) / 64;
+  // the whole A block in one 128-row tile when it is wider than 64 (the
+  // trailing columns B are then streamed once per split, not once per tile)
+  const bool wide = Mmax > 64;
+  const int64_t gx = (Nmax + 127) / 128, gy = wide ? (Mmax + 127) / 128 : (Mmax + 63) / 64;
   dim3 grid((unsigned)rbrp_xcd_grid(gx * gy * nsplit));
-  RBRP_LAUNCH_EV(e0, e1, k_gemm_tn32, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff, a_dyn, B,
-                 bcoff, b_dyn, ld, P, ldp, pstride, (int)gx, (int)gy, nsplit);
+  if (wide)
+    RBRP_LAUNCH_EV(e0, e1, k_gemm_tn32<128>, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff,
+                   a_dyn, B, bcoff, b_dyn, ld, P, ldp, pstride, (int)gx, (int)gy, nsplit);
+  else
+    RBRP_LAUNCH_EV(e0, e1, k_gemm_tn32<64>, grid, dim3(256), 0, st, M, N, K, kchunk, A, acoff,
+                   a_dyn, B, bcoff, b_dyn, ld, P, ldp, pstride, (int)gx, (int)gy, nsplit);
   return hipGetLastError();
 }
 
@@ -515,10 +538,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
   const int ncw = (int)max((int64_t)0, min((int64_t)32, N - cw));
   float* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + (ncw > 0 ? cw : c0)) * ldc;
   const int cmax = max(ncw, 1) - 1;
-  float b0[KQ], b1[KQ];  // B(KQ lk + q, cw + li) and B(KQ lk + q, cw + 16 + li)
+  // MFMA q of a sub-block takes k = 4q .. 4q+3 (k = 4q + lk in lane group lk),
+  // so every q with 4q >= K is skipped (the kept count is often below KB)
+  const int qa = (int)min((int64_t)KQ, ((K + 7) / 8) * 2);  // MFMA steps in use, even
+  float b0[KQ], b1[KQ];  // B(4q + lk, cw + li) and B(4q + lk, cw + 16 + li)
 #pragma unroll
   for (int q = 0; q < KQ; ++q) {
-    const int64_t k = KQ * lk + q;
+    const int64_t k = 4 * q + lk;
     b0[q] = (k < K && li < ncw) ? Bt[k * ldbt + cw + li] : 0.0f;
     b1[q] = (k < K && 16 + li < ncw) ? Bt[k * ldbt + cw + 16 + li] : 0.0f;
   }
@@ -531,9 +557,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
     const float* ap_ = acol + (row0);                                                 \
     _Pragma("unroll") for (int j = 0; j < RPT / 4; ++j) sv[j] = *(const float4*)(ap_ + 4 * j); \
   } while (0)
+  // column ka sits at k position (ka % 4) KQ + ka / 4 of its row pair, so the k
+  // values 4q + lk of lane group lk are contiguous
 #define UPD32_STOREA(bb)                                                              \
   do {                                                                                \
-    float* d_ = &As[bb][0] + (ar / 2) * LDP + 2 * ka;                                 \
+    float* d_ = &As[bb][0] + (ar / 2) * LDP + 2 * ((ka & 3) * KQ + (ka >> 2));        \
     _Pragma("unroll") for (int j = 0; j < RPT / 4; ++j) {                             \
       *(float2*)(d_ + (2 * j) * LDP) = make_float2(sv[j].x, sv[j].y);                 \
       *(float2*)(d_ + (2 * j + 1) * LDP) = make_float2(sv[j].z, sv[j].w);             \
@@ -570,11 +598,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd32(int64_t M, DimSpec dN, Di
     for (int sb = 0; sb < 2; ++sb) {
       const int64_t r = row0 + 32 * sb;
       UPD32_LOADC(cn, sb == 0 ? r + 32 : rnext);
-      const float* apair = as + (16 * sb + li) * LDP + 2 * KQ * lk;  // k = KQ lk .., two rows
+      const float* apair = as + (16 * sb + li) * LDP + 2 * KQ * lk;  // k = 4q + lk, two rows
       v4f a00 = (v4f){0.f, 0.f, 0.f, 0.f}, a01 = a00, a10 = a00, a11 = a00;  // [row of pair][h]
 #pragma unroll
       for (int q = 0; q < KQ; q += 2) {
-        const float4 x = *(const float4*)(apair + 2 * q);  // (k, row 0), (k, row 1), (k+1, row 0), (k+1, row 1)
+        if (q >= qa) break;  // uniform
+        // (4q + lk, row 0), (4q + lk, row 1), (4q + 4 + lk, row 0), (4q + 4 + lk, row 1)
+        const float4 x = *(const float4*)(apair + 2 * q);
         a00 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[q], x.x, a00, 0, 0, 0);
         a01 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1[q], x.x, a01, 0, 0, 0);
         a10 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0[q], x.y, a10, 0, 0, 0);

@@ -29,7 +29,7 @@ struct rbrp_handle_s {
   hipStream_t stream = nullptr;
   int device = 0;
   // workgroups of the split-K projection kernels resident at once (0: unknown)
-  int tn64_resident = 0, tn32_resident = 0, nn64_resident = 0;
+  int tn64_resident = 0, tn32_resident = 0, tn32w_resident = 0, nn64_resident = 0;
   RbrpWorkspace ws;
   void* sk_arena = nullptr;  // workspace of the sketch comparators
   size_t sk_bytes = 0;

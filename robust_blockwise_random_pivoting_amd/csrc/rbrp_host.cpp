@@ -135,8 +135,9 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
 // `resident` workgroups (the kernel's whole-device residency: one round, no
 // tail of a few late workgroups) or about 1024 workgroups when that is unknown.
 void choose_split(int64_t Mmax, int64_t Nmax, int64_t K, int64_t ldp, int64_t Pcap, int* nsplit,
-                  int64_t* kchunk, int64_t align = 16, int64_t tile_n = 64, int resident = 0) {
-  const int64_t tiles = ((Mmax + 63) / 64) * ((Nmax + tile_n - 1) / tile_n);
+                  int64_t* kchunk, int64_t align = 16, int64_t tile_n = 64, int resident = 0,
+                  int64_t tile_m = 64) {
+  const int64_t tiles = ((Mmax + tile_m - 1) / tile_m) * ((Nmax + tile_n - 1) / tile_n);
   const int64_t target = resident > 0 ? resident : 1024;
   int64_t ns = std::max<int64_t>(1, target / std::max<int64_t>(tiles, 1));
   ns = std::min<int64_t>(ns, tiles <= 4 ? 256 : 128);
@@ -355,7 +356,8 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
     return (int)e;
   }
   h->tn64_resident = rbrp_gemm_tn64_resident();
-  h->tn32_resident = rbrp_gemm_tn32_resident();
+  h->tn32_resident = rbrp_gemm_tn32_resident(0);
+  h->tn32w_resident = rbrp_gemm_tn32_resident(1);
   h->nn64_resident = rbrp_gemm_nn64_resident();
   *handle = h;
   return 0;
@@ -634,8 +636,11 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (nrem_max > 0) {
       int ns;
       int64_t kc;
+      // (the FP32 kernel takes a block wider than 64 in one 128-row tile)
+      const bool wide32 = mixed && b > 64;
       choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128,
-                   mixed ? h->tn32_resident : h->tn64_resident);
+                   mixed ? (wide32 ? h->tn32w_resident : h->tn32_resident) : h->tn64_resident,
+                   wide32 ? 128 : 64);
       hipEvent_t ev0, ev1;
       ph_proj = pt.begin_kernel(RBRP_PHASE_PROJECT, &ev0, &ev1);
       if (mixed)

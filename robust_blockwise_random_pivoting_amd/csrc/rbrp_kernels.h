@@ -96,9 +96,10 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t ldp, int64_t pstride, hipEvent_t e0 = nullptr,
                                  hipEvent_t e1 = nullptr);
 // Workgroups of k_gemm_tn64 / k_gemm_tn32 resident at once on the current
-// device (compute units x occupancy); 0 if the runtime cannot tell.
+// device (compute units x occupancy); 0 if the runtime cannot tell.  `wide`:
+// the FP32 kernel's 128-row tile form (launched when A is wider than 64).
 int rbrp_gemm_tn64_resident();
-int rbrp_gemm_tn32_resident();
+int rbrp_gemm_tn32_resident(int wide = 0);
 
 // C(r, c) (op)= sum_{k < K} A(r, k) B(k, c) for a tall C with 64-column tiles
 // and row ranges of at most 64 rows balanced over `resident` workgroups (the
@@ -226,7 +227,8 @@ hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const Rb
 hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
                                  const RbrpBlockState* bs, const double* Q, int64_t ldq, float* Aw,
                                  int64_t ld);
-// Split-K TN product of FP32 operands with FP64 partials (see k_gemm_tn);
+// Split-K TN product of FP32 operands with FP64 partials (see k_gemm_tn; a
+// single 128-row tile when M > 64);
 // A's and B's first columns are acoff + *a_dyn and bcoff + *b_dyn (dyn
 // optional), common leading dimension ld (multiple of 4); K and kchunk are
 // multiples of 32.

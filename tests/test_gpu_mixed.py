@@ -45,7 +45,9 @@ def _stream():
 
 
 @pytest.mark.parametrize("M,N,K,nsplit", [(1, 1, 32, 1), (17, 5, 64, 3), (64, 64, 256, 4),
-                                          (100, 130, 1024, 7), (29, 64, 4096, 16), (64, 300, 50048, 40)])
+                                          (100, 130, 1024, 7), (29, 64, 4096, 16), (64, 300, 50048, 40),
+                                          (128, 300, 4096, 8), (65, 129, 2048, 3), (300, 40, 512, 2),
+                                          (128, 1000, 100000, 48)])
 def test_gemm_tn32_within_fp32_bound(librbrp, M, N, K, nsplit):
     import torch
     rng = np.random.default_rng(M * 1000 + N + K)
