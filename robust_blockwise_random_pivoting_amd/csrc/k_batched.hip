@@ -103,17 +103,36 @@ __device__ void bq_tn(int M, int N, int K, FA colA, FB colB, double* part, FS st
     const double* pa = colA(min(r0 + li, M - 1));
     const double* pb = colB(min(c0 + li, N - 1));
     const int kb = pid * kc, ke = min(K, kb + kc);
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
+    // two 16-row steps per iteration: eight loads in flight per lane, the
+    // MFMAs alternating over two accumulators
+    v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = acc;
+    int k0 = kb;
+    for (; k0 + 32 <= ke; k0 += 32) {
+      double2 a[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a[2 * u] = *(const double2*)(pa + k0 + 16 * u + 4 * lg);
+        a[2 * u + 1] = *(const double2*)(pa + k0 + 16 * u + 4 * lg + 2);
+        bv[2 * u] = *(const double2*)(pb + k0 + 16 * u + 4 * lg);
+        bv[2 * u + 1] = *(const double2*)(pb + k0 + 16 * u + 4 * lg + 2);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, bv[u].x, acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, bv[u].y, acc2, 0, 0, 0);
+      }
+    }
+    for (; k0 < ke; k0 += 16) {
       const double2 a0 = *(const double2*)(pa + k0 + 4 * lg);
       const double2 a1 = *(const double2*)(pa + k0 + 4 * lg + 2);
       const double2 b0 = *(const double2*)(pb + k0 + 4 * lg);
       const double2 b1 = *(const double2*)(pb + k0 + 4 * lg + 2);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, b0.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, b0.y, acc, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, b0.y, acc2, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.x, b1.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, b1.y, acc, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, b1.y, acc2, 0, 0, 0);
     }
+    acc += acc2;
     if (nsplit == 1) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -477,7 +496,7 @@ __device__ void bq_chol2(const BqLds& L, int bt) {
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BQ_NT) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
+__global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(3))) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
                                                          const int* __restrict__ order, int nprob,
                                                          int* queue, RbrpBatchResult* results,
                                                          RbrpBatchShape shape) {
