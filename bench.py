@@ -320,7 +320,12 @@ def main():
 
     torch.cuda.set_device(0)
     m, n = args.m, args.n
-    if args.load_matrix:
+    # the 50000 x 4000 matrix serves the main and mixed legs and the probes only
+    need_a = (not args.no_main or not args.no_mixed or args.traffic_probe
+              or bool(args.save_matrix))
+    if not need_a:
+        A = None
+    elif args.load_matrix:
         At = np.load(args.load_matrix, mmap_mode="r")            # (n, m) row-major == A column-major
         A = torch.from_numpy(np.ascontiguousarray(At)).to("cuda").t()
         del At
@@ -328,7 +333,8 @@ def main():
         A = spectrum_matrix_torch(m, n, exp_spectrum(n, args.rate), seed=args.matrix_seed,
                                   device="cuda")
     torch.cuda.synchronize()
-    print(f"[bench] matrix {m}x{n} ready", file=sys.stderr, flush=True)
+    if A is not None:
+        print(f"[bench] matrix {m}x{n} ready", file=sys.stderr, flush=True)
     if args.save_matrix:
         np.save(args.save_matrix, A.t().contiguous().cpu().numpy())
         return

@@ -103,21 +103,21 @@ __device__ void bq_tn(int M, int N, int K, FA colA, FB colB, double* part, FS st
     const double* pa = colA(min(r0 + li, M - 1));
     const double* pb = colB(min(c0 + li, N - 1));
     const int kb = pid * kc, ke = min(K, kb + kc);
-    // two 16-row steps per iteration: eight loads in flight per lane, the
+    // four 16-row steps per iteration: sixteen loads in flight per lane, the
     // MFMAs alternating over two accumulators
     v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = acc;
     int k0 = kb;
-    for (; k0 + 32 <= ke; k0 += 32) {
-      double2 a[4], bv[4];
+    for (; k0 + 64 <= ke; k0 += 64) {
+      double2 a[8], bv[8];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < 4; ++u) {
         a[2 * u] = *(const double2*)(pa + k0 + 16 * u + 4 * lg);
         a[2 * u + 1] = *(const double2*)(pa + k0 + 16 * u + 4 * lg + 2);
         bv[2 * u] = *(const double2*)(pb + k0 + 16 * u + 4 * lg);
         bv[2 * u + 1] = *(const double2*)(pb + k0 + 16 * u + 4 * lg + 2);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, bv[u].x, acc, 0, 0, 0);
         acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, bv[u].y, acc2, 0, 0, 0);
       }
@@ -175,24 +175,44 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
   const int TNn = (N + 15) >> 4;
   for (int tn = w; tn < TNn; tn += 4) {
     const int c0 = tn * 16;
+    const bool cok = c0 + li < N;
     double nsq[4] = {0.0, 0.0, 0.0, 0.0};
+    // B fragment of the first 16 k (the whole reduction for K <= 16: the
+    // update and the CholQR2 products), loaded once for the column tile
+    double bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = 4 * q + lg;
+      bq[q] = (k < K && cok) ? belem(k, c0 + li) : 0.0;
+    }
     for (int r0 = 0; r0 < rows; r0 += 16) {
       const int r = r0 + li;
       const bool rok = r < rows;
+      // every load of a 16-k chunk (and C) is issued before its MFMAs
+      double cv[4];
+      if (SUB) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cv[v] = rok ? colC(min(c0 + 4 * v + lg, N - 1))[r] : 0.0;
+      }
       v4d acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k0 = 0; k0 < K; k0 += 4) {
-        const int k = k0 + lg;
-        const double bv = (k < K && c0 + li < N) ? belem(k, c0 + li) : 0.0;
-        const double av = (k < K && rok) ? colA(k)[r] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av, acc, 0, 0, 0);
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int k = k0 + 4 * q + lg;
+          av[q] = (k < K && rok) ? colA(k)[r] : 0.0;
+          bv[q] = k0 == 0 ? bq[q] : ((k < K && cok) ? belem(k, c0 + li) : 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[q], av[q], acc, 0, 0, 0);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int c = c0 + 4 * v + lg;
         if (c < N && rok) {
-          double* pc = colC(c) + r;
-          const double nv = SUB ? *pc - acc[v] : acc[v];
-          *pc = nv;
+          const double nv = SUB ? cv[v] - acc[v] : acc[v];
+          colC(c)[r] = nv;
           nsq[v] = fma(nv, nv, nsq[v]);
         }
       }
@@ -211,6 +231,22 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
     }
   }
   __syncthreads();
+}
+
+// *dst(idx) = *src(idx) for idx < count over the workgroup's threads, the
+// loads of four iterations issued before their stores (a copy otherwise
+// waits for each load in turn).  Source and destination must not overlap.
+template <class FS, class FD>
+__device__ inline void bq_copy(int count, FS src, FD dst) {
+  int idx = threadIdx.x;
+  for (; idx + 3 * BQ_NT < count; idx += 4 * BQ_NT) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = *src(idx + u * BQ_NT);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) *dst(idx + u * BQ_NT) = v[u];
+  }
+  for (; idx < count; idx += BQ_NT) *dst(idx) = *src(idx);
 }
 
 // ---------------------------------------------------------------------------
@@ -496,7 +532,7 @@ __device__ void bq_chol2(const BqLds& L, int bt) {
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(3))) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
+__global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
                                                          const int* __restrict__ order, int nprob,
                                                          int* queue, RbrpBatchResult* results,
                                                          RbrpBatchShape shape) {
@@ -535,10 +571,22 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(3))) void
         const double* a = P.A + (int64_t)j * P.lda;
         double* aw = Aw + (int64_t)j * mp;
         double acc = 0.0;
-        for (int r = lane; r < mp; r += 64) {
-          const double v = r < m ? a[r] : 0.0;
-          aw[r] = v;
-          acc = fma(v, v, acc);
+        // four rows per lane and iteration, loaded before they are stored
+        for (int r0 = lane; r0 < mp; r0 += 256) {
+          double v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = r0 + 64 * u;
+            v[u] = r < m ? a[r] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = r0 + 64 * u;
+            if (r < mp) {
+              aw[r] = v[u];
+              acc = fma(v[u], v[u], acc);
+            }
+          }
         }
         acc = rbrp_wave_sum(acc);
         if (lane == 0) L.diags[j] = acc;
@@ -608,34 +656,34 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(3))) void
         L.dout[tid] = L.diags[L.Jout[tid]];
       }
       // R rows 0..i of the kept / leaving columns into Tbuf
-      for (int idx = tid; idx < i * (bt + nsw); idx += BQ_NT) {
-        const int r = idx % i, q = idx / i;
-        const int col = q < bt ? L.Jk[q] : L.Jout[q - bt];
-        Tbuf[r + (int64_t)q * ldr] = R[r + (int64_t)col * ldr];
-      }
+      bq_copy(
+          i * (bt + nsw),
+          [&](int idx) {
+            const int r = idx % i, q = idx / i;
+            return R + r + (int64_t)(q < bt ? L.Jk[q] : L.Jout[q - bt]) * ldr;
+          },
+          [&](int idx) { return Tbuf + idx % i + (int64_t)(idx / i) * ldr; });
       __syncthreads();
       if (tid < nsw) {
         P.perm[L.Jin[tid]] = L.to[tid];
         L.diags[L.Jin[tid]] = L.dout[tid];
       }
-      for (int idx = tid; idx < i * nsw; idx += BQ_NT) {
-        const int r = idx % i, q = idx / i;
-        R[r + (int64_t)L.Jin[q] * ldr] = Tbuf[r + (int64_t)(bt + q) * ldr];
-      }
+      bq_copy(
+          i * nsw, [&](int idx) { return Tbuf + idx % i + (int64_t)(bt + idx / i) * ldr; },
+          [&](int idx) { return R + idx % i + (int64_t)L.Jin[idx / i] * ldr; });
       // working-matrix columns leaving the block slots move to the vacated ones
-      for (int idx = tid; idx < mp * nsw; idx += BQ_NT) {
-        const int r = idx % mp, q = idx / mp;
-        Aw[r + (int64_t)L.Jin[q] * mp] = Aw[r + (int64_t)L.Jout[q] * mp];
-      }
+      // (Jin and Jout are disjoint)
+      bq_copy(
+          mp * nsw, [&](int idx) { return Aw + idx % mp + (int64_t)L.Jout[idx / mp] * mp; },
+          [&](int idx) { return Aw + idx % mp + (int64_t)L.Jin[idx / mp] * mp; });
       __syncthreads();
       if (tid < bt) {
         P.perm[i + tid] = L.tk[tid];
         L.diags[i + tid] = L.dk[tid];
       }
-      for (int idx = tid; idx < i * bt; idx += BQ_NT) {
-        const int r = idx % i, q = idx / i;
-        R[r + (int64_t)(i + q) * ldr] = Tbuf[r + (int64_t)q * ldr];
-      }
+      bq_copy(
+          i * bt, [&](int idx) { return Tbuf + idx % i + (int64_t)(idx / i) * ldr; },
+          [&](int idx) { return R + idx % i + (int64_t)(i + idx / i) * ldr; });
       // new diagonal block of R, zeros left of it in its rows
       for (int idx = tid; idx < bt * bt; idx += BQ_NT) {
         const int r = idx % bt, q = idx / bt;
