@@ -723,10 +723,9 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
     // ---- X = R11^{-1} R12 ------------------------------------------------------
     if (!bad && P.X && k > 0 && n > k) {
       const int nc = n - k;
-      for (int idx = tid; idx < k * nc; idx += BQ_NT) {
-        const int r = idx % k, c = idx / k;
-        P.X[r + (int64_t)c * P.ldx] = R[r + (int64_t)(k + c) * ldr];
-      }
+      bq_copy(
+          k * nc, [&](int idx) { return R + idx % k + (int64_t)(k + idx / k) * ldr; },
+          [&](int idx) { return P.X + idx % k + (int64_t)(idx / k) * P.ldx; });
       __syncthreads();
       for (int rb = ((k - 1) / 16) * 16; rb >= 0; rb -= 16) {
         const int re = min(rb + 16, k);
