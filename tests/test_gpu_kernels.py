@@ -78,9 +78,16 @@ def test_gemm_tn64_matches_numpy(librbrp, M, N, K, nsplit):
 
 
 @pytest.mark.parametrize("M,N,K,mode", [(64, 64, 16, 0), (132, 70, 33, 1), (4, 3, 1, 2),
-                                        (50048, 64, 1000, 1), (256, 17, 2047, 2), (128, 64, 48, 1)])
+                                        (50048, 64, 1000, 1), (256, 17, 2047, 2), (128, 64, 48, 1),
+                                        # row ranges of 1..4 blocks, rows not a multiple of 16,
+                                        # two column tiles, more ranges than one per workgroup
+                                        (1004, 64, 100, 1), (16, 3, 5, 0), (70000, 128, 300, 1),
+                                        (20, 64, 64, 2), (300000, 64, 40, 1)])
 def test_gemm_nn64_matches_numpy(librbrp, M, N, K, mode):
-    """The long-reduction NN kernel against NumPy at FP64 rounding (set / subtract / add)."""
+    """The long-reduction NN kernel against NumPy at FP64 rounding (set / subtract / add).
+    Its rows are split into ranges of 16-row blocks balanced over the resident
+    workgroups, so the cases cover ranges of one to four blocks, partial blocks,
+    two column tiles and more ranges than resident workgroups."""
     import torch
     rng = np.random.default_rng(M + 7 * N + 11 * K)
     A = rng.standard_normal((M, K))

@@ -108,6 +108,42 @@ def test_rbrp_edge_cases():
     assert_parity(A, g, r, "max_rank")
 
 
+def test_phase_timing_levels():
+    """Level 1 times every phase, level 2 only the projection and the update (their
+    own dispatch events), 0 nothing; the ID is the same at every level, and the
+    next-block front that each ID queues past its last block (a no-op) leaves no
+    sampling record."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import Handle, rbrp_colid
+    A = _dev(spectrum_matrix(3000, 700, exp_spectrum(700, 40.0), 5))
+    outs = {}
+    with Handle() as h:
+        for level in (1, 2, 0):
+            h.set_timing(level)
+            h.reset_stats()
+            g = rbrp_colid(A, b=32, tol=1e-8, seed=3, handle=h)
+            torch.cuda.synchronize()
+            outs[level] = (g, h.stats())
+        with pytest.raises(Exception):
+            h.set_timing(3)
+    g1, st1 = outs[1]
+    nb = len(g1.ranks)
+    assert nb >= 3
+    assert st1["sample"]["launches"] == nb
+    for ph in ("sample", "panel", "permute", "project", "update", "resid"):
+        assert st1[ph]["launches"] > 0 and st1[ph]["ms"] > 0.0, ph
+    g2, st2 = outs[2]
+    assert st2["project"]["launches"] > 0 and st2["project"]["ms"] > 0.0
+    assert st2["update"]["launches"] > 0 and st2["update"]["ms"] > 0.0
+    assert st2["project"]["flops"] > 0.0 and st2["update"]["bytes"] > 0.0
+    for ph in ("norms", "sample", "cgs", "panel", "permute", "project_reduce", "resid", "solve"):
+        assert st2[ph]["launches"] == 0 and st2[ph]["ms"] == 0.0, ph
+    g0, st0 = outs[0]
+    assert all(v["launches"] == 0 for v in st0.values())
+    for g in (g2, g0):
+        assert g.k == g1.k and torch.equal(g.perm, g1.perm) and torch.equal(g.X, g1.X)
+
+
 def test_rbrp_rejects_bad_arguments():
     import torch
     from robust_blockwise_random_pivoting_amd import rbrp_colid
