@@ -163,17 +163,27 @@ __device__ void bq_tn(int M, int N, int K, FA colA, FB colB, double* part, FS st
 
 // C(r, c) = [C(r, c) -] sum_{k < K} A(r, k) B(k, c) for r < rows, c < N.  The
 // MFMA computes the transposed block, so lane l covers row r0 + l%16 and
-// columns c0 + 4v + l/16: every C access moves four 128-byte runs.  Wave w
-// owns the column tiles w, w + 4, ...; with NORMS, normout(c, sum over rows of
-// the new C(r, c)^2) for each column.  colA(k) / colC(c) give column pointers
-// (rows addressed directly); belem(k, c) gives B.  Rows >= rows are neither
-// read nor written; rows is rounded up to 16 internally only for the loop.
-template <bool SUB, bool NORMS, class FA, class FB, class FC, class FN>
-__device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN normout) {
+// columns c0 + 4v + l/16: every C access moves four 128-byte runs.  With four
+// or more 16-column tiles wave w owns the column tiles w, w + 4, ...; with
+// fewer, each column tile is dealt to 4 / tiles waves that take every
+// (4 / tiles)-th 16-row tile, so no wave idles on the narrow products (the
+// re-orthogonalisation, CholQR2 and late updates).  A wave takes BQ_RT of its
+// row tiles at a time with every load of those issued before their MFMAs
+// (BQ_RT > 1 only where the register budget of the kernel allows it).
+// With NORMS, normout(c, sum over rows of the new C(r, c)^2) for each column;
+// the partial sums of the waves sharing a column tile go through part (4 x 16
+// doubles) and are added in wave order.  colA(k) / colC(c) give column
+// pointers (rows addressed directly); belem(k, c) gives B.  Rows >= rows are
+// neither read nor written.
+template <bool SUB, bool NORMS, int BQ_RT, class FA, class FB, class FC, class FN>
+__device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN normout,
+                      double* part) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int TNn = (N + 15) >> 4;
-  for (int tn = w; tn < TNn; tn += 4) {
+  const int TNn = (N + 15) >> 4, nrt = (rows + 15) >> 4;
+  const int ws = TNn >= 4 ? 1 : 4 / max(TNn, 1);  // waves per column tile (uniform)
+  const int sub = w % ws;
+  for (int tn = w / ws; tn < TNn; tn += 4 / ws) {
     const int c0 = tn * 16;
     const bool cok = c0 + li < N;
     double nsq[4] = {0.0, 0.0, 0.0, 0.0};
@@ -185,37 +195,52 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
       const int k = 4 * q + lg;
       bq[q] = (k < K && cok) ? belem(k, c0 + li) : 0.0;
     }
-    for (int r0 = 0; r0 < rows; r0 += 16) {
-      const int r = r0 + li;
-      const bool rok = r < rows;
-      // every load of a 16-k chunk (and C) is issued before its MFMAs
-      double cv[4];
+    for (int t0 = sub; t0 < nrt; t0 += BQ_RT * ws) {
+      // row tiles t0 + u * ws, u < BQ_RT
+      int rr[BQ_RT];
+      bool rok[BQ_RT];
+#pragma unroll
+      for (int u = 0; u < BQ_RT; ++u) {
+        rr[u] = (t0 + u * ws) * 16 + li;
+        rok[u] = rr[u] < rows;
+      }
+      double cv[BQ_RT][4];
       if (SUB) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) cv[v] = rok ? colC(min(c0 + 4 * v + lg, N - 1))[r] : 0.0;
+        for (int u = 0; u < BQ_RT; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            cv[u][v] = rok[u] ? colC(min(c0 + 4 * v + lg, N - 1))[rr[u]] : 0.0;
       }
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      v4d acc[BQ_RT];
+#pragma unroll
+      for (int u = 0; u < BQ_RT; ++u) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
       for (int k0 = 0; k0 < K; k0 += 16) {
-        double av[4], bv[4];
+        double av[BQ_RT][4], bv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int k = k0 + 4 * q + lg;
-          av[q] = (k < K && rok) ? colA(k)[r] : 0.0;
           bv[q] = k0 == 0 ? bq[q] : ((k < K && cok) ? belem(k, c0 + li) : 0.0);
+#pragma unroll
+          for (int u = 0; u < BQ_RT; ++u) av[u][q] = (k < K && rok[u]) ? colA(k)[rr[u]] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[q], av[q], acc, 0, 0, 0);
+#pragma unroll
+          for (int u = 0; u < BQ_RT; ++u)
+            acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[q], av[u][q], acc[u], 0, 0, 0);
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int c = c0 + 4 * v + lg;
-        if (c < N && rok) {
-          const double nv = SUB ? cv[v] - acc[v] : acc[v];
-          colC(c)[r] = nv;
-          nsq[v] = fma(nv, nv, nsq[v]);
+      for (int u = 0; u < BQ_RT; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = c0 + 4 * v + lg;
+          if (c < N && rok[u]) {
+            const double nv = SUB ? cv[u][v] - acc[u][v] : acc[u][v];
+            colC(c)[rr[u]] = nv;
+            nsq[v] = fma(nv, nv, nsq[v]);
+          }
         }
-      }
     }
     if (NORMS) {
 #pragma unroll
@@ -226,8 +251,22 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
         x += __shfl_xor(x, 4, 64);
         x += __shfl_xor(x, 8, 64);
         const int c = c0 + 4 * v + lg;
-        if (li == 0 && c < N) normout(c, x);
+        if (ws == 1) {
+          if (li == 0 && c < N) normout(c, x);
+        } else if (li == 0) {
+          part[w * 16 + 4 * v + lg] = x;
+        }
       }
+    }
+  }
+  if (NORMS && ws > 1) {
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 16 * TNn) {
+      const int tn = t >> 4, cl = t & 15;
+      double s = 0.0;
+      for (int q = 0; q < ws; ++q) s += part[(tn * ws + q) * 16 + cl];
+      if (tn * 16 + cl < N) normout(tn * 16 + cl, s);
     }
   }
   __syncthreads();
@@ -617,10 +656,10 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
                 Cbuf[r + c * ldc] = v;
                 R[r + (int64_t)L.J[c] * ldr] += v;
               });
-        bq_nn<true, false>(
+        bq_nn<true, false, 1>(
             mp, s, i, [&](int k) { return (const double*)(Aw + (int64_t)k * mp); },
             [&](int k, int c) { return Cbuf[k + c * ldc]; },
-            [&](int c) { return Aw + (int64_t)L.J[c] * mp; }, [&](int, double) {});
+            [&](int c) { return Aw + (int64_t)L.J[c] * mp; }, [&](int, double) {}, L.part);
       }
       // ---- Gram, pivoted Cholesky, filter -------------------------------------
       bq_tn(s, s, mp, [&](int r) { return (const double*)(Aw + (int64_t)L.J[r] * mp); },
@@ -633,10 +672,10 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
         break;
       }
       // ---- CholQR2: Q1 = V(:, Jk) Rb^{-1}, G2 = Q1^T Q1, R2 -------------------
-      bq_nn<false, false>(
+      bq_nn<false, false, 1>(
           mp, bt, bt, [&](int k) { return (const double*)(Aw + (int64_t)L.Jk[k] * mp); },
           [&](int k, int c) { return L.Rinv[k * ld + c]; },
-          [&](int c) { return Qbuf + (int64_t)c * mp; }, [&](int, double) {});
+          [&](int c) { return Qbuf + (int64_t)c * mp; }, [&](int, double) {}, L.part);
       bq_tn(bt, bt, mp, [&](int r) { return (const double*)(Qbuf + (int64_t)r * mp); },
             [&](int c) { return (const double*)(Qbuf + (int64_t)c * mp); }, L.part,
             [&](int r, int c, double v) { L.G2[r * ld + c] = v; });
@@ -695,21 +734,21 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
       }
       __syncthreads();
       // ---- Q_b = Q1 R2^{-1} into the block's columns ---------------------------
-      bq_nn<false, false>(
+      bq_nn<false, false, 1>(
           mp, bt, bt, [&](int k) { return (const double*)(Qbuf + (int64_t)k * mp); },
           [&](int k, int c) { return L.R2inv[k * ld + c]; },
-          [&](int c) { return Aw + (int64_t)(i + c) * mp; }, [&](int, double) {});
+          [&](int c) { return Aw + (int64_t)(i + c) * mp; }, [&](int, double) {}, L.part);
       // ---- projection W = Q_b^T A_rem into R's block rows, then update -------
       const int nrem = n - i - bt;
       if (nrem > 0) {
         bq_tn(bt, nrem, mp, [&](int r) { return (const double*)(Aw + (int64_t)(i + r) * mp); },
               [&](int c) { return (const double*)(Aw + (int64_t)(i + bt + c) * mp); }, L.part,
               [&](int r, int c, double v) { R[(i + r) + (int64_t)(i + bt + c) * ldr] = v; });
-        bq_nn<true, true>(
+        bq_nn<true, true, 2>(
             mp, nrem, bt, [&](int k) { return (const double*)(Aw + (int64_t)(i + k) * mp); },
             [&](int k, int c) { return R[(i + k) + (int64_t)(i + bt + c) * ldr]; },
             [&](int c) { return Aw + (int64_t)(i + bt + c) * mp; },
-            [&](int c, double v) { L.diags[i + bt + c] = v; });
+            [&](int c, double v) { L.diags[i + bt + c] = v; }, L.part);
       }
       double pr = 0.0;
       for (int j = i + bt + tid; j < n; j += BQ_NT) pr += L.diags[j];
@@ -730,11 +769,11 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
       for (int rb = ((k - 1) / 16) * 16; rb >= 0; rb -= 16) {
         const int re = min(rb + 16, k);
         if (re < k) {  // X[rb:re, :] -= R[rb:re, re:k] X[re:k, :]
-          bq_nn<true, false>(
+          bq_nn<true, false, 1>(
               re - rb, nc, k - re,
               [&](int kk) { return (const double*)(R + rb + (int64_t)(re + kk) * ldr); },
               [&](int kk, int c) { return P.X[(re + kk) + (int64_t)c * P.ldx]; },
-              [&](int c) { return P.X + rb + (int64_t)c * P.ldx; }, [&](int, double) {});
+              [&](int c) { return P.X + rb + (int64_t)c * P.ldx; }, [&](int, double) {}, L.part);
         }
         // the diagonal triangle, one column per thread
         for (int c = tid; c < nc; c += BQ_NT) {
