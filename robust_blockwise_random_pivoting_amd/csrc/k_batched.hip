@@ -272,6 +272,125 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
   __syncthreads();
 }
 
+// Projection and update of the trailing columns fused per 16-column tile, for
+// bt <= 16 kept columns and N >= 49 trailing columns (every wave owns a
+// tile): wave w takes the tiles w, w + 4, ...; for each it forms
+// W = Q^T C(:, tile) over all K rows (the loop of bq_tn), stores W into
+// Wout (row r = kept column, leading dimension ldw), and at once updates the
+// same columns C(:, tile) -= Q W with their new squared norms (the loop of
+// bq_nn with BQ_RT = 2): the MFMA output of the projection, lane (j, g)
+// holding W(4v + g, c0 + j), is already the B fragment of the update.  The
+// arithmetic is that of bq_tn followed by bq_nn, without the barrier between
+// them, and the second pass over a tile follows its first while its rows
+// are still in the caches.  Q (K x bt) and C (K x N) have leading dimension K
+// (a multiple of 16, rows >= rows zero in Q); rows >= rows of C are neither
+// read nor written by the update.
+template <class FN>
+__device__ void bq_proj_update(int rows, int K, int bt, int N, const double* Q, double* C,
+                               double* Wout, int64_t ldw, FN normout) {
+  constexpr int RT = 2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int TNn = (N + 15) >> 4, nrt = (rows + 15) >> 4;
+  const double* pa = Q + (int64_t)min(li, bt - 1) * K;
+  for (int tn = w; tn < TNn; tn += 4) {
+    const int c0 = tn * 16;
+    const bool cok = c0 + li < N;
+    const double* pb = C + (int64_t)min(c0 + li, N - 1) * K;
+    // ---- W(r, c0 + li) for r = 4v + lg
+    v4d acc = {0.0, 0.0, 0.0, 0.0}, acc2 = acc;
+    int k0 = 0;
+    for (; k0 + 64 <= K; k0 += 64) {
+      double2 a[8], bv[8];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[2 * u] = *(const double2*)(pa + k0 + 16 * u + 4 * lg);
+        a[2 * u + 1] = *(const double2*)(pa + k0 + 16 * u + 4 * lg + 2);
+        bv[2 * u] = *(const double2*)(pb + k0 + 16 * u + 4 * lg);
+        bv[2 * u + 1] = *(const double2*)(pb + k0 + 16 * u + 4 * lg + 2);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].x, bv[u].x, acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u].y, bv[u].y, acc2, 0, 0, 0);
+      }
+    }
+    for (; k0 < K; k0 += 16) {
+      const double2 a0 = *(const double2*)(pa + k0 + 4 * lg);
+      const double2 a1 = *(const double2*)(pa + k0 + 4 * lg + 2);
+      const double2 b0 = *(const double2*)(pb + k0 + 4 * lg);
+      const double2 b1 = *(const double2*)(pb + k0 + 4 * lg + 2);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, b0.x, acc, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, b0.y, acc2, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.x, b1.x, acc, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, b1.y, acc2, 0, 0, 0);
+    }
+    acc += acc2;
+    // rows r >= bt of acc come from the clamped column of Q: zero them for the
+    // update; the kept rows go to Wout
+    double bq[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 4 * v + lg;
+      bq[v] = (r < bt && cok) ? acc[v] : 0.0;
+      if (r < bt && cok) Wout[r + (int64_t)(c0 + li) * ldw] = acc[v];
+    }
+    // ---- C(:, tile) -= Q W(:, tile), with the new squared norms
+    double nsq[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < nrt; t0 += RT) {
+      int rr[RT];
+      bool rok[RT];
+#pragma unroll
+      for (int u = 0; u < RT; ++u) {
+        rr[u] = (t0 + u) * 16 + li;
+        rok[u] = rr[u] < rows;
+      }
+      double cv[RT][4], av[RT][4];
+#pragma unroll
+      for (int u = 0; u < RT; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          cv[u][v] = rok[u] ? C[rr[u] + (int64_t)min(c0 + 4 * v + lg, N - 1) * K] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * q + lg;
+#pragma unroll
+        for (int u = 0; u < RT; ++u) av[u][q] = (k < bt && rok[u]) ? Q[rr[u] + (int64_t)k * K] : 0.0;
+      }
+      v4d up[RT];
+#pragma unroll
+      for (int u = 0; u < RT; ++u) up[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < RT; ++u)
+          up[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(bq[q], av[u][q], up[u], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < RT; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int c = c0 + 4 * v + lg;
+          if (c < N && rok[u]) {
+            const double nv = cv[u][v] - up[u][v];
+            C[rr[u] + (int64_t)c * K] = nv;
+            nsq[v] = fma(nv, nv, nsq[v]);
+          }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double x = nsq[v];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      x += __shfl_xor(x, 8, 64);
+      const int c = c0 + 4 * v + lg;
+      if (li == 0 && c < N) normout(c, x);
+    }
+  }
+  __syncthreads();
+}
+
 // *dst(idx) = *src(idx) for idx < count over the workgroup's threads, the
 // loads of four iterations issued before their stores (a copy otherwise
 // waits for each load in turn).  Source and destination must not overlap.
@@ -740,7 +859,11 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
           [&](int c) { return Aw + (int64_t)(i + c) * mp; }, [&](int, double) {}, L.part);
       // ---- projection W = Q_b^T A_rem into R's block rows, then update -------
       const int nrem = n - i - bt;
-      if (nrem > 0) {
+      if (bt <= 16 && nrem >= 49) {
+        bq_proj_update(mp, mp, bt, nrem, Aw + (int64_t)i * mp, Aw + (int64_t)(i + bt) * mp,
+                       R + i + (int64_t)(i + bt) * ldr, ldr,
+                       [&](int c, double v) { L.diags[i + bt + c] = v; });
+      } else if (nrem > 0) {
         bq_tn(bt, nrem, mp, [&](int r) { return (const double*)(Aw + (int64_t)(i + r) * mp); },
               [&](int c) { return (const double*)(Aw + (int64_t)(i + bt + c) * mp); }, L.part,
               [&](int r, int c, double v) { R[(i + r) + (int64_t)(i + bt + c) * ldr] = v; });
