@@ -75,6 +75,21 @@ int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int fil
                     const int32_t* J, int32_t* Jk, double* Rb, int32_t* Jin, int32_t* Jout,
                     int32_t* bt, int32_t* nswap, int32_t* status);
 
+/* The same panel step with the kept factor's inverse: Rinv (device, b x b,
+ * ld b) receives Rb(0:bt, 0:bt)^{-1}, zero below the diagonal. */
+int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int filter,
+                        int64_t i, const int32_t* J, int32_t* Jk, double* Rb, double* Rinv,
+                        int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
+                        int32_t* status);
+
+/* Second Cholesky-QR pass of a block of bt kept columns (b >= bt the block
+ * capacity): from G2 (device, bt x bt, ld bt) and the first factor Rb (device,
+ * b x b, ld b) it writes R2 = chol(G2) and R2inv = R2^{-1} (device, b x b,
+ * ld b) and Rout = R2 * Rb (device, bt x bt, ld bt); status 2 reports a
+ * non-positive pivot. */
+int rbrp_test_chol2(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
+                    double* R2, double* R2inv, double* Rout, int32_t* status);
+
 /* Sampler kernel alone: s = min(b, kmax - i, nnz(diags[i:n])) indices (absolute,
  * first-occurrence order) into J (device, capacity b); draws_in is the number
  * of uniforms already consumed; the new count and s go to host ints. */

@@ -102,6 +102,10 @@ def lib():
         "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
         "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
                                     POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+        "rbrp_test_panel_inv": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, vp, i32p,
+                                        i32p, POINTER(c_int32), POINTER(c_int32),
+                                        POINTER(c_int32)]),
+        "rbrp_test_chol2": (c_int, [vp, vp, i64, i64, vp, vp, vp, vp, POINTER(c_int32)]),
         "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
                                      POINTER(c_int32), POINTER(c_uint64)]),
     }

@@ -13,8 +13,8 @@
 // largest Schur diagonal among the columns not yet chosen, the lowest position
 // on ties (a wave max and a ballot; the position tie-break runs only when
 // values tie); the Schur diagonals are downdated as d -= R(j, c)^2 by a fused
-// multiply-add.  The triangular inverses (back substitution) and R2 * Rb use
-// the same row-dot pattern.
+// multiply-add.  The triangular inverses and R2 * Rb run as 16 x 16 block
+// products on the FP64 matrix core.
 //
 // Reference: the pivoted QR of the sampled block and its filter
 // (gpu/rbrp.cu:166-234, rbrp.hpp CPQR + filter), CholQR2 of the kept columns
@@ -181,33 +181,78 @@ __device__ __forceinline__ void pw_ldot8(const double* U, int r0, const double* 
   for (int q = 0; q < 8; ++q) acc[q] = e[q] + o[q];
 }
 
-// X = U^{-1} by back substitution, lane t computing column t, eight rows per
-// block from the bottom up.  U is bt x bt upper triangular with row r at
-// U + r * PW_LDE (zero left of the diagonal and in columns bt .. PW_N - 1, rows
-// bt .. round8(bt) zero); dinv[r] = 1 / U(r, r).  XT (lane t's row holds column
-// t of X) must be zero on entry; X(r, t) also goes to Xo[r * PW_LDO + t] for a
-// coalesced store.
-__device__ __forceinline__ void pw_upper_inverse(const double* U, const double* dinv, double* XT,
-                                                 double* Xo, int bt) {
-  const int t = threadIdx.x;
-  double* xr = XT + t * PW_LDE;
-  for (int r0 = (bt - 1) & ~7; r0 >= 0; r0 -= 8) {
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    pw_ldot8(U, r0, xr, r0 + 8, bt, acc);  // rows of X already computed
-    double x[8];
+// ---------------------------------------------------------------------------
+// 16 x 16 block products on the FP64 matrix core for the one-wave kernels.
+// v_mfma_f64_16x16x4f64 operands: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k; the result D[i][j] sits in lane j + 16 (i % 4), element i / 4.  A
+// 16-deep product is four MFMAs, step s taking k = 4 s + (lane / 16), so lane
+// (j, g) = j + 16 g holds D[4 v + g][j] in element v, which is exactly its B
+// operand of step v for a following product: a chain of products stays in
+// registers.
+// ---------------------------------------------------------------------------
+// acc += A(I, K) * B(K, J) for one 16 x 16 block pair: A's row r at
+// A + r * lda (row-major), B's column c at Bt + c * ldb (B transposed), block
+// origins (ra, ka) and (kb, cb).
+__device__ __forceinline__ v4d pw_mfma_block(v4d acc, const double* A, int lda, int ra, int ka,
+                                             const double* Bt, int ldb, int kb, int cb) {
+  const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
 #pragma unroll
-    for (int q = 7; q >= 0; --q) {
-      const int r = r0 + q;
-      double a = acc[q];
+  for (int s = 0; s < 4; ++s) {
+    const double a = A[(ra + j) * lda + ka + 4 * s + g];
+    const double b = Bt[(cb + j) * ldb + kb + 4 * s + g];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// X = U^{-1} for the n x n upper-triangular U (row r at U + r * PW_LDE, zero
+// outside its n x n block; dinv[r] = 1 / U(r, r) for r < n), written to
+// Xo[r * PW_LDO + c] for the blocks on and above the diagonal of the
+// ceil(n / 16) x ceil(n / 16) block grid (the rows and columns in
+// [n, 16 ceil(n / 16)) hold the identity; blocks below the diagonal are not
+// written).  The diagonal 16 x 16 blocks are inverted by back substitution,
+// lane group g (lanes 16 g .. 16 g + 15) on block g with lane j computing its
+// column j; then, by distance d = J - I from the diagonal,
+//   X(I, J) = -X(I, I) * sum_{K = I+1}^{J} U(I, K) X(K, J)
+// on the matrix core, the sum's MFMA output feeding the second product as
+// its B operand.
+__device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, double* Xo, int n) {
+  const int lane = threadIdx.x, j = lane & 15, g = lane >> 4;
+  const int nbk = (n + 15) >> 4;  // uniform
+  if (g < nbk) {
+    const int R0 = 16 * g;
+    double x[16];
 #pragma unroll
-      for (int p = q + 1; p < 8; ++p) a = fma(U[r * PW_LDE + r0 + p], x[p], a);
-      const double dr = dinv[r];
-      x[q] = (r < bt && r <= t) ? ((r == t) ? dr : -a * dr) : 0.0;
+    for (int r = 15; r >= 0; --r) {
+      const int R = R0 + r;
+      double a = (r == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int p = r + 1; p < 16; ++p) a = fma(-U[R * PW_LDE + R0 + p], x[p], a);
+      x[r] = (R < n) ? ((r <= j) ? a * dinv[R] : 0.0) : ((r == j) ? 1.0 : 0.0);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      xr[r0 + q] = x[q];
-      Xo[(r0 + q) * PW_LDO + t] = x[q];
+    for (int r = 0; r < 16; ++r) Xo[(R0 + r) * PW_LDO + R0 + j] = x[r];
+  }
+  for (int d = 1; d < nbk; ++d) {
+    for (int I = 0; I + d < nbk; ++I) {
+      const int J = I + d;
+      v4d y = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int K = I + 1; K <= J; ++K) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const double a = U[(16 * I + j) * PW_LDE + 16 * K + 4 * s + g];
+          const double b = Xo[(16 * K + 4 * s + g) * PW_LDO + 16 * J + j];
+          y = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, y, 0, 0, 0);
+        }
+      }
+      v4d xo = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double a = Xo[(16 * I + j) * PW_LDO + 16 * I + 4 * s + g];
+        xo = __builtin_amdgcn_mfma_f64_16x16x4f64(a, y[s], xo, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Xo[(16 * I + 4 * v + g) * PW_LDO + 16 * J + j] = -xo[v];
     }
   }
 }
@@ -356,8 +401,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   }
   if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
   pw_store_rows(Gs, bt, true, Rb, b);
-  pw_zero_row(RT);
-  pw_upper_inverse(Ws, dinv, RT, Gs, bt);
+  pw_upper_inverse_mfma(Ws, dinv, Gs, bt);
   pw_store_rows(Gs, bt, true, Rinv, b);
   // index sets: Jk (pivot order), Jin = sort(Jk) beyond the block, Jout =
   // block positions not in Jk (ascending)
@@ -478,16 +522,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   }
   if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
   pw_store_rows(Gs, bt, true, R2, b);
-  // Rout(r, c) = sum_{l >= r} R2(r, l) Rb(l, c), eight rows at a time
-  for (int r0 = 0; r0 < bt; r0 += 8) {
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    pw_ldot8(Ws, r0, Bt + c * PW_LDE, r0, bt, acc);
+  // Rout = R2 * Rb on the matrix core, block (I, J) = sum_{K=I}^{J} R2(I, K) Rb(K, J)
+  {
+    const int nbk = (bt + 15) >> 4, j = c & 15, g = c >> 4;
+    for (int I = 0; I < nbk; ++I)
+      for (int J = I; J < nbk; ++J) {
+        v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int K = I; K <= J; ++K)
+          acc = pw_mfma_block(acc, Ws, PW_LDE, 16 * I, 16 * K, Bt, PW_LDE, 16 * K, 16 * J);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Gs[(r0 + q) * PW_LDO + c] = acc[q];
+        for (int v = 0; v < 4; ++v) Gs[(16 * I + 4 * v + g) * PW_LDO + 16 * J + j] = acc[v];
+      }
   }
   pw_store_rows(Gs, bt, true, Rout, ldr);
-  pw_zero_row(RT);
-  pw_upper_inverse(Ws, dinv, RT, Gs, bt);
+  pw_upper_inverse_mfma(Ws, dinv, Gs, bt);
   pw_store_rows(Gs, bt, true, R2inv, b);
 }
 
@@ -522,8 +570,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
       Ws[(t0 + u) * PW_LDE + c] = (c < nb && t0 + u <= c) ? v[u] : 0.0;
   }
   if (c < nb) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
-  pw_zero_row(RT);
-  pw_upper_inverse(Ws, dinv, RT, Gs, nb);
+  pw_upper_inverse_mfma(Ws, dinv, Gs, nb);
   pw_store_rows(Gs, nb, true, out + (int64_t)blockIdx.x * 4096, 64);
 }
 

@@ -147,6 +147,14 @@ int rbrp_test_gemm_upd32(void* stream, int64_t M, int64_t N, int64_t K, const fl
 int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int filter, int64_t i,
                     const int32_t* J, int32_t* Jk, double* Rb, int32_t* Jin, int32_t* Jout,
                     int32_t* bt, int32_t* nswap, int32_t* status) {
+  return rbrp_test_panel_inv(stream, G, s, b, filter, i, J, Jk, Rb, nullptr, Jin, Jout, bt, nswap,
+                             status);
+}
+
+int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int filter,
+                        int64_t i, const int32_t* J, int32_t* Jk, double* Rb, double* Rinv_out,
+                        int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
+                        int32_t* status) {
   if (s < 1 || s > b) return -3;
   if (b < 1 || b > 128) return -4;
   hipStream_t st = (hipStream_t)stream;
@@ -160,12 +168,35 @@ int rbrp_test_panel(void* stream, const double* G, int64_t s, int64_t b, int fil
   if (e == hipSuccess)
     e = rbrp_launch_pivchol_filter(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk, Jin, Jout);
   if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && Rinv_out)
+    e = hipMemcpyAsync(Rinv_out, Rinv, sizeof(double) * b * b, hipMemcpyDeviceToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(d);
   (void)hipFree(Rinv);
   if (e != hipSuccess) return (int)e;
   *bt = hs.bt;
   *nswap = hs.nswap;
+  *status = hs.status;
+  return 0;
+}
+
+int rbrp_test_chol2(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
+                    double* R2, double* R2inv, double* Rout, int32_t* status) {
+  if (bt < 1 || bt > b) return -3;
+  if (b < 1 || b > 128) return -4;
+  hipStream_t st = (hipStream_t)stream;
+  RbrpBlockState* d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(RbrpBlockState)));
+  RbrpBlockState hs{};
+  hs.s = (int)bt;
+  hs.bt = (int)bt;
+  hipError_t e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = rbrp_launch_chol2(st, G2, bt, (int)b, d, Rb, R2, R2inv, Rout, bt);
+  if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(d);
+  if (e != hipSuccess) return (int)e;
   *status = hs.status;
   return 0;
 }

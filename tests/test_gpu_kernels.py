@@ -185,10 +185,11 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
     dJin = torch.zeros(b, dtype=torch.int32, device="cuda")
     dJout = torch.zeros(b, dtype=torch.int32, device="cuda")
     dRb = torch.zeros(b * b, dtype=torch.float64, device="cuda")
+    dRinv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
     bt, ns, status = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    code = librbrp.rbrp_test_panel(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk), _ptr(dRb),
-                                   _ptr(dJin), _ptr(dJout), ctypes.byref(bt), ctypes.byref(ns),
-                                   ctypes.byref(status))
+    code = librbrp.rbrp_test_panel_inv(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
+                                       _ptr(dRb), _ptr(dRinv), _ptr(dJin), _ptr(dJout),
+                                       ctypes.byref(bt), ctypes.byref(ns), ctypes.byref(status))
     assert code == 0 and status.value == 0
     _, R, piv = sla.qr(V, mode="economic", pivoting=True)
     bt_ref = robust_filter(R, s, b, "sampled")
@@ -198,10 +199,47 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
     Rb = _host(dRb).reshape(b, b).T[:bt_ref, :bt_ref]
     Rref = np.abs(R[:bt_ref, :bt_ref])
     assert np.abs(np.abs(Rb) - Rref).max() <= 1e-9 * Rref.max()
+    # the factor's inverse: upper triangular, and Rb Rinv = I to the conditioning of Rb
+    Rinv = _host(dRinv).reshape(b, b).T[:bt_ref, :bt_ref]
+    assert np.array_equal(np.tril(Rinv, -1), np.zeros_like(Rinv))
+    kappa = np.linalg.cond(Rb)
+    assert np.abs(Rb @ Rinv - np.eye(bt_ref)).max() <= 1e-14 * kappa * bt_ref
+    np.testing.assert_allclose(Rinv, sla.solve_triangular(Rb, np.eye(bt_ref)),
+                               rtol=0, atol=1e-14 * kappa * np.abs(Rinv).max() * bt_ref)
     jin, jout = symmetric_difference(i, bt_ref, Jk_ref)
     assert ns.value == len(jin)
     assert np.array_equal(_host(dJin)[:len(jin)], jin)
     assert np.array_equal(_host(dJout)[:len(jout)], jout)
+
+
+@pytest.mark.parametrize("bt,b", [(1, 8), (5, 8), (16, 64), (17, 64), (40, 64), (63, 64),
+                                  (64, 64), (100, 128), (128, 128)])
+def test_chol2_factor_inverse_product(librbrp, bt, b):
+    """Second Cholesky-QR pass: R2 = chol(G2) (upper, positive diagonal), R2inv =
+    R2^{-1}, Rout = R2 Rb, against LAPACK on a near-orthonormal block's Gram."""
+    import torch
+    rng = np.random.default_rng(bt * 131 + b)
+    Q1 = np.linalg.qr(rng.standard_normal((500, bt)))[0] + 1e-3 * rng.standard_normal((500, bt))
+    G2 = Q1.T @ Q1
+    Rb = np.triu(rng.standard_normal((bt, bt))) + 4.0 * np.eye(bt)
+    Rbf = np.zeros((b, b))
+    Rbf[:bt, :bt] = Rb
+    dG2, dRb = _cm(G2), _cm(Rbf)
+    dR2 = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
+    dR2inv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
+    dRout = torch.full((bt * bt,), np.nan, dtype=torch.float64, device="cuda")
+    status = ctypes.c_int32()
+    code = librbrp.rbrp_test_chol2(_stream(), _ptr(dG2), bt, b, _ptr(dRb), _ptr(dR2),
+                                   _ptr(dR2inv), _ptr(dRout), ctypes.byref(status))
+    assert code == 0 and status.value == 0
+    R2_ref = np.linalg.cholesky(G2).T
+    R2 = _host(dR2).reshape(b, b).T[:bt, :bt]
+    R2inv = _host(dR2inv).reshape(b, b).T[:bt, :bt]
+    Rout = _host(dRout).reshape(bt, bt).T
+    np.testing.assert_allclose(R2, R2_ref, rtol=0, atol=1e-13)
+    assert np.array_equal(np.tril(R2inv, -1), np.zeros((bt, bt)))
+    np.testing.assert_allclose(R2inv, np.linalg.inv(R2_ref), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(Rout, R2_ref @ Rb, rtol=0, atol=1e-12 * np.abs(Rb).max())
 
 
 @pytest.mark.parametrize("N,i,b,skew,seed", [(10, 0, 4, 1, 0), (4000, 0, 64, 1, 1),
