@@ -17,7 +17,9 @@ host).  Further legs in the same line:
               tol = 1e-7: one persistent launch vs one launch sequence per problem;
   scale    -- 1,000,000 x 1024 FP32-stored matrix (s_i = 1/i, 500 Cauchy rows),
               b = 128, tol = 1e-5: update GB/s and projection MFMA utilisation;
-  comparators -- RandCPQR_OS / RandLUPP_OS at the rank RBRP found.
+  comparators -- RandCPQR_OS / RandLUPP_OS at the rank RBRP found;
+  cpqr     -- the full-matrix CPQR ID at ranks of the RBRP trace (time model,
+              errors, and the Householder steps' HBM GB/s).
 """
 
 from __future__ import annotations
@@ -63,7 +65,7 @@ def parse():
                     help="write the generated matrix (column-major .npy) and exit")
     ap.add_argument("--load-matrix", default="",
                     help="read the matrix from this .npy instead of generating it")
-    ap.add_argument("--legs", default="main,comparators,mixed,batched,scale,cpu",
+    ap.add_argument("--legs", default="main,comparators,cpqr,mixed,batched,scale,cpu",
                     help="comma list of legs to run (main = the timed FP64 IDs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--all-phase-timing", action="store_true",
@@ -80,6 +82,7 @@ def parse():
     args = ap.parse_args()
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
     args.no_comparators = args.no_comparators or "comparators" not in legs
+    args.no_cpqr = "cpqr" not in legs
     args.no_mixed = args.no_mixed or "mixed" not in legs
     args.no_batched = args.no_batched or "batched" not in legs
     args.no_cpu = args.no_cpu or "cpu" not in legs
@@ -378,13 +381,15 @@ def main():
     dt = time.perf_counter() - t0
     stats = h.stats()
     if args.all_phase_timing:
+        pstats = stats
         phase_ms = {k: v["ms"] / args.steps for k, v in stats.items()}
     else:
         h.set_timing(1)
         h.reset_stats()
         rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=h)
         torch.cuda.synchronize()
-        phase_ms = {k: v["ms"] for k, v in h.stats().items()}
+        pstats = h.stats()
+        phase_ms = {k: v["ms"] for k, v in pstats.items()}
         h.set_timing(0)
 
     # independent check of the returned ID on the device
@@ -407,6 +412,18 @@ def main():
                               frac=ach / PEAK_HBM_GBS, avg_ms=avg_ms, launches=st["launches"],
                               bytes_per_launch=st["bytes"] / st["launches"])
     dom = max(kern, key=lambda k: stats[k]["ms"]) if kern else None
+    # the initial column-norm pass (fused with the working copy: e*m*n read,
+    # e*mp*n written) and the per-block sampler (weights read, CDF written),
+    # timed by the phase events of the breakdown ID
+    for name in ("norms", "sample"):
+        st = pstats.get(name)
+        if not st or st["launches"] == 0 or st["ms"] <= 0:
+            continue
+        ach = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+        kern[name] = dict(bound="hbm", achieved=ach, peak=PEAK_HBM_GBS, unit="GB/s",
+                          frac=ach / PEAK_HBM_GBS, avg_ms=st["ms"] / st["launches"],
+                          launches=st["launches"], bytes_per_launch=st["bytes"] / st["launches"],
+                          timing="phase events, breakdown ID")
     roof = None
     if dom:
         d = kern[dom]
@@ -435,6 +452,28 @@ def main():
                            "rel_error": (torch.linalg.norm(E) / torch.linalg.norm(A)).item()}
             del E
             print(f"[bench] {name}: {dtc * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    # full-matrix CPQR ID (gpu/id_cpqr.cu:30-172) at ranks of the RBRP trace,
+    # as the reference's driver compares them (gpu/driver.cpp:197-205)
+    cpqr = None
+    if not args.no_cpqr:
+        from robust_blockwise_random_pivoting_amd import CPQR
+        nb = len(res.ranks)
+        idx = sorted({max(nb // 4 - 1, 0), max(nb // 2 - 1, 0), max(3 * nb // 4 - 1, 0), nb - 1})
+        ranks = [int(res.ranks[t]) for t in idx]
+        print(f"[bench] cpqr at ranks {ranks} ...", file=sys.stderr, flush=True)
+        e_q, t_q, t_full, fl_q, _ = CPQR(A, ranks, handle=h)
+        q = min(m, n)
+        # unblocked Householder steps: step j reads and writes the (m-j) x (n-j-1) trailing block
+        qbytes = sum(16.0 * (m - j) * (n - j - 1) for j in range(q))
+        cpqr = {"m": m, "n": n, "t_cpqr_ms": t_full * 1e3, "ranks": ranks,
+                "rel_error": [float(x) ** 0.5 for x in e_q],
+                "t_ranks_ms": [x * 1e3 for x in t_q], "flops": fl_q,
+                "rbrp_rel_error": [float(res.error[t]) ** 0.5 for t in idx],
+                "rbrp_t_ranks_ms": [float(res.t_ranks[t]) * 1e3 for t in idx],
+                "qrcp_GBs": qbytes / t_full / 1e9, "qrcp_hbm_frac": qbytes / t_full / 1e9 / PEAK_HBM_GBS,
+                "qrcp_bytes": qbytes}
+        print(f"[bench] cpqr: {t_full * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     mixed = None if args.no_mixed else mixed_leg(A, args)
     batched = None if args.no_batched else batched_leg(args)
@@ -470,6 +509,7 @@ def main():
         "kernels": kern,
         "phase_ms_per_step": phase_ms,
         "comparators": comps,
+        "cpqr": cpqr,
         "mixed": mixed,
         "batched": batched,
         "scale": scale,

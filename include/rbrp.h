@@ -156,6 +156,24 @@ int rbrp_randlupp_os_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_
                        int32_t* d_sk, int32_t* d_rd, double* dT, int64_t ldt);
 
 /*
+ * Full-matrix CPQR ID baseline.  Replaces `void CPQR(int m, int n, double *A,
+ * const std::vector<int> ranks, std::vector<double> &err, std::vector<double>
+ * &t_ranks, double &t_cpqr, double &flops)` (gpu/id_cpqr.hpp:6-8,
+ * gpu/id_cpqr.cu:30-172): Householder QR with column pivoting (first maximum
+ * of the trailing squared column norms, dgeqp3's pivot rule) of a copy of A
+ * (A is not written) to min(m, n) steps, timed into *t_cpqr (seconds).  For
+ * each ranks[t] (1 <= r <= min(m, n)): T = R11^{-1} R12 and error[t] =
+ * ||A(:, P(r:n)) - A(:, P(0:r)) T||_F^2 / ||A||_F^2 (squared, as the
+ * reference); t_ranks[t] = the partial-factorization time model of
+ * id_cpqr.cu:93-104 plus the solve's time.  *flops = FLOPS_DGEQRF(m, n) +
+ * sum_t r^2 (n - r).  error and t_ranks are host arrays of nranks; d_perm
+ * (optional, device, n entries) receives the pivot order (0-based).
+ */
+int rbrp_cpqr_id_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                   int64_t nranks, const int32_t* ranks, double* error, double* t_ranks,
+                   double* t_cpqr, double* flops, int32_t* d_perm);
+
+/*
  * Oversampled interpolation matrix.  Replaces Compute_interpolation_matrix
  * (gpu/interp.cu:45-105): Yp = Y(:, P); QR of Yp(:, 0:k); T = R^{-1} Q^T Yp(:, k:n)
  * (k x (n-k), leading dimension ldt).  Y is l x n (ldy); P (device) has n entries.

@@ -165,6 +165,27 @@ inline void RandCPQR_OS(const double* A, int m, int n, int rank, int* h_sk, int*
   flops = 2.0 * mm * nn * k + rbrp_detail::flops_dgeqrf(k, nn) + k * k * (nn - k);
 }
 
+// gpu/id_cpqr.hpp:6-8.  A is an m x n column-major FP64 device pointer (lda = m),
+// left unchanged.  Full column-pivoted QR of A, then for each rank r in ranks
+// (1 <= r <= min(m, n); the reference asserts 0 < r < min(m, n)): err = the
+// squared relative error of the CPQR ID of rank r, t_ranks = the reference's
+// partial-factorization time model plus the solve time, t_cpqr = the
+// factorization's seconds, flops = FLOPS_DGEQRF(m, n) + sum r^2 (n - r).
+inline void CPQR(int m, int n, double* A, const std::vector<int> ranks, std::vector<double>& err,
+                 std::vector<double>& t_ranks, double& t_cpqr, double& flops) {
+  err.assign(ranks.size(), 0.0);
+  t_ranks.assign(ranks.size(), 0.0);
+  t_cpqr = 0.0;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (!st)
+    st = rbrp_cpqr_id_d(h, A, m, n, m, (int64_t)ranks.size(), ranks.data(), err.data(),
+                        t_ranks.data(), &t_cpqr, &flops, nullptr);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+}
+
 // gpu/rid.hpp:12-13.  sk, rd (device int arrays) and T (device, rank x
 // (n - rank), leading dimension rank) are allocated here with hipMalloc; the
 // caller frees them with hipFree.  flops as randLUPP_OS.cu:100.

@@ -26,3 +26,4 @@ from .sketch_oracle import (  # noqa: F401
     rand_lupp_os,
     pivots_to_permutation,
 )
+from .cpqr_oracle import cpqr_id, partial_time_fraction  # noqa: F401

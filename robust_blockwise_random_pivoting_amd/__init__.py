@@ -6,8 +6,8 @@ package does not load the library; the first call does, and raises
 ImportError if it has not been built.
 """
 
-from .rbrp import (RBRP, Batch, ColID, Handle, RandCPQR_OS, RandLUPP_OS, batched_tune,  # noqa: F401
+from .rbrp import (CPQR, RBRP, Batch, ColID, Handle, RandCPQR_OS, RandLUPP_OS, batched_tune,  # noqa: F401
                    colmajor, gaussian_sketch, interp_os, rbrp_colid, rbrp_colid_batched, uniforms)
 
-__all__ = ["RBRP", "Batch", "ColID", "Handle", "RandCPQR_OS", "RandLUPP_OS", "batched_tune", "colmajor",
+__all__ = ["CPQR", "RBRP", "Batch", "ColID", "Handle", "RandCPQR_OS", "RandLUPP_OS", "batched_tune", "colmajor",
            "gaussian_sketch", "interp_os", "rbrp_colid", "rbrp_colid_batched", "uniforms"]

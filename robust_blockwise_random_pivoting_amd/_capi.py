@@ -92,6 +92,7 @@ def lib():
         "rbrp_randlupp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
                                        i64]),
         "rbrp_interp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, i64]),
+        "rbrp_cpqr_id_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
         "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
         "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),

@@ -21,7 +21,7 @@ LIB = os.path.join(PKG, "librbrp.so")
 CXX_CHECK = os.path.join(PKG, "bin", "rbrp_hpp_check")
 
 SOURCES = ["k_gemm.hip", "k_rbrp.hip", "k_sketch.hip", "k_mixed.hip", "k_batched.hip", "k_panel.hip",
-           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_batched.cpp", "rbrp_testing.cpp"]
+           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_batched.cpp", "rbrp_testing.cpp"]
 
 ARCH = os.environ.get("RBRP_OFFLOAD_ARCH", "gfx950")
 

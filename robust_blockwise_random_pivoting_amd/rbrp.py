@@ -359,6 +359,37 @@ def RandCPQR_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
     return sk, rd[: n - rank], Tb[: n - rank].t()
 
 
+def CPQR(A, ranks, handle: Optional[Handle] = None):
+    """Mirror of ``CPQR(m, n, A, ranks, err, t_ranks, t_cpqr, flops)`` (gpu/id_cpqr.hpp:6-8).
+
+    Full-matrix column-pivoted QR of ``A`` (device, FP64, m x n), then for each
+    rank r in ``ranks`` the interpolation matrix R11^{-1} R12 and its squared
+    relative error (gpu/id_cpqr.cu:30-172).  Returns ``(error, t_ranks, t_cpqr,
+    flops, perm)``: host lists of len(ranks), the factorization's seconds, the
+    flop count and the device pivot order (int32, n entries).
+    """
+    torch = _torch()
+    A = colmajor(A)
+    m, n = A.shape
+    rk = np.ascontiguousarray(np.asarray(ranks, dtype=np.int32))
+    err = np.zeros(max(len(rk), 1))
+    tr = np.zeros(max(len(rk), 1))
+    tq, fl = ctypes.c_double(), ctypes.c_double()
+    perm = torch.empty(n, dtype=torch.int32, device=A.device)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_cpqr_id_d", h._lib.rbrp_cpqr_id_d(
+            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), len(rk),
+            ctypes.c_void_p(rk.ctypes.data), ctypes.c_void_p(err.ctypes.data),
+            ctypes.c_void_p(tr.ctypes.data), ctypes.byref(tq), ctypes.byref(fl),
+            ctypes.c_void_p(perm.data_ptr())))
+    finally:
+        if own:
+            h.close()
+    return list(err[: len(rk)]), list(tr[: len(rk)]), tq.value, fl.value, perm
+
+
 def RandLUPP_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
                 handle: Optional[Handle] = None):
     """Mirror of ``RandLUPP_OS(A, m, n, rank, sk, rd, T, flops)`` (gpu/rid.hpp:12-13).

@@ -77,6 +77,15 @@ int main() {
   if (lT) (void)hipFree(lT);
   if (lsk) (void)hipFree(lsk);
   if (lrd) (void)hipFree(lrd);
+  // full-matrix CPQR ID at half the rank and at the rank the ID found
+  std::vector<double> qerr, qt;
+  double tq = 0, qfl = 0;
+  CPQR(m, n, dA, std::vector<int>{k / 2 > 0 ? k / 2 : 1, k}, qerr, qt, tq, qfl);
+  const int st5 = RBRP_last_status();
+  std::printf("{\"status5\": %d, \"cpqr_sq_errors\": [%.6e, %.6e], \"t_cpqr\": %.6e, "
+              "\"t_ranks\": [%.6e, %.6e], \"cpqr_flops\": %.6e}\n",
+              st5, qerr.size() > 1 ? qerr[0] : -1.0, qerr.size() > 1 ? qerr[1] : -1.0, tq,
+              qt.size() > 1 ? qt[0] : -1.0, qt.size() > 1 ? qt[1] : -1.0, qfl);
   std::printf("{\"status3\": %d, \"status4\": %d, \"cpqr_os_error\": %.6e, \"lupp_os_error\": %.6e}\n",
               st3, st4, ecp, elu);
   std::printf("{\"status1\": %d, \"final_rank\": %d, \"nblocks\": %zu, \"status2\": %d, \"k\": %d, "
@@ -84,5 +93,5 @@ int main() {
               st1, ranks.empty() ? 0 : ranks.back(), ranks.size(), st2, k, std::sqrt(num / den), flops);
   if (X) (void)hipFree(X);
   (void)hipFree(dA);
-  return (st1 == 0 && st2 == 0 && st3 == 0 && st4 == 0) ? 0 : 1;
+  return (st1 == 0 && st2 == 0 && st3 == 0 && st4 == 0 && st5 == 0) ? 0 : 1;
 }

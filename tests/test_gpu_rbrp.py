@@ -208,3 +208,10 @@ def test_cxx_call_surface_drop_in():
     assert cmp["status3"] == 0 and cmp["status4"] == 0
     # the sketch IDs at the same rank are accurate to within a modest factor
     assert 0 <= cmp["cpqr_os_error"] <= 1e-4 and 0 <= cmp["lupp_os_error"] <= 1e-4
+    # the full-matrix CPQR ID: its error at the ID's rank is at most the ID's
+    # tolerance-level error (up to a modest factor), and decreases with rank
+    cq = json.loads(lines[-3])
+    assert cq["status5"] == 0
+    e_half, e_full = cq["cpqr_sq_errors"]
+    assert 0 <= e_full <= e_half and np.sqrt(e_full) <= 1e-4
+    assert 0 < cq["t_ranks"][0] <= cq["t_ranks"][1] and cq["t_cpqr"] > 0
