@@ -174,6 +174,23 @@ int rbrp_cpqr_id_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n,
                    double* t_cpqr, double* flops, int32_t* d_perm);
 
 /*
+ * Test matrices of the reference's driver, generated on the host (no GPU
+ * needed) into A (column-major, leading dimension = its row count) from a
+ * seeded std::mt19937 stream.  Replace Matrix_GE, Matrix_SNN and Matrix_GMM
+ * (gpu/driver.cpp:23-47, :50-91, :93-111), which draw from unseeded
+ * generators: same families, not the same matrices.
+ *   GE:  n x n, U diag(s) V^T with U, V the Q factors of uniform [-1, 1]
+ *        matrices, s_i = 1 (i < 100), max(0.8^(i-100), 1e-5) after.
+ *   SNN: n x n, (U D) V with sparse U, V (ceil(s n) random entries per
+ *        column on average, uniform [0, 1)), D = diag(0.1 (i < 100), 1/(i-99)).
+ *   GMM: d x n, N(0, 1) plus 10 (j + 1) on row j of the n/100 columns of
+ *        cluster j (100 clusters); needs n >= 100 and d > (n - 1) / (n / 100).
+ */
+int rbrp_matrix_ge_h(int64_t n, uint64_t seed, double* A);
+int rbrp_matrix_snn_h(int64_t n, double s, uint64_t seed, double* A);
+int rbrp_matrix_gmm_h(int64_t n, int64_t d, uint64_t seed, double* A);
+
+/*
  * Oversampled interpolation matrix.  Replaces Compute_interpolation_matrix
  * (gpu/interp.cu:45-105): Yp = Y(:, P); QR of Yp(:, 0:k); T = R^{-1} Q^T Yp(:, k:n)
  * (k x (n-k), leading dimension ldt).  Y is l x n (ldy); P (device) has n entries.

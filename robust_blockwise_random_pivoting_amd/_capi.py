@@ -93,6 +93,9 @@ def lib():
                                        i64]),
         "rbrp_interp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, i64]),
         "rbrp_cpqr_id_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
+        "rbrp_matrix_ge_h": (c_int, [i64, u64, vp]),
+        "rbrp_matrix_snn_h": (c_int, [i64, c_double, u64, vp]),
+        "rbrp_matrix_gmm_h": (c_int, [i64, i64, u64, vp]),
         "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
         "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),

@@ -19,9 +19,10 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "librbrp.so")
 CXX_CHECK = os.path.join(PKG, "bin", "rbrp_hpp_check")
+DRIVER = os.path.join(PKG, "bin", "rbrp_driver")
 
 SOURCES = ["k_gemm.hip", "k_rbrp.hip", "k_sketch.hip", "k_mixed.hip", "k_batched.hip", "k_panel.hip",
-           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_batched.cpp", "rbrp_testing.cpp"]
+           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_matrices.cpp", "rbrp_batched.cpp", "rbrp_testing.cpp"]
 
 ARCH = os.environ.get("RBRP_OFFLOAD_ARCH", "gfx950")
 
@@ -75,6 +76,13 @@ def build(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"rbrp.hpp check failed to build:\n{r.stdout}\n{r.stderr}")
+    # the experiment driver (gpu/driver.cpp's flags and output) over include/rbrp.hpp
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(PKG, "driver", "rbrp_driver.cpp"), "-L", PKG, "-lrbrp",
+           "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"rbrp_driver failed to build:\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB
