@@ -17,7 +17,8 @@ host).  Further legs in the same line:
               tol = 1e-7: one persistent launch vs one launch sequence per problem;
   scale    -- 1,000,000 x 1024 FP32-stored matrix (s_i = 1/i, 500 Cauchy rows),
               b = 128, tol = 1e-5: update GB/s and projection MFMA utilisation;
-  comparators -- RandCPQR_OS / RandLUPP_OS at the rank RBRP found;
+  comparators -- RandCPQR_OS / RandLUPP_OS (and RandCPQR / RandLUPP, without
+              oversampling) at the rank RBRP found;
   cpqr     -- the full-matrix CPQR ID at ranks of the RBRP trace (time model,
               errors, and the Householder steps' HBM GB/s).
 """
@@ -437,8 +438,9 @@ def main():
     # sketch of 2k rows, oversampling 2, interpolation as gpu/interp.cu
     comps = {}
     if not args.no_comparators:
-        from robust_blockwise_random_pivoting_amd import RandCPQR_OS, RandLUPP_OS
-        for name, fn in (("randcpqr_os", RandCPQR_OS), ("randlupp_os", RandLUPP_OS)):
+        from robust_blockwise_random_pivoting_amd import RandCPQR, RandCPQR_OS, RandLUPP, RandLUPP_OS
+        for name, fn in (("randcpqr_os", RandCPQR_OS), ("randlupp_os", RandLUPP_OS),
+                         ("randcpqr", RandCPQR), ("randlupp", RandLUPP)):
             fn(A, res.k, seed=args.seed, handle=h)            # warm-up (workspace growth)
             torch.cuda.synchronize()
             t1 = time.perf_counter()

@@ -191,6 +191,29 @@ int rbrp_matrix_snn_h(int64_t n, double s, uint64_t seed, double* A);
 int rbrp_matrix_gmm_h(int64_t n, int64_t d, uint64_t seed, double* A);
 
 /*
+ * Randomised CPQR ID without oversampling.  Replaces `void RandCPQR(const
+ * double *A, int m, int n, int rank, int *h_sk, int *h_rd, double *&T, double&
+ * flops)` (gpu/rid.hpp:16-17, gpu/randCPQR.cu:15-67): Y = G A with G (r x m) ~
+ * N(0, 1/m); pivoted QR of all of Y gives sk = jpvt[0:r], rd = jpvt[r:n] (host
+ * arrays); T = R11^{-1} R12 from the same factor (r x (n-r), device, ldt).
+ * dG (r x m, ldg) is an optional caller sketch; NULL draws it from (seed, offset).
+ */
+int rbrp_randcpqr_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                    int64_t rank, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                    int32_t* h_sk, int32_t* h_rd, double* dT, int64_t ldt);
+
+/*
+ * Randomised LUPP ID without oversampling.  Replaces `void RandLUPP(const
+ * double *A, int m, int n, int rank, int *&sk, int *&rd, double *&T, double&
+ * flops)` (gpu/rid.hpp:9-10, gpu/randLUPP.cu:13-107): Y = G A (G is k x m); LU
+ * with partial pivoting of Y^T (n x k) = P^T [L11; L21] U; sk = P[0:k],
+ * rd = P[k:n] (device arrays, caller-allocated); T = (L21 L11^{-1})^T.
+ */
+int rbrp_randlupp_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                    int64_t rank, uint64_t seed, uint64_t offset, const double* dG, int64_t ldg,
+                    int32_t* d_sk, int32_t* d_rd, double* dT, int64_t ldt);
+
+/*
  * Oversampled interpolation matrix.  Replaces Compute_interpolation_matrix
  * (gpu/interp.cu:45-105): Yp = Y(:, P); QR of Yp(:, 0:k); T = R^{-1} Q^T Yp(:, k:n)
  * (k x (n-k), leading dimension ldt).  Y is l x n (ldy); P (device) has n entries.

@@ -186,6 +186,44 @@ inline void CPQR(int m, int n, double* A, const std::vector<int> ranks, std::vec
   rbrp_detail::last_status() = st;
 }
 
+// gpu/rid.hpp:16-17.  As RandCPQR_OS without oversampling: the sketch G is
+// rank x m and T comes from its own pivoted-QR factor.  flops as
+// randCPQR.cu:67: 2 m n r + FLOPS_DGEQRF(r, n) + r^2 (n - r).
+inline void RandCPQR(const double* A, int m, int n, int rank, int* h_sk, int* h_rd, double*& T,
+                     double& flops, uint64_t seed = 0) {
+  T = nullptr;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (!st && n > rank) st = (int)hipMalloc(&T, sizeof(double) * (size_t)rank * (size_t)(n - rank));
+  if (!st) st = rbrp_randcpqr_d(h, A, m, n, m, rank, seed, 0, nullptr, 0, h_sk, h_rd, T, rank);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  const double k = rank, mm = m, nn = n;
+  flops = 2.0 * mm * nn * k + rbrp_detail::flops_dgeqrf(k, nn) + k * k * (nn - k);
+}
+
+// gpu/rid.hpp:9-10.  As RandLUPP_OS without oversampling (G is rank x m,
+// T = (L21 L11^{-1})^T); sk, rd and T are allocated here with hipMalloc and
+// released by the caller with hipFree.  flops as randLUPP.cu:109.
+inline void RandLUPP(const double* A, int m, int n, int rank, int*& sk, int*& rd, double*& T,
+                     double& flops, uint64_t seed = 0) {
+  sk = rd = nullptr;
+  T = nullptr;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int st = rbrp_create(&h, nullptr);
+  if (!st) st = (int)hipMalloc(&sk, sizeof(int) * (size_t)rank);
+  if (!st && n > rank) st = (int)hipMalloc(&rd, sizeof(int) * (size_t)(n - rank));
+  if (!st && n > rank) st = (int)hipMalloc(&T, sizeof(double) * (size_t)rank * (size_t)(n - rank));
+  if (!st)
+    st = rbrp_randlupp_d(h, A, m, n, m, rank, seed, 0, nullptr, 0, (int32_t*)sk, (int32_t*)rd, T, rank);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+  const double k = rank, mm = m, nn = n;
+  flops = 2.0 * mm * nn * k + 2.0 * mm * k * k / 3.0 + k * k * (mm - k);
+}
+
 // gpu/rid.hpp:12-13.  sk, rd (device int arrays) and T (device, rank x
 // (n - rank), leading dimension rank) are allocated here with hipMalloc; the
 // caller frees them with hipFree.  flops as randLUPP_OS.cu:100.

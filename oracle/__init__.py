@@ -24,6 +24,8 @@ from .sketch_oracle import (  # noqa: F401
     compute_interpolation_matrix_os,
     rand_cpqr_os,
     rand_lupp_os,
+    rand_cpqr,
+    rand_lupp,
     pivots_to_permutation,
 )
 from .cpqr_oracle import cpqr_id, partial_time_fraction  # noqa: F401

@@ -68,3 +68,35 @@ def rand_lupp_os(A, k: int, G):
     P = pivots_to_permutation(ipiv, np.arange(n))
     T = compute_interpolation_matrix_os(Y, P, k)
     return P[:k].copy(), P[k:].copy(), T
+
+
+def rand_cpqr(A, r: int, G):
+    """Randomised CPQR ID without oversampling; returns ``(sk, rd, T)``.
+
+    gpu/randCPQR.cu (spec matlab/randCPQR.m): ``Y = G A`` (:42, G is r x m),
+    pivoted QR of all of ``Y`` (:52), ``sk = jpvt[:r]``, ``rd = jpvt[r:]``
+    (:70-72), ``T = R11^{-1} R12`` from that factor (:60-64).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    Y = np.asarray(G, dtype=np.float64) @ A
+    _, R, jpvt = sla.qr(Y, mode="economic", pivoting=True)
+    T = sla.solve_triangular(R[:r, :r], R[:r, r:], lower=False)
+    return jpvt[:r].copy(), jpvt[r:].copy(), T
+
+
+def rand_lupp(A, k: int, G):
+    """Randomised LUPP ID without oversampling; returns ``(sk, rd, T)``.
+
+    gpu/randLUPP.cu (spec matlab/randLUPP.m): ``Y = G A`` (:43, G is k x m), LU
+    with partial pivoting of ``Y^T`` (:47-78), permutation from the pivots
+    (:82-85), ``T = (L21 L11^{-1})^T`` (:89-106).
+    """
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[1]
+    Y = np.asarray(G, dtype=np.float64) @ A
+    LU, ipiv = sla.lu_factor(Y.T.copy(), check_finite=False)
+    P = pivots_to_permutation(ipiv, np.arange(n))
+    L11 = np.tril(LU[:k, :k], -1) + np.eye(k)
+    L21 = LU[k:, :k]
+    T = sla.solve_triangular(L11, L21.T, lower=True, trans="T", unit_diagonal=True)
+    return P[:k].copy(), P[k:].copy(), T

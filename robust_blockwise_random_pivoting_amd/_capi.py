@@ -91,6 +91,8 @@ def lib():
                                        i64]),
         "rbrp_randlupp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
                                        i64]),
+        "rbrp_randcpqr_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp, i64]),
+        "rbrp_randlupp_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp, i64]),
         "rbrp_interp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, i64]),
         "rbrp_cpqr_id_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
         "rbrp_matrix_ge_h": (c_int, [i64, u64, vp]),

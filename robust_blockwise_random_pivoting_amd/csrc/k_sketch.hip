@@ -82,6 +82,20 @@ hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, con
   return hipGetLastError();
 }
 
+// R (k x k, ld) <- its strictly upper part with a unit diagonal and zeros
+// below: the unit upper-triangular factor L^T of an LU whose transposed block
+// shares storage with U^T.
+__global__ void k_unit_upper(int64_t k, double* __restrict__ R, int64_t ld) {
+  const int64_t c = blockIdx.x;
+  for (int64_t r = c + threadIdx.x; r < k; r += blockDim.x) R[r + c * ld] = (r == c) ? 1.0 : 0.0;
+}
+
+hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld) {
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unit_upper, dim3((unsigned)k), dim3(256), 0, st, k, R, ld);
+  return hipGetLastError();
+}
+
 // vn[c] = sum_{i < rows} W(i, c)^2, one workgroup per column.
 __global__ __launch_bounds__(SK_NT) void k_colnorm2(int64_t rows, const double* __restrict__ W,
                                                     int64_t ld, double* __restrict__ vn) {

@@ -253,6 +253,7 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
                                    int64_t lds, const int* perm, double* dst, int64_t ldd);
 hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                  int64_t lds, double* dst, int64_t ldd);
+hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld);
 hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, const double* W,
                                 int64_t ld, double* vn);
 hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,

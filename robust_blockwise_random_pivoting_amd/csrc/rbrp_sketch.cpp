@@ -52,6 +52,47 @@ hipError_t interp_from_permuted(hipStream_t st, int64_t l, int64_t n, int64_t k,
   return hipSuccess;
 }
 
+// LU with partial pivoting of Yt (n x k, ldy = n) in place, then the row
+// interchanges applied in order to the identity (gpu/permute.cu:8-23); the
+// permutation goes to P (device, n entries), its first k entries to d_sk and
+// the rest to d_rd.
+hipError_t lupp_permutation(hipStream_t st, int64_t n, int64_t k, double* Yt, int* ipiv, int* P,
+                            int32_t* d_sk, int32_t* d_rd) {
+  const int64_t steps = std::min(n, k);
+  hipError_t e;
+  for (int64_t j = 0; j < steps; ++j) {
+    if ((e = rbrp_launch_lu_pivot(st, n, k, j, Yt, n, ipiv)) != hipSuccess) return e;
+    if ((e = rbrp_launch_lu_update(st, n, k, j, Yt, n)) != hipSuccess) return e;
+  }
+  std::vector<int> hpiv((size_t)steps), hP((size_t)n);
+  if ((e = hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * steps, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  for (int64_t q = 0; q < n; ++q) hP[q] = (int)q;
+  for (int64_t j = 0; j < steps; ++j) std::swap(hP[j], hP[hpiv[j]]);
+  if ((e = hipMemcpyAsync(P, hP.data(), sizeof(int) * n, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(d_sk, P, sizeof(int) * k, hipMemcpyDeviceToDevice, st)) != hipSuccess) return e;
+  if (n > k && (e = hipMemcpyAsync(d_rd, P + k, sizeof(int) * (n - k), hipMemcpyDeviceToDevice, st)) != hipSuccess)
+    return e;
+  // (hP is a stack vector: wait for the upload before it goes out of scope)
+  return hipStreamSynchronize(st);
+}
+
+// Column-pivoted Householder QR of W (r x n, ldw = r) in place to min(r, n)
+// steps; the pivot order goes to jpvt (device, n entries).
+hipError_t qrcp_in_place(hipStream_t st, int64_t r, int64_t n, double* W, double* vn, int* jpvt,
+                         double* tau) {
+  hipError_t e;
+  if ((e = rbrp_launch_colnorm2(st, r, n, W, r, vn)) != hipSuccess) return e;
+  if ((e = rbrp_launch_iota(st, jpvt, n)) != hipSuccess) return e;
+  const int64_t steps = std::min(r, n);
+  for (int64_t j = 0; j < steps; ++j) {
+    if ((e = rbrp_launch_hh_reflect(st, r, n, j, W, r, vn, jpvt, tau, 1)) != hipSuccess) return e;
+    if ((e = rbrp_launch_hh_apply(st, r, n, j, W, r, tau, vn, 1)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int check_common(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda, int64_t r,
                  const double* dG, int64_t ldg, double* dT, int64_t ldt) {
   if (!h) return -1;
@@ -137,13 +178,7 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randCPQR_OS.cu:42
   // pivoted QR of Y(0:r, :) (randCPQR_OS.cu:55-72): only the pivots are used
   HIPCHK(rbrp_launch_copy_block(st, r, n, Y, l, W, r));
-  HIPCHK(rbrp_launch_colnorm2(st, r, n, W, r, vn));
-  HIPCHK(rbrp_launch_iota(st, jpvt, n));
-  const int64_t steps = std::min(r, n);
-  for (int64_t j = 0; j < steps; ++j) {
-    HIPCHK(rbrp_launch_hh_reflect(st, r, n, j, W, r, vn, jpvt, tau, 1));
-    HIPCHK(rbrp_launch_hh_apply(st, r, n, j, W, r, tau, vn, 1));
-  }
+  HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));
   // T by the oversampled interpolation (randCPQR_OS.cu:95, interp.cu:45-105)
   HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, jpvt, Yp, l));
   HIPCHK(interp_from_permuted(st, l, n, r, Yp, l, tau, dT, ldt));
@@ -189,23 +224,102 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randLUPP_OS.cu:43
   // LU with partial pivoting of Y(0:k, :)^T (n x k) (randLUPP_OS.cu:52-82)
   HIPCHK(rbrp_launch_transpose(st, k, n, Y, l, Yt, n));
-  const int64_t steps = std::min(n, k);
-  for (int64_t j = 0; j < steps; ++j) {
-    HIPCHK(rbrp_launch_lu_pivot(st, n, k, j, Yt, n, ipiv));
-    HIPCHK(rbrp_launch_lu_update(st, n, k, j, Yt, n));
-  }
-  // pivots -> permutation (permute.cu:8-23), applied in order on the host
-  std::vector<int> hip_((size_t)steps), hP((size_t)n);
-  HIPCHK(hipMemcpyAsync(hip_.data(), ipiv, sizeof(int) * steps, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (int64_t q = 0; q < n; ++q) hP[q] = (int)q;
-  for (int64_t j = 0; j < steps; ++j) std::swap(hP[j], hP[hip_[j]]);
-  HIPCHK(hipMemcpyAsync(P, hP.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(d_sk, P, sizeof(int) * k, hipMemcpyDeviceToDevice, st));
-  if (n > k) HIPCHK(hipMemcpyAsync(d_rd, P + k, sizeof(int) * (n - k), hipMemcpyDeviceToDevice, st));
+  // pivots -> permutation (randLUPP_OS.cu:85, permute.cu:8-23)
+  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd));
   // T by the oversampled interpolation (randLUPP_OS.cu:96, interp.cu:45-105)
   HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, P, Yp, l));
   HIPCHK(interp_from_permuted(st, l, n, k, Yp, l, tau, dT, ldt));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda, int64_t r,
+                    uint64_t seed, uint64_t offset, const double* dG, int64_t ldg, int32_t* h_sk,
+                    int32_t* h_rd, double* dT, int64_t ldt) {
+  int c = check_common(h, dA, m, n, lda, r, nullptr, 0, dT, ldt);
+  if (c) return c;
+  if (dG && ldg < r) return -10;
+  if (!h_sk) return -11;
+  if (n > r && !h_rd) return -12;
+  hipStream_t st = h->stream;
+  const size_t bG = dG ? 0 : align256(sizeof(double) * r * m);
+  const size_t bW = align256(sizeof(double) * r * n), bvn = align256(sizeof(double) * n);
+  const size_t btau = align256(sizeof(double) * r), bjp = align256(sizeof(int) * n);
+  const size_t bDinv = align256(sizeof(double) * 4 * ((r + 255) / 256) * 4096);
+  void* arena = nullptr;
+  HIPCHK(rbrp_sketch_arena(h, bG + bW + bvn + btau + bjp + bDinv, &arena));
+  char* p = (char*)arena;
+  double* G = dG ? (double*)dG : (double*)p;
+  const int64_t ldG = dG ? ldg : r;
+  p += bG;
+  double* W = (double*)p;
+  p += bW;
+  double* vn = (double*)p;
+  p += bvn;
+  double* tau = (double*)p;
+  p += btau;
+  int* jpvt = (int*)p;
+  p += bjp;
+  double* Dinv = (double*)p;
+  if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, r, m, 1.0 / std::sqrt((double)m), G, r));
+  HIPCHK(sketch_gemm(st, r, m, n, G, ldG, dA, lda, W));          // randCPQR.cu:42
+  HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));              // randCPQR.cu:52
+  if (n > r) {
+    // T = R11^{-1} R12 straight from the sketch's factor (randCPQR.cu:60-64)
+    HIPCHK(rbrp_launch_copy_block(st, r, n - r, W + r * r, r, dT, ldt));
+    HIPCHK(rbrp_trsm_upper_left(st, r, n - r, W, r, dT, ldt, nullptr, Dinv));
+  }
+  std::vector<int32_t> hp((size_t)n);
+  HIPCHK(hipMemcpyAsync(hp.data(), jpvt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  std::copy(hp.begin(), hp.begin() + r, h_sk);
+  if (n > r) std::copy(hp.begin() + r, hp.end(), h_rd);
+  return 0;
+}
+
+int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda, int64_t k,
+                    uint64_t seed, uint64_t offset, const double* dG, int64_t ldg, int32_t* d_sk,
+                    int32_t* d_rd, double* dT, int64_t ldt) {
+  int c = check_common(h, dA, m, n, lda, k, nullptr, 0, dT, ldt);
+  if (c) return c;
+  if (dG && ldg < k) return -10;
+  if (!d_sk) return -11;
+  if (n > k && !d_rd) return -12;
+  hipStream_t st = h->stream;
+  const size_t bG = dG ? 0 : align256(sizeof(double) * k * m);
+  const size_t bY = align256(sizeof(double) * k * n), bYt = bY, bL = align256(sizeof(double) * k * k);
+  const size_t bip = align256(sizeof(int) * k), bP = align256(sizeof(int) * n);
+  const size_t bDinv = align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
+  void* arena = nullptr;
+  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bL + bip + bP + bDinv, &arena));
+  char* p = (char*)arena;
+  double* G = dG ? (double*)dG : (double*)p;
+  const int64_t ldG = dG ? ldg : k;
+  p += bG;
+  double* Y = (double*)p;
+  p += bY;
+  double* Yt = (double*)p;
+  p += bYt;
+  double* Lt = (double*)p;
+  p += bL;
+  int* ipiv = (int*)p;
+  p += bip;
+  int* P = (int*)p;
+  p += bP;
+  double* Dinv = (double*)p;
+  if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, k, m, 1.0 / std::sqrt((double)m), G, k));
+  HIPCHK(sketch_gemm(st, k, m, n, G, ldG, dA, lda, Y));          // randLUPP.cu:43
+  // LU with partial pivoting of Y^T (n x k) (randLUPP.cu:47-78) and its permutation (:82-85)
+  HIPCHK(rbrp_launch_transpose(st, k, n, Y, k, Yt, n));
+  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd));
+  if (n > k) {
+    // T = (L21 L11^{-1})^T = L11^{-T} L21^T (randLUPP.cu:89-106), solved as an
+    // upper unit-triangular system with L11^T
+    HIPCHK(rbrp_launch_transpose(st, k, k, Yt, n, Lt, k));
+    HIPCHK(rbrp_launch_unit_upper(st, k, Lt, k));
+    HIPCHK(rbrp_launch_transpose(st, n - k, k, Yt + k, n, dT, ldt));
+    HIPCHK(rbrp_trsm_upper_left(st, k, n - k, Lt, k, dT, ldt, nullptr, Dinv));
+  }
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }

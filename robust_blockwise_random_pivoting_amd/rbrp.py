@@ -332,6 +332,33 @@ def _sketch_args(A, G):
     return A, None, 0
 
 
+def _sketch_id(cname: str, host_idx: bool, A, rank: int, seed: int, offset: int, G, handle):
+    """One sketch ID call through the C ABI: ``(sk, rd, T)`` with ``sk``/``rd`` host
+    int arrays (``host_idx``, the CPQR forms) or device tensors (the LUPP forms)."""
+    torch = _torch()
+    A, gp, ldg = _sketch_args(A, G)
+    m, n = A.shape
+    if host_idx:
+        sk = np.zeros(rank, dtype=np.int32)
+        rd = np.zeros(max(n - rank, 1), dtype=np.int32)
+        skp, rdp = ctypes.c_void_p(sk.ctypes.data), ctypes.c_void_p(rd.ctypes.data)
+    else:
+        sk = torch.empty(rank, dtype=torch.int32, device=A.device)
+        rd = torch.empty(max(n - rank, 1), dtype=torch.int32, device=A.device)
+        skp, rdp = ctypes.c_void_p(sk.data_ptr()), ctypes.c_void_p(rd.data_ptr())
+    Tb = torch.empty((max(n - rank, 1), rank), dtype=torch.float64, device=A.device)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check(cname, getattr(h._lib, cname)(
+            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), rank, seed, offset, gp, ldg,
+            skp, rdp, ctypes.c_void_p(Tb.data_ptr()), rank))
+    finally:
+        if own:
+            h.close()
+    return sk, rd[: n - rank], Tb[: n - rank].t()
+
+
 def RandCPQR_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
                 handle: Optional[Handle] = None):
     """Mirror of ``RandCPQR_OS(A, m, n, rank, h_sk, h_rd, T, flops)`` (gpu/rid.hpp:19-20).
@@ -340,23 +367,37 @@ def RandCPQR_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
     and the device ``T`` (rank x (n - rank)).  ``G`` optionally supplies the
     2*rank x m sketch; otherwise it is drawn from the Philox stream (seed, offset).
     """
-    torch = _torch()
-    A, gp, ldg = _sketch_args(A, G)
-    m, n = A.shape
-    sk = np.zeros(rank, dtype=np.int32)
-    rd = np.zeros(max(n - rank, 1), dtype=np.int32)
-    Tb = torch.empty((max(n - rank, 1), rank), dtype=torch.float64, device=A.device)
-    own = handle is None
-    h = Handle() if own else handle
-    try:
-        _capi.check("rbrp_randcpqr_os_d", h._lib.rbrp_randcpqr_os_d(
-            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), rank, seed, offset, gp, ldg,
-            ctypes.c_void_p(sk.ctypes.data), ctypes.c_void_p(rd.ctypes.data),
-            ctypes.c_void_p(Tb.data_ptr()), rank))
-    finally:
-        if own:
-            h.close()
-    return sk, rd[: n - rank], Tb[: n - rank].t()
+    return _sketch_id("rbrp_randcpqr_os_d", True, A, rank, seed, offset, G, handle)
+
+
+def RandCPQR(A, rank: int, seed: int = 0, offset: int = 0, G=None,
+             handle: Optional[Handle] = None):
+    """Mirror of ``RandCPQR(A, m, n, rank, h_sk, h_rd, T, flops)`` (gpu/rid.hpp:16-17).
+
+    As :func:`RandCPQR_OS` without oversampling: ``G`` is rank x m and ``T`` comes
+    from the sketch's own triangular factor (gpu/randCPQR.cu:15-67).
+    """
+    return _sketch_id("rbrp_randcpqr_d", True, A, rank, seed, offset, G, handle)
+
+
+def RandLUPP_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
+                handle: Optional[Handle] = None):
+    """Mirror of ``RandLUPP_OS(A, m, n, rank, sk, rd, T, flops)`` (gpu/rid.hpp:12-13).
+
+    Returns ``(sk, rd, T)`` as device tensors (the reference's ``sk``/``rd`` are
+    device pointers too).
+    """
+    return _sketch_id("rbrp_randlupp_os_d", False, A, rank, seed, offset, G, handle)
+
+
+def RandLUPP(A, rank: int, seed: int = 0, offset: int = 0, G=None,
+             handle: Optional[Handle] = None):
+    """Mirror of ``RandLUPP(A, m, n, rank, sk, rd, T, flops)`` (gpu/rid.hpp:9-10).
+
+    As :func:`RandLUPP_OS` without oversampling: ``G`` is rank x m and
+    ``T = (L21 L11^{-1})^T`` from the LU factor (gpu/randLUPP.cu:13-107).
+    """
+    return _sketch_id("rbrp_randlupp_d", False, A, rank, seed, offset, G, handle)
 
 
 def CPQR(A, ranks, handle: Optional[Handle] = None):
@@ -388,32 +429,6 @@ def CPQR(A, ranks, handle: Optional[Handle] = None):
         if own:
             h.close()
     return list(err[: len(rk)]), list(tr[: len(rk)]), tq.value, fl.value, perm
-
-
-def RandLUPP_OS(A, rank: int, seed: int = 0, offset: int = 0, G=None,
-                handle: Optional[Handle] = None):
-    """Mirror of ``RandLUPP_OS(A, m, n, rank, sk, rd, T, flops)`` (gpu/rid.hpp:12-13).
-
-    Returns ``(sk, rd, T)`` as device tensors (the reference's ``sk``/``rd`` are
-    device pointers too).
-    """
-    torch = _torch()
-    A, gp, ldg = _sketch_args(A, G)
-    m, n = A.shape
-    sk = torch.empty(rank, dtype=torch.int32, device=A.device)
-    rd = torch.empty(max(n - rank, 1), dtype=torch.int32, device=A.device)
-    Tb = torch.empty((max(n - rank, 1), rank), dtype=torch.float64, device=A.device)
-    own = handle is None
-    h = Handle() if own else handle
-    try:
-        _capi.check("rbrp_randlupp_os_d", h._lib.rbrp_randlupp_os_d(
-            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), rank, seed, offset, gp, ldg,
-            ctypes.c_void_p(sk.data_ptr()), ctypes.c_void_p(rd.data_ptr()),
-            ctypes.c_void_p(Tb.data_ptr()), rank))
-    finally:
-        if own:
-            h.close()
-    return sk, rd[: n - rank], Tb[: n - rank].t()
 
 
 def interp_os(Y, P, k: int, handle: Optional[Handle] = None):

@@ -292,3 +292,26 @@ def test_sketch_comparators():
     assert np.allclose(T2, Tls2, atol=1e-8 * max(1, np.abs(Tls2).max()))
     T3 = compute_interpolation_matrix_os(Y, np.concatenate([sk, rd]), r)
     assert np.allclose(T3, T)
+
+
+def test_sketch_ids_without_oversampling_solve_the_square_sketch():
+    """With as many sketch rows as the rank, Y(:, sk) is square and both forms'
+    T equal Y(:, sk)^{-1} Y(:, rd) (a closed form that skips their factorizations);
+    the QR form's pivots are LAPACK dgeqp3's, the LU form's skeleton is the
+    partial-pivoting row order of scipy.linalg.lu."""
+    from oracle import rand_cpqr, rand_lupp
+    rng = np.random.default_rng(21)
+    A = spectrum_matrix(300, 120, exp_spectrum(120, 15.0), 4)
+    for r in (5, 30, 60):
+        G = rng.standard_normal((r, 300)) / np.sqrt(300)
+        Y = G @ A
+        for fn in (rand_cpqr, rand_lupp):
+            sk, rd, T = fn(A, r, G)
+            assert sorted(np.concatenate([sk, rd]).tolist()) == list(range(120))
+            ref = np.linalg.solve(Y[:, sk], Y[:, rd])
+            np.testing.assert_allclose(T, ref, rtol=0, atol=1e-9 * np.abs(ref).max() *
+                                       np.linalg.cond(Y[:, sk]) / 1e3 + 1e-12)
+        _, _, piv = sla.qr(Y, mode="economic", pivoting=True)
+        assert np.array_equal(rand_cpqr(A, r, G)[0], piv[:r])
+        P, _, _ = sla.lu(Y.T)
+        assert np.array_equal(rand_lupp(A, r, G)[0], np.argmax(P, axis=0)[:r])
