@@ -105,8 +105,9 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   const size_t bDinv = align256(sizeof(double) * 4 * ((q + 255) / 256) * 4096);
   const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * q);
   const size_t bjp = align256(sizeof(int) * n);
+  const size_t bscr = align256(sizeof(double) * rbrp_hh_tall_scratch(m));
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bW + bE + bT + bDinv + 2 * bvn + btau + bjp, &arena));
+  HIPCHK(rbrp_sketch_arena(h, bW + bE + bT + bDinv + 2 * bvn + btau + bjp + bscr, &arena));
   char* p = (char*)arena;
   double* W = (double*)p;
   p += bW;
@@ -123,6 +124,10 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   double* tau = (double*)p;
   p += btau;
   int* jpvt = (int*)p;
+  p += bjp;
+  double* scr = (double*)p;
+  // long columns: the reflector's column work spread over workgroups
+  const bool tall = m >= 8192;
 
   // ||A||_F^2 before the timed factorization (id_cpqr.cu:37-38)
   std::vector<double> hcn((size_t)n);
@@ -138,7 +143,10 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   HIPCHK(rbrp_launch_colnorm2(st, m, n, W, m, vn));
   HIPCHK(rbrp_launch_iota(st, jpvt, n));
   for (int64_t j = 0; j < q; ++j) {
-    HIPCHK(rbrp_launch_hh_reflect(st, m, n, j, W, m, vn, jpvt, tau, 1));
+    if (tall)
+      HIPCHK(rbrp_launch_hh_reflect_tall(st, m, n, j, W, m, vn, jpvt, tau, 1, scr));
+    else
+      HIPCHK(rbrp_launch_hh_reflect(st, m, n, j, W, m, vn, jpvt, tau, 1));
     HIPCHK(rbrp_launch_hh_apply(st, m, n, j, W, m, tau, vn, 1));
   }
   HIPCHK(hipEventRecord(ev.b, st));

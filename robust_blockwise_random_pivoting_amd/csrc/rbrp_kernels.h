@@ -254,6 +254,12 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
 hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                  int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld);
+// Householder step j of k_hh_reflect for long columns, spread over workgroups
+// of rows; scratch holds rbrp_hh_tall_scratch(rows) doubles.
+int64_t rbrp_hh_tall_scratch(int64_t rows);
+hipError_t rbrp_launch_hh_reflect_tall(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
+                                       int64_t ld, double* vn, int* jpvt, double* tau, int pivoting,
+                                       double* scratch);
 hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, const double* W,
                                 int64_t ld, double* vn);
 hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
