@@ -96,6 +96,58 @@ hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t 
   return hipGetLastError();
 }
 
+// V (rows_pad x jb, ldv) <- the Householder vectors of a factored panel
+// (rows x jb at src, lds): unit diagonal, the stored vectors below it, zeros
+// above it and in the pad rows [rows, rows_pad).
+__global__ void k_panel_vectors(int64_t rows, int64_t rows_pad, int jb, const double* __restrict__ src,
+                                int64_t lds, double* __restrict__ V, int64_t ldv) {
+  const int c = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_pad || c >= jb) return;
+  double x = 0.0;
+  if (i < rows) x = (i == c) ? 1.0 : (i > c ? src[i + (int64_t)c * lds] : 0.0);
+  V[i + (int64_t)c * ldv] = x;
+}
+
+hipError_t rbrp_launch_panel_vectors(hipStream_t st, int64_t rows, int64_t rows_pad, int jb,
+                                     const double* src, int64_t lds, double* V, int64_t ldv) {
+  if (rows_pad <= 0 || jb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_panel_vectors, dim3((unsigned)((rows_pad + 255) / 256), (unsigned)jb), dim3(256), 0,
+                     st, rows, rows_pad, jb, src, lds, V, ldv);
+  return hipGetLastError();
+}
+
+// Tt (jb x jb, ldt) <- T^T for the block reflector H_0 H_1 ... H_{jb-1} =
+// I - V T V^T (forward, column-wise: T(i, i) = tau_i, T(0:i, i) = -tau_i
+// T(0:i, 0:i) S(0:i, i) with S = V^T V given, ld lds).  One workgroup of 64
+// threads, jb <= 64; thread r owns row r of T.
+__global__ __launch_bounds__(64) void k_wy_factor(int jb, const double* __restrict__ S, int64_t lds,
+                                                  const double* __restrict__ tau,
+                                                  double* __restrict__ Tt, int64_t ldt) {
+  __shared__ double T[64 * 64];  // T(r, c) at r + 64 c
+  const int r = threadIdx.x;
+  for (int i = 0; i < jb; ++i) {
+    const double ti = tau[i];
+    if (r < i) {
+      double acc = 0.0;
+      for (int c = r; c < i; ++c) acc = fma(T[r + 64 * c], S[c + (int64_t)i * lds], acc);
+      T[r + 64 * i] = -ti * acc;
+    } else if (r == i) {
+      T[r + 64 * i] = ti;
+    }
+    __syncthreads();
+  }
+  if (r < jb)
+    for (int c = 0; c < jb; ++c) Tt[c + (int64_t)r * ldt] = (r <= c) ? T[r + 64 * c] : 0.0;
+}
+
+hipError_t rbrp_launch_wy_factor(hipStream_t st, int jb, const double* S, int64_t lds, const double* tau,
+                                 double* Tt, int64_t ldt) {
+  if (jb <= 0 || jb > 64) return jb <= 0 ? hipSuccess : hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_wy_factor, dim3(1), dim3(64), 0, st, jb, S, lds, tau, Tt, ldt);
+  return hipGetLastError();
+}
+
 // vn[c] = sum_{i < rows} W(i, c)^2, one workgroup per column.
 __global__ __launch_bounds__(SK_NT) void k_colnorm2(int64_t rows, const double* __restrict__ W,
                                                     int64_t ld, double* __restrict__ vn) {

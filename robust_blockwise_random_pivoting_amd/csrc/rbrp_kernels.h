@@ -254,6 +254,13 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
 hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                  int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld);
+// Block-reflector pieces of the blocked Householder QR: the panel's explicit
+// vectors (unit diagonal, zero above and in the pad rows) and T^T of
+// I - V T V^T from S = V^T V and tau (jb <= 64).
+hipError_t rbrp_launch_panel_vectors(hipStream_t st, int64_t rows, int64_t rows_pad, int jb,
+                                     const double* src, int64_t lds, double* V, int64_t ldv);
+hipError_t rbrp_launch_wy_factor(hipStream_t st, int jb, const double* S, int64_t lds, const double* tau,
+                                 double* Tt, int64_t ldt);
 // Householder step j of k_hh_reflect for long columns, spread over workgroups
 // of rows; scratch holds rbrp_hh_tall_scratch(rows) doubles.
 int64_t rbrp_hh_tall_scratch(int64_t rows);

@@ -31,24 +31,105 @@ hipError_t sketch_gemm(hipStream_t st, int64_t l, int64_t m, int64_t n, const do
                              nullptr, 0);
 }
 
-// In place on Yp (l x n, ldy): k unpivoted Householder steps applied to all n
-// columns, then T (k x (n-k), ldt) = R(0:k,0:k)^{-1} Yp(0:k, k:n).  This is
-// geqrf + ormqr(Q^T) + trsm of interp.cu:82-97 with the same result.
-hipError_t interp_from_permuted(hipStream_t st, int64_t l, int64_t n, int64_t k, double* Yp,
-                                int64_t ldy, double* tau, double* T, int64_t ldt) {
+// The oversampled interpolation matrix (interp.cu:45-105): Yp = Y(:, P), QR of
+// Yp(:, 0:k) applied to all of Yp (geqrf + ormqr with Q^T), then
+// T (k x (n-k), ldt) = R(0:k, 0:k)^{-1} Yp(0:k, k:n).
+//
+// Blocked Householder QR on a copy of Yp whose leading dimension lp is l
+// rounded up to 16 with zero pad rows: each panel of up to IP_NB columns is
+// factored column by column (the reflector applied to the panel's columns
+// only), then the trailing columns C get the block reflector at once,
+// C -= V (T^T (V^T C)), with V the panel's explicit vectors (zero pad rows, so
+// every product runs over a multiple of 16 rows), S = V^T V and T^T from
+// k_wy_factor.  V^T C and S are split-K MFMA products; T^T W and V W2 are
+// 64 x 64-tile MFMA products.
+constexpr int64_t IP_NB = 32;      // panel width
+constexpr int64_t IP_SPLITS = 16;  // split-K groups of V^T C at most
+
+int64_t round16(int64_t x) { return (x + 15) / 16 * 16; }
+
+size_t interp_work_bytes(int64_t l, int64_t n, int64_t k) {
+  const int64_t lp = round16(l), nb = IP_NB;
+  return align256(sizeof(double) * lp * n) + align256(sizeof(double) * lp * nb) +
+         2 * align256(sizeof(double) * nb * nb) + 2 * align256(sizeof(double) * nb * n) +
+         align256(sizeof(double) * k) +
+         align256(sizeof(double) * std::max(IP_SPLITS * nb * n, 64 * nb * nb)) +
+         align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
+}
+
+// split-K of a product over K rows (K a multiple of 16) for a (rows x cols)
+// output: about 1024 workgroups, at most max_splits, chunks of >= 64 rows
+void tn_split(int64_t K, int64_t tiles, int64_t max_splits, int* ns, int64_t* kc) {
+  int64_t q = std::max<int64_t>(1, 1024 / std::max<int64_t>(tiles, 1));
+  q = std::min(q, std::min(max_splits, std::max<int64_t>(1, K / 64)));
+  int64_t c = round16((K + q - 1) / q);
+  *kc = c;
+  *ns = (int)std::max<int64_t>(1, (K + c - 1) / c);
+}
+
+hipError_t interp_os(hipStream_t st, int64_t l, int64_t n, int64_t k, const double* Y, int64_t ldy,
+                     const int* P, double* T, int64_t ldt, void* work) {
+  const int64_t lp = round16(l), nb = IP_NB;
+  char* p = (char*)work;
+  double* Yp = (double*)p;
+  p += align256(sizeof(double) * lp * n);
+  double* V = (double*)p;
+  p += align256(sizeof(double) * lp * nb);
+  double* S = (double*)p;
+  p += align256(sizeof(double) * nb * nb);
+  double* Tt = (double*)p;
+  p += align256(sizeof(double) * nb * nb);
+  double* W = (double*)p;
+  p += align256(sizeof(double) * nb * n);
+  double* W2 = (double*)p;
+  p += align256(sizeof(double) * nb * n);
+  double* tau = (double*)p;
+  p += align256(sizeof(double) * k);
+  double* Pbuf = (double*)p;
+  p += align256(sizeof(double) * std::max(IP_SPLITS * nb * n, 64 * nb * nb));
+  double* Dinv = (double*)p;
   hipError_t e;
-  for (int64_t j = 0; j < k; ++j) {
-    e = rbrp_launch_hh_reflect(st, l, n, j, Yp, ldy, nullptr, nullptr, tau, 0);
-    if (e != hipSuccess) return e;
-    e = rbrp_launch_hh_apply(st, l, n, j, Yp, ldy, tau, nullptr, 0);
-    if (e != hipSuccess) return e;
+#define IPCHK(x)                          \
+  do {                                    \
+    if ((e = (x)) != hipSuccess) return e; \
+  } while (0)
+  // (the calls that reach here synchronise before returning, so no graph
+  // captures this memset)
+  IPCHK(hipMemsetAsync(Yp, 0, sizeof(double) * lp * n, st));
+  IPCHK(rbrp_launch_gather_cols(st, l, n, Y, ldy, P, Yp, lp));
+  for (int64_t j0 = 0; j0 < k; j0 += nb) {
+    const int64_t jb = std::min(nb, k - j0), je = j0 + jb;
+    for (int64_t j = j0; j < je; ++j) {
+      IPCHK(rbrp_launch_hh_reflect(st, l, n, j, Yp, lp, nullptr, nullptr, tau, 0));
+      IPCHK(rbrp_launch_hh_apply(st, l, je, j, Yp, lp, tau, nullptr, 0));
+    }
+    if (je >= n) continue;
+    const int64_t rows = l - j0, rows_pad = lp - j0, nc = n - je;
+    double* C = Yp + j0 + je * lp;
+    IPCHK(rbrp_launch_panel_vectors(st, rows, rows_pad, (int)jb, Yp + j0 + j0 * lp, lp, V, rows_pad));
+    int ns;
+    int64_t kc;
+    tn_split(rows_pad, 1, 64, &ns, &kc);
+    IPCHK(rbrp_launch_gemm_tn(st, dim_static(jb), dim_static(jb), jb, jb, rows_pad, ns, kc, mat(V, rows_pad),
+                              mat(V, rows_pad), Pbuf, jb, jb * jb));
+    IPCHK(rbrp_launch_reduce_splits(st, dim_static(jb), dim_static(jb), jb, jb, Pbuf, jb, jb * jb, ns,
+                                    mat(S, jb), GEMM_SET));
+    IPCHK(rbrp_launch_wy_factor(st, (int)jb, S, jb, tau + j0, Tt, jb));
+    tn_split(rows_pad, (nc + 63) / 64, IP_SPLITS, &ns, &kc);
+    IPCHK(rbrp_launch_gemm_tn(st, dim_static(jb), dim_static(nc), jb, nc, rows_pad, ns, kc, mat(V, rows_pad),
+                              mat(C, lp), Pbuf, jb, jb * nc));
+    IPCHK(rbrp_launch_reduce_splits(st, dim_static(jb), dim_static(nc), jb, nc, Pbuf, jb, jb * nc, ns,
+                                    mat(W, jb), GEMM_SET));
+    IPCHK(rbrp_launch_gemm_nn(st, dim_static(jb), jb, dim_static(nc), nc, dim_static(jb), mat(Tt, jb),
+                              mat(W, jb), mat(W2, jb), GEMM_SET, nullptr, 0));
+    IPCHK(rbrp_launch_gemm_nn(st, dim_static(rows_pad), rows_pad, dim_static(nc), nc, dim_static(jb),
+                              mat(V, rows_pad), mat(W2, jb), mat(C, lp), GEMM_SUB, nullptr, 0));
   }
   if (n > k) {
-    e = rbrp_launch_copy_block(st, k, n - k, Yp + k * ldy, ldy, T, ldt);
-    if (e != hipSuccess) return e;
-    e = rbrp_trsm_upper_left(st, k, n - k, Yp, ldy, T, ldt, nullptr);
-    if (e != hipSuccess) return e;
+    IPCHK(rbrp_launch_copy_block(st, k, n - k, Yp + k * lp, lp, T, ldt));
+    IPCHK(rbrp_trsm_upper_left(st, k, n - k, Yp, lp, T, ldt, nullptr, Dinv));
   }
+#undef IPCHK
   return hipSuccess;
 }
 
@@ -134,12 +215,8 @@ int rbrp_interp_os_d(rbrp_handle_t h, const double* dY, int64_t l, int64_t n, in
   if (n > k && (!dT || ldt < k)) return -8;
   hipStream_t st = h->stream;
   void* arena = nullptr;
-  const size_t bYp = align256(sizeof(double) * l * n), btau = align256(sizeof(double) * k);
-  HIPCHK(rbrp_sketch_arena(h, bYp + btau, &arena));
-  double* Yp = (double*)arena;
-  double* tau = (double*)((char*)arena + bYp);
-  HIPCHK(rbrp_launch_gather_cols(st, l, n, dY, ldy, dP, Yp, l));
-  HIPCHK(interp_from_permuted(st, l, n, k, Yp, l, tau, dT, ldt));
+  HIPCHK(rbrp_sketch_arena(h, interp_work_bytes(l, n, k), &arena));
+  HIPCHK(interp_os(st, l, n, k, dY, ldy, dP, dT, ldt, arena));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
@@ -158,14 +235,12 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * l);
   const size_t bjp = align256(sizeof(int) * n);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + 2 * bY + bW + bvn + btau + bjp, &arena));
+  HIPCHK(rbrp_sketch_arena(h, bG + bY + bW + bvn + btau + bjp + interp_work_bytes(l, n, r), &arena));
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : l;
   p += bG;
   double* Y = (double*)p;
-  p += bY;
-  double* Yp = (double*)p;
   p += bY;
   double* W = (double*)p;
   p += bW;
@@ -174,14 +249,15 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   double* tau = (double*)p;
   p += btau;
   int* jpvt = (int*)p;
+  p += bjp;
+  void* iwork = p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
   HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randCPQR_OS.cu:42
   // pivoted QR of Y(0:r, :) (randCPQR_OS.cu:55-72): only the pivots are used
   HIPCHK(rbrp_launch_copy_block(st, r, n, Y, l, W, r));
   HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));
   // T by the oversampled interpolation (randCPQR_OS.cu:95, interp.cu:45-105)
-  HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, jpvt, Yp, l));
-  HIPCHK(interp_from_permuted(st, l, n, r, Yp, l, tau, dT, ldt));
+  HIPCHK(interp_os(st, l, n, r, Y, l, jpvt, dT, ldt, iwork));
   std::vector<int32_t> hp((size_t)n);
   HIPCHK(hipMemcpyAsync(hp.data(), jpvt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -201,25 +277,23 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const int64_t l = 2 * k;  // oversampling (randLUPP_OS.cu:18)
   const size_t bG = dG ? 0 : align256(sizeof(double) * l * m);
   const size_t bY = align256(sizeof(double) * l * n), bYt = align256(sizeof(double) * n * k);
-  const size_t btau = align256(sizeof(double) * l), bip = align256(sizeof(int) * k);
+  const size_t bip = align256(sizeof(int) * k);
   const size_t bP = align256(sizeof(int) * n);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + 2 * bY + bYt + btau + bip + bP, &arena));
+  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bip + bP + interp_work_bytes(l, n, k), &arena));
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : l;
   p += bG;
   double* Y = (double*)p;
   p += bY;
-  double* Yp = (double*)p;
-  p += bY;
   double* Yt = (double*)p;
   p += bYt;
-  double* tau = (double*)p;
-  p += btau;
   int* ipiv = (int*)p;
   p += bip;
   int* P = (int*)p;
+  p += bP;
+  void* iwork = p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
   HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randLUPP_OS.cu:43
   // LU with partial pivoting of Y(0:k, :)^T (n x k) (randLUPP_OS.cu:52-82)
@@ -227,8 +301,7 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   // pivots -> permutation (randLUPP_OS.cu:85, permute.cu:8-23)
   HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd));
   // T by the oversampled interpolation (randLUPP_OS.cu:96, interp.cu:45-105)
-  HIPCHK(rbrp_launch_gather_cols(st, l, n, Y, l, P, Yp, l));
-  HIPCHK(interp_from_permuted(st, l, n, k, Yp, l, tau, dT, ldt));
+  HIPCHK(interp_os(st, l, n, k, Y, l, P, dT, ldt, iwork));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
