@@ -426,7 +426,7 @@ hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int6
 // columns, ipiv[j] = pivot (0-based), scale Y(j+1:, j) by 1 / Y(j, j) (skipped
 // for an exact zero pivot, as dgetf2 does).  One workgroup.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(SK_NT) void k_lu_pivot(int64_t nrows, int64_t ncols, int64_t j,
+__global__ __launch_bounds__(SK_NT) void k_lu_pivot(int64_t nrows, int64_t c0, int64_t ncols, int64_t j,
                                                     double* __restrict__ Y, int64_t ld,
                                                     int* __restrict__ ipiv) {
   __shared__ double rv[SK_NT];
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(SK_NT) void k_lu_pivot(int64_t nrows, int64_t ncols
   __syncthreads();
   const int64_t p = s_p;
   if (p != j) {
-    for (int64_t c = tid; c < ncols; c += SK_NT) {
+    for (int64_t c = c0 + tid; c < ncols; c += SK_NT) {
       const double t0 = Y[j + c * ld];
       Y[j + c * ld] = Y[p + c * ld];
       Y[p + c * ld] = t0;
@@ -477,9 +477,69 @@ __global__ __launch_bounds__(SK_NT) void k_lu_pivot(int64_t nrows, int64_t ncols
   }
 }
 
-hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
-                                int64_t ld, int* ipiv) {
-  hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(SK_NT), 0, st, nrows, ncols, j, Y, ld, ipiv);
+hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t c0, int64_t ncols, int64_t j,
+                                double* Y, int64_t ld, int* ipiv) {
+  hipLaunchKernelGGL(k_lu_pivot, dim3(1), dim3(SK_NT), 0, st, nrows, c0, ncols, j, Y, ld, ipiv);
+  return hipGetLastError();
+}
+
+// The row interchanges ipiv[j0:j1) of a factored panel applied, in order, to
+// every column c in [0, ncols) outside [j0, j1): one thread per column.
+__global__ void k_lu_laswp(int64_t ncols, int64_t j0, int64_t j1, double* __restrict__ Y, int64_t ld,
+                           const int* __restrict__ ipiv) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols || (c >= j0 && c < j1)) return;
+  double* y = Y + c * ld;
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t p = ipiv[j];
+    if (p != j) {
+      const double t = y[j];
+      y[j] = y[p];
+      y[p] = t;
+    }
+  }
+}
+
+hipError_t rbrp_launch_lu_laswp(hipStream_t st, int64_t ncols, int64_t j0, int64_t j1, double* Y,
+                                int64_t ld, const int* ipiv) {
+  if (ncols <= 0 || j1 <= j0) return hipSuccess;
+  hipLaunchKernelGGL(k_lu_laswp, dim3((unsigned)((ncols + 255) / 256)), dim3(256), 0, st, ncols, j0, j1, Y,
+                     ld, ipiv);
+  return hipGetLastError();
+}
+
+// Linv (jb x jb, ldi) <- L^{-1} for the unit lower-triangular jb x jb block L
+// (strictly lower part at L, ld), jb <= 64: thread c forward-substitutes
+// column c of the identity.
+__global__ __launch_bounds__(64) void k_unit_lower_inv(int jb, const double* __restrict__ L, int64_t ld,
+                                                       double* __restrict__ Linv, int64_t ldi) {
+  __shared__ double Ls[64 * 64];  // L(i, r) at i + 64 r
+  __shared__ double X[64 * 64];   // column c of the inverse at 64 c
+  const int t = threadIdx.x;
+  for (int r = 0; r < jb; ++r)
+    if (t < jb) Ls[t + 64 * r] = L[t + (int64_t)r * ld];
+  __syncthreads();
+  if (t < jb) {
+    double* x = X + 64 * t;
+    for (int i = 0; i < jb; ++i) {
+      if (i < t) {
+        x[i] = 0.0;
+      } else if (i == t) {
+        x[i] = 1.0;
+      } else {
+        double acc = 0.0;
+        for (int r = t; r < i; ++r) acc = fma(Ls[i + 64 * r], x[r], acc);
+        x[i] = -acc;
+      }
+    }
+    for (int i = 0; i < jb; ++i) Linv[i + (int64_t)t * ldi] = x[i];
+  }
+}
+
+hipError_t rbrp_launch_unit_lower_inv(hipStream_t st, int jb, const double* L, int64_t ld, double* Linv,
+                                      int64_t ldi) {
+  if (jb <= 0 || jb > 64) return jb <= 0 ? hipSuccess : hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_unit_lower_inv, dim3(1), dim3(64), 0, st, jb, L, ld, Linv, ldi);
   return hipGetLastError();
 }
 

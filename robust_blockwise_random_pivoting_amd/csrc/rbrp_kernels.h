@@ -273,7 +273,15 @@ hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, in
                                   int64_t ld, double* vn, int* jpvt, double* tau, int pivoting);
 hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
                                 int64_t ld, const double* tau, double* vn, int norms);
-hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
-                                int64_t ld, int* ipiv);
+// LU step j: pivot search down column j, the row swap over columns [c0, ncols),
+// the column's scaling.
+hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t c0, int64_t ncols, int64_t j,
+                                double* Y, int64_t ld, int* ipiv);
+// A panel's interchanges ipiv[j0:j1) applied to the columns outside [j0, j1).
+hipError_t rbrp_launch_lu_laswp(hipStream_t st, int64_t ncols, int64_t j0, int64_t j1, double* Y,
+                                int64_t ld, const int* ipiv);
+// Linv (ld ldi) <- the inverse of the jb x jb unit lower-triangular block at L (jb <= 64).
+hipError_t rbrp_launch_unit_lower_inv(hipStream_t st, int jb, const double* L, int64_t ld, double* Linv,
+                                      int64_t ldi);
 hipError_t rbrp_launch_lu_update(hipStream_t st, int64_t nrows, int64_t ncols, int64_t j, double* Y,
                                  int64_t ld);

@@ -133,17 +133,42 @@ hipError_t interp_os(hipStream_t st, int64_t l, int64_t n, int64_t k, const doub
   return hipSuccess;
 }
 
-// LU with partial pivoting of Yt (n x k, ldy = n) in place, then the row
-// interchanges applied in order to the identity (gpu/permute.cu:8-23); the
-// permutation goes to P (device, n entries), its first k entries to d_sk and
-// the rest to d_rd.
+// LU with partial pivoting of Yt (n x k, ldy = n, k <= n) in place, blocked
+// right-looking (dgetrf's order): each LU_NB-column panel is factored column by
+// column with its row swaps confined to the panel, then the panel's swaps
+// reach the other columns (k_lu_laswp), U12 = L11^{-1} A12 is formed in place
+// (a one-row-tile product, so it may overwrite its own operand) and
+// A22 -= L21 U12 is one MFMA product.  Then the row interchanges are applied
+// in order to the identity (gpu/permute.cu:8-23); the permutation goes to P
+// (device, n entries), its first k entries to d_sk and the rest to d_rd.
+// Linv is LU_NB x LU_NB scratch.
+constexpr int64_t LU_NB = 32;
+
 hipError_t lupp_permutation(hipStream_t st, int64_t n, int64_t k, double* Yt, int* ipiv, int* P,
-                            int32_t* d_sk, int32_t* d_rd) {
+                            int32_t* d_sk, int32_t* d_rd, double* Linv) {
   const int64_t steps = std::min(n, k);
   hipError_t e;
-  for (int64_t j = 0; j < steps; ++j) {
-    if ((e = rbrp_launch_lu_pivot(st, n, k, j, Yt, n, ipiv)) != hipSuccess) return e;
-    if ((e = rbrp_launch_lu_update(st, n, k, j, Yt, n)) != hipSuccess) return e;
+  for (int64_t j0 = 0; j0 < steps; j0 += LU_NB) {
+    const int64_t je = std::min(j0 + LU_NB, steps), jb = je - j0;
+    for (int64_t j = j0; j < je; ++j) {
+      if ((e = rbrp_launch_lu_pivot(st, n, j0, je, j, Yt, n, ipiv)) != hipSuccess) return e;
+      if ((e = rbrp_launch_lu_update(st, n, je, j, Yt, n)) != hipSuccess) return e;
+    }
+    if ((e = rbrp_launch_lu_laswp(st, k, j0, je, Yt, n, ipiv)) != hipSuccess) return e;
+    if (je < k) {
+      double* A12 = Yt + j0 + je * n;
+      if ((e = rbrp_launch_unit_lower_inv(st, (int)jb, Yt + j0 + j0 * n, n, Linv, LU_NB)) != hipSuccess)
+        return e;
+      if ((e = rbrp_launch_gemm_nn(st, dim_static(jb), jb, dim_static(k - je), k - je, dim_static(jb),
+                                   mat(Linv, LU_NB), mat(A12, n), mat(A12, n), GEMM_SET, nullptr, 0)) !=
+          hipSuccess)
+        return e;
+      if (je < n &&
+          (e = rbrp_launch_gemm_nn(st, dim_static(n - je), n - je, dim_static(k - je), k - je, dim_static(jb),
+                                   mat(Yt + je + j0 * n, n), mat(A12, n), mat(Yt + je + je * n, n), GEMM_SUB,
+                                   nullptr, 0)) != hipSuccess)
+        return e;
+    }
   }
   std::vector<int> hpiv((size_t)steps), hP((size_t)n);
   if ((e = hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * steps, hipMemcpyDeviceToHost, st)) != hipSuccess)
@@ -280,7 +305,8 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const size_t bip = align256(sizeof(int) * k);
   const size_t bP = align256(sizeof(int) * n);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bip + bP + interp_work_bytes(l, n, k), &arena));
+  const size_t bLinv = align256(sizeof(double) * LU_NB * LU_NB);
+  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bip + bP + bLinv + interp_work_bytes(l, n, k), &arena));
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : l;
@@ -293,13 +319,15 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   p += bip;
   int* P = (int*)p;
   p += bP;
+  double* Linv = (double*)p;
+  p += bLinv;
   void* iwork = p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
   HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randLUPP_OS.cu:43
   // LU with partial pivoting of Y(0:k, :)^T (n x k) (randLUPP_OS.cu:52-82)
   HIPCHK(rbrp_launch_transpose(st, k, n, Y, l, Yt, n));
   // pivots -> permutation (randLUPP_OS.cu:85, permute.cu:8-23)
-  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd));
+  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd, Linv));
   // T by the oversampled interpolation (randLUPP_OS.cu:96, interp.cu:45-105)
   HIPCHK(interp_os(st, l, n, k, Y, l, P, dT, ldt, iwork));
   HIPCHK(hipStreamSynchronize(st));
@@ -364,7 +392,8 @@ int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   const size_t bip = align256(sizeof(int) * k), bP = align256(sizeof(int) * n);
   const size_t bDinv = align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bL + bip + bP + bDinv, &arena));
+  const size_t bLinv = align256(sizeof(double) * LU_NB * LU_NB);
+  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bL + bip + bP + bDinv + bLinv, &arena));
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : k;
@@ -380,11 +409,13 @@ int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   int* P = (int*)p;
   p += bP;
   double* Dinv = (double*)p;
+  p += bDinv;
+  double* Linv = (double*)p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, k, m, 1.0 / std::sqrt((double)m), G, k));
   HIPCHK(sketch_gemm(st, k, m, n, G, ldG, dA, lda, Y));          // randLUPP.cu:43
   // LU with partial pivoting of Y^T (n x k) (randLUPP.cu:47-78) and its permutation (:82-85)
   HIPCHK(rbrp_launch_transpose(st, k, n, Y, k, Yt, n));
-  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd));
+  HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd, Linv));
   if (n > k) {
     // T = (L21 L11^{-1})^T = L11^{-T} L21^T (randLUPP.cu:89-106), solved as an
     // upper unit-triangular system with L11^T
