@@ -124,13 +124,17 @@ hipError_t rbrp_launch_panel_vectors(hipStream_t st, int64_t rows, int64_t rows_
 __global__ __launch_bounds__(64) void k_wy_factor(int jb, const double* __restrict__ S, int64_t lds,
                                                   const double* __restrict__ tau,
                                                   double* __restrict__ Tt, int64_t ldt) {
-  __shared__ double T[64 * 64];  // T(r, c) at r + 64 c
+  __shared__ double T[64 * 64];   // T(r, c) at r + 64 c
+  __shared__ double Ss[64 * 64];  // S(c, i) at c + 64 i
   const int r = threadIdx.x;
+  if (r < jb)
+    for (int i = 0; i < jb; ++i) Ss[r + 64 * i] = S[r + (int64_t)i * lds];
+  __syncthreads();
   for (int i = 0; i < jb; ++i) {
     const double ti = tau[i];
     if (r < i) {
       double acc = 0.0;
-      for (int c = r; c < i; ++c) acc = fma(T[r + 64 * c], S[c + (int64_t)i * lds], acc);
+      for (int c = r; c < i; ++c) acc = fma(T[r + 64 * c], Ss[c + 64 * i], acc);
       T[r + 64 * i] = -ti * acc;
     } else if (r == i) {
       T[r + 64 * i] = ti;
@@ -514,25 +518,27 @@ hipError_t rbrp_launch_lu_laswp(hipStream_t st, int64_t ncols, int64_t j0, int64
 __global__ __launch_bounds__(64) void k_unit_lower_inv(int jb, const double* __restrict__ L, int64_t ld,
                                                        double* __restrict__ Linv, int64_t ldi) {
   __shared__ double Ls[64 * 64];  // L(i, r) at i + 64 r
-  __shared__ double X[64 * 64];   // column c of the inverse at 64 c
+  __shared__ double X[64 * 64];   // element i of column t of the inverse at t + 64 i
   const int t = threadIdx.x;
   for (int r = 0; r < jb; ++r)
     if (t < jb) Ls[t + 64 * r] = L[t + (int64_t)r * ld];
   __syncthreads();
   if (t < jb) {
-    double* x = X + 64 * t;
+    // (element-major layout: the 64 threads' columns sit in distinct banks)
     for (int i = 0; i < jb; ++i) {
+      double xi;
       if (i < t) {
-        x[i] = 0.0;
+        xi = 0.0;
       } else if (i == t) {
-        x[i] = 1.0;
+        xi = 1.0;
       } else {
         double acc = 0.0;
-        for (int r = t; r < i; ++r) acc = fma(Ls[i + 64 * r], x[r], acc);
-        x[i] = -acc;
+        for (int r = t; r < i; ++r) acc = fma(Ls[i + 64 * r], X[t + 64 * r], acc);
+        xi = -acc;
       }
+      X[t + 64 * i] = xi;
     }
-    for (int i = 0; i < jb; ++i) Linv[i + (int64_t)t * ldi] = x[i];
+    for (int i = 0; i < jb; ++i) Linv[i + (int64_t)t * ldi] = X[t + 64 * i];
   }
 }
 

@@ -100,8 +100,10 @@ int rbrp_test_gemm_nn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
   if (mode < GEMM_SET || mode > GEMM_ADD) return -11;
   if (M == 0 || N == 0 || K == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
+  // row ranges sized to the kernel's residency on this device, as the ID loop does
   HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(M), M, dim_static(N), N, dim_static(K),
-                               mat((double*)A, lda), mat((double*)B, ldb), mat(C, ldc), mode));
+                               mat((double*)A, lda), mat((double*)B, ldb), mat(C, ldc), mode,
+                               rbrp_gemm_nn64_resident()));
   HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
