@@ -23,9 +23,32 @@ namespace {
 
 inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Y (l x n, ldy = l) = G (l x m, ldg) * A (m x n, lda).
-hipError_t sketch_gemm(hipStream_t st, int64_t l, int64_t m, int64_t n, const double* G, int64_t ldg,
-                       const double* A, int64_t lda, double* Y) {
+// Whether Y = G A (l x m times m x n) runs on the split-K 64 x 128 TN kernel
+// as Y = (G^T)^T A: whole 16-row steps of the reduction, even leading
+// dimensions, and enough output tiles to fill the device without splitting K.
+bool sketch_on_tn64(rbrp_handle_t h, int64_t l, int64_t m, int64_t n, int64_t lda) {
+  const int64_t tiles = ((l + 63) / 64) * ((n + 127) / 128);
+  return m % 16 == 0 && lda % 2 == 0 && h->tn64_resident > 0 && 2 * tiles >= h->tn64_resident;
+}
+
+// Scratch bytes of sketch_gemm: G^T (m x l) when the TN kernel takes the product.
+size_t sketch_scratch_bytes(rbrp_handle_t h, int64_t l, int64_t m, int64_t n, int64_t lda) {
+  return sketch_on_tn64(h, l, m, n, lda) ? align256(sizeof(double) * m * l) : 0;
+}
+
+// Y (l x n, ldy = l) = G (l x m, ldg) * A (m x n, lda).  With Gt (scratch of
+// sketch_scratch_bytes) the sketch is transposed once (0.7 ms at 4094 x 50000)
+// and the product runs on the projection's TN kernel, written straight into Y
+// (one K chunk); otherwise on the 64 x 64-tile NN kernel.
+hipError_t sketch_gemm(rbrp_handle_t h, int64_t l, int64_t m, int64_t n, const double* G, int64_t ldg,
+                       const double* A, int64_t lda, double* Y, double* Gt) {
+  hipStream_t st = h->stream;
+  if (Gt && sketch_on_tn64(h, l, m, n, lda)) {
+    hipError_t e = rbrp_launch_transpose(st, l, m, G, ldg, Gt, m);
+    if (e != hipSuccess) return e;
+    return rbrp_launch_gemm_tn64(st, dim_static(l), dim_static(n), l, n, m, 1, m, mat(Gt, m),
+                                 mat((double*)A, lda), Y, l, l * n);
+  }
   return rbrp_launch_gemm_nn(st, dim_static(l), l, dim_static(n), n, dim_static(m),
                              mat((double*)G, ldg), mat((double*)A, lda), mat(Y, l), GEMM_SET,
                              nullptr, 0);
@@ -260,7 +283,10 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * l);
   const size_t bjp = align256(sizeof(int) * n);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + bY + bW + bvn + btau + bjp + interp_work_bytes(l, n, r), &arena));
+  const size_t bGt = sketch_scratch_bytes(h, l, m, n, lda);
+  const size_t base = bG + bY + bW + bvn + btau + bjp + interp_work_bytes(l, n, r);
+  HIPCHK(rbrp_sketch_arena(h, base + bGt, &arena));
+  double* Gt = bGt ? (double*)((char*)arena + base) : nullptr;
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : l;
@@ -277,7 +303,7 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   p += bjp;
   void* iwork = p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
-  HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randCPQR_OS.cu:42
+  HIPCHK(sketch_gemm(h, l, m, n, G, ldG, dA, lda, Y, Gt));      // randCPQR_OS.cu:42
   // pivoted QR of Y(0:r, :) (randCPQR_OS.cu:55-72): only the pivots are used
   HIPCHK(rbrp_launch_copy_block(st, r, n, Y, l, W, r));
   HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));
@@ -306,7 +332,10 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const size_t bP = align256(sizeof(int) * n);
   void* arena = nullptr;
   const size_t bLinv = align256(sizeof(double) * LU_NB * LU_NB);
-  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bip + bP + bLinv + interp_work_bytes(l, n, k), &arena));
+  const size_t bGt = sketch_scratch_bytes(h, l, m, n, lda);
+  const size_t base = bG + bY + bYt + bip + bP + bLinv + interp_work_bytes(l, n, k);
+  HIPCHK(rbrp_sketch_arena(h, base + bGt, &arena));
+  double* Gt = bGt ? (double*)((char*)arena + base) : nullptr;
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : l;
@@ -323,7 +352,7 @@ int rbrp_randlupp_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   p += bLinv;
   void* iwork = p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, l, m, 1.0 / std::sqrt((double)m), G, l));
-  HIPCHK(sketch_gemm(st, l, m, n, G, ldG, dA, lda, Y));          // randLUPP_OS.cu:43
+  HIPCHK(sketch_gemm(h, l, m, n, G, ldG, dA, lda, Y, Gt));      // randLUPP_OS.cu:43
   // LU with partial pivoting of Y(0:k, :)^T (n x k) (randLUPP_OS.cu:52-82)
   HIPCHK(rbrp_launch_transpose(st, k, n, Y, l, Yt, n));
   // pivots -> permutation (randLUPP_OS.cu:85, permute.cu:8-23)
@@ -348,7 +377,10 @@ int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   const size_t btau = align256(sizeof(double) * r), bjp = align256(sizeof(int) * n);
   const size_t bDinv = align256(sizeof(double) * 4 * ((r + 255) / 256) * 4096);
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bG + bW + bvn + btau + bjp + bDinv, &arena));
+  const size_t bGt = sketch_scratch_bytes(h, r, m, n, lda);
+  const size_t base = bG + bW + bvn + btau + bjp + bDinv;
+  HIPCHK(rbrp_sketch_arena(h, base + bGt, &arena));
+  double* Gt = bGt ? (double*)((char*)arena + base) : nullptr;
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : r;
@@ -363,7 +395,7 @@ int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   p += bjp;
   double* Dinv = (double*)p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, r, m, 1.0 / std::sqrt((double)m), G, r));
-  HIPCHK(sketch_gemm(st, r, m, n, G, ldG, dA, lda, W));          // randCPQR.cu:42
+  HIPCHK(sketch_gemm(h, r, m, n, G, ldG, dA, lda, W, Gt));      // randCPQR.cu:42
   HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));              // randCPQR.cu:52
   if (n > r) {
     // T = R11^{-1} R12 straight from the sketch's factor (randCPQR.cu:60-64)
@@ -393,7 +425,10 @@ int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   const size_t bDinv = align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
   void* arena = nullptr;
   const size_t bLinv = align256(sizeof(double) * LU_NB * LU_NB);
-  HIPCHK(rbrp_sketch_arena(h, bG + bY + bYt + bL + bip + bP + bDinv + bLinv, &arena));
+  const size_t bGt = sketch_scratch_bytes(h, k, m, n, lda);
+  const size_t base = bG + bY + bYt + bL + bip + bP + bDinv + bLinv;
+  HIPCHK(rbrp_sketch_arena(h, base + bGt, &arena));
+  double* Gt = bGt ? (double*)((char*)arena + base) : nullptr;
   char* p = (char*)arena;
   double* G = dG ? (double*)dG : (double*)p;
   const int64_t ldG = dG ? ldg : k;
@@ -412,7 +447,7 @@ int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   p += bDinv;
   double* Linv = (double*)p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, k, m, 1.0 / std::sqrt((double)m), G, k));
-  HIPCHK(sketch_gemm(st, k, m, n, G, ldG, dA, lda, Y));          // randLUPP.cu:43
+  HIPCHK(sketch_gemm(h, k, m, n, G, ldG, dA, lda, Y, Gt));      // randLUPP.cu:43
   // LU with partial pivoting of Y^T (n x k) (randLUPP.cu:47-78) and its permutation (:82-85)
   HIPCHK(rbrp_launch_transpose(st, k, n, Y, k, Yt, n));
   HIPCHK(lupp_permutation(st, n, k, Yt, ipiv, P, d_sk, d_rd, Linv));

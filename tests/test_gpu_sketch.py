@@ -105,6 +105,27 @@ def test_rand_id_without_oversampling_parity(name, gen, r, form):
     assert rel <= bound, (name, rel, bound)
 
 
+@pytest.mark.parametrize("form", ["cpqr_os", "lupp_os"])
+def test_sketch_product_on_the_tn_kernel_parity(form):
+    """A sketch big enough (1536 x 4096 output) for Y = G A to run on the split-K TN kernel
+    from the transposed sketch: same skeleton and T as the oracle fed the same G."""
+    from robust_blockwise_random_pivoting_amd import RandCPQR_OS, RandLUPP_OS, gaussian_sketch
+    m, n, r = 1024, 4096, 768
+    A = spectrum_matrix(m, n, exp_spectrum(m, 400.0), 17)
+    G = gaussian_sketch(2 * r, m, 1.0 / np.sqrt(m), seed=13)
+    fn, ref = (RandCPQR_OS, rand_cpqr_os) if form == "cpqr_os" else (RandLUPP_OS, rand_lupp_os)
+    print(f"{form} {m}x{n} r={r}", flush=True)
+    sk, rd, T = fn(_dev(A), r, G=G)
+    sk = sk if isinstance(sk, np.ndarray) else sk.cpu().numpy()
+    rd = rd if isinstance(rd, np.ndarray) else rd.cpu().numpy()
+    Gh = G.cpu().numpy()
+    sk_ref, rd_ref, T_ref = ref(A, r, Gh)
+    assert np.array_equal(sk, sk_ref) and np.array_equal(rd, rd_ref)
+    bound = _kappa_bound(Gh @ A, np.concatenate([sk_ref, rd_ref]), r)
+    rel = np.linalg.norm(T.cpu().numpy() - T_ref) / np.linalg.norm(T_ref)
+    assert rel <= bound, (rel, bound)
+
+
 def test_interp_os_parity_and_internal_sketch():
     from robust_blockwise_random_pivoting_amd import RandCPQR_OS, interp_os
     rng = np.random.default_rng(1)
