@@ -425,6 +425,82 @@ hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int6
 }
 
 // ---------------------------------------------------------------------------
+// Unpivoted Householder step j of a panel in one launch (the blocked QR's
+// panel steps): workgroup 0 finishes the previous step's column (scales its
+// vector and stores beta, from scr), then stores this step's tau[j] and
+// beta / scale / flag in scr; workgroup b >= 1 applies H_j to column j + b.
+// Every workgroup forms x(j+1:)^2's sum with k_hh_reflect's reduction and
+// v = x * scale with its rounding, so the results equal the two-launch
+// k_hh_reflect + k_hh_apply bit for bit.  Column j itself is written only by
+// the next step's launch (or k_hh_finish), after every reader is done.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(SK_NT) void k_hh_step(int64_t rows, int64_t j, double* __restrict__ W,
+                                                   int64_t ld, double* __restrict__ tau,
+                                                   double* __restrict__ scr, int has_prev) {
+  __shared__ double red[SK_NT / 64];
+  const double* x = W + j * ld;
+  double acc = 0.0;
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) acc = fma(x[i], x[i], acc);
+  const double xn2 = rbrp_block_sum(acc, red);
+  const double alpha = x[j];
+  double tj = 0.0, beta = 0.0, scal = 0.0;
+  if (xn2 != 0.0) {
+    const double nrm = sqrt(alpha * alpha + xn2);
+    beta = (alpha >= 0.0) ? -nrm : nrm;
+    scal = 1.0 / (alpha - beta);
+    tj = (beta - alpha) / beta;
+  }
+  if (blockIdx.x == 0) {
+    if (has_prev && scr[2] != 0.0) {
+      double* y = W + (j - 1) * ld;
+      const double sp = scr[0];
+      for (int64_t i = j + threadIdx.x; i < rows; i += SK_NT) y[i] *= sp;
+      if (threadIdx.x == 0) y[j - 1] = scr[1];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      tau[j] = tj;
+      scr[0] = scal;
+      scr[1] = beta;
+      scr[2] = (xn2 != 0.0) ? 1.0 : 0.0;
+    }
+    return;
+  }
+  if (xn2 == 0.0) return;  // tau_j = 0: H_j is the identity
+  double* w = W + (j + blockIdx.x) * ld;
+  double dacc = 0.0;
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) dacc = fma(x[i] * scal, w[i], dacc);
+  const double dot = rbrp_block_sum(dacc, red) + w[j];
+  const double f = tj * dot;
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) w[i] -= f * (x[i] * scal);
+  __syncthreads();
+  if (threadIdx.x == 0) w[j] -= f;
+}
+
+// The last step's column: its vector scaled and beta stored (from scr).
+__global__ __launch_bounds__(SK_NT) void k_hh_finish(int64_t rows, int64_t j, double* __restrict__ W,
+                                                     int64_t ld, const double* __restrict__ scr) {
+  if (scr[2] == 0.0) return;
+  double* y = W + j * ld;
+  const double sp = scr[0];
+  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) y[i] *= sp;
+  if (threadIdx.x == 0) y[j] = scr[1];
+}
+
+hipError_t rbrp_launch_hh_step(hipStream_t st, int64_t rows, int64_t cend, int64_t j, double* W,
+                               int64_t ld, double* tau, double* scr, int has_prev) {
+  hipLaunchKernelGGL(k_hh_step, dim3((unsigned)(cend - j)), dim3(SK_NT), 0, st, rows, j, W, ld, tau, scr,
+                     has_prev);
+  return hipGetLastError();
+}
+
+hipError_t rbrp_launch_hh_finish(hipStream_t st, int64_t rows, int64_t j, double* W, int64_t ld,
+                                 const double* scr) {
+  hipLaunchKernelGGL(k_hh_finish, dim3(1), dim3(SK_NT), 0, st, rows, j, W, ld, scr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // LU with partial pivoting, step j of Y (nrows x ncols): pivot = first index of
 // max |Y(i, j)| over i >= j (LAPACK idamax), swap rows j and pivot across all
 // columns, ipiv[j] = pivot (0-based), scale Y(j+1:, j) by 1 / Y(j, j) (skipped

@@ -254,6 +254,13 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
 hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                  int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld);
+// One unpivoted Householder step j of a panel ending at column cend (the
+// reflector and its application in one launch; column j is completed by the
+// next step or by rbrp_launch_hh_finish); scr holds 3 doubles between steps.
+hipError_t rbrp_launch_hh_step(hipStream_t st, int64_t rows, int64_t cend, int64_t j, double* W,
+                               int64_t ld, double* tau, double* scr, int has_prev);
+hipError_t rbrp_launch_hh_finish(hipStream_t st, int64_t rows, int64_t j, double* W, int64_t ld,
+                                 const double* scr);
 // Block-reflector pieces of the blocked Householder QR: the panel's explicit
 // vectors (unit diagonal, zero above and in the pad rows) and T^T of
 // I - V T V^T from S = V^T V and tau (jb <= 64).

@@ -60,8 +60,8 @@ hipError_t sketch_gemm(rbrp_handle_t h, int64_t l, int64_t m, int64_t n, const d
 //
 // Blocked Householder QR on a copy of Yp whose leading dimension lp is l
 // rounded up to 16 with zero pad rows: each panel of up to IP_NB columns is
-// factored column by column (the reflector applied to the panel's columns
-// only), then the trailing columns C get the block reflector at once,
+// factored column by column, one launch per column (the reflector applied to
+// the panel's columns only), then the trailing columns C get the block reflector at once,
 // C -= V (T^T (V^T C)), with V the panel's explicit vectors (zero pad rows, so
 // every product runs over a multiple of 16 rows), S = V^T V and T^T from
 // k_wy_factor.  V^T C and S are split-K MFMA products; T^T W and V W2 are
@@ -77,7 +77,7 @@ size_t interp_work_bytes(int64_t l, int64_t n, int64_t k) {
          2 * align256(sizeof(double) * nb * nb) + 2 * align256(sizeof(double) * nb * n) +
          align256(sizeof(double) * k) +
          align256(sizeof(double) * std::max(IP_SPLITS * nb * n, 64 * nb * nb)) +
-         align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
+         align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096) + align256(sizeof(double) * 4);
 }
 
 // split-K of a product over K rows (K a multiple of 16) for a (rows x cols)
@@ -111,6 +111,8 @@ hipError_t interp_os(hipStream_t st, int64_t l, int64_t n, int64_t k, const doub
   double* Pbuf = (double*)p;
   p += align256(sizeof(double) * std::max(IP_SPLITS * nb * n, 64 * nb * nb));
   double* Dinv = (double*)p;
+  p += align256(sizeof(double) * 4 * ((k + 255) / 256) * 4096);
+  double* scr = (double*)p;
   hipError_t e;
 #define IPCHK(x)                          \
   do {                                    \
@@ -122,10 +124,9 @@ hipError_t interp_os(hipStream_t st, int64_t l, int64_t n, int64_t k, const doub
   IPCHK(rbrp_launch_gather_cols(st, l, n, Y, ldy, P, Yp, lp));
   for (int64_t j0 = 0; j0 < k; j0 += nb) {
     const int64_t jb = std::min(nb, k - j0), je = j0 + jb;
-    for (int64_t j = j0; j < je; ++j) {
-      IPCHK(rbrp_launch_hh_reflect(st, l, n, j, Yp, lp, nullptr, nullptr, tau, 0));
-      IPCHK(rbrp_launch_hh_apply(st, l, je, j, Yp, lp, tau, nullptr, 0));
-    }
+    // panel steps: one launch per column (reflector and its application to the panel)
+    for (int64_t j = j0; j < je; ++j) IPCHK(rbrp_launch_hh_step(st, l, je, j, Yp, lp, tau, scr, j > j0));
+    IPCHK(rbrp_launch_hh_finish(st, l, je - 1, Yp, lp, scr));
     if (je >= n) continue;
     const int64_t rows = l - j0, rows_pad = lp - j0, nc = n - je;
     double* C = Yp + j0 + je * lp;
