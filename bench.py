@@ -60,8 +60,11 @@ def parse():
     ap.add_argument("--profile-steps", action="store_true",
                     help="print a line per step (for profiler runs)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="run one ID and a 1 GiB device copy, print nothing else (for "
-                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes)")
+                    help="run a 1 GiB device copy and one workload (--probe-what), print nothing "
+                         "else (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); needs "
+                         "--load-matrix, so no solver library runs in the counted process")
+    ap.add_argument("--probe-what", default="colid", choices=["colid", "cpqr"],
+                    help="workload of --traffic-probe: the FP64 RBRP ID or the full-matrix CPQR ID")
     ap.add_argument("--save-matrix", default="",
                     help="write the generated matrix (column-major .npy) and exit")
     ap.add_argument("--load-matrix", default="",
@@ -81,6 +84,12 @@ def parse():
     ap.add_argument("--batch-cpu-sample", type=int, default=48,
                     help="problems of the batch timed with the CPU oracle")
     args = ap.parse_args()
+    if args.traffic_probe and not args.load_matrix:
+        # the device generator factors its Haar matrices with torch.linalg.qr
+        # (rocSOLVER), which has crashed inside counter-collection processes:
+        # the matrix must come from a file written by a separate process
+        ap.error("--traffic-probe needs --load-matrix (write it first with --save-matrix in a "
+                 "process of its own)")
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
     args.no_comparators = args.no_comparators or "comparators" not in legs
     args.no_cpqr = "cpqr" not in legs
@@ -350,9 +359,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         with Handle() as hp:
-            r = rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=hp)
+            if args.probe_what == "cpqr":
+                from robust_blockwise_random_pivoting_amd import CPQR
+                t_cpqr = CPQR(A, [min(m, n) // 2], handle=hp)[2]
+                what = f"one CPQR ID t={t_cpqr:.3f}s"
+            else:
+                r = rbrp_colid(A, b=args.b, tol=args.tol, seed=args.seed, handle=hp)
+                what = f"one ID k={r.k}"
         torch.cuda.synchronize()
-        print(f"[bench] traffic probe: copy 1 GiB, one ID k={r.k}", file=sys.stderr, flush=True)
+        print(f"[bench] traffic probe: copy 1 GiB, {what}", file=sys.stderr, flush=True)
         return
 
     if args.no_main:

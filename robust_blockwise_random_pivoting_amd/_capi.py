@@ -1,4 +1,5 @@
-"""ctypes binding of the C ABI declared in ``include/rbrp.h`` and ``include/rbrp_testing.h``.
+"""ctypes binding of the C ABI declared in ``include/rbrp.h`` (``librbrp.so``) and of the
+test hooks of ``include/rbrp_testing.h`` (``librbrp_testing.so``).
 
 Only plain pointers and integers cross this boundary.  Loading fails loudly
 (ImportError) when ``librbrp.so`` has not been built: there is no fallback
@@ -13,6 +14,7 @@ from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_int, c_int3
                     c_uint64, c_void_p)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librbrp.so")
+TESTING_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librbrp_testing.so")
 
 RBRP_ERR_NUMERIC = 100000
 RBRP_ERR_ALLOC = 100001
@@ -55,6 +57,49 @@ class RbrpPhaseStats(Structure):
 
 
 _lib = None
+_hooks = None
+
+
+def _hook_signatures():
+    vp, i64, u64, i32p = c_void_p, c_int64, c_uint64, c_void_p
+    return {
+        "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
+        "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
+        "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
+        "rbrp_test_gemm_tn64": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
+        "rbrp_test_gemm_nn64": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
+        "rbrp_test_resident": (c_int, [c_int]),
+        "rbrp_test_gemm_tn32": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
+        "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
+        "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
+                                    POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+        "rbrp_test_panel_inv": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, vp, i32p,
+                                        i32p, POINTER(c_int32), POINTER(c_int32),
+                                        POINTER(c_int32)]),
+        "rbrp_test_chol2": (c_int, [vp, vp, i64, i64, vp, vp, vp, vp, POINTER(c_int32)]),
+        "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
+                                     POINTER(c_int32), POINTER(c_uint64)]),
+    }
+
+
+def testing_lib():
+    """Load the kernel test hooks ``librbrp_testing.so`` (include/rbrp_testing.h) once.
+
+    Separate from ``librbrp.so`` so the product library exports only the
+    include/rbrp.h surface; raises ImportError if it is missing."""
+    global _hooks
+    if _hooks is not None:
+        return _hooks
+    lib()
+    if not os.path.exists(TESTING_LIB_PATH):
+        raise ImportError(f"{TESTING_LIB_PATH} is not built; run build_lib")
+    L = ctypes.CDLL(TESTING_LIB_PATH)
+    for name, (res, args) in _hook_signatures().items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _hooks = L
+    return L
 
 
 def lib():
@@ -98,22 +143,6 @@ def lib():
         "rbrp_matrix_ge_h": (c_int, [i64, u64, vp]),
         "rbrp_matrix_snn_h": (c_int, [i64, c_double, u64, vp]),
         "rbrp_matrix_gmm_h": (c_int, [i64, i64, u64, vp]),
-        "rbrp_test_gemm_tn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
-        "rbrp_test_gemm_nn": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int, vp]),
-        "rbrp_test_gemm_upd": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
-        "rbrp_test_gemm_tn64": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
-        "rbrp_test_gemm_nn64": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
-        "rbrp_test_resident": (c_int, [c_int]),
-        "rbrp_test_gemm_tn32": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
-        "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),
-        "rbrp_test_panel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, i32p, i32p,
-                                    POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
-        "rbrp_test_panel_inv": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, vp, i32p,
-                                        i32p, POINTER(c_int32), POINTER(c_int32),
-                                        POINTER(c_int32)]),
-        "rbrp_test_chol2": (c_int, [vp, vp, i64, i64, vp, vp, vp, vp, POINTER(c_int32)]),
-        "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
-                                     POINTER(c_int32), POINTER(c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -153,5 +182,5 @@ def philox_block(ctr, key):
     return [int(x) for x in out]
 
 
-__all__ = ["lib", "check", "RBRPError", "RbrpOpts", "RbrpResult", "RbrpPhaseStats", "PHASES",
+__all__ = ["lib", "testing_lib", "check", "RBRPError", "RbrpOpts", "RbrpResult", "RbrpPhaseStats", "PHASES",
            "FILTERS", "PRECISIONS", "philox_block", "byref", "LIB_PATH"]

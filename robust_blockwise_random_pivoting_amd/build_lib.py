@@ -1,7 +1,10 @@
 """Build the in-tree HIP library ``librbrp.so`` for gfx950 with hipcc.
 
 Each source is compiled to an object in ``_build/`` (in parallel), then linked
-into ``robust_blockwise_random_pivoting_amd/librbrp.so``.  Run as
+into ``robust_blockwise_random_pivoting_amd/librbrp.so``.  The kernel-level
+test hooks (``include/rbrp_testing.h``) go into a separate
+``librbrp_testing.so`` linked against it, so the product library exports only
+the ``include/rbrp.h`` surface.  Run as
 ``python -m robust_blockwise_random_pivoting_amd.build_lib``.
 """
 
@@ -18,11 +21,13 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "librbrp.so")
+TESTING_LIB = os.path.join(PKG, "librbrp_testing.so")
 CXX_CHECK = os.path.join(PKG, "bin", "rbrp_hpp_check")
 DRIVER = os.path.join(PKG, "bin", "rbrp_driver")
 
 SOURCES = ["k_gemm.hip", "k_rbrp.hip", "k_sketch.hip", "k_mixed.hip", "k_batched.hip", "k_panel.hip",
-           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_matrices.cpp", "rbrp_batched.cpp", "rbrp_testing.cpp"]
+           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_matrices.cpp", "rbrp_batched.cpp"]
+TESTING_SOURCES = ["rbrp_testing.cpp"]
 
 ARCH = os.environ.get("RBRP_OFFLOAD_ARCH", "gfx950")
 
@@ -60,14 +65,24 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES) + len(TESTING_SOURCES))) as ex:
+        objs = list(ex.map(_compile, SOURCES + TESTING_SOURCES))
+    tobjs = objs[len(SOURCES):]
+    objs = objs[:len(SOURCES)]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # test hooks: their own library, resolving the kernel launchers in librbrp.so
+    tmp = TESTING_LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + tobjs + \
+        ["-L", PKG, "-lrbrp", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"test-hook library link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, TESTING_LIB)
     # C++ drop-in check of include/rbrp.hpp, linked against the library
     os.makedirs(os.path.dirname(CXX_CHECK), exist_ok=True)
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),

@@ -2,6 +2,8 @@
 // weighted sampler, the LDS-resident pivoted Cholesky + robust filter of the
 // sampled block, CholQR2 re-factorisation, the symmetric-difference
 // permutation, residual sums and the diagonal-block solve of X = R11^{-1} R12.
+#include <algorithm>
+
 #include "rbrp_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -67,6 +69,20 @@ __global__ __launch_bounds__(1024) void k_sum_to_state(const double* __restrict_
 hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
                                     int which) {
   hipLaunchKernelGGL(k_sum_to_state, dim3(1), dim3(1024), 0, st, x, n, bs, which);
+  return hipGetLastError();
+}
+
+// p[0:count) = 0 (a kernel rather than a memset node, so a captured graph
+// replays it like any other launch).
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, int64_t count) {
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < count; j += stride) p[j] = 0.0;
+}
+
+hipError_t rbrp_launch_zero(hipStream_t st, double* p, int64_t count) {
+  if (count <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((count + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, count);
   return hipGetLastError();
 }
 
@@ -739,8 +755,8 @@ hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockS
 // fields are written first, then `seq` with a system-scope release, so a host
 // that reads seq == the expected number (with acquire) sees the whole state.
 __device__ void resid_total_publish(const double* __restrict__ diags, int64_t c0, int64_t n,
-                                    RbrpBlockState* bs, RbrpBlockState* pub, int seq,
-                                    double* red) {
+                                    RbrpBlockState* bs, RbrpBlockState* pub, RbrpBlockState* snap,
+                                    int seq, double* red) {
   double acc = 0.0;
   for (int64_t j = c0 + threadIdx.x; j < n; j += 1024) acc += diags[j];
   const double t = rbrp_block_sum(acc, red);
@@ -748,16 +764,30 @@ __device__ void resid_total_publish(const double* __restrict__ diags, int64_t c0
     const double err = (bs->N0 > 0.0) ? t / bs->N0 : 0.0;
     bs->resid = t;
     bs->err = err;
+    RbrpBlockState rec;
+    rec.s = bs->s;
+    rec.bt = bs->bt;
+    rec.nswap = bs->nswap;
+    rec.status = bs->status;
+    rec.draws = bs->draws;
+    rec.resid = t;
+    rec.err = err;
+    rec.N0 = bs->N0;
+    rec.nnz = bs->nnz;
+    rec.seq = seq;
+    // device snapshot of this block's record: no later kernel writes it, so
+    // the host can copy it after the stream has gone idle
+    if (snap) *snap = rec;
     if (pub) {
-      pub->s = bs->s;
-      pub->bt = bs->bt;
-      pub->nswap = bs->nswap;
-      pub->status = bs->status;
-      pub->draws = bs->draws;
-      pub->resid = t;
-      pub->err = err;
-      pub->N0 = bs->N0;
-      pub->nnz = bs->nnz;
+      pub->s = rec.s;
+      pub->bt = rec.bt;
+      pub->nswap = rec.nswap;
+      pub->status = rec.status;
+      pub->draws = rec.draws;
+      pub->resid = rec.resid;
+      pub->err = rec.err;
+      pub->N0 = rec.N0;
+      pub->nnz = rec.nnz;
       __hip_atomic_store(&pub->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
@@ -816,14 +846,16 @@ hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_
 // Residual total after the block (see resid_total_publish).
 __global__ __launch_bounds__(1024) void k_resid_total(const double* __restrict__ diags, int64_t i,
                                                       int64_t n, RbrpBlockState* bs,
-                                                      RbrpBlockState* pub, int seq) {
+                                                      RbrpBlockState* pub, RbrpBlockState* snap,
+                                                      int seq) {
   __shared__ double red[16];
-  resid_total_publish(diags, i + bs->bt, n, bs, pub, seq, red);
+  resid_total_publish(diags, i + bs->bt, n, bs, pub, snap, seq, red);
 }
 
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
-                                   RbrpBlockState* bs, RbrpBlockState* pub, int seq) {
-  hipLaunchKernelGGL(k_resid_total, dim3(1), dim3(1024), 0, st, diags, i, n, bs, pub, seq);
+                                   RbrpBlockState* bs, RbrpBlockState* pub, int seq,
+                                   RbrpBlockState* snap) {
+  hipLaunchKernelGGL(k_resid_total, dim3(1), dim3(1024), 0, st, diags, i, n, bs, pub, snap, seq);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ struct RbrpWorkspace {
   double *Q64, *Vd;
   int *J, *Jk, *Jin, *Jout, *Jkpos;
   RbrpBlockState* st;
+  RbrpBlockState* st_snap;  // the last published block record (k_resid_total)
   int64_t Pcap;
 };
 

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -49,83 +50,110 @@ struct Carver {
 
 namespace {
 
-int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed) {
+// Byte offsets of the loop's buffers in one arena (every buffer 256-byte aligned).
+struct WsLayout {
+  size_t oAw, oQ64, oVd, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
+      onp, oWt, otk, oto, oJ, oJk, oJin, oJout, oJkpos, oDinv, ost;
+  size_t need;
+  int64_t Pcap;
+};
+
+// Split-K partials of the full-size arena: 64 splits of a 128-row product
+// over max(n, kmax) columns.
+int64_t full_pcap(int64_t n, int64_t kmax) {
+  return 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
+}
+
+WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed, int64_t Pcap) {
   const int64_t ldr = std::max<int64_t>(kmax, 1);
-  const int64_t Pcap = 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
+  WsLayout L;
   Carver c;
-  size_t oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
-  size_t oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
-  size_t oVd = mixed ? c.take<double>(mp * b) : 0;
-  size_t oR = c.take<double>(ldr * n);
-  size_t odiags = c.take<double>(n);
-  size_t ow = c.take<double>(n);
-  size_t ocdf = c.take<double>(n);
-  size_t oG = c.take<double>(b * b);
-  size_t oG2 = c.take<double>(b * b);
-  size_t oRb = c.take<double>(b * b);
-  size_t oRinv = c.take<double>(b * b);
-  size_t oR2 = c.take<double>(b * b);
-  size_t oR2inv = c.take<double>(b * b);
-  size_t oCbuf = c.take<double>(ldr * b);
-  size_t oQbuf = c.take<double>(mp * b);
-  size_t oP = c.take<double>(Pcap);
-  size_t onp = c.take<double>(((mp + 63) / 64) * n);  // 64-row (FP64) or 128-row (FP32) tiles
-  size_t oWt = c.take<double>(b * n);  // FP32 in the mixed loop (over-sized)
-  size_t otk = c.take<double>(ldr * b);
-  size_t oto = c.take<double>(ldr * b);
-  size_t oJ = c.take<int>(b);
-  size_t oJk = c.take<int>(b);
-  size_t oJin = c.take<int>(b);
-  size_t oJout = c.take<int>(b);
-  size_t oJkpos = c.take<int>(b);
-  size_t oDinv = c.take<double>(4 * ((kmax + 255) / 256) * 4096);  // diagonal-block inverses
-  size_t ost = c.take<RbrpBlockState>(1);
-  const size_t need = c.off;
+  L.oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
+  L.oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
+  L.oVd = mixed ? c.take<double>(mp * b) : 0;
+  L.oR = c.take<double>(ldr * n);
+  L.ost = c.take<RbrpBlockState>(2);  // the block state and its published snapshot (after R:
+                                      // one fill kernel clears both)
+  L.odiags = c.take<double>(n);
+  L.ow = c.take<double>(n);
+  L.ocdf = c.take<double>(n);
+  L.oG = c.take<double>(b * b);
+  L.oG2 = c.take<double>(b * b);
+  L.oRb = c.take<double>(b * b);
+  L.oRinv = c.take<double>(b * b);
+  L.oR2 = c.take<double>(b * b);
+  L.oR2inv = c.take<double>(b * b);
+  L.oCbuf = c.take<double>(ldr * b);
+  L.oQbuf = c.take<double>(mp * b);
+  L.oP = c.take<double>(Pcap);
+  L.onp = c.take<double>(((mp + 63) / 64) * n);  // 64-row (FP64) or 128-row (FP32) tiles
+  L.oWt = c.take<double>(b * n);                  // FP32 in the mixed loop (over-sized)
+  L.otk = c.take<double>(ldr * b);
+  L.oto = c.take<double>(ldr * b);
+  L.oJ = c.take<int>(b);
+  L.oJk = c.take<int>(b);
+  L.oJin = c.take<int>(b);
+  L.oJout = c.take<int>(b);
+  L.oJkpos = c.take<int>(b);
+  L.oDinv = c.take<double>(4 * ((kmax + 255) / 256) * 4096);  // diagonal-block inverses
+  L.need = c.off;
+  L.Pcap = Pcap;
+  return L;
+}
+
+// Points the workspace's buffers into the arena at `base`.
+void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixed) {
+  ws.Aw = mixed ? nullptr : (double*)(base + L.oAw);
+  ws.Aw32 = mixed ? (float*)(base + L.oAw) : nullptr;
+  ws.Q64 = mixed ? (double*)(base + L.oQ64) : nullptr;
+  ws.Vd = mixed ? (double*)(base + L.oVd) : nullptr;
+  ws.R = (double*)(base + L.oR);
+  ws.diags = (double*)(base + L.odiags);
+  ws.w = (double*)(base + L.ow);
+  ws.cdf = (double*)(base + L.ocdf);
+  ws.G = (double*)(base + L.oG);
+  ws.G2 = (double*)(base + L.oG2);
+  ws.Rb = (double*)(base + L.oRb);
+  ws.Rinv = (double*)(base + L.oRinv);
+  ws.R2 = (double*)(base + L.oR2);
+  ws.R2inv = (double*)(base + L.oR2inv);
+  ws.Cbuf = (double*)(base + L.oCbuf);
+  ws.Qbuf = (double*)(base + L.oQbuf);
+  ws.P = (double*)(base + L.oP);
+  ws.normpart = (double*)(base + L.onp);
+  ws.Wt = mixed ? nullptr : (double*)(base + L.oWt);
+  ws.Wt32 = mixed ? (float*)(base + L.oWt) : nullptr;
+  ws.tmpk = (double*)(base + L.otk);
+  ws.tmpo = (double*)(base + L.oto);
+  ws.J = (int*)(base + L.oJ);
+  ws.Jk = (int*)(base + L.oJk);
+  ws.Jin = (int*)(base + L.oJin);
+  ws.Jout = (int*)(base + L.oJout);
+  ws.Jkpos = (int*)(base + L.oJkpos);
+  ws.Dinv = (double*)(base + L.oDinv);
+  ws.st = (RbrpBlockState*)(base + L.ost);
+  ws.st_snap = ws.st + 1;
+  ws.Pcap = L.Pcap;
+}
+
+// The handle's cached arena, grown on demand.
+int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed) {
+  const WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax));
   RbrpWorkspace& ws = h->ws;
-  if (ws.arena_bytes < need) {
+  if (ws.arena_bytes < L.need) {
     if (ws.arena) {
       (void)hipFree(ws.arena);
       ws.arena = nullptr;
       ws.arena_bytes = 0;
     }
-    hipError_t e = hipMalloc(&ws.arena, need);
+    hipError_t e = hipMalloc(&ws.arena, L.need);
     if (e != hipSuccess) {
       ws.arena = nullptr;
       return RBRP_ERR_ALLOC;
     }
-    ws.arena_bytes = need;
+    ws.arena_bytes = L.need;
   }
-  char* base = (char*)ws.arena;
-  ws.Aw = mixed ? nullptr : (double*)(base + oAw);
-  ws.Aw32 = mixed ? (float*)(base + oAw) : nullptr;
-  ws.Q64 = mixed ? (double*)(base + oQ64) : nullptr;
-  ws.Vd = mixed ? (double*)(base + oVd) : nullptr;
-  ws.R = (double*)(base + oR);
-  ws.diags = (double*)(base + odiags);
-  ws.w = (double*)(base + ow);
-  ws.cdf = (double*)(base + ocdf);
-  ws.G = (double*)(base + oG);
-  ws.G2 = (double*)(base + oG2);
-  ws.Rb = (double*)(base + oRb);
-  ws.Rinv = (double*)(base + oRinv);
-  ws.R2 = (double*)(base + oR2);
-  ws.R2inv = (double*)(base + oR2inv);
-  ws.Cbuf = (double*)(base + oCbuf);
-  ws.Qbuf = (double*)(base + oQbuf);
-  ws.P = (double*)(base + oP);
-  ws.normpart = (double*)(base + onp);
-  ws.Wt = mixed ? nullptr : (double*)(base + oWt);
-  ws.Wt32 = mixed ? (float*)(base + oWt) : nullptr;
-  ws.tmpk = (double*)(base + otk);
-  ws.tmpo = (double*)(base + oto);
-  ws.J = (int*)(base + oJ);
-  ws.Jk = (int*)(base + oJk);
-  ws.Jin = (int*)(base + oJin);
-  ws.Jout = (int*)(base + oJout);
-  ws.Jkpos = (int*)(base + oJkpos);
-  ws.Dinv = (double*)(base + oDinv);
-  ws.st = (RbrpBlockState*)(base + ost);
-  ws.Pcap = Pcap;
+  carve_workspace(ws, (char*)ws.arena, L, mixed);
   return 0;
 }
 
@@ -227,8 +255,10 @@ struct PhaseTimer {
 // Wait until the block's last kernel has published state number `seq` into
 // the pinned host copy (k_resid_total): spin on the sequence number, polling
 // the stream now and then so that an error or a stream that finished without
-// the publication ends the wait (the state is then copied).
-hipError_t wait_published(rbrp_handle_t h, hipStream_t st, const RbrpBlockState* dev_state, int seq) {
+// the publication ends the wait.  In the latter case the record is copied from
+// the device snapshot the same kernel wrote (`snap`, which the next block's
+// queued kernels do not touch); a snapshot of another block is an error.
+hipError_t wait_published(rbrp_handle_t h, hipStream_t st, const RbrpBlockState* snap, int seq) {
   for (uint64_t it = 1;; ++it) {
     if (__atomic_load_n(&h->host_state->seq, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
     if ((it & 1023) == 0) {
@@ -236,7 +266,12 @@ hipError_t wait_published(rbrp_handle_t h, hipStream_t st, const RbrpBlockState*
       if (q == hipErrorNotReady) continue;
       if (q != hipSuccess) return q;
       if (__atomic_load_n(&h->host_state->seq, __ATOMIC_ACQUIRE) == seq) return hipSuccess;
-      return hipMemcpy(h->host_state, dev_state, sizeof(RbrpBlockState), hipMemcpyDeviceToHost);
+      RbrpBlockState rec;
+      const hipError_t e = hipMemcpy(&rec, snap, sizeof(RbrpBlockState), hipMemcpyDeviceToHost);
+      if (e != hipSuccess) return e;
+      if (rec.seq != seq) return hipErrorUnknown;
+      *h->host_state = rec;
+      return hipSuccess;
     }
 #if defined(__x86_64__)
     __builtin_ia32_pause();
@@ -469,8 +504,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     res->t_total = now_s() - t_start;
     return 0;
   }
-  HIPCHK(rbrp_launch_iota(st, res->perm, n));
   if (m == 0) {
+    HIPCHK(rbrp_launch_iota(st, res->perm, n));
     HIPCHK(hipStreamSynchronize(st));
     res->t_total = now_s() - t_start;
     return 0;
@@ -480,10 +515,12 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const int64_t mp = round_up(m, 64);
   const int64_t ldr = std::max<int64_t>(kmax, 1);
   {
-    int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
+    // (before the first launch: a workspace that cannot be had leaves every output untouched)
+    const int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
     if (e) return e;
   }
   RbrpWorkspace& ws = h->ws;
+  HIPCHK(rbrp_launch_iota(st, res->perm, n));
   int* perm = res->perm;
   const double tol2 = opts->tol * opts->tol;
   const uint64_t key0 = opts->seed, key1 = 0;
@@ -492,7 +529,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const double ew = mixed ? 4.0 : 8.0;      // bytes per element of the working matrix
   PhaseTimer pt(h);
 
-  HIPCHK(hipMemsetAsync(ws.R, 0, sizeof(double) * ldr * n, st));
+  // R and the two block-state records (laid out right after it) start at zero
+  HIPCHK(rbrp_launch_zero(st, ws.R, (int64_t)(((char*)(ws.st + 2) - (char*)ws.R) / sizeof(double))));
   int ph = pt.begin(RBRP_PHASE_NORMS);
   if (mixed)
     HIPCHK(rbrp_launch_copy_colnorm32(st, dA, a_f32 ? 1 : 0, lda, m, mp, n, ws.Aw32, ws.diags));
@@ -680,7 +718,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     ph = pt.begin(RBRP_PHASE_RESID);
     if (nrem_max > 0)
       HIPCHK(rbrp_launch_colnorm_reduce(st, ws.normpart, nrem_max, groups, i, n, ws.st, ws.diags));
-    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq));
+    HIPCHK(rbrp_launch_resid_total(st, ws.diags, i, n, ws.st, h->host_state, seq, ws.st_snap));
     pt.end(ph, nrem_max > 0 ? 2 : 1);
 
     // the next block's front goes on the stream before this block's state is
@@ -691,7 +729,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       const int e = queue_front(i, true, &ph_cgs_next);
       if (e != 0) return e;
     }
-    HIPCHK(wait_published(h, st, ws.st, seq));
+    HIPCHK(wait_published(h, st, ws.st_snap, seq));
     const RbrpBlockState hs = *h->host_state;
     if (hs.status != 0) {
       pt.flush();

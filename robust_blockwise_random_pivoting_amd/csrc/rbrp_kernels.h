@@ -152,6 +152,8 @@ hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda
 hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
                                     int which);
 hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n);
+// p[0:count) = 0 by a fill kernel.
+hipError_t rbrp_launch_zero(hipStream_t st, double* p, int64_t count);
 // Sampler of a block whose first column is i (+ *i_dyn with i_dyn: then the
 // state holds the previous block's results and the call does nothing, with
 // s = 0, when that block ended the loop; tol2 is the squared tolerance).
@@ -198,9 +200,11 @@ hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockS
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
                                       int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
 // bs->resid / err after the block; with pub (pinned host-coherent memory) the
-// state is also published there, seq written last with a system-scope release.
+// state is also published there, seq written last with a system-scope release;
+// with snap the same record (seq included) is also stored there on the device.
 hipError_t rbrp_launch_resid_total(hipStream_t st, const double* diags, int64_t i, int64_t n,
-                                   RbrpBlockState* bs, RbrpBlockState* pub = nullptr, int seq = 0);
+                                   RbrpBlockState* bs, RbrpBlockState* pub = nullptr, int seq = 0,
+                                   RbrpBlockState* snap = nullptr);
 hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                   int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,

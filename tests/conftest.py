@@ -39,3 +39,10 @@ def librbrp():
         from robust_blockwise_random_pivoting_amd import build_lib
         build_lib.build()
     return _capi.lib()
+
+
+@pytest.fixture(scope="session")
+def hooks(librbrp):
+    """The kernel test hooks (include/rbrp_testing.h), a library of their own."""
+    from robust_blockwise_random_pivoting_amd import _capi
+    return _capi.testing_lib()

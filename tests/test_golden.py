@@ -100,7 +100,7 @@ def test_device_uniforms_match_fixture(librbrp):
 
 
 @pytest.mark.gpu
-def test_device_sampler_matches_choice_fixture(librbrp):
+def test_device_sampler_matches_choice_fixture(librbrp, hooks):
     import torch
     d = _load("sampler_choice.npz")
     for q in _cases(d):
@@ -110,7 +110,7 @@ def test_device_sampler_matches_choice_fixture(librbrp):
         dJ = torch.zeros(s, dtype=torch.int32, device="cuda")
         s_out, draws = ctypes.c_int32(), ctypes.c_uint64()
         print(f"sampler fixture case {q}: N={N} s={s}", flush=True)
-        code = librbrp.rbrp_test_sample(_stream(), _ptr(dp), 0, N, N, s, seed, off, pre, _ptr(dJ),
+        code = hooks.rbrp_test_sample(_stream(), _ptr(dp), 0, N, N, s, seed, off, pre, _ptr(dJ),
                                         ctypes.byref(s_out), ctypes.byref(draws))
         assert code == 0
         assert s_out.value == s, q
@@ -119,7 +119,7 @@ def test_device_sampler_matches_choice_fixture(librbrp):
 
 
 @pytest.mark.gpu
-def test_device_panel_matches_dgeqp3_fixture(librbrp):
+def test_device_panel_matches_dgeqp3_fixture(librbrp, hooks):
     """Gram + pivoted Cholesky on the device: dgeqp3's pivots through the filter's kept count."""
     import torch
     d = _load("dgeqp3_pivots.npz")
@@ -138,7 +138,7 @@ def test_device_panel_matches_dgeqp3_fixture(librbrp):
         dRb = torch.zeros(b * b, dtype=torch.float64, device="cuda")
         bt, ns, status = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         print(f"panel fixture case {q}: s={s}", flush=True)
-        code = librbrp.rbrp_test_panel(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
+        code = hooks.rbrp_test_panel(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
                                        _ptr(dRb), _ptr(dJin), _ptr(dJout), ctypes.byref(bt),
                                        ctypes.byref(ns), ctypes.byref(status))
         assert code == 0 and status.value == 0, q

@@ -44,6 +44,7 @@ def _check_against_oracle(A, g, b, tol, seed, offset=0, filt="sampled", max_rank
         X = g.X.cpu().numpy()
         kappa = np.linalg.cond(r.R[:, : r.k])
         rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
+        print(f"[{name}] X rel diff {rel:.2e} (bound {max(1e-10, 100 * kappa * EPS):.2e})")
         assert rel <= max(1e-10, 100 * kappa * EPS), (name, rel, kappa)
         eg = id_error(A, perm, g.k, X)
         er = id_error(A, r.perm, r.k, r.X)

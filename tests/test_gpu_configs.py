@@ -1,0 +1,215 @@
+"""The benchmark workloads at their own shapes (GPU).
+
+* Config 5 (FP32-stored 1/i spectrum with standard-Cauchy outlier rows, b =
+  128, tol 1e-5): the mixed form and the FP64 loop on the FP32 input against
+  the FP64 oracle on the widened values at 30000 x 1024 (with the config's
+  0.05% outlier rows, and with 500 of them so the run takes eight blocks), and
+  the full 1,000,000 x 1024 run through size-independent properties.
+* Config 4 (Gaussian-kernel blocks, m ~ U{64..1024}, n ~ U{64..512}, b = 16,
+  tol 1e-7): the persistent batched launch against the oracle on a sample of
+  that range.
+* Config 3 (100000 x 2000 Gaussian mixture with a x50 outlier component, b =
+  64, tol 1e-6) and config 2 (50000 x 4000 exp-decay, b = 64): the oracle at
+  full size for the first five blocks (max_rank), and full runs through
+  properties.
+
+Oracle bar as in the other parity tests: identical skeleton, ranks and draws
+for the FP64 loop; for the mixed form, rank within one block, error within
+10% of max(tol, oracle error), tracked error equal to the recomputed one.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import id_error, rbrp_colid as oracle_colid
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(np.float64).eps
+
+
+def _dev(A):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(A.T)).cuda().t()
+
+
+def _x_rel(g, r):
+    X = g.X.cpu().numpy()
+    kappa = np.linalg.cond(r.R[:, : r.k])
+    rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
+    return X, rel, max(1e-10, 100 * kappa * EPS), kappa
+
+
+def _fp64_parity(A64, g, r, name):
+    n = A64.shape[1]
+    perm = g.perm.cpu().numpy()
+    assert g.k == r.k, name
+    assert np.array_equal(perm[: g.k], r.perm[: r.k]), name
+    assert np.array_equal(np.sort(perm), np.arange(n)), name
+    assert np.array_equal(g.ranks, r.ranks), name
+    assert g.draws == r.draws, name
+    if 0 < g.k < n:
+        X, rel, bound, kappa = _x_rel(g, r)
+        print(f"[{name}] k={g.k} X rel diff {rel:.2e} (bound {bound:.2e}, kappa {kappa:.2e})")
+        assert rel <= bound, (name, rel, kappa)
+        eg = id_error(A64, perm, g.k, X)
+        er = id_error(A64, r.perm, r.k, r.X)
+        assert abs(eg - er) <= 0.01 * er + 1e-15, (name, eg, er)
+    np.testing.assert_allclose(g.error, r.error, rtol=1e-6, atol=1e-15)
+
+
+def _c5_matrix(m, n_outlier=None):
+    from robust_blockwise_random_pivoting_amd.matrices import poly_outlier_torch
+    return poly_outlier_torch(m, 1024, seed=5, n_outlier=n_outlier)
+
+
+C5_CASES = [("c5 30000x1024 0.05% Cauchy", 30000, None),
+            ("c5 30000x1024 500 Cauchy rows", 30000, 500)]
+
+
+@pytest.mark.parametrize("name,m,nout", C5_CASES)
+def test_config5_fp64_loop_on_fp32_input_matches_oracle(name, m, nout):
+    """rbrp_colid_s with FP64 arithmetic == the oracle on the widened values (b = 128)."""
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    A32 = _c5_matrix(m, nout)
+    A64 = A32.cpu().numpy().astype(np.float64)
+    g = rbrp_colid(A32, b=128, tol=1e-5, seed=1, precision="fp64")
+    r = oracle_colid(np.asfortranarray(A64), 128, 1e-5, seed=1)
+    _fp64_parity(A64, g, r, name)
+
+
+@pytest.mark.parametrize("name,m,nout", C5_CASES)
+def test_config5_mixed_against_oracle(name, m, nout):
+    """The mixed form on the FP32-stored matrix (FP32 residual, b = 128) against the
+    FP64 oracle on the widened values."""
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    A32 = _c5_matrix(m, nout)
+    A64 = A32.cpu().numpy().astype(np.float64)
+    tol, b = 1e-5, 128
+    g = rbrp_colid(A32, b=b, tol=tol, seed=1, precision="fp32")
+    r = oracle_colid(np.asfortranarray(A64), b, tol, seed=1)
+    n = A64.shape[1]
+    perm = g.perm.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(n)), name
+    e32 = id_error(A64, perm, g.k, g.X.cpu().numpy()) if g.k < n else 0.0
+    e64 = id_error(A64, r.perm, r.k, r.X) if r.k < n else 0.0
+    same = int(np.sum(perm[: min(g.k, r.k)] == r.perm[: min(g.k, r.k)]))
+    print(f"[{name} mixed] k32={g.k} k64={r.k} err32={e32:.4e} err64={e64:.4e} "
+          f"tracked32={g.err_final:.4e} same-prefix={same}")
+    assert abs(g.k - r.k) <= b, name
+    assert e32 <= 1.1 * tol, (name, e32)
+    assert e32 <= 1.1 * max(e64, tol), (name, e32, e64)
+    assert g.err_final == pytest.approx(e32, rel=0.05, abs=1e-8), name
+
+
+def test_config5_full_size_properties():
+    """1,000,000 x 1024 FP32-stored, 500 Cauchy rows, b = 128, tol 1e-5, mixed form:
+    perm is a permutation, the device-recomputed FP64 error is within tol and equals
+    the tracked one, and the FP64 loop on the same input agrees."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    A32 = _c5_matrix(1000000)
+    n, tol = 1024, 1e-5
+    g32 = rbrp_colid(A32, b=128, tol=tol, seed=1, precision="fp32")
+    A = A32.double()
+    nA = torch.linalg.norm(A)
+
+    def err(g):
+        if g.k >= n:
+            return 0.0
+        J, rd = g.perm[: g.k].long(), g.perm[g.k:].long()
+        return float(torch.linalg.norm(A[:, rd] - A[:, J] @ g.X) / nA)
+
+    perm = g32.perm.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    e32 = err(g32)
+    g64 = rbrp_colid(A, b=128, tol=tol, seed=1)
+    e64 = err(g64)
+    print(f"[c5 1e6x1024] k32={g32.k} k64={g64.k} err32={e32:.4e} err64={e64:.4e} "
+          f"tracked32={g32.err_final:.4e} blocks={len(g32.ranks)}")
+    assert np.all(np.diff(g32.error) <= 0)
+    assert e32 <= tol and e64 <= tol
+    assert g32.err_final == pytest.approx(e32, rel=0.05)
+    assert g64.err_final == pytest.approx(e64, rel=1e-3)
+    assert abs(g32.k - g64.k) <= 128
+
+
+def test_config4_range_matches_oracle():
+    """16 kernel blocks drawn from config 4's full range (m ~ U{64..1024}, n ~
+    U{64..512}), one persistent launch, each against the oracle."""
+    from robust_blockwise_random_pivoting_amd import rbrp_colid_batched
+    from robust_blockwise_random_pivoting_amd.matrices import kernel_blocks
+    As = kernel_blocks(16, 77, m_range=(64, 1024), n_range=(64, 512))
+    assert max(A.shape[0] for A in As) > 600 and max(A.shape[1] for A in As) > 300
+    out, status = rbrp_colid_batched([_dev(A) for A in As], b=16, tol=1e-7, mode="persistent")
+    assert (status == 0).all()
+    worst = 0.0
+    for q, A in enumerate(As):
+        g = out[q]
+        r = oracle_colid(A, 16, 1e-7, seed=q)
+        nm = f"kernel {q} {A.shape}"
+        perm = g.perm.cpu().numpy()
+        assert g.k == r.k, nm
+        assert np.array_equal(perm[: g.k], r.perm[: r.k]), nm
+        assert g.draws == r.draws, nm
+        if 0 < g.k < A.shape[1]:
+            X, rel, bound, _ = _x_rel(g, r)
+            worst = max(worst, rel / bound)
+            assert rel <= bound, (nm, rel)
+            eg = id_error(A, perm, g.k, X)
+            er = id_error(A, r.perm, r.k, r.X)
+            assert abs(eg - er) <= 0.01 * er + 1e-15, (nm, eg, er)
+    print(f"[config4 range] 16 problems, worst X rel diff / bound = {worst:.2e}")
+
+
+def _first_blocks_parity(A_dev, b, tol, name, max_rank=320):
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    g = rbrp_colid(A_dev, b=b, tol=tol, seed=1, max_rank=max_rank)
+    A64 = np.asfortranarray(A_dev.cpu().numpy())
+    r = oracle_colid(A64, b, tol, seed=1, max_rank=max_rank)
+    _fp64_parity(A64, g, r, name)
+    return g
+
+
+def test_config3_full_size_first_blocks_match_oracle():
+    """100000 x 2000 Gaussian mixture (x50 outlier component), b = 64: the first
+    320 columns (five blocks) identical to the oracle's at full size."""
+    from robust_blockwise_random_pivoting_amd.matrices import gmm_rows_torch
+    A = gmm_rows_torch(100000, 2000, seed=3)
+    _first_blocks_parity(A, 64, 1e-6, "c3 100000x2000 first 320")
+
+
+def test_config3_full_size_properties():
+    """100000 x 2000 Gaussian mixture, b = 64, tol 1e-6, FP64 and mixed: perm is
+    a permutation, the rank is near full (the noise floor is about 1% of ||A||^2),
+    the error traces decrease, the recomputed errors are within tol, and the two
+    forms' traces agree while the residual is far above FP32 resolution."""
+    import torch
+    from robust_blockwise_random_pivoting_amd import rbrp_colid
+    from robust_blockwise_random_pivoting_amd.matrices import gmm_rows_torch
+    A = gmm_rows_torch(100000, 2000, seed=3)
+    n = 2000
+    g64 = rbrp_colid(A, b=64, tol=1e-6, seed=1)
+    g32 = rbrp_colid(A, b=64, tol=1e-6, seed=1, precision="fp32")
+    nA = torch.linalg.norm(A)
+    for g in (g64, g32):
+        perm = g.perm.cpu().numpy()
+        assert np.array_equal(np.sort(perm), np.arange(n))
+        assert g.k >= 0.9 * n
+        assert np.all(np.diff(g.error) <= 1e-15)
+        if g.k < n:
+            J, rd = g.perm[: g.k].long(), g.perm[g.k:].long()
+            e = float(torch.linalg.norm(A[:, rd] - A[:, J] @ g.X) / nA)
+            assert e <= 1e-6 * 1.01, e
+    nb = min(len(g64.error), len(g32.error))
+    big = g64.error[:nb] > 1e-6
+    np.testing.assert_allclose(g32.error[:nb][big], g64.error[:nb][big], rtol=1e-3)
+    print(f"[c3 100000x2000] k64={g64.k} k32={g32.k} blocks {len(g64.ranks)}/{len(g32.ranks)}")
+
+
+def test_config2_full_size_first_blocks_match_oracle():
+    """50000 x 4000 exp-decay (s_i = exp(-i/100)), b = 64, tol 1e-8: the first
+    320 columns identical to the oracle's at full size."""
+    from robust_blockwise_random_pivoting_amd.matrices import exp_spectrum, spectrum_matrix_torch
+    A = spectrum_matrix_torch(50000, 4000, exp_spectrum(4000, 100.0), seed=2)
+    _first_blocks_parity(A, 64, 1e-8, "c2 50000x4000 first 320")

@@ -45,14 +45,14 @@ def test_uniforms_match_numpy_philox(librbrp):
 
 @pytest.mark.parametrize("M,N,K,nsplit", [(1, 1, 16, 1), (17, 5, 64, 3), (64, 64, 256, 4),
                                           (100, 130, 1024, 7), (29, 64, 4096, 16)])
-def test_gemm_tn_matches_numpy(librbrp, M, N, K, nsplit):
+def test_gemm_tn_matches_numpy(librbrp, hooks, M, N, K, nsplit):
     import torch
     rng = np.random.default_rng(M * 1000 + N)
     A = rng.standard_normal((K, M))
     B = rng.standard_normal((K, N))
     dA, dB = _cm(A), _cm(B)
     dC = _cm(np.zeros((M, N)))
-    code = librbrp.rbrp_test_gemm_tn(_stream(), M, N, K, _ptr(dA), K, _ptr(dB), K, _ptr(dC), M, nsplit)
+    code = hooks.rbrp_test_gemm_tn(_stream(), M, N, K, _ptr(dA), K, _ptr(dB), K, _ptr(dC), M, nsplit)
     assert code == 0
     ref = A.T @ B
     assert np.abs(_host(dC) - ref).max() <= 1e-12 * np.abs(ref).max() * np.sqrt(K)
@@ -62,7 +62,7 @@ def test_gemm_tn_matches_numpy(librbrp, M, N, K, nsplit):
 @pytest.mark.parametrize("M,N,K,nsplit", [(1, 1, 16, 1), (17, 5, 64, 3), (64, 64, 256, 4),
                                           (100, 130, 1024, 7), (29, 64, 4096, 16),
                                           (64, 300, 50048, 40), (64, 129, 512, 1)])
-def test_gemm_tn64_matches_numpy(librbrp, M, N, K, nsplit):
+def test_gemm_tn64_matches_numpy(librbrp, hooks, M, N, K, nsplit):
     """The 64 x 128-tile projection kernel against NumPy at FP64 rounding."""
     import torch
     rng = np.random.default_rng(M * 1000 + N + K)
@@ -70,7 +70,7 @@ def test_gemm_tn64_matches_numpy(librbrp, M, N, K, nsplit):
     B = rng.standard_normal((K, N))
     dA, dB = _cm(A), _cm(B)
     dC = _cm(np.zeros((M, N)))
-    code = librbrp.rbrp_test_gemm_tn64(_stream(), M, N, K, _ptr(dA), _ptr(dB), K, _ptr(dC), M, nsplit)
+    code = hooks.rbrp_test_gemm_tn64(_stream(), M, N, K, _ptr(dA), _ptr(dB), K, _ptr(dC), M, nsplit)
     assert code == 0
     ref = A.T @ B
     assert np.abs(_host(dC) - ref).max() <= 1e-12 * np.abs(ref).max() * np.sqrt(K)
@@ -83,7 +83,7 @@ def test_gemm_tn64_matches_numpy(librbrp, M, N, K, nsplit):
                                         # two column tiles, more ranges than one per workgroup
                                         (1004, 64, 100, 1), (16, 3, 5, 0), (70000, 128, 300, 1),
                                         (20, 64, 64, 2), (300000, 64, 40, 1)])
-def test_gemm_nn64_matches_numpy(librbrp, M, N, K, mode):
+def test_gemm_nn64_matches_numpy(librbrp, hooks, M, N, K, mode):
     """The long-reduction NN kernel against NumPy at FP64 rounding (set / subtract / add).
     Its rows are split into ranges of 16-row blocks balanced over the resident
     workgroups, so the cases cover ranges of one to four blocks, partial blocks,
@@ -94,7 +94,7 @@ def test_gemm_nn64_matches_numpy(librbrp, M, N, K, mode):
     B = rng.standard_normal((K, N))
     C0 = rng.standard_normal((M, N))
     dA, dB, dC = _cm(A), _cm(B), _cm(C0)
-    code = librbrp.rbrp_test_gemm_nn64(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode)
+    code = hooks.rbrp_test_gemm_nn64(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode)
     assert code == 0
     prod = A @ B
     ref = prod if mode == 0 else (C0 - prod if mode == 1 else C0 + prod)
@@ -103,23 +103,23 @@ def test_gemm_nn64_matches_numpy(librbrp, M, N, K, mode):
     torch.cuda.synchronize()
 
 
-def test_gemm_nn64_rejects_bad_shapes(librbrp):
+def test_gemm_nn64_rejects_bad_shapes(librbrp, hooks):
     import torch
     z = torch.zeros(64, dtype=torch.float64, device="cuda")
-    assert librbrp.rbrp_test_gemm_nn64(_stream(), 6, 1, 1, _ptr(z), 6, _ptr(z), 1, _ptr(z), 6, 0) == -2
-    assert librbrp.rbrp_test_gemm_nn64(_stream(), 4, 1, 1, _ptr(z), 5, _ptr(z), 1, _ptr(z), 4, 0) == -6
+    assert hooks.rbrp_test_gemm_nn64(_stream(), 6, 1, 1, _ptr(z), 6, _ptr(z), 1, _ptr(z), 6, 0) == -2
+    assert hooks.rbrp_test_gemm_nn64(_stream(), 4, 1, 1, _ptr(z), 5, _ptr(z), 1, _ptr(z), 4, 0) == -6
 
 
-def test_gemm_tn64_rejects_bad_shapes(librbrp):
+def test_gemm_tn64_rejects_bad_shapes(librbrp, hooks):
     import torch
     z = torch.zeros(64, dtype=torch.float64, device="cuda")
-    assert librbrp.rbrp_test_gemm_tn64(_stream(), 1, 1, 8, _ptr(z), _ptr(z), 8, _ptr(z), 1, 1) == -4
-    assert librbrp.rbrp_test_gemm_tn64(_stream(), 1, 1, 16, _ptr(z), _ptr(z), 17, _ptr(z), 1, 1) == -7
+    assert hooks.rbrp_test_gemm_tn64(_stream(), 1, 1, 8, _ptr(z), _ptr(z), 8, _ptr(z), 1, 1) == -4
+    assert hooks.rbrp_test_gemm_tn64(_stream(), 1, 1, 16, _ptr(z), _ptr(z), 17, _ptr(z), 1, 1) == -7
 
 
 @pytest.mark.parametrize("M,N,K,mode", [(64, 64, 16, 0), (130, 70, 33, 1), (5, 3, 1, 2),
                                         (256, 200, 64, 1), (64, 1, 100, 0)])
-def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
+def test_gemm_nn_matches_numpy(librbrp, hooks, M, N, K, mode):
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N))
@@ -128,7 +128,7 @@ def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
     ntile = (M + 63) // 64
     import torch
     dP = torch.zeros(ntile * N, dtype=torch.float64, device="cuda")
-    code = librbrp.rbrp_test_gemm_nn(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode,
+    code = hooks.rbrp_test_gemm_nn(_stream(), M, N, K, _ptr(dA), M, _ptr(dB), K, _ptr(dC), M, mode,
                                      _ptr(dP))
     assert code == 0
     AB = A @ B
@@ -143,7 +143,7 @@ def test_gemm_nn_matches_numpy(librbrp, M, N, K, mode):
                                    (300, 17, 128), (64, 1, 100), (5000, 300, 64), (129, 33, 97),
                                    (70000, 40, 60), (50048, 300, 64), (640, 257, 33), (64, 1, 7),
                                    (1024, 129, 64), (4096, 96, 1)])
-def test_gemm_trail_update_matches_numpy(librbrp, M, N, K):
+def test_gemm_trail_update_matches_numpy(librbrp, hooks, M, N, K):
     """The trailing-update launcher against NumPy: shapes with M a multiple of 64
     and K <= 64 take the 128-column kernel with the A block shared through LDS,
     the others the 32-column kernel."""
@@ -156,7 +156,7 @@ def test_gemm_trail_update_matches_numpy(librbrp, M, N, K):
     dBt = torch.as_tensor(np.ascontiguousarray(B)).cuda()   # row-major (k, c) at k*N + c
     ntile = (M + 63) // 64
     dP = torch.zeros(ntile * N, dtype=torch.float64, device="cuda")
-    code = librbrp.rbrp_test_gemm_upd(_stream(), M, N, K, _ptr(dA), M, _ptr(dBt), N, _ptr(dC), M,
+    code = hooks.rbrp_test_gemm_upd(_stream(), M, N, K, _ptr(dA), M, _ptr(dBt), N, _ptr(dC), M,
                                       _ptr(dP))
     assert code == 0
     ref = C0 - A @ B
@@ -168,7 +168,7 @@ def test_gemm_trail_update_matches_numpy(librbrp, M, N, K):
 
 @pytest.mark.parametrize("s,b,deps", [(1, 8, 0), (8, 8, 0), (40, 40, 0), (64, 64, 5), (64, 64, 30),
                                       (100, 128, 10), (128, 128, 0)])
-def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
+def test_panel_pivots_filter_match_dgeqp3(librbrp, hooks, s, b, deps):
     import torch
     rng = np.random.default_rng(s * 7 + deps)
     m = 600
@@ -187,7 +187,7 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
     dRb = torch.zeros(b * b, dtype=torch.float64, device="cuda")
     dRinv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
     bt, ns, status = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    code = librbrp.rbrp_test_panel_inv(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
+    code = hooks.rbrp_test_panel_inv(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
                                        _ptr(dRb), _ptr(dRinv), _ptr(dJin), _ptr(dJout),
                                        ctypes.byref(bt), ctypes.byref(ns), ctypes.byref(status))
     assert code == 0 and status.value == 0
@@ -214,7 +214,7 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, s, b, deps):
 
 @pytest.mark.parametrize("bt,b", [(1, 8), (5, 8), (16, 64), (17, 64), (40, 64), (63, 64),
                                   (64, 64), (100, 128), (128, 128)])
-def test_chol2_factor_inverse_product(librbrp, bt, b):
+def test_chol2_factor_inverse_product(librbrp, hooks, bt, b):
     """Second Cholesky-QR pass: R2 = chol(G2) (upper, positive diagonal), R2inv =
     R2^{-1}, Rout = R2 Rb, against LAPACK on a near-orthonormal block's Gram."""
     import torch
@@ -229,7 +229,7 @@ def test_chol2_factor_inverse_product(librbrp, bt, b):
     dR2inv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
     dRout = torch.full((bt * bt,), np.nan, dtype=torch.float64, device="cuda")
     status = ctypes.c_int32()
-    code = librbrp.rbrp_test_chol2(_stream(), _ptr(dG2), bt, b, _ptr(dRb), _ptr(dR2),
+    code = hooks.rbrp_test_chol2(_stream(), _ptr(dG2), bt, b, _ptr(dRb), _ptr(dR2),
                                    _ptr(dR2inv), _ptr(dRout), ctypes.byref(status))
     assert code == 0 and status.value == 0
     R2_ref = np.linalg.cholesky(G2).T
@@ -245,7 +245,7 @@ def test_chol2_factor_inverse_product(librbrp, bt, b):
 @pytest.mark.parametrize("N,i,b,skew,seed", [(10, 0, 4, 1, 0), (4000, 0, 64, 1, 1),
                                              (4000, 1800, 64, 6, 2), (300, 250, 64, 1, 3),
                                              (5000, 10, 128, 12, 4), (1, 0, 8, 1, 5)])
-def test_sampler_matches_oracle(librbrp, N, i, b, skew, seed):
+def test_sampler_matches_oracle(librbrp, hooks, N, i, b, skew, seed):
     import torch
     rng = np.random.default_rng(seed)
     diags = rng.random(N) ** skew
@@ -261,7 +261,7 @@ def test_sampler_matches_oracle(librbrp, N, i, b, skew, seed):
     dd = torch.as_tensor(diags).cuda()
     dJ = torch.zeros(b, dtype=torch.int32, device="cuda")
     s_out, draws = ctypes.c_int32(), ctypes.c_uint64()
-    code = librbrp.rbrp_test_sample(_stream(), _ptr(dd), i, N, kmax, b, seed, 9, 7, _ptr(dJ),
+    code = hooks.rbrp_test_sample(_stream(), _ptr(dd), i, N, kmax, b, seed, 9, 7, _ptr(dJ),
                                     ctypes.byref(s_out), ctypes.byref(draws))
     assert code == 0
     assert s_out.value == s_ref

@@ -48,14 +48,14 @@ def _stream():
                                           (100, 130, 1024, 7), (29, 64, 4096, 16), (64, 300, 50048, 40),
                                           (128, 300, 4096, 8), (65, 129, 2048, 3), (300, 40, 512, 2),
                                           (128, 1000, 100000, 48)])
-def test_gemm_tn32_within_fp32_bound(librbrp, M, N, K, nsplit):
+def test_gemm_tn32_within_fp32_bound(librbrp, hooks, M, N, K, nsplit):
     import torch
     rng = np.random.default_rng(M * 1000 + N + K)
     A = rng.standard_normal((K, M)).astype(np.float32)
     B = rng.standard_normal((K, N)).astype(np.float32)
     dA, dB = _cm32(A), _cm32(B)
     dC = torch.zeros((N, M), dtype=torch.float64, device="cuda").t()
-    code = librbrp.rbrp_test_gemm_tn32(_stream(), M, N, K, _ptr(dA), _ptr(dB), K, _ptr(dC), M, nsplit)
+    code = hooks.rbrp_test_gemm_tn32(_stream(), M, N, K, _ptr(dA), _ptr(dB), K, _ptr(dC), M, nsplit)
     assert code == 0
     A64, B64 = A.astype(np.float64), B.astype(np.float64)
     ref = A64.T @ B64
@@ -71,7 +71,7 @@ def test_gemm_tn32_within_fp32_bound(librbrp, M, N, K, nsplit):
                                    (70000, 40, 60), (50048, 300, 64), (640, 257, 33), (64, 1, 7),
                                    (1024, 129, 64), (4096, 96, 1), (640, 200, 128), (1024, 129, 97),
                                    (50048, 64, 128), (128, 33, 65)])
-def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
+def test_gemm_trail32_within_fp32_bound(librbrp, hooks, M, N, K):
     """The FP32 update launcher against an FP64 product: shapes with M a multiple
     of 64 take the 128-column kernel with A shared through LDS (K <= 64 and
     K <= 128 forms), the others the 32-column kernel."""
@@ -84,7 +84,7 @@ def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
     dBt = torch.as_tensor(np.ascontiguousarray(B)).cuda()   # row-major (k, c) at k*N + c
     ntile = (M + 63) // 64      # at most one partial row per 64 rows
     dP = torch.zeros(ntile * N, dtype=torch.float64, device="cuda")
-    code = librbrp.rbrp_test_gemm_upd32(_stream(), M, N, K, _ptr(dA), M, _ptr(dBt), N, _ptr(dC), M,
+    code = hooks.rbrp_test_gemm_upd32(_stream(), M, N, K, _ptr(dA), M, _ptr(dBt), N, _ptr(dC), M,
                                         _ptr(dP))
     assert code == 0
     A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
@@ -97,11 +97,11 @@ def test_gemm_trail32_within_fp32_bound(librbrp, M, N, K):
     np.testing.assert_allclose(parts.sum(axis=0), exact, rtol=64 * U32, atol=1e-300)
 
 
-def test_trail32_rejects_odd_heights(librbrp):
+def test_trail32_rejects_odd_heights(librbrp, hooks):
     import torch
     z = torch.zeros(16, dtype=torch.float32, device="cuda")
-    assert librbrp.rbrp_test_gemm_upd32(_stream(), 5, 1, 1, _ptr(z), 6, _ptr(z), 1, _ptr(z), 6, None) == -2
-    assert librbrp.rbrp_test_gemm_tn32(_stream(), 1, 1, 16, _ptr(z), _ptr(z), 16, _ptr(z), 1, 1) == -4
+    assert hooks.rbrp_test_gemm_upd32(_stream(), 5, 1, 1, _ptr(z), 6, _ptr(z), 1, _ptr(z), 6, None) == -2
+    assert hooks.rbrp_test_gemm_tn32(_stream(), 1, 1, 16, _ptr(z), _ptr(z), 16, _ptr(z), 1, 1) == -4
 
 
 def test_fp32_input_fp64_loop_matches_oracle():

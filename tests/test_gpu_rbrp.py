@@ -43,6 +43,7 @@ def assert_parity(A, g, r, name=""):
         kappa = np.linalg.cond(r.R[:, : r.k])
         rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
         bound = max(1e-10, 100 * kappa * EPS)
+        print(f"[{name}] X rel diff {rel:.2e} (bound {bound:.2e}, kappa {kappa:.2e})")
         assert rel <= bound, (name, rel, kappa)
         eg = id_error(A, perm, g.k, X)
         er = id_error(A, r.perm, r.k, r.X)
