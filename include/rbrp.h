@@ -24,6 +24,11 @@ extern "C" {
 
 #define RBRP_ERR_NUMERIC 100000
 #define RBRP_ERR_ALLOC 100001
+/* The problem's workspace does not fit the static scratch of rbrp_colid_d_noalloc. */
+#define RBRP_ERR_SCRATCH 100002
+
+/* Size of the per-device static scratch array used by rbrp_colid_d_noalloc. */
+#define RBRP_NOALLOC_SCRATCH_BYTES (64ull << 20)
 
 typedef struct rbrp_handle_s* rbrp_handle_t;
 
@@ -111,6 +116,26 @@ int rbrp_release_workspace(rbrp_handle_t handle);
  */
 int rbrp_colid_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
                  const rbrp_opts_t* opts, rbrp_result_t* result);
+
+/*
+ * rbrp_colid_d without any device allocation.  No counterpart in the reference
+ * (gpu/rbrp.cu:333-503 allocates per call); it is for callers that cannot
+ * allocate: code running while a HIP graph is being captured, tight serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments, results and status
+ * codes as rbrp_colid_d; the workspace comes from a static device array of
+ * RBRP_NOALLOC_SCRATCH_BYTES per GPU (held by each GPU the library runs on
+ * until the process ends), and a problem whose workspace does not fit it
+ * (roughly m*n + k*n doubles plus small buffers: e.g. 2000 x 1500) is rejected
+ * with RBRP_ERR_SCRATCH before anything is launched.  The scratch address is
+ * looked up by rbrp_create on the handle's device.  Phase timing is not
+ * recorded by this entry point (timing needs events).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other handles,
+ * streams or threads) race and must be serialised by the caller.  When the
+ * arena form's split-K partials fit, the launches and results are identical
+ * to rbrp_colid_d's.
+ */
+int rbrp_colid_d_noalloc(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                         const rbrp_opts_t* opts, rbrp_result_t* result);
 
 /*
  * The same column ID for an FP32-stored input (dA is m x n FP32, leading

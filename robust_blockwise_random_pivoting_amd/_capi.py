@@ -18,6 +18,8 @@ TESTING_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib
 
 RBRP_ERR_NUMERIC = 100000
 RBRP_ERR_ALLOC = 100001
+RBRP_ERR_SCRATCH = 100002
+NOALLOC_SCRATCH_BYTES = 64 << 20
 FILTERS = {"sampled": 0, "nominal": 1}
 PRECISIONS = {"fp64": 0, "fp32": 1}
 
@@ -127,6 +129,8 @@ def lib():
         "rbrp_uniforms_d": (c_int, [vp, u64, u64, u64, i64, vp]),
         "rbrp_colid_d": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
         "rbrp_colid_s": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts), POINTER(RbrpResult)]),
+        "rbrp_colid_d_noalloc": (c_int, [vp, vp, i64, i64, i64, POINTER(RbrpOpts),
+                                         POINTER(RbrpResult)]),
         "rbrp_colid_batched_d": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
                                          POINTER(RbrpProblemResult), c_int]),
         "rbrp_batched_tune": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
@@ -162,6 +166,8 @@ class RBRPError(RuntimeError):
             what = "non-finite input or numerical breakdown"
         elif code == RBRP_ERR_ALLOC:
             what = "device allocation failed"
+        elif code == RBRP_ERR_SCRATCH:
+            what = "the problem's workspace exceeds the static scratch"
         else:
             what = f"HIP error {code}"
         super().__init__(f"{fn} returned {code}: {what}")

@@ -157,6 +157,72 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
   return 0;
 }
 
+// ---- static scratch of the no-allocation entry point -----------------------
+// One array per device, loaded with the library's code object and held until
+// the process ends.  Host code never uses the array's name as an address: the
+// device address of each GPU's copy is looked up once per device (rbrp_create,
+// with that device current) and kept in g_scratch.
+static __device__ __align__(256) unsigned char scratch[RBRP_NOALLOC_SCRATCH_BYTES];
+
+struct ScratchSlot {
+  std::atomic<int> state{0};  // 0 not looked up, 1 looking up, 2 done
+  void* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchSlot g_scratch[RBRP_MAX_DEVICES];
+
+// Looks up the current device's scratch address (once per device).
+hipError_t scratch_setup(int dev) {
+  if (dev < 0 || dev >= RBRP_MAX_DEVICES) return hipErrorInvalidDevice;
+  ScratchSlot& sl = g_scratch[dev];
+  int expect = 0;
+  if (sl.state.compare_exchange_strong(expect, 1, std::memory_order_acq_rel)) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    sl.ptr = e == hipSuccess ? ptr : nullptr;
+    sl.err = e;
+    sl.state.store(2, std::memory_order_release);
+    return e;
+  }
+  while (sl.state.load(std::memory_order_acquire) != 2) {
+  }
+  return sl.err;
+}
+
+// The device's scratch address, or null (with *err set) if the lookup failed
+// or was never made on this device.
+void* scratch_ptr(int dev, hipError_t* err) {
+  if (dev < 0 || dev >= RBRP_MAX_DEVICES) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  ScratchSlot& sl = g_scratch[dev];
+  if (sl.state.load(std::memory_order_acquire) != 2) {
+    *err = hipErrorNotInitialized;
+    return nullptr;
+  }
+  *err = sl.err;
+  return sl.ptr;
+}
+
+// Carves the loop's workspace out of the device's static scratch: with the
+// arena path's split-K partials when they fit (same launch shapes, so the
+// same results as rbrp_colid_d), else with partials for at most 16 splits.
+int static_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed,
+                     RbrpWorkspace* ws) {
+  hipError_t e = hipSuccess;
+  void* base = scratch_ptr(h->device, &e);
+  if (!base) return e != hipSuccess ? (int)e : (int)hipErrorNotInitialized;
+  WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax));
+  if (L.need > RBRP_NOALLOC_SCRATCH_BYTES)
+    L = layout_workspace(mp, n, kmax, b, mixed,
+                         16 * b * std::max<int64_t>(std::max(n, kmax), b));
+  if (L.need > RBRP_NOALLOC_SCRATCH_BYTES) return RBRP_ERR_SCRATCH;
+  *ws = RbrpWorkspace();
+  carve_workspace(*ws, (char*)base, L, mixed);
+  return 0;
+}
+
 // Split-K shape for a TN product of Mmax x Nmax over K rows: aim for about
 // 1024 workgroups, keep the partials within the workspace.
 // Split count and chunk length of a split-K TN launch: as many splits as fill
@@ -191,7 +257,8 @@ struct PhaseTimer {
   };
   std::vector<Rec> recs;
   int next = 0;
-  explicit PhaseTimer(rbrp_handle_t hh) : h(hh) {}
+  bool enabled;  // false: no events at all (the no-allocation entry point)
+  explicit PhaseTimer(rbrp_handle_t hh, bool on = true) : h(hh), enabled(on) {}
   hipEvent_t ev(int idx) {
     while ((int)h->events.size() <= idx) {
       hipEvent_t e;
@@ -201,7 +268,7 @@ struct PhaseTimer {
     return h->events[idx];
   }
   int open(int phase, bool by_kernel) {
-    if (!h->timing) return -1;
+    if (!enabled || !h->timing) return -1;
     if (h->timing == 2 && phase != RBRP_PHASE_PROJECT && phase != RBRP_PHASE_UPDATE) return -1;
     Rec r{phase, next, next + 1, 0, 0.0, 0.0, by_kernel};
     next += 2;
@@ -390,6 +457,9 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
     delete h;
     return (int)e;
   }
+  // the no-allocation entry point's scratch address on this device (a failed
+  // lookup is reported by that entry point, not here)
+  (void)scratch_setup(h->device);
   h->tn64_resident = rbrp_gemm_tn64_resident();
   h->tn32_resident = rbrp_gemm_tn32_resident(0);
   h->tn32w_resident = rbrp_gemm_tn32_resident(1);
@@ -472,7 +542,7 @@ namespace {
 // of Q; the FP64 Q lives in Q64 and the sampled block is re-orthogonalised and
 // factorised in the FP64 buffer Vd.
 int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n, int64_t lda,
-               const rbrp_opts_t* opts, rbrp_result_t* res) {
+               const rbrp_opts_t* opts, rbrp_result_t* res, bool noalloc = false) {
   const double t_start = now_s();
   if (!h) return -1;
   if (m < 0) return -3;
@@ -514,12 +584,14 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const int b = (int)opts->b;
   const int64_t mp = round_up(m, 64);
   const int64_t ldr = std::max<int64_t>(kmax, 1);
+  RbrpWorkspace ws_static;
   {
     // (before the first launch: a workspace that cannot be had leaves every output untouched)
-    const int e = ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
+    const int e = noalloc ? static_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, &ws_static)
+                          : ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
     if (e) return e;
   }
-  RbrpWorkspace& ws = h->ws;
+  RbrpWorkspace& ws = noalloc ? ws_static : h->ws;
   HIPCHK(rbrp_launch_iota(st, res->perm, n));
   int* perm = res->perm;
   const double tol2 = opts->tol * opts->tol;
@@ -527,7 +599,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
   const double ea = a_f32 ? 4.0 : 8.0;      // bytes per element of the input
   const double ew = mixed ? 4.0 : 8.0;      // bytes per element of the working matrix
-  PhaseTimer pt(h);
+  PhaseTimer pt(h, !noalloc);
 
   // R and the two block-state records (laid out right after it) start at zero
   HIPCHK(rbrp_launch_zero(st, ws.R, (int64_t)(((char*)(ws.st + 2) - (char*)ws.R) / sizeof(double))));
@@ -800,6 +872,11 @@ int rbrp_colid_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_
 int rbrp_colid_s(rbrp_handle_t h, const float* dA, int64_t m, int64_t n, int64_t lda,
                  const rbrp_opts_t* opts, rbrp_result_t* res) {
   return colid_impl(h, dA, true, m, n, lda, opts, res);
+}
+
+int rbrp_colid_d_noalloc(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                         const rbrp_opts_t* opts, rbrp_result_t* res) {
+  return colid_impl(h, dA, false, m, n, lda, opts, res, true);
 }
 
 }  // extern "C"

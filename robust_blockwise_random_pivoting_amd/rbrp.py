@@ -121,16 +121,20 @@ class ColID:
 
 def rbrp_colid(A, b: int = 64, tol: float = 0.0, seed: int = 0, offset: int = 0, max_rank: int = 0,
                filter: str = "sampled", precision: str = "fp64", want_x: bool = True,
-               handle: Optional[Handle] = None) -> ColID:
+               handle: Optional[Handle] = None, noalloc: bool = False) -> ColID:
     """RBRP column ID of the device matrix ``A``; see the module docstring.
 
     ``A`` is m x n FP64, or FP32 (kept in FP32 and passed to ``rbrp_colid_s``).
     ``precision`` is "fp64" (the reference's arithmetic) or "fp32" (FP32-stored
     residual with FP32 MFMA projection / update; FP64 sampled-block
-    factorisation, re-orthogonalisation, stopping sum and solve)."""
+    factorisation, re-orthogonalisation, stopping sum and solve).  ``noalloc``
+    calls ``rbrp_colid_d_noalloc`` (FP64 input; no device allocation inside the
+    call, workspace from the library's static scratch)."""
     torch = _torch()
-    A = colmajor(A, keep_fp32=True)
+    A = colmajor(A, keep_fp32=not noalloc)
     fn = "rbrp_colid_s" if A.dtype == torch.float32 else "rbrp_colid_d"
+    if noalloc:
+        fn = "rbrp_colid_d_noalloc"
     m, n = A.shape
     own = handle is None
     h = Handle() if own else handle
