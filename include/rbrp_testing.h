@@ -97,6 +97,13 @@ int rbrp_test_sample(void* stream, const double* diags, int64_t i, int64_t n, in
                      int64_t b, uint64_t seed, uint64_t offset, uint64_t draws_in, int32_t* J,
                      int32_t* s_out, uint64_t* draws_out);
 
+/* Device time of the sampler per call over diags[0:n) (b draws per block,
+ * Philox key (seed, 0)): one untimed call, then `reps` calls between two
+ * events; *ms_per_call receives the mean.  Candidate ranges above 65536 take
+ * the multi-workgroup form. */
+int rbrp_test_sample_time(void* stream, const double* diags, int64_t n, int64_t b, uint64_t seed,
+                          int reps, double* ms_per_call);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,10 +142,24 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // LDS offset (doubles) of 16-byte unit u (rows 2u, 2u+1) of staged column col
 #define TN64_AT(col, u) ((col) * TN64_KS + 2 * ((u) ^ ((col) & 7)))
 
+// A zero word that masked loads read instead of an address past a column's end.
+__device__ double rbrp_zero_word[2];
+
+// Address of row `row` of column `col` (true height `h`), or of the zero word
+// past it: the select is on the address, so the load itself is unconditional.
+__device__ __forceinline__ const double* first_src(const double* col, int64_t row, int64_t h) {
+  return row < h ? col + row : rbrp_zero_word;
+}
+
+// With FIRST, B is the caller's matrix of the FP64 loop's first block (no
+// working copy yet): column c of B is fs column fs.cols[fs.coff + *fs.coff_dyn
+// + c] and its rows >= fs.mtrue read as zero (B itself is unused).
+template <bool FIRST>
 __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, int64_t K,
                                                       int64_t kchunk, MatRef A, MatRef B,
                                                       double* __restrict__ P, int64_t ldp,
-                                                      int64_t pstride, int gx, int gy, int gz) {
+                                                      int64_t pstride, int gx, int gy, int gz,
+                                                      FirstSrc fs) {
   // task = N tile + gx * (M tile + gy * split), XCD-aware (the tiles of one
   // split share its rows of A through one L2)
   const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gy * gz);
@@ -168,7 +182,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
   const int ac = tid >> 2, aq = (tid & 3) * 4;   // A: column ac, rows aq .. aq+3
   const int bc = tid >> 1, bq = (tid & 1) * 8;   // B: column bc, rows bq .. bq+7
   const double* ap = mat_col(A, min(m0 + ac, M - 1)) + aq;
-  const double* bp = mat_col(B, min(n0 + bc, N - 1)) + bq;
+  const double* bp;
+  if (FIRST)
+    bp = fs.p + (int64_t)fs.cols[fs.coff + (fs.coff_dyn ? (int64_t)*fs.coff_dyn : 0) +
+                                 min(n0 + bc, N - 1)] * fs.ld;
+  else
+    bp = mat_col(B, min(n0 + bc, N - 1)) + bq;
   const int au = aq >> 1, bu = bq >> 1;  // first 16-byte unit of the stager
   // row blocks holding rows < M (the kept count is often below the tile height)
   const int mba = (int)min((int64_t)4, (M - m0 + 15) / 16);
@@ -182,14 +201,22 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
   // Two sets of staging registers (named scalars: small arrays here get
   // placed in LDS): the loads of step k+2 are issued before the MFMAs of step
   // k, and step k+1's registers are written to LDS after them.
-#define TN64_LOAD(S, kk)                                  \
-  do {                                                    \
-    S##a0 = *(const double2*)(ap + (kk));                 \
-    S##a1 = *(const double2*)(ap + (kk) + 2);             \
-    S##b0 = *(const double2*)(bp + (kk));                 \
-    S##b1 = *(const double2*)(bp + (kk) + 2);             \
-    S##b2 = *(const double2*)(bp + (kk) + 4);             \
-    S##b3 = *(const double2*)(bp + (kk) + 6);             \
+#define TN64_LOAD(S, kk)                                                              \
+  do {                                                                                \
+    S##a0 = *(const double2*)(ap + (kk));                                             \
+    S##a1 = *(const double2*)(ap + (kk) + 2);                                         \
+    if (FIRST) {                                                                      \
+      const int64_t tl_r = (kk) + bq;                                                 \
+      S##b0 = make_double2(*first_src(bp, tl_r, fs.mtrue), *first_src(bp, tl_r + 1, fs.mtrue)); \
+      S##b1 = make_double2(*first_src(bp, tl_r + 2, fs.mtrue), *first_src(bp, tl_r + 3, fs.mtrue)); \
+      S##b2 = make_double2(*first_src(bp, tl_r + 4, fs.mtrue), *first_src(bp, tl_r + 5, fs.mtrue)); \
+      S##b3 = make_double2(*first_src(bp, tl_r + 6, fs.mtrue), *first_src(bp, tl_r + 7, fs.mtrue)); \
+    } else {                                                                          \
+      S##b0 = *(const double2*)(bp + (kk));                                           \
+      S##b1 = *(const double2*)(bp + (kk) + 2);                                       \
+      S##b2 = *(const double2*)(bp + (kk) + 4);                                       \
+      S##b3 = *(const double2*)(bp + (kk) + 6);                                       \
+    }                                                                                 \
   } while (0)
 #define TN64_STORE(S, bb)                                 \
   do {                                                    \
@@ -274,21 +301,28 @@ int rbrp_gemm_tn64_resident() {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_tn64, 256, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gemm_tn64<false>, 256, 0) != hipSuccess)
     return 0;
   return cus * per;
 }
 
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
-                                 int64_t ldp, int64_t pstride, hipEvent_t e0, hipEvent_t e1) {
+                                 int64_t ldp, int64_t pstride, hipEvent_t e0, hipEvent_t e1,
+                                 const FirstSrc* first) {
   if (Mmax <= 0 || Nmax <= 0 || nsplit <= 0) return hipSuccess;
-  if (kchunk % TN64_KS != 0 || K % TN64_KS != 0 || A.ld % 2 != 0 || B.ld % 2 != 0)
+  if (kchunk % TN64_KS != 0 || K % TN64_KS != 0 || A.ld % 2 != 0 || (!first && B.ld % 2 != 0))
     return hipErrorInvalidValue;
+  if (first && (!first->p || !first->cols)) return hipErrorInvalidValue;
   const int64_t gx = (Nmax + 127) / 128, gy = (Mmax + 63) / 64;
   dim3 grid((unsigned)rbrp_xcd_grid(gx * gy * nsplit));
-  RBRP_LAUNCH_EV(e0, e1, k_gemm_tn64, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp, pstride,
-                     (int)gx, (int)gy, nsplit);
+  const FirstSrc fs = first ? *first : FirstSrc{nullptr, 0, nullptr, 0, nullptr, 0};
+  if (first)
+    RBRP_LAUNCH_EV(e0, e1, k_gemm_tn64<true>, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp,
+                   pstride, (int)gx, (int)gy, nsplit, fs);
+  else
+    RBRP_LAUNCH_EV(e0, e1, k_gemm_tn64<false>, grid, dim3(256), 0, st, M, N, K, kchunk, A, B, P, ldp,
+                   pstride, (int)gx, (int)gy, nsplit, fs);
   return hipGetLastError();
 }
 
@@ -860,7 +894,10 @@ __global__ __launch_bounds__(256) void k_gemm_trail(int64_t M, DimSpec dN, DimSp
 // ---------------------------------------------------------------------------
 #define UPD_LDA 66
 
-template <bool NORMS>
+// With FIRST the old C is read from the caller's matrix (fs: the permuted
+// column list, rows >= fs.mtrue as zero) and the new C is written to C: the
+// FP64 loop's first update, which fills the working copy.
+template <bool NORMS, bool FIRST>
 __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, DimSpec dK, int T,
                                                        int ncol, int nrg,
                                                        const double* __restrict__ A0, int64_t lda,
@@ -868,7 +905,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
                                                        const double* __restrict__ Bt, int64_t ldbt,
                                                        double* __restrict__ C0, int64_t ldc,
                                                        int64_t ccoff, const int* ccoff_dyn,
-                                                       double* __restrict__ normpart, int64_t ldnp) {
+                                                       double* __restrict__ normpart, int64_t ldnp,
+                                                       FirstSrc fs) {
   __shared__ double As[2][64 * UPD_LDA];
   const int64_t N = dim_eval(dN), K = dim_eval(dK);
   // task = column tile + ncol * row group, XCD-aware (a row group's column
@@ -888,6 +926,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
   const int ncw = (int)max((int64_t)0, min((int64_t)32, N - cw));
   double* C = C0 + (ccoff + (ccoff_dyn ? *ccoff_dyn : 0) + (ncw > 0 ? cw : c0)) * ldc;
   const int cmax = max(ncw, 1) - 1;
+  // FIRST: the source column of each of the lane's eight C columns
+  const double* cin[2][4];
+  if (FIRST) {
+    const int* fcols = fs.cols + fs.coff + (fs.coff_dyn ? (int64_t)*fs.coff_dyn : 0) + (ncw > 0 ? cw : c0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) cin[h][v] = fs.p + (int64_t)fcols[min(16 * h + 4 * v + lk, cmax)] * fs.ld;
+  }
   // B block of this wave: b0[q] = B(16 lk + q, cw + li), b1[q] = B(16 lk + q, cw + 16 + li)
   double b0[16], b1[16];
 #pragma unroll
@@ -924,11 +971,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
     UPD_PUT(d_, 3, s3);                                  \
   } while (0)
   // C of one 16-row sub-block: cv[h][v] = C(r + li, 16 h + 4 v + lk)
-#define UPD_LOADC(cv, r)                                                        \
-  do {                                                                          \
-    _Pragma("unroll") for (int h = 0; h < 2; ++h)                               \
-      _Pragma("unroll") for (int v = 0; v < 4; ++v)                             \
-        cv[h][v] = C[(r) + li + (int64_t)min(16 * h + 4 * v + lk, cmax) * ldc]; \
+#define UPD_LOADC(cv, r)                                                          \
+  do {                                                                            \
+    _Pragma("unroll") for (int h = 0; h < 2; ++h)                                 \
+      _Pragma("unroll") for (int v = 0; v < 4; ++v) {                             \
+        if (FIRST)                                                                \
+          cv[h][v] = *first_src(cin[h][v], (r) + li, fs.mtrue);                   \
+        else                                                                      \
+          cv[h][v] = C[(r) + li + (int64_t)min(16 * h + 4 * v + lk, cmax) * ldc]; \
+      }                                                                           \
   } while (0)
 
   double nsq[2][4];
@@ -1007,16 +1058,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_upd64(int64_t M, DimSpec dN, Di
   }
 }
 
+bool rbrp_gemm_trail_first_ok(int64_t M, int64_t Kmax, int64_t lda) {
+  return Kmax <= 64 && M % 64 == 0 && lda % 2 == 0;
+}
+
 hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t Nmax, DimSpec K,
                                   int64_t Kmax, const double* A, int64_t lda, int64_t acoff,
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
                                   double* normpart, int64_t ldnp, int64_t* groups, hipEvent_t e0,
-                                  hipEvent_t e1) {
+                                  hipEvent_t e1, const FirstSrc* first) {
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128) return hipErrorInvalidValue;
+  if (first && (!rbrp_gemm_trail_first_ok(M, Kmax, lda) || !first->p || !first->cols))
+    return hipErrorInvalidValue;
   if (Kmax <= 64 && M % 64 == 0 && lda % 2 == 0) {
+    const FirstSrc fs = first ? *first : FirstSrc{nullptr, 0, nullptr, 0, nullptr, 0};
     // 128-column workgroups with the A block shared through LDS
     const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;
     const int64_t T = std::max<int64_t>(
@@ -1024,14 +1082,22 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
     const int64_t nrg = (nrow + T - 1) / T;
     dim3 grid((unsigned)rbrp_xcd_grid(ncol * nrg));
     if (groups) *groups = nrg;
-    if (normpart)
-      RBRP_LAUNCH_EV(e0, e1, k_gemm_upd64<true>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
-                         (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
-                         normpart, ldnp);
+    if (first && normpart)
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_upd64<true, true>), grid, dim3(256), 0, st, M, N, K, (int)T,
+                     (int)ncol, (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff,
+                     ccoff_dyn, normpart, ldnp, fs);
+    else if (first)
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_upd64<false, true>), grid, dim3(256), 0, st, M, N, K, (int)T,
+                     (int)ncol, (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff,
+                     ccoff_dyn, normpart, ldnp, fs);
+    else if (normpart)
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_upd64<true, false>), grid, dim3(256), 0, st, M, N, K, (int)T,
+                     (int)ncol, (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff,
+                     ccoff_dyn, normpart, ldnp, fs);
     else
-      RBRP_LAUNCH_EV(e0, e1, k_gemm_upd64<false>, grid, dim3(256), 0, st, M, N, K, (int)T, (int)ncol,
-                         (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff, ccoff_dyn,
-                         normpart, ldnp);
+      RBRP_LAUNCH_EV(e0, e1, (k_gemm_upd64<false, false>), grid, dim3(256), 0, st, M, N, K, (int)T,
+                     (int)ncol, (int)nrg, A, lda, acoff, acoff_dyn, Bt, ldbt, C, ldc, ccoff,
+                     ccoff_dyn, normpart, ldnp, fs);
     return hipGetLastError();
   }
   const bool small = Kmax <= 64;

@@ -3,6 +3,7 @@
 // sampled block, CholQR2 re-factorisation, the symmetric-difference
 // permutation, residual sums and the diagonal-block solve of X = R11^{-1} R12.
 #include <algorithm>
+#include <climits>
 
 #include "rbrp_kernels.h"
 
@@ -42,6 +43,98 @@ hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda
                                     int64_t mp, int64_t n, double* Aw, double* diags) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_copy_colnorm, dim3((unsigned)n), dim3(256), 0, st, A, lda, m, mp, Aw, diags);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Norm pass without a working copy: diags[j] = ||A(:, j)||^2, a pure read of
+// A (the FP64 loop's first block reads A itself; see k_gemm_tn64 / the update
+// kernels' first-block forms).  One workgroup per column, each lane streaming
+// 16-byte loads (four in flight) when the column starts on a 16-byte boundary
+// (A aligned and lda even), else single doubles.  The per-lane sums are added
+// in a fixed order, then the workgroup sum.
+// Reference: column_norm_squared (gpu/rbrp.cu:82-99, called at :354).
+// ---------------------------------------------------------------------------
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_colnorm(const double* __restrict__ A, int64_t lda,
+                                                 int64_t m, double* __restrict__ diags) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const double* a = A + j * lda;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+  int64_t r;
+  if (VEC) {
+    // nontemporal loads: A is streamed once here (measured 7.0 vs 6.1 TB/s
+    // for the same loop with ordinary loads on a 1.6 GB matrix)
+    const v2d* a2 = (const v2d*)a;
+    const int64_t m2 = m >> 1;  // whole pairs
+    int64_t q = threadIdx.x;
+    for (; q + 768 < m2; q += 1024) {
+      const v2d v0 = __builtin_nontemporal_load(a2 + q), v1 = __builtin_nontemporal_load(a2 + q + 256),
+                v2 = __builtin_nontemporal_load(a2 + q + 512), v3 = __builtin_nontemporal_load(a2 + q + 768);
+      acc0 = fma(v0.x, v0.x, acc0);
+      acc1 = fma(v0.y, v0.y, acc1);
+      acc2 = fma(v1.x, v1.x, acc2);
+      acc3 = fma(v1.y, v1.y, acc3);
+      acc0 = fma(v2.x, v2.x, acc0);
+      acc1 = fma(v2.y, v2.y, acc1);
+      acc2 = fma(v3.x, v3.x, acc2);
+      acc3 = fma(v3.y, v3.y, acc3);
+    }
+    for (; q < m2; q += 256) {
+      const v2d v0 = __builtin_nontemporal_load(a2 + q);
+      acc0 = fma(v0.x, v0.x, acc0);
+      acc1 = fma(v0.y, v0.y, acc1);
+    }
+    r = 2 * m2 + threadIdx.x;  // the odd last row
+  } else {
+    r = threadIdx.x;
+    for (; r + 768 < m; r += 1024) {
+      const double v0 = a[r], v1 = a[r + 256], v2 = a[r + 512], v3 = a[r + 768];
+      acc0 = fma(v0, v0, acc0);
+      acc1 = fma(v1, v1, acc1);
+      acc2 = fma(v2, v2, acc2);
+      acc3 = fma(v3, v3, acc3);
+    }
+  }
+  for (; r < m; r += 256) acc0 = fma(a[r], a[r], acc0);
+  const double t = rbrp_block_sum((acc0 + acc1) + (acc2 + acc3), red);
+  if (threadIdx.x == 0) diags[j] = t;
+}
+
+hipError_t rbrp_launch_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m, int64_t n,
+                               double* diags) {
+  if (n <= 0) return hipSuccess;
+  const bool vec = ((uintptr_t)A % 16 == 0) && (lda % 2 == 0);
+  if (vec)
+    hipLaunchKernelGGL(k_colnorm<true>, dim3((unsigned)n), dim3(256), 0, st, A, lda, m, diags);
+  else
+    hipLaunchKernelGGL(k_colnorm<false>, dim3((unsigned)n), dim3(256), 0, st, A, lda, m, diags);
+  return hipGetLastError();
+}
+
+// V(:, p) = [A(0:m, J[p]); 0 (rows m..mp)] for p < bs->s: the sampled columns
+// of the FP64 loop's first block, read from A (no working copy exists yet).
+__global__ __launch_bounds__(256) void k_gather_first(const double* __restrict__ A, int64_t lda,
+                                                      int64_t m, int64_t mp,
+                                                      const RbrpBlockState* __restrict__ bs,
+                                                      const int* __restrict__ J,
+                                                      double* __restrict__ V, int64_t ldv) {
+  const int p = blockIdx.y;
+  if (p >= bs->s) return;
+  const double* a = A + (int64_t)J[p] * lda;
+  double* v = V + (int64_t)p * ldv;
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < mp; r += (int64_t)gridDim.x * 256)
+    v[r] = r < m ? a[r] : 0.0;
+}
+
+hipError_t rbrp_launch_gather_first(hipStream_t st, const double* A, int64_t lda, int64_t m,
+                                    int64_t mp, int b, const RbrpBlockState* bs, const int* J,
+                                    double* V, int64_t ldv) {
+  if (b <= 0 || mp <= 0) return hipSuccess;
+  const int64_t bx = std::min<int64_t>((mp + 255) / 256, 64);
+  hipLaunchKernelGGL(k_gather_first, dim3((unsigned)bx, (unsigned)b), dim3(256), 0, st, A, lda, m, mp,
+                     bs, J, V, ldv);
   return hipGetLastError();
 }
 
@@ -264,11 +357,397 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide sampler: the same rounds algorithm for a candidate range N larger than
+// SAMPLE_WIDE_N, over every compute unit.  The weights are cut into tiles of
+// WIDE_TILE (thread k of a 1024-thread workgroup owns the tile's elements 4k
+// .. 4k+3).  The CDF value of element j of tile t is defined as
+//   cdf[j] = P[t] + run(j),
+// run(j) being the inclusive running sum inside the tile in one fixed order
+// (thread k starts from the exclusive prefix of the 1024 thread sums and adds
+// its four weights in turn) and P[t] the tile prefixes, with the total =
+// P[last] + S[last] = cdf[N - 1] (S[t] = run of the tile's last element).
+// Every kernel that needs run() computes it with the same device function, so
+// the tile sums, the tile search and the in-tile search agree exactly and the
+// CDF is non-decreasing.  The CDF is never stored: one pass reads the weights
+// (8 bytes per candidate) and writes S and the non-zero counts per tile; one
+// workgroup forms P and the total, draws the round's uniforms and picks each
+// draw's tile; one workgroup per draw re-scans its tile to find the index;
+// one workgroup keeps the first occurrences and runs any further round itself
+// (the tiles holding found indices re-summed with those weights zeroed).
+// ---------------------------------------------------------------------------
+#define WIDE_TILE 4096
+#define SAMPLE_WIDE_N 65536
+
+struct WideHdr {
+  int64_t i, N, ntiles;
+  int s, r, nnz, done;
+  double total;
+  unsigned long long draws;
+};
+
+__device__ inline bool wide_found(const int* fs, int nf, int64_t j) {
+  int lo = 0, hi = nf;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)fs[mid] < j)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < nf && (int64_t)fs[lo] == j;
+}
+
+// This thread's four inclusive running sums of tile t (clipped weights, the
+// fs[0:nf] sorted found positions zeroed) and its count of positive weights.
+// tl: LDS staging of WIDE_TILE doubles (coalesced loads); wsum: 16 doubles.
+__device__ inline void wide_tile(const double* __restrict__ diags, int64_t i, int64_t N, int64_t t,
+                                 const int* fs, int nf, double* tl, double* wsum, double run[4],
+                                 int* cnt) {
+  const int tid = threadIdx.x;
+  const int64_t base = t * WIDE_TILE;
+  __syncthreads();  // tl may still be read by a previous call
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t j = base + tid + 1024 * q;
+    double v = 0.0;
+    if (j < N) {
+      v = diags[i + j];
+      v = (v > 0.0) ? v : 0.0;
+      if (nf > 0 && v > 0.0 && wide_found(fs, nf, j)) v = 0.0;
+    }
+    tl[tid + 1024 * q] = v;
+  }
+  __syncthreads();
+  const double2 p0 = *(const double2*)&tl[4 * tid];
+  const double2 p1 = *(const double2*)&tl[4 * tid + 2];
+  const double w[4] = {p0.x, p0.y, p1.x, p1.y};
+  double loc = 0.0;
+  int c = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    loc += w[q];
+    c += (w[q] > 0.0) ? 1 : 0;
+  }
+  double r = block_exclusive_scan(loc, wsum);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    r += w[q];
+    run[q] = r;
+  }
+  *cnt = c;
+}
+
+// Scratch of the wide sampler, carved from the sampler's w / cdf arrays.
+struct WideScratch {
+  double *S, *P, *u;
+  int *tnnz, *tilesel, *newidx;
+  WideHdr* hdr;
+};
+
+__host__ __device__ inline WideScratch wide_scratch(double* gw, double* gcdf, int64_t ntiles_max) {
+  WideScratch w;
+  w.S = gw;
+  w.P = gw + ntiles_max;
+  w.hdr = (WideHdr*)gcdf;
+  w.u = gcdf + 8;
+  w.tilesel = (int*)(gcdf + 8 + SAMPLE_MAXB);
+  w.newidx = w.tilesel + SAMPLE_MAXB;
+  w.tnnz = w.newidx + SAMPLE_MAXB;
+  return w;
+}
+
+__device__ inline int64_t wide_first_col(int64_t i0, const int* i_dyn) {
+  return i0 + (i_dyn ? (int64_t)*i_dyn : 0);
+}
+
+// Tile pass: S[t] and the positive-weight count of every tile.
+__global__ __launch_bounds__(1024) void k_wide_tiles(const double* __restrict__ diags, int64_t i0,
+                                                     const int* i_dyn, int64_t n, WideScratch ws) {
+  __shared__ __align__(16) double tl[WIDE_TILE];
+  __shared__ double wsum[16];
+  __shared__ int ired[16];
+  const int64_t i = wide_first_col(i0, i_dyn);
+  const int64_t N = n - i;
+  const int64_t t = blockIdx.x;
+  if (t * WIDE_TILE >= N) return;  // uniform
+  double run[4];
+  int c;
+  wide_tile(diags, i, N, t, nullptr, 0, tl, wsum, run, &c);
+  const int cnt = rbrp_block_sum_int(c, ired);
+  if (threadIdx.x == 1023) ws.S[t] = run[3];
+  if (threadIdx.x == 0) ws.tnnz[t] = cnt;
+}
+
+// P[t] (exclusive tile prefixes) and the total from S: thread k owns a run of
+// consecutive tiles; returns the total to every thread.
+__device__ inline double wide_prefix(const WideScratch& ws, int64_t ntiles, double* wsum,
+                                     double* bcast) {
+  const int tid = threadIdx.x;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t t0 = min(ntiles, (int64_t)tid * per), t1 = min(ntiles, t0 + per);
+  double loc = 0.0;
+  for (int64_t t = t0; t < t1; ++t) loc += ws.S[t];
+  double run = block_exclusive_scan(loc, wsum);
+  for (int64_t t = t0; t < t1; ++t) {
+    ws.P[t] = run;
+    run += ws.S[t];
+  }
+  if (t0 < t1 && t1 == ntiles) *bcast = run;  // P[last] + S[last]
+  __threadfence();
+  __syncthreads();
+  return *bcast;
+}
+
+// First tile t with (P[t] + S[t]) / total > u.
+__device__ inline int64_t wide_pick_tile(const WideScratch& ws, int64_t ntiles, double total,
+                                         double u) {
+  int64_t lo = 0, hi = ntiles;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((ws.P[mid] + ws.S[mid]) / total > u)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo < ntiles ? lo : ntiles - 1;
+}
+
+// First position j of tile t with (P[t] + run(j)) / total > u (all threads
+// call; every thread returns it).
+__device__ inline int64_t wide_find(const double* __restrict__ diags, int64_t i, int64_t N,
+                                    const WideScratch& ws, int64_t t, double total, double u,
+                                    const int* fs, int nf, double* tl, double* wsum, int* imin) {
+  double run[4];
+  int c;
+  wide_tile(diags, i, N, t, fs, nf, tl, wsum, run, &c);
+  const double Pt = ws.P[t];
+  int best = INT_MAX;
+#pragma unroll
+  for (int q = 3; q >= 0; --q) {
+    const int64_t j = t * WIDE_TILE + 4 * (int64_t)threadIdx.x + q;
+    if (j < N && (Pt + run[q]) / total > u) best = (int)(j - t * WIDE_TILE);
+  }
+  // workgroup minimum
+  for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o, RBRP_WAVE));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) imin[threadIdx.x >> 6] = best;
+  __syncthreads();
+  int m = INT_MAX;
+  for (int q = 0; q < 16; ++q) m = min(m, imin[q]);
+  if (m == INT_MAX) return min(N, (t + 1) * WIDE_TILE) - 1;
+  return t * WIDE_TILE + m;
+}
+
+// Round 1: s, the tile prefixes, the total, the round's uniforms and their tiles.
+__global__ __launch_bounds__(1024) void k_wide_pick(int64_t i0, const int* i_dyn, int check_prev,
+                                                    double tol2, int64_t n, int64_t kmax, int b,
+                                                    uint64_t key0, uint64_t key1, uint64_t c0,
+                                                    uint64_t c1, uint64_t c2, uint64_t c3,
+                                                    WideScratch ws, RbrpBlockState* bs) {
+  __shared__ double wsum[16];
+  __shared__ int ired[16];
+  __shared__ double bcast;
+  __shared__ int s_done;
+  const int tid = threadIdx.x;
+  const int64_t i = wide_first_col(i0, i_dyn);
+  if (tid == 0) {
+    s_done = check_prev && (bs->status != 0 || bs->s <= 0 || bs->bt <= 0 || bs->err <= tol2 || i >= kmax);
+    if (s_done) {
+      bs->s = 0;
+      ws.hdr->done = 1;
+    }
+  }
+  __syncthreads();
+  if (s_done) return;
+  const int64_t N = n - i;
+  const int64_t ntiles = N > 0 ? (N + WIDE_TILE - 1) / WIDE_TILE : 0;
+  int c = 0;
+  for (int64_t t = tid; t < ntiles; t += 1024) c += ws.tnnz[t];
+  const int nnz = rbrp_block_sum_int(c, ired);
+  int64_t s64 = (int64_t)b;
+  if (kmax - i < s64) s64 = kmax - i;
+  if ((int64_t)nnz < s64) s64 = nnz;
+  const int s = (int)(s64 > 0 ? s64 : 0);
+  if (s <= 0) {
+    if (tid == 0) {
+      bs->s = 0;
+      bs->nnz = nnz;
+      ws.hdr->done = 1;
+    }
+    return;
+  }
+  const double total = wide_prefix(ws, ntiles, wsum, &bcast);
+  const unsigned long long draws = bs->draws;
+  const uint64_t base[4] = {c0, c1, c2, c3};
+  if (tid < s) {
+    const double u = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draws + tid));
+    ws.u[tid] = u;
+    ws.tilesel[tid] = (int)wide_pick_tile(ws, ntiles, total, u);
+  }
+  if (tid == 0) {
+    WideHdr* h = ws.hdr;
+    h->i = i;
+    h->N = N;
+    h->ntiles = ntiles;
+    h->s = s;
+    h->r = s;
+    h->nnz = nnz;
+    h->done = 0;
+    h->total = total;
+    h->draws = draws;
+  }
+}
+
+// Round 1's in-tile searches: workgroup d resolves draw d.
+__global__ __launch_bounds__(1024) void k_wide_resolve(const double* __restrict__ diags,
+                                                       WideScratch ws) {
+  __shared__ __align__(16) double tl[WIDE_TILE];
+  __shared__ double wsum[16];
+  __shared__ int imin[16];
+  const WideHdr* h = ws.hdr;
+  const int d = blockIdx.x;
+  if (h->done || d >= h->r) return;  // uniform
+  const int64_t j = wide_find(diags, h->i, h->N, ws, ws.tilesel[d], h->total, ws.u[d], nullptr, 0,
+                              tl, wsum, imin);
+  if (threadIdx.x == 0) ws.newidx[d] = (int)j;
+}
+
+// First occurrences of the round's picks appended to found[0:*nf] (thread 0 of
+// the workgroup updates *nf; call with all threads).
+__device__ inline void wide_collect(const int* newidx, int r, int* found, int* nf, int* isnew) {
+  const int tid = threadIdx.x;
+  const int nf0 = *nf;
+  if (tid < r) {
+    const int v = newidx[tid];
+    int dup = 0;
+    for (int q = 0; q < nf0; ++q) dup |= (found[q] == v) ? 1 : 0;
+    for (int q = 0; q < tid; ++q) dup |= (newidx[q] == v) ? 1 : 0;
+    isnew[tid] = !dup;
+  }
+  __syncthreads();
+  if (tid < r && isnew[tid]) {
+    int pos = nf0;
+    for (int q = 0; q < tid; ++q) pos += isnew[q];
+    found[pos] = newidx[tid];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int add = 0;
+    for (int q = 0; q < r; ++q) add += isnew[q];
+    *nf = nf0 + add;
+  }
+  __syncthreads();
+}
+
+// Keeps round 1's first occurrences, runs the later rounds, writes J and the state.
+__global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__ diags, int64_t kmax,
+                                                      uint64_t key0, uint64_t key1, uint64_t c0,
+                                                      uint64_t c1, uint64_t c2, uint64_t c3,
+                                                      WideScratch ws, RbrpBlockState* bs,
+                                                      int* __restrict__ J) {
+  __shared__ __align__(16) double tl[WIDE_TILE];
+  __shared__ double wsum[16];
+  __shared__ int imin[16];
+  __shared__ double bcast;
+  __shared__ int found[SAMPLE_MAXB], fs[SAMPLE_MAXB], newidx[SAMPLE_MAXB], isnew[SAMPLE_MAXB];
+  __shared__ int tilesel[SAMPLE_MAXB];
+  __shared__ double uu[SAMPLE_MAXB];
+  __shared__ int nf, ntl;
+  const WideHdr* h = ws.hdr;
+  if (h->done) return;  // uniform (the state already says s = 0)
+  const int tid = threadIdx.x;
+  const int64_t i = h->i, N = h->N, ntiles = h->ntiles;
+  const int s = h->s;
+  double total = h->total;
+  unsigned long long draws = h->draws;
+  if (tid == 0) nf = 0;
+  if (tid < h->r) newidx[tid] = ws.newidx[tid];
+  __syncthreads();
+  wide_collect(newidx, h->r, found, &nf, isnew);
+  draws += (unsigned long long)h->r;
+  const uint64_t base[4] = {c0, c1, c2, c3};
+  // every round adds at least one new index, so s rounds suffice
+  for (int round = 1; round <= s && nf < s; ++round) {
+    const int n_f = nf;
+    // sorted copy of the found positions, and the tiles that hold them
+    if (tid == 0) {
+      for (int q = 0; q < n_f; ++q) {
+        int v = found[q], p = q;
+        while (p > 0 && fs[p - 1] > v) {
+          fs[p] = fs[p - 1];
+          --p;
+        }
+        fs[p] = v;
+      }
+      int k = 0;
+      for (int q = 0; q < n_f; ++q) {
+        const int tt = fs[q] / WIDE_TILE;
+        if (k == 0 || tilesel[k - 1] != tt) tilesel[k++] = tt;
+      }
+      ntl = k;
+    }
+    __syncthreads();
+    const int nt = ntl;
+    for (int q = 0; q < nt; ++q) {
+      const int64_t tt = tilesel[q];
+      double run[4];
+      int c;
+      wide_tile(diags, i, N, tt, fs, n_f, tl, wsum, run, &c);
+      if (tid == 1023) ws.S[tt] = run[3];
+    }
+    __threadfence();
+    __syncthreads();
+    total = wide_prefix(ws, ntiles, wsum, &bcast);
+    const int r = s - n_f;
+    if (tid < r) {
+      const double u = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draws + tid));
+      uu[tid] = u;
+      newidx[tid] = (int)wide_pick_tile(ws, ntiles, total, u);  // tile, resolved below
+    }
+    __syncthreads();
+    for (int d = 0; d < r; ++d) {
+      const int64_t j = wide_find(diags, i, N, ws, newidx[d], total, uu[d], fs, n_f, tl, wsum, imin);
+      __syncthreads();
+      if (tid == 0) newidx[d] = (int)j;
+    }
+    __syncthreads();
+    wide_collect(newidx, r, found, &nf, isnew);
+    draws += (unsigned long long)r;
+  }
+  if (nf < s) {
+    if (tid == 0) {
+      bs->s = 0;
+      bs->status = 4;
+    }
+    return;
+  }
+  for (int t = tid; t < s; t += 1024) J[t] = (int)(i + found[t]);
+  if (tid == 0) {
+    bs->s = s;
+    bs->draws = draws;
+    bs->nnz = h->nnz;
+  }
+}
+
 hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
                               int64_t kmax, int b, uint64_t key0, uint64_t key1,
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
                               int* J, const int* i_dyn, double tol2) {
   if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
+  if (n - i - (i_dyn ? 1 : 0) > SAMPLE_WIDE_N) {
+    // (with i_dyn the range is at most n - i - 1: the previous block kept a column)
+    const int64_t ntiles_max = (n - i + WIDE_TILE - 1) / WIDE_TILE;
+    const WideScratch ws = wide_scratch(w, cdf, ntiles_max);
+    hipLaunchKernelGGL(k_wide_tiles, dim3((unsigned)ntiles_max), dim3(1024), 0, st, diags, i, i_dyn, n,
+                       ws);
+    hipLaunchKernelGGL(k_wide_pick, dim3(1), dim3(1024), 0, st, i, i_dyn, i_dyn ? 1 : 0, tol2, n, kmax,
+                       b, key0, key1, ctr[0], ctr[1], ctr[2], ctr[3], ws, bs);
+    hipLaunchKernelGGL(k_wide_resolve, dim3((unsigned)b), dim3(1024), 0, st, diags, ws);
+    hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(1024), 0, st, diags, kmax, key0, key1, ctr[0],
+                       ctr[1], ctr[2], ctr[3], ws, bs, J);
+    return hipGetLastError();
+  }
   // with i_dyn the first column is at least i + 1 (the previous block kept a
   // column, or the call does nothing)
   const bool in_lds = (n - i - (i_dyn ? 1 : 0)) <= SAMPLE_LDS_N;

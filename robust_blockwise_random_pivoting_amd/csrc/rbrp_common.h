@@ -7,6 +7,7 @@
 #define RBRP_WAVE 64
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 // ---------------------------------------------------------------------------
 // Philox4x64-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy

@@ -64,13 +64,16 @@ int64_t full_pcap(int64_t n, int64_t kmax) {
   return 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
 }
 
-WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed, int64_t Pcap) {
+// `vd`: the FP64 block buffer Vd (the mixed loop's sampled block, the FP64
+// loop's first sampled block when that block reads A directly).
+WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed, int64_t Pcap,
+                          bool vd) {
   const int64_t ldr = std::max<int64_t>(kmax, 1);
   WsLayout L;
   Carver c;
   L.oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
   L.oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
-  L.oVd = mixed ? c.take<double>(mp * b) : 0;
+  L.oVd = (mixed || vd) ? c.take<double>(mp * b) : 0;
   L.oR = c.take<double>(ldr * n);
   L.ost = c.take<RbrpBlockState>(2);  // the block state and its published snapshot (after R:
                                       // one fill kernel clears both)
@@ -102,11 +105,11 @@ WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool m
 }
 
 // Points the workspace's buffers into the arena at `base`.
-void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixed) {
+void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixed, bool vd) {
   ws.Aw = mixed ? nullptr : (double*)(base + L.oAw);
   ws.Aw32 = mixed ? (float*)(base + L.oAw) : nullptr;
   ws.Q64 = mixed ? (double*)(base + L.oQ64) : nullptr;
-  ws.Vd = mixed ? (double*)(base + L.oVd) : nullptr;
+  ws.Vd = (mixed || vd) ? (double*)(base + L.oVd) : nullptr;
   ws.R = (double*)(base + L.oR);
   ws.diags = (double*)(base + L.odiags);
   ws.w = (double*)(base + L.ow);
@@ -137,8 +140,9 @@ void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixe
 }
 
 // The handle's cached arena, grown on demand.
-int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed) {
-  const WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax));
+int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed,
+                     bool vd) {
+  const WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax), vd);
   RbrpWorkspace& ws = h->ws;
   if (ws.arena_bytes < L.need) {
     if (ws.arena) {
@@ -153,7 +157,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
     }
     ws.arena_bytes = L.need;
   }
-  carve_workspace(ws, (char*)ws.arena, L, mixed);
+  carve_workspace(ws, (char*)ws.arena, L, mixed, vd);
   return 0;
 }
 
@@ -209,17 +213,16 @@ void* scratch_ptr(int dev, hipError_t* err) {
 // arena path's split-K partials when they fit (same launch shapes, so the
 // same results as rbrp_colid_d), else with partials for at most 16 splits.
 int static_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed,
-                     RbrpWorkspace* ws) {
+                     bool vd, RbrpWorkspace* ws) {
   hipError_t e = hipSuccess;
   void* base = scratch_ptr(h->device, &e);
   if (!base) return e != hipSuccess ? (int)e : (int)hipErrorNotInitialized;
-  WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax));
+  WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax), vd);
   if (L.need > RBRP_NOALLOC_SCRATCH_BYTES)
-    L = layout_workspace(mp, n, kmax, b, mixed,
-                         16 * b * std::max<int64_t>(std::max(n, kmax), b));
+    L = layout_workspace(mp, n, kmax, b, mixed, 16 * b * std::max<int64_t>(std::max(n, kmax), b), vd);
   if (L.need > RBRP_NOALLOC_SCRATCH_BYTES) return RBRP_ERR_SCRATCH;
   *ws = RbrpWorkspace();
-  carve_workspace(*ws, (char*)base, L, mixed);
+  carve_workspace(*ws, (char*)base, L, mixed, vd);
   return 0;
 }
 
@@ -584,11 +587,18 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const int b = (int)opts->b;
   const int64_t mp = round_up(m, 64);
   const int64_t ldr = std::max<int64_t>(kmax, 1);
+  // The FP64 loop on FP64 input reads A itself in its first block (no working
+  // copy is made up front: the norm pass only reads A, the first block's
+  // sampled columns are gathered from A, and its projection and update read
+  // the trailing columns from A through the permutation, the update writing
+  // the working copy).  This needs the 128-column update kernel (b <= 64).
+  const bool direct = !mixed && !a_f32 && rbrp_gemm_trail_first_ok(mp, b, mp);
   RbrpWorkspace ws_static;
   {
     // (before the first launch: a workspace that cannot be had leaves every output untouched)
-    const int e = noalloc ? static_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, &ws_static)
-                          : ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed);
+    const int e = noalloc ? static_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, direct,
+                                             &ws_static)
+                          : ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, direct);
     if (e) return e;
   }
   RbrpWorkspace& ws = noalloc ? ws_static : h->ws;
@@ -604,13 +614,15 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   // R and the two block-state records (laid out right after it) start at zero
   HIPCHK(rbrp_launch_zero(st, ws.R, (int64_t)(((char*)(ws.st + 2) - (char*)ws.R) / sizeof(double))));
   int ph = pt.begin(RBRP_PHASE_NORMS);
-  if (mixed)
+  if (direct)
+    HIPCHK(rbrp_launch_colnorm(st, (const double*)dA, lda, m, n, ws.diags));
+  else if (mixed)
     HIPCHK(rbrp_launch_copy_colnorm32(st, dA, a_f32 ? 1 : 0, lda, m, mp, n, ws.Aw32, ws.diags));
   else if (a_f32)
     HIPCHK(rbrp_launch_copy_colnorm_from32(st, (const float*)dA, lda, m, mp, n, ws.Aw, ws.diags));
   else
     HIPCHK(rbrp_launch_copy_colnorm(st, (const double*)dA, lda, m, mp, n, ws.Aw, ws.diags));
-  pt.end(ph, 1, 2.0 * m * n, ea * m * n + ew * mp * n + 8.0 * n);
+  pt.end(ph, 1, 2.0 * m * n, ea * m * n + (direct ? 0.0 : ew * mp * n) + 8.0 * n);
   HIPCHK(rbrp_launch_sum_to_state(st, ws.diags, n, ws.st, 0));
   HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -621,13 +633,22 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const double N0 = h->host_state->N0;
   if (!std::isfinite(N0)) return RBRP_ERR_NUMERIC;
 
-  // FP64 operands of the sampled block and of the skeleton basis
-  const MatRef Vblk = mixed ? mat(ws.Vd, mp) : mat_cols(ws.Aw, mp, ws.J);
-  const MatRef Vkept = mixed ? mat_cols(ws.Vd, mp, ws.Jkpos) : mat_cols(ws.Aw, mp, ws.Jk);
+  // FP64 operands of the sampled block (gathered into Vd in the mixed loop
+  // and in the direct first block, else read in place) and of the skeleton basis
+  auto in_vd = [&](int64_t ib) { return mixed || (direct && ib == 0); };
+  auto vblk = [&](int64_t ib) { return in_vd(ib) ? mat(ws.Vd, mp) : mat_cols(ws.Aw, mp, ws.J); };
+  auto vkept = [&](int64_t ib) {
+    return in_vd(ib) ? mat_cols(ws.Vd, mp, ws.Jkpos) : mat_cols(ws.Aw, mp, ws.Jk);
+  };
   const MatRef Qall = mixed ? mat(ws.Q64, mp) : mat(ws.Aw, mp);
+
 
   const int* d_s = &ws.st->s;
   const int* d_bt = &ws.st->bt;
+  // the caller's matrix in the permuted order from column ib + bt on (direct first block)
+  auto first_src = [&](int64_t ib) {
+    return FirstSrc{(const double*)dA, lda, perm, ib, d_bt, m};
+  };
   // Front of a block: sampling and the second Gram-Schmidt pass.  With `dyn`
   // the block's first column is ibase + *d_bt (the state still holds the
   // previous block's kept count): the front is queued before the previous
@@ -644,6 +665,9 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     if (mixed) {
       HIPCHK(rbrp_launch_gather_v64(st, mp, b, ws.st, ws.Aw32, mp, ws.J, ws.Vd, mp));
       ++cgs_launches;
+    } else if (direct && ibase == 0 && !dyn) {
+      HIPCHK(rbrp_launch_gather_first(st, (const double*)dA, lda, m, mp, b, ws.st, ws.J, ws.Vd, mp));
+      ++cgs_launches;
     }
     const int64_t imax = dyn ? std::min<int64_t>(ibase + b, kmax) : ibase;
     if (imax > 0) {
@@ -653,14 +677,14 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       // C = Q^T V computed as its transpose V^T Q (the sampled block on the
       // 64-row side of the tiles), reduced into C column-major
       choose_split(b, imax, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
-      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, Vblk, Qall, ws.P,
+      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, vblk(imax), Qall, ws.P,
                                    b, imax * (int64_t)b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), di, b, imax, ws.P, b,
                                        imax * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
                                        GEMM_SET, ws.Cbuf, ldr));
       HIPCHK(rbrp_launch_add_cols(st, di, imax, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, Qall,
-                                   mat(ws.Cbuf, ldr), Vblk, GEMM_SUB, h->nn64_resident));
+                                   mat(ws.Cbuf, ldr), vblk(imax), GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
       if (mixed) {
         // the corrected columns replace the stored residual (the filtered-out
@@ -697,14 +721,14 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
         return wide ? rbrp_launch_gemm_tn64(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b)
                     : rbrp_launch_gemm_tn(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b);
       };
-      HIPCHK(gram(dim_dyn(0, d_s, 1), Vblk));
+      HIPCHK(gram(dim_dyn(0, d_s, 1), vblk(i)));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, ws.P, b,
                                        (int64_t)b * b, ns, mat(ws.G, b), GEMM_SET));
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
                                         ws.Jk, ws.Jin, ws.Jout, ws.Jkpos));
       // Q1 = V(:, Jk) Rb^{-1}
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b,
-                                   dim_dyn(0, d_bt, 1), Vkept, mat(ws.Rinv, b), mat(ws.Qbuf, mp),
+                                   dim_dyn(0, d_bt, 1), vkept(i), mat(ws.Rinv, b), mat(ws.Qbuf, mp),
                                    GEMM_SET, h->nn64_resident));
       HIPCHK(gram(dim_dyn(0, d_bt, 1), mat(ws.Qbuf, mp)));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_bt, 1), dim_dyn(0, d_bt, 1), b, b, ws.P, b,
@@ -726,8 +750,10 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     }
     if (mixed)
       HIPCHK(rbrp_launch_move_cols32(st, mp, b, ws.st, ws.Aw32, mp, ws.Jin, ws.Jout));
-    else
+    else if (!(direct && i == 0))  // (the direct first block reads A through perm instead)
       HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
+    else
+      --perm_launches;
     // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
     HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
                                  mat(ws.Qbuf, mp), mat(ws.R2inv, b),
@@ -757,10 +783,13 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, ws.Aw32, i, nullptr, ws.Aw32, i, d_bt, mp, ws.P, b,
                                      (int64_t)b * nrem_max, ev0, ev1));
-      else
+      else {
+        const FirstSrc fs = first_src(i + 0);
         HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, mat_off(ws.Aw, mp, i), mat_off(ws.Aw, mp, i, d_bt),
-                                     ws.P, b, (int64_t)b * nrem_max, ev0, ev1));
+                                     ws.P, b, (int64_t)b * nrem_max, ev0, ev1,
+                                     (direct && i == 0) ? &fs : nullptr));
+      }
       pt.end(ph_proj, 1);
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
       if (mixed)
@@ -778,10 +807,13 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
         HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                         b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
                                         d_bt, ws.normpart, nrem_max, &groups, ev0, ev1));
-      else
+      else {
+        const FirstSrc fs = first_src(i + 0);
         HIPCHK(rbrp_launch_gemm_trail(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                       b, ws.Aw, mp, i, nullptr, ws.Wt, nrem_max, ws.Aw, mp, i, d_bt,
-                                      ws.normpart, nrem_max, &groups, ev0, ev1));
+                                      ws.normpart, nrem_max, &groups, ev0, ev1,
+                                      (direct && i == 0) ? &fs : nullptr));
+      }
       pt.end(ph_upd, 1);
     }
     // column norms, residual total and the state's publication to the host

@@ -80,6 +80,18 @@ __device__ inline double* mat_col(const MatRef& r, int64_t c) {
 
 enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
 
+// The caller's matrix as the source of the FP64 loop's first block (before a
+// working copy exists): column c is p + cols[coff + *coff_dyn + c] * ld (the
+// permuted order), rows >= mtrue read as zero.
+struct FirstSrc {
+  const double* p;
+  int64_t ld;
+  const int* cols;
+  int64_t coff;
+  const int* coff_dyn;
+  int64_t mtrue;
+};
+
 // ---- k_gemm.hip -----------------------------------------------------------
 // Split-K TN product: P[z](r, c) = sum_{kk in chunk z} A(kk, r) * B(kk, c),
 // r < M, c < N.  K must be a multiple of 16 and rows kk >= true height must be
@@ -91,10 +103,11 @@ hipError_t rbrp_launch_gemm_tn(hipStream_t st, DimSpec M, DimSpec N, int64_t Mma
 // The same product on 64 x 128 output tiles with double-buffered LDS (the
 // projection and the re-orthogonalisation coefficients).  K and kchunk
 // multiples of 16, both leading dimensions even.
+// With `first`, B is read from the caller's matrix instead (B is then unused).
 hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
                                  int64_t ldp, int64_t pstride, hipEvent_t e0 = nullptr,
-                                 hipEvent_t e1 = nullptr);
+                                 hipEvent_t e1 = nullptr, const FirstSrc* first = nullptr);
 // Workgroups of k_gemm_tn64 / k_gemm_tn32 resident at once on the current
 // device (compute units x occupancy); 0 if the runtime cannot tell.  `wide`:
 // the FP32 kernel's 128-row tile form (launched when A is wider than 64).
@@ -144,11 +157,22 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
                                   const int* acoff_dyn, const double* Bt, int64_t ldbt, double* C,
                                   int64_t ldc, int64_t ccoff, const int* ccoff_dyn,
                                   double* normpart, int64_t ldnp, int64_t* groups = nullptr,
-                                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                                  const FirstSrc* first = nullptr);
+// Whether rbrp_launch_gemm_trail takes a `first` source for these sizes (the
+// 128-column kernel: K <= 64, M a multiple of 64, lda even).
+bool rbrp_gemm_trail_first_ok(int64_t M, int64_t Kmax, int64_t lda);
 
 // ---- k_rbrp.hip -----------------------------------------------------------
 hipError_t rbrp_launch_copy_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m,
                                     int64_t mp, int64_t n, double* Aw, double* diags);
+// diags[j] = ||A(:, j)||^2 (m rows), a read of A only.
+hipError_t rbrp_launch_colnorm(hipStream_t st, const double* A, int64_t lda, int64_t m, int64_t n,
+                               double* diags);
+// V(:, p) = [A(0:m, J[p]); zero rows up to mp], p < bs->s (grid sized by b).
+hipError_t rbrp_launch_gather_first(hipStream_t st, const double* A, int64_t lda, int64_t m,
+                                    int64_t mp, int b, const RbrpBlockState* bs, const int* J,
+                                    double* V, int64_t ldv);
 hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
                                     int which);
 hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n);

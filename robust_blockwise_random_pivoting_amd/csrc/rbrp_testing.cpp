@@ -229,4 +229,43 @@ int rbrp_test_sample(void* stream, const double* diags, int64_t i, int64_t n, in
   return hs.status ? 100000 : 0;
 }
 
+int rbrp_test_sample_time(void* stream, const double* diags, int64_t n, int64_t b, uint64_t seed,
+                          int reps, double* ms_per_call) {
+  if (n < 1) return -3;
+  if (b < 1 || b > 256) return -4;
+  if (reps < 1) return -6;
+  if (!ms_per_call) return -7;
+  hipStream_t st = (hipStream_t)stream;
+  RbrpBlockState* d = nullptr;
+  double* scratch = nullptr;
+  int* J = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(RbrpBlockState)));
+  hipError_t e = hipMalloc(&scratch, sizeof(double) * 2 * n);
+  if (e == hipSuccess) e = hipMalloc(&J, sizeof(int) * b);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  RbrpBlockState hs{};
+  if (e == hipSuccess) e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
+  const uint64_t ctr[4] = {0, 0, 0, 0};
+  // one untimed call, then reps calls between two events (the draws accumulate)
+  if (e == hipSuccess) e = rbrp_launch_sample(st, diags, 0, n, n, (int)b, seed, 0, ctr, scratch, scratch + n, d, J);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    e = rbrp_launch_sample(st, diags, 0, n, n, (int)b, seed, 0, ctr, scratch, scratch + n, d, J);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) e = hipMemcpy(&hs, d, sizeof(hs), hipMemcpyDeviceToHost);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(d);
+  (void)hipFree(scratch);
+  (void)hipFree(J);
+  if (e != hipSuccess) return (int)e;
+  *ms_per_call = ms / reps;
+  return hs.status ? 100000 : 0;
+}
+
 }  // extern "C"

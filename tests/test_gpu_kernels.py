@@ -267,3 +267,36 @@ def test_sampler_matches_oracle(librbrp, hooks, N, i, b, skew, seed):
     assert s_out.value == s_ref
     assert np.array_equal(_host(dJ)[:s_ref], i + Js)
     assert draws.value == 7 + d
+
+
+@pytest.mark.parametrize("N,i,b,skew,dominant,seed", [
+    (70000, 0, 64, 1, False, 11),        # just above the single-workgroup range
+    (300000, 1000, 128, 8, True, 12),    # skewed + dominant weight: several rounds
+    (1 << 20, 0, 64, 1, False, 13),
+    (200000, 5, 256, 20, True, 14),      # most weight on a few hundred entries
+])
+def test_wide_sampler_matches_oracle(librbrp, hooks, N, i, b, skew, dominant, seed):
+    """The multi-workgroup sampler (candidate range > 65536) against the oracle:
+    same indices in the same order, same number of uniforms consumed."""
+    import torch
+    rng = np.random.default_rng(seed)
+    diags = rng.random(N) ** skew
+    diags[rng.random(N) < 0.1] = 0.0
+    if dominant:
+        diags[rng.integers(i, N, size=3)] = diags.sum() * 20
+    kmax = N
+    w = diags[i:]
+    s_ref = min(b, kmax - i, int(np.count_nonzero(w > 0)))
+    ga = uniform_stream(seed, 9)
+    ga.random(3)
+    Js, d = sample_without_replacement(w, s_ref, ga)
+    dd = torch.as_tensor(diags).cuda()
+    dJ = torch.zeros(b, dtype=torch.int32, device="cuda")
+    s_out, draws = ctypes.c_int32(), ctypes.c_uint64()
+    code = hooks.rbrp_test_sample(_stream(), _ptr(dd), i, N, kmax, b, seed, 9, 3, _ptr(dJ),
+                                  ctypes.byref(s_out), ctypes.byref(draws))
+    assert code == 0
+    assert s_out.value == s_ref
+    assert np.array_equal(_host(dJ)[:s_ref], i + Js)
+    assert draws.value == 3 + d
+    print(f"[wide sampler N={N}] s={s_ref} rounds' draws={d}")

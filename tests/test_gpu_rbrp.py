@@ -216,3 +216,13 @@ def test_cxx_call_surface_drop_in():
     e_half, e_full = cq["cpqr_sq_errors"]
     assert 0 <= e_full <= e_half and np.sqrt(e_full) <= 1e-4
     assert 0 < cq["t_ranks"][0] <= cq["t_ranks"][1] and cq["t_cpqr"] > 0
+
+
+def test_rbrp_wide_candidate_range_matches_oracle():
+    """n = 70000 columns (m = 160): the first blocks sample from more than 65536
+    candidates, which takes the multi-workgroup sampler inside the loop."""
+    rng = np.random.default_rng(31)
+    A = rng.standard_normal((160, 70000)) * np.exp(-np.arange(70000) / 20000.0)
+    A = np.asfortranarray(A)
+    g, r = run_both(A, 16, 1e-3, seed=4)
+    assert_parity(A, g, r, "wide 160x70000")
