@@ -361,7 +361,19 @@ __device__ __forceinline__ double nn64_mask(double x, long long m) {
 
 // The K loop of one workgroup for R row blocks (R a template argument so that
 // a range of R blocks issues exactly R blocks of MFMAs).
-template <int R>
+// Element `aoff` of column c of A: FP64, or (A32) an FP32 matrix held in
+// A.p as a float pointer, widened.
+template <bool A32>
+__device__ __forceinline__ double nn64_a(const MatRef& A, int64_t c, uint32_t aoff) {
+  if (A32) {
+    const int64_t cc =
+        A.cols ? (int64_t)A.cols[c] : (A.coff + (A.coff_dyn ? (int64_t)*A.coff_dyn : 0) + c);
+    return (double)((const float*)A.p)[cc * A.ld + aoff];
+  }
+  return mat_col(A, c)[aoff];
+}
+
+template <int R, bool A32 = false>
 __device__ __forceinline__ void nn64_loop(int64_t K, const MatRef& A, uint32_t aoff,
                                           const double* __restrict__ bcol, double* As0,
                                           double* Bs0, v4d* acc) {
@@ -381,10 +393,10 @@ __device__ __forceinline__ void nn64_loop(int64_t K, const MatRef& A, uint32_t a
 #define NN64_LOAD(S, kk)                                                                 \
   do {                                                                                   \
     const int64_t nl_ka = (kk) + ak;                                                     \
-    S##a0 = make_double2(mat_col(A, min(nl_ka, K - 1))[aoff],                            \
-                         mat_col(A, min(nl_ka + 1, K - 1))[aoff]);                       \
-    S##a1 = make_double2(mat_col(A, min(nl_ka + 2, K - 1))[aoff],                        \
-                         mat_col(A, min(nl_ka + 3, K - 1))[aoff]);                       \
+    S##a0 = make_double2(nn64_a<A32>(A, min(nl_ka, K - 1), aoff),                        \
+                         nn64_a<A32>(A, min(nl_ka + 1, K - 1), aoff));                   \
+    S##a1 = make_double2(nn64_a<A32>(A, min(nl_ka + 2, K - 1), aoff),                    \
+                         nn64_a<A32>(A, min(nl_ka + 3, K - 1), aoff));                   \
     const int64_t nl_kb = (kk) + bk;                                                     \
     S##b0 = make_double2(bcol[min(nl_kb, K - 1)], bcol[min(nl_kb + 1, K - 1)]);          \
     S##b1 = make_double2(bcol[min(nl_kb + 2, K - 1)], bcol[min(nl_kb + 3, K - 1)]);      \
@@ -505,6 +517,101 @@ __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, Di
         }
     }
   }
+}
+
+// Norms form of k_gemm_nn64 (the FP64 residual recompute of the mixed loop's
+// stopping test): E = C - A B is not stored; normpart[g * ldnp + c] receives
+// sum over the rows of row range g of E(r, c)^2, for r < M, c < N.  A and C
+// may be the caller's matrix (FP64, or FP32 with A32: A.p / C.p hold float
+// pointers) read through index-listed MatRefs; M is the true height (A's rows
+// are clamped to M - 1 by the stager, C's rows >= M are never read).
+template <bool A32>
+__global__ __launch_bounds__(256, 4) void k_gemm_nn64_resnorm(int64_t M, int64_t N, int64_t K,
+                                                              MatRef A, MatRef B, MatRef C, int gx,
+                                                              int gr, int64_t rbt,
+                                                              double* __restrict__ normpart,
+                                                              int64_t ldnp) {
+  const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gr);
+  if (task >= (int64_t)gx * gr) return;
+  const int64_t g = task / gx;
+  const int64_t rb0 = g * rbt / gr, rb1 = (g + 1) * rbt / gr;
+  const int R = (int)(rb1 - rb0);
+  const int64_t r0 = rb0 * 16, c0 = (task % gx) * 64;
+  if (R <= 0 || r0 >= M || c0 >= N || K <= 0) return;  // uniform per workgroup
+
+  __shared__ double As[2 * 64 * NN64_KS];
+  __shared__ double Bs[2 * 64 * NN64_KS];
+
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lj = lane & 15, lg = lane >> 4;
+  const int64_t arow = min(r0 + min(tid & 63, 16 * R - 1), M - 1);
+  MatRef Ar = A;
+  if (A32)
+    Ar.p = (double*)((const float*)A.p + r0);
+  else
+    Ar.p += r0;
+  const uint32_t aoff = (uint32_t)(arow - r0);
+  const double* bcol = mat_col(B, min(c0 + (tid >> 2), N - 1));
+
+  v4d acc[NN64_RB];
+#pragma unroll
+  for (int rb = 0; rb < NN64_RB; ++rb) acc[rb] = (v4d){0.0, 0.0, 0.0, 0.0};
+  switch (R) {
+    case 1: nn64_loop<1, A32>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    case 2: nn64_loop<2, A32>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    case 3: nn64_loop<3, A32>(K, Ar, aoff, bcol, As, Bs, acc); break;
+    default: nn64_loop<4, A32>(K, Ar, aoff, bcol, As, Bs, acc); break;
+  }
+  const int64_t c = c0 + 16 * wid + lg;
+  double sq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int rb = 0; rb < NN64_RB; ++rb) {
+    const int64_t r = r0 + 16 * rb + lj;
+    if (rb < R && r < M) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (c + 4 * v < N) {
+          const double cin = A32 ? (double)nn64_a<true>(C, c + 4 * v, (uint32_t)r)
+                                 : mat_col(C, c + 4 * v)[r];
+          const double e = cin - acc[rb][v];
+          sq[v] = fma(e, e, sq[v]);
+        }
+    }
+  }
+  // the sixteen lanes of one column (lj) added in a fixed order
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double x = sq[v];
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 8, 64);
+    if (lj == 0 && c + 4 * v < N) normpart[g * ldnp + c + 4 * v] = x;
+  }
+}
+
+// Row ranges of k_gemm_nn64 (and its norms form) for an Mmax x Nmax output.
+int64_t rbrp_gemm_nn64_ranges(int64_t Mmax, int64_t Nmax, int resident) {
+  const int64_t gx = (Nmax + 63) / 64, rbt = (Mmax + 15) / 16;
+  const int64_t per = std::max<int64_t>(1, (resident > 0 ? resident : 1024) / gx);
+  return std::max<int64_t>((rbt + NN64_RB - 1) / NN64_RB, std::min<int64_t>(rbt, per));
+}
+
+hipError_t rbrp_launch_gemm_nn64_resnorm(hipStream_t st, int64_t M, int64_t N, int64_t K, MatRef A,
+                                         MatRef B, MatRef C, int a32, double* normpart, int64_t ldnp,
+                                         int resident) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  const int64_t gx = (N + 63) / 64, rbt = (M + 15) / 16;
+  const int64_t gr = rbrp_gemm_nn64_ranges(M, N, resident);
+  if (ldnp < N) return hipErrorInvalidValue;
+  dim3 grid((unsigned)rbrp_xcd_grid(gx * gr));
+  if (a32)
+    hipLaunchKernelGGL(k_gemm_nn64_resnorm<true>, grid, dim3(256), 0, st, M, N, K, A, B, C, (int)gx,
+                       (int)gr, rbt, normpart, ldnp);
+  else
+    hipLaunchKernelGGL(k_gemm_nn64_resnorm<false>, grid, dim3(256), 0, st, M, N, K, A, B, C, (int)gx,
+                       (int)gr, rbt, normpart, ldnp);
+  return hipGetLastError();
 }
 
 int rbrp_gemm_nn64_resident() {

@@ -179,6 +179,20 @@ hipError_t rbrp_launch_zero(hipStream_t st, double* p, int64_t count) {
   return hipGetLastError();
 }
 
+// *out = sum of x[0:n) (one workgroup, fixed order).
+__global__ __launch_bounds__(1024) void k_sum(const double* __restrict__ x, int64_t n, double* out) {
+  __shared__ double red[16];
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 1024) acc += x[j];
+  const double t = rbrp_block_sum(acc, red);
+  if (threadIdx.x == 0) *out = t;
+}
+
+hipError_t rbrp_launch_sum(hipStream_t st, const double* x, int64_t n, double* out) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, st, x, n, out);
+  return hipGetLastError();
+}
+
 __global__ void k_iota(int* perm, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j < n) perm[j] = (int)j;

@@ -20,6 +20,8 @@ struct RbrpWorkspace {
   // are used instead)
   float *Aw32, *Wt32;
   double *Q64, *Vd;
+  // mixed loop: chunk totals and total of the FP64 residual recompute
+  double *rcol, *r64;
   int *J, *Jk, *Jin, *Jout, *Jkpos;
   RbrpBlockState* st;
   RbrpBlockState* st_snap;  // the last published block record (k_resid_total)

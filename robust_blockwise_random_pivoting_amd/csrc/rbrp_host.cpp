@@ -52,7 +52,7 @@ namespace {
 
 // Byte offsets of the loop's buffers in one arena (every buffer 256-byte aligned).
 struct WsLayout {
-  size_t oAw, oQ64, oVd, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
+  size_t oAw, oQ64, oVd, oRcol, oR64, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
       onp, oWt, otk, oto, oJ, oJk, oJin, oJout, oJkpos, oDinv, ost;
   size_t need;
   int64_t Pcap;
@@ -74,6 +74,9 @@ WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool m
   L.oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
   L.oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
   L.oVd = (mixed || vd) ? c.take<double>(mp * b) : 0;
+  // mixed loop: per-chunk totals and the total of the FP64 residual recompute
+  L.oRcol = mixed ? c.take<double>(n) : 0;
+  L.oR64 = mixed ? c.take<double>(1) : 0;
   L.oR = c.take<double>(ldr * n);
   L.ost = c.take<RbrpBlockState>(2);  // the block state and its published snapshot (after R:
                                       // one fill kernel clears both)
@@ -110,6 +113,8 @@ void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixe
   ws.Aw32 = mixed ? (float*)(base + L.oAw) : nullptr;
   ws.Q64 = mixed ? (double*)(base + L.oQ64) : nullptr;
   ws.Vd = (mixed || vd) ? (double*)(base + L.oVd) : nullptr;
+  ws.rcol = mixed ? (double*)(base + L.oRcol) : nullptr;
+  ws.r64 = mixed ? (double*)(base + L.oR64) : nullptr;
   ws.R = (double*)(base + L.oR);
   ws.diags = (double*)(base + L.odiags);
   ws.w = (double*)(base + L.ow);
@@ -697,6 +702,49 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     return 0;
   };
 
+  // Mixed loop: the FP64 squared ID residual ||A(:, perm[k:n]) - A(:, perm[0:k]) X||_F^2
+  // with X = R11^{-1} R12 (the error the ID returns), for the stopping test.  In
+  // column chunks: X's chunk by the blocked back substitution into the split-K
+  // scratch, then E = A(:, rd) - A(:, J) X by the norms form of the tall NN
+  // kernel reading A in place (FP64, or FP32 widened), which stores only
+  // per-row-range squared column sums; all sums in a fixed order.  Synchronises
+  // the stream; *out < 0 when a chunk's buffers cannot hold one column.
+  auto recompute64 = [&](int64_t k, double* out) -> int {
+    *out = -1.0;
+    const int64_t nrem = n - k;
+    if (nrem <= 0) {
+      *out = 0.0;
+      return 0;
+    }
+    if (k <= 0 || k > ws.Pcap) return 0;
+    const int64_t npcap = ((mp + 63) / 64) * n;  // norm-partial capacity
+    int64_t nc = std::min<int64_t>(nrem, ws.Pcap / k);
+    while (nc > 1 && rbrp_gemm_nn64_ranges(m, nc, h->nn64_resident) * nc > npcap) nc = (nc + 1) / 2;
+    if (rbrp_gemm_nn64_ranges(m, nc, h->nn64_resident) * nc > npcap) return 0;
+    const int ph64 = pt.begin(RBRP_PHASE_RESID);
+    int launches = 0;
+    int64_t chunks = 0;
+    double* const Aj = (double*)dA;  // (an FP32 matrix's float pointer when a_f32)
+    for (int64_t c0 = 0; c0 < nrem; c0 += nc, ++chunks) {
+      const int64_t ncc = std::min(nc, nrem - c0);
+      HIPCHK(rbrp_launch_copy_block(st, k, ncc, ws.R + (k + c0) * ldr, ldr, ws.P, k));
+      HIPCHK(rbrp_trsm_upper_left(st, k, ncc, ws.R, ldr, ws.P, k, &launches, ws.Dinv));
+      HIPCHK(rbrp_launch_gemm_nn64_resnorm(st, m, ncc, k, MatRef{Aj, lda, perm, 0, nullptr},
+                                           mat(ws.P, k), MatRef{Aj, lda, perm + k + c0, 0, nullptr},
+                                           a_f32 ? 1 : 0, ws.normpart, ncc, h->nn64_resident));
+      HIPCHK(rbrp_launch_sum(st, ws.normpart, rbrp_gemm_nn64_ranges(m, ncc, h->nn64_resident) * ncc,
+                             ws.rcol + chunks));
+      launches += 3;
+    }
+    HIPCHK(rbrp_launch_sum(st, ws.rcol, chunks, ws.r64));
+    pt.end(ph64, launches + 1, 2.0 * m * k * nrem + (double)k * k * nrem, ea * m * n);
+    double v = 0.0;
+    HIPCHK(hipMemcpyAsync(&v, ws.r64, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = v;
+    return 0;
+  };
+
   int64_t i = 0, nb = 0;
   double err = (N0 > 0.0) ? 1.0 : 0.0;
   int ph_cgs = -1;
@@ -855,15 +903,37 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     res->draws = hs.draws;
     err = hs.err;
     i += bt;
+    bool stop = err <= tol2 || i >= kmax;
+    // Mixed loop: the sum of the FP32-stored residual carries the rounding
+    // noise of every FP32 update, so where it cannot decide the test (from a
+    // factor 4 below to a factor 2 above tol^2) the FP64 recompute of the ID
+    // residual decides, and is the error recorded for the block.
+    bool requeue = false;
+    if (mixed && tol2 > 0.0 && i < kmax && i < n && err <= 2.0 * tol2 && err > 0.25 * tol2) {
+      double r64;
+      const int e = recompute64(i, &r64);
+      if (e != 0) return e;
+      if (r64 >= 0.0) {
+        err = r64 / N0;
+        // the queued front did nothing when the FP32 sum was already below tol^2
+        requeue = err > tol2 && hs.err <= tol2;
+        stop = err <= tol2;
+      }
+    }
     if (nb < res->trace_cap) {
       res->ranks[nb] = i;
       res->error[nb] = err;
       res->t_ranks[nb] = now_s() - t_start;
     }
     ++nb;
-    if (err <= tol2 || i >= kmax) {
-      pt.recs.resize(next_mark);  // the front that does nothing
+    if (stop) {
+      pt.recs.resize(next_mark);  // the front that does nothing or is not needed
       break;
+    }
+    if (requeue) {
+      pt.recs.resize(next_mark);
+      const int e = queue_front(i, false, &ph_cgs_next);
+      if (e != 0) return e;
     }
     ph_cgs = ph_cgs_next;
   }

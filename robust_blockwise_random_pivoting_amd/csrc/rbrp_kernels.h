@@ -124,6 +124,15 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
                                  DimSpec K, MatRef A, MatRef B, MatRef C, int mode,
                                  int resident = 0);
 int rbrp_gemm_nn64_resident();
+// Row ranges of k_gemm_nn64 and its norms form for an Mmax x Nmax output.
+int64_t rbrp_gemm_nn64_ranges(int64_t Mmax, int64_t Nmax, int resident);
+// Norms form: normpart[g * ldnp + c] = sum over row range g of
+// (C(r, c) - sum_k A(r, k) B(k, c))^2, r < M (true height), c < N, for g <
+// rbrp_gemm_nn64_ranges(M, N, resident); E is not stored.  With a32, A.p and
+// C.p hold FP32 matrices (float pointers), widened on load.
+hipError_t rbrp_launch_gemm_nn64_resnorm(hipStream_t st, int64_t M, int64_t N, int64_t K, MatRef A,
+                                         MatRef B, MatRef C, int a32, double* normpart, int64_t ldnp,
+                                         int resident = 0);
 
 // dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order (skipped when dst.p is
 // null); when dst2 != nullptr the sum is also stored row-major at
@@ -176,6 +185,8 @@ hipError_t rbrp_launch_gather_first(hipStream_t st, const double* A, int64_t lda
 hipError_t rbrp_launch_sum_to_state(hipStream_t st, const double* x, int64_t n, RbrpBlockState* bs,
                                     int which);
 hipError_t rbrp_launch_iota(hipStream_t st, int* perm, int64_t n);
+// *out = sum of x[0:n) in a fixed order (one workgroup).
+hipError_t rbrp_launch_sum(hipStream_t st, const double* x, int64_t n, double* out);
 // p[0:count) = 0 by a fill kernel.
 hipError_t rbrp_launch_zero(hipStream_t st, double* p, int64_t count);
 // Sampler of a block whose first column is i (+ *i_dyn with i_dyn: then the
