@@ -52,7 +52,7 @@ namespace {
 
 // Byte offsets of the loop's buffers in one arena (every buffer 256-byte aligned).
 struct WsLayout {
-  size_t oAw, oQ64, oVd, oRcol, oR64, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
+  size_t oAw, oAw32, oQ64, oVd, oRcol, oR64, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
       onp, oWt, otk, oto, oJ, oJk, oJin, oJout, oJkpos, oDinv, ost;
   size_t need;
   int64_t Pcap;
@@ -64,14 +64,18 @@ int64_t full_pcap(int64_t n, int64_t kmax) {
   return 64 * RBRP_MAX_B * std::max<int64_t>(std::max(n, kmax), RBRP_MAX_B);
 }
 
-// `vd`: the FP64 block buffer Vd (the mixed loop's sampled block, the FP64
-// loop's first sampled block when that block reads A directly).
-WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed, int64_t Pcap,
-                          bool vd) {
+// `mixed`: the FP32 working matrix, the FP64 skeleton basis and the buffers of
+// the FP64 recompute; `w64`: the FP64 working matrix (the FP64 loop, and the
+// mixed loop's FP64 first blocks); `vd`: the FP64 block buffer Vd (the mixed
+// loop's sampled block, the FP64 loop's first sampled block when that block
+// reads A directly).
+WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed, bool w64,
+                          int64_t Pcap, bool vd) {
   const int64_t ldr = std::max<int64_t>(kmax, 1);
   WsLayout L;
   Carver c;
-  L.oAw = mixed ? c.take<float>(mp * n) : c.take<double>(mp * n);
+  L.oAw = w64 ? c.take<double>(mp * n) : 0;
+  L.oAw32 = mixed ? c.take<float>(mp * n) : 0;
   L.oQ64 = mixed ? c.take<double>(mp * ldr) : 0;
   L.oVd = (mixed || vd) ? c.take<double>(mp * b) : 0;
   // mixed loop: per-chunk totals and the total of the FP64 residual recompute
@@ -108,9 +112,10 @@ WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool m
 }
 
 // Points the workspace's buffers into the arena at `base`.
-void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixed, bool vd) {
-  ws.Aw = mixed ? nullptr : (double*)(base + L.oAw);
-  ws.Aw32 = mixed ? (float*)(base + L.oAw) : nullptr;
+void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixed, bool w64,
+                     bool vd) {
+  ws.Aw = w64 ? (double*)(base + L.oAw) : nullptr;
+  ws.Aw32 = mixed ? (float*)(base + L.oAw32) : nullptr;
   ws.Q64 = mixed ? (double*)(base + L.oQ64) : nullptr;
   ws.Vd = (mixed || vd) ? (double*)(base + L.oVd) : nullptr;
   ws.rcol = mixed ? (double*)(base + L.oRcol) : nullptr;
@@ -129,7 +134,7 @@ void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixe
   ws.Qbuf = (double*)(base + L.oQbuf);
   ws.P = (double*)(base + L.oP);
   ws.normpart = (double*)(base + L.onp);
-  ws.Wt = mixed ? nullptr : (double*)(base + L.oWt);
+  ws.Wt = (double*)(base + L.oWt);  // (one of the two per block: FP64 or FP32 storage)
   ws.Wt32 = mixed ? (float*)(base + L.oWt) : nullptr;
   ws.tmpk = (double*)(base + L.otk);
   ws.tmpo = (double*)(base + L.oto);
@@ -146,8 +151,8 @@ void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixe
 
 // The handle's cached arena, grown on demand.
 int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed,
-                     bool vd) {
-  const WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax), vd);
+                     bool w64, bool vd) {
+  const WsLayout L = layout_workspace(mp, n, kmax, b, mixed, w64, full_pcap(n, kmax), vd);
   RbrpWorkspace& ws = h->ws;
   if (ws.arena_bytes < L.need) {
     if (ws.arena) {
@@ -162,7 +167,7 @@ int ensure_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64
     }
     ws.arena_bytes = L.need;
   }
-  carve_workspace(ws, (char*)ws.arena, L, mixed, vd);
+  carve_workspace(ws, (char*)ws.arena, L, mixed, w64, vd);
   return 0;
 }
 
@@ -218,16 +223,17 @@ void* scratch_ptr(int dev, hipError_t* err) {
 // arena path's split-K partials when they fit (same launch shapes, so the
 // same results as rbrp_colid_d), else with partials for at most 16 splits.
 int static_workspace(rbrp_handle_t h, int64_t mp, int64_t n, int64_t kmax, int64_t b, bool mixed,
-                     bool vd, RbrpWorkspace* ws) {
+                     bool w64, bool vd, RbrpWorkspace* ws) {
   hipError_t e = hipSuccess;
   void* base = scratch_ptr(h->device, &e);
   if (!base) return e != hipSuccess ? (int)e : (int)hipErrorNotInitialized;
-  WsLayout L = layout_workspace(mp, n, kmax, b, mixed, full_pcap(n, kmax), vd);
+  WsLayout L = layout_workspace(mp, n, kmax, b, mixed, w64, full_pcap(n, kmax), vd);
   if (L.need > RBRP_NOALLOC_SCRATCH_BYTES)
-    L = layout_workspace(mp, n, kmax, b, mixed, 16 * b * std::max<int64_t>(std::max(n, kmax), b), vd);
+    L = layout_workspace(mp, n, kmax, b, mixed, w64, 16 * b * std::max<int64_t>(std::max(n, kmax), b),
+                         vd);
   if (L.need > RBRP_NOALLOC_SCRATCH_BYTES) return RBRP_ERR_SCRATCH;
   *ws = RbrpWorkspace();
-  carve_workspace(*ws, (char*)base, L, mixed, vd);
+  carve_workspace(*ws, (char*)base, L, mixed, w64, vd);
   return 0;
 }
 
@@ -592,18 +598,35 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const int b = (int)opts->b;
   const int64_t mp = round_up(m, 64);
   const int64_t ldr = std::max<int64_t>(kmax, 1);
+  // Storage schedule of the mixed loop.  An FP32 update of a residual of
+  // relative size rho adds rounding noise of about 8 u32 rho ||A||_F (u32 =
+  // 2^-24; measured 6.4 on the exp-decay matrices), and the residual shrinks
+  // by a factor of about two per block, so the noise of all FP32 blocks is
+  // about 8 u32 * 1.2 * rho_s ||A||_F for a loop that turns to FP32 storage at
+  // relative residual rho_s.  The loop runs its first blocks in FP64 until the
+  // squared relative residual is at most rstar2 = rho_s^2 with that noise at
+  // 6% of tol (kNoiseFrac); with tol = 0, or a tol large enough that rho_s >= 1,
+  // it is FP32 from the start.  The tracked FP32 sum then decides the stopping
+  // test except within (1 -/+ 2 kNoiseFrac) tol^2, where the FP64 recompute of
+  // the ID residual decides.
+  constexpr double kNoiseFrac = 0.06, kNoisePerRho = 8.0 * 1.2, kU32 = 5.9604644775390625e-08;
+  const double rstar = opts->tol > 0.0 ? kNoiseFrac * opts->tol / (kNoisePerRho * kU32) : HUGE_VAL;
+  const bool phase64 = mixed && rstar < 1.0;  // FP64 storage first, FP32 from rstar2 on
+  const double rstar2 = rstar * rstar;
+  bool m32 = mixed && !phase64;               // the current block's storage is FP32
   // The FP64 loop on FP64 input reads A itself in its first block (no working
   // copy is made up front: the norm pass only reads A, the first block's
   // sampled columns are gathered from A, and its projection and update read
   // the trailing columns from A through the permutation, the update writing
   // the working copy).  This needs the 128-column update kernel (b <= 64).
-  const bool direct = !mixed && !a_f32 && rbrp_gemm_trail_first_ok(mp, b, mp);
+  const bool direct = !m32 && !a_f32 && rbrp_gemm_trail_first_ok(mp, b, mp);
   RbrpWorkspace ws_static;
   {
     // (before the first launch: a workspace that cannot be had leaves every output untouched)
-    const int e = noalloc ? static_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, direct,
-                                             &ws_static)
-                          : ensure_workspace(h, mp, n, kmax, std::max<int64_t>(b, 1), mixed, direct);
+    const int64_t bb = std::max<int64_t>(b, 1);
+    const bool w64 = !m32, vd = mixed || direct;
+    const int e = noalloc ? static_workspace(h, mp, n, kmax, bb, mixed, w64, vd, &ws_static)
+                          : ensure_workspace(h, mp, n, kmax, bb, mixed, w64, vd);
     if (e) return e;
   }
   RbrpWorkspace& ws = noalloc ? ws_static : h->ws;
@@ -613,7 +636,6 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   const uint64_t key0 = opts->seed, key1 = 0;
   const uint64_t ctr[4] = {opts->stream_offset, 0, 0, 0};
   const double ea = a_f32 ? 4.0 : 8.0;      // bytes per element of the input
-  const double ew = mixed ? 4.0 : 8.0;      // bytes per element of the working matrix
   PhaseTimer pt(h, !noalloc);
 
   // R and the two block-state records (laid out right after it) start at zero
@@ -621,13 +643,13 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
   int ph = pt.begin(RBRP_PHASE_NORMS);
   if (direct)
     HIPCHK(rbrp_launch_colnorm(st, (const double*)dA, lda, m, n, ws.diags));
-  else if (mixed)
+  else if (m32)
     HIPCHK(rbrp_launch_copy_colnorm32(st, dA, a_f32 ? 1 : 0, lda, m, mp, n, ws.Aw32, ws.diags));
   else if (a_f32)
     HIPCHK(rbrp_launch_copy_colnorm_from32(st, (const float*)dA, lda, m, mp, n, ws.Aw, ws.diags));
   else
     HIPCHK(rbrp_launch_copy_colnorm(st, (const double*)dA, lda, m, mp, n, ws.Aw, ws.diags));
-  pt.end(ph, 1, 2.0 * m * n, ea * m * n + (direct ? 0.0 : ew * mp * n) + 8.0 * n);
+  pt.end(ph, 1, 2.0 * m * n, ea * m * n + (direct ? 0.0 : (m32 ? 4.0 : 8.0) * mp * n) + 8.0 * n);
   HIPCHK(rbrp_launch_sum_to_state(st, ws.diags, n, ws.st, 0));
   HIPCHK(hipMemcpyAsync(h->host_state, ws.st, sizeof(RbrpBlockState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -640,12 +662,12 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
 
   // FP64 operands of the sampled block (gathered into Vd in the mixed loop
   // and in the direct first block, else read in place) and of the skeleton basis
-  auto in_vd = [&](int64_t ib) { return mixed || (direct && ib == 0); };
+  auto in_vd = [&](int64_t ib) { return m32 || (direct && ib == 0); };
   auto vblk = [&](int64_t ib) { return in_vd(ib) ? mat(ws.Vd, mp) : mat_cols(ws.Aw, mp, ws.J); };
   auto vkept = [&](int64_t ib) {
     return in_vd(ib) ? mat_cols(ws.Vd, mp, ws.Jkpos) : mat_cols(ws.Aw, mp, ws.Jk);
   };
-  const MatRef Qall = mixed ? mat(ws.Q64, mp) : mat(ws.Aw, mp);
+  auto qall = [&]() { return m32 ? mat(ws.Q64, mp) : mat(ws.Aw, mp); };
 
 
   const int* d_s = &ws.st->s;
@@ -667,7 +689,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     pt.end(ph, 1, 0.0, 16.0 * (n - ibase));
     *ph_cgs = pt.begin(RBRP_PHASE_CGS);
     int cgs_launches = 0;
-    if (mixed) {
+    if (m32) {
       HIPCHK(rbrp_launch_gather_v64(st, mp, b, ws.st, ws.Aw32, mp, ws.J, ws.Vd, mp));
       ++cgs_launches;
     } else if (direct && ibase == 0 && !dyn) {
@@ -682,16 +704,16 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       // C = Q^T V computed as its transpose V^T Q (the sampled block on the
       // 64-row side of the tiles), reduced into C column-major
       choose_split(b, imax, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
-      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, vblk(imax), Qall, ws.P,
+      HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, vblk(imax), qall(), ws.P,
                                    b, imax * (int64_t)b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), di, b, imax, ws.P, b,
                                        imax * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
                                        GEMM_SET, ws.Cbuf, ldr));
       HIPCHK(rbrp_launch_add_cols(st, di, imax, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
-      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, Qall,
+      HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, qall(),
                                    mat(ws.Cbuf, ldr), vblk(imax), GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
-      if (mixed) {
+      if (m32) {
         // the corrected columns replace the stored residual (the filtered-out
         // ones stay in the trailing matrix)
         HIPCHK(rbrp_launch_scatter_v32(st, mp, b, ws.st, ws.Vd, mp, ws.J, ws.Aw32, mp));
@@ -796,7 +818,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       HIPCHK(rbrp_launch_perm_scatter_r(st, i, b, i, ws.st, ws.R, ldr, ws.Jin, ws.tmpk, ws.tmpo, ldr));
       perm_launches += 2;
     }
-    if (mixed)
+    if (m32)
       HIPCHK(rbrp_launch_move_cols32(st, mp, b, ws.st, ws.Aw32, mp, ws.Jin, ws.Jout));
     else if (!(direct && i == 0))  // (the direct first block reads A through perm instead)
       HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
@@ -805,9 +827,9 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
     HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
                                  mat(ws.Qbuf, mp), mat(ws.R2inv, b),
-                                 mixed ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
+                                 m32 ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
                                  h->nn64_resident));
-    if (mixed) {
+    if (m32) {
       HIPCHK(rbrp_launch_store_q32(st, mp, b, i, ws.st, ws.Q64, mp, ws.Aw32, mp));
       ++perm_launches;
     }
@@ -821,13 +843,13 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       int ns;
       int64_t kc;
       // (the FP32 kernel takes a block wider than 64 in one 128-row tile)
-      const bool wide32 = mixed && b > 64;
-      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, mixed ? 32 : 16, 128,
-                   mixed ? (wide32 ? h->tn32w_resident : h->tn32_resident) : h->tn64_resident,
+      const bool wide32 = m32 && b > 64;
+      choose_split(b, nrem_max, mp, b, ws.Pcap, &ns, &kc, m32 ? 32 : 16, 128,
+                   m32 ? (wide32 ? h->tn32w_resident : h->tn32_resident) : h->tn64_resident,
                    wide32 ? 128 : 64);
       hipEvent_t ev0, ev1;
       ph_proj = pt.begin_kernel(RBRP_PHASE_PROJECT, &ev0, &ev1);
-      if (mixed)
+      if (m32)
         HIPCHK(rbrp_launch_gemm_tn32(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b, nrem_max,
                                      mp, ns, kc, ws.Aw32, i, nullptr, ws.Aw32, i, d_bt, mp, ws.P, b,
                                      (int64_t)b * nrem_max, ev0, ev1));
@@ -840,7 +862,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       }
       pt.end(ph_proj, 1);
       ph = pt.begin(RBRP_PHASE_PROJECT_RED);
-      if (mixed)
+      if (m32)
         HIPCHK(rbrp_launch_reduce_splits_f32copy(st, dim_dyn(0, d_bt, 1), dim_dyn(n - i, d_bt, -1), b,
                                                  nrem_max, ws.P, b, (int64_t)b * nrem_max, ns,
                                                  mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt32,
@@ -851,7 +873,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                          mat_off(ws.R + i, ldr, i, d_bt), GEMM_SET, ws.Wt, nrem_max));
       pt.end(ph, 1);
       ph_upd = pt.begin_kernel(RBRP_PHASE_UPDATE, &ev0, &ev1);
-      if (mixed)
+      if (m32)
         HIPCHK(rbrp_launch_gemm_trail32(st, mp, dim_dyn(n - i, d_bt, -1), nrem_max, dim_dyn(0, d_bt, 1),
                                         b, ws.Aw32, mp, i, nullptr, ws.Wt32, nrem_max, ws.Aw32, mp, i,
                                         d_bt, ws.normpart, nrem_max, &groups, ev0, ev1));
@@ -895,6 +917,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     const int64_t nrem = n - i - bt;
     if (h->timing) {
       // algorithmic work, now that the block's sizes are known
+      const double ew = m32 ? 4.0 : 8.0;  // bytes per element of the working matrix
       pt.set_work(ph_cgs, i > 0 ? 4.0 * m * i * s : 0.0, i > 0 ? 2.0 * 8.0 * m * i + 16.0 * m * s : 0.0);
       pt.set_work(ph_proj, 2.0 * m * bt * nrem, ew * (m * nrem + m * bt) + 8.0 * bt * nrem);
       pt.set_work(ph_upd, 2.0 * m * bt * nrem, ew * (2.0 * m * nrem + m * bt + bt * nrem));
@@ -904,12 +927,14 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     err = hs.err;
     i += bt;
     bool stop = err <= tol2 || i >= kmax;
-    // Mixed loop: the sum of the FP32-stored residual carries the rounding
-    // noise of every FP32 update, so where it cannot decide the test (from a
-    // factor 4 below to a factor 2 above tol^2) the FP64 recompute of the ID
+    // FP32 storage: the sum of the stored residual carries the rounding noise
+    // of the FP32 updates (at most kNoiseFrac tol by the schedule above), so
+    // where that noise could decide the test the FP64 recompute of the ID
     // residual decides, and is the error recorded for the block.
     bool requeue = false;
-    if (mixed && tol2 > 0.0 && i < kmax && i < n && err <= 2.0 * tol2 && err > 0.25 * tol2) {
+    const double band_lo = (1.0 - 2.0 * kNoiseFrac) * tol2,
+                 band_hi = (1.0 + 2.0 * kNoiseFrac + kNoiseFrac * kNoiseFrac) * tol2;
+    if (m32 && tol2 > 0.0 && i < kmax && i < n && err <= band_hi && err > band_lo) {
       double r64;
       const int e = recompute64(i, &r64);
       if (e != 0) return e;
@@ -934,6 +959,19 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       pt.recs.resize(next_mark);
       const int e = queue_front(i, false, &ph_cgs_next);
       if (e != 0) return e;
+    }
+    if (phase64 && !m32 && err <= rstar2) {
+      // FP32 storage from the next block on (its front, already queued, has
+      // re-orthogonalised its sampled columns in the FP64 working matrix):
+      // the FP64 skeleton basis, the FP32 residual with the norms of the stored
+      // values, and the sampled block's FP64 copy for the panel
+      ph = pt.begin(RBRP_PHASE_NORMS);
+      HIPCHK(rbrp_launch_copy_block(st, mp, i, ws.Aw, mp, ws.Q64, mp));
+      HIPCHK(rbrp_launch_copy_colnorm32(st, ws.Aw + i * mp, 0, mp, mp, mp, n - i, ws.Aw32 + i * mp,
+                                        ws.diags + i));
+      HIPCHK(rbrp_launch_gather_first(st, ws.Aw, mp, mp, mp, b, ws.st, ws.J, ws.Vd, mp));
+      pt.end(ph, 3, 0.0, 8.0 * mp * n + 4.0 * mp * (n - i));
+      m32 = true;
     }
     ph_cgs = ph_cgs_next;
   }

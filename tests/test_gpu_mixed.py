@@ -135,10 +135,12 @@ def _mixed_vs_oracle(A, b, tol, seed=7, name=""):
     print(f"[mixed {name}] k32={g.k} k64={r.k} err32={e32:.4e} err64={e64:.4e} "
           f"tracked32={g.err_final:.4e} same-prefix={same}")
     assert abs(g.k - r.k) <= b, name
-    assert e32 <= 1.1 * tol, (name, e32)
-    assert e32 <= 1.1 * max(e64, tol), (name, e32, e64)
-    # the tracked FP32-residual error agrees with the recomputed one
-    assert g.err_final == pytest.approx(e32, rel=0.05, abs=1e-7), name
+    assert e32 <= 1.01 * tol, (name, e32)
+    # the same error as the FP64 oracle within 10% (the storage schedule keeps
+    # the FP32 rounding noise at a few percent of tol)
+    assert abs(e32 - e64) <= 0.1 * e64 + 1e-13, (name, e32, e64)
+    # the reported error agrees with the recomputed one
+    assert g.err_final == pytest.approx(e32, rel=0.02, abs=1e-12), name
     return g, r
 
 
