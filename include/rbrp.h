@@ -239,6 +239,35 @@ int rbrp_randlupp_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n
                     int32_t* d_sk, int32_t* d_rd, double* dT, int64_t ldt);
 
 /*
+ * Adaptive randomised LUPP ID.  Replaces `void RandAdapLUPP(const double *A,
+ * int m, int n, int *&sk, int *&rd, double *&T, int &rank, double &flops,
+ * double tol=1e-8, int blk=16)` (gpu/rid.hpp:5-7, declared there and never
+ * defined under gpu/), with the semantics of blocked_rlupp_id
+ * (python/cache.ipynb c8:80-143) in column form.  Block t draws a Gaussian
+ * sketch Y_t = G_t A (blk rows, G_t ~ N(0, 1/blk) from the Philox stream with
+ * key (seed, 1) and counter offset + t*blk*m).  The residual of Y_0 is Y_0;
+ * each block takes an LU with partial pivoting of the candidates' residual
+ * transposed (candidates x blk): its first blk pivots join the skeleton, the
+ * interpolation matrix M = L2 L1^{-1} of the accumulated unit-lower factor is
+ * extended, and the next sketch gives both the residual
+ * Y(:, cand)^T - M Y(:, skel)^T of the next block and the error estimate
+ * ||that residual||_F (an unbiased estimate of ||A(:, rd) - A(:, sk) T||_F).
+ * Stops when the estimate is at most tol * ||A||_F (relative; the notebook
+ * compares it with an absolute tol) or the rank reaches min(m, n).
+ * Outputs (device, caller-allocated with worst-case sizes): d_sk (n entries;
+ * the first *rank are the skeleton), d_rd (n entries; the first n - *rank are
+ * the rest), dT (*rank x (n - *rank), leading dimension ldt >= min(m, n)),
+ * a block being cut short where it would take the rank past min(m, n),
+ * T = M^T so that A(:, rd) ~ A(:, sk) T.  err_est (optional) receives the
+ * last estimate relative to ||A||_F, flops (optional) the arithmetic count.
+ * blk is 1..32.
+ */
+int rbrp_randadaplupp_d(rbrp_handle_t handle, const double* dA, int64_t m, int64_t n, int64_t lda,
+                        double tol, int64_t blk, uint64_t seed, uint64_t offset, int32_t* d_sk,
+                        int32_t* d_rd, double* dT, int64_t ldt, int64_t* rank, double* err_est,
+                        double* flops);
+
+/*
  * Oversampled interpolation matrix.  Replaces Compute_interpolation_matrix
  * (gpu/interp.cu:45-105): Yp = Y(:, P); QR of Yp(:, 0:k); T = R^{-1} Q^T Yp(:, k:n)
  * (k x (n-k), leading dimension ldt).  Y is l x n (ldy); P (device) has n entries.

@@ -246,4 +246,47 @@ inline void RandLUPP_OS(const double* A, int m, int n, int rank, int*& sk, int*&
   flops = 2.0 * mm * nn * k + 2.0 * mm * k * k / 3.0 + k * k * (mm - k);
 }
 
+// gpu/rid.hpp:5-7 (declared there, defined nowhere under gpu/; the semantics
+// are blocked_rlupp_id of python/cache.ipynb c8:80-143).  Adaptive rank: sk,
+// rd (device int arrays of rank and n - rank entries) and T (device, rank x
+// (n - rank), leading dimension rank) are allocated here with hipMalloc and
+// released by the caller with hipFree; tol is relative to ||A||_F.
+inline void RandAdapLUPP(const double* A, int m, int n, int*& sk, int*& rd, double*& T, int& rank,
+                         double& flops, double tol = 1e-8, int blk = 16, uint64_t seed = 0) {
+  sk = rd = nullptr;
+  T = nullptr;
+  rank = 0;
+  flops = 0.0;
+  rbrp_handle_t h = nullptr;
+  int32_t *wsk = nullptr, *wrd = nullptr;
+  double* wT = nullptr;
+  const size_t p = (size_t)(m < n ? m : n);
+  int st = rbrp_create(&h, nullptr);
+  if (!st) st = (int)hipMalloc(&wsk, sizeof(int32_t) * (size_t)n);
+  if (!st) st = (int)hipMalloc(&wrd, sizeof(int32_t) * (size_t)n);
+  if (!st) st = (int)hipMalloc(&wT, sizeof(double) * p * (size_t)n);
+  int64_t k = 0;
+  if (!st)
+    st = rbrp_randadaplupp_d(h, A, m, n, m, tol, blk, seed, 0, wsk, wrd, wT, (int64_t)p, &k, nullptr,
+                             &flops);
+  if (!st) {
+    // exact-size outputs (T repacked with leading dimension rank)
+    rank = (int)k;
+    st = (int)hipMalloc(&sk, sizeof(int) * (size_t)(k > 0 ? k : 1));
+    if (!st) st = (int)hipMemcpy(sk, wsk, sizeof(int) * (size_t)k, hipMemcpyDeviceToDevice);
+    if (!st && n > k) st = (int)hipMalloc(&rd, sizeof(int) * (size_t)(n - k));
+    if (!st && n > k)
+      st = (int)hipMemcpy(rd, wrd, sizeof(int) * (size_t)(n - k), hipMemcpyDeviceToDevice);
+    if (!st && n > k && k > 0) st = (int)hipMalloc(&T, sizeof(double) * (size_t)k * (size_t)(n - k));
+    if (!st && n > k && k > 0)
+      st = (int)hipMemcpy2D(T, sizeof(double) * (size_t)k, wT, sizeof(double) * p, sizeof(double) * (size_t)k,
+                            (size_t)(n - k), hipMemcpyDeviceToDevice);
+  }
+  if (wsk) (void)hipFree(wsk);
+  if (wrd) (void)hipFree(wrd);
+  if (wT) (void)hipFree(wT);
+  if (h) rbrp_destroy(h);
+  rbrp_detail::last_status() = st;
+}
+
 #endif  // RBRP_HPP_

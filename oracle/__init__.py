@@ -26,6 +26,7 @@ from .sketch_oracle import (  # noqa: F401
     rand_lupp_os,
     rand_cpqr,
     rand_lupp,
+    rand_adap_lupp,
     pivots_to_permutation,
 )
 from .cpqr_oracle import cpqr_id, partial_time_fraction  # noqa: F401

@@ -2,7 +2,9 @@
 
 ``rand_cpqr_os`` restates gpu/randCPQR_OS.cu:20-111 (spec matlab/randCPQR_OS.m:1-49)
 and ``rand_lupp_os`` restates gpu/randLUPP_OS.cu:14-119 (spec
-matlab/randLUPP_OS.m:1-54).  Both take the Gaussian sketch matrix ``G`` as an
+matlab/randLUPP_OS.m:1-54).  ``rand_adap_lupp`` restates blocked_rlupp_id
+(python/cache.ipynb c8:80-143), the only definition of the behaviour that
+gpu/rid.hpp:5-7 declares as ``RandAdapLUPP``.  Both take the Gaussian sketch matrix ``G`` as an
 explicit argument so that a device implementation and this oracle can be fed
 the identical sketch.  Pivoted QR and LU come from SciPy's LAPACK
 (``dgeqp3``, ``dgetrf``), which is what the reference calls through MAGMA and
@@ -100,3 +102,61 @@ def rand_lupp(A, k: int, G):
     L21 = LU[k:, :k]
     T = sla.solve_triangular(L11, L21.T, lower=True, trans="T", unit_diagonal=True)
     return P[:k].copy(), P[k:].copy(), T
+
+
+def rand_adap_lupp(A, tol: float, b: int, sketch):
+    """Adaptive randomised LUPP ID; returns ``(sk, rd, T, rank, est)``.
+
+    Column form of blocked_rlupp_id (python/cache.ipynb c8:80-143) with one
+    estimation sketch per block (its ``num_smp`` = 1, as
+    blocked_rlupp_id_local, c8:1-60).  ``sketch(t)`` returns block t's sketch
+    matrix ``G_t`` (b x m).  Block 0's residual is ``(G_0 A)^T``; each block
+    factors the candidates' residual (candidates x bt) with partial pivoting
+    (c8:94), moves its first bt pivots to the skeleton (c8:106-107), and keeps
+    the interpolation matrix ``M = L2 L1^{-1}`` of the accumulated unit-lower
+    factor (c8:97-103, 111-116) up to date; the next sketch ``Y`` gives the
+    next residual ``Y(:, cand)^T - M Y(:, skel)^T`` (c8:115-116) and the error
+    estimate, its Frobenius norm (c8:117).  Stops when the estimate is at most
+    ``tol * ||A||_F`` (the notebook compares with an absolute tol, c8:128) or
+    the rank reaches min(m, n).  ``T = M^T``, so ``A(:, rd) ~ A(:, sk) T``;
+    ``est`` is the last estimate relative to ``||A||_F``.
+    """
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    p = min(m, n)
+    normA2 = float(np.sum(A * A))
+    P = np.arange(n)
+    M = np.zeros((n, p))
+    Rt = np.zeros((n, b))
+    Y = np.asarray(sketch(0), dtype=np.float64) @ A
+    Rt[:, :] = Y[:, P].T
+    k, t = 0, 0
+    est2 = normA2
+    while k < p:
+        bt = min(b, p - k)           # (never past min(m, n))
+        nc = n - k
+        lu, ipiv = sla.lu_factor(Rt[k:, :bt].copy(), check_finite=False)
+        L = np.tril(lu, -1)[:, :bt] + np.eye(nc, bt)
+        for j, q in enumerate(np.asarray(ipiv, dtype=np.int64)):
+            if q != j:
+                a, c = k + j, k + int(q)
+                M[[a, c], :k] = M[[c, a], :k]
+                P[[a, c]] = P[[c, a]]
+        if nc > bt:
+            Mb = sla.solve_triangular(L[:bt].T, L[bt:].T, lower=False, unit_diagonal=True).T
+            if k > 0:
+                M[k + bt:, :k] -= Mb @ M[k:k + bt, :k]
+            M[k + bt:, k:k + bt] = Mb
+        k += bt
+        t += 1
+        if k >= p:
+            est2 = 0.0
+            break
+        Y = np.asarray(sketch(t), dtype=np.float64) @ A
+        Rt[k:, :] = Y[:, P[k:]].T - M[k:, :k] @ Y[:, P[:k]].T
+        est2 = float(np.sum(Rt[k:, :] ** 2))
+        if est2 <= tol * tol * normA2:
+            break
+    T = M[k:, :k].T.copy()
+    est = float(np.sqrt(est2 / normA2)) if normA2 > 0 else 0.0
+    return P[:k].copy(), P[k:].copy(), T, k, est

@@ -6,10 +6,10 @@ package does not load the library; the first call does, and raises
 ImportError if it has not been built.
 """
 
-from .rbrp import (CPQR, RBRP, Batch, ColID, Handle, RandCPQR, RandCPQR_OS,  # noqa: F401
-                   RandLUPP, RandLUPP_OS, batched_tune, colmajor, gaussian_sketch, interp_os,
+from .rbrp import (CPQR, RBRP, Batch, ColID, Handle, RandAdapLUPP, RandCPQR,  # noqa: F401
+                   RandCPQR_OS, RandLUPP, RandLUPP_OS, batched_tune, colmajor, gaussian_sketch, interp_os,
                    rbrp_colid, rbrp_colid_batched, uniforms)
 
-__all__ = ["CPQR", "RBRP", "Batch", "ColID", "Handle", "RandCPQR", "RandCPQR_OS", "RandLUPP",
+__all__ = ["CPQR", "RBRP", "Batch", "ColID", "Handle", "RandAdapLUPP", "RandCPQR", "RandCPQR_OS", "RandLUPP",
            "RandLUPP_OS", "batched_tune", "colmajor", "gaussian_sketch", "interp_os", "rbrp_colid",
            "rbrp_colid_batched", "uniforms"]

@@ -144,6 +144,8 @@ def lib():
         "rbrp_randcpqr_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp, i64]),
         "rbrp_randlupp_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp, i64]),
         "rbrp_interp_os_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, i64]),
+        "rbrp_randadaplupp_d": (c_int, [vp, vp, i64, i64, i64, c_double, i64, u64, u64, vp, vp, vp,
+                                        i64, POINTER(c_int64), POINTER(c_double), POINTER(c_double)]),
         "rbrp_cpqr_id_d": (c_int, [vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp]),
         "rbrp_matrix_ge_h": (c_int, [i64, u64, vp]),
         "rbrp_matrix_snn_h": (c_int, [i64, c_double, u64, vp]),

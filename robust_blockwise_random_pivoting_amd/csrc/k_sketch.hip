@@ -57,6 +57,63 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
   return hipGetLastError();
 }
 
+// out(r, j) = Y(j, P[p0 + r]) for r < nr, j < nj: the transposed sketch
+// columns of the given positions of the permutation P (RandAdapLUPP).
+__global__ void k_gather_cols_t(int64_t nr, const double* __restrict__ Y, int64_t ldy,
+                                const int* __restrict__ P, int64_t p0, double* __restrict__ out,
+                                int64_t ldo) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (r >= nr) return;
+  out[r + j * ldo] = Y[j + (int64_t)P[p0 + r] * ldy];
+}
+
+hipError_t rbrp_launch_gather_cols_t(hipStream_t st, int64_t nr, int64_t nj, const double* Y,
+                                     int64_t ldy, const int* P, int64_t p0, double* out, int64_t ldo) {
+  if (nr <= 0 || nj <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_cols_t, dim3((unsigned)((nr + 255) / 256), (unsigned)nj), dim3(256), 0,
+                     st, nr, Y, ldy, P, p0, out, ldo);
+  return hipGetLastError();
+}
+
+// The row interchanges base + j <-> base + ipiv[j], j = 0 .. nb-1 in order,
+// applied to columns 0 .. cols-1 of M (ld ldm) and to the int array P: one
+// thread per column (thread `cols` takes P).
+__global__ void k_swap_rows(int nb, int64_t base, const int* __restrict__ ipiv, double* M, int64_t ldm,
+                            int64_t cols, int* P) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < cols) {
+    double* col = M + c * ldm + base;
+    for (int j = 0; j < nb; ++j) {
+      const int q = ipiv[j];
+      if (q != j) {
+        const double t = col[j];
+        col[j] = col[q];
+        col[q] = t;
+      }
+    }
+  } else if (c == cols && P) {
+    int* pp = P + base;
+    for (int j = 0; j < nb; ++j) {
+      const int q = ipiv[j];
+      if (q != j) {
+        const int t = pp[j];
+        pp[j] = pp[q];
+        pp[q] = t;
+      }
+    }
+  }
+}
+
+hipError_t rbrp_launch_swap_rows(hipStream_t st, int nb, int64_t base, const int* ipiv, double* M,
+                                 int64_t ldm, int64_t cols, int* P) {
+  if (nb <= 0) return hipSuccess;
+  const int64_t threads = cols + 1;
+  hipLaunchKernelGGL(k_swap_rows, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, nb, base,
+                     ipiv, M, ldm, cols, P);
+  return hipGetLastError();
+}
+
 // dst (cols x rows, ldd) = src(0:rows, 0:cols)^T, 32x32 tiles through LDS.
 __global__ void k_transpose(int64_t rows, int64_t cols, const double* __restrict__ src, int64_t lds,
                             double* __restrict__ dst, int64_t ldd) {

@@ -292,6 +292,13 @@ hipError_t rbrp_launch_gather_cols(hipStream_t st, int64_t rows, int64_t cols, c
                                    int64_t lds, const int* perm, double* dst, int64_t ldd);
 hipError_t rbrp_launch_transpose(hipStream_t st, int64_t rows, int64_t cols, const double* src,
                                  int64_t lds, double* dst, int64_t ldd);
+// out(r, j) = Y(j, P[p0 + r]), r < nr, j < nj (transposed gather of sketch columns).
+hipError_t rbrp_launch_gather_cols_t(hipStream_t st, int64_t nr, int64_t nj, const double* Y,
+                                     int64_t ldy, const int* P, int64_t p0, double* out, int64_t ldo);
+// Row interchanges base + j <-> base + ipiv[j] (j < nb, in order) on columns
+// 0 .. cols-1 of M and on P (optional).
+hipError_t rbrp_launch_swap_rows(hipStream_t st, int nb, int64_t base, const int* ipiv, double* M,
+                                 int64_t ldm, int64_t cols, int* P);
 hipError_t rbrp_launch_unit_upper(hipStream_t st, int64_t k, double* R, int64_t ld);
 // One unpivoted Householder step j of a panel ending at column cend (the
 // reflector and its application in one launch; column j is completed by the

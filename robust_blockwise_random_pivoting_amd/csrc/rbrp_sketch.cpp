@@ -168,32 +168,45 @@ hipError_t interp_os(hipStream_t st, int64_t l, int64_t n, int64_t k, const doub
 // Linv is LU_NB x LU_NB scratch.
 constexpr int64_t LU_NB = 32;
 
-hipError_t lupp_permutation(hipStream_t st, int64_t n, int64_t k, double* Yt, int* ipiv, int* P,
-                            int32_t* d_sk, int32_t* d_rd, double* Linv) {
-  const int64_t steps = std::min(n, k);
+// Blocked LU with partial pivoting of Y (rows x cols, leading dimension ld)
+// in place to min(rows, cols) steps: LU_NB-column panels with the swaps
+// confined to the panel, then the panel's interchanges on the other columns,
+// U12 = L11^{-1} A12 and A22 -= L21 U12 (dgetrf's order).  ipiv[j] (0-based,
+// device) is the row swapped with row j.
+hipError_t lupp_factor(hipStream_t st, int64_t rows, int64_t cols, double* Y, int64_t ld, int* ipiv,
+                       double* Linv) {
+  const int64_t steps = std::min(rows, cols);
   hipError_t e;
   for (int64_t j0 = 0; j0 < steps; j0 += LU_NB) {
     const int64_t je = std::min(j0 + LU_NB, steps), jb = je - j0;
     for (int64_t j = j0; j < je; ++j) {
-      if ((e = rbrp_launch_lu_pivot(st, n, j0, je, j, Yt, n, ipiv)) != hipSuccess) return e;
-      if ((e = rbrp_launch_lu_update(st, n, je, j, Yt, n)) != hipSuccess) return e;
+      if ((e = rbrp_launch_lu_pivot(st, rows, j0, je, j, Y, ld, ipiv)) != hipSuccess) return e;
+      if ((e = rbrp_launch_lu_update(st, rows, je, j, Y, ld)) != hipSuccess) return e;
     }
-    if ((e = rbrp_launch_lu_laswp(st, k, j0, je, Yt, n, ipiv)) != hipSuccess) return e;
-    if (je < k) {
-      double* A12 = Yt + j0 + je * n;
-      if ((e = rbrp_launch_unit_lower_inv(st, (int)jb, Yt + j0 + j0 * n, n, Linv, LU_NB)) != hipSuccess)
+    if ((e = rbrp_launch_lu_laswp(st, cols, j0, je, Y, ld, ipiv)) != hipSuccess) return e;
+    if (je < cols) {
+      double* A12 = Y + j0 + je * ld;
+      if ((e = rbrp_launch_unit_lower_inv(st, (int)jb, Y + j0 + j0 * ld, ld, Linv, LU_NB)) != hipSuccess)
         return e;
-      if ((e = rbrp_launch_gemm_nn(st, dim_static(jb), jb, dim_static(k - je), k - je, dim_static(jb),
-                                   mat(Linv, LU_NB), mat(A12, n), mat(A12, n), GEMM_SET, nullptr, 0)) !=
+      if ((e = rbrp_launch_gemm_nn(st, dim_static(jb), jb, dim_static(cols - je), cols - je, dim_static(jb),
+                                   mat(Linv, LU_NB), mat(A12, ld), mat(A12, ld), GEMM_SET, nullptr, 0)) !=
           hipSuccess)
         return e;
-      if (je < n &&
-          (e = rbrp_launch_gemm_nn(st, dim_static(n - je), n - je, dim_static(k - je), k - je, dim_static(jb),
-                                   mat(Yt + je + j0 * n, n), mat(A12, n), mat(Yt + je + je * n, n), GEMM_SUB,
-                                   nullptr, 0)) != hipSuccess)
+      if (je < rows &&
+          (e = rbrp_launch_gemm_nn(st, dim_static(rows - je), rows - je, dim_static(cols - je), cols - je,
+                                   dim_static(jb), mat(Y + je + j0 * ld, ld), mat(A12, ld),
+                                   mat(Y + je + je * ld, ld), GEMM_SUB, nullptr, 0)) != hipSuccess)
         return e;
     }
   }
+  return hipSuccess;
+}
+
+hipError_t lupp_permutation(hipStream_t st, int64_t n, int64_t k, double* Yt, int* ipiv, int* P,
+                            int32_t* d_sk, int32_t* d_rd, double* Linv) {
+  const int64_t steps = std::min(n, k);
+  hipError_t e;
+  if ((e = lupp_factor(st, n, k, Yt, n, ipiv, Linv)) != hipSuccess) return e;
   std::vector<int> hpiv((size_t)steps), hP((size_t)n);
   if ((e = hipMemcpyAsync(hpiv.data(), ipiv, sizeof(int) * steps, hipMemcpyDeviceToHost, st)) != hipSuccess)
     return e;
@@ -461,6 +474,144 @@ int rbrp_randlupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
     HIPCHK(rbrp_trsm_upper_left(st, k, n - k, Lt, k, dT, ldt, nullptr, Dinv));
   }
   HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int rbrp_randadaplupp_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda,
+                        double tol, int64_t blk, uint64_t seed, uint64_t offset, int32_t* d_sk,
+                        int32_t* d_rd, double* dT, int64_t ldt, int64_t* rank, double* err_est,
+                        double* flops) {
+  if (!h) return -1;
+  if (m < 1) return -3;
+  if (n < 1) return -4;
+  if (lda < m) return -5;
+  if (!dA) return -2;
+  if (!(tol >= 0.0) || !std::isfinite(tol)) return -6;
+  if (blk < 1 || blk > LU_NB) return -7;
+  if (!d_sk) return -10;
+  if (!d_rd) return -11;
+  const int64_t p = std::min(m, n);
+  if (!dT) return -12;
+  if (ldt < p) return -13;
+  if (!rank) return -14;
+  hipStream_t st = h->stream;
+  const int64_t b = blk;
+  // workspace: sketch G (b x m), Y = G A (b x n), the transposed residual Rt
+  // (n x b, rows in permutation order), the skeleton part Ys (p x b), the
+  // interpolation matrix M (n x p, rows in permutation order), P, pivots, the
+  // unit-lower inverse of a panel, column norms and their sum
+  const size_t bG = align256(sizeof(double) * b * m), bY = align256(sizeof(double) * b * n);
+  const size_t bRt = align256(sizeof(double) * n * b), bYs = align256(sizeof(double) * p * b);
+  const size_t bM = align256(sizeof(double) * n * p), bP = align256(sizeof(int) * n);
+  const size_t bip = align256(sizeof(int) * b), bLinv = align256(sizeof(double) * LU_NB * LU_NB);
+  const size_t bcn = align256(sizeof(double) * (std::max(n, b) + 1));
+  const size_t bGt = sketch_scratch_bytes(h, b, m, n, lda);
+  const size_t base = bG + bY + bRt + bYs + bM + bP + bip + bLinv + bcn;
+  void* arena = nullptr;
+  HIPCHK(rbrp_sketch_arena(h, base + bGt, &arena));
+  char* q = (char*)arena;
+  double* G = (double*)q;
+  q += bG;
+  double* Y = (double*)q;
+  q += bY;
+  double* Rt = (double*)q;
+  q += bRt;
+  double* Ys = (double*)q;
+  q += bYs;
+  double* M = (double*)q;
+  q += bM;
+  int* P = (int*)q;
+  q += bP;
+  int* ipiv = (int*)q;
+  q += bip;
+  double* Linv = (double*)q;
+  q += bLinv;
+  double* cn = (double*)q;
+  double* Gt = bGt ? (double*)((char*)arena + base) : nullptr;
+  double fl = 0.0;
+  auto sumsq = [&](const double* X, int64_t rows, int64_t cols, int64_t ld, double* out) -> int {
+    HIPCHK(rbrp_launch_colnorm(st, X, ld, rows, cols, cn));
+    HIPCHK(rbrp_launch_sum(st, cn, cols, cn + std::max(n, b)));
+    HIPCHK(hipMemcpyAsync(out, cn + std::max(n, b), sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  };
+  // block t's sketch: G_t ~ N(0, 1/b) from the Philox stream (seed, offset + t b m)
+  auto sketch = [&](int64_t t) -> int {
+    HIPCHK(rbrp_launch_gaussian(st, seed, offset + (uint64_t)t * (uint64_t)(b * m), b, m,
+                                1.0 / std::sqrt((double)b), G, b));
+    HIPCHK(sketch_gemm(h, b, m, n, G, b, dA, lda, Y, Gt));
+    fl += 2.0 * b * m * n;
+    return 0;
+  };
+  double normA2 = 0.0;
+  {
+    const int e = sumsq(dA, m, n, lda, &normA2);
+    if (e) return e;
+  }
+  const double tol2 = tol * tol * normA2;
+  HIPCHK(rbrp_launch_iota(st, P, n));
+  // block 0: the residual is the first sketch itself
+  {
+    const int e = sketch(0);
+    if (e) return e;
+  }
+  HIPCHK(rbrp_launch_gather_cols_t(st, n, b, Y, b, P, 0, Rt, n));
+  int64_t k = 0, t = 0;
+  double est2 = normA2;
+  while (k < p) {
+    // (a block never takes the rank past min(m, n), the width of M)
+    const int64_t bt = std::min(b, p - k), nc = n - k;
+    // LU with partial pivoting of the candidates' residual (nc x bt)
+    HIPCHK(lupp_factor(st, nc, bt, Rt + k, n, ipiv, Linv));
+    fl += (double)nc * bt * bt;
+    // its interchanges on the candidates' rows of M and on P
+    HIPCHK(rbrp_launch_swap_rows(st, (int)bt, k, ipiv, M, n, k, P));
+    // M_b = L2b Ltt^{-1} for the new columns; the old columns M(rest) -= M_b M(new)
+    if (nc > bt) {
+      HIPCHK(rbrp_launch_unit_lower_inv(st, (int)bt, Rt + k, n, Linv, LU_NB));
+      HIPCHK(rbrp_launch_gemm_nn(st, dim_static(nc - bt), nc - bt, dim_static(bt), bt, dim_static(bt),
+                                 mat(Rt + k + bt, n), mat(Linv, LU_NB), mat(M + (k + bt) + k * n, n),
+                                 GEMM_SET, nullptr, 0));
+      if (k > 0)
+        HIPCHK(rbrp_launch_gemm_nn(st, dim_static(nc - bt), nc - bt, dim_static(k), k, dim_static(bt),
+                                   mat(M + (k + bt) + k * n, n), mat(M + k, n), mat(M + (k + bt), n),
+                                   GEMM_SUB, nullptr, 0));
+      fl += 2.0 * (nc - bt) * bt * (bt + k);
+    }
+    k += bt;
+    ++t;
+    if (k >= p || k >= n) {
+      est2 = 0.0;
+      break;
+    }
+    // a fresh sketch: the estimate of the ID error and the next block's residual,
+    // Rt(k:n) = Y(:, P[k:n])^T - M(k:n, 0:k) Y(:, P[0:k])^T
+    {
+      const int e = sketch(t);
+      if (e) return e;
+    }
+    HIPCHK(rbrp_launch_gather_cols_t(st, k, b, Y, b, P, 0, Ys, p));
+    HIPCHK(rbrp_launch_gather_cols_t(st, n - k, b, Y, b, P, k, Rt + k, n));
+    HIPCHK(rbrp_launch_gemm_nn(st, dim_static(n - k), n - k, dim_static(b), b, dim_static(k),
+                               mat(M + k, n), mat(Ys, p), mat(Rt + k, n), GEMM_SUB, nullptr, 0));
+    fl += 2.0 * (n - k) * k * b;
+    {
+      const int e = sumsq(Rt + k, n - k, b, n, &est2);
+      if (e) return e;
+    }
+    if (est2 <= tol2) break;
+  }
+  // outputs: sk = P[0:k], rd = P[k:n], T = M(k:n, 0:k)^T (k x (n-k))
+  HIPCHK(hipMemcpyAsync(d_sk, P, sizeof(int) * k, hipMemcpyDeviceToDevice, st));
+  if (n > k) {
+    HIPCHK(hipMemcpyAsync(d_rd, P + k, sizeof(int) * (n - k), hipMemcpyDeviceToDevice, st));
+    HIPCHK(rbrp_launch_transpose(st, n - k, k, M + k, n, dT, ldt));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  *rank = k;
+  if (err_est) *err_est = normA2 > 0.0 ? std::sqrt(std::max(est2, 0.0) / normA2) : 0.0;
+  if (flops) *flops = fl;
   return 0;
 }
 

@@ -435,6 +435,40 @@ def CPQR(A, ranks, handle: Optional[Handle] = None):
     return list(err[: len(rk)]), list(tr[: len(rk)]), tq.value, fl.value, perm
 
 
+def RandAdapLUPP(A, tol: float = 1e-8, blk: int = 16, seed: int = 0, offset: int = 0,
+                 handle: Optional[Handle] = None):
+    """Mirror of ``RandAdapLUPP(A, m, n, sk, rd, T, rank, flops, tol, blk)``
+    (gpu/rid.hpp:5-7; the semantics of blocked_rlupp_id, python/cache.ipynb
+    c8:80-143, in column form): adaptive-rank randomised LUPP ID of the device
+    matrix ``A`` (FP64, m x n), one Gaussian sketch of ``blk`` rows per block,
+    stopping when the sketched error estimate is at most ``tol * ||A||_F``.
+    Returns ``(sk, rd, T, rank, err_est, flops)``: device int32 skeleton and
+    redundant indices, ``T`` (rank x (n - rank)) with ``A[:, rd] ~ A[:, sk] @ T``,
+    the last relative estimate and the arithmetic count."""
+    torch = _torch()
+    A = colmajor(A)
+    m, n = A.shape
+    p = min(m, n)
+    sk = torch.empty(max(n, 1), dtype=torch.int32, device=A.device)
+    rd = torch.empty(max(n, 1), dtype=torch.int32, device=A.device)
+    Tb = torch.empty((max(n, 1), max(p, 1)), dtype=torch.float64, device=A.device)
+    rank, est, fl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_randadaplupp_d", h._lib.rbrp_randadaplupp_d(
+            h.ptr, ctypes.c_void_p(A.data_ptr()), m, n, leading_dim(A), float(tol), int(blk),
+            int(seed), int(offset), ctypes.c_void_p(sk.data_ptr()), ctypes.c_void_p(rd.data_ptr()),
+            ctypes.c_void_p(Tb.data_ptr()), max(p, 1), ctypes.byref(rank), ctypes.byref(est),
+            ctypes.byref(fl)))
+    finally:
+        if own:
+            h.close()
+    k = int(rank.value)
+    T = Tb[: n - k, :k].t()
+    return sk[:k], rd[: n - k], T, k, float(est.value), float(fl.value)
+
+
 def interp_os(Y, P, k: int, handle: Optional[Handle] = None):
     """Mirror of ``Compute_interpolation_matrix`` (gpu/interp.cu:45-105): T (k x (n-k))."""
     torch = _torch()

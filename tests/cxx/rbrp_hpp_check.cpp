@@ -77,6 +77,33 @@ int main() {
   if (lT) (void)hipFree(lT);
   if (lsk) (void)hipFree(lsk);
   if (lrd) (void)hipFree(lrd);
+  // adaptive randomised LUPP ID (gpu/rid.hpp:5-7) at a relative tolerance
+  int *ask = nullptr, *ard = nullptr, arank = 0;
+  double* aT = nullptr;
+  double afl = 0;
+  RandAdapLUPP(dA, m, n, ask, ard, aT, arank, afl, 1e-4, 16, 7);
+  const int st6 = RBRP_last_status();
+  double eada = -1.0;
+  if (st6 == 0 && arank > 0 && arank < n) {
+    std::vector<int> ah((size_t)arank), arh((size_t)(n - arank));
+    (void)hipMemcpy(ah.data(), ask, sizeof(int) * arank, hipMemcpyDeviceToHost);
+    (void)hipMemcpy(arh.data(), ard, sizeof(int) * (n - arank), hipMemcpyDeviceToHost);
+    std::vector<double> hT((size_t)arank * (n - arank));
+    (void)hipMemcpy(hT.data(), aT, sizeof(double) * hT.size(), hipMemcpyDeviceToHost);
+    double num2 = 0;
+    for (int j = 0; j < n - arank; ++j)
+      for (int i = 0; i < m; ++i) {
+        double v = A[i + (size_t)arh[j] * m];
+        for (int q = 0; q < arank; ++q) v -= A[i + (size_t)ah[q] * m] * hT[q + (size_t)j * arank];
+        num2 += v * v;
+      }
+    eada = std::sqrt(num2 / den);
+  }
+  if (ask) (void)hipFree(ask);
+  if (ard) (void)hipFree(ard);
+  if (aT) (void)hipFree(aT);
+  std::printf("{\"status6\": %d, \"adap_rank\": %d, \"adap_error\": %.6e, \"adap_flops\": %.6e}\n",
+              st6, arank, eada, afl);
   // full-matrix CPQR ID at half the rank and at the rank the ID found
   std::vector<double> qerr, qt;
   double tq = 0, qfl = 0;
@@ -93,5 +120,5 @@ int main() {
               st1, ranks.empty() ? 0 : ranks.back(), ranks.size(), st2, k, std::sqrt(num / den), flops);
   if (X) (void)hipFree(X);
   (void)hipFree(dA);
-  return (st1 == 0 && st2 == 0 && st3 == 0 && st4 == 0 && st5 == 0) ? 0 : 1;
+  return (st1 == 0 && st2 == 0 && st3 == 0 && st4 == 0 && st5 == 0 && st6 == 0) ? 0 : 1;
 }

@@ -216,6 +216,10 @@ def test_cxx_call_surface_drop_in():
     e_half, e_full = cq["cpqr_sq_errors"]
     assert 0 <= e_full <= e_half and np.sqrt(e_full) <= 1e-4
     assert 0 < cq["t_ranks"][0] <= cq["t_ranks"][1] and cq["t_cpqr"] > 0
+    ad = json.loads(lines[-4])
+    assert ad["status6"] == 0 and ad["adap_rank"] > 0 and ad["adap_rank"] % 16 == 0
+    # the sketched estimate stops near tol 1e-4 (an estimate, so within a factor)
+    assert 0 <= ad["adap_error"] <= 3e-4 and ad["adap_flops"] > 0
 
 
 def test_rbrp_wide_candidate_range_matches_oracle():

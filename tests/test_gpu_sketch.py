@@ -8,8 +8,8 @@ kappa(R1) * eps relative.
 import numpy as np
 import pytest
 
-from oracle import (compute_interpolation_matrix_os, rand_cpqr, rand_cpqr_os, rand_lupp, rand_lupp_os,
-                    uniform_stream)
+from oracle import (compute_interpolation_matrix_os, rand_adap_lupp, rand_cpqr, rand_cpqr_os,
+                    rand_lupp, rand_lupp_os, uniform_stream)
 from robust_blockwise_random_pivoting_amd.matrices import exp_spectrum, gmm_rows, spectrum_matrix
 
 pytestmark = pytest.mark.gpu
@@ -140,3 +140,47 @@ def test_interp_os_parity_and_internal_sketch():
     sk, rd, T = RandCPQR_OS(_dev(A), 60, seed=3)
     E = A[:, rd] - A[:, sk] @ T.cpu().numpy()
     assert np.linalg.norm(E) / np.linalg.norm(A) < 1e-2
+
+
+ADAP_CASES = [
+    ("exp 400x300 b16 tol1e-4", lambda: spectrum_matrix(400, 300, exp_spectrum(300, 12.0), 21), 16, 1e-4),
+    ("exp 300x500 b8 tol1e-6", lambda: spectrum_matrix(300, 500, exp_spectrum(300, 10.0), 22), 8, 1e-6),
+    ("gmm 900x200 b32 tol1e-2", lambda: gmm_rows(900, 200, 23, components=12), 32, 1e-2),
+    ("exp 2000x1200 b16 tol1e-7", lambda: spectrum_matrix(2000, 1200, exp_spectrum(1200, 40.0), 24), 16,
+     1e-7),
+]
+
+
+@pytest.mark.parametrize("name,gen,b,tol", ADAP_CASES)
+def test_rand_adap_lupp_parity(name, gen, b, tol):
+    """RandAdapLUPP (gpu/rid.hpp:5-7 with blocked_rlupp_id's semantics) against the
+    oracle fed the identical device sketches: same skeleton order and rank, T to
+    a conditioning-aware bound, same estimate; the ID's true error tracks it."""
+    from robust_blockwise_random_pivoting_amd import RandAdapLUPP, gaussian_sketch
+    A = gen()
+    m, n = A.shape
+    seed, off = 5, 11
+    sk, rd, T, k, est, fl = RandAdapLUPP(_dev(A), tol=tol, blk=b, seed=seed, offset=off)
+    cache = {}
+
+    def sketch(t):
+        if t not in cache:
+            cache[t] = gaussian_sketch(b, m, 1.0 / np.sqrt(b), seed, off + t * b * m).cpu().numpy()
+        return cache[t]
+
+    sk_r, rd_r, T_r, k_r, est_r = rand_adap_lupp(A, tol, b, sketch)
+    assert k == k_r, name
+    assert np.array_equal(sk.cpu().numpy(), sk_r), name
+    assert np.array_equal(rd.cpu().numpy(), rd_r), name
+    if 0 < k < n:
+        Th = T.cpu().numpy()
+        rel = np.linalg.norm(Th - T_r) / max(np.linalg.norm(T_r), 1e-300)
+        kappa = np.linalg.cond(A[:, sk_r])
+        print(f"[adap {name}] k={k} est={est:.3e} T rel diff {rel:.2e} (kappa(A_sk) {kappa:.2e})")
+        assert rel <= max(1e-8, 1e3 * kappa * EPS), (name, rel)
+        e = np.linalg.norm(A[:, rd_r] - A[:, sk_r] @ Th) / np.linalg.norm(A)
+        # the sketched estimate is an unbiased estimate of the true error
+        assert e <= 4 * max(est, tol) and est <= 4 * max(e, 1e-15), (name, e, est)
+    assert est == pytest.approx(est_r, rel=1e-6, abs=1e-14), name
+    assert est <= tol or k == min(m, n)
+    assert fl > 0

@@ -315,3 +315,105 @@ def test_sketch_ids_without_oversampling_solve_the_square_sketch():
         assert np.array_equal(rand_cpqr(A, r, G)[0], piv[:r])
         P, _, _ = sla.lu(Y.T)
         assert np.array_equal(rand_lupp(A, r, G)[0], np.argmax(P, axis=0)[:r])
+
+
+# ---- RandAdapLUPP (blocked_rlupp_id, python/cache.ipynb c8:80-143) -------------
+
+def _notebook_rlupp(A, tol, b, sketch):
+    """The notebook's steps restated directly (row ID of A^T, one estimation sample):
+    scipy.linalg.lu of the residual, the stacked unit-lower L with its rows permuted
+    and new columns appended, the residual sample[cand] - L2 L1^{-1} sample[skel]
+    by a triangular solve, the stop on the estimate (relative here)."""
+    At = np.asarray(A, dtype=np.float64).T          # rows = the column ID's columns
+    n = At.shape[0]
+    normA = np.linalg.norm(At)
+    residual = At @ sketch(0).T                     # sample (n x b)
+    skeletons = np.array([], dtype=int)
+    candidates = np.arange(n)
+    L = None
+    for t in range(n // b):
+        P_, L_t, _ = sla.lu(residual)
+        p_row = (P_.T @ np.arange(residual.shape[0]).reshape(-1, 1)).astype(int).reshape(-1)
+        if L is None:
+            L = L_t
+        else:
+            loc = np.arange(len(skeletons), n)
+            L[loc, :] = L[loc[p_row], :]
+            L = np.hstack((L, np.vstack((np.zeros((n - L_t.shape[0], b)), L_t))))
+        skeletons = np.concatenate((skeletons, candidates[p_row[:b]])).astype(int)
+        candidates = candidates[p_row[b:]].astype(int)
+        k = L.shape[1]
+        L1, L2 = L[:k, :], L[k:, :]
+        sample = At @ sketch(t + 1).T
+        residual = sample[candidates, :] - L2 @ sla.solve_triangular(L1, sample[skeletons, :], lower=True)
+        est = np.linalg.norm(residual)
+        if est <= tol * normA:
+            break
+    W = sla.solve_triangular(L1, np.eye(k), lower=True)   # L1^{-1}
+    T = (L2 @ W).T
+    return skeletons, candidates, T, est / normA, np.linalg.cond(L1)
+
+
+def _sketches(b, m, seed):
+    rng = np.random.default_rng(seed)
+    cache = {}
+
+    def g(t):
+        if t not in cache:
+            cache[t] = rng.standard_normal((b, m)) / np.sqrt(b)
+        return cache[t]
+    return g
+
+
+@pytest.mark.parametrize("m,n,b,tol,seed", [(300, 240, 16, 1e-4, 1), (200, 320, 8, 1e-4, 2),
+                                             (500, 200, 32, 1e-2, 3)])
+def test_rand_adap_lupp_matches_notebook_steps(m, n, b, tol, seed):
+    from oracle import rand_adap_lupp
+    A = spectrum_matrix(m, n, exp_spectrum(min(m, n), 15.0), seed)
+    g = _sketches(b, m, seed)
+    sk, rd, T, k, est = rand_adap_lupp(A, tol, b, g)
+    sk2, rd2, T2, est2, kappa = _notebook_rlupp(A, tol, b, g)
+    assert np.array_equal(sk, sk2) and np.array_equal(rd, rd2)
+    # the two forms of L2 L1^{-1} (kept up to date vs solved at the end) agree
+    # to rounding (a wrong index or term would differ at O(1))
+    bound = max(1e-8, 1e3 * kappa * np.finfo(np.float64).eps)
+    assert np.linalg.norm(T - T2) <= bound * np.linalg.norm(T2)
+    assert est == pytest.approx(est2, rel=max(1e-8, bound))
+    assert est <= tol and k % b == 0 and k < min(m, n)
+    e1 = np.linalg.norm(A[:, rd] - A[:, sk] @ T)
+    e2 = np.linalg.norm(A[:, rd2] - A[:, sk2] @ T2)
+    assert e1 == pytest.approx(e2, rel=1e-6)
+
+
+def test_rand_adap_lupp_exact_rank_is_exact():
+    """A of exact rank r <= b: one block's skeleton spans the range, the next
+    sketch's residual is zero, and T reproduces A(:, rd) exactly."""
+    from oracle import rand_adap_lupp
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((120, 10)) @ rng.standard_normal((10, 90))
+    sk, rd, T, k, est = rand_adap_lupp(A, 1e-10, 16, _sketches(16, 120, 5))
+    assert k == 16 and est < 1e-12
+    assert np.linalg.norm(A[:, rd] - A[:, sk] @ T) <= 1e-10 * np.linalg.norm(A)
+
+
+def test_rand_adap_lupp_first_block_is_lu_pivot_order():
+    """The first block's skeleton is the first b pivot rows of scipy's LU of (G_0 A)^T."""
+    from oracle import rand_adap_lupp
+    A = spectrum_matrix(200, 150, exp_spectrum(150, 5.0), 6)
+    g = _sketches(12, 200, 7)
+    sk, _, _, _, _ = rand_adap_lupp(A, 0.5, 12, g)
+    P_, _, _ = sla.lu((g(0) @ A).T)
+    order = (P_.T @ np.arange(150).reshape(-1, 1)).astype(int).reshape(-1)
+    assert np.array_equal(sk[:12], order[:12])
+
+
+def test_rand_adap_lupp_estimate_is_unbiased():
+    """E ||G E||_F^2 = ||E||_F^2 for G ~ N(0, 1/b): the mean squared estimate over
+    many fresh sketches equals the ID's squared error (here within 5%)."""
+    from oracle import rand_adap_lupp
+    A = spectrum_matrix(200, 160, exp_spectrum(160, 10.0), 8)
+    sk, rd, T, k, _ = rand_adap_lupp(A, 0.05, 16, _sketches(16, 200, 9))
+    E = A[:, rd] - A[:, sk] @ T
+    rng = np.random.default_rng(10)
+    vals = [np.sum(((rng.standard_normal((16, 200)) / 4.0) @ E) ** 2) for _ in range(600)]
+    assert np.mean(vals) == pytest.approx(np.sum(E * E), rel=0.05)
