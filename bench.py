@@ -69,7 +69,9 @@ def parse():
                     help="write the generated matrix (column-major .npy) and exit")
     ap.add_argument("--load-matrix", default="",
                     help="read the matrix from this .npy instead of generating it")
-    ap.add_argument("--legs", default="main,comparators,cpqr,mixed,batched,scale,cpu",
+    ap.add_argument("--cpu-reps", type=int, default=3,
+                    help="oracle IDs of the benchmark matrix timed for the CPU baseline (median)")
+    ap.add_argument("--legs", default="main,comparators,cpqr,mixed,batched,scale,ceilings,sampler,cpu",
                     help="comma list of legs to run (main = the timed FP64 IDs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--all-phase-timing", action="store_true",
@@ -97,6 +99,8 @@ def parse():
     args.no_batched = args.no_batched or "batched" not in legs
     args.no_cpu = args.no_cpu or "cpu" not in legs
     args.no_scale = "scale" not in legs
+    args.no_ceilings = "ceilings" not in legs
+    args.no_sampler = "sampler" not in legs
     args.no_main = "main" not in legs
     return args
 
@@ -152,7 +156,7 @@ def mixed_leg(A2, args):
     the 50000 x 4000 exp-decay matrix at tol 1e-6 (the tightest tolerance FP32
     storage resolves on it) and the 100000 x 2000 Gaussian-mixture matrix at 1e-6."""
     import torch
-    from robust_blockwise_random_pivoting_amd import Handle
+    from robust_blockwise_random_pivoting_amd import Handle, rbrp_colid
     from robust_blockwise_random_pivoting_amd.matrices import gmm_rows_torch
     out = {}
     with Handle() as h:
@@ -167,7 +171,17 @@ def mixed_leg(A2, args):
             e64, e32 = id_rel_error(A, r64), id_rel_error(A, r32)
             out[label] = {"fp64_ms": t64 * 1e3, "fp32_ms": t32 * 1e3, "speedup": t64 / t32,
                           "k_fp64": r64.k, "k_fp32": r32.k, "rel_error_fp64": e64,
-                          "rel_error_fp32": e32, "tracked_error_fp32": r32.err_final}
+                          "rel_error_fp32": e32,
+                          "rel_error_diff": abs(e32 - e64) / e64 if e64 > 0 else 0.0,
+                          "reported_error_fp32": r32.err_final}
+            # where the time goes in each form (one more ID each, every phase timed)
+            for prec in ("fp64", "fp32"):
+                h.set_timing(1)
+                h.reset_stats()
+                rbrp_colid(A, b=b, tol=tol, seed=args.seed, precision=prec, handle=h)
+                _sync()
+                out[label][f"phase_ms_{prec}"] = {k: v["ms"] for k, v in h.stats().items()}
+                h.set_timing(0)
             print(f"[bench] mixed {label}: fp64 {t64 * 1e3:.1f} ms k={r64.k} err={e64:.3e} | "
                   f"fp32 {t32 * 1e3:.1f} ms k={r32.k} err={e32:.3e}", file=sys.stderr, flush=True)
             if A is not A2:
@@ -289,34 +303,128 @@ def batched_leg(args):
     return out
 
 
-def cpu_baseline(A_host, args, k_gpu, perm_gpu):
-    """The oracle (NumPy/SciPy LAPACK, multi-threaded BLAS) on the same matrix and stream."""
-    from oracle import rbrp_colid as oracle_colid
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        cores = os.cpu_count() or 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+def ceilings_leg():
+    """Measured ceilings of this part (committed micro-kernels of the test-hook
+    library): a streaming read and copy (nontemporal 16-byte accesses) and loops
+    of independent FP64 / FP32 16x16x4 matrix-core instructions."""
+    import ctypes
+    import torch
+    from robust_blockwise_random_pivoting_amd import _capi
+    L = _capi.testing_lib()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for kind, name, nbytes, unit, peak in ((0, "read", 2 << 30, "GB/s", PEAK_HBM_GBS),
+                                           (1, "copy", 1 << 30, "GB/s", PEAK_HBM_GBS),
+                                           (2, "mfma_f64", 1 << 20, "TFLOP/s", PEAK_FP64_TFLOPS),
+                                           (3, "mfma_f32", 1 << 20, "TFLOP/s", PEAK_FP32_TFLOPS)):
+        ms, rate = ctypes.c_double(), ctypes.c_double()
+        _capi.check("rbrp_test_ceiling", L.rbrp_test_ceiling(st, kind, nbytes, 10, ctypes.byref(ms),
+                                                             ctypes.byref(rate)))
+        out[name] = {"rate": rate.value, "unit": unit, "peak": peak, "frac": rate.value / peak,
+                     "ms": ms.value}
+        print(f"[bench] ceiling {name}: {rate.value:.1f} {unit}", file=sys.stderr, flush=True)
+    return out
+
+
+def sampler_leg(args):
+    """The weighted sampler at a candidate range of 2^26 (the multi-workgroup form):
+    one block of b = 64 draws from 2^26 random weights, device time per call from
+    events over 10 calls.  Bytes moved: the weights once (8 per candidate) plus
+    12 per 4096-candidate tile; the SURVEY's model charges 16 per candidate
+    (weights read and a CDF written: this form never stores the CDF)."""
+    import ctypes
+    import torch
+    from robust_blockwise_random_pivoting_amd import _capi
+    L = _capi.testing_lib()
+    n, b = 1 << 26, 64
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(3)
+    w = torch.rand(n, dtype=torch.float64, device="cuda", generator=gen)
+    _sync()
+    ms = ctypes.c_double()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _capi.check("rbrp_test_sample_time", L.rbrp_test_sample_time(
+        st, ctypes.c_void_p(w.data_ptr()), n, b, args.seed, 10, ctypes.byref(ms)))
+    t = ms.value * 1e-3
+    moved = 8.0 * n + 12.0 * (n // 4096)
+    out = {"n": n, "b": b, "ms": ms.value, "bytes_moved": moved, "GBs": moved / t / 1e9,
+           "hbm_frac": moved / t / 1e9 / PEAK_HBM_GBS, "model_bytes": 16.0 * n,
+           "model_GBs": 16.0 * n / t / 1e9, "model_frac": 16.0 * n / t / 1e9 / PEAK_HBM_GBS}
+    print(f"[bench] sampler n=2^26: {ms.value:.3f} ms, {out['GBs']:.0f} GB/s moved",
+          file=sys.stderr, flush=True)
+    del w
+    torch.cuda.empty_cache()
+    return out
+
+
+def _blas_threads(default):
     try:
         from threadpoolctl import threadpool_info
         info = threadpool_info()
-        threads = max([t.get("num_threads", 1) for t in info if t.get("user_api") == "blas"] or [threads])
+        return max([t.get("num_threads", 1) for t in info if t.get("user_api") == "blas"] or [default])
     except Exception:
-        pass
-    t0 = time.perf_counter()
-    r = oracle_colid(A_host, args.b, args.tol, seed=args.seed)
-    dt = time.perf_counter() - t0
+        return default
+
+
+def cpu_baseline(A_host, args, k_gpu, perm_gpu):
+    """The oracle (NumPy/SciPy LAPACK, multi-threaded BLAS) on the same matrix and
+    stream: the median of args.cpu_reps IDs of the benchmark matrix (the reported
+    rate); plus config 1 (1000 x 1000, s_i = exp(-i/20), b = 40, tol 1e-6) as the
+    median of 10 with every BLAS thread and with one thread."""
+    import statistics
+    from oracle import id_error, rbrp_colid as oracle_colid
+    from robust_blockwise_random_pivoting_amd.matrices import exp_spectrum, spectrum_matrix
+    try:
+        affinity = sorted(os.sched_getaffinity(0))
+    except Exception:
+        affinity = list(range(os.cpu_count() or 1))
+    cores = len(affinity)
+    threads = _blas_threads(int(os.environ.get("OMP_NUM_THREADS", cores)))
+    times, r = [], None
+    for _ in range(max(1, args.cpu_reps)):
+        t0 = time.perf_counter()
+        r = oracle_colid(A_host, args.b, args.tol, seed=args.seed)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
     same = bool(r.k == k_gpu and np.array_equal(r.perm[: r.k], perm_gpu[:k_gpu]))
+    # config 1, all threads and one thread
+    A1 = spectrum_matrix(1000, 1000, exp_spectrum(1000, 20.0), 1)
+    c1 = {}
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:
+        threadpool_limits = None
+    for label, lim in (("all_threads", None), ("one_thread", 1)):
+        ts = []
+        for _ in range(10):
+            if threadpool_limits is not None and lim is not None:
+                with threadpool_limits(limits=lim):
+                    t0 = time.perf_counter()
+                    r1 = oracle_colid(A1, 40, 1e-6, seed=42)
+                    ts.append(time.perf_counter() - t0)
+            else:
+                t0 = time.perf_counter()
+                r1 = oracle_colid(A1, 40, 1e-6, seed=42)
+                ts.append(time.perf_counter() - t0)
+        c1[label] = {"ms_median": statistics.median(ts) * 1e3, "ms_min": min(ts) * 1e3,
+                     "threads": lim if lim else threads}
+    c1.update({"k": int(r1.k), "rel_error": float(id_error(A1, r1.perm, r1.k, r1.X)),
+               "config": "1000x1000, s_i = exp(-i/20), b = 40, tol 1e-6, median of 10"})
     return {
         "value": 1.0 / dt,
         "unit": "IDs/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one full ID of the same {A_host.shape[0]}x{A_host.shape[1]} matrix to tol "
-                  f"{args.tol:g} (b={args.b}), {dt:.2f} s; host cores visible {cores}",
+        "sample": f"median of {len(times)} full IDs of the same {A_host.shape[0]}x{A_host.shape[1]} "
+                  f"matrix to tol {args.tol:g} (b={args.b}), {dt:.2f} s each; BLAS threads "
+                  f"{threads}, host cores visible {cores} (affinity {affinity[0]}..{affinity[-1]})",
         "seconds": dt,
+        "seconds_all": times,
+        "nproc": os.cpu_count(),
+        "affinity": [affinity[0], affinity[-1], cores],
         "k": int(r.k),
         "skeleton_equal_to_gpu": same,
+        "config1": c1,
     }
 
 
@@ -373,7 +481,9 @@ def main():
     if args.no_main:
         out = {"mixed": None if args.no_mixed else mixed_leg(A, args),
                "batched": None if args.no_batched else batched_leg(args),
-               "scale": None if args.no_scale else scale_leg(args)}
+               "scale": None if args.no_scale else scale_leg(args),
+               "ceilings": None if args.no_ceilings else ceilings_leg(),
+               "sampler": None if args.no_sampler else sampler_leg(args)}
         print(json.dumps(out))
         return
 
@@ -428,8 +538,8 @@ def main():
                               frac=ach / PEAK_HBM_GBS, avg_ms=avg_ms, launches=st["launches"],
                               bytes_per_launch=st["bytes"] / st["launches"])
     dom = max(kern, key=lambda k: stats[k]["ms"]) if kern else None
-    # the initial column-norm pass (fused with the working copy: e*m*n read,
-    # e*mp*n written) and the per-block sampler (weights read, CDF written),
+    # the initial column-norm pass (a read of A: e*m*n bytes, 8*n written) and
+    # the per-block sampler (weights read, CDF written: 16 bytes per candidate),
     # timed by the phase events of the breakdown ID
     for name in ("norms", "sample"):
         st = pstats.get(name)
@@ -495,6 +605,8 @@ def main():
     mixed = None if args.no_mixed else mixed_leg(A, args)
     batched = None if args.no_batched else batched_leg(args)
     scale = None if args.no_scale else scale_leg(args)
+    ceilings = None if args.no_ceilings else ceilings_leg()
+    sampler = None if args.no_sampler else sampler_leg(args)
 
     cpu = None
     if not args.no_cpu:
@@ -530,6 +642,8 @@ def main():
         "mixed": mixed,
         "batched": batched,
         "scale": scale,
+        "ceilings": ceilings,
+        "sampler": sampler,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -104,6 +104,13 @@ int rbrp_test_sample(void* stream, const double* diags, int64_t i, int64_t n, in
 int rbrp_test_sample_time(void* stream, const double* diags, int64_t n, int64_t b, uint64_t seed,
                           int reps, double* ms_per_call);
 
+/* Ceilings of the part for the benchmark: kind 0 a streaming read of `bytes`
+ * (rate in GB/s), 1 a streaming copy of `bytes` (GB/s read + written), 2 / 3
+ * loops of independent FP64 / FP32 16x16x4 matrix-core instructions on every
+ * compute unit (TFLOP/s); two untimed launches, then `reps` between events. */
+int rbrp_test_ceiling(void* stream, int kind, int64_t bytes, int reps, double* ms_per_launch,
+                      double* rate);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,7 @@ def _hook_signatures():
         "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
                                      POINTER(c_int32), POINTER(c_uint64)]),
         "rbrp_test_sample_time": (c_int, [vp, vp, i64, i64, u64, c_int, POINTER(c_double)]),
+        "rbrp_test_ceiling": (c_int, [vp, c_int, i64, c_int, POINTER(c_double), POINTER(c_double)]),
     }
 
 

@@ -412,30 +412,27 @@ __device__ inline bool wide_found(const int* fs, int nf, int64_t j) {
   return lo < nf && (int64_t)fs[lo] == j;
 }
 
-// This thread's four inclusive running sums of tile t (clipped weights, the
-// fs[0:nf] sorted found positions zeroed) and its count of positive weights.
-// tl: LDS staging of WIDE_TILE doubles (coalesced loads); wsum: 16 doubles.
-__device__ inline void wide_tile(const double* __restrict__ diags, int64_t i, int64_t N, int64_t t,
-                                 const int* fs, int nf, double* tl, double* wsum, double run[4],
-                                 int* cnt) {
-  const int tid = threadIdx.x;
-  const int64_t base = t * WIDE_TILE;
-  __syncthreads();  // tl may still be read by a previous call
+// This thread's four weights of tile t (elements 4k .. 4k+3 for thread k),
+// clipped at zero, with the fs[0:nf] sorted found positions zeroed.
+__device__ inline void wide_load(const double* __restrict__ diags, int64_t i, int64_t N, int64_t t,
+                                 const int* fs, int nf, double w[4]) {
+  const int64_t j0 = t * WIDE_TILE + 4 * (int64_t)threadIdx.x;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int64_t j = base + tid + 1024 * q;
+    const int64_t j = j0 + q;
     double v = 0.0;
     if (j < N) {
       v = diags[i + j];
       v = (v > 0.0) ? v : 0.0;
       if (nf > 0 && v > 0.0 && wide_found(fs, nf, j)) v = 0.0;
     }
-    tl[tid + 1024 * q] = v;
+    w[q] = v;
   }
-  __syncthreads();
-  const double2 p0 = *(const double2*)&tl[4 * tid];
-  const double2 p1 = *(const double2*)&tl[4 * tid + 2];
-  const double w[4] = {p0.x, p0.y, p1.x, p1.y};
+}
+
+// Inclusive running sums of this thread's four weights (from wide_load) in the
+// tile's fixed order, and its count of positive weights.  wsum: 16 doubles.
+__device__ inline void wide_scan(const double w[4], double* wsum, double run[4], int* cnt) {
   double loc = 0.0;
   int c = 0;
 #pragma unroll
@@ -450,6 +447,14 @@ __device__ inline void wide_tile(const double* __restrict__ diags, int64_t i, in
     run[q] = r;
   }
   *cnt = c;
+}
+
+// wide_load + wide_scan of tile t.
+__device__ inline void wide_tile(const double* __restrict__ diags, int64_t i, int64_t N, int64_t t,
+                                 const int* fs, int nf, double* wsum, double run[4], int* cnt) {
+  double w[4];
+  wide_load(diags, i, N, t, fs, nf, w);
+  wide_scan(w, wsum, run, cnt);
 }
 
 // Scratch of the wide sampler, carved from the sampler's w / cdf arrays.
@@ -475,22 +480,32 @@ __device__ inline int64_t wide_first_col(int64_t i0, const int* i_dyn) {
   return i0 + (i_dyn ? (int64_t)*i_dyn : 0);
 }
 
-// Tile pass: S[t] and the positive-weight count of every tile.
+// Tile pass: S[t] and the positive-weight count of every tile.  Each
+// workgroup takes tiles blockIdx.x, + gridDim.x, ... and loads its next tile
+// while it scans the current one.
 __global__ __launch_bounds__(1024) void k_wide_tiles(const double* __restrict__ diags, int64_t i0,
                                                      const int* i_dyn, int64_t n, WideScratch ws) {
-  __shared__ __align__(16) double tl[WIDE_TILE];
   __shared__ double wsum[16];
   __shared__ int ired[16];
   const int64_t i = wide_first_col(i0, i_dyn);
   const int64_t N = n - i;
-  const int64_t t = blockIdx.x;
-  if (t * WIDE_TILE >= N) return;  // uniform
-  double run[4];
-  int c;
-  wide_tile(diags, i, N, t, nullptr, 0, tl, wsum, run, &c);
-  const int cnt = rbrp_block_sum_int(c, ired);
-  if (threadIdx.x == 1023) ws.S[t] = run[3];
-  if (threadIdx.x == 0) ws.tnnz[t] = cnt;
+  const int64_t ntiles = N > 0 ? (N + WIDE_TILE - 1) / WIDE_TILE : 0;
+  int64_t t = blockIdx.x;
+  if (t >= ntiles) return;  // uniform
+  double w[4], wn[4];
+  wide_load(diags, i, N, t, nullptr, 0, w);
+  for (; t < ntiles; t += gridDim.x) {
+    const int64_t tn = t + gridDim.x;
+    if (tn < ntiles) wide_load(diags, i, N, tn, nullptr, 0, wn);
+    double run[4];
+    int c;
+    wide_scan(w, wsum, run, &c);
+    const int cnt = rbrp_block_sum_int(c, ired);
+    if (threadIdx.x == 1023) ws.S[t] = run[3];
+    if (threadIdx.x == 0) ws.tnnz[t] = cnt;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = wn[q];
+  }
 }
 
 // P[t] (exclusive tile prefixes) and the total from S: thread k owns a run of
@@ -531,10 +546,10 @@ __device__ inline int64_t wide_pick_tile(const WideScratch& ws, int64_t ntiles, 
 // call; every thread returns it).
 __device__ inline int64_t wide_find(const double* __restrict__ diags, int64_t i, int64_t N,
                                     const WideScratch& ws, int64_t t, double total, double u,
-                                    const int* fs, int nf, double* tl, double* wsum, int* imin) {
+                                    const int* fs, int nf, double* wsum, int* imin) {
   double run[4];
   int c;
-  wide_tile(diags, i, N, t, fs, nf, tl, wsum, run, &c);
+  wide_tile(diags, i, N, t, fs, nf, wsum, run, &c);
   const double Pt = ws.P[t];
   int best = INT_MAX;
 #pragma unroll
@@ -616,14 +631,13 @@ __global__ __launch_bounds__(1024) void k_wide_pick(int64_t i0, const int* i_dyn
 // Round 1's in-tile searches: workgroup d resolves draw d.
 __global__ __launch_bounds__(1024) void k_wide_resolve(const double* __restrict__ diags,
                                                        WideScratch ws) {
-  __shared__ __align__(16) double tl[WIDE_TILE];
   __shared__ double wsum[16];
   __shared__ int imin[16];
   const WideHdr* h = ws.hdr;
   const int d = blockIdx.x;
   if (h->done || d >= h->r) return;  // uniform
   const int64_t j = wide_find(diags, h->i, h->N, ws, ws.tilesel[d], h->total, ws.u[d], nullptr, 0,
-                              tl, wsum, imin);
+                              wsum, imin);
   if (threadIdx.x == 0) ws.newidx[d] = (int)j;
 }
 
@@ -660,7 +674,6 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
                                                       uint64_t c1, uint64_t c2, uint64_t c3,
                                                       WideScratch ws, RbrpBlockState* bs,
                                                       int* __restrict__ J) {
-  __shared__ __align__(16) double tl[WIDE_TILE];
   __shared__ double wsum[16];
   __shared__ int imin[16];
   __shared__ double bcast;
@@ -707,7 +720,7 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
       const int64_t tt = tilesel[q];
       double run[4];
       int c;
-      wide_tile(diags, i, N, tt, fs, n_f, tl, wsum, run, &c);
+      wide_tile(diags, i, N, tt, fs, n_f, wsum, run, &c);
       if (tid == 1023) ws.S[tt] = run[3];
     }
     __threadfence();
@@ -721,7 +734,7 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
     }
     __syncthreads();
     for (int d = 0; d < r; ++d) {
-      const int64_t j = wide_find(diags, i, N, ws, newidx[d], total, uu[d], fs, n_f, tl, wsum, imin);
+      const int64_t j = wide_find(diags, i, N, ws, newidx[d], total, uu[d], fs, n_f, wsum, imin);
       __syncthreads();
       if (tid == 0) newidx[d] = (int)j;
     }
@@ -753,8 +766,14 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
     // (with i_dyn the range is at most n - i - 1: the previous block kept a column)
     const int64_t ntiles_max = (n - i + WIDE_TILE - 1) / WIDE_TILE;
     const WideScratch ws = wide_scratch(w, cdf, ntiles_max);
-    hipLaunchKernelGGL(k_wide_tiles, dim3((unsigned)ntiles_max), dim3(1024), 0, st, diags, i, i_dyn, n,
-                       ws);
+    // at most two passes of resident workgroups (two per compute unit), each
+    // looping over tiles
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    const int64_t tgrid = std::min<int64_t>(ntiles_max, 4 * (int64_t)cus);
+    hipLaunchKernelGGL(k_wide_tiles, dim3((unsigned)tgrid), dim3(1024), 0, st, diags, i, i_dyn, n, ws);
     hipLaunchKernelGGL(k_wide_pick, dim3(1), dim3(1024), 0, st, i, i_dyn, i_dyn ? 1 : 0, tol2, n, kmax,
                        b, key0, key1, ctr[0], ctr[1], ctr[2], ctr[3], ws, bs);
     hipLaunchKernelGGL(k_wide_resolve, dim3((unsigned)b), dim3(1024), 0, st, diags, ws);
