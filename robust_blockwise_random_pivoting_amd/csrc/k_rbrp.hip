@@ -229,12 +229,73 @@ __device__ inline double block_exclusive_scan(double v, double* wsum) {
   __syncthreads();
   if (lane == 63) wsum[wid] = x;
   __syncthreads();
+  // off = wsum[0] + ... + wsum[wid - 1] in order (the +0.0 of the later
+  // waves leaves it unchanged; the 16 reads issue together)
   double off = 0.0;
-  for (int q = 0; q < wid; ++q) off += wsum[q];
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const double y = wsum[q < nw ? q : 0];
+    off += (q < wid) ? y : 0.0;
+  }
   return off + ex;
 }
 
 #define SAMPLE_MAXB 256
+
+// First occurrences of a round's picks.  Appends to found[nf0:] the values of
+// newidx[0:r] that are not in found[0:nf0], each once, in order of first
+// occurrence, and returns the new count to every thread (nf0 + r <=
+// SAMPLE_MAXB; 1024-thread workgroup).  An LDS open-addressing table maps
+// every value to the smallest draw holding it (-1 for the found ones); draw d
+// is new when it owns its value.  The positions follow from a ballot per wave
+// and the wave counts.
+#define COLLECT_SLOTS 512
+struct CollectLds {
+  int key[COLLECT_SLOTS];
+  int own[COLLECT_SLOTS];
+  int wcnt[16];
+};
+
+__device__ inline int collect_slot(CollectLds& c, int v, int d) {
+  unsigned h = ((unsigned)v * 2654435761u) >> 23;  // 9 bits
+  // at most SAMPLE_MAXB distinct values in COLLECT_SLOTS slots: a free or
+  // matching slot is always found
+  while (true) {
+    const int prev = atomicCAS(&c.key[h], -1, v);
+    if (prev == -1 || prev == v) {
+      atomicMin(&c.own[h], d);
+      return (int)h;
+    }
+    h = (h + 1) & (COLLECT_SLOTS - 1);
+  }
+}
+
+__device__ inline int collect_first(const int* newidx, int r, int* found, int nf0, CollectLds& c) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int q = tid; q < COLLECT_SLOTS; q += 1024) {
+    c.key[q] = -1;
+    c.own[q] = INT_MAX;
+  }
+  __syncthreads();
+  if (tid < nf0) (void)collect_slot(c, found[tid], -1);
+  int slot = 0;
+  if (tid < r) slot = collect_slot(c, newidx[tid], tid);
+  __syncthreads();
+  const bool isnew = tid < r && c.own[slot] == tid;
+  const unsigned long long m = __ballot(isnew);
+  const int below = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) c.wcnt[wid] = __popcll(m);
+  __syncthreads();
+  int off = nf0, tot = nf0;
+  for (int q = 0; q < 16; ++q) {
+    off += (q < wid) ? c.wcnt[q] : 0;
+    tot += c.wcnt[q];
+  }
+  if (isnew) found[off + below] = newidx[tid];
+  __syncthreads();
+  return tot;
+}
 
 // Weights and CDF live in LDS when the candidate range fits (N <= SAMPLE_LDS_N),
 // otherwise in the global scratch arrays w / cdf.
@@ -257,10 +318,9 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
   extern __shared__ double lds[];
   __shared__ int found[SAMPLE_MAXB];
   __shared__ int newidx[SAMPLE_MAXB];
-  __shared__ int isnew[SAMPLE_MAXB];
+  __shared__ CollectLds cl;
   __shared__ int ired[16];
   __shared__ double wsum[16];
-  __shared__ int s_nfound;
   __shared__ int s_done;
   const int tid = threadIdx.x;
   const int64_t i = i0 + (i_dyn ? (int64_t)*i_dyn : 0);
@@ -297,8 +357,7 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
     }
     return;
   }
-  if (tid == 0) s_nfound = 0;
-  __syncthreads();
+  int nfound = 0;
   const uint64_t base[4] = {c0, c1, c2, c3};
   const int64_t chunk = (N + 1023) / 1024;
   const int64_t j0 = min(N, (int64_t)tid * chunk);
@@ -306,7 +365,7 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
   // every round adds at least one new index (the first draw of a round can
   // only land on a positive, not yet found weight), so s rounds suffice
   for (int round = 0; round <= s; ++round) {
-    const int nf = s_nfound;
+    const int nf = nfound;
     if (nf >= s) break;
     const int r = s - nf;
     for (int t = tid; t < nf; t += 1024) w[found[t]] = 0.0;
@@ -335,28 +394,10 @@ __global__ __launch_bounds__(1024) void k_sample(const double* __restrict__ diag
     }
     __syncthreads();
     // keep the first occurrence of each index not found before
-    if (tid < r) {
-      const int v = newidx[tid];
-      int dup = 0;
-      for (int q = 0; q < nf; ++q) dup |= (found[q] == v) ? 1 : 0;
-      for (int q = 0; q < tid; ++q) dup |= (newidx[q] == v) ? 1 : 0;
-      isnew[tid] = !dup;
-    }
-    __syncthreads();
-    if (tid < r && isnew[tid]) {
-      int pos = nf;
-      for (int q = 0; q < tid; ++q) pos += isnew[q];
-      found[pos] = newidx[tid];
-    }
-    if (tid == 0) {
-      int add = 0;
-      for (int q = 0; q < r; ++q) add += isnew[q];
-      s_nfound = nf + add;
-    }
+    nfound = collect_first(newidx, r, found, nf, cl);
     draws += (unsigned long long)r;
-    __syncthreads();
   }
-  if (s_nfound < s) {
+  if (nfound < s) {
     if (tid == 0) {
       bs->s = 0;
       bs->status = 4;
@@ -413,20 +454,21 @@ __device__ inline bool wide_found(const int* fs, int nf, int64_t j) {
 }
 
 // This thread's four weights of tile t (elements 4k .. 4k+3 for thread k),
-// clipped at zero, with the fs[0:nf] sorted found positions zeroed.
+// clipped at zero, with the fs[0:nf] sorted found positions zeroed (N >= 1).
 __device__ inline void wide_load(const double* __restrict__ diags, int64_t i, int64_t N, int64_t t,
                                  const int* fs, int nf, double w[4]) {
+  // (the loads are unconditional, at positions clamped into [0, N), so that
+  // they issue back to back; out-of-range positions get weight zero after)
   const int64_t j0 = t * WIDE_TILE + 4 * (int64_t)threadIdx.x;
+  double v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = diags[i + min(j0 + q, N - 1)];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int64_t j = j0 + q;
-    double v = 0.0;
-    if (j < N) {
-      v = diags[i + j];
-      v = (v > 0.0) ? v : 0.0;
-      if (nf > 0 && v > 0.0 && wide_found(fs, nf, j)) v = 0.0;
-    }
-    w[q] = v;
+    double x = (j < N && v[q] > 0.0) ? v[q] : 0.0;
+    if (nf > 0 && x > 0.0 && wide_found(fs, nf, j)) x = 0.0;
+    w[q] = x;
   }
 }
 
@@ -459,7 +501,7 @@ __device__ inline void wide_tile(const double* __restrict__ diags, int64_t i, in
 
 // Scratch of the wide sampler, carved from the sampler's w / cdf arrays.
 struct WideScratch {
-  double *S, *P, *u;
+  double *S, *P, *u, *tp;  // tp[d]: P of draw d's tile
   int *tnnz, *tilesel, *newidx;
   WideHdr* hdr;
 };
@@ -470,7 +512,8 @@ __host__ __device__ inline WideScratch wide_scratch(double* gw, double* gcdf, in
   w.P = gw + ntiles_max;
   w.hdr = (WideHdr*)gcdf;
   w.u = gcdf + 8;
-  w.tilesel = (int*)(gcdf + 8 + SAMPLE_MAXB);
+  w.tp = gcdf + 8 + SAMPLE_MAXB;
+  w.tilesel = (int*)(gcdf + 8 + 2 * SAMPLE_MAXB);
   w.newidx = w.tilesel + SAMPLE_MAXB;
   w.tnnz = w.newidx + SAMPLE_MAXB;
   return w;
@@ -480,9 +523,13 @@ __device__ inline int64_t wide_first_col(int64_t i0, const int* i_dyn) {
   return i0 + (i_dyn ? (int64_t)*i_dyn : 0);
 }
 
-// Tile pass: S[t] and the positive-weight count of every tile.  Each
-// workgroup takes tiles blockIdx.x, + gridDim.x, ... and loads its next tile
-// while it scans the current one.
+// Tile pass: S[t] and the positive-weight count of every tile, the
+// workgroups striding over the tiles.  One barrier pair per tile: the wave
+// totals and the wave counts (ballots) go to LDS together.  S[t] follows
+// wide_scan's arithmetic for the tile's last element (the sum of waves 0 ..
+// 14, then the last wave's exclusive prefix, then the thread's four weights
+// in turn).  wide_load's loads are plain ones: the nontemporal form of these
+// 8-byte loads ran at two thirds of the rate.
 __global__ __launch_bounds__(1024) void k_wide_tiles(const double* __restrict__ diags, int64_t i0,
                                                      const int* i_dyn, int64_t n, WideScratch ws) {
   __shared__ double wsum[16];
@@ -490,51 +537,197 @@ __global__ __launch_bounds__(1024) void k_wide_tiles(const double* __restrict__ 
   const int64_t i = wide_first_col(i0, i_dyn);
   const int64_t N = n - i;
   const int64_t ntiles = N > 0 ? (N + WIDE_TILE - 1) / WIDE_TILE : 0;
-  int64_t t = blockIdx.x;
-  if (t >= ntiles) return;  // uniform
-  double w[4], wn[4];
-  wide_load(diags, i, N, t, nullptr, 0, w);
-  for (; t < ntiles; t += gridDim.x) {
-    const int64_t tn = t + gridDim.x;
-    if (tn < ntiles) wide_load(diags, i, N, tn, nullptr, 0, wn);
-    double run[4];
-    int c;
-    wide_scan(w, wsum, run, &c);
-    const int cnt = rbrp_block_sum_int(c, ired);
-    if (threadIdx.x == 1023) ws.S[t] = run[3];
-    if (threadIdx.x == 0) ws.tnnz[t] = cnt;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // full tiles of a 16-byte aligned range load as two 16-byte units a thread
+  const bool vec = (((uintptr_t)(diags + i)) & 15) == 0;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    double w[4];
+    if (vec && t + 1 < ntiles) {
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      const v2d* p = (const v2d*)(diags + i + t * WIDE_TILE + 4 * (int64_t)tid);
+      const v2d a = p[0], b = p[1];
+      w[0] = a.x > 0.0 ? a.x : 0.0;
+      w[1] = a.y > 0.0 ? a.y : 0.0;
+      w[2] = b.x > 0.0 ? b.x : 0.0;
+      w[3] = b.y > 0.0 ? b.y : 0.0;
+    } else {
+      wide_load(diags, i, N, t, nullptr, 0, w);
+    }
+    double x = 0.0;
+    int c = 0;  // the wave's count of positive weights
 #pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = wn[q];
+    for (int q = 0; q < 4; ++q) {
+      x += w[q];
+      c += __popcll(__ballot(w[q] > 0.0));
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    __syncthreads();
+    if (lane == 63) wsum[wid] = x;
+    if (lane == 0) ired[wid] = c;
+    __syncthreads();
+    if (tid == 1023) {
+      double r = 0.0;
+      for (int q = 0; q < 15; ++q) r += wsum[q];
+      r += ex;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) r += w[q];
+      ws.S[t] = r;
+    } else if (tid == 0) {
+      int cnt = 0;
+      for (int q = 0; q < 16; ++q) cnt += ired[q];
+      ws.tnnz[t] = cnt;
+    }
   }
 }
 
-// P[t] (exclusive tile prefixes) and the total from S: thread k owns a run of
-// consecutive tiles; returns the total to every thread.
-__device__ inline double wide_prefix(const WideScratch& ws, int64_t ntiles, double* wsum,
-                                     double* bcast) {
+// LDS of the pick and finish kernels when the launch reserves it (E !=
+// nullptr): the tile end values P[t] + S[t], thread k's run of per tiles at
+// E[k (per + 1) + q], with the run's starting prefix in the run's extra slot
+// E[k (per + 1) + per] (the odd stride also keeps the per-thread walks free
+// of bank conflicts).
+struct WideLds {
+  double* E;
+  int per;       // tiles per run, ceil(ntiles / 1024)
+  uint64_t inv;  // ceil(2^32 / per): t / per = (t inv) >> 32 for t < 2^15
+};
+
+__host__ __device__ inline int64_t wide_lds_doubles(int64_t ntiles) {
+  return 1024 * ((ntiles + 1023) / 1024 + 1);
+}
+
+// The LDS path holds per <= WIDE_MAX_PER (ntiles <= WIDE_LDS_MAX_TILES <
+// 2^15), where (t ceil(2^32 / per)) >> 32 = floor(t / per) exactly (the
+// rounding error t (inv - 2^32 / per) / 2^32 < 2^-17 stays below 1 / per).
+#define WIDE_MAX_PER 20
+#define WIDE_LDS_MAX_TILES (WIDE_MAX_PER * 1024)
+
+// LDS slot of tile t.
+__device__ inline int wide_slot(const WideLds& L, int t) {
+  const int k = (int)(((uint64_t)t * L.inv) >> 32);
+  return k * (L.per + 1) + (t - k * L.per);
+}
+
+// Loads S[t] (and, with cnt, adds tnnz[t]) for this thread's tiles t = tid +
+// 1024 r into their LDS slots: coalesced loads, 8 tiles in flight.
+__device__ inline void wide_stage(const WideScratch& ws, int64_t ntiles, const WideLds& L, int* cnt) {
+  const int tid = threadIdx.x;
+  for (int t = tid; t < (int)ntiles; t += 8 * 1024) {
+    // (loads at clamped positions, so that all issue back to back)
+    double v[8];
+    int c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int tq = min(t + 1024 * q, (int)ntiles - 1);
+      v[q] = ws.S[tq];
+      c[q] = cnt ? ws.tnnz[tq] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (t + 1024 * q < (int)ntiles) {
+        L.E[wide_slot(L, t + 1024 * q)] = v[q];
+        if (cnt) *cnt += c[q];
+      }
+    }
+  }
+}
+
+// P[t] (exclusive tile prefixes) and the total from S.  Thread k owns the run
+// of tiles [k per, (k + 1) per) (per = ceil(ntiles / 1024)): it sums the run
+// in order, takes the workgroup's exclusive scan of the run sums, and walks
+// the run again adding S[t] to that prefix (P[t] is the value before the
+// addition, P[t] + S[t] the value after).  With L.E the walks run in LDS (S
+// staged there by wide_stage, here unless `staged`), and the end values and
+// the run prefixes stay there for wide_pick_tile and wide_P; without it they
+// read S from global memory and write P there.  The arithmetic is the same
+// either way.  Every thread returns the total.
+__device__ inline double wide_prefix(const WideScratch& ws, int64_t ntiles, const WideLds& L,
+                                     bool staged, double* wsum, double* bcast) {
   const int tid = threadIdx.x;
   const int64_t per = (ntiles + 1023) / 1024;
   const int64_t t0 = min(ntiles, (int64_t)tid * per), t1 = min(ntiles, t0 + per);
-  double loc = 0.0;
-  for (int64_t t = t0; t < t1; ++t) loc += ws.S[t];
-  double run = block_exclusive_scan(loc, wsum);
-  for (int64_t t = t0; t < t1; ++t) {
-    ws.P[t] = run;
-    run += ws.S[t];
+  if (L.E) {
+    const int ld = L.per + 1;
+    if (!staged) {
+      wide_stage(ws, ntiles, L, nullptr);
+      __syncthreads();
+    }
+    // (the run read from LDS at once; P stays in LDS, read by wide_P)
+    double* e = L.E + tid * ld;
+    const int cnt = (int)(t1 - t0);
+    // (slot min(q, per) stays inside the thread's run; the values past cnt
+    // become +0.0, which leaves the sums unchanged)
+    double v[WIDE_MAX_PER];
+#pragma unroll
+    for (int q = 0; q < WIDE_MAX_PER; ++q) v[q] = e[q < L.per ? q : L.per];
+#pragma unroll
+    for (int q = 0; q < WIDE_MAX_PER; ++q) v[q] = (q < cnt) ? v[q] : 0.0;
+    double loc = 0.0;
+#pragma unroll
+    for (int q = 0; q < WIDE_MAX_PER; ++q) loc += v[q];
+    double run = block_exclusive_scan(loc, wsum);
+    e[L.per] = run;
+#pragma unroll
+    for (int q = 0; q < WIDE_MAX_PER; ++q) {
+      if (q < cnt) {
+        run += v[q];
+        e[q] = run;
+      }
+    }
+    if (cnt > 0 && t1 == ntiles) *bcast = run;  // P[last] + S[last]
+    (void)ws;
+  } else {
+    // (eight loads issued together per step; adding the +0.0 of a missing
+    // tile leaves every sum unchanged, so the order of the additions is the
+    // plain left-to-right one)
+    double loc = 0.0;
+    for (int64_t t = t0; t < t1; t += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (t + q < t1) ? ws.S[t + q] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) loc += v[q];
+    }
+    double run = block_exclusive_scan(loc, wsum);
+    for (int64_t t = t0; t < t1; t += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (t + q < t1) ? ws.S[t + q] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (t + q < t1) ws.P[t + q] = run;
+        run += v[q];
+      }
+    }
+    if (t0 < t1 && t1 == ntiles) *bcast = run;  // P[last] + S[last]
   }
-  if (t0 < t1 && t1 == ntiles) *bcast = run;  // P[last] + S[last]
-  __threadfence();
+  // (P is read back within this workgroup only, or by later kernels: the
+  // barrier's workgroup-scope ordering suffices)
   __syncthreads();
   return *bcast;
 }
 
-// First tile t with (P[t] + S[t]) / total > u.
-__device__ inline int64_t wide_pick_tile(const WideScratch& ws, int64_t ntiles, double total,
-                                         double u) {
+// P[t] after wide_prefix (from LDS when it left the values there).
+__device__ inline double wide_P(const WideScratch& ws, const WideLds& L, int64_t t) {
+  if (!L.E) return ws.P[t];
+  const int k = (int)(((uint64_t)t * L.inv) >> 32), q = (int)t - k * L.per;
+  return L.E[k * (L.per + 1) + (q ? q - 1 : L.per)];
+}
+
+// First tile t with (P[t] + S[t]) / total > u (the end values from LDS when
+// wide_prefix left them there).
+__device__ inline int64_t wide_pick_tile(const WideScratch& ws, const WideLds& L, int64_t ntiles,
+                                         double total, double u) {
   int64_t lo = 0, hi = ntiles;
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
-    if ((ws.P[mid] + ws.S[mid]) / total > u)
+    const double e = L.E ? L.E[wide_slot(L, (int)mid)] : ws.P[mid] + ws.S[mid];
+    if (e / total > u)
       hi = mid;
     else
       lo = mid + 1;
@@ -545,12 +738,11 @@ __device__ inline int64_t wide_pick_tile(const WideScratch& ws, int64_t ntiles, 
 // First position j of tile t with (P[t] + run(j)) / total > u (all threads
 // call; every thread returns it).
 __device__ inline int64_t wide_find(const double* __restrict__ diags, int64_t i, int64_t N,
-                                    const WideScratch& ws, int64_t t, double total, double u,
-                                    const int* fs, int nf, double* wsum, int* imin) {
+                                    double Pt, int64_t t, double total, double u, const int* fs,
+                                    int nf, double* wsum, int* imin) {
   double run[4];
   int c;
   wide_tile(diags, i, N, t, fs, nf, wsum, run, &c);
-  const double Pt = ws.P[t];
   int best = INT_MAX;
 #pragma unroll
   for (int q = 3; q >= 0; --q) {
@@ -568,12 +760,22 @@ __device__ inline int64_t wide_find(const double* __restrict__ diags, int64_t i,
   return t * WIDE_TILE + m;
 }
 
-// Round 1: s, the tile prefixes, the total, the round's uniforms and their tiles.
+__device__ inline WideLds wide_lds(double* lds, int lds_ok, int64_t ntiles) {
+  WideLds L;
+  L.E = (lds_ok && ntiles <= WIDE_LDS_MAX_TILES) ? lds : nullptr;  // (per <= WIDE_MAX_PER)
+  L.per = (int)((ntiles + 1023) / 1024);
+  L.inv = L.E ? (0x100000000ull + (uint64_t)L.per - 1) / (uint64_t)L.per : 0ull;
+  return L;
+}
+
+// Round 1: s, the tile prefixes, the total, the round's uniforms and their
+// tiles.  lds_ok: the launch reserved wide_lds_doubles(ntiles) doubles of LDS.
 __global__ __launch_bounds__(1024) void k_wide_pick(int64_t i0, const int* i_dyn, int check_prev,
                                                     double tol2, int64_t n, int64_t kmax, int b,
                                                     uint64_t key0, uint64_t key1, uint64_t c0,
                                                     uint64_t c1, uint64_t c2, uint64_t c3,
-                                                    WideScratch ws, RbrpBlockState* bs) {
+                                                    int lds_ok, WideScratch ws, RbrpBlockState* bs) {
+  extern __shared__ double wide_dyn[];
   __shared__ double wsum[16];
   __shared__ int ired[16];
   __shared__ double bcast;
@@ -591,8 +793,19 @@ __global__ __launch_bounds__(1024) void k_wide_pick(int64_t i0, const int* i_dyn
   if (s_done) return;
   const int64_t N = n - i;
   const int64_t ntiles = N > 0 ? (N + WIDE_TILE - 1) / WIDE_TILE : 0;
+  const WideLds L = wide_lds(wide_dyn, lds_ok, ntiles);
   int c = 0;
-  for (int64_t t = tid; t < ntiles; t += 1024) c += ws.tnnz[t];
+  if (L.E) {
+    wide_stage(ws, ntiles, L, &c);  // (the barrier in the count's sum orders the LDS writes)
+  } else {
+    for (int64_t t = tid; t < ntiles; t += 8 * 1024) {
+      int v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = (t + 1024 * q < ntiles) ? ws.tnnz[t + 1024 * q] : 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) c += v[q];
+    }
+  }
   const int nnz = rbrp_block_sum_int(c, ired);
   int64_t s64 = (int64_t)b;
   if (kmax - i < s64) s64 = kmax - i;
@@ -606,13 +819,15 @@ __global__ __launch_bounds__(1024) void k_wide_pick(int64_t i0, const int* i_dyn
     }
     return;
   }
-  const double total = wide_prefix(ws, ntiles, wsum, &bcast);
+  const double total = wide_prefix(ws, ntiles, L, true, wsum, &bcast);
   const unsigned long long draws = bs->draws;
   const uint64_t base[4] = {c0, c1, c2, c3};
   if (tid < s) {
     const double u = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draws + tid));
+    const int64_t t = wide_pick_tile(ws, L, ntiles, total, u);
     ws.u[tid] = u;
-    ws.tilesel[tid] = (int)wide_pick_tile(ws, ntiles, total, u);
+    ws.tilesel[tid] = (int)t;
+    ws.tp[tid] = wide_P(ws, L, t);
   }
   if (tid == 0) {
     WideHdr* h = ws.hdr;
@@ -636,51 +851,27 @@ __global__ __launch_bounds__(1024) void k_wide_resolve(const double* __restrict_
   const WideHdr* h = ws.hdr;
   const int d = blockIdx.x;
   if (h->done || d >= h->r) return;  // uniform
-  const int64_t j = wide_find(diags, h->i, h->N, ws, ws.tilesel[d], h->total, ws.u[d], nullptr, 0,
-                              wsum, imin);
+  const int64_t j = wide_find(diags, h->i, h->N, ws.tp[d], ws.tilesel[d], h->total, ws.u[d], nullptr,
+                              0, wsum, imin);
   if (threadIdx.x == 0) ws.newidx[d] = (int)j;
 }
 
-// First occurrences of the round's picks appended to found[0:*nf] (thread 0 of
-// the workgroup updates *nf; call with all threads).
-__device__ inline void wide_collect(const int* newidx, int r, int* found, int* nf, int* isnew) {
-  const int tid = threadIdx.x;
-  const int nf0 = *nf;
-  if (tid < r) {
-    const int v = newidx[tid];
-    int dup = 0;
-    for (int q = 0; q < nf0; ++q) dup |= (found[q] == v) ? 1 : 0;
-    for (int q = 0; q < tid; ++q) dup |= (newidx[q] == v) ? 1 : 0;
-    isnew[tid] = !dup;
-  }
-  __syncthreads();
-  if (tid < r && isnew[tid]) {
-    int pos = nf0;
-    for (int q = 0; q < tid; ++q) pos += isnew[q];
-    found[pos] = newidx[tid];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int add = 0;
-    for (int q = 0; q < r; ++q) add += isnew[q];
-    *nf = nf0 + add;
-  }
-  __syncthreads();
-}
-
-// Keeps round 1's first occurrences, runs the later rounds, writes J and the state.
+// Keeps round 1's first occurrences, runs the later rounds, writes J and the
+// state (lds_ok as for k_wide_pick).
 __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__ diags, int64_t kmax,
                                                       uint64_t key0, uint64_t key1, uint64_t c0,
                                                       uint64_t c1, uint64_t c2, uint64_t c3,
-                                                      WideScratch ws, RbrpBlockState* bs,
+                                                      int lds_ok, WideScratch ws, RbrpBlockState* bs,
                                                       int* __restrict__ J) {
+  extern __shared__ double wide_dyn[];
   __shared__ double wsum[16];
   __shared__ int imin[16];
   __shared__ double bcast;
-  __shared__ int found[SAMPLE_MAXB], fs[SAMPLE_MAXB], newidx[SAMPLE_MAXB], isnew[SAMPLE_MAXB];
+  __shared__ int found[SAMPLE_MAXB], fs[SAMPLE_MAXB], newidx[SAMPLE_MAXB];
   __shared__ int tilesel[SAMPLE_MAXB];
-  __shared__ double uu[SAMPLE_MAXB];
-  __shared__ int nf, ntl;
+  __shared__ double uu[SAMPLE_MAXB], ptt[SAMPLE_MAXB];
+  __shared__ CollectLds cl;
+  __shared__ int ntl;
   const WideHdr* h = ws.hdr;
   if (h->done) return;  // uniform (the state already says s = 0)
   const int tid = threadIdx.x;
@@ -688,12 +879,12 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
   const int s = h->s;
   double total = h->total;
   unsigned long long draws = h->draws;
-  if (tid == 0) nf = 0;
   if (tid < h->r) newidx[tid] = ws.newidx[tid];
   __syncthreads();
-  wide_collect(newidx, h->r, found, &nf, isnew);
+  int nf = collect_first(newidx, h->r, found, 0, cl);
   draws += (unsigned long long)h->r;
   const uint64_t base[4] = {c0, c1, c2, c3};
+  const WideLds L = wide_lds(wide_dyn, lds_ok, ntiles);
   // every round adds at least one new index, so s rounds suffice
   for (int round = 1; round <= s && nf < s; ++round) {
     const int n_f = nf;
@@ -723,23 +914,24 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
       wide_tile(diags, i, N, tt, fs, n_f, wsum, run, &c);
       if (tid == 1023) ws.S[tt] = run[3];
     }
-    __threadfence();
-    __syncthreads();
-    total = wide_prefix(ws, ntiles, wsum, &bcast);
+    __syncthreads();  // (S read back by this workgroup only)
+    total = wide_prefix(ws, ntiles, L, false, wsum, &bcast);
     const int r = s - n_f;
     if (tid < r) {
       const double u = rbrp_u64_to_unit(rbrp_philox_word(base, key0, key1, draws + tid));
+      const int64_t t = wide_pick_tile(ws, L, ntiles, total, u);
       uu[tid] = u;
-      newidx[tid] = (int)wide_pick_tile(ws, ntiles, total, u);  // tile, resolved below
+      newidx[tid] = (int)t;  // tile, resolved below
+      ptt[tid] = wide_P(ws, L, t);
     }
     __syncthreads();
     for (int d = 0; d < r; ++d) {
-      const int64_t j = wide_find(diags, i, N, ws, newidx[d], total, uu[d], fs, n_f, wsum, imin);
+      const int64_t j = wide_find(diags, i, N, ptt[d], newidx[d], total, uu[d], fs, n_f, wsum, imin);
       __syncthreads();
       if (tid == 0) newidx[d] = (int)j;
     }
     __syncthreads();
-    wide_collect(newidx, r, found, &nf, isnew);
+    nf = collect_first(newidx, r, found, n_f, cl);
     draws += (unsigned long long)r;
   }
   if (nf < s) {
@@ -757,6 +949,32 @@ __global__ __launch_bounds__(1024) void k_wide_finish(const double* __restrict__
   }
 }
 
+// Bytes of LDS that k_wide_pick and k_wide_finish reserve for ntiles_max
+// tiles on the current device: wide_lds_doubles(ntiles_max) doubles when that
+// plus each kernel's static LDS fits the device's per-workgroup limit (and
+// the allowance is granted), else 0 (the global-memory walks).
+static size_t wide_lds_bytes(int64_t ntiles_max) {
+  if (ntiles_max > WIDE_LDS_MAX_TILES) return 0;
+  const size_t want = (size_t)wide_lds_doubles(ntiles_max) * sizeof(double);
+  int dev = 0, maxlds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return 0;
+  hipFuncAttributes fp, ff;
+  if (hipFuncGetAttributes(&fp, (const void*)k_wide_pick) != hipSuccess ||
+      hipFuncGetAttributes(&ff, (const void*)k_wide_finish) != hipSuccess)
+    return 0;
+  const size_t stat = std::max(fp.sharedSizeBytes, ff.sharedSizeBytes);
+  if (want + stat > (size_t)maxlds) return 0;
+  static std::atomic<int> set_pick[RBRP_MAX_DEVICES], set_finish[RBRP_MAX_DEVICES];
+  if (rbrp_allow_lds((const void*)k_wide_pick, (int)want, set_pick) != hipSuccess ||
+      rbrp_allow_lds((const void*)k_wide_finish, (int)want, set_finish) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return want;
+}
+
 hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, int64_t n,
                               int64_t kmax, int b, uint64_t key0, uint64_t key1,
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
@@ -766,19 +984,20 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
     // (with i_dyn the range is at most n - i - 1: the previous block kept a column)
     const int64_t ntiles_max = (n - i + WIDE_TILE - 1) / WIDE_TILE;
     const WideScratch ws = wide_scratch(w, cdf, ntiles_max);
-    // at most two passes of resident workgroups (two per compute unit), each
-    // looping over tiles
+    // up to four workgroups per compute unit, each looping over tiles
     int cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
     const int64_t tgrid = std::min<int64_t>(ntiles_max, 4 * (int64_t)cus);
+    const size_t wl = wide_lds_bytes(ntiles_max);
+    const int lds_ok = wl > 0 ? 1 : 0;
     hipLaunchKernelGGL(k_wide_tiles, dim3((unsigned)tgrid), dim3(1024), 0, st, diags, i, i_dyn, n, ws);
-    hipLaunchKernelGGL(k_wide_pick, dim3(1), dim3(1024), 0, st, i, i_dyn, i_dyn ? 1 : 0, tol2, n, kmax,
-                       b, key0, key1, ctr[0], ctr[1], ctr[2], ctr[3], ws, bs);
+    hipLaunchKernelGGL(k_wide_pick, dim3(1), dim3(1024), wl, st, i, i_dyn, i_dyn ? 1 : 0, tol2, n,
+                       kmax, b, key0, key1, ctr[0], ctr[1], ctr[2], ctr[3], lds_ok, ws, bs);
     hipLaunchKernelGGL(k_wide_resolve, dim3((unsigned)b), dim3(1024), 0, st, diags, ws);
-    hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(1024), 0, st, diags, kmax, key0, key1, ctr[0],
-                       ctr[1], ctr[2], ctr[3], ws, bs, J);
+    hipLaunchKernelGGL(k_wide_finish, dim3(1), dim3(1024), wl, st, diags, kmax, key0, key1, ctr[0],
+                       ctr[1], ctr[2], ctr[3], lds_ok, ws, bs, J);
     return hipGetLastError();
   }
   // with i_dyn the first column is at least i + 1 (the previous block kept a
