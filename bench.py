@@ -591,8 +591,10 @@ def main():
         print(f"[bench] cpqr at ranks {ranks} ...", file=sys.stderr, flush=True)
         e_q, t_q, t_full, fl_q, _ = CPQR(A, ranks, handle=h)
         q = min(m, n)
-        # unblocked Householder steps: step j reads and writes the (m-j) x (n-j-1) trailing block
-        qbytes = sum(16.0 * (m - j) * (n - j - 1) for j in range(q))
+        # blocked pivoted QR: step j reads the (m-j) x (n-j-1) trailing block once (the F
+        # column), each 32-column panel reads and writes the trailing block at its end
+        qbytes = sum(8.0 * (m - j) * (n - j - 1) for j in range(q)) + \
+            sum(16.0 * (m - r0) * (n - r0) for r0 in range(32, q, 32))
         cpqr = {"m": m, "n": n, "t_cpqr_ms": t_full * 1e3, "ranks": ranks,
                 "rel_error": [float(x) ** 0.5 for x in e_q],
                 "t_ranks_ms": [x * 1e3 for x in t_q], "flops": fl_q,

@@ -25,8 +25,8 @@ TESTING_LIB = os.path.join(PKG, "librbrp_testing.so")
 CXX_CHECK = os.path.join(PKG, "bin", "rbrp_hpp_check")
 DRIVER = os.path.join(PKG, "bin", "rbrp_driver")
 
-SOURCES = ["k_gemm.hip", "k_rbrp.hip", "k_sketch.hip", "k_mixed.hip", "k_batched.hip", "k_panel.hip",
-           "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_matrices.cpp", "rbrp_batched.cpp"]
+SOURCES = ["k_gemm.hip", "k_rbrp.hip", "k_sketch.hip", "k_qrcp.hip", "k_mixed.hip", "k_batched.hip",
+           "k_panel.hip", "rbrp_host.cpp", "rbrp_sketch.cpp", "rbrp_cpqr.cpp", "rbrp_matrices.cpp", "rbrp_batched.cpp"]
 TESTING_SOURCES = ["rbrp_testing.cpp", "k_ceiling.hip"]
 
 ARCH = os.environ.get("RBRP_OFFLOAD_ARCH", "gfx950")

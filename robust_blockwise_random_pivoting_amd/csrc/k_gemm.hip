@@ -1178,9 +1178,12 @@ hipError_t rbrp_launch_gemm_trail(hipStream_t st, int64_t M, DimSpec N, int64_t 
   if (groups) *groups = 0;
   if (M <= 0 || Nmax <= 0) return hipSuccess;
   if (Kmax > 128) return hipErrorInvalidValue;
-  if (first && (!rbrp_gemm_trail_first_ok(M, Kmax, lda) || !first->p || !first->cols))
+  if (first && (!rbrp_gemm_trail_first_ok(M, Kmax, lda) || ((uintptr_t)A & 15) || !first->p ||
+                !first->cols))
     return hipErrorInvalidValue;
-  if (Kmax <= 64 && M % 64 == 0 && lda % 2 == 0) {
+  // (the 128-column kernel reads A in 16-byte units: A's columns must be
+  // 16-byte aligned)
+  if (Kmax <= 64 && M % 64 == 0 && lda % 2 == 0 && ((uintptr_t)A & 15) == 0) {
     const FirstSrc fs = first ? *first : FirstSrc{nullptr, 0, nullptr, 0, nullptr, 0};
     // 128-column workgroups with the A block shared through LDS
     const int64_t ncol = (Nmax + 127) / 128, nrow = M / 64;

@@ -228,268 +228,14 @@ hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, cons
 }
 
 // ---------------------------------------------------------------------------
-// Householder QR step j on W (rows x cols): optional column pivoting on the
-// trailing squared column norms vn (first maximum, LAPACK dgeqp3 rule: the
-// pivot column swaps with column j), then the reflector of W(j:rows, j) with
-// LAPACK dlarfg conventions: beta = -sign(alpha) ||x||, tau = (beta - alpha) /
-// beta, v = x(2:) / (alpha - beta), stored below the diagonal, beta on it.
-// One workgroup.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(SK_NT) void k_hh_reflect(int64_t rows, int64_t cols, int64_t j,
-                                                      double* __restrict__ W, int64_t ld,
-                                                      double* __restrict__ vn, int* __restrict__ jpvt,
-                                                      double* __restrict__ tau, int pivoting) {
-  __shared__ double rv[SK_NT];
-  __shared__ int ri[SK_NT];
-  __shared__ double red[SK_NT / 64];
-  __shared__ int s_p;
-  const int tid = threadIdx.x;
-  if (pivoting) {
-    double bv = -1.0;
-    int bi = 0x7fffffff;
-    for (int64_t c = j + tid; c < cols; c += SK_NT) {
-      const double d = vn[c];
-      if (d > bv) {  // increasing c per thread: keeps the first maximum
-        bv = d;
-        bi = (int)c;
-      }
-    }
-    rv[tid] = bv;
-    ri[tid] = bi;
-    __syncthreads();
-    for (int o = SK_NT / 2; o > 0; o >>= 1) {
-      if (tid < o) {
-        const double ov = rv[tid + o];
-        const int oi = ri[tid + o];
-        if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
-          rv[tid] = ov;
-          ri[tid] = oi;
-        }
-      }
-      __syncthreads();
-    }
-    if (tid == 0) s_p = (ri[0] == 0x7fffffff) ? (int)j : ri[0];
-    __syncthreads();
-    const int64_t p = s_p;
-    if (p != j) {
-      for (int64_t i = tid; i < rows; i += SK_NT) {
-        const double t0 = W[i + j * ld];
-        W[i + j * ld] = W[i + p * ld];
-        W[i + p * ld] = t0;
-      }
-      if (tid == 0) {
-        const int t1 = jpvt[j];
-        jpvt[j] = jpvt[p];
-        jpvt[p] = t1;
-        const double t2 = vn[j];
-        vn[j] = vn[p];
-        vn[p] = t2;
-      }
-      __syncthreads();
-    }
-  }
-  double* x = W + j * ld;
-  double acc = 0.0;
-  for (int64_t i = j + 1 + tid; i < rows; i += SK_NT) acc = fma(x[i], x[i], acc);
-  const double xn2 = rbrp_block_sum(acc, red);
-  const double alpha = x[j];
-  __syncthreads();
-  if (xn2 == 0.0) {
-    if (tid == 0) tau[j] = 0.0;
-    return;
-  }
-  const double nrm = sqrt(alpha * alpha + xn2);
-  const double beta = (alpha >= 0.0) ? -nrm : nrm;
-  const double scal = 1.0 / (alpha - beta);
-  for (int64_t i = j + 1 + tid; i < rows; i += SK_NT) x[i] *= scal;
-  if (tid == 0) {
-    tau[j] = (beta - alpha) / beta;
-    x[j] = beta;
-  }
-}
-
-hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                  int64_t ld, double* vn, int* jpvt, double* tau, int pivoting) {
-  hipLaunchKernelGGL(k_hh_reflect, dim3(1), dim3(SK_NT), 0, st, rows, cols, j, W, ld, vn, jpvt, tau,
-                     pivoting);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// The same reflector step for long columns, in three launches so that the
-// column work spreads over many workgroups (one workgroup streams a 50000-row
-// column at a small fraction of HBM bandwidth):
-//   k_hh_pivot:    one workgroup: first maximum of vn[j:cols), swaps of jpvt
-//                  and vn, the pivot index to scr_i[0];
-//   k_hh_swapnorm: HH_CHUNK rows per workgroup: swap columns j and p, then the
-//                  workgroup's partial sum of x(j+1:rows)^2 to part[blk] and
-//                  alpha = x(j) to scr_d[0];
-//   k_hh_scale:    HH_CHUNK rows per workgroup: every workgroup adds the
-//                  partials in index order (the same sum everywhere), forms
-//                  beta and tau as k_hh_reflect, scales its rows of x(j+1:);
-//                  workgroup 0 writes tau[j] and x(j) = beta.
-// ---------------------------------------------------------------------------
-#define HH_CHUNK 1024
-
-__global__ __launch_bounds__(SK_NT) void k_hh_pivot(int64_t cols, int64_t j, double* __restrict__ vn,
-                                                    int* __restrict__ jpvt, int* __restrict__ scr_i) {
-  __shared__ double rv[SK_NT];
-  __shared__ int ri[SK_NT];
-  const int tid = threadIdx.x;
-  double bv = -1.0;
-  int bi = 0x7fffffff;
-  for (int64_t c = j + tid; c < cols; c += SK_NT) {
-    const double d = vn[c];
-    if (d > bv) {  // increasing c per thread: keeps the first maximum
-      bv = d;
-      bi = (int)c;
-    }
-  }
-  rv[tid] = bv;
-  ri[tid] = bi;
-  __syncthreads();
-  for (int o = SK_NT / 2; o > 0; o >>= 1) {
-    if (tid < o) {
-      const double ov = rv[tid + o];
-      const int oi = ri[tid + o];
-      if (ov > rv[tid] || (ov == rv[tid] && oi < ri[tid])) {
-        rv[tid] = ov;
-        ri[tid] = oi;
-      }
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    const int p = (ri[0] == 0x7fffffff) ? (int)j : ri[0];
-    scr_i[0] = p;
-    if (p != j) {
-      const int t1 = jpvt[j];
-      jpvt[j] = jpvt[p];
-      jpvt[p] = t1;
-      const double t2 = vn[j];
-      vn[j] = vn[p];
-      vn[p] = t2;
-    }
-  }
-}
-
-__global__ __launch_bounds__(SK_NT) void k_hh_swapnorm(int64_t rows, int64_t j, double* __restrict__ W,
-                                                       int64_t ld, const int* __restrict__ scr_i,
-                                                       double* __restrict__ part,
-                                                       double* __restrict__ scr_d) {
-  __shared__ double red[SK_NT / 64];
-  const int64_t p = scr_i ? scr_i[0] : j;
-  double* x = W + j * ld;
-  double* y = W + p * ld;
-  const int64_t r0 = (int64_t)blockIdx.x * HH_CHUNK, r1 = min(rows, r0 + HH_CHUNK);
-  double acc = 0.0;
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += SK_NT) {
-    double xi = x[i];
-    if (p != j) {
-      const double yi = y[i];
-      y[i] = xi;
-      x[i] = yi;
-      xi = yi;
-    }
-    if (i > j) acc = fma(xi, xi, acc);
-    if (i == j) scr_d[0] = xi;
-  }
-  const double t = rbrp_block_sum(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(SK_NT) void k_hh_scale(int64_t rows, int64_t j, double* __restrict__ W,
-                                                    int64_t ld, const double* __restrict__ part,
-                                                    int nparts, const double* __restrict__ scr_d,
-                                                    double* __restrict__ tau) {
-  double xn2 = 0.0;
-  for (int q = 0; q < nparts; ++q) xn2 += part[q];
-  double* x = W + j * ld;
-  if (xn2 == 0.0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) tau[j] = 0.0;
-    return;
-  }
-  const double alpha = scr_d[0];
-  const double nrm = sqrt(alpha * alpha + xn2);
-  const double beta = (alpha >= 0.0) ? -nrm : nrm;
-  const double scal = 1.0 / (alpha - beta);
-  const int64_t r0 = max((int64_t)blockIdx.x * HH_CHUNK, j + 1), r1 = min(rows, ((int64_t)blockIdx.x + 1) * HH_CHUNK);
-  for (int64_t i = r0 + threadIdx.x; i < r1; i += SK_NT) x[i] *= scal;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    tau[j] = (beta - alpha) / beta;
-    x[j] = beta;
-  }
-}
-
-int64_t rbrp_hh_tall_scratch(int64_t rows) { return (rows + HH_CHUNK - 1) / HH_CHUNK + 2; }
-
-hipError_t rbrp_launch_hh_reflect_tall(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                       int64_t ld, double* vn, int* jpvt, double* tau, int pivoting,
-                                       double* scratch) {
-  // scratch: rbrp_hh_tall_scratch(rows) doubles: alpha, the pivot (as an int)
-  // and one partial sum per chunk
-  if (j >= rows) return hipSuccess;
-  const int nparts = (int)((rows + HH_CHUNK - 1) / HH_CHUNK);
-  double* scr_d = scratch;
-  int* scr_i = (int*)(scratch + 1);
-  double* part = scratch + 2;
-  if (pivoting)
-    hipLaunchKernelGGL(k_hh_pivot, dim3(1), dim3(SK_NT), 0, st, cols, j, vn, jpvt, scr_i);
-  hipLaunchKernelGGL(k_hh_swapnorm, dim3((unsigned)nparts), dim3(SK_NT), 0, st, rows, j, W, ld,
-                     pivoting ? (const int*)scr_i : nullptr, part, scr_d);
-  hipLaunchKernelGGL(k_hh_scale, dim3((unsigned)nparts), dim3(SK_NT), 0, st, rows, j, W, ld,
-                     (const double*)part, nparts, (const double*)scr_d, tau);
-  return hipGetLastError();
-}
-
-// Apply H_j = I - tau v v^T (v(j) = 1) to columns c in (j, cols) of W, one
-// workgroup per column; optionally recompute vn[c] = ||W(j+1:rows, c)||^2
-// from the updated values (exact norms instead of downdating).
-__global__ __launch_bounds__(SK_NT) void k_hh_apply(int64_t rows, int64_t cols, int64_t j,
-                                                    double* __restrict__ W, int64_t ld,
-                                                    const double* __restrict__ tau,
-                                                    double* __restrict__ vn, int norms) {
-  __shared__ double red[SK_NT / 64];
-  const int64_t c = j + 1 + blockIdx.x;
-  if (c >= cols) return;
-  const double tj = tau[j];
-  const double* v = W + j * ld;
-  double* w = W + c * ld;
-  double acc = 0.0;
-  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) acc = fma(v[i], w[i], acc);
-  const double dot = rbrp_block_sum(acc, red) + w[j];
-  const double f = tj * dot;
-  double nacc = 0.0;
-  for (int64_t i = j + 1 + threadIdx.x; i < rows; i += SK_NT) {
-    const double nv = w[i] - f * v[i];
-    w[i] = nv;
-    nacc = fma(nv, nv, nacc);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) w[j] -= f;
-  if (norms) {
-    const double t = rbrp_block_sum(nacc, red);
-    if (threadIdx.x == 0) vn[c] = t;
-  }
-}
-
-hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                int64_t ld, const double* tau, double* vn, int norms) {
-  if (cols - j - 1 <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hh_apply, dim3((unsigned)(cols - j - 1)), dim3(SK_NT), 0, st, rows, cols, j, W,
-                     ld, tau, vn, norms);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // Unpivoted Householder step j of a panel in one launch (the blocked QR's
 // panel steps): workgroup 0 finishes the previous step's column (scales its
 // vector and stores beta, from scr), then stores this step's tau[j] and
 // beta / scale / flag in scr; workgroup b >= 1 applies H_j to column j + b.
-// Every workgroup forms x(j+1:)^2's sum with k_hh_reflect's reduction and
-// v = x * scale with its rounding, so the results equal the two-launch
-// k_hh_reflect + k_hh_apply bit for bit.  Column j itself is written only by
-// the next step's launch (or k_hh_finish), after every reader is done.
+// Every workgroup forms x(j+1:)^2's sum with the same workgroup reduction and
+// v = x * scale with the same rounding, so all of them agree on the
+// reflector.  Column j itself is written only by the next step's launch (or
+// k_hh_finish), after every reader is done.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(SK_NT) void k_hh_step(int64_t rows, int64_t j, double* __restrict__ W,
                                                    int64_t ld, double* __restrict__ tau,

@@ -2,10 +2,10 @@
 // of the reference's experiments, CPQR of gpu/id_cpqr.hpp:6-8 and
 // gpu/id_cpqr.cu:30-172.
 //
-// A working copy of A is factored by Householder QR with column pivoting to
-// min(m, n) steps (the pivot is the first maximum of the trailing squared
-// column norms, LAPACK dgeqp3's rule; the norms are recomputed exactly in the
-// pass that applies each reflector, so no downdating is needed).  For each
+// A working copy of A is factored by the blocked Householder QR with column
+// pivoting (rbrp_qrcp: dgeqp3's pivot rule on downdated partial column
+// norms, 32-column panels with the trailing update on MFMA) to min(m, n)
+// steps.  For each
 // requested rank r: T = R11^{-1} R12 by the blocked TRSM, and the squared
 // relative error ||A(:, P(r:n)) - A(:, P(0:r)) T||_F^2 / ||A||_F^2, with the
 // permuted columns read through the pivot list rather than a permuted copy.
@@ -105,9 +105,9 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   const size_t bDinv = align256(sizeof(double) * 4 * ((q + 255) / 256) * 4096);
   const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * q);
   const size_t bjp = align256(sizeof(int) * n);
-  const size_t bscr = align256(sizeof(double) * rbrp_hh_tall_scratch(m));
+  const size_t bqp = align256(rbrp_qrcp_work_bytes(m, n));
   void* arena = nullptr;
-  HIPCHK(rbrp_sketch_arena(h, bW + bE + bT + bDinv + 2 * bvn + btau + bjp + bscr, &arena));
+  HIPCHK(rbrp_sketch_arena(h, bW + bE + bT + bDinv + bvn + btau + bjp + bqp, &arena));
   char* p = (char*)arena;
   double* W = (double*)p;
   p += bW;
@@ -117,17 +117,13 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   p += bT;
   double* Dinv = (double*)p;
   p += bDinv;
-  double* vn = (double*)p;
-  p += bvn;
   double* cn = (double*)p;
   p += bvn;
   double* tau = (double*)p;
   p += btau;
   int* jpvt = (int*)p;
   p += bjp;
-  double* scr = (double*)p;
-  // long columns: the reflector's column work spread over workgroups
-  const bool tall = m >= 8192;
+  void* qwork = p;
 
   // ||A||_F^2 before the timed factorization (id_cpqr.cu:37-38)
   std::vector<double> hcn((size_t)n);
@@ -140,15 +136,7 @@ extern "C" int rbrp_cpqr_id_d(rbrp_handle_t h, const double* dA, int64_t m, int6
   HIPCHK(ev.create());
   HIPCHK(hipEventRecord(ev.a, st));
   HIPCHK(rbrp_launch_copy_block(st, m, n, dA, lda, W, m));
-  HIPCHK(rbrp_launch_colnorm2(st, m, n, W, m, vn));
-  HIPCHK(rbrp_launch_iota(st, jpvt, n));
-  for (int64_t j = 0; j < q; ++j) {
-    if (tall)
-      HIPCHK(rbrp_launch_hh_reflect_tall(st, m, n, j, W, m, vn, jpvt, tau, 1, scr));
-    else
-      HIPCHK(rbrp_launch_hh_reflect(st, m, n, j, W, m, vn, jpvt, tau, 1));
-    HIPCHK(rbrp_launch_hh_apply(st, m, n, j, W, m, tau, vn, 1));
-  }
+  HIPCHK(rbrp_qrcp(st, m, n, W, m, jpvt, tau, qwork, h->host_scalar));
   HIPCHK(hipEventRecord(ev.b, st));
   double tq = 0.0;
   HIPCHK(ev.seconds(&tq));

@@ -37,6 +37,7 @@ struct rbrp_handle_s {
   void* sk_arena = nullptr;  // workspace of the sketch comparators
   size_t sk_bytes = 0;
   RbrpBlockState* host_state = nullptr;  // pinned, host-coherent
+  int* host_scalar = nullptr;             // pinned: small read-backs (the QRCP panel width)
   int pub_seq = 0;                        // last publication number handed out
   int timing = 0;  // 0 off, 1 every phase, 2 projection and update only
   rbrp_phase_stats_t stats;

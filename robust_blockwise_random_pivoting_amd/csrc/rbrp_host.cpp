@@ -471,6 +471,12 @@ int rbrp_create(rbrp_handle_t* handle, void* stream) {
     delete h;
     return (int)e;
   }
+  e = hipHostMalloc((void**)&h->host_scalar, 16 * sizeof(int), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipHostFree(h->host_state);
+    delete h;
+    return (int)e;
+  }
   // the no-allocation entry point's scratch address on this device (a failed
   // lookup is reported by that entry point, not here)
   (void)scratch_setup(h->device);
@@ -488,6 +494,7 @@ int rbrp_destroy(rbrp_handle_t h) {
   if (h->ws.arena) (void)hipFree(h->ws.arena);
   if (h->sk_arena) (void)hipFree(h->sk_arena);
   if (h->host_state) (void)hipHostFree(h->host_state);
+  if (h->host_scalar) (void)hipHostFree(h->host_scalar);
   for (hipEvent_t e : h->events) (void)hipEventDestroy(e);
   delete h;
   return 0;

@@ -285,6 +285,19 @@ hipError_t rbrp_launch_gemm_trail32(hipStream_t st, int64_t M, DimSpec N, int64_
                                     double* normpart, int64_t ldnp, int64_t* groups = nullptr,
                                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 
+// ---- k_qrcp.hip ----------------------------------------------------------
+// Blocked column-pivoted Householder QR of W (rows x cols, ld) in place to
+// min(rows, cols) steps (dgeqp3 / dlaqps organisation: panels of 32 columns,
+// partial column norms downdated and recomputed when the downdate is
+// rejected): R on and above the diagonal, the reflectors' vectors below it
+// with tau, the pivot order in jpvt (device, cols entries).  work: device
+// scratch of rbrp_qrcp_work_bytes(rows, cols); host_kb: one pinned int (a
+// tall matrix's panel width is read back once per panel).  *panels
+// (optional): the panel count.
+size_t rbrp_qrcp_work_bytes(int64_t rows, int64_t cols);
+hipError_t rbrp_qrcp(hipStream_t st, int64_t rows, int64_t cols, double* W, int64_t ld, int* jpvt,
+                     double* tau, void* work, int* host_kb, int* panels = nullptr);
+
 // ---- k_sketch.hip ---------------------------------------------------------
 hipError_t rbrp_launch_gaussian(hipStream_t st, uint64_t seed, uint64_t offset, int64_t rows,
                                 int64_t cols, double scale, double* out, int64_t ldo);
@@ -314,18 +327,9 @@ hipError_t rbrp_launch_panel_vectors(hipStream_t st, int64_t rows, int64_t rows_
                                      const double* src, int64_t lds, double* V, int64_t ldv);
 hipError_t rbrp_launch_wy_factor(hipStream_t st, int jb, const double* S, int64_t lds, const double* tau,
                                  double* Tt, int64_t ldt);
-// Householder step j of k_hh_reflect for long columns, spread over workgroups
-// of rows; scratch holds rbrp_hh_tall_scratch(rows) doubles.
-int64_t rbrp_hh_tall_scratch(int64_t rows);
-hipError_t rbrp_launch_hh_reflect_tall(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                       int64_t ld, double* vn, int* jpvt, double* tau, int pivoting,
-                                       double* scratch);
+// vn[c] = ||W(0:rows, c)||^2, one workgroup per column.
 hipError_t rbrp_launch_colnorm2(hipStream_t st, int64_t rows, int64_t cols, const double* W,
                                 int64_t ld, double* vn);
-hipError_t rbrp_launch_hh_reflect(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                  int64_t ld, double* vn, int* jpvt, double* tau, int pivoting);
-hipError_t rbrp_launch_hh_apply(hipStream_t st, int64_t rows, int64_t cols, int64_t j, double* W,
-                                int64_t ld, const double* tau, double* vn, int norms);
 // LU step j: pivot search down column j, the row swap over columns [c0, ncols),
 // the column's scaling.
 hipError_t rbrp_launch_lu_pivot(hipStream_t st, int64_t nrows, int64_t c0, int64_t ncols, int64_t j,

@@ -1,8 +1,8 @@
 // Sketch-based ID comparators behind the C ABI: RandCPQR_OS (gpu/randCPQR_OS.cu:20-111),
 // RandLUPP_OS (gpu/randLUPP_OS.cu:14-119) and the oversampled interpolation
 // matrix (gpu/interp.cu:45-105).  The sketch GEMM runs on the MFMA kernel;
-// the pivoted QR / LU and the QR of the interpolation step are Householder /
-// right-looking LU steps, one pair of launches per column.
+// the pivoted QR is the blocked one of k_qrcp.hip (dgeqp3's organisation), the
+// LU and the QR of the interpolation step are blocked onto MFMA panels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -222,18 +222,11 @@ hipError_t lupp_permutation(hipStream_t st, int64_t n, int64_t k, double* Yt, in
 }
 
 // Column-pivoted Householder QR of W (r x n, ldw = r) in place to min(r, n)
-// steps; the pivot order goes to jpvt (device, n entries).
-hipError_t qrcp_in_place(hipStream_t st, int64_t r, int64_t n, double* W, double* vn, int* jpvt,
-                         double* tau) {
-  hipError_t e;
-  if ((e = rbrp_launch_colnorm2(st, r, n, W, r, vn)) != hipSuccess) return e;
-  if ((e = rbrp_launch_iota(st, jpvt, n)) != hipSuccess) return e;
-  const int64_t steps = std::min(r, n);
-  for (int64_t j = 0; j < steps; ++j) {
-    if ((e = rbrp_launch_hh_reflect(st, r, n, j, W, r, vn, jpvt, tau, 1)) != hipSuccess) return e;
-    if ((e = rbrp_launch_hh_apply(st, r, n, j, W, r, tau, vn, 1)) != hipSuccess) return e;
-  }
-  return hipSuccess;
+// steps by the blocked pivoted QR; the pivot order goes to jpvt (device, n
+// entries).  work: rbrp_qrcp_work_bytes(r, n) bytes.
+hipError_t qrcp_in_place(rbrp_handle_t h, int64_t r, int64_t n, double* W, int* jpvt, double* tau,
+                         void* work) {
+  return rbrp_qrcp(h->stream, r, n, W, r, jpvt, tau, work, h->host_scalar);
 }
 
 int check_common(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int64_t lda, int64_t r,
@@ -294,7 +287,7 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   const int64_t l = 2 * r;  // oversampling (randCPQR_OS.cu:24)
   const size_t bG = dG ? 0 : align256(sizeof(double) * l * m);
   const size_t bY = align256(sizeof(double) * l * n), bW = align256(sizeof(double) * r * n);
-  const size_t bvn = align256(sizeof(double) * n), btau = align256(sizeof(double) * l);
+  const size_t bvn = align256(rbrp_qrcp_work_bytes(r, n)), btau = align256(sizeof(double) * l);
   const size_t bjp = align256(sizeof(int) * n);
   void* arena = nullptr;
   const size_t bGt = sketch_scratch_bytes(h, l, m, n, lda);
@@ -309,7 +302,7 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   p += bY;
   double* W = (double*)p;
   p += bW;
-  double* vn = (double*)p;
+  void* qwork = p;
   p += bvn;
   double* tau = (double*)p;
   p += btau;
@@ -320,7 +313,7 @@ int rbrp_randcpqr_os_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, 
   HIPCHK(sketch_gemm(h, l, m, n, G, ldG, dA, lda, Y, Gt));      // randCPQR_OS.cu:42
   // pivoted QR of Y(0:r, :) (randCPQR_OS.cu:55-72): only the pivots are used
   HIPCHK(rbrp_launch_copy_block(st, r, n, Y, l, W, r));
-  HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));
+  HIPCHK(qrcp_in_place(h, r, n, W, jpvt, tau, qwork));
   // T by the oversampled interpolation (randCPQR_OS.cu:95, interp.cu:45-105)
   HIPCHK(interp_os(st, l, n, r, Y, l, jpvt, dT, ldt, iwork));
   std::vector<int32_t> hp((size_t)n);
@@ -387,7 +380,7 @@ int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   if (n > r && !h_rd) return -12;
   hipStream_t st = h->stream;
   const size_t bG = dG ? 0 : align256(sizeof(double) * r * m);
-  const size_t bW = align256(sizeof(double) * r * n), bvn = align256(sizeof(double) * n);
+  const size_t bW = align256(sizeof(double) * r * n), bvn = align256(rbrp_qrcp_work_bytes(r, n));
   const size_t btau = align256(sizeof(double) * r), bjp = align256(sizeof(int) * n);
   const size_t bDinv = align256(sizeof(double) * 4 * ((r + 255) / 256) * 4096);
   void* arena = nullptr;
@@ -401,7 +394,7 @@ int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   p += bG;
   double* W = (double*)p;
   p += bW;
-  double* vn = (double*)p;
+  void* qwork = p;
   p += bvn;
   double* tau = (double*)p;
   p += btau;
@@ -410,7 +403,7 @@ int rbrp_randcpqr_d(rbrp_handle_t h, const double* dA, int64_t m, int64_t n, int
   double* Dinv = (double*)p;
   if (!dG) HIPCHK(rbrp_launch_gaussian(st, seed, offset, r, m, 1.0 / std::sqrt((double)m), G, r));
   HIPCHK(sketch_gemm(h, r, m, n, G, ldG, dA, lda, W, Gt));      // randCPQR.cu:42
-  HIPCHK(qrcp_in_place(st, r, n, W, vn, jpvt, tau));              // randCPQR.cu:52
+  HIPCHK(qrcp_in_place(h, r, n, W, jpvt, tau, qwork));            // randCPQR.cu:52
   if (n > r) {
     // T = R11^{-1} R12 straight from the sketch's factor (randCPQR.cu:60-64)
     HIPCHK(rbrp_launch_copy_block(st, r, n - r, W + r * r, r, dT, ldt));

@@ -73,7 +73,10 @@ def _dev(A):
 @pytest.mark.gpu
 @pytest.mark.parametrize("m,n,rate,seed", [(240, 160, 40.0, 11), (120, 200, 30.0, 12),
                                            (300, 300, 1e9, 13), (1000, 64, 10.0, 14),
-                                           (9000, 48, 12.0, 15), (20000, 96, 25.0, 16)])
+                                           (9000, 48, 12.0, 15), (20000, 96, 25.0, 16),
+                                           # many rejected norm downdates: panels cut short
+                                           # (tall) and norms recomputed in the step (short)
+                                           (6000, 300, 20.0, 17), (400, 1200, 50.0, 18)])
 def test_cpqr_matches_oracle(librbrp, m, n, rate, seed):
     from robust_blockwise_random_pivoting_amd import CPQR
     A = _mat(m, n, rate, seed) if rate < 1e8 else np.random.default_rng(seed).standard_normal((m, n))
