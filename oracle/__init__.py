@@ -19,6 +19,13 @@ from .rbrp_oracle import (  # noqa: F401
     interpolation_matrix,
     id_error,
     RBRPResult,
+    pivot_margin,
+    filter_margin,
+    first_near_tie,
+    margin_summary,
+    tie_pivot,
+    TIE_CDF,
+    TIE_STOP,
 )
 from .sketch_oracle import (  # noqa: F401
     compute_interpolation_matrix_os,

@@ -43,7 +43,21 @@ Pins (each checked by ``tests/test_oracle.py``):
 * permutation: brute-force check of the reference's three copy statements;
 * whole loop: an independent left-looking restatement of matlab/RBRP.m fed the
   same uniforms selects the same skeleton, and the QR identity
-  ``||A(:,rd) - A(:,J) X||_F^2 == sum(residual diags)`` holds.
+  ``||A(:,rd) - A(:,J) X||_F^2 == sum(residual diags)`` holds;
+* near-tie margins (``block_margins``, ``first_near_tie``): closed forms on
+  planted ties (a uniform at a known distance from a CDF step, two columns of
+  known norms, a filter sum at a known distance from its threshold).
+
+Near ties.  Each block records how close its discrete decisions came to
+flipping: the sampler's CDF margin (the distance of every uniform from the
+CDF steps around the index it selected, in units of the total weight), the
+pivot margin (the relative gap between the chosen column's partial norm and
+the largest other one at every pivoted-QR step), the filter margin (the
+relative distance of the filter's running sums from its threshold) and the
+stop margin (the relative distance of the block's squared error from tol^2).
+An implementation that forms the same quantities in another order or by
+another factorization may decide differently only where a margin is below
+its rounding error; ``first_near_tie`` names the first such block.
 """
 
 from __future__ import annotations
@@ -68,14 +82,16 @@ def uniform_stream(seed: int, offset: int = 0) -> np.random.Generator:
     return np.random.Generator(np.random.Philox(key=int(seed), counter=int(offset)))
 
 
-def sample_without_replacement(w, s: int, rng: np.random.Generator):
+def sample_without_replacement(w, s: int, rng: np.random.Generator, margins=None):
     """Weighted sampling of ``s`` distinct indices of ``w`` (rounds algorithm).
 
     Restates ``rng.choice(len(w), size=s, p=w/sum(w), replace=False)`` as used
     at RBRP.ipynb c11:108: each round draws ``s - n_found`` uniforms, zeroes
     the weights already found, forms ``cdf = cumsum(w) / cdf[-1]``, takes
     ``searchsorted(cdf, u, side='right')`` and appends the new distinct picks in
-    order of first occurrence.  Returns ``(indices, draws_consumed)``.
+    order of first occurrence.  Returns ``(indices, draws_consumed)``.  With
+    ``margins`` (a list), appends each uniform's distance from the nearer of
+    the two CDF values around its index, ``min(cdf[j] - u, u - cdf[j-1])``.
     """
     w = np.array(w, dtype=np.float64, copy=True)
     if s > np.count_nonzero(w > 0):
@@ -91,6 +107,9 @@ def sample_without_replacement(w, s: int, rng: np.random.Generator):
         cdf = np.cumsum(w)
         cdf /= cdf[-1]
         new = cdf.searchsorted(u, side="right")
+        if margins is not None:
+            lower = np.where(new > 0, cdf[np.maximum(new - 1, 0)], 0.0)
+            margins.extend(np.minimum(cdf[new] - u, u - lower).tolist())
         _, first = np.unique(new, return_index=True)
         first.sort()
         found.extend(int(x) for x in new[first])
@@ -126,6 +145,94 @@ def robust_filter(R, s: int, b: int, mode: str = "sampled") -> int:
     return s - k
 
 
+EPS = float(np.finfo(np.float64).eps)
+# thresholds below which a margin is a near tie: the sampler's CDF is summed
+# in another order on the device (about 1e3 eps covers its rounding); the
+# device's pivots and filter sums come from a Gram matrix and a pivoted
+# Cholesky factorization, whose Schur diagonals carry about s^3 eps relative
+# error for a block of s columns; the error trace is compared to 1e-6.
+TIE_CDF = 1e3 * EPS
+TIE_STOP = 1e-6
+
+
+def tie_pivot(s: int) -> float:
+    """Near-tie threshold of the pivot and filter margins of an s-column block."""
+    return max(1e3 * EPS, float(s) ** 3 * EPS)
+
+
+def pivot_margin(Rv, steps: int = 0) -> float:
+    """Smallest relative gap between the chosen column's partial norm and the
+    largest other candidate's over the first ``steps`` steps (all when 0) of a
+    pivoted QR with factor ``Rv`` (columns in pivot order): at step k the
+    candidates t >= k have partial norms ``sqrt(sum_{r >= k} Rv[r, t]^2)`` and
+    the pivot is t = k.  Steps whose chosen norm is zero are skipped; +inf when
+    no step has two candidates.  (The block keeps its first bt pivots: later
+    steps reach the decisions only through the filter's sums.)
+    """
+    Rv = np.asarray(Rv, dtype=np.float64)
+    rows, s = Rv.shape
+    if s < 2:
+        return float("inf")
+    sq = Rv * Rv
+    S = np.cumsum(sq[::-1, :], axis=0)[::-1, :]       # S[k, t] = sum_{r >= k} Rv[r, t]^2
+    best = float("inf")
+    last = min(rows, s - 1) if steps <= 0 else min(rows, s - 1, steps)
+    for k in range(last):
+        chosen = np.sqrt(S[k, k])
+        if chosen == 0.0:
+            continue
+        other = np.sqrt(np.max(S[k, k + 1:]))
+        best = min(best, (chosen - other) / chosen)
+    return best
+
+
+def filter_margin(R, s: int, b: int, mode: str = "sampled") -> float:
+    """Relative distance of the filter's deciding running sums from its
+    threshold: ``min(csum[k] - thr, thr - csum[k-1]) / thr`` at the first k with
+    ``csum[k] >= thr`` (see ``robust_filter``); +inf for a zero threshold.  The
+    last sum is the total, at or above the threshold whatever the rounding, so
+    it counts only as the lower side.
+    """
+    R = np.asarray(R, dtype=np.float64)
+    sqr = np.array([float(np.sum(R[r, r:s] ** 2)) for r in range(s)])
+    csum = np.cumsum(sqr[::-1])
+    div = s if mode == "sampled" else b
+    thr = csum[-1] / div
+    if thr <= 0.0:
+        return float("inf")
+    k = int(np.argmax(csum >= thr))
+    m = csum[k] - thr if k < s - 1 else float("inf")
+    if k > 0:
+        m = min(m, thr - csum[k - 1])
+    return float(m / thr)
+
+
+def first_near_tie(res):
+    """``(block index, which margin, value, threshold)`` of the first block of
+    an ``RBRPResult`` with a margin below its threshold, or None."""
+    for t, blk in enumerate(res.blocks):
+        for key, thr in (("cdf_margin", TIE_CDF), ("pivot_margin", tie_pivot(blk["s"])),
+                         ("filter_margin", tie_pivot(blk["s"])), ("stop_margin", TIE_STOP)):
+            if blk[key] < thr:
+                return t, key, blk[key], thr
+    return None
+
+
+def margin_summary(res) -> str:
+    """One line: each margin's minimum over the blocks and where it occurs."""
+    parts = []
+    for key in ("cdf_margin", "pivot_margin", "filter_margin", "stop_margin"):
+        vals = [blk[key] for blk in res.blocks]
+        if not vals:
+            continue
+        t = int(np.argmin(vals))
+        parts.append(f"{key.split('_')[0]} {vals[t]:.2e} (block {t})")
+    tie = first_near_tie(res)
+    parts.append("near tie: none" if tie is None else
+                 f"near tie: block {tie[0]} {tie[1]} {tie[2]:.2e} < {tie[3]:.2e}")
+    return ", ".join(parts)
+
+
 def symmetric_difference(i: int, bt: int, jk):
     """``(Jin, Jout)``: ``Jin = sort(jk) \\ [i, i+bt)``, ``Jout = [i, i+bt) \\ jk``.
 
@@ -148,7 +255,7 @@ class RBRPResult:
     ranks: np.ndarray         # cumulative rank after each block
     error: np.ndarray         # squared relative residual after each block
     draws: int                # uniforms consumed
-    blocks: list              # per-block dict(i, s, bt, J, Jk)
+    blocks: list              # per-block dict(i, s, bt, J, Jk and the four margins)
     resid: np.ndarray         # final residual column norms^2 (positions k..n-1)
     N0: float
 
@@ -186,7 +293,8 @@ def rbrp_colid(A, b: int, tol: float = 0.0, seed: int = 0, offset: int = 0,
         if nnz == 0:                                             # RBRP.m:113-116
             break
         s = min(b, kmax - i, nnz)                                # RBRP.m:32
-        Js, d = sample_without_replacement(w, s, rng)            # c11:108
+        cm = []
+        Js, d = sample_without_replacement(w, s, rng, cm)        # c11:108
         draws += d
         J = i + Js
         V = np.asfortranarray(Aw[:, J])
@@ -221,7 +329,10 @@ def rbrp_colid(A, b: int, tol: float = 0.0, seed: int = 0, offset: int = 0,
                 Aw[:, i + bt:] = out
             diags[i + bt:] = _colnorm2(Aw[:, i + bt:])           # rbrp.cu:455-465
         err = float(np.sum(diags[i + bt:])) / N0 if N0 > 0 else 0.0   # rbrp.cu:467
-        blocks.append(dict(i=i, s=s, bt=bt, J=J.copy(), Jk=Jk.copy()))
+        blocks.append(dict(i=i, s=s, bt=bt, J=J.copy(), Jk=Jk.copy(),
+                           cdf_margin=float(min(cm)), pivot_margin=pivot_margin(Rv, bt),
+                           filter_margin=filter_margin(Rv, s, b, filter_mode),
+                           stop_margin=abs(err - tol2) / tol2 if tol2 > 0 else float("inf")))
         i += bt
         ranks.append(i)
         errors.append(err)

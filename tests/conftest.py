@@ -12,6 +12,15 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
 
 
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    """The parity tests' near-tie margins and X differences, after the run."""
+    from tests.parity import REPORT
+    if REPORT:
+        terminalreporter.section("parity margins and X differences")
+        for line in REPORT:
+            terminalreporter.write_line(line)
+
+
 def _gpu_available() -> bool:
     try:
         import torch

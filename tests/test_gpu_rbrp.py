@@ -1,7 +1,8 @@
 """End-to-end parity of the HIP RBRP column ID against the CPU oracle (GPU).
 
 Bar (fed the same Philox uniform stream):
-* skeleton indices and block ranks identical to the oracle's;
+* skeleton indices, block ranks and draws identical to the oracle's (up to the
+  first block with a near tie, when the oracle records one; tests/parity.py);
 * X = R11^{-1} R12 within 1e-10 relative (Frobenius) where the a-priori bound
   kappa(R11) * eps allows it, else within 100 * kappa(R11) * eps;
 * ID error ||A(:,rd) - A(:,J) X||_F / ||A||_F within 1% of the oracle's.
@@ -11,6 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import id_error, rbrp_colid as oracle_colid
+from tests.parity import assert_skeleton, report
 from robust_blockwise_random_pivoting_amd.matrices import (exp_spectrum, gmm_rows, kernel_block,
                                                             spectrum_matrix)
 
@@ -33,17 +35,15 @@ def run_both(A, b, tol, seed=7, offset=0, filt="sampled", max_rank=0):
 
 def assert_parity(A, g, r, name=""):
     perm = g.perm.cpu().numpy()
-    assert g.k == r.k, name
-    assert np.array_equal(perm[: g.k], r.perm[: r.k]), name
     assert np.array_equal(np.sort(perm), np.arange(A.shape[1])), name
-    assert np.array_equal(g.ranks, r.ranks), name
-    assert g.draws == r.draws, name
+    if not assert_skeleton(perm, g.k, g.ranks, g.draws, r, name):
+        return
     if g.k and g.k < A.shape[1]:
         X = g.X.cpu().numpy()
         kappa = np.linalg.cond(r.R[:, : r.k])
         rel = np.linalg.norm(X - r.X) / max(np.linalg.norm(r.X), 1e-300)
         bound = max(1e-10, 100 * kappa * EPS)
-        print(f"[{name}] X rel diff {rel:.2e} (bound {bound:.2e}, kappa {kappa:.2e})")
+        report(f"[{name}] X rel diff {rel:.2e} (bound {bound:.2e}, kappa {kappa:.2e})")
         assert rel <= bound, (name, rel, kappa)
         eg = id_error(A, perm, g.k, X)
         er = id_error(A, r.perm, r.k, r.X)

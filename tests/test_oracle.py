@@ -417,3 +417,69 @@ def test_rand_adap_lupp_estimate_is_unbiased():
     rng = np.random.default_rng(10)
     vals = [np.sum(((rng.standard_normal((16, 200)) / 4.0) @ E) ** 2) for _ in range(600)]
     assert np.mean(vals) == pytest.approx(np.sum(E * E), rel=0.05)
+
+
+# ---------------------------------------------------------------- near-tie margins
+
+class _FixedUniforms:
+    """Stands in for the generator: hands out preset uniforms."""
+
+    def __init__(self, us):
+        self.us = list(us)
+
+    def random(self, n):
+        out, self.us = self.us[:n], self.us[n:]
+        return np.asarray(out, dtype=np.float64)
+
+
+def test_cdf_margin_is_the_distance_to_the_nearer_step():
+    # weights 1, 1, 2: cdf = 0.25, 0.5, 1
+    for u, idx, margin in [(0.26, 1, 0.01), (0.49, 1, 0.01), (0.30, 1, 0.05), (0.10, 0, 0.10),
+                           (0.90, 2, 0.10), (0.55, 2, 0.05)]:
+        ms = []
+        got, draws = sample_without_replacement([1.0, 1.0, 2.0], 1, _FixedUniforms([u]), ms)
+        assert list(got) == [idx] and draws == 1
+        assert ms == [pytest.approx(margin, abs=1e-15)]
+
+
+def test_pivot_margin_closed_form():
+    from oracle import pivot_margin
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((20, 3)))
+    V = Q * np.array([2.9, 1.0, 3.0])           # orthogonal columns of norms 2.9, 1, 3
+    _, R, piv = sla.qr(V, mode="economic", pivoting=True)
+    assert list(piv) == [2, 0, 1]
+    # step 0: 3 against 2.9; step 1: 2.9 against 1
+    assert pivot_margin(R) == pytest.approx((3.0 - 2.9) / 3.0, rel=1e-12)
+    assert pivot_margin(R, steps=1) == pytest.approx((3.0 - 2.9) / 3.0, rel=1e-12)
+    W = Q * np.array([1.0, 2.0, 3.0])
+    _, R, _ = sla.qr(W, mode="economic", pivoting=True)
+    assert pivot_margin(R) == pytest.approx(min(1.0 / 3.0, 1.0 / 2.0), rel=1e-12)
+
+
+def test_filter_margin_closed_form():
+    from oracle import filter_margin
+    # row sums of squares 4, 3, 2, 1: running sums from the last row 1, 3, 6, 10;
+    # threshold 10 / 4 = 2.5 is first reached at 3: margin min(0.5, 1.5) / 2.5
+    R = np.diag(np.sqrt([4.0, 3.0, 2.0, 1.0]))
+    assert robust_filter(R, 4, 4) == 3
+    assert filter_margin(R, 4, 4) == pytest.approx(0.2, rel=1e-12)
+    # the nominal rule divides by b = 8: threshold 1.25, first reached at 3
+    assert filter_margin(R, 4, 8, "nominal") == pytest.approx(min(3 - 1.25, 1.25 - 1) / 1.25, rel=1e-12)
+    # one column: the only sum is the total, which always meets the threshold
+    assert filter_margin(np.array([[2.0]]), 1, 1) == float("inf")
+
+
+def test_first_near_tie_names_the_planted_block():
+    from oracle import first_near_tie, margin_summary, tie_pivot
+    # two orthogonal columns of equal norm: the block's pivot margin is zero
+    A = np.zeros((6, 4))
+    A[0, 0], A[1, 1], A[2, 2], A[3, 3] = 1.0, 3.0, 3.0, 0.5
+    r = rbrp_colid(A, 4, 0.0, seed=1)
+    assert r.blocks[0]["pivot_margin"] == pytest.approx(0.0, abs=1e-15)
+    tie = first_near_tie(r)
+    assert tie is not None and tie[0] == 0 and tie[1] == "pivot_margin" and tie[3] == tie_pivot(4)
+    assert "near tie: block 0 pivot_margin" in margin_summary(r)
+    # well separated norms: no near tie
+    A[2, 2] = 2.0
+    assert first_near_tie(rbrp_colid(A, 4, 0.0, seed=1)) is None
