@@ -246,7 +246,7 @@ def batched_leg(args):
     one persistent launch (autotuned, replayed from its hipGraph) vs one launch
     sequence per problem, and the CPU oracle on a sample of the same problems."""
     import torch
-    from robust_blockwise_random_pivoting_amd import Batch, Handle, batched_tune
+    from robust_blockwise_random_pivoting_amd import Batch, Handle, batched_launch_shape, batched_tune
     from robust_blockwise_random_pivoting_amd.matrices import kernel_blocks_torch
     nprob = args.batch_count
     As = kernel_blocks_torch(nprob, seed=4, m_range=(64, 1024), n_range=(64, 512))
@@ -256,6 +256,7 @@ def batched_leg(args):
     seeds = list(range(nprob))
     with Handle() as h:
         wg = batched_tune(As, b=b, tol=tol, seeds=seeds, handle=h)
+        _, tile = batched_launch_shape(As, b=b, tol=tol, handle=h)
         pb = Batch(As, b=b, tol=tol, seeds=seeds)
         st = pb.run("persistent", h)                       # warm-up: records the graph
         reps = 3
@@ -263,8 +264,8 @@ def batched_leg(args):
         for _ in range(reps):
             st = pb.run("persistent", h)
         tp = (time.perf_counter() - t0) / reps
-        print(f"[bench] batched persistent: {tp * 1e3:.1f} ms ({wg} WG/CU)", file=sys.stderr,
-              flush=True)
+        print(f"[bench] batched persistent: {tp * 1e3:.1f} ms ({wg} WG/CU, {tile} row tiles)",
+              file=sys.stderr, flush=True)
         if args.no_batch_loop:
             lb, stl, tl = None, np.zeros(nprob, np.int32), float("nan")
         else:
@@ -280,7 +281,8 @@ def batched_leg(args):
     errs = np.array([r.err_final for r in rp])
     out = {"problems": nprob, "b": b, "tol": tol, "persistent_ms": tp * 1e3,
            "persistent_ids_per_s": nprob / tp, "loop_ms": tl * 1e3, "loop_ids_per_s": nprob / tl,
-           "speedup_vs_loop": tl / tp, "wg_per_cu": wg, "status_nonzero": int((st != 0).sum()),
+           "speedup_vs_loop": tl / tp, "wg_per_cu": wg, "update_row_tiles": tile,
+           "status_nonzero": int((st != 0).sum()),
            "loop_status_nonzero": int((stl != 0).sum()), "same_as_loop": same,
            "k_mean": float(ks.mean()), "k_max": int(ks.max()),
            "max_tracked_error": float(errs.max())}

@@ -316,11 +316,21 @@ int rbrp_colid_batched_d(rbrp_handle_t handle, int64_t nprob, const rbrp_problem
                          const rbrp_opts_t* opts, rbrp_problem_result_t* results, int mode);
 
 /* Autotuner of the persistent launch: runs the batch once per candidate
- * number of workgroups per compute unit, keeps the fastest for later calls of
- * the same shape on this handle and returns it in *wg_per_cu (results of the
- * last run are left in the problems' outputs). */
+ * launch shape -- the row tiles (1, 2 or 4) a wave takes at once in the fused
+ * projection + trailing update, which sets the kernel's register footprint,
+ * times the number of workgroups per compute unit -- keeps the fastest for
+ * later calls of the same batch shape on this handle and returns its
+ * workgroups per CU in *wg_per_cu (results of the last run are left in the
+ * problems' outputs).  b is not tuned: it changes the result.  No counterpart
+ * in the reference: the north star's "tile shapes chosen by an autotuner". */
 int rbrp_batched_tune(rbrp_handle_t handle, int64_t nprob, const rbrp_problem_t* problems,
                       const rbrp_opts_t* opts, int* wg_per_cu);
+
+/* The launch shape later persistent calls for this batch shape will use on
+ * this handle: the tuner's pick, else 4 workgroups per CU and 2 row tiles.
+ * No counterpart in the reference (reports the autotuner's choice). */
+int rbrp_batched_launch_shape(rbrp_handle_t handle, int64_t nprob, const rbrp_problem_t* problems,
+                              const rbrp_opts_t* opts, int* wg_per_cu, int* tile_rows);
 
 /* Defaults: b = 64, tol = 0, max_rank = 0, seed = 0, offset = 0, FP64, sampled filter. */
 void rbrp_opts_default(rbrp_opts_t* opts);

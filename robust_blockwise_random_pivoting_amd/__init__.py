@@ -7,9 +7,9 @@ ImportError if it has not been built.
 """
 
 from .rbrp import (CPQR, RBRP, Batch, ColID, Handle, RandAdapLUPP, RandCPQR,  # noqa: F401
-                   RandCPQR_OS, RandLUPP, RandLUPP_OS, batched_tune, colmajor, gaussian_sketch, interp_os,
+                   RandCPQR_OS, RandLUPP, RandLUPP_OS, batched_launch_shape, batched_tune, colmajor, gaussian_sketch, interp_os,
                    rbrp_colid, rbrp_colid_batched, uniforms)
 
 __all__ = ["CPQR", "RBRP", "Batch", "ColID", "Handle", "RandAdapLUPP", "RandCPQR", "RandCPQR_OS", "RandLUPP",
-           "RandLUPP_OS", "batched_tune", "colmajor", "gaussian_sketch", "interp_os", "rbrp_colid",
+           "RandLUPP_OS", "batched_launch_shape", "batched_tune", "colmajor", "gaussian_sketch", "interp_os", "rbrp_colid",
            "rbrp_colid_batched", "uniforms"]

@@ -137,6 +137,8 @@ def lib():
                                          POINTER(RbrpProblemResult), c_int]),
         "rbrp_batched_tune": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
                                       POINTER(c_int)]),
+        "rbrp_batched_launch_shape": (c_int, [vp, i64, POINTER(RbrpProblem), POINTER(RbrpOpts),
+                                              POINTER(c_int), POINTER(c_int)]),
         "rbrp_gaussian_sketch_d": (c_int, [vp, u64, u64, i64, i64, c_double, vp, i64]),
         "rbrp_randcpqr_os_d": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, vp, i64, vp, vp, vp,
                                        i64]),

@@ -278,17 +278,18 @@ __device__ void bq_nn(int rows, int N, int K, FA colA, FB belem, FC colC, FN nor
 // W = Q^T C(:, tile) over all K rows (the loop of bq_tn), stores W into
 // Wout (row r = kept column, leading dimension ldw), and at once updates the
 // same columns C(:, tile) -= Q W with their new squared norms (the loop of
-// bq_nn with BQ_RT = 2): the MFMA output of the projection, lane (j, g)
+// bq_nn with BQ_RT = RT row tiles a wave at a time, RT a launch shape the
+// tuner picks: every element's sum is the same for any RT): the MFMA output
+// of the projection, lane (j, g)
 // holding W(4v + g, c0 + j), is already the B fragment of the update.  The
 // arithmetic is that of bq_tn followed by bq_nn, without the barrier between
 // them, and the second pass over a tile follows its first while its rows
 // are still in the caches.  Q (K x bt) and C (K x N) have leading dimension K
 // (a multiple of 16, rows >= rows zero in Q); rows >= rows of C are neither
 // read nor written by the update.
-template <class FN>
+template <int RT, class FN>
 __device__ void bq_proj_update(int rows, int K, int bt, int N, const double* Q, double* C,
                                double* Wout, int64_t ldw, FN normout) {
-  constexpr int RT = 2;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int TNn = (N + 15) >> 4, nrt = (rows + 15) >> 4;
@@ -690,6 +691,7 @@ __device__ void bq_chol2(const BqLds& L, int bt) {
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
+template <int RT>
 __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void k_colid_batched(const RbrpBatchProblem* __restrict__ probs,
                                                          const int* __restrict__ order, int nprob,
                                                          int* queue, RbrpBatchResult* results,
@@ -860,7 +862,7 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
       // ---- projection W = Q_b^T A_rem into R's block rows, then update -------
       const int nrem = n - i - bt;
       if (bt <= 16 && nrem >= 49) {
-        bq_proj_update(mp, mp, bt, nrem, Aw + (int64_t)i * mp, Aw + (int64_t)(i + bt) * mp,
+        bq_proj_update<RT>(mp, mp, bt, nrem, Aw + (int64_t)i * mp, Aw + (int64_t)(i + bt) * mp,
                        R + i + (int64_t)(i + bt) * ldr, ldr,
                        [&](int c, double v) { L.diags[i + bt + c] = v; });
       } else if (nrem > 0) {
@@ -935,6 +937,16 @@ __global__ __launch_bounds__(BQ_NT) __attribute__((amdgpu_waves_per_eu(2))) void
   }
 }
 
+// The instantiation for a launch shape's update row tiles (1, 2 or 4), or null.
+static const void* batched_kernel(int tile) {
+  switch (tile) {
+    case 1: return (const void*)k_colid_batched<1>;
+    case 2: return (const void*)k_colid_batched<2>;
+    case 4: return (const void*)k_colid_batched<4>;
+    default: return nullptr;
+  }
+}
+
 __global__ void k_queue_reset(int* queue) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0;
 }
@@ -946,24 +958,34 @@ hipError_t rbrp_launch_colid_batched(hipStream_t st, int grid, const RbrpBatchPr
   if (shape.b < 1 || shape.b > RBRP_BATCH_MAXB || shape.n_max > RBRP_BATCH_MAXN)
     return hipErrorInvalidValue;
   const size_t lds = rbrp_batched_lds_bytes(shape.n_max, shape.b);
+  if (!batched_kernel(shape.tile)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, queue);
-  hipLaunchKernelGGL(k_colid_batched, dim3((unsigned)grid), dim3(BQ_NT), lds, st, probs, order, nprob,
-                     queue, results, shape);
+  const dim3 g((unsigned)grid), t(BQ_NT);
+  switch (shape.tile) {
+    case 1: hipLaunchKernelGGL(k_colid_batched<1>, g, t, lds, st, probs, order, nprob, queue, results, shape); break;
+    case 2: hipLaunchKernelGGL(k_colid_batched<2>, g, t, lds, st, probs, order, nprob, queue, results, shape); break;
+    default: hipLaunchKernelGGL(k_colid_batched<4>, g, t, lds, st, probs, order, nprob, queue, results, shape); break;
+  }
   return hipGetLastError();
 }
 
 size_t rbrp_batched_lds_bytes(int n_max, int b) { return bq_lds_bytes(n_max, b); }
 
-int rbrp_batched_max_blocks_per_cu(int n_max, int b) {
+int rbrp_batched_max_blocks_per_cu(int n_max, int b, int tile) {
+  const void* fn = batched_kernel(tile);
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_colid_batched, BQ_NT,
-                                                   bq_lds_bytes(n_max, b)) != hipSuccess)
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, BQ_NT, bq_lds_bytes(n_max, b)) !=
+                 hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;
 }
 
 hipError_t rbrp_batched_prepare(int n_max, int b) {
-  return hipFuncSetAttribute((const void*)k_colid_batched,
-                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bq_lds_bytes(n_max, b));
+  for (int tile : {1, 2, 4}) {
+    const hipError_t e = hipFuncSetAttribute(batched_kernel(tile),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)bq_lds_bytes(n_max, b));
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }

@@ -85,6 +85,7 @@ int validate(int64_t nprob, const rbrp_problem_t* pr, const rbrp_opts_t* o, Batc
   sh.max_rank = (int)std::min<int64_t>(o->max_rank, 0x7fffffff);
   sh.tol2 = o->tol * o->tol;
   sh.ws = nullptr;
+  sh.tile = 2;
   return 0;
 }
 
@@ -132,8 +133,8 @@ int compute_units(rbrp_handle_t h, int* cus, size_t* lds_max) {
 // recorded graph when use_graph), downloads the results.  ms (optional)
 // receives the device time of the launch itself.
 int run_persistent(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, BatchPlan& pl,
-                   int wg_per_cu, bool use_graph, rbrp_problem_result_t* results, float* ms,
-                   int* per_cu_used = nullptr) {
+                   int wg_per_cu, int tile, bool use_graph, rbrp_problem_result_t* results,
+                   float* ms, int* per_cu_used = nullptr) {
   int cus;
   size_t lds_cu;
   {
@@ -142,9 +143,10 @@ int run_persistent(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, Bat
   }
   (void)lds_cu;
   HIPCHK(rbrp_batched_prepare(pl.shape.n_max, pl.shape.b));
+  pl.shape.tile = tile;
   // persistent workgroups beyond what is co-resident would only start once
   // the queue is drained: cap at the occupancy limit
-  const int fit = rbrp_batched_max_blocks_per_cu(pl.shape.n_max, pl.shape.b);
+  const int fit = rbrp_batched_max_blocks_per_cu(pl.shape.n_max, pl.shape.b, tile);
   const int per_cu = std::max(1, std::min(wg_per_cu, fit));
   if (per_cu_used) *per_cu_used = per_cu;
   const int grid = (int)std::min<int64_t>((int64_t)per_cu * cus, nprob);
@@ -197,7 +199,8 @@ int run_persistent(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, Bat
                                  (uint64_t)(uintptr_t)dm, s.slot_bytes,        s.off_R,
                                  s.off_Q,              s.off_C,               s.off_T,
                                  (uint64_t)s.ldr,      (uint64_t)s.n_max,     (uint64_t)s.b,
-                                 (uint64_t)s.filter,   (uint64_t)s.max_rank,  0};
+                                 (uint64_t)s.filter,   (uint64_t)s.max_rank,  (uint64_t)s.tile,
+                                 0};
     std::memcpy(&key.back(), &s.tol2, sizeof(double));
     if (!h->bt_exec || h->bt_key != key) {
       drop_graph(h);
@@ -262,11 +265,13 @@ int run_loop(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* pr, const rbr
   return 0;
 }
 
-int default_wg_per_cu(rbrp_handle_t h, const BatchPlan& pl) {
+// The launch shape for this batch shape: the tuner's pick on this handle, or
+// four workgroups per CU and two row tiles.  Packed as wg_per_cu | tile << 8.
+int launch_shape(rbrp_handle_t h, const BatchPlan& pl) {
   const std::vector<uint64_t> key = tune_key(pl);
   for (const auto& kv : h->bt_tuned)
     if (kv.first == key) return kv.second;
-  return 4;
+  return 4 | (2 << 8);
 }
 
 }  // namespace
@@ -299,7 +304,8 @@ int rbrp_colid_batched_d(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* p
   if (nprob == 0) return 0;
   if (mode == RBRP_BATCH_PERSISTENT && !pl.fits) return -4;
   if (mode == RBRP_BATCH_LOOP || !pl.fits) return run_loop(h, nprob, problems, opts, results);
-  return run_persistent(h, nprob, problems, pl, default_wg_per_cu(h, pl), true, results, nullptr);
+  const int shp = launch_shape(h, pl);
+  return run_persistent(h, nprob, problems, pl, shp & 0xff, shp >> 8, true, results, nullptr);
 }
 
 int rbrp_batched_tune(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* problems,
@@ -313,31 +319,54 @@ int rbrp_batched_tune(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* prob
   if (!wg_per_cu) return -5;
   if (nprob == 0 || !pl.fits) return -4;
   std::vector<rbrp_problem_result_t> res((size_t)nprob);
+  // launch shapes: row tiles per wave of the fused projection + update (the
+  // register footprint changes with it, and with it the occupancy) x
+  // workgroups per compute unit
   const int cand[] = {1, 2, 3, 4, 6, 8};
   float best = 0.f;
-  int pick = 0, last_used = 0;
-  for (int c : cand) {
-    float ms = 0.f;
-    BatchPlan p2 = pl;
-    int used = 0;
-    int e = run_persistent(h, nprob, problems, p2, c, false, res.data(), &ms, &used);
-    if (e) return e;
-    if (used == last_used) continue;  // capped to the same launch as the previous candidate
-    last_used = used;
-    if (pick == 0 || ms < best) {
-      best = ms;
-      pick = used;
+  int pick = 0, pick_tile = 2;
+  for (int tile : {1, 2, 4}) {
+    int last_used = 0;
+    for (int c : cand) {
+      float ms = 0.f;
+      BatchPlan p2 = pl;
+      int used = 0;
+      int e = run_persistent(h, nprob, problems, p2, c, tile, false, res.data(), &ms, &used);
+      if (e) return e;
+      if (used == last_used) continue;  // capped to the same launch as the previous candidate
+      last_used = used;
+      if (pick == 0 || ms < best) {
+        best = ms;
+        pick = used;
+        pick_tile = tile;
+      }
     }
   }
   const std::vector<uint64_t> key = tune_key(pl);
+  const int shp = pick | (pick_tile << 8);
   bool found = false;
   for (auto& kv : h->bt_tuned)
     if (kv.first == key) {
-      kv.second = pick;
+      kv.second = shp;
       found = true;
     }
-  if (!found) h->bt_tuned.push_back({key, pick});
+  if (!found) h->bt_tuned.push_back({key, shp});
   *wg_per_cu = pick;
+  return 0;
+}
+
+int rbrp_batched_launch_shape(rbrp_handle_t h, int64_t nprob, const rbrp_problem_t* problems,
+                              const rbrp_opts_t* opts, int* wg_per_cu, int* tile_rows) {
+  if (!h) return -1;
+  BatchPlan pl;
+  {
+    int e = validate(nprob, problems, opts, &pl);
+    if (e) return e;
+  }
+  if (!wg_per_cu || !tile_rows) return -5;
+  const int shp = launch_shape(h, pl);
+  *wg_per_cu = shp & 0xff;
+  *tile_rows = shp >> 8;
   return 0;
 }
 

@@ -41,6 +41,7 @@ struct RbrpBatchShape {
   int filter;
   int max_rank;
   double tol2;
+  int tile;  // row tiles a wave takes at once in the fused projection + update (1, 2 or 4)
 };
 
 size_t rbrp_batched_lds_bytes(int n_max, int b);
@@ -49,9 +50,10 @@ size_t rbrp_batched_lds_bytes(int n_max, int b);
 // (and before recording a launch into a graph).
 hipError_t rbrp_batched_prepare(int n_max, int b);
 
-// Workgroups of the persistent kernel that fit on one compute unit at once
-// (registers and LDS for (n_max, b)); call after rbrp_batched_prepare.
-int rbrp_batched_max_blocks_per_cu(int n_max, int b);
+// Workgroups of the persistent kernel (the instantiation for `tile`) that fit
+// on one compute unit at once (registers and LDS for (n_max, b)); call after
+// rbrp_batched_prepare.
+int rbrp_batched_max_blocks_per_cu(int n_max, int b, int tile);
 
 // Resets the queue counter and launches the persistent kernel on `grid` workgroups.
 hipError_t rbrp_launch_colid_batched(hipStream_t st, int grid, const RbrpBatchProblem* probs,

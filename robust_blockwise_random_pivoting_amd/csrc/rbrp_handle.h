@@ -54,7 +54,8 @@ struct rbrp_handle_s {
   hipGraph_t bt_graph = nullptr;
   hipGraphExec_t bt_exec = nullptr;
   std::vector<uint64_t> bt_key;
-  std::vector<std::pair<std::vector<uint64_t>, int>> bt_tuned;  // shape key -> workgroups per CU
+  // shape key -> the tuner's launch shape, workgroups per CU | update row tiles << 8
+  std::vector<std::pair<std::vector<uint64_t>, int>> bt_tuned;
 };
 
 // Releases the batched-call resources of a handle (rbrp_batched.cpp).

@@ -278,8 +278,9 @@ def rbrp_colid_batched(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=No
 
 def batched_tune(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None, max_rank: int = 0,
                  filter: str = "sampled", handle: Optional[Handle] = None) -> int:
-    """Run the persistent batched launch once per candidate workgroups-per-CU and
-    keep the fastest on ``handle`` (returned)."""
+    """Run the persistent batched launch once per candidate launch shape (update
+    row tiles x workgroups per CU) and keep the fastest on ``handle``; returns
+    its workgroups per CU (``batched_launch_shape`` gives both)."""
     probs, opts, keep, perms, xbufs = _batch_setup(As, b, tol, seeds, offsets, max_rank, filter,
                                                    True)
     pick = ctypes.c_int(0)
@@ -293,6 +294,24 @@ def batched_tune(As, b: int = 16, tol: float = 0.0, seeds=None, offsets=None, ma
             h.close()
     del keep, perms, xbufs
     return int(pick.value)
+
+
+def batched_launch_shape(As, b: int = 16, tol: float = 0.0, max_rank: int = 0,
+                         filter: str = "sampled", handle: Optional[Handle] = None):
+    """``(workgroups per CU, update row tiles)`` that persistent calls for this
+    batch shape use on ``handle`` (the tuner's pick, or the default)."""
+    probs, opts, keep, perms, xbufs = _batch_setup(As, b, tol, None, None, max_rank, filter, True)
+    wg, tile = ctypes.c_int(0), ctypes.c_int(0)
+    own = handle is None
+    h = Handle() if own else handle
+    try:
+        _capi.check("rbrp_batched_launch_shape", h._lib.rbrp_batched_launch_shape(
+            h.ptr, len(As), probs, ctypes.byref(opts), ctypes.byref(wg), ctypes.byref(tile)))
+    finally:
+        if own:
+            h.close()
+    del keep, perms, xbufs
+    return int(wg.value), int(tile.value)
 
 
 def uniforms(seed: int, offset: int, draw0: int, count: int, handle: Optional[Handle] = None):

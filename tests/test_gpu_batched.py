@@ -157,8 +157,10 @@ def test_batched_max_rank_and_no_x():
 
 def test_batched_graph_replay_and_tuner():
     """A second call with the same launch shape replays the recorded graph with
-    the new descriptors; the tuner's choice gives the same results."""
-    from robust_blockwise_random_pivoting_amd import Handle, batched_tune, rbrp_colid_batched
+    the new descriptors; the tuner's choice (row tiles x workgroups per CU) gives
+    the same bits as the default shape."""
+    from robust_blockwise_random_pivoting_amd import (Handle, batched_launch_shape, batched_tune,
+                                                      rbrp_colid_batched)
     As = _kernel_batch(12, 31)
     dA = [_dev(A) for A in As]
     with Handle() as h:
@@ -173,11 +175,17 @@ def test_batched_graph_replay_and_tuner():
         assert (st3 == 0).all()
         for q, A in enumerate(As):
             _check_against_oracle(A, o3[q], 16, 1e-7, seed=seeds[q])
+        assert batched_launch_shape(dA, b=16, tol=1e-7, handle=h) == (4, 2)   # the default
         pick = batched_tune(dA, b=16, tol=1e-7, handle=h)
         assert pick in (1, 2, 3, 4, 6, 8)
+        wg, tile = batched_launch_shape(dA, b=16, tol=1e-7, handle=h)
+        assert wg == pick and tile in (1, 2, 4)
         o4, _ = rbrp_colid_batched(dA, b=16, tol=1e-7, mode="persistent", handle=h)
         for a, c in zip(o1, o4):
             assert a.k == c.k and np.array_equal(a.perm.cpu().numpy(), c.perm.cpu().numpy())
+            if a.X is not None and a.k < a.perm.shape[0]:
+                assert np.array_equal(a.X.cpu().numpy(), c.X.cpu().numpy())
+
 
 
 def test_batched_rejects_bad_arguments(librbrp):
