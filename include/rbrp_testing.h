@@ -82,6 +82,13 @@ int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int
                         int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
                         int32_t* status);
 
+/* rbrp_test_panel_inv on a chosen kernel: 0 the library's choice for b, 1 the
+ * one-wave kernel (b <= 64), 2 the register-tile workgroup kernel (b <= 128). */
+int rbrp_test_panel_kernel(void* stream, const double* G, int64_t s, int64_t b, int filter,
+                           int64_t i, const int32_t* J, int32_t* Jk, double* Rb, double* Rinv,
+                           int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
+                           int32_t* status, int kernel);
+
 /* Second Cholesky-QR pass of a block of bt kept columns (b >= bt the block
  * capacity): from G2 (device, bt x bt, ld bt) and the first factor Rb (device,
  * b x b, ld b) it writes R2 = chol(G2) and R2inv = R2^{-1} (device, b x b,
@@ -89,6 +96,10 @@ int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int
  * non-positive pivot. */
 int rbrp_test_chol2(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
                     double* R2, double* R2inv, double* Rout, int32_t* status);
+
+/* rbrp_test_chol2 on a chosen kernel (0, 1, 2 as for rbrp_test_panel_kernel). */
+int rbrp_test_chol2_kernel(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
+                           double* R2, double* R2inv, double* Rout, int32_t* status, int kernel);
 
 /* Sampler kernel alone: s = min(b, kmax - i, nnz(diags[i:n])) indices (absolute,
  * first-occurrence order) into J (device, capacity b); draws_in is the number

@@ -78,7 +78,12 @@ def _hook_signatures():
         "rbrp_test_panel_inv": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, vp, i32p,
                                         i32p, POINTER(c_int32), POINTER(c_int32),
                                         POINTER(c_int32)]),
+        "rbrp_test_panel_kernel": (c_int, [vp, vp, i64, i64, c_int, i64, i32p, i32p, vp, vp,
+                                           i32p, i32p, POINTER(c_int32), POINTER(c_int32),
+                                           POINTER(c_int32), c_int]),
         "rbrp_test_chol2": (c_int, [vp, vp, i64, i64, vp, vp, vp, vp, POINTER(c_int32)]),
+        "rbrp_test_chol2_kernel": (c_int, [vp, vp, i64, i64, vp, vp, vp, vp, POINTER(c_int32),
+                                           c_int]),
         "rbrp_test_sample": (c_int, [vp, vp, i64, i64, i64, i64, u64, u64, u64, i32p,
                                      POINTER(c_int32), POINTER(c_uint64)]),
         "rbrp_test_sample_time": (c_int, [vp, vp, i64, i64, u64, c_int, POINTER(c_double)]),

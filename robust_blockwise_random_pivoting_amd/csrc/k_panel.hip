@@ -1,6 +1,8 @@
-// One-wave panel kernels for blocks of at most 64 columns: the pivoted
-// Cholesky + robust filter of the sampled block's Gram matrix, and the second
-// Cholesky-QR pass (with R2^{-1} and R2 * Rb).
+// Panel kernels: the pivoted Cholesky + robust filter of the sampled block's
+// Gram matrix, and the second Cholesky-QR pass (with R2^{-1} and R2 * Rb).
+// Blocks of at most 64 columns run on one wave (below); wider blocks (up to
+// 128) on a 1024-thread workgroup with the Schur complement in registers
+// (k_pivchol_regs, k_chol2_regs further down).
 //
 // Lane c owns column c.  The factor is kept transposed in LDS, one row per
 // column (RT[c][t] = R(t, c), pitch PW_LDE), so that step j of the
@@ -572,6 +574,556 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   if (c < nb) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
   pw_upper_inverse_mfma(Ws, dinv, Gs, nb);
   pw_store_rows(Gs, nb, true, out + (int64_t)blockIdx.x * 4096, 64);
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup panel kernels for blocks of up to SMAX = 128 columns.
+//
+// The Schur complement of the s x s Gram matrix lives in registers: thread
+// (a, t) of the (SMAX / 4)^2 threads holds the 4 x 4 tile of rows 4a .. 4a+3
+// and columns 4t .. 4t+3.  Rows and columns keep their sampled order; the
+// dgeqp3 positions are a permutation only.  Step j of the right-looking
+// pivoted factorisation (l = S(P, :) / r_jj, S -= l l^T):
+//   1. every wave finds the pivot from its own copy of the Schur diagonal
+//      (lane q holds entries q and q + 64 with their positions): the largest
+//      value among the columns not yet chosen, the lowest position on ties;
+//   2. the threads holding row P write l (zero in the chosen columns and at P)
+//      to one of two LDS vectors, and row j of the factor (sampled column
+//      order) to LDS;
+//   3. one barrier; every thread updates its tile, every wave its copy of the
+//      diagonal (d -= l .* l).
+// Alternating the two l vectors makes one barrier per step enough: the vector
+// written at step j + 1 was last read before the barrier of step j.  The
+// arithmetic per entry (a division by r_jj, one fused multiply-add per step)
+// is that of the textbook outer-product Cholesky.
+// ---------------------------------------------------------------------------
+template <int SMAX>
+struct PrShape {
+  static constexpr int TPR = SMAX / 4;  // column tiles per row band
+  static constexpr int NT = TPR * TPR;  // threads
+  static constexpr int H = SMAX / 64;   // diagonal entries per lane
+};
+
+// The thread's 4 x 4 tile, element (q, e) at 4 q + e.  A vector value rather
+// than an array: an element picked by a uniform runtime index is read with
+// register-indexed moves, without branches and without leaving registers.
+typedef double v16d __attribute__((ext_vector_type(16)));
+
+// f(r, c) for every r, c < n (n <= SMAX): indices from the SMAX x n block by
+// shifts and masks (no division by n), r fastest (coalesced column-major
+// stores).
+template <int SMAX, typename Fn>
+__device__ __forceinline__ void pr_for_square(int n, Fn f) {
+  for (int idx = threadIdx.x; idx < SMAX * n; idx += blockDim.x) {
+    const int r = idx % SMAX, c = idx / SMAX;
+    if (r < n) f(r, c);
+  }
+}
+
+// The n x n column-major X (leading dimension ldx) into the register tiles,
+// staged through F (row r at F + r * ld) for coalesced loads; zero outside
+// n x n.  F is free again after the caller's next barrier.
+template <int SMAX>
+__device__ __forceinline__ v16d pr_load_tiles(const double* __restrict__ X, int64_t ldx, int n,
+                                              double* F, int ld) {
+  constexpr int NT = PrShape<SMAX>::NT, TPR = PrShape<SMAX>::TPR, E = SMAX * SMAX / NT;
+  const int tid = threadIdx.x, a = tid / TPR, t = tid % TPR;
+  double v[E];  // every load in flight at once
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int idx = tid + k * NT, r = idx % SMAX, c = idx / SMAX;
+    v[k] = X[min(r, n - 1) + (int64_t)min(c, n - 1) * ldx];  // clamped: no branches
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int idx = tid + k * NT, r = idx % SMAX, c = idx / SMAX;
+    if (r < n && c < n) F[r * ld + c] = v[k];
+  }
+  __syncthreads();
+  v16d T;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = 4 * a + q, c = 4 * t + e;
+      T[4 * q + e] = (r < n && c < n) ? F[r * ld + c] : 0.0;
+    }
+  return T;
+}
+
+// Wave-wide maximum of an unsigned value; every lane ends with the result.
+template <int W>
+__device__ __forceinline__ unsigned pr_swap_maxu(unsigned v) {
+  auto x = W == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false)
+                   : __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return max(v, max((unsigned)x[0], (unsigned)x[1]));
+}
+__device__ __forceinline__ unsigned pr_wave_maxu(unsigned v) {
+  v = max(v, (unsigned)pw_dpp<0x128>((int)v));  // row_ror:8
+  v = max(v, (unsigned)pw_dpp<0x124>((int)v));  // row_ror:4
+  v = max(v, (unsigned)pw_dpp<0x122>((int)v));  // row_ror:2
+  v = max(v, (unsigned)pw_dpp<0x121>((int)v));  // row_ror:1
+  v = pr_swap_maxu<16>(v);
+  return pr_swap_maxu<32>(v);
+}
+
+// T -= l(rows) l(cols)^T for the thread's tile.
+template <int SMAX>
+__device__ __forceinline__ v16d pr_rank1(const double* l, v16d T) {
+  constexpr int TPR = PrShape<SMAX>::TPR;
+  const int a = threadIdx.x / TPR, t = threadIdx.x % TPR;
+  double lr[4], lc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) lr[q] = l[4 * a + q];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lc[e] = l[4 * t + e];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) T[4 * q + e] = T[4 * q + e] - lr[q] * lc[e];
+  return T;
+}
+
+// X(r, c) of the inverse being built by pr_upper_inverse: X(r, c) for r < c
+// in F[c * ld + r], the diagonal in dinv, zero below the diagonal and outside
+// n x n.
+__device__ __forceinline__ double pr_xload(const double* F, int ld, const double* dinv, int n,
+                                           int r, int c) {
+  if (r >= n || c >= n || r > c) return 0.0;
+  return r < c ? F[c * ld + r] : dinv[r];
+}
+// F(r, c), zero outside n x n.
+__device__ __forceinline__ double pr_uload(const double* F, int ld, int n, int r, int c) {
+  return (r < n && c < n) ? F[r * ld + c] : 0.0;
+}
+
+// X = U^{-1} for the n x n upper-triangular U in the upper triangle of F (row
+// r at F + r * ld), n <= 128, written to Xo (column-major, ldx) with zeros
+// below the diagonal.  X(l, c), l < c, is built in F[c * ld + l] (the strictly
+// lower triangle, free once U is final) and 1 / U(c, c) in dinv.  The 16 x 16
+// diagonal blocks are inverted by back substitution, lane group g = tid / 16
+// on block g with lane j computing its column j; then, by distance d = J - I
+// from the diagonal, one wave per block,
+//   X(I, J) = -X(I, I) * sum_{K = I+1}^{J} U(I, K) X(K, J)
+// on the FP64 matrix core (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[4 v + g][j] in lane j + 16 g, element v, so the
+// sum's output is the second product's B operand as it stands).
+template <int SMAX>
+__device__ __forceinline__ void pr_upper_inverse(double* F, int ld, int n, double* dinv,
+                                                 double* __restrict__ Xo, int64_t ldx) {
+  const int tid = threadIdx.x, NT = blockDim.x;
+  for (int c = tid; c < n; c += NT) dinv[c] = 1.0 / F[c * ld + c];
+  __syncthreads();
+  const int nbk = (n + 15) >> 4;
+  const int lane = tid & 63, j = lane & 15, g = lane >> 4, nw = NT >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+  if ((tid >> 4) < nbk) {
+    const int R0 = 16 * (tid >> 4);
+    double x[16];
+#pragma unroll
+    for (int r = 15; r >= 0; --r) {
+      const int R = R0 + r;
+      double acc = (r == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int p = r + 1; p < 16; ++p) {
+        acc = fma(-pr_uload(F, ld, n, R, R0 + p), x[p], acc);
+      }
+      x[r] = (R < n && r <= j) ? acc * dinv[R] : 0.0;
+    }
+    if (R0 + j < n) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < j) F[(R0 + j) * ld + R0 + r] = x[r];
+    }
+  }
+  __syncthreads();
+  for (int d = 1; d < nbk; ++d) {
+    for (int I = w; I + d < nbk; I += nw) {  // wave-uniform
+      const int Jb = I + d;
+      // four accumulators (one per k step of a 16-deep product): dependent
+      // MFMAs wait out the whole pipeline, independent ones overlap
+      v4d y4[4];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) y4[s4] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int K = I + 1; K <= Jb; ++K) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int ra = 16 * I + j, ca = 16 * K + 4 * s4 + g;
+          av[s4] = pr_uload(F, ld, n, ra, ca);
+          bv[s4] = pr_xload(F, ld, dinv, n, ca, 16 * Jb + j);
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+          y4[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s4], bv[s4], y4[s4], 0, 0, 0);
+      }
+      const v4d y = (y4[0] + y4[1]) + (y4[2] + y4[3]);
+      v4d x4[4];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const double av = pr_xload(F, ld, dinv, n, 16 * I + j, 16 * I + 4 * s4 + g);
+        x4[s4] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, y[s4], (v4d){0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+      }
+      const v4d xo = (x4[0] + x4[1]) + (x4[2] + x4[3]);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int xr = 16 * I + 4 * v + g, xc = 16 * Jb + j;
+        if (xr < n && xc < n) F[xc * ld + xr] = -xo[v];
+      }
+    }
+    __syncthreads();
+  }
+  pr_for_square<SMAX>(n, [&](int r, int c) {
+    Xo[r + (int64_t)c * ldx] = (r < c) ? F[c * ld + r] : ((r == c) ? dinv[c] : 0.0);
+  });
+}
+
+// Pivoted Cholesky of the sampled block's Gram matrix with the robust filter
+// (the outputs of k_pivchol_wave, for s <= SMAX).
+template <int SMAX>
+__global__ __launch_bounds__(PrShape<SMAX>::NT) void k_pivchol_regs(
+    const double* __restrict__ G, int64_t ldg, int b, int filter_mode, int64_t i,
+    const int* __restrict__ J, RbrpBlockState* bs, double* __restrict__ Rb,
+    double* __restrict__ Rinv, int* __restrict__ Jk, int* __restrict__ Jin,
+    int* __restrict__ Jout, int* __restrict__ Jkpos) {
+  constexpr int NT = PrShape<SMAX>::NT, TPR = PrShape<SMAX>::TPR, H = PrShape<SMAX>::H;
+  extern __shared__ double F[];  // s x (s + 1): the Gram matrix, then the factor
+  __shared__ double lv[2][SMAX];
+  __shared__ double sqr[SMAX], dinv[SMAX], dpv[SMAX];
+  __shared__ int piv[SMAX], sj[SMAX], sorted[SMAX], flag[SMAX];
+  __shared__ int s_bt;
+  const int tid = threadIdx.x, lane = tid & 63, a = tid / TPR, t = tid % TPR;
+  const int s = bs->s;
+  if (s <= 0) {
+    if (tid == 0) {
+      bs->bt = 0;
+      bs->nswap = 0;
+    }
+    return;
+  }
+  const int ld = s + 1;
+  v16d T = pr_load_tiles<SMAX>(G, ldg, s, F, ld);
+  double dg[H];
+  int ps[H];  // position of column lane + 64 h (-1 beyond s)
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int c = lane + 64 * h;
+    dg[h] = c < s ? F[c * ld + c] : 0.0;
+    ps[h] = c < s ? c : -1;
+  }
+  __syncthreads();  // F receives factor rows from here on
+  unsigned chosen = 0;  // bit e: column 4t + e is a pivot
+  int rank = s;
+  for (int j = 0; j < s; ++j) {
+    // The pivot: the candidates' values are positive doubles, whose bit
+    // patterns order like the values, so the wave maximum runs on the high
+    // words and then on the low words of the lanes that hold the high maximum.
+    uint64_t kb = 0;  // bits of the lane's best candidate (0: none)
+    int lp = 0x7fffffff, lc = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const uint64_t k = (ps[h] >= j && dg[h] > 0.0) ? (uint64_t)__double_as_longlong(dg[h]) : 0;
+      if (k > kb || (k == kb && k != 0 && ps[h] < lp)) {
+        kb = k;
+        lp = ps[h];
+        lc = lane + 64 * h;
+      }
+    }
+    const unsigned hi = (unsigned)(kb >> 32), lo = (unsigned)kb;
+    const unsigned mh = pr_wave_maxu(hi);
+    const unsigned ml = pr_wave_maxu(hi == mh ? lo : 0u);
+    if (mh == 0 && ml == 0) {  // no positive diagonal left (uniform: all waves agree)
+      rank = j;
+      break;
+    }
+    const bool top = hi == mh && lo == ml;
+    const uint64_t ties = __builtin_amdgcn_ballot_w64(top);
+    int P, bp;
+    if (__builtin_popcountll(ties) == 1) {
+      const int L = (int)__builtin_ctzll(ties);
+      bp = __builtin_amdgcn_readlane(lp, L);
+      P = __builtin_amdgcn_readlane(lc, L);
+    } else {
+      bp = pw_wave_min(top ? lp : 0x7fffffff);
+      const int L = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(top && lp == bp));
+      P = __builtin_amdgcn_readlane(lc, L);
+    }
+    const double m = __longlong_as_double((long long)(((uint64_t)mh << 32) | ml));
+    double* l = lv[j & 1];
+    if (a == (P >> 2)) {
+      const int q = P & 3;
+      const double inv = pw_rsqrt(m);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = 4 * t + e;
+        const bool zero = c >= s || ((chosen >> e) & 1u) || c == P;
+        const double x = T[4 * q + e] * inv;
+        const double v = zero ? 0.0 : x;
+        l[c] = v;
+        if (c < s) F[j * ld + c] = v;  // (the diagonal entry is set after the loop)
+        if (c == P) dpv[j] = m;
+      }
+    }
+    __syncthreads();
+    T = pr_rank1<SMAX>(l, T);
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const double x = l[lane + 64 * h];
+      dg[h] = dg[h] - x * x;
+      // the column at position j moves to the pivot's position, the pivot to j
+      if (ps[h] == j) ps[h] = bp;
+      if (lane + 64 * h == P) ps[h] = j;
+    }
+    if ((P >> 2) == t) chosen |= 1u << (P & 3);
+  }
+  if (tid < 64) {
+#pragma unroll
+    for (int h = 0; h < H; ++h)
+      if (lane + 64 * h < s) piv[ps[h]] = lane + 64 * h;
+  }
+  pr_for_square<SMAX>(s, [&](int r, int c) {  // rows at and beyond the rank
+    if (r >= rank) F[r * ld + c] = 0.0;
+  });
+  __syncthreads();
+  for (int u = tid; u < rank; u += NT) F[u * ld + piv[u]] = sqrt(dpv[u]);  // R(j, j)
+  __syncthreads();
+  // columns to position order: R(r, c) = F(r, piv[c]) for c >= r, zero below
+  {
+    constexpr int E = SMAX * SMAX / NT;
+    double v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int idx = tid + k * NT, r = idx / SMAX, c = idx % SMAX;
+      v[k] = (r < s && c < s && c >= r) ? F[r * ld + piv[c]] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int idx = tid + k * NT, r = idx / SMAX, c = idx % SMAX;
+      if (r < s && c < s) F[r * ld + c] = v[k];
+    }
+    __syncthreads();
+  }
+  // filter (gpu/rbrp.cu:193-207): row energies, csum from the last row up
+  for (int r = tid; r < s; r += NT) {
+    double acc = 0.0;
+#pragma unroll 8
+    for (int c = r; c < s; ++c) acc += F[r * ld + c] * F[r * ld + c];
+    sqr[r] = acc;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // sums in serial order from the last row up, loads in batches of eight
+    double total = 0.0;
+    for (int u0 = 0; u0 < s; u0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = sqr[max(s - 1 - u0 - u, 0)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u0 + u < s) total += v[u];
+    }
+    const double thr = total / (double)(filter_mode == 0 ? s : b);
+    double csum = 0.0;
+    int k = s;
+    for (int u0 = 0; u0 < s && k == s; u0 += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = sqr[max(s - 1 - u0 - u, 0)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (k == s && u0 + u < s) {
+          csum += v[u];
+          if (csum >= thr) k = u0 + u;
+        }
+      }
+    }
+    int bt = s - k;
+    if (rank == 0 || !(total > 0.0)) {
+      bt = 0;
+      bs->status = 1;
+    }
+    if (bt > rank) bt = rank;
+    s_bt = bt;
+  }
+  __syncthreads();
+  const int bt = s_bt;
+  for (int u = tid; u < bt; u += NT) {
+    const int jv = J[piv[u]];
+    Jk[u] = jv;
+    sj[u] = jv;
+    if (Jkpos) Jkpos[u] = piv[u];
+  }
+  pr_for_square<SMAX>(bt, [&](int r, int c) {
+    Rb[r + (int64_t)c * b] = (r <= c) ? F[r * ld + c] : 0.0;
+  });
+  __syncthreads();
+  pr_upper_inverse<SMAX>(F, ld, bt, dinv, Rinv, b);
+  // ascending sort of Jk by rank counting (entries are distinct); Jout flags
+  for (int u = tid; u < bt; u += NT) {
+    const int v = sj[u];
+    int rk = 0;
+    for (int q = 0; q < bt; ++q) rk += (sj[q] < v) ? 1 : 0;
+    sorted[rk] = v;
+    const int q0 = (int)(i + u);
+    int member = 0;
+    for (int q = 0; q < bt; ++q) member |= (sj[q] == q0) ? 1 : 0;
+    flag[u] = !member;
+  }
+  __syncthreads();
+  int nin = 0;
+  for (int u = 0; u < bt; ++u) nin += ((int64_t)sorted[u] >= i + bt) ? 1 : 0;
+  for (int u = tid; u < nin; u += NT) Jin[u] = sorted[bt - nin + u];
+  for (int u = tid; u < bt; u += NT) {
+    if (flag[u]) {
+      int p = 0;
+      for (int q = 0; q < u; ++q) p += flag[q];
+      Jout[p] = (int)(i + u);
+    }
+  }
+  if (tid == 0) {
+    int nout = 0;
+    for (int u = 0; u < bt; ++u) nout += flag[u];
+    if (nin != nout) bs->status = 3;
+    bs->bt = bt;
+    bs->nswap = nin;
+  }
+}
+
+// Second Cholesky-QR pass (the outputs of k_chol2_wave, for bt <= SMAX):
+// G2 = R2^T R2 without pivoting, R2inv = R2^{-1}, Rout = R2 * Rb.
+template <int SMAX>
+__global__ __launch_bounds__(PrShape<SMAX>::NT) void k_chol2_regs(
+    const double* __restrict__ G2, int64_t ldg, int b, RbrpBlockState* bs,
+    const double* __restrict__ Rb, double* __restrict__ R2, double* __restrict__ R2inv,
+    double* __restrict__ Rout, int64_t ldr) {
+  constexpr int NT = PrShape<SMAX>::NT, TPR = PrShape<SMAX>::TPR;
+  extern __shared__ double F[];  // bt x (bt + 1): G2, then R2 and its inverse
+  __shared__ double lv[2][SMAX];
+  __shared__ double dinv[SMAX], dpv[SMAX];
+  __shared__ int s_bad;
+  const int tid = threadIdx.x, a = tid / TPR, t = tid % TPR;
+  const int bt = bs->bt;
+  if (bt <= 0) return;
+  const int ld = bt + 1;
+  v16d T = pr_load_tiles<SMAX>(G2, ldg, bt, F, ld);
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  for (int j = 0; j < bt; ++j) {
+    const int A = j >> 2, q = j & 3;
+    // S(j, j) from the thread of tile (A, A): lane (A * TPR + A) % 64 of the
+    // wave that holds row band A (the other waves read a value they ignore)
+    const double d = pw_readlane(T[5 * q], (A * TPR + A) & 63);
+    double* l = lv[j & 1];
+    if (a == A) {
+      const double dd = d > 0.0 ? d : 1.0;
+      const double inv = pw_rsqrt(dd);
+      if (t == A) {
+        dpv[j] = dd;
+        if (!(d > 0.0)) s_bad = 1;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = 4 * t + e;
+        const double x = T[4 * q + e] * inv;
+        const double v = (c <= j || c >= bt) ? 0.0 : x;
+        l[c] = v;
+        if (c < bt) F[j * ld + c] = v;  // (the diagonal entry is set after the loop)
+      }
+    }
+    __syncthreads();
+    T = pr_rank1<SMAX>(l, T);
+  }
+  for (int u = tid; u < bt; u += NT) F[u * ld + u] = sqrt(dpv[u]);
+  __syncthreads();
+  if (tid == 0 && s_bad) bs->status = 2;
+  pr_for_square<SMAX>(bt, [&](int r, int c) {
+    R2[r + (int64_t)c * b] = (r <= c) ? F[r * ld + c] : 0.0;
+  });
+  // Rout = R2 * Rb on the matrix core: block (I, J) = sum_{K=I}^{J} R2(I, K)
+  // Rb(K, J).  Rb's upper triangle is staged in F's free strictly lower
+  // triangle, transposed (Rb(r, c), r < c, at F[c * ld + r]), its diagonal in
+  // dpv (R2's diagonal is final in F).  One wave per block.
+  {
+    __syncthreads();  // dpv (R2's pivots) consumed above
+    {
+      constexpr int E = SMAX * SMAX / NT;
+      double v[E];  // every load in flight at once (clamped addresses)
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int idx = tid + k * NT, r = idx % SMAX, c = idx / SMAX;
+        v[k] = Rb[min(r, bt - 1) + (int64_t)min(c, bt - 1) * b];
+      }
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int idx = tid + k * NT, r = idx % SMAX, c = idx / SMAX;
+        if (c < bt && r < c) F[c * ld + r] = v[k];
+        if (c < bt && r == c) dpv[r] = v[k];
+      }
+    }
+    __syncthreads();
+    const int lane = tid & 63, j = lane & 15, g = lane >> 4, nw = NT >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nbk = (bt + 15) >> 4;
+    for (int blk = w; blk < nbk * nbk; blk += nw) {  // wave-uniform
+      const int I = blk / nbk, Jb = blk % nbk;
+      v4d a4[2];
+      a4[0] = a4[1] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int K = I; K <= Jb; ++K) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int ka = 16 * K + 4 * s4 + g;
+          const double u = pr_uload(F, ld, bt, 16 * I + j, ka);  // R2(I, K)
+          av[s4] = (16 * I + j <= ka) ? u : 0.0;  // (F's lower triangle holds Rb)
+          bv[s4] = pr_xload(F, ld, dpv, bt, ka, 16 * Jb + j);  // Rb(K, J), staged
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4)
+          a4[s4 & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s4], bv[s4], a4[s4 & 1], 0, 0, 0);
+      }
+      const v4d acc = a4[0] + a4[1];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * I + 4 * v + g, c = 16 * Jb + j;
+        if (r < bt && c < bt) Rout[r + (int64_t)c * ldr] = acc[v];
+      }
+    }
+  }
+  __syncthreads();  // the inverse overwrites F's lower triangle, read above
+  pr_upper_inverse<SMAX>(F, ld, bt, dinv, R2inv, b);
+}
+
+template <int SMAX>
+static hipError_t pr_allow(const void* fn, std::atomic<int>* cache) {
+  return rbrp_allow_lds(fn, SMAX * (SMAX + 1) * (int)sizeof(double), cache);
+}
+
+hipError_t rbrp_launch_pivchol_regs(hipStream_t st, const double* G, int64_t ldg, int b,
+                                    int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                    double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                    int* Jkpos) {
+  if (b < 1 || b > 128) return hipErrorInvalidValue;
+  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
+  static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+  hipError_t e = pr_allow<128>((const void*)k_pivchol_regs<128>, lds_set);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pivchol_regs<128>, dim3(1), dim3(PrShape<128>::NT), lds, st, G, ldg, b,
+                     filter_mode, i, J, bs, Rb, Rinv, Jk, Jin, Jout, Jkpos);
+  return hipGetLastError();
+}
+
+hipError_t rbrp_launch_chol2_regs(hipStream_t st, const double* G2, int64_t ldg, int b,
+                                  RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
+                                  double* Rout, int64_t ldr) {
+  if (b < 1 || b > 128) return hipErrorInvalidValue;
+  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
+  static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
+  hipError_t e = pr_allow<128>((const void*)k_chol2_regs<128>, lds_set);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_chol2_regs<128>, dim3(1), dim3(PrShape<128>::NT), lds, st, G2, ldg, b, bs,
+                     Rb, R2, R2inv, Rout, ldr);
+  return hipGetLastError();
 }
 
 static size_t pw_lds_bytes() { return (size_t)PW_LDS_DOUBLES * sizeof(double); }

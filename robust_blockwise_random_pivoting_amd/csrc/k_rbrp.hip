@@ -1035,8 +1035,8 @@ hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
 }
 
 // ---------------------------------------------------------------------------
-// Pivoted Cholesky of the sampled block's Gram matrix G = V^T V (s x s, LDS
-// resident) + robust filter.
+// Pivoted Cholesky of the sampled block's Gram matrix G = V^T V (s x s) +
+// robust filter.
 //
 // The pivot at step j is the first index of the largest trailing Schur
 // diagonal, i.e. the column of V with the largest residual norm after
@@ -1053,333 +1053,29 @@ hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
 // Outputs: Jk = J[piv[0:bt]] (pivot order), Rb = R[0:bt][0:bt], its inverse,
 // and the ascending symmetric differences Jin = sort(Jk) \ [i, i+bt),
 // Jout = [i, i+bt) \ Jk (gpu/rbrp.cu:211-234).
+//
+// Blocks of at most 64 columns run the one-wave kernels (lane = column,
+// k_panel.hip); larger ones the workgroup kernels with the Schur complement in
+// registers.
 // ---------------------------------------------------------------------------
-// Upper-triangular inverse of the bt x bt factor held in the upper triangle of
-// S (leading dimension ld), computed by rows from the bottom up: row r of the
-// inverse needs only rows r+1.. of it.  Inverse entry (l, c), l < c, is kept
-// in S[c][l] (the strictly-lower triangle, free once the factorisation is
-// done) and the diagonal in dinv; the result is written to Rinv (ld ldi) with
-// zeros below the diagonal.
-__device__ inline void upper_inverse_lds(double* S, int ld, int bt, double* dinv, double* red,
-                                         double* __restrict__ Rinv, int64_t ldi) {
-  // red: workspace of (blockDim.x / 64) x 128 doubles.  Row r of the inverse is
-  // computed by all waves: wave w sums the terms l = r+1+w, r+1+w+nw, ...,
-  // then wave 0 adds the nw partials in wave order.
-  const int tid = threadIdx.x, NT = blockDim.x, nw = NT >> 6, w = tid >> 6, lane = tid & 63;
-  for (int c = tid; c < bt; c += NT) dinv[c] = 1.0 / S[c * ld + c];
-  __syncthreads();
-  for (int r = bt - 2; r >= 0; --r) {
-    for (int cc = 0; cc * 64 < bt; ++cc) {
-      const int c = lane + 64 * cc;
-      double acc = 0.0;
-      if (c > r && c < bt) {
-#pragma unroll 4
-        for (int l = r + 1 + w; l <= c; l += nw)
-          acc += S[r * ld + l] * ((l == c) ? dinv[c] : S[c * ld + l]);
-      }
-      red[w * 128 + c] = acc;
-    }
-    __syncthreads();
-    if (w == 0) {
-      for (int cc = 0; cc * 64 < bt; ++cc) {
-        const int c = lane + 64 * cc;
-        if (c > r && c < bt) {
-          double a = 0.0;
-          for (int q = 0; q < nw; ++q) a += red[q * 128 + c];
-          S[c * ld + r] = -a * dinv[r];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < bt * bt; idx += NT) {
-    const int r = idx % bt, c = idx / bt;
-    Rinv[r + (int64_t)c * ldi] = (r < c) ? S[c * ld + r] : ((r == c) ? dinv[c] : 0.0);
-  }
-}
-
-// Rank-1 update of the trailing part of the symmetric LDS matrix S after step
-// j: S[r][c] -= S[j][r] * S[j][c] for j < r, c < s.  Lane = column (two
-// passes of 64 columns when s > 64), the waves take interleaved rows, four
-// rows per batch of loads.
-__device__ inline void trailing_rank1_lds(double* S, int ld, int j, int s) {
-  const int tid = threadIdx.x, nw = blockDim.x >> 6, w = tid >> 6, lane = tid & 63;
-  for (int cc = 0; cc * 64 < s; ++cc) {
-    const int c = lane + 64 * cc;
-    if (c <= j || c >= s) continue;
-    const double rc = S[j * ld + c];
-    int r = j + 1 + w;
-    for (; r + 3 * nw < s; r += 4 * nw) {
-      const double a0 = S[j * ld + r], a1 = S[j * ld + r + nw];
-      const double a2 = S[j * ld + r + 2 * nw], a3 = S[j * ld + r + 3 * nw];
-      const double b0 = S[r * ld + c], b1 = S[(r + nw) * ld + c];
-      const double b2 = S[(r + 2 * nw) * ld + c], b3 = S[(r + 3 * nw) * ld + c];
-      S[r * ld + c] = b0 - a0 * rc;
-      S[(r + nw) * ld + c] = b1 - a1 * rc;
-      S[(r + 2 * nw) * ld + c] = b2 - a2 * rc;
-      S[(r + 3 * nw) * ld + c] = b3 - a3 * rc;
-    }
-    for (; r < s; r += nw) S[r * ld + c] -= S[j * ld + r] * rc;
-  }
-}
-
-// Wave-level argmax of S[c][c] over c in [j, s): largest value, lowest index on ties.
-__device__ inline void wave_argmax_diag(const double* S, int ld, int j, int s, double* best_out,
-                                        int* idx_out) {
-  const int lane = threadIdx.x & 63;
-  double bv = -1.0;
-  int bi = 0x7fffffff;
-  for (int c = j + lane; c < s; c += 64) {
-    const double d = S[c * ld + c];
-    if (d > bv || (d == bv && c < bi)) {
-      bv = d;
-      bi = c;
-    }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double ov = __shfl_xor(bv, o, 64);
-    const int oi = __shfl_xor(bi, o, 64);
-    if (ov > bv || (ov == bv && oi < bi)) {
-      bv = ov;
-      bi = oi;
-    }
-  }
-  *best_out = bv;
-  *idx_out = bi;
-}
-
-__global__ __launch_bounds__(256) void k_pivchol_filter(const double* __restrict__ G, int64_t ldg,
-                                                        int b, int filter_mode, int64_t i,
-                                                        const int* __restrict__ J,
-                                                        RbrpBlockState* bs, double* __restrict__ Rb,
-                                                        double* __restrict__ Rinv,
-                                                        int* __restrict__ Jk, int* __restrict__ Jin,
-                                                        int* __restrict__ Jout,
-                                                        int* __restrict__ Jkpos) {
-  extern __shared__ double S[];  // s x (s + 1)
-  __shared__ int piv[SAMPLE_MAXB];
-  __shared__ int sj[SAMPLE_MAXB];
-  __shared__ int sorted[SAMPLE_MAXB];
-  __shared__ int flag[SAMPLE_MAXB];
-  __shared__ double sqr[SAMPLE_MAXB];
-  __shared__ double dinv[SAMPLE_MAXB];
-  __shared__ double red[4 * 128];
-  __shared__ int s_pvt, s_stop, s_bt;
-  __shared__ double s_rjj;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  const int s = bs->s;
-  if (s <= 0) {
-    if (tid == 0) {
-      bs->bt = 0;
-      bs->nswap = 0;
-    }
-    return;
-  }
-  const int ld = s + 1;
-  for (int idx = tid; idx < s * s; idx += NT) {
-    const int r = idx % s, c = idx / s;
-    S[r * ld + c] = G[r + (int64_t)c * ldg];
-  }
-  for (int t = tid; t < s; t += NT) piv[t] = t;
-  __syncthreads();
-  int rank = s;
-  for (int j = 0; j < s; ++j) {
-    if (tid < 64) {
-      double best;
-      int p;
-      wave_argmax_diag(S, ld, j, s, &best, &p);
-      if (tid == 0) {
-        s_pvt = p;
-        s_stop = (best > 0.0) ? 0 : 1;
-        s_rjj = (best > 0.0) ? sqrt(best) : 0.0;
-      }
-    }
-    __syncthreads();
-    if (s_stop) {
-      rank = j;
-      break;
-    }
-    const int p = s_pvt;
-    if (p != j) {
-      for (int q = tid; q < s; q += NT) {
-        const double t0 = S[q * ld + j];
-        S[q * ld + j] = S[q * ld + p];
-        S[q * ld + p] = t0;
-      }
-      __syncthreads();
-      for (int c = j + tid; c < s; c += NT) {
-        const double t0 = S[j * ld + c];
-        S[j * ld + c] = S[p * ld + c];
-        S[p * ld + c] = t0;
-      }
-      if (tid == 0) {
-        const int t0 = piv[j];
-        piv[j] = piv[p];
-        piv[p] = t0;
-      }
-      __syncthreads();
-    }
-    const double rjj = s_rjj;
-    for (int c = j + tid; c < s; c += NT) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
-    __syncthreads();
-    trailing_rank1_lds(S, ld, j, s);
-    __syncthreads();
-  }
-  // rows at and beyond the numerical rank are zero
-  for (int idx = tid; idx < s * s; idx += NT) {
-    const int r = idx / s, c = idx % s;
-    if (r >= rank && c >= r) S[r * ld + c] = 0.0;
-  }
-  __syncthreads();
-  for (int r = tid; r < s; r += NT) {
-    double acc = 0.0;
-    for (int c = r; c < s; ++c) acc += S[r * ld + c] * S[r * ld + c];
-    sqr[r] = acc;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    // csum[t] = sqr[s-1] + ... + sqr[s-1-t] accumulated in that order
-    double total = 0.0;
-    for (int t = 0; t < s; ++t) total += sqr[s - 1 - t];
-    const double thr = total / (double)(filter_mode == 0 ? s : b);
-    double csum = 0.0;
-    int k = 0;
-    for (; k < s; ++k) {
-      csum += sqr[s - 1 - k];
-      if (csum >= thr) break;
-    }
-    int bt = s - k;
-    if (rank == 0 || !(total > 0.0)) {
-      bt = 0;
-      bs->status = 1;
-    }
-    if (bt > rank) bt = rank;
-    s_bt = bt;
-  }
-  __syncthreads();
-  const int bt = s_bt;
-  for (int t = tid; t < bt; t += NT) {
-    Jk[t] = J[piv[t]];
-    sj[t] = J[piv[t]];
-    if (Jkpos) Jkpos[t] = piv[t];
-  }
-  for (int idx = tid; idx < bt * bt; idx += NT) {
-    const int r = idx % bt, c = idx / bt;
-    Rb[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
-  }
-  __syncthreads();
-  upper_inverse_lds(S, ld, bt, dinv, red, Rinv, b);
-  // ascending sort of Jk by rank counting (entries are distinct)
-  for (int t = tid; t < bt; t += NT) {
-    const int v = sj[t];
-    int rk = 0;
-    for (int q = 0; q < bt; ++q) rk += (sj[q] < v) ? 1 : 0;
-    sorted[rk] = v;
-  }
-  // Jout membership flags for the block positions i .. i+bt-1
-  for (int t = tid; t < bt; t += NT) {
-    const int q = (int)(i + t);
-    int member = 0;
-    for (int a = 0; a < bt; ++a) member |= (sj[a] == q) ? 1 : 0;
-    flag[t] = !member;
-  }
-  __syncthreads();
-  int nin = 0;
-  for (int a = 0; a < bt; ++a) nin += ((int64_t)sorted[a] >= i + bt) ? 1 : 0;
-  for (int t = tid; t < nin; t += NT) Jin[t] = sorted[bt - nin + t];
-  for (int t = tid; t < bt; t += NT) {
-    if (flag[t]) {
-      int pos = 0;
-      for (int q = 0; q < t; ++q) pos += flag[q];
-      Jout[pos] = (int)(i + t);
-    }
-  }
-  if (tid == 0) {
-    int nout = 0;
-    for (int t = 0; t < bt; ++t) nout += flag[t];
-    if (nin != nout) bs->status = 3;
-    bs->bt = bt;
-    bs->nswap = nin;
-  }
-}
-
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
                                       int* Jkpos) {
-  if (b > SAMPLE_MAXB) return hipErrorInvalidValue;
-  if (b <= 64)  // one-wave kernel
+  if (b <= 64)
     return rbrp_launch_pivchol_wave(st, G, ldg, b, filter_mode, i, J, bs, Rb, Rinv, Jk, Jin, Jout,
                                     Jkpos);
-  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
-  if (lds > 48 * 1024) {
-    static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
-    hipError_t e = rbrp_allow_lds((const void*)k_pivchol_filter, (int)lds, lds_set);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_pivchol_filter, dim3(1), dim3(256), lds, st, G, ldg, b, filter_mode, i, J, bs,
-                     Rb, Rinv, Jk, Jin, Jout, Jkpos);
-  return hipGetLastError();
+  return rbrp_launch_pivchol_regs(st, G, ldg, b, filter_mode, i, J, bs, Rb, Rinv, Jk, Jin, Jout,
+                                  Jkpos);
 }
 
-// ---------------------------------------------------------------------------
 // Second Cholesky-QR pass: G2 = Q1^T Q1 = R2^T R2.  Writes R2 (b x b, zeros
-// below the diagonal) and R2inv = R2^{-1}; the caller forms R2 * Rb.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_chol2(const double* __restrict__ G2, int64_t ldg, int b,
-                                               RbrpBlockState* bs, double* __restrict__ R2,
-                                               double* __restrict__ R2inv) {
-  extern __shared__ double S[];
-  __shared__ double dinv[SAMPLE_MAXB];
-  __shared__ double red[4 * 128];
-  __shared__ int s_bad;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  const int bt = bs->bt;
-  if (bt <= 0) return;
-  const int ld = bt + 1;
-  for (int idx = tid; idx < bt * bt; idx += NT) {
-    const int r = idx % bt, c = idx / bt;
-    S[r * ld + c] = G2[r + (int64_t)c * ldg];
-  }
-  if (tid == 0) s_bad = 0;
-  __syncthreads();
-  for (int j = 0; j < bt; ++j) {
-    const double d = S[j * ld + j];
-    const double rjj = sqrt(d > 0.0 ? d : 1.0);
-    __syncthreads();
-    if (tid == 0 && !(d > 0.0)) s_bad = 1;
-    for (int c = j + tid; c < bt; c += NT) S[j * ld + c] = (c == j) ? rjj : S[j * ld + c] / rjj;
-    __syncthreads();
-    trailing_rank1_lds(S, ld, j, bt);
-    __syncthreads();
-  }
-  if (tid == 0 && s_bad) bs->status = 2;
-  for (int idx = tid; idx < bt * bt; idx += NT) {
-    const int r = idx % bt, c = idx / bt;
-    R2[r + (int64_t)c * b] = (r <= c) ? S[r * ld + c] : 0.0;
-  }
-  __syncthreads();
-  upper_inverse_lds(S, ld, bt, dinv, red, R2inv, b);
-}
-
+// below the diagonal), R2inv = R2^{-1} and Rout = R2 * Rb.
 hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int b,
                              RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
                              double* Rout, int64_t ldr) {
   if (b <= 64) return rbrp_launch_chol2_wave(st, G2, ldg, b, bs, Rb, R2, R2inv, Rout, ldr);
-  const size_t lds = (size_t)b * (size_t)(b + 1) * sizeof(double);
-  if (lds > 48 * 1024) {
-    static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
-    hipError_t e = rbrp_allow_lds((const void*)k_chol2, (int)lds, lds_set);
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_chol2, dim3(1), dim3(256), lds, st, G2, ldg, b, bs, R2, R2inv);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  // Rout = R2 * Rb (product of two upper-triangular factors)
-  const int* d_bt = &bs->bt;
-  return rbrp_launch_gemm_nn(st, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
-                             mat(R2, b), mat((double*)Rb, b), mat(Rout, ldr), GEMM_SET, nullptr, 0);
+  return rbrp_launch_chol2_regs(st, G2, ldg, b, bs, Rb, R2, R2inv, Rout, ldr);
 }
 
 // ---------------------------------------------------------------------------

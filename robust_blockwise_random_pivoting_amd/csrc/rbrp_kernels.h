@@ -215,6 +215,14 @@ hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg
 hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg, int b,
                                   RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
                                   double* Rout, int64_t ldr);
+// Workgroup forms (b <= 128, k_panel.hip): the Schur complement in registers.
+hipError_t rbrp_launch_pivchol_regs(hipStream_t st, const double* G, int64_t ldg, int b,
+                                    int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
+                                    double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
+                                    int* Jkpos);
+hipError_t rbrp_launch_chol2_regs(hipStream_t st, const double* G2, int64_t ldg, int b,
+                                  RbrpBlockState* bs, const double* Rb, double* R2, double* R2inv,
+                                  double* Rout, int64_t ldr);
 // Inverses of the diagonal blocks of the k x k upper-triangular R in the
 // partition of the blocked back substitution (256-row super-blocks from the
 // bottom, 64-row blocks within): 4 * ceil(k / 256) blocks of 64 x 64 doubles

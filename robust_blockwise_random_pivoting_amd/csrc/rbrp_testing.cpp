@@ -157,8 +157,17 @@ int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int
                         int64_t i, const int32_t* J, int32_t* Jk, double* Rb, double* Rinv_out,
                         int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
                         int32_t* status) {
+  return rbrp_test_panel_kernel(stream, G, s, b, filter, i, J, Jk, Rb, Rinv_out, Jin, Jout, bt,
+                                nswap, status, 0);
+}
+
+int rbrp_test_panel_kernel(void* stream, const double* G, int64_t s, int64_t b, int filter,
+                           int64_t i, const int32_t* J, int32_t* Jk, double* Rb, double* Rinv_out,
+                           int32_t* Jin, int32_t* Jout, int32_t* bt, int32_t* nswap,
+                           int32_t* status, int kernel) {
   if (s < 1 || s > b) return -3;
-  if (b < 1 || b > 128) return -4;
+  if (b < 1 || b > 128 || (kernel == 1 && b > 64)) return -4;
+  if (kernel < 0 || kernel > 2) return -5;
   hipStream_t st = (hipStream_t)stream;
   RbrpBlockState* d = nullptr;
   double* Rinv = nullptr;
@@ -168,7 +177,12 @@ int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int
   hs.s = (int)s;
   if (e == hipSuccess) e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
-    e = rbrp_launch_pivchol_filter(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk, Jin, Jout);
+    e = kernel == 1   ? rbrp_launch_pivchol_wave(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk,
+                                                 Jin, Jout, nullptr)
+        : kernel == 2 ? rbrp_launch_pivchol_regs(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk,
+                                                 Jin, Jout, nullptr)
+                      : rbrp_launch_pivchol_filter(st, G, s, (int)b, filter, i, J, d, Rb, Rinv, Jk,
+                                                   Jin, Jout);
   if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess && Rinv_out)
     e = hipMemcpyAsync(Rinv_out, Rinv, sizeof(double) * b * b, hipMemcpyDeviceToDevice, st);
@@ -184,8 +198,14 @@ int rbrp_test_panel_inv(void* stream, const double* G, int64_t s, int64_t b, int
 
 int rbrp_test_chol2(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
                     double* R2, double* R2inv, double* Rout, int32_t* status) {
+  return rbrp_test_chol2_kernel(stream, G2, bt, b, Rb, R2, R2inv, Rout, status, 0);
+}
+
+int rbrp_test_chol2_kernel(void* stream, const double* G2, int64_t bt, int64_t b, const double* Rb,
+                           double* R2, double* R2inv, double* Rout, int32_t* status, int kernel) {
   if (bt < 1 || bt > b) return -3;
-  if (b < 1 || b > 128) return -4;
+  if (b < 1 || b > 128 || (kernel == 1 && b > 64)) return -4;
+  if (kernel < 0 || kernel > 2) return -5;
   hipStream_t st = (hipStream_t)stream;
   RbrpBlockState* d = nullptr;
   HIPCHK(hipMalloc(&d, sizeof(RbrpBlockState)));
@@ -194,7 +214,9 @@ int rbrp_test_chol2(void* stream, const double* G2, int64_t bt, int64_t b, const
   hs.bt = (int)bt;
   hipError_t e = hipMemcpyAsync(d, &hs, sizeof(hs), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
-    e = rbrp_launch_chol2(st, G2, bt, (int)b, d, Rb, R2, R2inv, Rout, bt);
+    e = kernel == 1   ? rbrp_launch_chol2_wave(st, G2, bt, (int)b, d, Rb, R2, R2inv, Rout, bt)
+        : kernel == 2 ? rbrp_launch_chol2_regs(st, G2, bt, (int)b, d, Rb, R2, R2inv, Rout, bt)
+                      : rbrp_launch_chol2(st, G2, bt, (int)b, d, Rb, R2, R2inv, Rout, bt);
   if (e == hipSuccess) e = hipMemcpyAsync(&hs, d, sizeof(hs), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(d);

@@ -166,9 +166,16 @@ def test_gemm_trail_update_matches_numpy(librbrp, hooks, M, N, K):
     np.testing.assert_allclose(parts.sum(axis=0), np.sum(ref * ref, axis=0), rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("s,b,deps", [(1, 8, 0), (8, 8, 0), (40, 40, 0), (64, 64, 5), (64, 64, 30),
-                                      (100, 128, 10), (128, 128, 0)])
-def test_panel_pivots_filter_match_dgeqp3(librbrp, hooks, s, b, deps):
+# kernel 0: the library's choice for b; 1: one wave, lane = column (b <= 64);
+# 2: register tiles over a workgroup (b <= 128)
+_PANEL_SMALL = [(1, 8, 0), (8, 8, 0), (40, 40, 0), (64, 64, 5), (64, 64, 30)]
+_PANEL_CASES = ([c + (1,) for c in _PANEL_SMALL] + [c + (2,) for c in _PANEL_SMALL]
+                + [(65, 128, 0, 2), (96, 96, 5, 2), (100, 128, 10, 2), (128, 128, 0, 2),
+                   (128, 128, 30, 2), (40, 64, 0, 0), (100, 128, 10, 0)])
+
+
+@pytest.mark.parametrize("s,b,deps,kernel", _PANEL_CASES)
+def test_panel_pivots_filter_match_dgeqp3(librbrp, hooks, s, b, deps, kernel):
     import torch
     rng = np.random.default_rng(s * 7 + deps)
     m = 600
@@ -187,9 +194,10 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, hooks, s, b, deps):
     dRb = torch.zeros(b * b, dtype=torch.float64, device="cuda")
     dRinv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
     bt, ns, status = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    code = hooks.rbrp_test_panel_inv(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
-                                       _ptr(dRb), _ptr(dRinv), _ptr(dJin), _ptr(dJout),
-                                       ctypes.byref(bt), ctypes.byref(ns), ctypes.byref(status))
+    code = hooks.rbrp_test_panel_kernel(_stream(), _ptr(dG), s, b, 0, i, _ptr(dJ), _ptr(dJk),
+                                          _ptr(dRb), _ptr(dRinv), _ptr(dJin), _ptr(dJout),
+                                          ctypes.byref(bt), ctypes.byref(ns),
+                                          ctypes.byref(status), kernel)
     assert code == 0 and status.value == 0
     _, R, piv = sla.qr(V, mode="economic", pivoting=True)
     bt_ref = robust_filter(R, s, b, "sampled")
@@ -212,9 +220,13 @@ def test_panel_pivots_filter_match_dgeqp3(librbrp, hooks, s, b, deps):
     assert np.array_equal(_host(dJout)[:len(jout)], jout)
 
 
-@pytest.mark.parametrize("bt,b", [(1, 8), (5, 8), (16, 64), (17, 64), (40, 64), (63, 64),
-                                  (64, 64), (100, 128), (128, 128)])
-def test_chol2_factor_inverse_product(librbrp, hooks, bt, b):
+_CHOL2_SMALL = [(1, 8), (5, 8), (16, 64), (17, 64), (40, 64), (63, 64), (64, 64)]
+_CHOL2_CASES = ([c + (1,) for c in _CHOL2_SMALL] + [c + (2,) for c in _CHOL2_SMALL]
+                + [(65, 128, 2), (80, 96, 2), (100, 128, 2), (128, 128, 2), (100, 128, 0)])
+
+
+@pytest.mark.parametrize("bt,b,kernel", _CHOL2_CASES)
+def test_chol2_factor_inverse_product(librbrp, hooks, bt, b, kernel):
     """Second Cholesky-QR pass: R2 = chol(G2) (upper, positive diagonal), R2inv =
     R2^{-1}, Rout = R2 Rb, against LAPACK on a near-orthonormal block's Gram."""
     import torch
@@ -229,8 +241,8 @@ def test_chol2_factor_inverse_product(librbrp, hooks, bt, b):
     dR2inv = torch.full((b * b,), np.nan, dtype=torch.float64, device="cuda")
     dRout = torch.full((bt * bt,), np.nan, dtype=torch.float64, device="cuda")
     status = ctypes.c_int32()
-    code = hooks.rbrp_test_chol2(_stream(), _ptr(dG2), bt, b, _ptr(dRb), _ptr(dR2),
-                                   _ptr(dR2inv), _ptr(dRout), ctypes.byref(status))
+    code = hooks.rbrp_test_chol2_kernel(_stream(), _ptr(dG2), bt, b, _ptr(dRb), _ptr(dR2),
+                                          _ptr(dR2inv), _ptr(dRout), ctypes.byref(status), kernel)
     assert code == 0 and status.value == 0
     R2_ref = np.linalg.cholesky(G2).T
     R2 = _host(dR2).reshape(b, b).T[:bt, :bt]
