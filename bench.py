@@ -307,8 +307,10 @@ def batched_leg(args):
 
 def ceilings_leg():
     """Measured ceilings of this part (committed micro-kernels of the test-hook
-    library): a streaming read and copy (nontemporal 16-byte accesses) and loops
-    of independent FP64 / FP32 16x16x4 matrix-core instructions."""
+    library): a streaming read and copy (nontemporal 16-byte accesses), an
+    in-place read-modify-write sweep in the trailing update's tile order (the
+    ceiling of that access pattern), and loops of independent FP64 / FP32
+    16x16x4 matrix-core instructions."""
     import ctypes
     import torch
     from robust_blockwise_random_pivoting_amd import _capi
@@ -317,6 +319,7 @@ def ceilings_leg():
     out = {}
     for kind, name, nbytes, unit, peak in ((0, "read", 2 << 30, "GB/s", PEAK_HBM_GBS),
                                            (1, "copy", 1 << 30, "GB/s", PEAK_HBM_GBS),
+                                           (4, "update_sweep", 1200 << 20, "GB/s", PEAK_HBM_GBS),
                                            (2, "mfma_f64", 1 << 20, "TFLOP/s", PEAK_FP64_TFLOPS),
                                            (3, "mfma_f32", 1 << 20, "TFLOP/s", PEAK_FP32_TFLOPS)):
         ms, rate = ctypes.c_double(), ctypes.c_double()

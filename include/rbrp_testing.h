@@ -118,7 +118,9 @@ int rbrp_test_sample_time(void* stream, const double* diags, int64_t n, int64_t 
 /* Ceilings of the part for the benchmark: kind 0 a streaming read of `bytes`
  * (rate in GB/s), 1 a streaming copy of `bytes` (GB/s read + written), 2 / 3
  * loops of independent FP64 / FP32 16x16x4 matrix-core instructions on every
- * compute unit (TFLOP/s); two untimed launches, then `reps` between events. */
+ * compute unit (TFLOP/s), 4 an in-place read-modify-write sweep of a 50048-row
+ * column-major matrix of about `bytes` in the trailing update's tile order
+ * (GB/s read + written); two untimed launches, then `reps` between events. */
 int rbrp_test_ceiling(void* stream, int kind, int64_t bytes, int reps, double* ms_per_launch,
                       double* rate);
 
