@@ -467,9 +467,12 @@ __device__ __forceinline__ void nn64_loop(int64_t K, const MatRef& A, uint32_t a
 #undef NN64_STEP
 }
 
+// With C32, the new C is also stored in FP32 at C32[r + c * ld32] (the mixed
+// loop's FP32 copy of the block's basis).
 __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, DimSpec dK, MatRef A,
                                                       MatRef B, MatRef C, int mode, int gx,
-                                                      int gr, int64_t rbt) {
+                                                      int gr, int64_t rbt, float* __restrict__ C32,
+                                                      int64_t ld32) {
   // task = column tile + gx * row range, XCD-aware
   const int64_t task = rbrp_xcd_task(blockIdx.x, (int64_t)gx * gr);
   if (task >= (int64_t)gx * gr) return;
@@ -513,7 +516,9 @@ __global__ __launch_bounds__(256, 4) void k_gemm_nn64(DimSpec dM, DimSpec dN, Di
         if (c + 4 * v < N) {
           double* d = mat_col(C, c + 4 * v) + r;
           const double x = acc[rb][v];
-          *d = (mode == GEMM_SUB) ? *d - x : ((mode == GEMM_ADD) ? *d + x : x);
+          const double y = (mode == GEMM_SUB) ? *d - x : ((mode == GEMM_ADD) ? *d + x : x);
+          *d = y;
+          if (C32) C32[r + (c + 4 * v) * ld32] = (float)y;
         }
     }
   }
@@ -624,7 +629,8 @@ int rbrp_gemm_nn64_resident() {
 }
 
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
-                                 DimSpec K, MatRef A, MatRef B, MatRef C, int mode, int resident) {
+                                 DimSpec K, MatRef A, MatRef B, MatRef C, int mode, int resident,
+                                 float* C32, int64_t ld32) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   const int64_t gx = (Nmax + 63) / 64, rbt = (Mmax + 15) / 16;
   // as many row ranges as resident workgroups per column tile, each of at
@@ -633,7 +639,7 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
   const int64_t gr = std::max<int64_t>((rbt + NN64_RB - 1) / NN64_RB, std::min<int64_t>(rbt, per));
   dim3 grid((unsigned)rbrp_xcd_grid(gx * gr));
   hipLaunchKernelGGL(k_gemm_nn64, grid, dim3(256), 0, st, M, N, K, A, B, C, mode, (int)gx, (int)gr,
-                     rbt);
+                     rbt, C32, ld32);
   return hipGetLastError();
 }
 

@@ -97,13 +97,24 @@ __global__ void k_gather_v64(int64_t rows, const RbrpBlockState* bs, const float
   V[r + (int64_t)p * ldv] = (double)Aw[r + (int64_t)J[p] * ld];
 }
 
+// (kept: the sampled positions of the kept columns, kept[0:bs->bt]; those
+// columns become the block's basis, so only the filtered-out ones are written.
+// A workgroup strides over the rows, so the membership test runs once per
+// column and workgroup, not per 256 rows.)
 __global__ void k_scatter_v32(int64_t rows, const RbrpBlockState* bs, const double* __restrict__ V,
-                              int64_t ldv, const int* __restrict__ J, float* __restrict__ Aw,
-                              int64_t ld) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+                              int64_t ldv, const int* __restrict__ J, const int* __restrict__ kept,
+                              float* __restrict__ Aw, int64_t ld) {
   const int p = blockIdx.y;
-  if (r >= rows || p >= bs->s) return;
-  Aw[r + (int64_t)J[p] * ld] = (float)V[r + (int64_t)p * ldv];
+  if (p >= bs->s) return;  // uniform per workgroup
+  if (kept) {
+    const int bt = bs->bt;
+    for (int q = 0; q < bt; ++q)
+      if (kept[q] == p) return;  // uniform per workgroup
+  }
+  const double* v = V + (int64_t)p * ldv;
+  float* a = Aw + (int64_t)J[p] * ld;
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += (int64_t)gridDim.x * 256)
+    a[r] = (float)v[r];
 }
 
 hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
@@ -117,10 +128,11 @@ hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const Rbr
 
 hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
                                    const double* V, int64_t ldv, const int* J, float* Aw,
-                                   int64_t ld) {
+                                   int64_t ld, const int* kept) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter_v32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
-                     st, rows, bs, V, ldv, J, Aw, ld);
+  const int64_t gx = std::min<int64_t>((rows + 255) / 256, 256);
+  hipLaunchKernelGGL(k_scatter_v32, dim3((unsigned)gx, (unsigned)b), dim3(256), 0, st, rows, bs, V,
+                     ldv, J, kept, Aw, ld);
   return hipGetLastError();
 }
 
@@ -139,25 +151,6 @@ hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const Rb
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_move_cols32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
                      st, rows, bs, Aw, ld, Jin, Jout);
-  return hipGetLastError();
-}
-
-// FP32 copy of the new orthonormal block: Aw(:, i + c) = fp32(Q(:, i + c)), c < bt.
-__global__ void k_store_q32(int64_t rows, int64_t i, const RbrpBlockState* bs,
-                            const double* __restrict__ Q, int64_t ldq, float* __restrict__ Aw,
-                            int64_t ld) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= rows || c >= bs->bt) return;
-  Aw[r + (i + c) * ld] = (float)Q[r + (i + c) * ldq];
-}
-
-hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
-                                 const RbrpBlockState* bs, const double* Q, int64_t ldq, float* Aw,
-                                 int64_t ld) {
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_store_q32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0, st,
-                     rows, i, bs, Q, ldq, Aw, ld);
   return hipGetLastError();
 }
 

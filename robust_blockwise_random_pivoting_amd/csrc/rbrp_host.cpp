@@ -726,12 +726,6 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, qall(),
                                    mat(ws.Cbuf, ldr), vblk(imax), GEMM_SUB, h->nn64_resident));
       cgs_launches += 4;
-      if (m32) {
-        // the corrected columns replace the stored residual (the filtered-out
-        // ones stay in the trailing matrix)
-        HIPCHK(rbrp_launch_scatter_v32(st, mp, b, ws.st, ws.Vd, mp, ws.J, ws.Aw32, mp));
-        ++cgs_launches;
-      }
     }
     pt.end(*ph_cgs, cgs_launches);
     return 0;
@@ -809,6 +803,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
                                        (int64_t)b * b, ns, mat(ws.G, b), GEMM_SET));
       HIPCHK(rbrp_launch_pivchol_filter(st, ws.G, b, b, opts->filter, i, ws.J, ws.st, ws.Rb, ws.Rinv,
                                         ws.Jk, ws.Jin, ws.Jout, ws.Jkpos));
+      if (m32)  // the filtered-out columns' corrected values replace their stored residual
+        HIPCHK(rbrp_launch_scatter_v32(st, mp, b, ws.st, ws.Vd, mp, ws.J, ws.Aw32, mp, ws.Jkpos));
       // Q1 = V(:, Jk) Rb^{-1}
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b,
                                    dim_dyn(0, d_bt, 1), vkept(i), mat(ws.Rinv, b), mat(ws.Qbuf, mp),
@@ -820,7 +816,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       HIPCHK(rbrp_launch_chol2(st, ws.G2, b, b, ws.st, ws.Rb, ws.R2, ws.R2inv, ws.R + i + i * ldr,
                                ldr));
     }
-    pt.end(ph, b <= 64 ? 7 : 8);
+    pt.end(ph, (b <= 64 ? 7 : 8) + (m32 ? 1 : 0));
 
     // ---- symmetric-difference permutation -------------------------------
     ph = pt.begin(RBRP_PHASE_PERMUTE);
@@ -838,14 +834,11 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     else
       --perm_launches;
     // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
+    // (mixed loop: with its FP32 copy in the working matrix's block columns)
     HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),
                                  mat(ws.Qbuf, mp), mat(ws.R2inv, b),
                                  m32 ? mat_off(ws.Q64, mp, i) : mat_off(ws.Aw, mp, i), GEMM_SET,
-                                 h->nn64_resident));
-    if (m32) {
-      HIPCHK(rbrp_launch_store_q32(st, mp, b, i, ws.st, ws.Q64, mp, ws.Aw32, mp));
-      ++perm_launches;
-    }
+                                 h->nn64_resident, m32 ? ws.Aw32 + i * mp : nullptr, mp));
     pt.end(ph, perm_launches);
 
     // ---- trailing projection and update ----------------------------------

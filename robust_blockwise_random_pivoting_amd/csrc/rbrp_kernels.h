@@ -119,10 +119,11 @@ int rbrp_gemm_tn32_resident(int wide = 0);
 // kernel's whole-device residency, rbrp_gemm_nn64_resident; 0: about 1024),
 // double-buffered K steps (the re-orthogonalisation V -= Q C, the new block
 // Q_b = Q1 R2^{-1} and Q1 = V(:, Jk) Rb^{-1}); any K (a device size allowed);
-// any MatRef for A.
+// any MatRef for A.  With C32 the new C(r, c) is also stored in FP32 at
+// C32[r + c * ld32].
 hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpec N, int64_t Nmax,
                                  DimSpec K, MatRef A, MatRef B, MatRef C, int mode,
-                                 int resident = 0);
+                                 int resident = 0, float* C32 = nullptr, int64_t ld32 = 0);
 int rbrp_gemm_nn64_resident();
 // Row ranges of k_gemm_nn64 and its norms form for an Mmax x Nmax output.
 int64_t rbrp_gemm_nn64_ranges(int64_t Mmax, int64_t Nmax, int resident);
@@ -263,17 +264,15 @@ hipError_t rbrp_launch_copy_colnorm32(hipStream_t st, const void* A, int a_is_f3
 // FP64 working copy (and norms) of an FP32 input.
 hipError_t rbrp_launch_copy_colnorm_from32(hipStream_t st, const float* A, int64_t lda, int64_t m,
                                            int64_t mp, int64_t n, double* Aw, double* diags);
-// V(:, p) = fp64(Aw(:, J[p])), p < bs->s  /  Aw(:, J[p]) = fp32(V(:, p)).
+// V(:, p) = fp64(Aw(:, J[p])), p < bs->s  /  Aw(:, J[p]) = fp32(V(:, p)) for the
+// positions p < bs->s not in kept[0:bs->bt] (all of them when kept is null).
 hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
                                   const float* Aw, int64_t ld, const int* J, double* V, int64_t ldv);
 hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
-                                   const double* V, int64_t ldv, const int* J, float* Aw, int64_t ld);
+                                   const double* V, int64_t ldv, const int* J, float* Aw, int64_t ld,
+                                   const int* kept = nullptr);
 hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
                                    float* Aw, int64_t ld, const int* Jin, const int* Jout);
-// Aw(:, i + c) = fp32(Q(:, i + c)) for c < bs->bt.
-hipError_t rbrp_launch_store_q32(hipStream_t st, int64_t rows, int b, int64_t i,
-                                 const RbrpBlockState* bs, const double* Q, int64_t ldq, float* Aw,
-                                 int64_t ld);
 // Split-K TN product of FP32 operands with FP64 partials (see k_gemm_tn; a
 // single 128-row tile when M > 64);
 // A's and B's first columns are acoff + *a_dyn and bcoff + *b_dyn (dyn
