@@ -100,17 +100,15 @@ __global__ void k_gather_v64(int64_t rows, const RbrpBlockState* bs, const float
 // (kept: the sampled positions of the kept columns, kept[0:bs->bt]; those
 // columns become the block's basis, so only the filtered-out ones are written.
 // A workgroup strides over the rows, so the membership test runs once per
-// column and workgroup, not per 256 rows.)
+// column and workgroup, not per 256 rows; 256 threads.)
 __global__ void k_scatter_v32(int64_t rows, const RbrpBlockState* bs, const double* __restrict__ V,
                               int64_t ldv, const int* __restrict__ J, const int* __restrict__ kept,
                               float* __restrict__ Aw, int64_t ld) {
   const int p = blockIdx.y;
   if (p >= bs->s) return;  // uniform per workgroup
-  if (kept) {
-    const int bt = bs->bt;
-    for (int q = 0; q < bt; ++q)
-      if (kept[q] == p) return;  // uniform per workgroup
-  }
+  // one list entry per thread (bt <= 128 < 256), a block-wide OR
+  const int hit = kept && (int)threadIdx.x < bs->bt && kept[threadIdx.x] == p;
+  if (__syncthreads_or(hit)) return;  // uniform per workgroup
   const double* v = V + (int64_t)p * ldv;
   float* a = Aw + (int64_t)J[p] * ld;
   for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < rows; r += (int64_t)gridDim.x * 256)
