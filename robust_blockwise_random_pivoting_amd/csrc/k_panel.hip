@@ -5,13 +5,16 @@
 // (k_pivchol_regs, k_chol2_regs further down).
 //
 // Lane c owns column c.  The factor is kept transposed in LDS, one row per
-// column (RT[c][t] = R(t, c), pitch PW_LDE), so that step j of the
-// left-looking factorisation
+// column (RT[c][t] = R(t, c), pitch PW_LDE).  Step j of the pivoted
+// factorisation
 //   R(j, c) = (G(P, c) - sum_{t<j} R(t, P) R(t, c)) / R(j, j)
-// reads the pivot's row as 16-byte broadcast loads and its own row as 16-byte
-// per-lane loads (the pitch of 132 dwords puts the 16 lanes of a pass on
-// distinct banks), sixteen terms per batch.  The loops stay rolled: a single wave runs these kernels, so their
-// speed depends on the code fitting the instruction cache.  The pivot is the
+// runs in groups of eight: the terms of the steps before the group come from
+// the Schur complement G - R(0:j0, :)^T R(0:j0, :), kept in LDS and updated on
+// the matrix core after each group (one LDS read per step), the group's own
+// terms from the registers of the pivot's lane.  The second Cholesky-QR pass
+// (no pivoting) reads the rows of RT for the earlier groups instead.  The outer
+// loops stay rolled: a single wave runs these kernels, so their speed depends
+// on the code fitting the instruction cache.  The pivot is the
 // largest Schur diagonal among the columns not yet chosen, the lowest position
 // on ties (a wave max and a ballot; the position tie-break runs only when
 // values tie); the Schur diagonals are downdated as d -= R(j, c)^2 by a fused
@@ -85,34 +88,6 @@ __device__ __forceinline__ double pw_readlane(double x, int lane) {
   const int lo = __builtin_amdgcn_readlane((int)b, lane);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// sum_{t < t1} x[t] * y[t] over two 16-byte aligned LDS rows (either may be
-// a broadcast row), in batches of sixteen terms with all sixteen 16-byte loads
-// of a batch issued together; the first batch is loaded unconditionally.
-// Terms in [t1, max(16, round16(t1))) must vanish (a zero in x or y) and
-// round16(t1) <= PW_N.  Four partial sums, term t into sum t mod 4.
-__device__ __forceinline__ double pw_ldot0(const double* x, const double* y, int t1) {
-  const double2* X = reinterpret_cast<const double2*>(x);
-  const double2* Y = reinterpret_cast<const double2*>(y);
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int t = 0;
-  do {
-    double2 a[8], b[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = X[t + u];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) b[u] = Y[t + u];
-#pragma unroll
-    for (int u = 0; u < 8; u += 2) {
-      s0 = fma(a[u].x, b[u].x, s0);
-      s1 = fma(a[u].y, b[u].y, s1);
-      s2 = fma(a[u + 1].x, b[u + 1].x, s2);
-      s3 = fma(a[u + 1].y, b[u + 1].y, s3);
-    }
-    t += 8;
-  } while (2 * t < t1);
-  return (s0 + s1) + (s2 + s3);
 }
 
 // Lane c zeroes its row of an even-pitch tile (PW_LDE doubles).
@@ -259,6 +234,35 @@ __device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, doubl
   }
 }
 
+// S -= Rg^T Rg for the eight factor rows t = t0 .. t0+7 (Rg(t, p) = RT[p *
+// PW_LDE + t]) on the blocks (I, J), I <= J < nbk, of the 16 x 16 block grid of
+// the row-major S (odd pitch), the diagonal blocks whole.  Lane (i, k) = i + 16 k
+// holds x[I][h] = Rg(t0 + 4h + k, 16 I + i): the A operand of block row I and
+// the B operand of block column I; the block's result sits in lane (j, g) as
+// S(16 I + 4 v + g, 16 J + j), element v.
+__device__ void pw_schur8(double* S, const double* RT, int t0, int nbk) {
+  const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
+  double x[4][2];
+#pragma unroll
+  for (int I = 0; I < 4; ++I)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) x[I][h] = RT[(16 * I + j) * PW_LDE + t0 + 4 * h + g];
+#pragma unroll
+  for (int I = 0; I < 4; ++I)
+#pragma unroll
+    for (int J = I; J < 4; ++J) {
+      if (J >= nbk) continue;  // uniform
+      double* blk = S + (16 * I + g) * PW_LDO + 16 * J + j;
+      v4d acc;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = blk[4 * v * PW_LDO];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-x[I][0], x[J][0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-x[I][1], x[J][1], acc, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) blk[4 * v * PW_LDO] = acc[v];
+    }
+}
+
 }  // namespace
 
 // LDS (doubles): Gs (odd pitch) | RT (even) | Ws (even) | Bt (even) | three
@@ -302,39 +306,63 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   int pos = c;
   int rank = s;
   double* myrow = RT + c * PW_LDE;
-  for (int j = 0; j < s; ++j) {
-    const double key = (act && pos >= j) ? d : -1.0;
-    const double m = pw_wave_max(key);
-    if (!(m > 0.0)) {
-      rank = j;
-      break;
-    }
-    const uint64_t ties = __builtin_amdgcn_ballot_w64(key == m);
-    int P, bp;
-    if (__builtin_popcountll(ties) == 1) {
-      P = (int)__builtin_ctzll(ties);
-      bp = __builtin_amdgcn_readlane(pos, P);
-    } else {
-      bp = pw_wave_min(key == m ? pos : 0x7fffffff);
-      P = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(key == m && pos == bp));
-    }
-    const double rjj = sqrt(m);
-    const double inv = pw_rsqrt(m);
-    const double g = Gs[P * PW_LDO + c];
-    const double acc = pw_ldot0(RT + P * PW_LDE, myrow, j);
-    // the column at position j moves to the pivot's position, the pivot to j
-    if (pos == j) pos = bp;
-    if (c == P) pos = j;
-    double val = 0.0;
-    if (act) {
-      if (c == P) {
-        val = rjj;
-      } else if (pos > j) {
-        val = (g - acc) * inv;
-        d = fma(-val, val, d);
+  // Left-looking in groups of eight steps.  Gs holds the Schur complement
+  // S = G - R(0:j0, :)^T R(0:j0, :) of the steps before the group (blocks on and
+  // above the diagonal of its 16 x 16 grid, updated on the matrix core after
+  // each group); a step adds the group's earlier terms from the registers of
+  // the pivot's lane.  Lane c reads S(P, c) from row P when P's block is at
+  // most c's, else from column P (S is symmetric).
+  const int nbk = (s + 15) >> 4;
+  const int bc = c >> 4;
+  for (int j0 = 0; j0 < s; j0 += 8) {
+    double rv[8];  // rv[q] = R(j0 + q, c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rv[q] = 0.0;
+    bool stop = false;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = j0 + q;
+      if (stop || j >= s) continue;  // uniform
+      const double key = (act && pos >= j) ? d : -1.0;
+      const double m = pw_wave_max(key);
+      if (!(m > 0.0)) {
+        rank = j;
+        stop = true;
+        continue;
       }
+      const uint64_t ties = __builtin_amdgcn_ballot_w64(key == m);
+      int P, bp;
+      if (__builtin_popcountll(ties) == 1) {
+        P = (int)__builtin_ctzll(ties);
+        bp = __builtin_amdgcn_readlane(pos, P);
+      } else {
+        bp = pw_wave_min(key == m ? pos : 0x7fffffff);
+        P = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(key == m && pos == bp));
+      }
+      const double rjj = sqrt(m);
+      const double inv = pw_rsqrt(m);
+      const double g = Gs[((P >> 4) <= bc) ? P * PW_LDO + c : c * PW_LDO + P];
+      double acc = 0.0;
+#pragma unroll
+      for (int p = 0; p < q; ++p) acc = fma(pw_readlane(rv[p], P), rv[p], acc);
+      // the column at position j moves to the pivot's position, the pivot to j
+      if (pos == j) pos = bp;
+      if (c == P) pos = j;
+      double val = 0.0;
+      if (act) {
+        if (c == P) {
+          val = rjj;
+        } else if (pos > j) {
+          val = (g - acc) * inv;
+          d = fma(-val, val, d);
+        }
+      }
+      rv[q] = val;
     }
-    myrow[j] = val;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
+    if (stop || j0 + 8 >= s) break;  // uniform
+    pw_schur8(Gs, RT, j0, nbk);
   }
   if (act) pos2col[pos] = c;
   // energy of factor row r (rows at and beyond the rank are zero)
