@@ -92,7 +92,7 @@ __device__ __forceinline__ double pw_readlane(double x, int lane) {
 
 // Lane c zeroes its row of an even-pitch tile (PW_LDE doubles).
 __device__ __forceinline__ void pw_zero_row(double* T) {
-  double2* r = reinterpret_cast<double2*>(T + threadIdx.x * PW_LDE);
+  double2* r = reinterpret_cast<double2*>(T + (threadIdx.x & 63) * PW_LDE);
 #pragma unroll 11
   for (int q = 0; q < PW_LDE / 2; ++q) r[q] = make_double2(0.0, 0.0);
 }
@@ -100,18 +100,20 @@ __device__ __forceinline__ void pw_zero_row(double* T) {
 // every load in flight at once: v[q] = X(r, q), indices clamped to n - 1.
 __device__ __forceinline__ void pw_fetch_row(const double* __restrict__ X, int64_t ldx, int n,
                                              double (&v)[PW_N]) {
-  const int rc = min((int)threadIdx.x, n - 1);
+  const int rc = min((int)(threadIdx.x & 63), n - 1);
 #pragma unroll
   for (int q = 0; q < PW_N; ++q) v[q] = X[rc + (int64_t)min(q, n - 1) * ldx];
 }
 // Y(r, q) = T[r * PW_LDO + q] for r, q < n (zero below the diagonal when
-// upper is set); lane r writes row r.
+// upper is set); lane r writes row r, wave w of nw the columns in groups of
+// eight from 8 w on, every 8 nw.
 __device__ __forceinline__ void pw_store_rows(const double* T, int n, bool upper,
-                                              double* __restrict__ Y, int64_t ldy) {
-  const int r = threadIdx.x;
+                                              double* __restrict__ Y, int64_t ldy, int w = 0,
+                                              int nw = 1) {
+  const int r = threadIdx.x & 63;
   if (r >= n) return;
   double* y = Y + r;
-  for (int q0 = 0; q0 < n; q0 += 8) {
+  for (int q0 = 8 * w; q0 < n; q0 += 8 * nw) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = T[r * PW_LDO + q0 + u];
@@ -172,7 +174,7 @@ __device__ __forceinline__ void pw_ldot8(const double* U, int r0, const double* 
 // origins (ra, ka) and (kb, cb).
 __device__ __forceinline__ v4d pw_mfma_block(v4d acc, const double* A, int lda, int ra, int ka,
                                              const double* Bt, int ldb, int kb, int cb) {
-  const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int j = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const double a = A[(ra + j) * lda + ka + 4 * s + g];
@@ -192,11 +194,14 @@ __device__ __forceinline__ v4d pw_mfma_block(v4d acc, const double* A, int lda, 
 // column j; then, by distance d = J - I from the diagonal,
 //   X(I, J) = -X(I, I) * sum_{K = I+1}^{J} U(I, K) X(K, J)
 // on the matrix core, the sum's MFMA output feeding the second product as
-// its B operand.
-__device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, double* Xo, int n) {
-  const int lane = threadIdx.x, j = lane & 15, g = lane >> 4;
+// its B operand.  Called by every wave of the workgroup: wave 0 inverts the
+// diagonal blocks, the blocks at one distance are spread over the nw waves (a
+// barrier between distances), and Xo is complete when it returns.
+__device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, double* Xo, int n,
+                                      int w = 0, int nw = 1) {
+  const int lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
   const int nbk = (n + 15) >> 4;  // uniform
-  if (g < nbk) {
+  if (w == 0 && g < nbk) {
     const int R0 = 16 * g;
     double x[16];
 #pragma unroll
@@ -211,7 +216,8 @@ __device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, doubl
     for (int r = 0; r < 16; ++r) Xo[(R0 + r) * PW_LDO + R0 + j] = x[r];
   }
   for (int d = 1; d < nbk; ++d) {
-    for (int I = 0; I + d < nbk; ++I) {
+    __syncthreads();  // the blocks nearer the diagonal are written
+    for (int I = w; I + d < nbk; I += nw) {
       const int J = I + d;
       v4d y = (v4d){0.0, 0.0, 0.0, 0.0};
       for (int K = I + 1; K <= J; ++K) {
@@ -232,6 +238,7 @@ __device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, doubl
       for (int v = 0; v < 4; ++v) Xo[(16 * I + 4 * v + g) * PW_LDO + 16 * J + j] = -xo[v];
     }
   }
+  __syncthreads();
 }
 
 // S -= Rg^T Rg for the eight factor rows t = t0 .. t0+7 (Rg(t, p) = RT[p *
@@ -241,7 +248,7 @@ __device__ void pw_upper_inverse_mfma(const double* U, const double* dinv, doubl
 // the B operand of block column I; the block's result sits in lane (j, g) as
 // S(16 I + 4 v + g, 16 J + j), element v.
 __device__ void pw_schur8(double* S, const double* RT, int t0, int nbk) {
-  const int j = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int j = threadIdx.x & 15, g = (threadIdx.x & 63) >> 4;
   double x[4][2];
 #pragma unroll
   for (int I = 0; I < 4; ++I)
@@ -268,8 +275,12 @@ __device__ void pw_schur8(double* S, const double* RT, int t0, int nbk) {
 // LDS (doubles): Gs (odd pitch) | RT (even) | Ws (even) | Bt (even) | three
 // vectors of PW_N.
 #define PW_LDS_DOUBLES (PW_TILE_O + 3 * PW_TILE_E + 3 * PW_N)
+// Waves of the two panel kernels below: the factorisation runs on the first;
+// the stores of the 64 x 64 results, the triangular inverse and R2 * Rb are
+// shared by all of them.
+#define PW_NW 4
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_pivchol_wave(const double* __restrict__ G, int64_t ldg,
+__global__ __launch_bounds__(64 * PW_NW) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_pivchol_wave(const double* __restrict__ G, int64_t ldg,
                                                      int b, int filter_mode, int64_t i,
                                                      const int* __restrict__ J,
                                                      RbrpBlockState* bs, double* __restrict__ Rb,
@@ -284,140 +295,151 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   double* sqr = Ws + 2 * PW_TILE_E;
   double* dinv = sqr + PW_N;
   int* pos2col = reinterpret_cast<int*>(dinv + PW_N);
-  const int c = threadIdx.x;
+  __shared__ int kept[2];  // bt and the degenerate flag, from the first wave
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int s = bs->s;
   if (s <= 0) {
-    if (c == 0) {
+    if (threadIdx.x == 0) {
       bs->bt = 0;
       bs->nswap = 0;
     }
     return;
   }
   const int jv = J[min(c, s - 1)];  // sampled column index of lane c
-  {
-    double v[PW_N];
-    pw_fetch_row(G, ldg, s, v);
-    pw_zero_row(RT);
+  if (w == 0) {
+    {
+      double v[PW_N];
+      pw_fetch_row(G, ldg, s, v);
+      pw_zero_row(RT);
 #pragma unroll
-    for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // Gs[r][q] = G(r, q), r, q < s
-  }
-  const bool act = c < s;
-  double d = act ? Gs[c * PW_LDO + c] : 0.0;
-  int pos = c;
-  int rank = s;
-  double* myrow = RT + c * PW_LDE;
-  // Left-looking in groups of eight steps.  Gs holds the Schur complement
-  // S = G - R(0:j0, :)^T R(0:j0, :) of the steps before the group (blocks on and
-  // above the diagonal of its 16 x 16 grid, updated on the matrix core after
-  // each group); a step adds the group's earlier terms from the registers of
-  // the pivot's lane.  Lane c reads S(P, c) from row P when P's block is at
-  // most c's, else from column P (S is symmetric).
-  const int nbk = (s + 15) >> 4;
-  const int bc = c >> 4;
-  for (int j0 = 0; j0 < s; j0 += 8) {
-    double rv[8];  // rv[q] = R(j0 + q, c)
+      for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // Gs[r][q] = G(r, q), r, q < s
+    }
+    const bool act = c < s;
+    double d = act ? Gs[c * PW_LDO + c] : 0.0;
+    int pos = c;
+    int rank = s;
+    double* myrow = RT + c * PW_LDE;
+    // Left-looking in groups of eight steps.  Gs holds the Schur complement
+    // S = G - R(0:j0, :)^T R(0:j0, :) of the steps before the group (blocks on and
+    // above the diagonal of its 16 x 16 grid, updated on the matrix core after
+    // each group); a step adds the group's earlier terms from the registers of
+    // the pivot's lane.  Lane c reads S(P, c) from row P when P's block is at
+    // most c's, else from column P (S is symmetric).
+    const int nbk = (s + 15) >> 4;
+    const int bc = c >> 4;
+    for (int j0 = 0; j0 < s; j0 += 8) {
+      double rv[8];  // rv[q] = R(j0 + q, c)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) rv[q] = 0.0;
-    bool stop = false;
+      for (int q = 0; q < 8; ++q) rv[q] = 0.0;
+      bool stop = false;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int j = j0 + q;
-      if (stop || j >= s) continue;  // uniform
-      const double key = (act && pos >= j) ? d : -1.0;
-      const double m = pw_wave_max(key);
-      if (!(m > 0.0)) {
-        rank = j;
-        stop = true;
-        continue;
+      for (int q = 0; q < 8; ++q) {
+        const int j = j0 + q;
+        if (stop || j >= s) continue;  // uniform
+        const double key = (act && pos >= j) ? d : -1.0;
+        const double m = pw_wave_max(key);
+        if (!(m > 0.0)) {
+          rank = j;
+          stop = true;
+          continue;
+        }
+        const uint64_t ties = __builtin_amdgcn_ballot_w64(key == m);
+        int P, bp;
+        if (__builtin_popcountll(ties) == 1) {
+          P = (int)__builtin_ctzll(ties);
+          bp = __builtin_amdgcn_readlane(pos, P);
+        } else {
+          bp = pw_wave_min(key == m ? pos : 0x7fffffff);
+          P = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(key == m && pos == bp));
+        }
+        const double rjj = sqrt(m);
+        const double inv = pw_rsqrt(m);
+        const double g = Gs[((P >> 4) <= bc) ? P * PW_LDO + c : c * PW_LDO + P];
+        double acc = 0.0;
+#pragma unroll
+        for (int p = 0; p < q; ++p) acc = fma(pw_readlane(rv[p], P), rv[p], acc);
+        // the column at position j moves to the pivot's position, the pivot to j
+        if (pos == j) pos = bp;
+        if (c == P) pos = j;
+        double val = 0.0;
+        if (act) {
+          if (c == P) {
+            val = rjj;
+          } else if (pos > j) {
+            val = (g - acc) * inv;
+            d = fma(-val, val, d);
+          }
+        }
+        rv[q] = val;
       }
-      const uint64_t ties = __builtin_amdgcn_ballot_w64(key == m);
-      int P, bp;
-      if (__builtin_popcountll(ties) == 1) {
-        P = (int)__builtin_ctzll(ties);
-        bp = __builtin_amdgcn_readlane(pos, P);
-      } else {
-        bp = pw_wave_min(key == m ? pos : 0x7fffffff);
-        P = (int)__builtin_ctzll(__builtin_amdgcn_ballot_w64(key == m && pos == bp));
-      }
-      const double rjj = sqrt(m);
-      const double inv = pw_rsqrt(m);
-      const double g = Gs[((P >> 4) <= bc) ? P * PW_LDO + c : c * PW_LDO + P];
-      double acc = 0.0;
 #pragma unroll
-      for (int p = 0; p < q; ++p) acc = fma(pw_readlane(rv[p], P), rv[p], acc);
-      // the column at position j moves to the pivot's position, the pivot to j
-      if (pos == j) pos = bp;
-      if (c == P) pos = j;
-      double val = 0.0;
-      if (act) {
-        if (c == P) {
-          val = rjj;
-        } else if (pos > j) {
-          val = (g - acc) * inv;
-          d = fma(-val, val, d);
+      for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
+      if (stop || j0 + 8 >= s) break;  // uniform
+      pw_schur8(Gs, RT, j0, nbk);
+    }
+    if (act) pos2col[pos] = c;
+    // energy of factor row r (rows at and beyond the rank are zero)
+    {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      if (c < rank) {
+#pragma unroll 4
+        for (int q = 0; q < PW_N; q += 4) {
+          const double a0 = RT[q * PW_LDE + c], a1 = RT[(q + 1) * PW_LDE + c];
+          const double a2 = RT[(q + 2) * PW_LDE + c], a3 = RT[(q + 3) * PW_LDE + c];
+          s0 = fma(a0, a0, s0);
+          s1 = fma(a1, a1, s1);
+          s2 = fma(a2, a2, s2);
+          s3 = fma(a3, a3, s3);
         }
       }
-      rv[q] = val;
+      sqr[c] = (s0 + s1) + (s2 + s3);
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
-    if (stop || j0 + 8 >= s) break;  // uniform
-    pw_schur8(Gs, RT, j0, nbk);
-  }
-  if (act) pos2col[pos] = c;
-  // energy of factor row r (rows at and beyond the rank are zero)
-  {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    if (c < rank) {
-#pragma unroll 4
-      for (int q = 0; q < PW_N; q += 4) {
-        const double a0 = RT[q * PW_LDE + c], a1 = RT[(q + 1) * PW_LDE + c];
-        const double a2 = RT[(q + 2) * PW_LDE + c], a3 = RT[(q + 3) * PW_LDE + c];
-        s0 = fma(a0, a0, s0);
-        s1 = fma(a1, a1, s1);
-        s2 = fma(a2, a2, s2);
-        s3 = fma(a3, a3, s3);
-      }
-    }
-    sqr[c] = (s0 + s1) + (s2 + s3);
-  }
-  // robust filter: csum over the rows from the last one up (the same serial
-  // order in every lane); keep bt = s - k for the first k with csum >= thr
-  double total = 0.0;
-  for (int q0 = s - 1; q0 >= 0; q0 -= 8) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = sqr[max(q0 - u, 0)];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (q0 - u >= 0) total += v[u];
-  }
-  const double thr = total / (double)(filter_mode == 0 ? s : b);
-  int k = s;
-  {
-    double csum = 0.0;
-    for (int q0 = 0; q0 < s && k == s; q0 += 8) {
+    // robust filter: csum over the rows from the last one up (the same serial
+    // order in every lane); keep bt = s - k for the first k with csum >= thr
+    double total = 0.0;
+    for (int q0 = s - 1; q0 >= 0; q0 -= 8) {
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = sqr[max(s - 1 - q0 - u, 0)];
+      for (int u = 0; u < 8; ++u) v[u] = sqr[max(q0 - u, 0)];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (k == s && q0 + u < s) {
-          csum += v[u];
-          if (csum >= thr) k = q0 + u;
+      for (int u = 0; u < 8; ++u)
+        if (q0 - u >= 0) total += v[u];
+    }
+    const double thr = total / (double)(filter_mode == 0 ? s : b);
+    int k = s;
+    {
+      double csum = 0.0;
+      for (int q0 = 0; q0 < s && k == s; q0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = sqr[max(s - 1 - q0 - u, 0)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (k == s && q0 + u < s) {
+            csum += v[u];
+            if (csum >= thr) k = q0 + u;
+          }
         }
       }
     }
-  }
-  const bool degenerate = rank == 0 || !(total > 0.0);
-  int bt = degenerate ? 0 : s - k;
-  if (bt > rank) bt = rank;
+    const bool degen = rank == 0 || !(total > 0.0);
+    int bt0 = degen ? 0 : s - k;
+    if (bt0 > rank) bt0 = rank;
+    if (c == 0) {
+      kept[0] = bt0;
+      kept[1] = degen ? 1 : 0;
+    }
+  }  // first wave
+  __syncthreads();
+  const int bt = kept[0];
+  const bool degenerate = kept[1] != 0;
   // kept factor in pivot order: column p = column pos2col[p] of R, rows <= p,
-  // row-major into Ws (broadcast rows) and Gs (store staging)
+  // row-major into Ws (broadcast rows) and Gs (store staging), rows 8 w .. by
+  // wave
+  const int col = pos2col[min(c, max(bt - 1, 0))];
   {
-    const int col = pos2col[min(c, max(bt - 1, 0))];
     const double* src = RT + col * PW_LDE;
-    for (int t0 = 0; t0 < PW_N; t0 += 8) {
+    for (int t0 = 8 * w; t0 < PW_N; t0 += 8 * PW_NW) {
       double v[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) v[u] = src[t0 + u];
@@ -428,16 +450,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         Gs[(t0 + u) * PW_LDO + c] = x;
       }
     }
+    if (w == 0 && c < bt) dinv[c] = 1.0 / src[c];
   }
-  if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
-  pw_store_rows(Gs, bt, true, Rb, b);
-  pw_upper_inverse_mfma(Ws, dinv, Gs, bt);
-  pw_store_rows(Gs, bt, true, Rinv, b);
+  __syncthreads();
+  pw_store_rows(Gs, bt, true, Rb, b, w, PW_NW);
+  __syncthreads();
+  pw_upper_inverse_mfma(Ws, dinv, Gs, bt, w, PW_NW);
+  pw_store_rows(Gs, bt, true, Rinv, b, w, PW_NW);
+  if (w != PW_NW - 1) return;  // (the last wave has no inverse blocks of its own)
   // index sets: Jk (pivot order), Jin = sort(Jk) beyond the block, Jout =
   // block positions not in Jk (ascending)
   int v = 0x7fffffff;
   {
-    const int col = pos2col[min(c, max(bt - 1, 0))];
     const int jc = __builtin_amdgcn_ds_bpermute(col << 2, jv);  // J[col] from lane col
     if (c < bt) {
       v = jc;
@@ -469,7 +493,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 
 // Second Cholesky-QR pass fused with the new diagonal block of R: G2 = R2^T R2,
 // R2inv = R2^{-1}, Rout = R2 * Rb (bt x bt, upper triangular, zeros below).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_chol2_wave(const double* __restrict__ G2, int64_t ldg,
+// The first wave factorises while the second stages Rb; the products, the
+// inverse and the stores are spread over the four waves.
+__global__ __launch_bounds__(64 * PW_NW) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_chol2_wave(const double* __restrict__ G2, int64_t ldg,
                                                    int b, RbrpBlockState* bs,
                                                    const double* __restrict__ Rb,
                                                    double* __restrict__ R2,
@@ -481,68 +507,73 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   double* Ws = RT + PW_TILE_E;    // R2, row-major
   double* Bt = Ws + PW_TILE_E;    // Bt[c][l] = Rb(l, c)
   double* dinv = Bt + PW_TILE_E;
-  const int c = threadIdx.x;
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int bt = bs->bt;
   if (bt <= 0) return;
-  {
-    double v[PW_N];
-    pw_fetch_row(G2, ldg, bt, v);
-    pw_zero_row(RT);
-#pragma unroll
-    for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // G2(r, q), r, q < bt
-  }
-  // Rb transposed: Bt[q][c] = Rb(c, q) (zero below the diagonal and beyond bt)
-  {
+  if (w == 1) {
+    // Rb transposed: Bt[q][c] = Rb(c, q) (zero below the diagonal and beyond bt)
     double v[PW_N];
     pw_fetch_row(Rb, b, bt, v);
 #pragma unroll
     for (int q = 0; q < PW_N; ++q)
       Bt[q * PW_LDE + c] = (c < bt && q < bt && c <= q) ? v[q] : 0.0;
   }
-  double d = c < bt ? Gs[c * PW_LDO + c] : 0.0;
-  bool bad = false;
-  double* myrow = RT + c * PW_LDE;
-  // left-looking by blocks of eight columns: the terms of earlier blocks for
-  // all eight at once, then the in-block terms from the registers of lane j
-  for (int j0 = 0; j0 < bt; j0 += 8) {
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    pw_ldot8(RT, j0, myrow, 0, j0, acc);  // RT row j = column j of R2
-    double g[8];
+  if (w == 0) {
+    {
+      double v[PW_N];
+      pw_fetch_row(G2, ldg, bt, v);
+      pw_zero_row(RT);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) g[q] = Gs[(j0 + q) * PW_LDO + c];
-    double rv[8];
+      for (int q = 0; q < PW_N; ++q) Gs[c * PW_LDO + q] = v[q];  // G2(r, q), r, q < bt
+    }
+    double d = c < bt ? Gs[c * PW_LDO + c] : 0.0;
+    bool bad = false;
+    double* myrow = RT + c * PW_LDE;
+    // left-looking by blocks of eight columns: the terms of earlier blocks for
+    // all eight at once, then the in-block terms from the registers of lane j
+    for (int j0 = 0; j0 < bt; j0 += 8) {
+      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      pw_ldot8(RT, j0, myrow, 0, j0, acc);  // RT row j = column j of R2
+      double g[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int j = j0 + q;
-      rv[q] = 0.0;
-      if (j < bt) {
-        const double dj = pw_readlane(d, j);
-        double a = acc[q];
+      for (int q = 0; q < 8; ++q) g[q] = Gs[(j0 + q) * PW_LDO + c];
+      double rv[8];
 #pragma unroll
-        for (int p = 0; p < q; ++p) a = fma(pw_readlane(rv[p], j), rv[p], a);
-        const double dd = dj > 0.0 ? dj : 1.0;
-        const double rjj = sqrt(dd);
-        const double inv = pw_rsqrt(dd);
-        bad |= !(dj > 0.0);
-        if (c < bt) {
-          if (c == j) {
-            rv[q] = rjj;
-          } else if (c > j) {
-            rv[q] = (g[q] - a) * inv;
-            d = fma(-rv[q], rv[q], d);
+      for (int q = 0; q < 8; ++q) {
+        const int j = j0 + q;
+        rv[q] = 0.0;
+        if (j < bt) {
+          const double dj = pw_readlane(d, j);
+          double a = acc[q];
+#pragma unroll
+          for (int p = 0; p < q; ++p) a = fma(pw_readlane(rv[p], j), rv[p], a);
+          const double dd = dj > 0.0 ? dj : 1.0;
+          const double rjj = sqrt(dd);
+          const double inv = pw_rsqrt(dd);
+          bad |= !(dj > 0.0);
+          if (c < bt) {
+            if (c == j) {
+              rv[q] = rjj;
+            } else if (c > j) {
+              rv[q] = (g[q] - a) * inv;
+              d = fma(-rv[q], rv[q], d);
+            }
           }
         }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
+      for (int q = 0; q < 8; ++q) myrow[j0 + q] = rv[q];
+    }
+    if (c == 0 && bad) bs->status = 2;
+    if (c < bt) dinv[c] = 1.0 / myrow[c];
   }
-  if (c == 0 && bad) bs->status = 2;
-  // R2 row-major into Ws (broadcast rows) and Gs (store staging)
-  for (int t0 = 0; t0 < PW_N; t0 += 8) {
+  __syncthreads();
+  // R2 row-major into Ws (broadcast rows) and Gs (store staging), rows 8 w ..
+  // by wave
+  for (int t0 = 8 * w; t0 < PW_N; t0 += 8 * PW_NW) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = myrow[t0 + u];
+    for (int u = 0; u < 8; ++u) v[u] = RT[c * PW_LDE + t0 + u];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const double x = (c < bt && t0 + u < bt) ? v[u] : 0.0;
@@ -550,23 +581,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
       Gs[(t0 + u) * PW_LDO + c] = x;
     }
   }
-  if (c < bt) dinv[c] = 1.0 / Ws[c * PW_LDE + c];
-  pw_store_rows(Gs, bt, true, R2, b);
-  // Rout = R2 * Rb on the matrix core, block (I, J) = sum_{K=I}^{J} R2(I, K) Rb(K, J)
+  __syncthreads();
+  pw_store_rows(Gs, bt, true, R2, b, w, PW_NW);
+  __syncthreads();
+  // Rout = R2 * Rb on the matrix core, block (I, J) = sum_{K=I}^{J} R2(I, K)
+  // Rb(K, J): the ten blocks of the 4 x 4 grid spread over the waves by their
+  // J - I + 1 products (five per wave)
   {
     const int nbk = (bt + 15) >> 4, j = c & 15, g = c >> 4;
-    for (int I = 0; I < nbk; ++I)
-      for (int J = I; J < nbk; ++J) {
-        v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
-        for (int K = I; K <= J; ++K)
-          acc = pw_mfma_block(acc, Ws, PW_LDE, 16 * I, 16 * K, Bt, PW_LDE, 16 * K, 16 * J);
+    constexpr int kI[10] = {0, 0, 0, 0, 1, 1, 2, 1, 2, 3};
+    constexpr int kJ[10] = {3, 0, 2, 1, 3, 2, 3, 1, 2, 3};
+    constexpr int kW[10] = {0, 0, 1, 1, 2, 2, 3, 3, 3, 3};
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Gs[(16 * I + 4 * v + g) * PW_LDO + 16 * J + j] = acc[v];
-      }
+    for (int p = 0; p < 10; ++p) {
+      const int I = kI[p], J = kJ[p];
+      if (kW[p] != w || J >= nbk) continue;  // uniform
+      v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int K = I; K <= J; ++K)
+        acc = pw_mfma_block(acc, Ws, PW_LDE, 16 * I, 16 * K, Bt, PW_LDE, 16 * K, 16 * J);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Gs[(16 * I + 4 * v + g) * PW_LDO + 16 * J + j] = acc[v];
+    }
   }
-  pw_store_rows(Gs, bt, true, Rout, ldr);
-  pw_upper_inverse_mfma(Ws, dinv, Gs, bt);
-  pw_store_rows(Gs, bt, true, R2inv, b);
+  __syncthreads();
+  pw_store_rows(Gs, bt, true, Rout, ldr, w, PW_NW);
+  __syncthreads();
+  pw_upper_inverse_mfma(Ws, dinv, Gs, bt, w, PW_NW);
+  pw_store_rows(Gs, bt, true, R2inv, b, w, PW_NW);
 }
 
 // Inverses of the diagonal blocks of a k x k upper-triangular R, in the
@@ -1177,7 +1218,7 @@ hipError_t rbrp_launch_pivchol_wave(hipStream_t st, const double* G, int64_t ldg
   static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
   hipError_t e = rbrp_allow_lds((const void*)k_pivchol_wave, (int)lds, lds_set);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pivchol_wave, dim3(1), dim3(64), lds, st, G, ldg, b, filter_mode, i, J, bs,
+  hipLaunchKernelGGL(k_pivchol_wave, dim3(1), dim3(64 * PW_NW), lds, st, G, ldg, b, filter_mode, i, J, bs,
                      Rb, Rinv, Jk, Jin, Jout, Jkpos);
   return hipGetLastError();
 }
@@ -1190,7 +1231,7 @@ hipError_t rbrp_launch_chol2_wave(hipStream_t st, const double* G2, int64_t ldg,
   static std::atomic<int> lds_set[RBRP_MAX_DEVICES];
   hipError_t e = rbrp_allow_lds((const void*)k_chol2_wave, (int)lds, lds_set);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_chol2_wave, dim3(1), dim3(64), lds, st, G2, ldg, b, bs, Rb, R2, R2inv, Rout,
-                     ldr);
+  hipLaunchKernelGGL(k_chol2_wave, dim3(1), dim3(64 * PW_NW), lds, st, G2, ldg, b, bs, Rb, R2, R2inv,
+                     Rout, ldr);
   return hipGetLastError();
 }
