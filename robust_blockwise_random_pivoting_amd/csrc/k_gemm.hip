@@ -297,6 +297,155 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
       }
 }
 
+// ---------------------------------------------------------------------------
+// Split-K Gram matrix of a tall block of at most 64 columns:
+// P[z](r, c) = sum_{kk in chunk z} A(kk, r) A(kk, c) for r, c < N.
+//
+// The 64 x 64 result is cut into 16 x 16 blocks; the ten on and above the
+// diagonal are computed, three, three, two and two per wave (gram_block), and
+// each off-diagonal block is also stored transposed, so the partials hold the
+// whole (exactly symmetric) matrix.  A 16-row step of A is staged once, in
+// double-buffered LDS laid out as k_gemm_tn64's ([column][row], the 16-byte
+// units of a column XOR-swizzled), with the loads of the step after next in
+// flight; lane (j, g) reads rows 4g .. 4g+3 of a column as two ds_read_b128 and
+// MFMA q of a block takes element q of both operands.  K and kchunk are
+// multiples of 16 and the leading dimension is even.
+// ---------------------------------------------------------------------------
+// Block b of wave w, (I, J) packed as 4 I + J, or -1.
+__host__ __device__ constexpr int gram_block(int w, int b) {
+  return w == 0   ? (b == 0 ? 0 : (b == 1 ? 1 : 2))     // (0,0) (0,1) (0,2)
+         : w == 1 ? (b == 0 ? 3 : (b == 1 ? 5 : 6))     // (0,3) (1,1) (1,2)
+         : w == 2 ? (b == 0 ? 7 : (b == 1 ? 10 : -1))   // (1,3) (2,2)
+                  : (b == 0 ? 11 : (b == 1 ? 15 : -1));  // (2,3) (3,3)
+}
+
+template <int W>
+__device__ __forceinline__ void gram_step(const double* as, int lj, int lg, v4d (&acc)[3]) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int blk = gram_block(W, b);
+    if (blk < 0) continue;
+    const int I = blk >> 2, J = blk & 3;
+    const double2 a0 = *(const double2*)&as[TN64_AT(16 * I + lj, 2 * lg)];
+    const double2 a1 = *(const double2*)&as[TN64_AT(16 * I + lj, 2 * lg + 1)];
+    double2 b0 = a0, b1 = a1;
+    if (J != I) {
+      b0 = *(const double2*)&as[TN64_AT(16 * J + lj, 2 * lg)];
+      b1 = *(const double2*)&as[TN64_AT(16 * J + lj, 2 * lg + 1)];
+    }
+    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, b0.x, acc[b], 0, 0, 0);
+    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, b0.y, acc[b], 0, 0, 0);
+    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.x, b1.x, acc[b], 0, 0, 0);
+    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, b1.y, acc[b], 0, 0, 0);
+  }
+}
+
+// lane (j, g) holds G(16 I + 4 v + g, 16 J + j) of block b in acc[b][v]
+template <int W>
+__device__ __forceinline__ void gram_store(const v4d (&acc)[3], int lj, int lg, int64_t N,
+                                           double* __restrict__ Pz, int64_t ldp) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const int blk = gram_block(W, b);
+    if (blk < 0) continue;
+    const int I = blk >> 2, J = blk & 3;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = 16 * I + 4 * v + lg, c = 16 * J + lj;
+      if (r < N && c < N) {
+        Pz[r + c * ldp] = acc[b][v];
+        if (I != J) Pz[c + r * ldp] = acc[b][v];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gram64(DimSpec dN, int64_t K, int64_t kchunk, MatRef A,
+                                                double* __restrict__ P, int64_t ldp,
+                                                int64_t pstride) {
+  const int64_t N = dim_eval(dN);
+  const int64_t kb0 = (int64_t)blockIdx.x * kchunk;
+  const int64_t kb1 = min(K, kb0 + kchunk);
+  if (N <= 0 || kb0 >= kb1) return;  // uniform per workgroup
+  __shared__ double As[2][64 * TN64_KS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lj = lane & 15, lg = lane >> 4;
+  // stager: column ac (clamped to the last one: it only feeds outputs that
+  // are not stored), rows aq .. aq+3 of the step
+  const int ac = tid >> 2, aq = (tid & 3) * 4, au = aq >> 1;
+  const double* ap = mat_col(A, min((int64_t)ac, N - 1)) + aq;
+  v4d acc[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
+#define G64_LOAD(S, kk)                          \
+  do {                                           \
+    S##0 = *(const double2*)(ap + (kk));         \
+    S##1 = *(const double2*)(ap + (kk) + 2);     \
+  } while (0)
+#define G64_STORE(S, bb)                                \
+  do {                                                  \
+    *(double2*)&As[bb][TN64_AT(ac, au)] = S##0;         \
+    *(double2*)&As[bb][TN64_AT(ac, au + 1)] = S##1;     \
+  } while (0)
+#define G64_STEP(bb)                                           \
+  do {                                                         \
+    switch (wid) {                                             \
+      case 0: gram_step<0>(&As[bb][0], lj, lg, acc); break;    \
+      case 1: gram_step<1>(&As[bb][0], lj, lg, acc); break;    \
+      case 2: gram_step<2>(&As[bb][0], lj, lg, acc); break;    \
+      default: gram_step<3>(&As[bb][0], lj, lg, acc); break;   \
+    }                                                          \
+  } while (0)
+  // the same step schedule as k_gemm_tn64: unconditional loads (a step past
+  // the chunk re-reads its last step), one barrier per step
+  const int64_t klast = kb1 - TN64_KS;
+  double2 X0, X1, Y0, Y1;
+  G64_LOAD(X, kb0);
+  G64_STORE(X, 0);
+  G64_LOAD(X, min(kb0 + TN64_KS, klast));
+  __syncthreads();
+  const int64_t nsteps = (kb1 - kb0) / TN64_KS;
+  int64_t kb = kb0;
+  for (int64_t s2 = 2; s2 <= nsteps; s2 += 2, kb += 2 * TN64_KS) {
+    G64_LOAD(Y, min(kb + 2 * TN64_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    G64_STEP(0);
+    __builtin_amdgcn_sched_barrier(0);
+    G64_STORE(X, 1);
+    __syncthreads();
+    G64_LOAD(X, min(kb + 3 * TN64_KS, klast));
+    __builtin_amdgcn_sched_barrier(0);
+    G64_STEP(1);
+    __builtin_amdgcn_sched_barrier(0);
+    G64_STORE(Y, 0);
+    __syncthreads();
+  }
+  if (nsteps & 1) G64_STEP(0);
+#undef G64_LOAD
+#undef G64_STORE
+#undef G64_STEP
+  double* Pz = P + (int64_t)blockIdx.x * pstride;
+  switch (wid) {
+    case 0: gram_store<0>(acc, lj, lg, N, Pz, ldp); break;
+    case 1: gram_store<1>(acc, lj, lg, N, Pz, ldp); break;
+    case 2: gram_store<2>(acc, lj, lg, N, Pz, ldp); break;
+    default: gram_store<3>(acc, lj, lg, N, Pz, ldp); break;
+  }
+}
+
+hipError_t rbrp_launch_gram64(hipStream_t st, DimSpec N, int64_t Nmax, int64_t K, int nsplit,
+                              int64_t kchunk, MatRef A, double* P, int64_t ldp, int64_t pstride) {
+  if (Nmax <= 0 || nsplit <= 0) return hipSuccess;
+  if (Nmax > 64 || K % TN64_KS != 0 || kchunk % TN64_KS != 0 || kchunk <= 0 || A.ld % 2 != 0 ||
+      ((uintptr_t)A.p & 15) || (!A.cols && A.coff_dyn) || (!A.cols && A.coff % 2 != 0) ||
+      ldp < Nmax)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_gram64, dim3((unsigned)nsplit), dim3(256), 0, st, N, K, kchunk, A, P, ldp,
+                     pstride);
+  return hipGetLastError();
+}
+
 int rbrp_gemm_tn64_resident() {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||

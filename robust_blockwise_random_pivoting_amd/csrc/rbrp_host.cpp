@@ -786,7 +786,8 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
     // ---- panel: Gram, pivoted Cholesky + filter, CholQR2 ----------------
     ph = pt.begin(RBRP_PHASE_PANEL);
     {
-      // Gram products: 64 x 64 tiles up to b = 64, the 64 x 128-tile kernel above
+      // Gram products: the symmetric 64-column kernel up to b = 64, the
+      // 64 x 128-tile kernel above
       const bool wide = b > 64;
       int ns;
       int64_t kc;
@@ -796,7 +797,7 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
         choose_split(b, b, mp, b, ws.Pcap, &ns, &kc);
       auto gram = [&](DimSpec d, MatRef X) {
         return wide ? rbrp_launch_gemm_tn64(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b)
-                    : rbrp_launch_gemm_tn(st, d, d, b, b, mp, ns, kc, X, X, ws.P, b, (int64_t)b * b);
+                    : rbrp_launch_gram64(st, d, b, mp, ns, kc, X, ws.P, b, (int64_t)b * b);
       };
       HIPCHK(gram(dim_dyn(0, d_s, 1), vblk(i)));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), dim_dyn(0, d_s, 1), b, b, ws.P, b,
