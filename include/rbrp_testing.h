@@ -44,10 +44,10 @@ int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
                         const double* B, int64_t ld, double* C, int64_t ldc, int nsplit);
 
 /* Symmetric split-K Gram kernel of the panel: C (N x N, ldc) = A^T A with A
- * (K x N, lda even) column-major, N <= 64, K a multiple of 16, split over
+ * (K x N, lda even) column-major, N <= 128, K a multiple of 16, split over
  * about nsplit chunks of K. */
-int rbrp_test_gram64(void* stream, int64_t N, int64_t K, const double* A, int64_t lda, double* C,
-                     int64_t ldc, int nsplit);
+int rbrp_test_gram(void* stream, int64_t N, int64_t K, const double* A, int64_t lda, double* C,
+                   int64_t ldc, int nsplit);
 
 /* Workgroups of the split-K projection kernel resident at once on the current
  * device: which = 0 for the FP64 kernel, 1 for the FP32 one (0 if unknown). */

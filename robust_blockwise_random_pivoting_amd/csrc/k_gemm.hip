@@ -298,34 +298,50 @@ __global__ __launch_bounds__(256, 2) void k_gemm_tn64(DimSpec dM, DimSpec dN, in
 }
 
 // ---------------------------------------------------------------------------
-// Split-K Gram matrix of a tall block of at most 64 columns:
-// P[z](r, c) = sum_{kk in chunk z} A(kk, r) A(kk, c) for r, c < N.
+// Split-K Gram matrix of a tall block of at most 16 NBK columns (NBK = 4: 64,
+// NBK = 8: 128): P[z](r, c) = sum_{kk in chunk z} A(kk, r) A(kk, c) for r, c < N.
 //
-// The 64 x 64 result is cut into 16 x 16 blocks; the ten on and above the
-// diagonal are computed, three, three, two and two per wave (gram_block), and
-// each off-diagonal block is also stored transposed, so the partials hold the
-// whole (exactly symmetric) matrix.  A 16-row step of A is staged once, in
-// double-buffered LDS laid out as k_gemm_tn64's ([column][row], the 16-byte
-// units of a column XOR-swizzled), with the loads of the step after next in
-// flight; lane (j, g) reads rows 4g .. 4g+3 of a column as two ds_read_b128 and
-// MFMA q of a block takes element q of both operands.  K and kchunk are
-// multiples of 16 and the leading dimension is even.
+// The result is cut into 16 x 16 blocks; the NBK (NBK + 1) / 2 on and above the
+// diagonal are computed, consecutive runs of them (row by row) per wave
+// (gram_block), blocks of columns >= N skipped, and each off-diagonal block is
+// also stored transposed, so the partials hold the whole (exactly symmetric)
+// matrix.  A 16-row step of A is staged once, in double-buffered LDS laid out
+// as k_gemm_tn64's ([column][row], the 16-byte units of a column
+// XOR-swizzled), with the loads of the step after next in flight; lane (j, g)
+// reads rows 4g .. 4g+3 of a column as two ds_read_b128 and MFMA q of a block
+// takes element q of both operands.  K and kchunk are multiples of 16 and the
+// leading dimension is even.
 // ---------------------------------------------------------------------------
-// Block b of wave w, (I, J) packed as 4 I + J, or -1.
+template <int NB>
+struct GramShape {
+  static constexpr int NBLK = NB * (NB + 1) / 2;  // blocks on and above the diagonal
+  static constexpr int PER = (NBLK + 3) / 4;      // blocks per wave
+};
+// Block k of the upper triangle in row order, (I, J) packed as 8 I + J.
+__host__ __device__ constexpr int gram_upper(int k, int nb) {
+  int I = 0;
+  while (k >= nb - I) {
+    k -= nb - I;
+    ++I;
+  }
+  return 8 * I + I + k;
+}
+// Block b of wave w for an NB x NB block grid, or -1.
+template <int NB>
 __host__ __device__ constexpr int gram_block(int w, int b) {
-  return w == 0   ? (b == 0 ? 0 : (b == 1 ? 1 : 2))     // (0,0) (0,1) (0,2)
-         : w == 1 ? (b == 0 ? 3 : (b == 1 ? 5 : 6))     // (0,3) (1,1) (1,2)
-         : w == 2 ? (b == 0 ? 7 : (b == 1 ? 10 : -1))   // (1,3) (2,2)
-                  : (b == 0 ? 11 : (b == 1 ? 15 : -1));  // (2,3) (3,3)
+  return GramShape<NB>::PER * w + b < GramShape<NB>::NBLK
+             ? gram_upper(GramShape<NB>::PER * w + b, NB)
+             : -1;
 }
 
-template <int W>
-__device__ __forceinline__ void gram_step(const double* as, int lj, int lg, v4d (&acc)[3]) {
+template <int NB, int W>
+__device__ __forceinline__ void gram_step(const double* as, int lj, int lg,
+                                          v4d (&acc)[GramShape<NB>::PER]) {
 #pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const int blk = gram_block(W, b);
+  for (int b = 0; b < GramShape<NB>::PER; ++b) {
+    const int blk = gram_block<NB>(W, b);
     if (blk < 0) continue;
-    const int I = blk >> 2, J = blk & 3;
+    const int I = blk >> 3, J = blk & 7;
     const double2 a0 = *(const double2*)&as[TN64_AT(16 * I + lj, 2 * lg)];
     const double2 a1 = *(const double2*)&as[TN64_AT(16 * I + lj, 2 * lg + 1)];
     double2 b0 = a0, b1 = a1;
@@ -341,14 +357,14 @@ __device__ __forceinline__ void gram_step(const double* as, int lj, int lg, v4d 
 }
 
 // lane (j, g) holds G(16 I + 4 v + g, 16 J + j) of block b in acc[b][v]
-template <int W>
-__device__ __forceinline__ void gram_store(const v4d (&acc)[3], int lj, int lg, int64_t N,
-                                           double* __restrict__ Pz, int64_t ldp) {
+template <int NB, int W>
+__device__ __forceinline__ void gram_store(const v4d (&acc)[GramShape<NB>::PER], int lj, int lg,
+                                           int64_t N, double* __restrict__ Pz, int64_t ldp) {
 #pragma unroll
-  for (int b = 0; b < 3; ++b) {
-    const int blk = gram_block(W, b);
+  for (int b = 0; b < GramShape<NB>::PER; ++b) {
+    const int blk = gram_block<NB>(W, b);
     if (blk < 0) continue;
-    const int I = blk >> 2, J = blk & 3;
+    const int I = blk >> 3, J = blk & 7;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t r = 16 * I + 4 * v + lg, c = 16 * J + lj;
@@ -360,89 +376,135 @@ __device__ __forceinline__ void gram_store(const v4d (&acc)[3], int lj, int lg, 
   }
 }
 
-__global__ __launch_bounds__(256) void k_gram64(DimSpec dN, int64_t K, int64_t kchunk, MatRef A,
-                                                double* __restrict__ P, int64_t ldp,
-                                                int64_t pstride) {
-  const int64_t N = dim_eval(dN);
-  const int64_t kb0 = (int64_t)blockIdx.x * kchunk;
-  const int64_t kb1 = min(K, kb0 + kchunk);
-  if (N <= 0 || kb0 >= kb1) return;  // uniform per workgroup
-  __shared__ double As[2][64 * TN64_KS];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+// The K loop and the stores of one workgroup for NB column blocks (a compile-
+// time count, so that each wave's blocks are straight-line code); the stage
+// holds NBK blocks of columns (columns >= N re-read the last one).
+template <int NBK, int NB>
+__device__ __forceinline__ void gram_chunk(int64_t N, int64_t kb0, int64_t kb1, const double* ap,
+                                           int ac, int au, double (*As)[16 * NBK * TN64_KS],
+                                           double* __restrict__ Pz, int64_t ldp) {
+  constexpr int PT = 16 * NBK * 16 / 2 / 256;  // 16-byte units staged per thread: 2 or 4
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int lj = lane & 15, lg = lane >> 4;
-  // stager: column ac (clamped to the last one: it only feeds outputs that
-  // are not stored), rows aq .. aq+3 of the step
-  const int ac = tid >> 2, aq = (tid & 3) * 4, au = aq >> 1;
-  const double* ap = mat_col(A, min((int64_t)ac, N - 1)) + aq;
-  v4d acc[3];
+  v4d acc[GramShape<NB>::PER];
 #pragma unroll
-  for (int b = 0; b < 3; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
-#define G64_LOAD(S, kk)                          \
-  do {                                           \
-    S##0 = *(const double2*)(ap + (kk));         \
-    S##1 = *(const double2*)(ap + (kk) + 2);     \
+  for (int b = 0; b < GramShape<NB>::PER; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
+  // (named staging registers: small arrays here go to scratch)
+#define G_LOAD(S_, kk)                                     \
+  do {                                                     \
+    S_##0 = *(const double2*)(ap + (kk));                  \
+    S_##1 = *(const double2*)(ap + (kk) + 2);              \
+    if (PT > 2) {                                          \
+      S_##2 = *(const double2*)(ap + (kk) + 4);            \
+      S_##3 = *(const double2*)(ap + (kk) + 6);            \
+    }                                                      \
   } while (0)
-#define G64_STORE(S, bb)                                \
-  do {                                                  \
-    *(double2*)&As[bb][TN64_AT(ac, au)] = S##0;         \
-    *(double2*)&As[bb][TN64_AT(ac, au + 1)] = S##1;     \
+#define G_STORE(S_, bb)                                    \
+  do {                                                     \
+    *(double2*)&As[bb][TN64_AT(ac, au)] = S_##0;           \
+    *(double2*)&As[bb][TN64_AT(ac, au + 1)] = S_##1;       \
+    if (PT > 2) {                                          \
+      *(double2*)&As[bb][TN64_AT(ac, au + 2)] = S_##2;     \
+      *(double2*)&As[bb][TN64_AT(ac, au + 3)] = S_##3;     \
+    }                                                      \
   } while (0)
-#define G64_STEP(bb)                                           \
-  do {                                                         \
-    switch (wid) {                                             \
-      case 0: gram_step<0>(&As[bb][0], lj, lg, acc); break;    \
-      case 1: gram_step<1>(&As[bb][0], lj, lg, acc); break;    \
-      case 2: gram_step<2>(&As[bb][0], lj, lg, acc); break;    \
-      default: gram_step<3>(&As[bb][0], lj, lg, acc); break;   \
-    }                                                          \
+#define G_STEP(bb)                                                    \
+  do {                                                                \
+    switch (wid) {                                                    \
+      case 0: gram_step<NB, 0>(&As[bb][0], lj, lg, acc); break;       \
+      case 1: gram_step<NB, 1>(&As[bb][0], lj, lg, acc); break;       \
+      case 2: gram_step<NB, 2>(&As[bb][0], lj, lg, acc); break;       \
+      default: gram_step<NB, 3>(&As[bb][0], lj, lg, acc); break;      \
+    }                                                                 \
   } while (0)
   // the same step schedule as k_gemm_tn64: unconditional loads (a step past
   // the chunk re-reads its last step), one barrier per step
   const int64_t klast = kb1 - TN64_KS;
-  double2 X0, X1, Y0, Y1;
-  G64_LOAD(X, kb0);
-  G64_STORE(X, 0);
-  G64_LOAD(X, min(kb0 + TN64_KS, klast));
+  double2 X0, X1, X2, X3, Y0, Y1, Y2, Y3;
+  G_LOAD(X, kb0);
+  G_STORE(X, 0);
+  G_LOAD(X, min(kb0 + TN64_KS, klast));
   __syncthreads();
   const int64_t nsteps = (kb1 - kb0) / TN64_KS;
   int64_t kb = kb0;
   for (int64_t s2 = 2; s2 <= nsteps; s2 += 2, kb += 2 * TN64_KS) {
-    G64_LOAD(Y, min(kb + 2 * TN64_KS, klast));
+    G_LOAD(Y, min(kb + 2 * TN64_KS, klast));
     __builtin_amdgcn_sched_barrier(0);
-    G64_STEP(0);
+    G_STEP(0);
     __builtin_amdgcn_sched_barrier(0);
-    G64_STORE(X, 1);
+    G_STORE(X, 1);
     __syncthreads();
-    G64_LOAD(X, min(kb + 3 * TN64_KS, klast));
+    G_LOAD(X, min(kb + 3 * TN64_KS, klast));
     __builtin_amdgcn_sched_barrier(0);
-    G64_STEP(1);
+    G_STEP(1);
     __builtin_amdgcn_sched_barrier(0);
-    G64_STORE(Y, 0);
+    G_STORE(Y, 0);
     __syncthreads();
   }
-  if (nsteps & 1) G64_STEP(0);
-#undef G64_LOAD
-#undef G64_STORE
-#undef G64_STEP
-  double* Pz = P + (int64_t)blockIdx.x * pstride;
+  if (nsteps & 1) G_STEP(0);
+#undef G_LOAD
+#undef G_STORE
+#undef G_STEP
   switch (wid) {
-    case 0: gram_store<0>(acc, lj, lg, N, Pz, ldp); break;
-    case 1: gram_store<1>(acc, lj, lg, N, Pz, ldp); break;
-    case 2: gram_store<2>(acc, lj, lg, N, Pz, ldp); break;
-    default: gram_store<3>(acc, lj, lg, N, Pz, ldp); break;
+    case 0: gram_store<NB, 0>(acc, lj, lg, N, Pz, ldp); break;
+    case 1: gram_store<NB, 1>(acc, lj, lg, N, Pz, ldp); break;
+    case 2: gram_store<NB, 2>(acc, lj, lg, N, Pz, ldp); break;
+    default: gram_store<NB, 3>(acc, lj, lg, N, Pz, ldp); break;
   }
 }
 
-hipError_t rbrp_launch_gram64(hipStream_t st, DimSpec N, int64_t Nmax, int64_t K, int nsplit,
-                              int64_t kchunk, MatRef A, double* P, int64_t ldp, int64_t pstride) {
+template <int NBK>
+__global__ __launch_bounds__(256, 2) void k_gram(DimSpec dN, int64_t K, int64_t kchunk, MatRef A,
+                                                 double* __restrict__ P, int64_t ldp,
+                                                 int64_t pstride) {
+  const int64_t N = dim_eval(dN);
+  const int64_t kb0 = (int64_t)blockIdx.x * kchunk;
+  const int64_t kb1 = min(K, kb0 + kchunk);
+  if (N <= 0 || kb0 >= kb1) return;  // uniform per workgroup
+  __shared__ double As[2][16 * NBK * TN64_KS];
+  // stager: column ac (clamped to the last one: it only feeds outputs that
+  // are not stored), 2 or 4 16-byte units from row aq on
+  constexpr int TPC = 256 / (16 * NBK);  // threads per column
+  const int tid = threadIdx.x;
+  const int ac = tid / TPC, aq = (tid % TPC) * (16 / TPC), au = aq >> 1;
+  const double* ap = mat_col(A, min((int64_t)ac, N - 1)) + aq;
+  double* Pz = P + (int64_t)blockIdx.x * pstride;
+  const int nbk = (int)min((int64_t)NBK, (N + 15) / 16);
+  if (NBK == 4) {
+    switch (nbk) {
+      case 1: gram_chunk<NBK, 1>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 2: gram_chunk<NBK, 2>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 3: gram_chunk<NBK, 3>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      default: gram_chunk<NBK, 4>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+    }
+  } else {
+    switch (nbk) {
+      case 1: gram_chunk<NBK, 1>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 2: gram_chunk<NBK, 2>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 3: gram_chunk<NBK, 3>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 4: gram_chunk<NBK, 4>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 5: gram_chunk<NBK, 5>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 6: gram_chunk<NBK, 6>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      case 7: gram_chunk<NBK, 7>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+      default: gram_chunk<NBK, 8>(N, kb0, kb1, ap, ac, au, As, Pz, ldp); break;
+    }
+  }
+}
+
+hipError_t rbrp_launch_gram(hipStream_t st, DimSpec N, int64_t Nmax, int64_t K, int nsplit,
+                            int64_t kchunk, MatRef A, double* P, int64_t ldp, int64_t pstride) {
   if (Nmax <= 0 || nsplit <= 0) return hipSuccess;
-  if (Nmax > 64 || K % TN64_KS != 0 || kchunk % TN64_KS != 0 || kchunk <= 0 || A.ld % 2 != 0 ||
+  if (Nmax > 128 || K % TN64_KS != 0 || kchunk % TN64_KS != 0 || kchunk <= 0 || A.ld % 2 != 0 ||
       ((uintptr_t)A.p & 15) || (!A.cols && A.coff_dyn) || (!A.cols && A.coff % 2 != 0) ||
       ldp < Nmax)
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_gram64, dim3((unsigned)nsplit), dim3(256), 0, st, N, K, kchunk, A, P, ldp,
-                     pstride);
+  if (Nmax <= 64)
+    hipLaunchKernelGGL(k_gram<4>, dim3((unsigned)nsplit), dim3(256), 0, st, N, K, kchunk, A, P, ldp,
+                       pstride);
+  else
+    hipLaunchKernelGGL(k_gram<8>, dim3((unsigned)nsplit), dim3(256), 0, st, N, K, kchunk, A, P, ldp,
+                       pstride);
   return hipGetLastError();
 }
 

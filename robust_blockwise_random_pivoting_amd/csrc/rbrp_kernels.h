@@ -108,13 +108,13 @@ hipError_t rbrp_launch_gemm_tn64(hipStream_t st, DimSpec M, DimSpec N, int64_t M
                                  int64_t K, int nsplit, int64_t kchunk, MatRef A, MatRef B, double* P,
                                  int64_t ldp, int64_t pstride, hipEvent_t e0 = nullptr,
                                  hipEvent_t e1 = nullptr, const FirstSrc* first = nullptr);
-// Split-K Gram matrix of a tall block of at most 64 columns (the panel's
+// Split-K Gram matrix of a tall block of at most 128 columns (the panel's
 // V^T V and Q1^T Q1): P[z](r, c) = sum_{kk in chunk z} A(kk, r) A(kk, c) for
-// r, c < N <= Nmax <= 64, chunks of kchunk rows (a multiple of 16, as K), the
+// r, c < N <= Nmax <= 128, chunks of kchunk rows (a multiple of 16, as K), the
 // whole symmetric matrix written; A's leading dimension even, its columns
 // 16-byte aligned.
-hipError_t rbrp_launch_gram64(hipStream_t st, DimSpec N, int64_t Nmax, int64_t K, int nsplit,
-                              int64_t kchunk, MatRef A, double* P, int64_t ldp, int64_t pstride);
+hipError_t rbrp_launch_gram(hipStream_t st, DimSpec N, int64_t Nmax, int64_t K, int nsplit,
+                            int64_t kchunk, MatRef A, double* P, int64_t ldp, int64_t pstride);
 // Workgroups of k_gemm_tn64 / k_gemm_tn32 resident at once on the current
 // device (compute units x occupancy); 0 if the runtime cannot tell.  `wide`:
 // the FP32 kernel's 128-row tile form (launched when A is wider than 64).

@@ -87,9 +87,9 @@ int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
   return (int)e;
 }
 
-int rbrp_test_gram64(void* stream, int64_t N, int64_t K, const double* A, int64_t lda, double* C,
-                     int64_t ldc, int nsplit) {
-  if (N < 0 || N > 64) return -2;
+int rbrp_test_gram(void* stream, int64_t N, int64_t K, const double* A, int64_t lda, double* C,
+                   int64_t ldc, int nsplit) {
+  if (N < 0 || N > 128) return -2;
   if (K < 0 || K % 16) return -3;
   if (lda % 2 || lda < std::max<int64_t>(K, 1)) return -5;
   if (ldc < std::max<int64_t>(N, 1)) return -7;
@@ -101,8 +101,8 @@ int rbrp_test_gram64(void* stream, int64_t N, int64_t K, const double* A, int64_
   const int ns = (int)std::max<int64_t>(1, (K + kc - 1) / kc);
   double* P = nullptr;
   HIPCHK(hipMalloc(&P, sizeof(double) * N * N * ns));
-  hipError_t e = rbrp_launch_gram64(st, dim_static(N), N, K, ns, kc, mat((double*)A, lda), P, N,
-                                    N * N);
+  hipError_t e = rbrp_launch_gram(st, dim_static(N), N, K, ns, kc, mat((double*)A, lda), P, N,
+                                  N * N);
   if (e == hipSuccess)
     e = rbrp_launch_reduce_splits(st, dim_static(N), dim_static(N), N, N, P, N, N * N, ns,
                                   mat(C, ldc), GEMM_SET);

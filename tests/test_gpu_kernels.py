@@ -104,16 +104,19 @@ def test_gemm_nn64_matches_numpy(librbrp, hooks, M, N, K, mode):
 
 
 @pytest.mark.parametrize("N,K,nsplit", [(64, 50048, 241), (1, 16, 1), (17, 160, 3), (40, 3008, 16),
-                                        (64, 64, 1), (33, 1024, 7), (48, 50048, 256)])
-def test_gram64_matches_numpy(librbrp, hooks, N, K, nsplit):
-    """The panel's symmetric Gram kernel against NumPy at FP64 rounding: partial
-    column blocks, one to many splits; the result is exactly symmetric."""
+                                        (64, 64, 1), (33, 1024, 7), (48, 50048, 256),
+                                        (128, 100000, 256), (65, 4000, 9), (71, 1000000, 256),
+                                        (100, 336, 2)])
+def test_gram_matches_numpy(librbrp, hooks, N, K, nsplit):
+    """The panel's symmetric Gram kernel (64- and 128-column tiles) against NumPy
+    at FP64 rounding: partial column blocks, one to many splits; the result is
+    exactly symmetric."""
     import torch
     rng = np.random.default_rng(N * 7 + K)
     A = rng.standard_normal((K, N)) * np.exp(-np.arange(N) / 10.0)
     dA = _cm(A)
     dC = _cm(np.full((N, N), np.nan))
-    code = hooks.rbrp_test_gram64(_stream(), N, K, _ptr(dA), K, _ptr(dC), N, nsplit)
+    code = hooks.rbrp_test_gram(_stream(), N, K, _ptr(dA), K, _ptr(dC), N, nsplit)
     assert code == 0
     got = _host(dC)
     ref = A.T @ A
@@ -122,12 +125,12 @@ def test_gram64_matches_numpy(librbrp, hooks, N, K, nsplit):
     torch.cuda.synchronize()
 
 
-def test_gram64_rejects_bad_shapes(librbrp, hooks):
+def test_gram_rejects_bad_shapes(librbrp, hooks):
     import torch
     z = torch.zeros(64, dtype=torch.float64, device="cuda")
-    assert hooks.rbrp_test_gram64(_stream(), 65, 16, _ptr(z), 16, _ptr(z), 65, 1) == -2
-    assert hooks.rbrp_test_gram64(_stream(), 4, 8, _ptr(z), 16, _ptr(z), 4, 1) == -3
-    assert hooks.rbrp_test_gram64(_stream(), 4, 16, _ptr(z), 17, _ptr(z), 4, 1) == -5
+    assert hooks.rbrp_test_gram(_stream(), 129, 16, _ptr(z), 16, _ptr(z), 129, 1) == -2
+    assert hooks.rbrp_test_gram(_stream(), 4, 8, _ptr(z), 16, _ptr(z), 4, 1) == -3
+    assert hooks.rbrp_test_gram(_stream(), 4, 16, _ptr(z), 17, _ptr(z), 4, 1) == -5
 
 
 def test_gemm_nn64_rejects_bad_shapes(librbrp, hooks):
