@@ -855,7 +855,10 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
 }
 
 // ---------------------------------------------------------------------------
-// Split reduction: dst(r, c) (=|+=|-=) sum_{z=0}^{nsplit-1} P[z](r, c).
+// Split reduction: dst(r, c) (=|+=|-=) sum_{z=0}^{nsplit-1} P[z](r, c); with
+// dst2 also stored row-major (dst2[r * ld2 + c]), with radd also added to
+// radd[c + racols[r] * ldra] (the re-orthogonalisation coefficients into the
+// sampled columns of R).
 // One workgroup per (64-row slab, column): sixteen waves each sum every
 // sixteenth split with eight loads in flight per lane (eight accumulators),
 // then the sixteen partials are added in a fixed tree order, so the result
@@ -865,7 +868,8 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
 template <typename T2>
 __global__ __launch_bounds__(64 * RED_GROUPS) void k_reduce_splits(
     DimSpec dM, DimSpec dN, const double* __restrict__ P, int64_t ldp, int64_t pstride, int nsplit,
-    MatRef dst, int mode, T2* __restrict__ dst2, int64_t ld2) {
+    MatRef dst, int mode, T2* __restrict__ dst2, int64_t ld2, double* __restrict__ radd,
+    int64_t ldra, const int* __restrict__ racols) {
   __shared__ double part[RED_GROUPS][64];
   const int64_t M = dim_eval(dM), N = dim_eval(dN);
   const int64_t c = blockIdx.x;
@@ -905,16 +909,19 @@ __global__ __launch_bounds__(64 * RED_GROUPS) void k_reduce_splits(
         *d = s;
     }
     if (dst2) dst2[r * ld2 + c] = (T2)s;  // row-major copy
+    if (radd) radd[c + (int64_t)racols[r] * ldra] += s;
   }
 }
 
 hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
-                                     MatRef dst, int mode, double* dst2, int64_t ld2) {
+                                     MatRef dst, int mode, double* dst2, int64_t ld2, double* radd,
+                                     int64_t ldra, const int* racols) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
+  if (radd && !racols) return hipErrorInvalidValue;
   dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
   hipLaunchKernelGGL(k_reduce_splits<double>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
-                     pstride, nsplit, dst, mode, dst2, ld2);
+                     pstride, nsplit, dst, mode, dst2, ld2, radd, ldra, racols);
   return hipGetLastError();
 }
 
@@ -925,7 +932,8 @@ hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec 
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
   hipLaunchKernelGGL(k_reduce_splits<float>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
-                     pstride, nsplit, dst, mode, dst2, ld2);
+                     pstride, nsplit, dst, mode, dst2, ld2, (double*)nullptr, (int64_t)0,
+                     (const int*)nullptr);
   return hipGetLastError();
 }
 

@@ -1015,25 +1015,6 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
   return hipGetLastError();
 }
 
-// dst(r, cols[c]) += src(r, c), r < rows, c < *ncols_dyn.
-__global__ void k_add_cols(DimSpec drows, const int* ncols_dyn, const double* __restrict__ src,
-                           int64_t lds, double* __restrict__ dst, int64_t ldd,
-                           const int* __restrict__ cols) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= dim_eval(drows) || c >= *ncols_dyn) return;
-  dst[r + (int64_t)cols[c] * ldd] += src[r + (int64_t)c * lds];
-}
-
-hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
-                                const int* ncols_dyn, int64_t ncols_max, const double* src,
-                                int64_t lds, double* dst, int64_t ldd, const int* cols) {
-  if (rows_max <= 0 || ncols_max <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_add_cols, dim3((unsigned)((rows_max + 255) / 256), (unsigned)ncols_max),
-                     dim3(256), 0, st, rows, ncols_dyn, src, lds, dst, ldd, cols);
-  return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // Pivoted Cholesky of the sampled block's Gram matrix G = V^T V (s x s) +
 // robust filter.

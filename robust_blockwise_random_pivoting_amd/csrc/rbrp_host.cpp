@@ -715,17 +715,17 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
       int ns;
       int64_t kc;
       // C = Q^T V computed as its transpose V^T Q (the sampled block on the
-      // 64-row side of the tiles), reduced into C column-major
+      // 64-row side of the tiles), reduced into C column-major and added into
+      // the sampled columns of R (R(0:i, J) += C)
       choose_split(b, imax, mp, b, ws.Pcap, &ns, &kc, 16, 128, h->tn64_resident);
       HIPCHK(rbrp_launch_gemm_tn64(st, dim_dyn(0, d_s, 1), di, b, imax, mp, ns, kc, vblk(imax), qall(), ws.P,
                                    b, imax * (int64_t)b));
       HIPCHK(rbrp_launch_reduce_splits(st, dim_dyn(0, d_s, 1), di, b, imax, ws.P, b,
                                        imax * (int64_t)b, ns, MatRef{nullptr, 0, nullptr, 0, nullptr},
-                                       GEMM_SET, ws.Cbuf, ldr));
-      HIPCHK(rbrp_launch_add_cols(st, di, imax, d_s, b, ws.Cbuf, ldr, ws.R, ldr, ws.J));
+                                       GEMM_SET, ws.Cbuf, ldr, ws.R, ldr, ws.J));
       HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_s, 1), b, di, qall(),
                                    mat(ws.Cbuf, ldr), vblk(imax), GEMM_SUB, h->nn64_resident));
-      cgs_launches += 4;
+      cgs_launches += 3;
     }
     pt.end(*ph_cgs, cgs_launches);
     return 0;

@@ -144,10 +144,13 @@ hipError_t rbrp_launch_gemm_nn64_resnorm(hipStream_t st, int64_t M, int64_t N, i
 
 // dst(r, c) (=|+=|-=) sum_z P[z](r, c) in a fixed order (skipped when dst.p is
 // null); when dst2 != nullptr the sum is also stored row-major at
-// dst2[r * ld2 + c] (the transpose, for a column-major dst2).
+// dst2[r * ld2 + c] (the transpose, for a column-major dst2); when radd !=
+// nullptr it is also added to radd[c + racols[r] * ldra].
 hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax, int64_t Nmax,
                                      const double* P, int64_t ldp, int64_t pstride, int nsplit,
-                                     MatRef dst, int mode, double* dst2 = nullptr, int64_t ld2 = 0);
+                                     MatRef dst, int mode, double* dst2 = nullptr, int64_t ld2 = 0,
+                                     double* radd = nullptr, int64_t ldra = 0,
+                                     const int* racols = nullptr);
 
 // As rbrp_launch_reduce_splits, with the row-major copy stored in FP32.
 hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec N, int64_t Mmax,
@@ -204,9 +207,6 @@ hipError_t rbrp_launch_sample(hipStream_t st, const double* diags, int64_t i, in
                               int64_t kmax, int b, uint64_t key0, uint64_t key1,
                               const uint64_t ctr[4], double* w, double* cdf, RbrpBlockState* bs,
                               int* J, const int* i_dyn = nullptr, double tol2 = 0.0);
-hipError_t rbrp_launch_add_cols(hipStream_t st, DimSpec rows, int64_t rows_max,
-                                const int* ncols_dyn, int64_t ncols_max, const double* src,
-                                int64_t lds, double* dst, int64_t ldd, const int* cols);
 hipError_t rbrp_launch_pivchol_filter(hipStream_t st, const double* G, int64_t ldg, int b,
                                       int filter_mode, int64_t i, const int* J, RbrpBlockState* bs,
                                       double* Rb, double* Rinv, int* Jk, int* Jin, int* Jout,
