@@ -49,6 +49,13 @@ int rbrp_test_gemm_tn64(void* stream, int64_t M, int64_t N, int64_t K, const dou
 int rbrp_test_gram(void* stream, int64_t N, int64_t K, const double* A, int64_t lda, double* C,
                    int64_t ldc, int nsplit);
 
+/* The blocked back substitution of the X solve: X (k x nc, ldx) <- R^{-1} X for
+ * the k x k upper-triangular R (ldr) column-major on the device; with
+ * use_dinv the diagonal-block inverses and the one-launch column-tile solve,
+ * else the substitution path. */
+int rbrp_test_trsm(void* stream, int64_t k, int64_t nc, const double* R, int64_t ldr, double* X,
+                   int64_t ldx, int use_dinv);
+
 /* Workgroups of the split-K projection kernel resident at once on the current
  * device: which = 0 for the FP64 kernel, 1 for the FP32 one (0 if unknown). */
 int rbrp_test_resident(int which);

@@ -71,6 +71,7 @@ def _hook_signatures():
         "rbrp_test_gemm_tn64": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_nn64": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_gram": (c_int, [vp, i64, i64, vp, i64, vp, i64, c_int]),
+        "rbrp_test_trsm": (c_int, [vp, i64, i64, vp, i64, vp, i64, c_int]),
         "rbrp_test_resident": (c_int, [c_int]),
         "rbrp_test_gemm_tn32": (c_int, [vp, i64, i64, i64, vp, vp, i64, vp, i64, c_int]),
         "rbrp_test_gemm_upd32": (c_int, [vp, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp]),

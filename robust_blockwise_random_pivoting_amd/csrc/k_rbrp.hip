@@ -1326,6 +1326,115 @@ hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, cons
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// X(0:k, c) <- R^{-1} X(0:k, c) for the k x k upper-triangular R (ldr) in one
+// launch, one workgroup per 16 columns of X: the 64-row blocks from the bottom
+// (block q = rows [max(0, k - 64 (q+1)), k - 64 q), the partition of
+// k_upper_inv_blocks, whose inverses are at dinv + 4096 q, ld 64), each as
+//   T = X(blk, c) - R(blk, below) X(below, c),   X(blk, c) = Rinv(blk) T
+// on the FP64 matrix core.  Sixteen waves: wave (rw, ks) takes rows 16 rw ..
+// 16 rw + 15 of the block and every TRSM_KS-th 16-deep step of the rows below
+// from step ks (the 4 q steps of block q split evenly); MFMA s of a step takes
+// row kb + 4 g + s of X from lane group g, and the slice's next step is loaded
+// before the current step's products (clamped to the last step, so every load
+// is unconditional).  The slices' partial sums meet in LDS and are added in
+// slice order; the first slice then forms T, multiplies it by the block
+// inverse (entries past the block's size masked) and stores the block's rows,
+// a workgroup barrier ordering them before the next block's loads.
+// ---------------------------------------------------------------------------
+#define TRSM_KS 4
+__global__ __launch_bounds__(256 * TRSM_KS) void k_trsm_cols(int64_t k, int64_t nc,
+                                                             const double* __restrict__ R, int64_t ldr,
+                                                             const double* __restrict__ dinv,
+                                                             double* X, int64_t ldx) {
+  __shared__ double Tp[TRSM_KS][64 * 17];  // slice partials, then T in Tp[0]: (t, col) at t * 17 + col
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int rw = w & 3, ks = w >> 2;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t c0 = (int64_t)blockIdx.x * 16;
+  if (c0 >= nc) return;  // uniform per workgroup
+  const bool cok = c0 + j < nc;
+  double* xc = X + min(c0 + j, nc - 1) * ldx;  // this lane's column (clamped)
+  const int nq = (int)((k + 63) / 64);
+  for (int q = 0; q < nq; ++q) {
+    const int64_t r1 = k - 64 * (int64_t)q, r0 = max((int64_t)0, r1 - 64);
+    const int nb = (int)(r1 - r0);
+    // A operand: row r0 + 16 rw + j of R (clamped into the block)
+    const double* rrow = R + r0 + min(16 * rw + j, nb - 1);
+    v4d acc = (v4d){0.0, 0.0, 0.0, 0.0};
+    if (q > 0) {
+      // this slice's steps: r1 + 16 (ks + TRSM_KS t), t < q (k - r1 = 64 q)
+      const int64_t klast = k - 16;
+      int64_t kb = r1 + 16 * ks;
+      double a[4], b[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        a[s] = rrow[(kb + 4 * g + s) * ldr];
+        b[s] = xc[kb + 4 * g + s];
+      }
+      for (int t = 0; t < q; ++t, kb += 16 * TRSM_KS) {
+        const int64_t kn = min(kb + 16 * TRSM_KS, klast);
+        double an[4], bn[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          an[s] = rrow[(kn + 4 * g + s) * ldr];
+          bn[s] = xc[kn + 4 * g + s];
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          a[s] = an[s];
+          b[s] = bn[s];
+        }
+      }
+    }
+    // lane (j, g) holds row 16 rw + 4 v + g of the slice's product in acc[v]
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Tp[ks][(16 * rw + 4 * v + g) * 17 + j] = acc[v];
+    __syncthreads();
+    if (ks == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rl = 16 * rw + 4 * v + g;
+        double sum = Tp[0][rl * 17 + j];
+#pragma unroll
+        for (int z = 1; z < TRSM_KS; ++z) sum += Tp[z][rl * 17 + j];
+        Tp[0][rl * 17 + j] = (rl < nb) ? xc[r0 + rl] - sum : 0.0;
+      }
+    }
+    __syncthreads();
+    if (ks == 0) {
+      const double* dq = dinv + 4096 * (int64_t)q;  // Rinv(row, t) at dq[row + 64 t]
+      const int row = 16 * rw + j;
+      v4d y = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int tb = 0; tb < 64; tb += 16)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int t = tb + 4 * g + s;
+          const double av = (row < nb && t < nb) ? dq[row + 64 * t] : 0.0;
+          y = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Tp[0][t * 17 + j], y, 0, 0, 0);
+        }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rl = 16 * rw + 4 * v + g;
+        if (rl < nb && cok) xc[r0 + rl] = y[v];
+      }
+    }
+    __syncthreads();  // the block's rows are stored and Tp is free before the next block
+  }
+}
+
+hipError_t rbrp_launch_trsm_cols(hipStream_t st, int64_t k, int64_t nc, const double* R, int64_t ldr,
+                                 const double* dinv, double* X, int64_t ldx) {
+  if (k <= 0 || nc <= 0) return hipSuccess;
+  if (!dinv || ldr < k || ldx < k) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_trsm_cols, dim3((unsigned)((nc + 15) / 16)), dim3(256 * TRSM_KS), 0, st, k, nc,
+                     R, ldr, dinv, X, ldx);
+  return hipGetLastError();
+}
+
 // out[t] = uniform number draw0 + t of the stream (key0, key1, ctr).
 __global__ void k_uniforms(uint64_t key0, uint64_t key1, uint64_t c0, uint64_t c1, uint64_t c2,
                            uint64_t c3, uint64_t draw0, int64_t count, double* __restrict__ out) {

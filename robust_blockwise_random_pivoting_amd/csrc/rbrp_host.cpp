@@ -366,27 +366,29 @@ double now_s() {
 
 }  // namespace
 
-// X (k x nc, ldx) <- R^{-1} X for the k x k upper-triangular R (ldr): backward
-// blocked solve with 256-row super-blocks, each solved with 64-row diagonal
-// blocks and small updates, followed by one large update of every row above
-// the super-block (k_gemm_nn).  With a scratch `dinv` (4 * ceil(k / 256) x
-// 4096 doubles) the diagonal blocks are inverted in one launch up front and
-// applied as in-place products (a workgroup reads all rows of its columns
-// before writing them); without it each block is solved by substitution
-// (k_trsm_diag, one thread per column).
+// X (k x nc, ldx) <- R^{-1} X for the k x k upper-triangular R (ldr).  With a
+// scratch `dinv` (4 * ceil(k / 256) x 4096 doubles) the 64-row diagonal
+// blocks are inverted in one launch and every column tile then runs the whole
+// backward blocked solve in one more (k_trsm_cols).  Without it: 256-row
+// super-blocks, each solved with 64-row diagonal blocks by substitution
+// (k_trsm_diag, one thread per column) and small updates, followed by one
+// large update of every row above the super-block (k_gemm_nn).
 hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R, int64_t ldr,
                                 double* X, int64_t ldx, int* launches, double* dinv) {
   hipError_t e;
   if (dinv && k > 0 && nc > 0) {
+    // the diagonal-block inverses, then every block of every column tile in
+    // one launch
     e = rbrp_launch_upper_inv_blocks(st, k, R, ldr, dinv);
     if (e != hipSuccess) return e;
-    if (launches) ++*launches;
+    e = rbrp_launch_trsm_cols(st, k, nc, R, ldr, dinv, X, ldx);
+    if (e != hipSuccess) return e;
+    if (launches) *launches += 2;
+    return hipSuccess;
   }
-  int64_t jsup = 0;
-  for (int64_t s1 = k; s1 > 0; s1 -= 256, ++jsup) {
+  for (int64_t s1 = k; s1 > 0; s1 -= 256) {
     const int64_t s0 = std::max<int64_t>(0, s1 - 256);
-    int64_t iblk = 0;
-    for (int64_t r1 = s1; r1 > s0; r1 -= 64, ++iblk) {
+    for (int64_t r1 = s1; r1 > s0; r1 -= 64) {
       const int64_t r0 = std::max<int64_t>(s0, r1 - 64);
       if (r1 < s1) {
         e = rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
@@ -395,12 +397,7 @@ hipError_t rbrp_trsm_upper_left(hipStream_t st, int64_t k, int64_t nc, double* R
         if (e != hipSuccess) return e;
         if (launches) ++*launches;
       }
-      if (dinv)
-        e = rbrp_launch_gemm_nn(st, dim_static(r1 - r0), r1 - r0, dim_static(nc), nc,
-                                dim_static(r1 - r0), mat(dinv + (4 * jsup + iblk) * 4096, 64),
-                                mat(X + r0, ldx), mat(X + r0, ldx), GEMM_SET, nullptr, 0);
-      else
-        e = rbrp_launch_trsm_diag(st, r1 - r0, nc, R + r0 + r0 * ldr, ldr, X + r0, ldx);
+      e = rbrp_launch_trsm_diag(st, r1 - r0, nc, R + r0 + r0 * ldr, ldr, X + r0, ldx);
       if (e != hipSuccess) return e;
       if (launches) ++*launches;
     }

@@ -260,6 +260,11 @@ hipError_t rbrp_launch_copy_block(hipStream_t st, int64_t rows, int64_t cols, co
                                   int64_t lds, double* dst, int64_t ldd);
 hipError_t rbrp_launch_trsm_diag(hipStream_t st, int64_t nb, int64_t ncols, const double* Rdiag,
                                  int64_t ldr, double* X, int64_t ldx);
+// X (k x nc, ldx) <- R^{-1} X for the k x k upper-triangular R (ldr) in one
+// launch (16 columns per workgroup), with the diagonal-block inverses of
+// rbrp_launch_upper_inv_blocks at dinv.
+hipError_t rbrp_launch_trsm_cols(hipStream_t st, int64_t k, int64_t nc, const double* R, int64_t ldr,
+                                 const double* dinv, double* X, int64_t ldx);
 hipError_t rbrp_launch_uniforms(hipStream_t st, uint64_t key0, uint64_t key1, const uint64_t ctr[4],
                                 uint64_t draw0, int64_t count, double* out);
 

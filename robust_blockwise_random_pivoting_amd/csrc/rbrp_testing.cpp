@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "../../include/rbrp_testing.h"
+#include "rbrp_handle.h"
 #include "rbrp_kernels.h"
 
 #define HIPCHK(x)                         \
@@ -108,6 +109,22 @@ int rbrp_test_gram(void* stream, int64_t N, int64_t K, const double* A, int64_t 
                                   mat(C, ldc), GEMM_SET);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(P);
+  return (int)e;
+}
+
+int rbrp_test_trsm(void* stream, int64_t k, int64_t nc, const double* R, int64_t ldr, double* X,
+                   int64_t ldx, int use_dinv) {
+  if (k < 0) return -2;
+  if (nc < 0) return -3;
+  if (ldr < std::max<int64_t>(k, 1)) return -5;
+  if (ldx < std::max<int64_t>(k, 1)) return -7;
+  if (k == 0 || nc == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  double* dinv = nullptr;
+  if (use_dinv) HIPCHK(hipMalloc(&dinv, sizeof(double) * 4 * ((k + 255) / 256) * 4096));
+  hipError_t e = rbrp_trsm_upper_left(st, k, nc, (double*)R, ldr, X, ldx, nullptr, dinv);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (dinv) (void)hipFree(dinv);
   return (int)e;
 }
 
