@@ -865,6 +865,53 @@ hipError_t rbrp_launch_gemm_nn64(hipStream_t st, DimSpec M, int64_t Mmax, DimSpe
 // does not depend on scheduling.
 // ---------------------------------------------------------------------------
 #define RED_GROUPS 16
+// The stores of one reduced sum s of (r, c).
+template <typename T2>
+__device__ __forceinline__ void reduce_store(double s, int64_t r, int64_t c, const MatRef& dst,
+                                             int mode, T2* __restrict__ dst2, int64_t ld2,
+                                             double* __restrict__ radd, int64_t ldra,
+                                             const int* __restrict__ racols) {
+  if (dst.p) {
+    double* d = mat_col(dst, c) + r;
+    if (mode == GEMM_ADD)
+      *d += s;
+    else if (mode == GEMM_SUB)
+      *d -= s;
+    else
+      *d = s;
+  }
+  if (dst2) dst2[r * ld2 + c] = (T2)s;  // row-major copy
+  if (radd) radd[c + (int64_t)racols[r] * ldra] += s;
+}
+
+// Few splits (at most RED_FEW): one thread per (r, c), four columns of 64 rows
+// per workgroup; split z goes to accumulator z % 8, the eight added in a fixed
+// tree, all loads of a group of eight splits in flight.
+#define RED_FEW 32
+template <typename T2>
+__global__ __launch_bounds__(256) void k_reduce_few(
+    DimSpec dM, DimSpec dN, const double* __restrict__ P, int64_t ldp, int64_t pstride, int nsplit,
+    MatRef dst, int mode, T2* __restrict__ dst2, int64_t ld2, double* __restrict__ radd,
+    int64_t ldra, const int* __restrict__ racols) {
+  const int64_t M = dim_eval(dM), N = dim_eval(dN);
+  const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t r = (int64_t)blockIdx.y * 64 + (threadIdx.x & 63);
+  if (c >= N || r >= M) return;
+  const double* p = P + r + c * ldp;
+  double acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = 0.0;
+  for (int z0 = 0; z0 < nsplit; z0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = (z0 + u < nsplit) ? p[(int64_t)(z0 + u) * pstride] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += v[u];
+  }
+  const double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  reduce_store<T2>(s, r, c, dst, mode, dst2, ld2, radd, ldra, racols);
+}
+
 template <typename T2>
 __global__ __launch_bounds__(64 * RED_GROUPS) void k_reduce_splits(
     DimSpec dM, DimSpec dN, const double* __restrict__ P, int64_t ldp, int64_t pstride, int nsplit,
@@ -898,18 +945,7 @@ __global__ __launch_bounds__(64 * RED_GROUPS) void k_reduce_splits(
     for (int w = RED_GROUPS / 4; w >= 1; w >>= 1)
 #pragma unroll
       for (int g = 0; g < w; ++g) t[g] = t[2 * g] + t[2 * g + 1];
-    const double s = t[0];
-    if (dst.p) {
-      double* d = mat_col(dst, c) + r;
-      if (mode == GEMM_ADD)
-        *d += s;
-      else if (mode == GEMM_SUB)
-        *d -= s;
-      else
-        *d = s;
-    }
-    if (dst2) dst2[r * ld2 + c] = (T2)s;  // row-major copy
-    if (radd) radd[c + (int64_t)racols[r] * ldra] += s;
+    reduce_store<T2>(t[0], r, c, dst, mode, dst2, ld2, radd, ldra, racols);
   }
 }
 
@@ -919,9 +955,14 @@ hipError_t rbrp_launch_reduce_splits(hipStream_t st, DimSpec M, DimSpec N, int64
                                      int64_t ldra, const int* racols) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
   if (radd && !racols) return hipErrorInvalidValue;
-  dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_reduce_splits<double>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
-                     pstride, nsplit, dst, mode, dst2, ld2, radd, ldra, racols);
+  if (nsplit <= RED_FEW)
+    hipLaunchKernelGGL(k_reduce_few<double>, dim3((unsigned)((Nmax + 3) / 4), (unsigned)((Mmax + 63) / 64)),
+                       dim3(256), 0, st, M, N, P, ldp, pstride, nsplit, dst, mode, dst2, ld2, radd,
+                       ldra, racols);
+  else
+    hipLaunchKernelGGL(k_reduce_splits<double>, dim3((unsigned)Nmax, (unsigned)((Mmax + 63) / 64)),
+                       dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp, pstride, nsplit, dst, mode, dst2,
+                       ld2, radd, ldra, racols);
   return hipGetLastError();
 }
 
@@ -930,10 +971,14 @@ hipError_t rbrp_launch_reduce_splits_f32copy(hipStream_t st, DimSpec M, DimSpec 
                                              int64_t pstride, int nsplit, MatRef dst, int mode,
                                              float* dst2, int64_t ld2) {
   if (Mmax <= 0 || Nmax <= 0) return hipSuccess;
-  dim3 grid((unsigned)Nmax, (unsigned)((Mmax + 63) / 64));
-  hipLaunchKernelGGL(k_reduce_splits<float>, grid, dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp,
-                     pstride, nsplit, dst, mode, dst2, ld2, (double*)nullptr, (int64_t)0,
-                     (const int*)nullptr);
+  if (nsplit <= RED_FEW)
+    hipLaunchKernelGGL(k_reduce_few<float>, dim3((unsigned)((Nmax + 3) / 4), (unsigned)((Mmax + 63) / 64)),
+                       dim3(256), 0, st, M, N, P, ldp, pstride, nsplit, dst, mode, dst2, ld2,
+                       (double*)nullptr, (int64_t)0, (const int*)nullptr);
+  else
+    hipLaunchKernelGGL(k_reduce_splits<float>, dim3((unsigned)Nmax, (unsigned)((Mmax + 63) / 64)),
+                       dim3(64 * RED_GROUPS), 0, st, M, N, P, ldp, pstride, nsplit, dst, mode, dst2,
+                       ld2, (double*)nullptr, (int64_t)0, (const int*)nullptr);
   return hipGetLastError();
 }
 
