@@ -134,24 +134,6 @@ hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const Rb
   return hipGetLastError();
 }
 
-// Symmetric-difference column moves in the FP32 working matrix.
-__global__ void k_move_cols32(int64_t rows, const RbrpBlockState* bs, float* __restrict__ Aw,
-                              int64_t ld, const int* __restrict__ Jin,
-                              const int* __restrict__ Jout) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= rows || c >= bs->nswap) return;
-  Aw[r + (int64_t)Jin[c] * ld] = Aw[r + (int64_t)Jout[c] * ld];
-}
-
-hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
-                                   float* Aw, int64_t ld, const int* Jin, const int* Jout) {
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_move_cols32, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
-                     st, rows, bs, Aw, ld, Jin, Jout);
-  return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // Split-K TN in FP32: P[z](r, c) = sum_{kk in chunk z} A(kk, r) B(kk, c) with
 // FP32 MFMA products and accumulation, stored as FP64 partials (the split sum

@@ -1060,101 +1060,161 @@ hipError_t rbrp_launch_chol2(hipStream_t st, const double* G2, int64_t ldg, int 
 }
 
 // ---------------------------------------------------------------------------
-// Symmetric-difference permutation (gpu/rbrp.cu:237-294):
+// Symmetric-difference permutation (gpu/rbrp.cu:237-294), one launch:
 //   perm: tmp = perm[Jk]; perm[Jin] = perm[Jout]; perm[i:i+bt] = tmp
-//   diags: diags[Jin] = diags[Jout]
-//   R(0:i, :) columns: same move as perm (matlab/RBRP.m:91-93)
+//   diags: the same move as perm
+//   R(0:i, :) columns: the same move as perm (matlab/RBRP.m:91-93)
 //   Aw columns: Aw(:, Jin) = Aw(:, Jout) (the block columns then receive Q_b)
+// The three parts touch disjoint data, so each workgroup takes one part:
+//   workgroup 0: perm and diags (bt, nswap <= RBRP_MAX_B entries);
+//   the next ceil(i / rt) workgroups: rt rows of R each, rt = PERM_R_VALS /
+//     (2 b), every source value of those rows read into LDS before the
+//     barrier and written to its destination after it (a row's sources and
+//     destinations overlap: Jin lies inside Jk, Jout inside the front);
+//   the rest: Aw in tiles of PERM_AW_ROWS rows x PERM_AW_COLS column pairs,
+//     all loads of a thread before its stores (Jin, the kept columns outside
+//     the front, and Jout, the front columns not kept, are disjoint).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_perm_small(int64_t i, RbrpBlockState* bs,
+#define PERM_R_VALS 4096
+#define PERM_AW_ROWS 512
+#define PERM_AW_COLS 8
+
+static inline int64_t perm_r_rows(int b) { return PERM_R_VALS / (2 * b); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_perm_block(int64_t i, const RbrpBlockState* bs,
                                                     const int* __restrict__ Jk,
                                                     const int* __restrict__ Jin,
                                                     const int* __restrict__ Jout, int* perm,
-                                                    double* diags) {
-  __shared__ int tk[SAMPLE_MAXB], to[SAMPLE_MAXB];
-  __shared__ double dk[SAMPLE_MAXB], dout[SAMPLE_MAXB];
+                                                    double* diags, double* R, int64_t ldr, int rt,
+                                                    int r_groups, T* Aw, int64_t rows, int64_t lda,
+                                                    int aw_tiles) {
+  __shared__ double sv[PERM_R_VALS];
+  __shared__ int src[2 * RBRP_MAX_B], dst[2 * RBRP_MAX_B];
+  __shared__ double dk[RBRP_MAX_B], dout[RBRP_MAX_B];
   const int bt = bs->bt, ns = bs->nswap;
   const int tid = threadIdx.x;
-  if (tid < bt) {
-    tk[tid] = perm[Jk[tid]];
-    dk[tid] = diags[Jk[tid]];
+  int g = blockIdx.x;
+  if (g <= r_groups) {
+    // source / destination columns of the moves: Jk -> the front, Jout -> Jin
+    // (indices staged once, so that no data load waits on an index load)
+    if (tid < bt + ns) {
+      src[tid] = tid < bt ? Jk[tid] : Jout[tid - bt];
+      dst[tid] = tid < bt ? (int)i + tid : Jin[tid - bt];
+    }
+    __syncthreads();
   }
-  if (tid < ns) {
-    to[tid] = perm[Jout[tid]];
-    dout[tid] = diags[Jout[tid]];
+  if (g == 0) {
+    int pv = 0;
+    if (tid < bt + ns) {
+      pv = perm[src[tid]];
+      if (tid < bt)
+        dk[tid] = diags[src[tid]];
+      else
+        dout[tid - bt] = diags[src[tid]];
+    }
+    __syncthreads();
+    if (tid >= bt && tid < bt + ns) {
+      perm[dst[tid]] = pv;
+      diags[dst[tid]] = dout[tid - bt];
+    }
+    __syncthreads();
+    if (tid < bt) {
+      perm[dst[tid]] = pv;
+      diags[dst[tid]] = dk[tid];
+    }
+    return;
   }
-  __syncthreads();
-  if (tid < ns) {
-    perm[Jin[tid]] = to[tid];
-    diags[Jin[tid]] = dout[tid];
+  g -= 1;
+  if (g < r_groups) {
+    // slot s: source column s / rt (Jk, then Jout), row s % rt of the group;
+    // all of a thread's loads are issued before the first LDS write waits
+    const int64_t r0 = (int64_t)g * rt;
+    const int nsl = rt * (bt + ns);
+    double v[PERM_R_VALS / 256];
+#pragma unroll
+    for (int u = 0; u < PERM_R_VALS / 256; ++u) {
+      const int s = tid + 256 * u;
+      const int c = s / rt;
+      const int64_t r = r0 + (s - c * rt);
+      v[u] = 0.0;
+      if (s < nsl && r < i) v[u] = R[r + (int64_t)src[c] * ldr];
+    }
+#pragma unroll
+    for (int u = 0; u < PERM_R_VALS / 256; ++u)
+      if (tid + 256 * u < nsl) sv[tid + 256 * u] = v[u];
+    __syncthreads();
+    for (int s = tid; s < nsl; s += 256) {
+      const int c = s / rt;
+      const int64_t r = r0 + (s - c * rt);
+      if (r < i) R[r + (int64_t)dst[c] * ldr] = sv[s];
+    }
+    return;
   }
-  __syncthreads();
-  if (tid < bt) {
-    perm[i + tid] = tk[tid];
-    diags[i + tid] = dk[tid];
+  g -= r_groups;
+  if (g >= aw_tiles) return;
+  const int rtiles = (int)((rows + PERM_AW_ROWS - 1) / PERM_AW_ROWS);
+  const int cg = g / rtiles;
+  const int c0 = cg * PERM_AW_COLS;
+  if (c0 >= ns) return;
+  const int nc = min(PERM_AW_COLS, ns - c0);
+  // the tile's column indices, uniform over the workgroup (outside the
+  // row-dependent branches, so they are read once before the data loads)
+  int64_t co[PERM_AW_COLS], ci[PERM_AW_COLS];
+#pragma unroll
+  for (int c = 0; c < PERM_AW_COLS; ++c) {
+    co[c] = c < nc ? (int64_t)Jout[c0 + c] * lda : 0;
+    ci[c] = c < nc ? (int64_t)Jin[c0 + c] * lda : 0;
   }
+  const int64_t rb = (int64_t)(g - cg * rtiles) * PERM_AW_ROWS + tid;
+  T v[PERM_AW_COLS][PERM_AW_ROWS / 256];
+#pragma unroll
+  for (int c = 0; c < PERM_AW_COLS; ++c)
+#pragma unroll
+    for (int q = 0; q < PERM_AW_ROWS / 256; ++q) {
+      const int64_t r = rb + 256 * q;
+      if (c < nc && r < rows) v[c][q] = Aw[r + co[c]];
+    }
+#pragma unroll
+  for (int c = 0; c < PERM_AW_COLS; ++c)
+#pragma unroll
+    for (int q = 0; q < PERM_AW_ROWS / 256; ++q) {
+      const int64_t r = rb + 256 * q;
+      if (c < nc && r < rows) Aw[r + ci[c]] = v[c][q];
+    }
 }
 
-hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
-                                  const int* Jin, const int* Jout, int* perm, double* diags) {
-  hipLaunchKernelGGL(k_perm_small, dim3(1), dim3(256), 0, st, i, bs, Jk, Jin, Jout, perm, diags);
+template <typename T>
+static hipError_t launch_perm_block(hipStream_t st, int64_t i, int b, const RbrpBlockState* bs,
+                                    const int* Jk, const int* Jin, const int* Jout, int* perm,
+                                    double* diags, double* R, int64_t ldr, T* Aw, int64_t rows,
+                                    int64_t lda) {
+  if (b < 1 || b > RBRP_MAX_B || i < 0 || rows < 0) return hipErrorInvalidValue;
+  const int64_t rt = perm_r_rows(b);
+  const int64_t r_groups = (i + rt - 1) / rt;
+  const int64_t aw_tiles = Aw ? ((rows + PERM_AW_ROWS - 1) / PERM_AW_ROWS) *
+                                    ((b + PERM_AW_COLS - 1) / PERM_AW_COLS)
+                              : 0;
+  const int64_t grid = 1 + r_groups + aw_tiles;
+  if (grid > INT_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_perm_block<T>, dim3((unsigned)grid), dim3(256), 0, st, i, bs, Jk, Jin,
+                     Jout, perm, diags, R, ldr, (int)rt, (int)r_groups, Aw, rows, lda,
+                     (int)aw_tiles);
   return hipGetLastError();
 }
 
-__global__ void k_perm_gather_r(int64_t rows, RbrpBlockState* bs, const double* __restrict__ R,
-                                int64_t ldr, const int* __restrict__ Jk,
-                                const int* __restrict__ Jout, double* __restrict__ tmpk,
-                                double* __restrict__ tmpo, int64_t ldt) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= rows) return;
-  if (c < bs->bt) tmpk[r + c * ldt] = R[r + (int64_t)Jk[c] * ldr];
-  if (c < bs->nswap) tmpo[r + c * ldt] = R[r + (int64_t)Jout[c] * ldr];
+hipError_t rbrp_launch_perm_block(hipStream_t st, int64_t i, int b, const RbrpBlockState* bs,
+                                  const int* Jk, const int* Jin, const int* Jout, int* perm,
+                                  double* diags, double* R, int64_t ldr, double* Aw, int64_t rows,
+                                  int64_t lda) {
+  return launch_perm_block<double>(st, i, b, bs, Jk, Jin, Jout, perm, diags, R, ldr, Aw, rows, lda);
 }
 
-hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
-                                     const double* R, int64_t ldr, const int* Jk, const int* Jout,
-                                     double* tmpk, double* tmpo, int64_t ldt) {
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_perm_gather_r, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0,
-                     st, rows, bs, R, ldr, Jk, Jout, tmpk, tmpo, ldt);
-  return hipGetLastError();
-}
-
-__global__ void k_perm_scatter_r(int64_t rows, int64_t i, RbrpBlockState* bs, double* __restrict__ R,
-                                 int64_t ldr, const int* __restrict__ Jin,
-                                 const double* __restrict__ tmpk, const double* __restrict__ tmpo,
-                                 int64_t ldt) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= rows) return;
-  if (c < bs->nswap) R[r + (int64_t)Jin[c] * ldr] = tmpo[r + c * ldt];
-  if (c < bs->bt) R[r + (i + c) * ldr] = tmpk[r + c * ldt];
-}
-
-hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64_t i,
-                                      RbrpBlockState* bs, double* R, int64_t ldr, const int* Jin,
-                                      const double* tmpk, const double* tmpo, int64_t ldt) {
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_perm_scatter_r, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256),
-                     0, st, rows, i, bs, R, ldr, Jin, tmpk, tmpo, ldt);
-  return hipGetLastError();
-}
-
-__global__ void k_move_cols(int64_t rows, RbrpBlockState* bs, double* __restrict__ Aw, int64_t ld,
-                            const int* __restrict__ Jin, const int* __restrict__ Jout) {
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int c = blockIdx.y;
-  if (r >= rows || c >= bs->nswap) return;
-  Aw[r + (int64_t)Jin[c] * ld] = Aw[r + (int64_t)Jout[c] * ld];
-}
-
-hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
-                                 int64_t ld, const int* Jin, const int* Jout) {
-  if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_move_cols, dim3((unsigned)((rows + 255) / 256), (unsigned)b), dim3(256), 0, st,
-                     rows, bs, Aw, ld, Jin, Jout);
-  return hipGetLastError();
+hipError_t rbrp_launch_perm_block32(hipStream_t st, int64_t i, int b, const RbrpBlockState* bs,
+                                    const int* Jk, const int* Jin, const int* Jout, int* perm,
+                                    double* diags, double* R, int64_t ldr, float* Aw, int64_t rows,
+                                    int64_t lda) {
+  return launch_perm_block<float>(st, i, b, bs, Jk, Jin, Jout, perm, diags, R, ldr, Aw, rows, lda);
 }
 
 // diags[i + bt + c] = sum_{t < ntiles} part[t * ldp + c], c < n - i - bt.

@@ -13,7 +13,7 @@ struct RbrpWorkspace {
   size_t arena_bytes = 0;
   // carved pointers of the RBRP loop
   double *Aw, *R, *diags, *w, *cdf, *G, *G2, *Rb, *Rinv, *R2, *R2inv, *Cbuf, *Qbuf, *P, *normpart,
-      *tmpk, *tmpo, *Wt;
+      *Wt;
   double* Dinv;  // diagonal-block inverses of R for the X solve
   // mixed precision: FP32 working matrix and row-major W, FP64 skeleton basis
   // and FP64 copy of the sampled block (null in the FP64 loop, where Aw, Wt

@@ -24,8 +24,6 @@
 #include "rbrp_handle.h"
 #include "rbrp_kernels.h"
 
-#define RBRP_MAX_B 128
-
 #define HIPCHK(x)                               \
   do {                                          \
     hipError_t e_ = (x);                        \
@@ -53,7 +51,7 @@ namespace {
 // Byte offsets of the loop's buffers in one arena (every buffer 256-byte aligned).
 struct WsLayout {
   size_t oAw, oAw32, oQ64, oVd, oRcol, oR64, oR, odiags, ow, ocdf, oG, oG2, oRb, oRinv, oR2, oR2inv, oCbuf, oQbuf, oP,
-      onp, oWt, otk, oto, oJ, oJk, oJin, oJout, oJkpos, oDinv, ost;
+      onp, oWt, oJ, oJk, oJin, oJout, oJkpos, oDinv, ost;
   size_t need;
   int64_t Pcap;
 };
@@ -98,8 +96,6 @@ WsLayout layout_workspace(int64_t mp, int64_t n, int64_t kmax, int64_t b, bool m
   L.oP = c.take<double>(Pcap);
   L.onp = c.take<double>(((mp + 63) / 64) * n);  // 64-row (FP64) or 128-row (FP32) tiles
   L.oWt = c.take<double>(b * n);                  // FP32 in the mixed loop (over-sized)
-  L.otk = c.take<double>(ldr * b);
-  L.oto = c.take<double>(ldr * b);
   L.oJ = c.take<int>(b);
   L.oJk = c.take<int>(b);
   L.oJin = c.take<int>(b);
@@ -136,8 +132,6 @@ void carve_workspace(RbrpWorkspace& ws, char* base, const WsLayout& L, bool mixe
   ws.normpart = (double*)(base + L.onp);
   ws.Wt = (double*)(base + L.oWt);  // (one of the two per block: FP64 or FP32 storage)
   ws.Wt32 = mixed ? (float*)(base + L.oWt) : nullptr;
-  ws.tmpk = (double*)(base + L.otk);
-  ws.tmpo = (double*)(base + L.oto);
   ws.J = (int*)(base + L.oJ);
   ws.Jk = (int*)(base + L.oJk);
   ws.Jin = (int*)(base + L.oJin);
@@ -823,19 +817,13 @@ int colid_impl(rbrp_handle_t h, const void* dA, bool a_f32, int64_t m, int64_t n
 
     // ---- symmetric-difference permutation -------------------------------
     ph = pt.begin(RBRP_PHASE_PERMUTE);
-    int perm_launches = 3;
-    HIPCHK(rbrp_launch_perm_small(st, i, ws.st, ws.Jk, ws.Jin, ws.Jout, perm, ws.diags));
-    if (i > 0) {
-      HIPCHK(rbrp_launch_perm_gather_r(st, i, b, ws.st, ws.R, ldr, ws.Jk, ws.Jout, ws.tmpk, ws.tmpo, ldr));
-      HIPCHK(rbrp_launch_perm_scatter_r(st, i, b, i, ws.st, ws.R, ldr, ws.Jin, ws.tmpk, ws.tmpo, ldr));
-      perm_launches += 2;
-    }
+    const int perm_launches = 2;  // the moves and the Q_b product
     if (m32)
-      HIPCHK(rbrp_launch_move_cols32(st, mp, b, ws.st, ws.Aw32, mp, ws.Jin, ws.Jout));
-    else if (!(direct && i == 0))  // (the direct first block reads A through perm instead)
-      HIPCHK(rbrp_launch_move_cols(st, mp, b, ws.st, ws.Aw, mp, ws.Jin, ws.Jout));
-    else
-      --perm_launches;
+      HIPCHK(rbrp_launch_perm_block32(st, i, b, ws.st, ws.Jk, ws.Jin, ws.Jout, perm, ws.diags, ws.R,
+                                      ldr, ws.Aw32, mp, mp));
+    else  // (the direct first block reads A through perm: no working-matrix move)
+      HIPCHK(rbrp_launch_perm_block(st, i, b, ws.st, ws.Jk, ws.Jin, ws.Jout, perm, ws.diags, ws.R,
+                                    ldr, (direct && i == 0) ? nullptr : ws.Aw, mp, mp));
     // Q_b = Q1 R2^{-1} into the block's columns of the skeleton basis
     // (mixed loop: with its FP32 copy in the working matrix's block columns)
     HIPCHK(rbrp_launch_gemm_nn64(st, dim_static(mp), mp, dim_dyn(0, d_bt, 1), b, dim_dyn(0, d_bt, 1),

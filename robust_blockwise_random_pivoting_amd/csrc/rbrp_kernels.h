@@ -11,6 +11,9 @@
 
 #include "rbrp_common.h"
 
+// Largest block size b of the column ID.
+#define RBRP_MAX_B 128
+
 // Kernel launch that, given start / stop events (either may be null), has the
 // dispatch itself time them (hipExtLaunchKernelGGL): no marker packets enter
 // the stream around the kernel.  The kernel name may be parenthesised.
@@ -237,16 +240,19 @@ hipError_t rbrp_launch_chol2_regs(hipStream_t st, const double* G2, int64_t ldg,
 // (ld 64) at out.
 hipError_t rbrp_launch_upper_inv_blocks(hipStream_t st, int64_t k, const double* R, int64_t ldr,
                                         double* out);
-hipError_t rbrp_launch_perm_small(hipStream_t st, int64_t i, RbrpBlockState* bs, const int* Jk,
-                                  const int* Jin, const int* Jout, int* perm, double* diags);
-hipError_t rbrp_launch_perm_gather_r(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs,
-                                     const double* R, int64_t ldr, const int* Jk, const int* Jout,
-                                     double* tmpk, double* tmpo, int64_t ldt);
-hipError_t rbrp_launch_perm_scatter_r(hipStream_t st, int64_t rows, int b, int64_t i,
-                                      RbrpBlockState* bs, double* R, int64_t ldr, const int* Jin,
-                                      const double* tmpk, const double* tmpo, int64_t ldt);
-hipError_t rbrp_launch_move_cols(hipStream_t st, int64_t rows, int b, RbrpBlockState* bs, double* Aw,
-                                 int64_t ld, const int* Jin, const int* Jout);
+// Symmetric-difference permutation of one block in one launch: perm and diags
+// (Jin <- Jout, the front <- Jk), the same column move in R(0:i, :), and
+// Aw(:, Jin) = Aw(:, Jout) over `rows` rows (Aw null: no working-matrix move).
+// b <= RBRP_MAX_B.
+hipError_t rbrp_launch_perm_block(hipStream_t st, int64_t i, int b, const RbrpBlockState* bs,
+                                  const int* Jk, const int* Jin, const int* Jout, int* perm,
+                                  double* diags, double* R, int64_t ldr, double* Aw, int64_t rows,
+                                  int64_t lda);
+// The same with the FP32 working matrix of the mixed form.
+hipError_t rbrp_launch_perm_block32(hipStream_t st, int64_t i, int b, const RbrpBlockState* bs,
+                                    const int* Jk, const int* Jin, const int* Jout, int* perm,
+                                    double* diags, double* R, int64_t ldr, float* Aw, int64_t rows,
+                                    int64_t lda);
 // diags[c] for the remaining columns from the update's norm partials.
 hipError_t rbrp_launch_colnorm_reduce(hipStream_t st, const double* part, int64_t ldp, int64_t ntiles,
                                       int64_t i, int64_t n, RbrpBlockState* bs, double* diags);
@@ -283,8 +289,6 @@ hipError_t rbrp_launch_gather_v64(hipStream_t st, int64_t rows, int b, const Rbr
 hipError_t rbrp_launch_scatter_v32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
                                    const double* V, int64_t ldv, const int* J, float* Aw, int64_t ld,
                                    const int* kept = nullptr);
-hipError_t rbrp_launch_move_cols32(hipStream_t st, int64_t rows, int b, const RbrpBlockState* bs,
-                                   float* Aw, int64_t ld, const int* Jin, const int* Jout);
 // Split-K TN product of FP32 operands with FP64 partials (see k_gemm_tn; a
 // single 128-row tile when M > 64);
 // A's and B's first columns are acoff + *a_dyn and bcoff + *b_dyn (dyn
